@@ -1,0 +1,1032 @@
+// spaQR oracle: CPU FP64 restatement of the pspaQR ParTree path on one rank.
+//
+// TEST INFRASTRUCTURE ONLY.  This is the checker the HIP path is compared against, and the
+// "port" CPU baseline that bench.py times beside it; the product never links it.
+//
+// What it restates (reference file:line):
+//   ordering      GeometricPartitionAtA / bissect_geo / getInterfacesUsingA
+//                 (src/partition.cpp:241-350,514-587), initPermutation (src/tree.cpp:48-57),
+//                 merge_if (src/cluster.cpp:23-32), cluster hierarchy (src/ptree.cpp:587-691)
+//   assembly      ParTree::assemble (src/ptree.cpp:696-864), block2dense (src/util.cpp:853-899)
+//   factorize     ParTree::factorize (src/ptree.cpp:873-2154) with the task bodies
+//                 geqrt_cluster :170-195, tsqrt_edge :197-214, larfb_edge :216-221,
+//                 ssrfb_edges :223-266, scale_cluster :269-286, trsm_edge :365-370,
+//                 sparsify_rrqr_only :373-439, scale apply :1621-1633,
+//                 sparsify apply :1867-1897, compute_new_edges :289-340,
+//                 fill-in get_sparsity/alloc_fillin :866-871,140-153
+//   solve         ParTree::solve(x) (src/ptree.cpp:2788-3210) with helpers :2160-2262
+//   spmv          ParTree::spmv (src/ptree.cpp:3212-3262)
+//   GMRES         dist_gmres (src/is.cpp:447-527) and LUSolve (src/util.cpp:1019-1032)
+//
+// Orders the reference leaves to its task scheduler or to pointer hashing are fixed here
+// to cluster order: edgesIn lists, fill-in pair order, the order in which interiors of one
+// level update a shared block (and so order_of_trans), and the solve's reductions.
+// The RRQR is full geqp3 + choose_rank (the reference's non-MKL branch,
+// src/ptree.cpp:411-416).  Only square systems (rperm == cperm, c2r_count == 1).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <set>
+#include <stdexcept>
+#include <vector>
+
+#include "dense.hpp"
+
+namespace orc {
+
+// ------------------------------------------------------------------ IDs
+struct SepID {
+    int lvl = -1, sep = 0;
+    SepID() {}
+    SepID(int l, int s) : lvl(l), sep(s) {}
+    bool operator==(const SepID& o) const { return lvl == o.lvl && sep == o.sep; }
+    bool operator!=(const SepID& o) const { return !(*this == o); }
+    bool operator<(const SepID& o) const { return lvl < o.lvl || (lvl == o.lvl && sep < o.sep); }
+};
+
+struct ClusterID {  // include/cluster.h:61-135
+    SepID self, l, r;
+    int part = 2, section = 0;
+    bool operator==(const ClusterID& o) const {
+        return self == o.self && l == o.l && r == o.r && part == o.part && section == o.section;
+    }
+    bool operator!=(const ClusterID& o) const { return !(*this == o); }
+    bool operator<(const ClusterID& o) const {
+        if (self != o.self) return self < o.self;
+        if (l != o.l) return l < o.l;
+        if (r != o.r) return r < o.r;
+        if (section != o.section) return section < o.section;
+        return part < o.part;
+    }
+};
+
+static SepID merge_sep(const SepID& s) { return SepID(s.lvl + 1, s.sep / 2); }
+
+static ClusterID merge_if(const ClusterID& c, int lvl) {  // src/cluster.cpp:23-32
+    ClusterID p;
+    p.self = c.self;
+    p.l = c.l.lvl < lvl ? merge_sep(c.l) : c.l;
+    p.r = c.r.lvl < lvl ? merge_sep(c.r) : c.r;
+    p.section = lvl > 0 ? c.section / 2 : c.section;
+    p.part = c.self.lvl == lvl ? 2 : c.part;
+    return p;
+}
+
+static SepID find_highest_common(SepID n1, SepID n2) {  // src/partition.cpp:63-86
+    int l1 = n1.lvl, l2 = n2.lvl, s1 = n1.sep, s2 = n2.sep;
+    while (l1 < l2) { l1++; s1 /= 2; }
+    while (l2 < l1) { l2++; s2 /= 2; }
+    while (s1 != s2) { l1++; l2++; s1 /= 2; s2 /= 2; }
+    return SepID(l1, s1);
+}
+
+static SepID find_lowest_common(SepID n1, SepID n2) {  // src/partition.cpp:88-100
+    return n1.lvl < n2.lvl ? n1 : n2;
+}
+
+// ------------------------------------------------------------------ sparse (CSC)
+struct CSC {
+    int nr = 0, nc = 0;
+    std::vector<int> colptr, rowind;
+    std::vector<double> val;
+};
+
+static CSC transpose(const CSC& A) {
+    CSC T;
+    T.nr = A.nc; T.nc = A.nr;
+    T.colptr.assign(A.nr + 1, 0);
+    for (int k = 0; k < A.colptr[A.nc]; ++k) T.colptr[A.rowind[k] + 1]++;
+    for (int i = 0; i < A.nr; ++i) T.colptr[i + 1] += T.colptr[i];
+    T.rowind.resize(A.colptr[A.nc]);
+    T.val.resize(A.colptr[A.nc]);
+    std::vector<int> pos(T.colptr.begin(), T.colptr.end() - 1);
+    for (int j = 0; j < A.nc; ++j)
+        for (int k = A.colptr[j]; k < A.colptr[j + 1]; ++k) {
+            int p = pos[A.rowind[k]]++;
+            T.rowind[p] = j;
+            T.val[p] = A.val[k];
+        }
+    return T;  // row indices come out sorted because j increases
+}
+
+// Structure of A^T A (columns i, j adjacent iff they share a row), diagonal included.
+static CSC ata_structure(const CSC& A) {
+    CSC At = transpose(A);
+    CSC G;
+    G.nr = G.nc = A.nc;
+    G.colptr.assign(A.nc + 1, 0);
+    std::vector<int> mark(A.nc, -1);
+    std::vector<int> tmp;
+    for (int j = 0; j < A.nc; ++j) {
+        tmp.clear();
+        for (int k = A.colptr[j]; k < A.colptr[j + 1]; ++k) {
+            int r = A.rowind[k];
+            for (int q = At.colptr[r]; q < At.colptr[r + 1]; ++q) {
+                int i = At.rowind[q];
+                if (mark[i] != j) { mark[i] = j; tmp.push_back(i); }
+            }
+        }
+        std::sort(tmp.begin(), tmp.end());
+        G.rowind.insert(G.rowind.end(), tmp.begin(), tmp.end());
+        G.colptr[j + 1] = (int)G.rowind.size();
+    }
+    G.val.assign(G.rowind.size(), 1.0);
+    return G;
+}
+
+// A(rperm, cperm): entry (rperm[i], cperm[j]) of A lands at (i, j).
+static CSC permute(const CSC& A, const std::vector<int>& rperm, const std::vector<int>& cperm) {
+    std::vector<int> rinv(A.nr);
+    for (int i = 0; i < A.nr; ++i) rinv[rperm[i]] = i;
+    CSC P;
+    P.nr = A.nr; P.nc = A.nc;
+    P.colptr.assign(A.nc + 1, 0);
+    std::vector<std::pair<int, double>> col;
+    for (int j = 0; j < A.nc; ++j) {
+        int oj = cperm[j];
+        col.clear();
+        for (int k = A.colptr[oj]; k < A.colptr[oj + 1]; ++k) col.push_back({rinv[A.rowind[k]], A.val[k]});
+        std::sort(col.begin(), col.end(), [](auto& a, auto& b) { return a.first < b.first; });
+        for (auto& e : col) { P.rowind.push_back(e.first); P.val.push_back(e.second); }
+        P.colptr[j + 1] = (int)P.rowind.size();
+    }
+    return P;
+}
+
+// ------------------------------------------------------------------ ordering
+// Tensor coordinates of linspace_nd (src/util.cpp:901-930): id = sum x_i n^(d-1-i).
+static std::vector<double> linspace_nd(int n, int d) {
+    int N = 1;
+    for (int i = 0; i < d; ++i) N *= n;
+    std::vector<double> X((size_t)d * N);
+    for (int id = 0; id < N; ++id) {
+        int rem = id;
+        for (int dim = d - 1; dim >= 0; --dim) {
+            X[dim + (size_t)id * d] = rem % n;
+            rem /= n;
+        }
+    }
+    return X;
+}
+
+// bissect_geo (src/partition.cpp:241-279): split on the widest dimension at the value of
+// the N/2-th smallest coordinate; <= goes left.  The value (not the tie order) decides.
+static void bissect_geo(const std::vector<int>& dofs, std::vector<int>& parts, const std::vector<double>& X, int d,
+                        int start) {
+    int N = (int)dofs.size();
+    if (N == 0) return;
+    int bestdim = -1;
+    double maxrange = -1.0;
+    for (int dim = 0; dim < d; ++dim) {
+        double mx = -INFINITY, mn = INFINITY;
+        for (int i : dofs) { mx = std::max(mx, X[dim + (size_t)i * d]); mn = std::min(mn, X[dim + (size_t)i * d]); }
+        if (mx - mn > maxrange) { bestdim = dim; maxrange = mx - mn; }
+    }
+    std::vector<double> v(N);
+    for (int i = 0; i < N; ++i) v[i] = X[bestdim + (size_t)dofs[i] * d];
+    std::nth_element(v.begin(), v.begin() + N / 2, v.end());
+    double midv = v[N / 2];
+    for (int j : dofs) parts[j] = (X[bestdim + (size_t)j * d] <= midv) ? start : start + 1;
+}
+
+static int part_at_lvl(int part_leaf, int lvl) {  // src/partition.cpp:54-61
+    for (int l = 0; l < lvl - 1; ++l) part_leaf /= 2;
+    return part_leaf;
+}
+
+// GeometricPartitionAtA + getInterfacesUsingA for a square matrix without matching.
+static std::vector<ClusterID> geometric_partition(const CSC& A, int nlevels, const std::vector<double>& X, int d) {
+    const int c = A.nc;
+    CSC G = ata_structure(A);
+    std::vector<int> parts(c, 0);
+    for (int depth = 0; depth < nlevels - 1; ++depth) {
+        int np = 1 << depth;
+        std::vector<std::vector<int>> dofs(np);
+        for (int i = 0; i < c; ++i) dofs[parts[i]].push_back(i);
+        for (int k = 0; k < np; ++k) bissect_geo(dofs[k], parts, X, d, 2 * k);
+    }
+    std::vector<ClusterID> cmap(c);
+    for (int depth = 0; depth < nlevels - 1; ++depth) {
+        int l = nlevels - depth - 1;
+        for (int i = 0; i < c; ++i) {
+            if (cmap[i].self.lvl != -1) continue;
+            int pi = part_at_lvl(parts[i], l);
+            if (pi % 2 != 0) continue;
+            bool sep = false;
+            for (int k = G.colptr[i]; k < G.colptr[i + 1]; ++k) {
+                int j = G.rowind[k];
+                if (cmap[j].self.lvl == -1 && part_at_lvl(parts[j], l) == pi + 1) { sep = true; break; }
+            }
+            if (sep) { cmap[i].self = SepID(l, pi / 2); cmap[i].section = parts[i]; }
+        }
+    }
+    for (int i = 0; i < c; ++i)
+        if (cmap[i].self.lvl == -1) {
+            cmap[i].self = cmap[i].l = cmap[i].r = SepID(0, parts[i]);
+            cmap[i].section = parts[i];
+        }
+    // getInterfacesUsingA (src/partition.cpp:514-587); square, rmap == cmap.
+    std::vector<ClusterID> rmap = cmap;
+    CSC At = transpose(A);  // column i of At = row i of A
+    for (int i = 0; i < A.nr; ++i) {
+        SepID self = rmap[i].self;
+        if (self.lvl <= 0) continue;
+        SepID left(-1, 0), right(-1, 0);
+        for (int k = At.colptr[i]; k < At.colptr[i + 1]; ++k) {
+            int j = At.rowind[k];
+            SepID nbr = cmap[j].self;
+            if (nbr.lvl < self.lvl) {
+                int pj = nbr.sep >> (self.lvl - nbr.lvl - 1);
+                if (pj % 2 == 0) left = (left.lvl == -1) ? nbr : find_lowest_common(nbr, left);
+                else right = (right.lvl == -1) ? nbr : find_lowest_common(nbr, right);
+            }
+        }
+        rmap[i].l = left;
+        rmap[i].r = right;
+    }
+    cmap = rmap;
+    for (int i = 0; i < A.nr; ++i) {
+        SepID self = rmap[i].self, left = rmap[i].l, right = rmap[i].r;
+        if (right == SepID()) {
+            for (int k = At.colptr[i]; k < At.colptr[i + 1]; ++k) {
+                int j = At.rowind[k];
+                if (self == cmap[j].self && cmap[j].r != SepID())
+                    right = (right == SepID()) ? cmap[j].r : find_highest_common(right, cmap[j].r);
+            }
+        } else if (left == SepID()) {
+            for (int k = At.colptr[i]; k < At.colptr[i + 1]; ++k) {
+                int j = At.rowind[k];
+                if (self == cmap[j].self && cmap[j].l != SepID())
+                    left = (left == SepID()) ? cmap[j].l : find_highest_common(left, cmap[j].l);
+            }
+        }
+        rmap[i].l = left;
+        rmap[i].r = right;
+    }
+    for (int i = 0; i < A.nr; ++i) {
+        if (rmap[i].l == SepID()) rmap[i].l = rmap[i].self;
+        if (rmap[i].r == SepID()) rmap[i].r = rmap[i].self;
+    }
+    return rmap;
+}
+
+// initPermutation (src/tree.cpp:48-57).
+static std::vector<int> init_permutation(int nlevels, const std::vector<ClusterID>& cmap) {
+    std::vector<int> cperm(cmap.size());
+    for (size_t i = 0; i < cperm.size(); ++i) cperm[i] = (int)i;
+    std::vector<ClusterID> pm = cmap;
+    auto cmp = [&pm](int i, int j) { return pm[i] < pm[j]; };
+    std::stable_sort(cperm.begin(), cperm.end(), cmp);
+    for (int lvl = 1; lvl < nlevels; ++lvl) {
+        for (auto& s : pm) s = merge_if(s, lvl);
+        std::stable_sort(cperm.begin(), cperm.end(), cmp);
+    }
+    return cperm;
+}
+
+// ------------------------------------------------------------------ clusters and edges
+struct Cluster;
+struct Edge {  // include/edge.h:14-58: block A(n2, n1), rows(n2) x cols(n1)
+    Cluster* n1;
+    Cluster* n2;
+    Mat A;
+    bool spars = false;
+};
+
+struct Cluster {  // include/cluster.h:142-389
+    int cstart, csize, rstart, rsize;
+    ClusterID id;
+    int order, merge_level;
+    bool eliminated = false;
+    int rank;
+    int csize_org, rsize_org;
+    Cluster* parent = nullptr;
+    std::vector<Cluster*> children;
+    int rposparent = -1, cposparent = -1;
+    std::set<int> dist2;  // cluster orders (deterministic stand-in for the pointer set)
+    std::vector<Edge*> edgesOut, edgesIn, sparsOut, sparsIn;
+    Edge* eself = nullptr;
+    int col_interior_deps = 0;
+    // factors
+    Mat T;                         // T of the diagonal geqrt (k x k, nb = k)
+    std::map<int, Mat> Tmap;       // tpqrt T per neighbour order (nb x k)
+    Mat Qs, Ts;                    // scaling
+    Mat Qsp, Tsp;                  // sparsification
+    std::vector<double> tausp;
+    bool sparsified = false;
+    std::vector<double> x;
+
+    Cluster(int cs, int cz, int rs, int rz, ClusterID id_, int order_, int ml)
+        : cstart(cs), csize(cz), rstart(rs), rsize(rz), id(id_), order(order_), merge_level(ml) {
+        rank = cz;
+        csize_org = cz;
+        rsize_org = rz;
+        x.assign(rz, 0.0);
+    }
+    int lvl() const { return id.self.lvl; }
+    int cols() const { return csize; }
+    int rows() const { return rsize; }
+    void reset_size(int r, int c) { rsize = r; csize = c; rank = c; }
+    void set_size(int r, int c) { reset_size(r, c); x.assign(r, 0.0); }
+    Edge* find_out(const Cluster* n) const {
+        for (Edge* e : edgesOut) if (e->n2 == n) return e;
+        return nullptr;
+    }
+};
+
+struct OpTrace {  // per-level record of the dense work (shapes), for flop/byte accounting
+    int level, op;  // op: 0 geqrt, 1 larfb, 2 tpqrt, 3 tpmqrt, 4 scale_geqrt, 5 scale_apply, 6 geqp3, 7 spars_apply
+    int m, n, k;
+};
+
+struct Tree {
+    int nlevels, skip;
+    double tol = 0.1;
+    int n = 0;
+    CSC A, App;
+    std::vector<double> X;
+    int d = 0;
+    std::vector<int> cperm;
+    std::vector<ClusterID> cpermed;
+    std::vector<std::vector<Cluster*>> bottoms, interiors, interfaces;
+    std::vector<std::unique_ptr<Cluster>> owned_c;
+    std::vector<std::unique_ptr<Edge>> owned_e;
+    int max_order = 0;
+    int current_bottom = 0;
+    std::vector<OpTrace> trace;
+    std::vector<std::vector<int>> ranks_before, ranks_after;  // per level, sparsified interfaces
+    double t_factor = 0.0;
+
+    Tree(int nl, int sk) : nlevels(nl), skip(sk) {
+        bottoms.resize(nl);
+        interiors.resize(nl);
+        interfaces.resize(nl);
+        ranks_before.resize(nl);
+        ranks_after.resize(nl);
+    }
+
+    Cluster* new_cluster(int cs, int cz, int rs, int rz, ClusterID id, int ml) {
+        owned_c.emplace_back(new Cluster(cs, cz, rs, rz, id, max_order++, ml));
+        return owned_c.back().get();
+    }
+
+    bool want_sparsify(const Cluster* c) const {  // src/ptree.cpp:29-41
+        if (c->merge_level < skip || c->merge_level >= nlevels - 2 || tol == 0) return false;
+        if (c->lvl() > c->merge_level) {
+            if (c->id.l.lvl <= c->merge_level && c->id.r.lvl <= c->merge_level) return true;
+        }
+        return false;
+    }
+
+    Cluster* by_order(int o) const {
+        // orders at one merge level are contiguous; search all levels
+        for (auto& lv : bottoms) {
+            if (lv.empty()) continue;
+            int b = lv[0]->order;
+            if (o >= b && o < b + (int)lv.size()) return lv[o - b];
+        }
+        return nullptr;
+    }
+
+    // ---------------------------------------------------------- partition
+    void partition() {  // src/ptree.cpp:495-694
+        std::vector<ClusterID> cmap = geometric_partition(A, nlevels, X, d);
+        cperm = init_permutation(nlevels, cmap);
+        cpermed.resize(n);
+        for (int k = 0; k < n; ++k) cpermed[k] = cmap[cperm[k]];
+        // leaf clusters: runs of equal ClusterID
+        for (int k = 0, l = 0; k < n;) {
+            int kn = k + 1;
+            while (kn < n && cpermed[kn] == cpermed[k]) kn++;
+            Cluster* s = new_cluster(k, kn - k, l, kn - k, cpermed[k], 0);
+            bottoms[0].push_back(s);
+            (s->lvl() == 0 ? interiors[0] : interfaces[0]).push_back(s);
+            l += kn - k;
+            k = kn;
+        }
+        for (int lvl = 1; lvl < nlevels; ++lvl) {
+            auto& prev = bottoms[lvl - 1];
+            size_t b = 0;
+            while (b < prev.size() && prev[b]->lvl() < lvl) b++;
+            for (size_t k = b; k < prev.size();) {
+                ClusterID pid = merge_if(prev[k]->id, lvl);
+                std::vector<Cluster*> ch;
+                int cs = 0, rs = 0;
+                int cst = prev[k]->cstart, rst = prev[k]->rstart;
+                while (k < prev.size() && merge_if(prev[k]->id, lvl) == pid) {
+                    ch.push_back(prev[k]);
+                    cs += prev[k]->cols();
+                    rs += prev[k]->rows();
+                    k++;
+                }
+                Cluster* p = new_cluster(cst, cs, rst, rs, pid, lvl);
+                std::sort(ch.begin(), ch.end(), [](Cluster* a, Cluster* b2) { return a->id < b2->id; });
+                for (Cluster* c : ch) { c->parent = p; p->children.push_back(c); }
+                bottoms[lvl].push_back(p);
+                (pid.self.lvl == lvl ? interiors[lvl] : interfaces[lvl]).push_back(p);
+            }
+        }
+    }
+
+    // ---------------------------------------------------------- edges
+    Edge* new_edge(Cluster* c1, Cluster* c2, int cur) {  // new_edgeOut, src/ptree.cpp:101-120
+        owned_e.emplace_back(new Edge{c1, c2, Mat(c2->rows(), c1->cols())});
+        Edge* e = owned_e.back().get();
+        c1->edgesOut.push_back(e);
+        bool sp = c1->lvl() > cur && c2->lvl() > cur && c1 != c2;
+        e->spars = sp;
+        if (sp) c1->sparsOut.push_back(e);
+        if (c1 != c2) add_edge_in(c2, e, sp);
+        return e;
+    }
+    void add_edge_in(Cluster* c, Edge* e, bool sp) {  // src/cluster.cpp:85-90
+        c->edgesIn.push_back(e);
+        if (sp) c->sparsIn.push_back(e);
+        if (e->n1->lvl() == e->n1->merge_level) c->col_interior_deps++;
+    }
+    static void sort_out(Cluster* c) {
+        std::sort(c->edgesOut.begin(), c->edgesOut.end(),
+                  [](Edge* a, Edge* b) { return a->n2->order < b->n2->order; });
+    }
+
+    // ---------------------------------------------------------- assemble
+    void assemble() {  // src/ptree.cpp:696-864
+        App = permute(A, cperm, cperm);
+        std::vector<Cluster*> cl(n);
+        for (Cluster* s : bottoms[0])
+            for (int k = s->cstart; k < s->cstart + s->cols(); ++k) cl[k] = s;
+        for (Cluster* self : bottoms[0]) {
+            std::set<int> nb;
+            for (int j = self->cstart; j < self->cstart + self->cols(); ++j)
+                for (int k = App.colptr[j]; k < App.colptr[j + 1]; ++k) nb.insert(cl[App.rowind[k]]->order);
+            nb.insert(self->order);
+            for (int o : nb) {
+                Cluster* nbr = bottoms[0][o - bottoms[0][0]->order];
+                owned_e.emplace_back(new Edge{self, nbr, Mat(nbr->rows(), self->cols())});
+                Edge* e = owned_e.back().get();
+                for (int j = 0; j < self->cols(); ++j) {
+                    int col = self->cstart + j;
+                    for (int k = App.colptr[col]; k < App.colptr[col + 1]; ++k) {
+                        int r = App.rowind[k];
+                        if (r >= nbr->rstart && r < nbr->rstart + nbr->rows()) e->A(r - nbr->rstart, j) = App.val[k];
+                    }
+                }
+                self->edgesOut.push_back(e);
+                if (self != nbr) {
+                    bool sp = self->lvl() > 0 && nbr->lvl() > 0;
+                    e->spars = sp;
+                    add_edge_in(nbr, e, sp);
+                    if (sp) self->sparsOut.push_back(e);
+                } else {
+                    self->eself = e;
+                }
+            }
+            sort_out(self);
+        }
+        // distance-two connections (src/ptree.cpp:832-854)
+        CSC G = ata_structure(App);
+        std::vector<char> visited(bottoms[0].size(), 0);
+        int o0 = bottoms[0][0]->order;
+        for (Cluster* self : bottoms[0]) {
+            std::fill(visited.begin(), visited.end(), 0);
+            for (int col = self->cstart; col < self->cstart + self->cols(); ++col)
+                for (int k = G.colptr[col]; k < G.colptr[col + 1]; ++k) {
+                    Cluster* pn = cl[G.rowind[k]];
+                    if (visited[pn->order - o0]) continue;
+                    if (pn->lvl() >= self->lvl() && pn != self) self->dist2.insert(pn->order);
+                    else if (pn->lvl() < self->lvl()) self->dist2.insert(pn->dist2.begin(), pn->dist2.end());
+                    visited[pn->order - o0] = 1;
+                }
+        }
+    }
+
+    // ---------------------------------------------------------- factorize
+    void factorize() {
+        auto t0 = std::chrono::steady_clock::now();
+        for (int ilvl = 0; ilvl < nlevels; ++ilvl) {
+            current_bottom = ilvl;
+            fillin();
+            eliminate(ilvl);
+            scale(ilvl);
+            if (ilvl >= skip && ilvl < nlevels - 2 && tol != 0) sparsify(ilvl);
+            if (ilvl < nlevels - 1) merge();
+        }
+        t_factor = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    void fillin() {  // src/ptree.cpp:885-1010
+        for (Cluster* c : interiors[current_bottom]) {
+            for (Edge* e : c->edgesIn) c->dist2.insert(e->n1->order);  // get_sparsity :866-871
+        }
+        for (Cluster* c : interiors[current_bottom]) {
+            std::vector<int> d2(c->dist2.begin(), c->dist2.end());
+            for (int no : d2) {
+                Cluster* nn = by_order(no);
+                // alloc_fillin(c, nn), src/ptree.cpp:140-153
+                std::vector<Edge*> outs = c->edgesOut;
+                for (Edge* e : outs) {
+                    Cluster* nbr = e->n2;
+                    if (!nn->find_out(nbr)) new_edge(nn, nbr, current_bottom);
+                }
+            }
+        }
+    }
+
+    void eliminate(int ilvl) {
+        auto& ints = interiors[current_bottom];
+        // geqrt_cluster (src/ptree.cpp:170-195)
+        for (Cluster* c : ints) {
+            if (c->cols() == 0) throw std::runtime_error("cluster has zero cols");
+            int k = c->cols(), m = c->rows();
+            c->T = Mat(k, k);
+            geqrt(m, k, k, c->eself->A.data(), m, c->T.data(), k);
+            c->eliminated = true;
+            trace.push_back({ilvl, 0, m, k, k});
+        }
+        // larfb_edge on every edgesIn (src/ptree.cpp:216-221)
+        for (Cluster* c : ints) {
+            int k = c->cols();
+            for (Edge* e : c->edgesIn) {
+                larfb_left_trans(c->rows(), e->A.c, k, c->eself->A.data(), c->rows(), c->T.data(), k, e->A.data(),
+                                 e->A.r);
+                trace.push_back({ilvl, 1, c->rows(), e->A.c, k});
+            }
+        }
+        // tsqrt chain over edgesOut after self (src/ptree.cpp:197-214, 1134-1141, 1453-1461)
+        for (Cluster* c : ints) {
+            int k = c->cols();
+            int nb = std::min(32, k);
+            if (c->edgesOut.empty() || c->edgesOut[0]->n2 != c) throw std::runtime_error("first edgeOut must be self");
+            for (size_t j = 1; j < c->edgesOut.size(); ++j) {
+                Edge* e = c->edgesOut[j];
+                Mat T(nb, k);
+                Mat R = c->eself->A.block(0, 0, k, k);
+                tpqrt(e->A.r, k, nb, R.data(), k, e->A.data(), e->A.r, T.data(), nb);
+                c->eself->A.set_block(0, 0, R);
+                c->Tmap[e->n2->order] = std::move(T);
+                trace.push_back({ilvl, 2, e->A.r, k, k});
+            }
+        }
+        // ssrfb_edges: for every m in edgesIn(c), the chain over n in edgesOut(c)
+        // (src/ptree.cpp:223-266, 1366-1406); interiors in cluster order.
+        for (Cluster* c : ints) {
+            int k = c->cols();
+            int nb = std::min(32, k);
+            for (Edge* cm : c->edgesIn) {
+                Cluster* m = cm->n1;
+                Mat top = cm->A.block(0, 0, k, cm->A.c);
+                for (size_t j = 1; j < c->edgesOut.size(); ++j) {
+                    Edge* cn = c->edgesOut[j];
+                    Edge* mn = m->find_out(cn->n2);
+                    if (!mn) throw std::runtime_error("missing fill-in block");
+                    const Mat& T = c->Tmap.at(cn->n2->order);
+                    tpmqrt_lt(mn->A.r, mn->A.c, k, nb, cn->A.data(), cn->A.r, T.data(), nb, top.data(), k,
+                              mn->A.data(), mn->A.r);
+                    trace.push_back({ilvl, 3, mn->A.r, mn->A.c, k});
+                }
+                cm->A.set_block(0, 0, top);
+            }
+        }
+    }
+
+    void scale(int ilvl) {
+        auto& itf = interfaces[current_bottom];
+        // scale_cluster (src/ptree.cpp:269-286)
+        for (Cluster* c : itf) {
+            if (!want_sparsify(c)) continue;
+            Edge* e = c->eself;
+            int k = c->cols(), m = c->rows();
+            c->Qs = e->A;
+            c->Ts = Mat(k, k);
+            geqrt(m, k, k, c->Qs.data(), m, c->Ts.data(), k);
+            for (int j = 0; j < k; ++j)
+                for (int i = 0; i < m; ++i) e->A(i, j) = (i == j) ? 1.0 : 0.0;
+            trace.push_back({ilvl, 4, m, k, k});
+        }
+        // scale apply on every interface-interface edge (src/ptree.cpp:1621-1633)
+        for (Cluster* n1 : itf)
+            for (Edge* e : n1->sparsOut) {
+                Cluster* c = e->n2;
+                if (want_sparsify(c))
+                    larfb_left_trans(e->A.r, e->A.c, c->cols(), c->Qs.data(), c->Qs.r, c->Ts.data(), c->Ts.r,
+                                     e->A.data(), e->A.r);
+                if (want_sparsify(n1)) {
+                    int k = n1->cols();
+                    trsm_right_upper(e->A.r, e->A.c, n1->Qs.data(), n1->Qs.r, e->A.data(), e->A.r);
+                    (void)k;
+                }
+                if (want_sparsify(c) || want_sparsify(n1)) trace.push_back({ilvl, 5, e->A.r, e->A.c, 0});
+            }
+    }
+
+    void sparsify(int ilvl) {
+        auto& itf = interfaces[current_bottom];
+        // sparsify_rrqr_only (src/ptree.cpp:373-439)
+        for (Cluster* c : itf) {
+            if (!want_sparsify(c)) continue;
+            int k = c->cols();
+            int smp = 0, spm = 0;
+            for (Edge* e : c->sparsOut) smp += e->A.r;
+            for (Edge* e : c->sparsIn) spm += e->A.c;
+            Mat C(k, smp + spm);
+            int is = 0;
+            for (Edge* e : c->sparsOut) {
+                for (int j = 0; j < e->A.r; ++j)
+                    for (int i = 0; i < k; ++i) C(i, is + j) = e->A(j, i);
+                is += e->A.r;
+            }
+            for (Edge* e : c->sparsIn) {
+                for (int j = 0; j < e->A.c; ++j)
+                    for (int i = 0; i < k; ++i) C(i, is + j) = e->A(i, j);
+                is += e->A.c;
+            }
+            int w = C.c;
+            int mn = std::min(k, w);
+            std::vector<int> jpvt(w);
+            std::vector<double> tau(mn, 0.0);
+            int rank = 0;
+            if (w > 0 && k > 0) {
+                geqp3(k, w, C.data(), k, jpvt.data(), tau.data());
+                std::vector<double> rii(mn);
+                for (int i = 0; i < mn; ++i) rii[i] = C(i, i);
+                rank = choose_rank(rii.data(), mn, tol);
+            } else {
+                // geqp3 returns without touching C (src/util.cpp:504-505); diagonal is empty
+                rank = choose_rank(nullptr, 0, tol);
+            }
+            trace.push_back({ilvl, 6, k, w, rank});
+            c->Qsp = C.block(0, 0, k, rank);
+            c->tausp.assign(tau.begin(), tau.begin() + rank);
+            c->Tsp = Mat(rank, rank);
+            larft(k, rank, c->Qsp.data(), k, c->tausp.data(), c->Tsp.data(), rank);
+            c->sparsified = true;
+            ranks_before[ilvl].push_back(k);
+            ranks_after[ilvl].push_back(rank);
+            c->rank = rank;
+            c->eself->A = Mat(rank, rank);
+            for (int i = 0; i < rank; ++i) c->eself->A(i, i) = 1.0;
+            c->reset_size(rank, rank);
+        }
+        // sparsify apply (src/ptree.cpp:1867-1897) on every interface-interface edge
+        for (Cluster* n : itf)
+            for (Edge* e : n->sparsOut) {
+                Cluster* c = e->n2;
+                Mat Acopy = e->A;
+                if (want_sparsify(n)) {
+                    larfb_right_notrans(Acopy.r, Acopy.c, n->Qsp.c, n->Qsp.data(), n->Qsp.r, n->Tsp.data(), n->Tsp.r,
+                                        Acopy.data(), Acopy.r);
+                    e->A = Acopy;
+                }
+                e->A.conservative_resize(c->rank, n->rank);
+                if (want_sparsify(c)) {
+                    Mat A11 = Acopy.block(0, 0, c->Qsp.r, n->rank);
+                    larfb_left_trans(A11.r, A11.c, c->Qsp.c, c->Qsp.data(), c->Qsp.r, c->Tsp.data(), c->Tsp.r,
+                                     A11.data(), A11.r);
+                    for (int j = 0; j < n->rank; ++j)
+                        for (int i = 0; i < c->rank; ++i) e->A(i, j) = A11(i, j);
+                }
+                if (want_sparsify(n) || want_sparsify(c)) trace.push_back({ilvl, 7, Acopy.r, Acopy.c, 0});
+            }
+    }
+
+    void merge() {  // src/ptree.cpp:1940-2123 (one rank)
+        for (Cluster* c : interiors[current_bottom]) c->eliminated = true;
+        current_bottom++;
+        for (Cluster* snew : bottoms[current_bottom]) {
+            int rs = 0, cs = 0;
+            for (Cluster* so : snew->children) {
+                so->rposparent = rs;
+                so->cposparent = cs;
+                rs += so->rows();
+                cs += so->cols();
+            }
+            snew->set_size(rs, cs);
+            snew->rsize_org = rs;
+            snew->csize_org = cs;
+        }
+        // compute_new_edges (src/ptree.cpp:289-340): snew in cluster order, neighbours sorted
+        for (Cluster* snew : bottoms[current_bottom]) {
+            std::set<int> nbrs;
+            for (Cluster* so : snew->children) {
+                for (Edge* eo : so->edgesOut)
+                    if (!eo->n2->eliminated) nbrs.insert(eo->n2->parent->order);
+                for (int d2o : so->dist2) {
+                    Cluster* d2c = by_order(d2o);
+                    if (d2c->lvl() >= current_bottom) snew->dist2.insert(d2c->parent->order);
+                }
+            }
+            for (int o : nbrs) {
+                Cluster* nn = by_order(o);
+                Edge* e = new_edge(snew, nn, current_bottom);
+                if (snew == nn) snew->eself = e;
+            }
+            sort_out(snew);
+            for (Cluster* so : snew->children)
+                for (Edge* eo : so->edgesOut) {
+                    Cluster* no = eo->n2;
+                    if (no->eliminated) continue;
+                    Edge* f = snew->find_out(no->parent);
+                    if (eo->A.r != no->rows() || eo->A.c != so->cols()) throw std::runtime_error("merge size mismatch");
+                    f->A.set_block(no->rposparent, so->cposparent, eo->A);
+                    eo->A = Mat();  // the block now lives in the parent
+                }
+        }
+    }
+
+    // ---------------------------------------------------------- solve (preconditioner)
+    void solve(double* xv) {  // src/ptree.cpp:2788-3210
+        for (Cluster* c : bottoms[0]) {
+            c->x.assign(xv + c->cstart, xv + c->cstart + c->csize_org);
+        }
+        // forward, level by level
+        for (int L = 0; L < nlevels; ++L) {
+            if (L > 0)
+                for (Cluster* p : bottoms[L]) {  // merge_fwd :2244-2252
+                    int k = 0;
+                    for (Cluster* ch : p->children)
+                        for (int i = 0; i < ch->rows(); ++i) p->x[k++] = ch->x[i];
+                }
+            for (Cluster* c : interiors[L]) {  // fwd_geqrt :2846-2851
+                int m = c->eself->A.r, k = c->eself->A.c;
+                gemqrt_lt(m, 1, k, k, c->eself->A.data(), m, c->T.data(), k, c->x.data(), m);
+            }
+            for (Cluster* c : interiors[L]) {  // fwd_tsqrt :2160-2186, order_of_trans = cluster order
+                int k = c->cols(), nb = std::min(32, k);
+                for (size_t j = 1; j < c->edgesOut.size(); ++j) {
+                    Edge* e = c->edgesOut[j];
+                    Cluster* nn = e->n2;
+                    const Mat& T = c->Tmap.at(nn->order);
+                    tpmqrt_lt(e->A.r, 1, k, nb, e->A.data(), e->A.r, T.data(), nb, c->x.data(), k, nn->x.data(),
+                              e->A.r);
+                }
+            }
+            for (Cluster* c : interfaces[L]) {  // fwd_spars :2957-2963
+                if (!want_sparsify(c)) continue;
+                int m = c->Qs.r, k = c->Qs.c;
+                gemqrt_lt(m, 1, k, k, c->Qs.data(), m, c->Ts.data(), k, c->x.data(), m);
+                ormqr_lt_vec(c->csize_org, c->Qsp.c, c->Qsp.data(), c->Qsp.r, c->tausp.data(), c->x.data());
+            }
+        }
+        // backward
+        for (int L = nlevels - 1; L >= 0; --L) {
+            for (Cluster* c : interfaces[L]) {  // bwd_spars :3146-3174
+                if (want_sparsify(c)) {
+                    ormqr_ln_vec(c->csize_org, c->Qsp.c, c->Qsp.data(), c->Qsp.r, c->tausp.data(), c->x.data());
+                    trsv_upper(c->csize_org, c->Qs.data(), c->Qs.r, c->x.data());
+                }
+            }
+            for (Cluster* c : interfaces[L])
+                for (Edge* e : c->edgesOut) {
+                    Cluster* n2 = e->n2;
+                    if (n2->lvl() != c->merge_level) continue;
+                    int s = e->A.c, k2 = n2->cols();
+                    for (int j = 0; j < s; ++j) {
+                        double xj = c->x[j];
+                        for (int i = 0; i < k2; ++i) n2->x[i] -= e->A(i, j) * xj;
+                    }
+                }
+            for (Cluster* c : interiors[L])  // bwd_trsv :2188-2192
+                trsv_upper(c->cols(), c->eself->A.data(), c->eself->A.r, c->x.data());
+            if (L > 0)
+                for (Cluster* p : bottoms[L]) {  // bwd_merge :3105-3123
+                    int k = 0;
+                    for (Cluster* ch : p->children)
+                        for (int i = 0; i < ch->cols(); ++i) ch->x[i] = p->x[k++];
+                }
+        }
+        for (Cluster* c : bottoms[0])
+            for (int i = 0; i < c->csize_org; ++i) xv[c->cstart + i] = c->x[i];
+    }
+
+    void spmv(const double* x, double* y) const {  // src/ptree.cpp:3212-3262
+        for (int i = 0; i < n; ++i) y[i] = 0.0;
+        for (int j = 0; j < n; ++j)
+            for (int k = App.colptr[j]; k < App.colptr[j + 1]; ++k) y[App.rowind[k]] += App.val[k] * x[j];
+    }
+};
+
+static double dot(const std::vector<double>& a, const double* b) {
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+
+// dist_gmres (src/is.cpp:447-527) with x0 = 0, MGS Arnoldi and Givens; break on the
+// preconditioned residual as the reference driver does (verb = true).  restart > 0 is an
+// extension (restarted GMRES, x0 = current iterate) not present in the reference.
+struct GmresResult { int iters; double prec_res; };
+
+static GmresResult gmres_cycle(Tree& t, const std::vector<double>& b, std::vector<double>& x, int iter, double tol,
+                               std::vector<double>* hist) {
+    const int n = t.n;
+    std::vector<double> r(n), tmp(n);
+    // r = M^{-1}(b - A x)
+    t.spmv(x.data(), tmp.data());
+    for (int i = 0; i < n; ++i) r[i] = b[i] - tmp[i];
+    t.solve(r.data());
+    double r_norm = std::sqrt(dot(r, r.data()));
+    if (r_norm == 0.0) return {0, 0.0};
+    std::vector<double> sn(iter, 0.0), cs(iter, 0.0), beta(iter + 1, 0.0);
+    std::vector<std::vector<double>> Q;
+    Q.push_back(r);
+    for (auto& v : Q[0]) v /= r_norm;
+    beta[0] = r_norm;
+    std::vector<double> H((size_t)(iter + 1) * iter, 0.0);
+    auto h = [&](int i, int j) -> double& { return H[i + (size_t)j * (iter + 1)]; };
+    int k = 0;
+    double err = 1.0;
+    for (; k < iter; ++k) {
+        std::vector<double> q(n);
+        t.spmv(Q[k].data(), q.data());
+        t.solve(q.data());
+        for (int i = 0; i < k + 1; ++i) {
+            h(i, k) = dot(q, Q[i].data());
+            for (int l = 0; l < n; ++l) q[l] -= h(i, k) * Q[i][l];
+        }
+        h(k + 1, k) = std::sqrt(dot(q, q.data()));
+        for (auto& v : q) v /= h(k + 1, k);
+        Q.push_back(q);
+        for (int i = 0; i < k; ++i) {
+            double temp = cs[i] * h(i, k) + sn[i] * h(i + 1, k);
+            h(i + 1, k) = -sn[i] * h(i, k) + cs[i] * h(i + 1, k);
+            h(i, k) = temp;
+        }
+        if (h(k, k) == 0) { cs[k] = 0; sn[k] = 1; }
+        else {
+            double tt = std::sqrt(h(k, k) * h(k, k) + h(k + 1, k) * h(k + 1, k));
+            cs[k] = std::fabs(h(k, k)) / tt;
+            sn[k] = cs[k] * h(k + 1, k) / h(k, k);
+        }
+        h(k, k) = cs[k] * h(k, k) + sn[k] * h(k + 1, k);
+        h(k + 1, k) = 0.0;
+        beta[k + 1] = -sn[k] * beta[k];
+        beta[k] = cs[k] * beta[k];
+        err = std::fabs(beta[k + 1]) / r_norm;
+        if (hist) hist->push_back(err);
+        if (err <= tol) break;
+    }
+    k = std::min(k, iter - 1);
+    // LUSolve (src/util.cpp:1019-1032) and x += Q y
+    std::vector<double> y(k + 1, 0.0);
+    for (int i = k; i >= 0; --i) {
+        y[i] = beta[i];
+        for (int j = i + 1; j < k + 1; ++j) y[i] -= h(i, j) * y[j];
+        y[i] /= h(i, i);
+    }
+    for (int j = 0; j <= k; ++j)
+        for (int l = 0; l < n; ++l) x[l] += Q[j][l] * y[j];
+    return {k, err * r_norm};
+}
+
+}  // namespace orc
+
+// ====================================================================== C API (ctypes)
+using namespace orc;
+
+extern "C" {
+
+void* orc_tree_create(int n, const int* colptr, const int* rowind, const double* val, int cn, int cd, int nlevels,
+                      double tol, int skip) {
+    Tree* t = new Tree(nlevels, skip);
+    t->tol = tol;
+    t->n = n;
+    t->A.nr = t->A.nc = n;
+    t->A.colptr.assign(colptr, colptr + n + 1);
+    t->A.rowind.assign(rowind, rowind + colptr[n]);
+    t->A.val.assign(val, val + colptr[n]);
+    t->X = linspace_nd(cn, cd);
+    t->d = cd;
+    return t;
+}
+
+void orc_tree_destroy(void* h) { delete (Tree*)h; }
+
+int orc_partition_assemble(void* h) {
+    try {
+        Tree* t = (Tree*)h;
+        t->partition();
+        t->assemble();
+        return 0;
+    } catch (std::exception& e) {
+        fprintf(stderr, "oracle: %s\n", e.what());
+        return -1;
+    }
+}
+
+int orc_factorize(void* h) {
+    try {
+        ((Tree*)h)->factorize();
+        return 0;
+    } catch (std::exception& e) {
+        fprintf(stderr, "oracle: %s\n", e.what());
+        return -1;
+    }
+}
+
+double orc_factor_seconds(void* h) { return ((Tree*)h)->t_factor; }
+
+void orc_get_cperm(void* h, int* out) {
+    Tree* t = (Tree*)h;
+    std::memcpy(out, t->cperm.data(), sizeof(int) * t->n);
+}
+
+// Cluster table: per cluster (order) -> merge_level, self.lvl, self.sep, l.lvl, l.sep, r.lvl,
+// r.sep, section, part, cstart, rows_at_creation(cols_org), parent order, final rank (or -1).
+int orc_num_clusters(void* h) { return (int)((Tree*)h)->owned_c.size(); }
+void orc_get_clusters(void* h, int* out /* 13 * ncl */) {
+    Tree* t = (Tree*)h;
+    for (auto& up : t->owned_c) {
+        Cluster* c = up.get();
+        int* o = out + 13 * c->order;
+        o[0] = c->merge_level; o[1] = c->id.self.lvl; o[2] = c->id.self.sep;
+        o[3] = c->id.l.lvl; o[4] = c->id.l.sep; o[5] = c->id.r.lvl; o[6] = c->id.r.sep;
+        o[7] = c->id.section; o[8] = c->id.part; o[9] = c->cstart; o[10] = c->csize_org;
+        o[11] = c->parent ? c->parent->order : -1;
+        o[12] = c->sparsified ? c->rank : -1;
+    }
+}
+
+// Per-interior factor blocks for parity: diagonal R (upper triangle of A(c,c)).
+int orc_get_diag_R(void* h, int order, double* out, int maxn) {
+    Tree* t = (Tree*)h;
+    Cluster* c = t->owned_c[order].get();
+    if (!c->eself) return -1;
+    int k = c->eself->A.c;
+    if ((size_t)k * k > (size_t)maxn) return -2;
+    for (int j = 0; j < k; ++j)
+        for (int i = 0; i < k; ++i) out[i + (size_t)j * k] = (i <= j) ? c->eself->A(i, j) : 0.0;
+    return k;
+}
+
+void orc_solve(void* h, double* x) { ((Tree*)h)->solve(x); }
+void orc_spmv(void* h, const double* x, double* y) { ((Tree*)h)->spmv(x, y); }
+
+// GMRES on App x = b (permuted space).  restart <= 0: one cycle of `iters` (the reference).
+// Returns total iterations (0-based k of the last cycle + cycles*restart, as counted by the
+// reference's returned k); writes x and the preconditioned-residual history.
+int orc_gmres(void* h, const double* b, double* x, int iters, double tol, int restart, double true_tol,
+              double* hist, int hist_cap, int* hist_len) {
+    Tree* t = (Tree*)h;
+    std::vector<double> bb(b, b + t->n), xx(t->n, 0.0), hv;
+    int total = 0;
+    if (restart <= 0) {
+        GmresResult r = gmres_cycle(*t, bb, xx, iters, tol, &hv);
+        total = r.iters;
+    } else {
+        int done = 0;
+        double bn = std::sqrt(dot(bb, bb.data()));
+        while (done < iters) {
+            int cyc = std::min(restart, iters - done);
+            GmresResult r = gmres_cycle(*t, bb, xx, cyc, tol, &hv);
+            done += r.iters + 1;
+            std::vector<double> ax(t->n);
+            t->spmv(xx.data(), ax.data());
+            double rn = 0.0;
+            for (int i = 0; i < t->n; ++i) rn += (bb[i] - ax[i]) * (bb[i] - ax[i]);
+            if (std::sqrt(rn) / bn <= true_tol) break;
+        }
+        total = done - 1;
+    }
+    std::memcpy(x, xx.data(), sizeof(double) * t->n);
+    int nh = std::min((int)hv.size(), hist_cap);
+    for (int i = 0; i < nh; ++i) hist[i] = hv[i];
+    *hist_len = nh;
+    return total;
+}
+
+// Trace of dense work: 5 ints per record (level, op, m, n, k).
+int orc_trace_len(void* h) { return (int)((Tree*)h)->trace.size(); }
+void orc_get_trace(void* h, int* out) {
+    Tree* t = (Tree*)h;
+    for (size_t i = 0; i < t->trace.size(); ++i) {
+        auto& r = t->trace[i];
+        out[5 * i] = r.level; out[5 * i + 1] = r.op; out[5 * i + 2] = r.m; out[5 * i + 3] = r.n; out[5 * i + 4] = r.k;
+    }
+}
+
+// ---------------- dense kernels for unit tests (column-major, lda = rows)
+void orc_geqrt(int m, int n, int nb, double* A, double* T) { geqrt(m, n, nb, A, m, T, nb); }
+void orc_tpqrt(int m, int n, int nb, double* A, double* B, double* T) { tpqrt(m, n, nb, A, n, B, m, T, nb); }
+void orc_tpmqrt_lt(int m, int n, int k, int nb, const double* V, const double* T, double* A, double* B) {
+    tpmqrt_lt(m, n, k, nb, V, m, T, nb, A, k, B, m);
+}
+void orc_larft(int m, int k, const double* V, const double* tau, double* T) { larft(m, k, V, m, tau, T, k); }
+void orc_larfb_lt(int m, int n, int k, const double* V, const double* T, double* C) {
+    larfb_left_trans(m, n, k, V, m, T, k, C, m);
+}
+void orc_larfb_rn(int m, int n, int k, const double* V, const double* T, double* C) {
+    larfb_right_notrans(m, n, k, V, n, T, k, C, m);
+}
+void orc_geqp3(int m, int n, double* A, int* jpvt, double* tau) { geqp3(m, n, A, m, jpvt, tau); }
+int orc_choose_rank(const double* s, int n, double tol) { return choose_rank(s, n, tol); }
+void orc_trsm_right_upper(int m, int n, const double* R, double* B) { trsm_right_upper(m, n, R, n, B, m); }
+void orc_trsv_upper(int n, const double* R, double* x) { trsv_upper(n, R, n, x); }
+void orc_ormqr_lt(int m, int k, const double* V, const double* tau, double* x) { ormqr_lt_vec(m, k, V, m, tau, x); }
+void orc_ormqr_ln(int m, int k, const double* V, const double* tau, double* x) { ormqr_ln_vec(m, k, V, m, tau, x); }
+
+}  // extern "C"

@@ -1,0 +1,218 @@
+"""Synthetic test matrices and right-hand sides for the spaQR path.
+
+Every generator returns a square scipy CSC matrix in natural tensor-grid ordering
+(id = sum_i x_i * n^(d-1-i), the ordering of ``linspace_nd`` in the reference,
+src/util.cpp:901-930), so the geometric partition ``-n n -d d`` applies to it.
+
+Random streams reproduce ``std::mt19937`` + ``std::uniform_real_distribution<double>``
+bit for bit (the reference's ``random()``, src/util.cpp:992-1001): numpy's legacy
+``RandomState(seed)`` seeds MT19937 with the same ``init_genrand`` and its raw 32-bit
+outputs are combined the way libstdc++'s ``generate_canonical<double, 53>`` does.
+Normal variates use Box-Muller on those canonical draws (portable across standard
+libraries, unlike ``std::normal_distribution``).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.sparse as sp
+
+_TWO32 = 4294967296.0
+_TWO64 = 18446744073709551616.0
+
+
+def mt_canonical(seed: int, count: int) -> np.ndarray:
+    """``count`` draws of generate_canonical<double,53>(std::mt19937(seed))."""
+    rs = np.random.RandomState(seed)
+    raw = rs._bit_generator.random_raw(2 * count).astype(np.float64)
+    s = raw[0::2] + raw[1::2] * _TWO32
+    u = s / _TWO64
+    return np.minimum(u, np.nextafter(1.0, 0.0))
+
+
+def mt_uniform(seed: int, count: int, a: float = -1.0, b: float = 1.0) -> np.ndarray:
+    """std::uniform_real_distribution<double>(a, b) over std::mt19937(seed)."""
+    return mt_canonical(seed, count) * (b - a) + a
+
+
+def mt_normal(seed: int, count: int, sigma: float = 1.0) -> np.ndarray:
+    """N(0, sigma^2) by Box-Muller on canonical mt19937 draws (pairs -> two variates)."""
+    npairs = (count + 1) // 2
+    u = mt_canonical(seed, 2 * npairs)
+    u1, u2 = u[0::2], u[1::2]
+    rad = np.sqrt(-2.0 * np.log1p(-u1))
+    z = np.empty(2 * npairs)
+    z[0::2] = rad * np.cos(2.0 * math.pi * u2)
+    z[1::2] = rad * np.sin(2.0 * math.pi * u2)
+    return sigma * z[:count]
+
+
+def _grid_index(n: int, d: int):
+    idx = np.arange(n ** d).reshape((n,) * d)
+    return idx
+
+
+def _assemble(n: int, d: int, face_coef, diag_extra=None, upwind=None) -> sp.csc_matrix:
+    """7/5-point operator: sum over axes of -kappa_face (u_nbr - u_i); Dirichlet boundary.
+
+    face_coef(axis) -> array of shape grid with the coefficient of the face between cell i
+    and cell i + e_axis (last index along the axis is the boundary face to the right);
+    the left boundary face of cell 0 uses face_coef_left(axis).
+    """
+    N = n ** d
+    idx = _grid_index(n, d)
+    rows, cols, vals = [], [], []
+    diag = np.zeros((n,) * d)
+    for ax in range(d):
+        kf, kl = face_coef(ax)  # kf: right faces (n along ax incl. boundary), kl: left boundary face
+        # interior faces between i and i+e_ax: slices [0:n-1] along ax
+        sl_a = [slice(None)] * d
+        sl_b = [slice(None)] * d
+        sl_a[ax] = slice(0, n - 1)
+        sl_b[ax] = slice(1, n)
+        k_int = kf[tuple(sl_a)]
+        ia = idx[tuple(sl_a)].ravel()
+        ib = idx[tuple(sl_b)].ravel()
+        kv = k_int.ravel()
+        rows += [ia, ib]
+        cols += [ib, ia]
+        vals += [-kv, -kv]
+        diag[tuple(sl_a)] += k_int
+        diag[tuple(sl_b)] += k_int
+        # boundary faces
+        sl_last = [slice(None)] * d
+        sl_last[ax] = slice(n - 1, n)
+        diag[tuple(sl_last)] += kf[tuple(sl_last)]
+        sl_first = [slice(None)] * d
+        sl_first[ax] = slice(0, 1)
+        diag[tuple(sl_first)] += kl
+    if upwind is not None:
+        # first-order upwind convection b . grad u with b_ax >= 0: b_ax (u_i - u_{i-e_ax})
+        for ax, bax in enumerate(upwind):
+            if bax == 0.0:
+                continue
+            diag += bax
+            sl_a = [slice(None)] * d
+            sl_b = [slice(None)] * d
+            sl_a[ax] = slice(1, n)
+            sl_b[ax] = slice(0, n - 1)
+            ia = idx[tuple(sl_a)].ravel()
+            ib = idx[tuple(sl_b)].ravel()
+            rows.append(ia)
+            cols.append(ib)
+            vals.append(np.full(ia.size, -bax))
+    rows.append(idx.ravel())
+    cols.append(idx.ravel())
+    vals.append(diag.ravel())
+    A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+    A = A.tocsc()
+    A.sum_duplicates()
+    A.sort_indices()
+    return A
+
+
+def poisson(n: int, d: int) -> sp.csc_matrix:
+    """5-point (d=2) / 7-point (d=3) Poisson, Dirichlet: A_ii = 2d, A_ij = -1."""
+    ones = np.ones((n,) * d)
+    return _assemble(n, d, lambda ax: (ones, ones[tuple([slice(0, 1) if a == ax else slice(None) for a in range(d)])]))
+
+
+def _harmonic_faces(kcell: np.ndarray, d: int, axis_scale=None):
+    n = kcell.shape[0]
+
+    def face(ax):
+        s = 1.0 if axis_scale is None else axis_scale[ax]
+        kf = np.empty_like(kcell)
+        sl_a = [slice(None)] * d
+        sl_b = [slice(None)] * d
+        sl_a[ax] = slice(0, n - 1)
+        sl_b[ax] = slice(1, n)
+        a, b = kcell[tuple(sl_a)], kcell[tuple(sl_b)]
+        kf[tuple(sl_a)] = 2.0 * a * b / (a + b)
+        sl_last = [slice(None)] * d
+        sl_last[ax] = slice(n - 1, n)
+        kf[tuple(sl_last)] = kcell[tuple(sl_last)]  # boundary face uses the cell value
+        sl_first = [slice(None)] * d
+        sl_first[ax] = slice(0, 1)
+        return s * kf, s * kcell[tuple(sl_first)]
+
+    return face
+
+
+def diffusion_varcoef(n: int, d: int, seed: int, sigma: float = 1.0, axis_scale=None) -> sp.csc_matrix:
+    """Cell coefficient exp(N(0, sigma^2)); face value = harmonic mean of its two cells;
+    boundary faces take the cell value; optional per-axis conductivity factors."""
+    kcell = np.exp(mt_normal(seed, n ** d, sigma)).reshape((n,) * d)
+    return _assemble(n, d, _harmonic_faces(kcell, d, axis_scale))
+
+
+def convdiff_upwind(n: int, d: int, velocity=(1.0, 0.5, 0.25), peclet: float = 10.0) -> sp.csc_matrix:
+    """-eps Lap u + b . grad u, first-order upwind, h = 1, eps = |b|_inf h / Pe (nonsymmetric)."""
+    b = np.asarray(velocity[:d], dtype=np.float64)
+    eps = float(np.max(np.abs(b))) / peclet
+    ones = np.full((n,) * d, eps)
+    first = lambda ax: ones[tuple([slice(0, 1) if a == ax else slice(None) for a in range(d)])]
+    return _assemble(n, d, lambda ax: (ones, first(ax)), upwind=tuple(b))
+
+
+def column_scale(A: sp.csc_matrix) -> sp.csc_matrix:
+    """A <- A D 10 with D_jj = 1/||A(:,j)||_2 (pspaQR.cpp:132-147)."""
+    A = A.tocsc(copy=True)
+    sq = np.zeros(A.shape[1])
+    for j in range(A.shape[1]):
+        v = A.data[A.indptr[j]:A.indptr[j + 1]]
+        sq[j] = np.sum(v * v)
+    # the reference sums in index order; np.sum on tiny columns is sequential enough, but do it
+    # explicitly to keep the same rounding
+    d = 1.0 / np.sqrt(sq)
+    A.data = A.data * np.repeat(d, np.diff(A.indptr)) * 10.0
+    return A
+
+
+@dataclass
+class Problem:
+    name: str
+    A: sp.csc_matrix          # scaled, natural ordering
+    b: np.ndarray             # natural ordering
+    n_side: int
+    dim: int
+    nlevels: int
+    tol: float
+    skip: int = 0
+    meta: dict = field(default_factory=dict)
+
+
+def default_levels(ncols: int) -> int:
+    """nlevels = ceil(log2(ncols/64)) with integer division (pspaQR.cpp:108-113)."""
+    return int(math.ceil(math.log2(ncols // 64)))
+
+
+def make_problem(cfg: int | str, scale_side: int | None = None) -> Problem:
+    """The five workloads; ``scale_side`` shrinks the grid (parity tests at oracle sizes)."""
+    if cfg in (1, "poisson2d"):
+        n = scale_side or 64
+        A = poisson(n, 2)
+        b = mt_uniform(0, n * n)
+        lv = 6 if scale_side is None else max(2, default_levels(n * n))
+        return Problem("poisson2d_%d" % n, column_scale(A), b, n, 2, lv, 1e-2)
+    if cfg in (2, "diffusion3d"):
+        n = scale_side or 32
+        A = diffusion_varcoef(n, 3, seed=1, sigma=1.0)
+        b = mt_uniform(1, n ** 3)
+        lv = 9 if scale_side is None else max(2, default_levels(n ** 3))
+        return Problem("diffusion3d_%d" % n, column_scale(A), b, n, 3, lv, 1e-3)
+    if cfg in (3, "convdiff3d"):
+        n = scale_side or 64
+        A = convdiff_upwind(n, 3)
+        b = mt_normal(2, n ** 3)
+        lv = 12 if scale_side is None else max(2, default_levels(n ** 3))
+        return Problem("convdiff3d_%d" % n, column_scale(A), b, n, 3, lv, 1e-2, meta={"restart": 50})
+    if cfg in (5, "anisotropic3d"):
+        n = scale_side or 96
+        A = diffusion_varcoef(n, 3, seed=7, sigma=0.5, axis_scale=(1.0, 1e-2, 1e-4))
+        b = mt_normal(7 + 1000, n ** 3)
+        lv = default_levels(n ** 3)
+        return Problem("anisotropic3d_%d" % n, column_scale(A), b, n, 3, lv, 1e-2)
+    raise ValueError("unknown config %r" % (cfg,))
