@@ -1,0 +1,164 @@
+/*
+ * spaqr_gpu.h -- C ABI of the MI355X (gfx950) spaQR hot path.
+ *
+ * Two layers, both plain C (pointers and sizes only):
+ *
+ *  1. The ParTree facade: the host API of the reference's parallel tree
+ *     (include/ptree.h:123-159, include/tree.h:137-143, src/is.cpp:447-527), so a
+ *     caller that drove ParTree + dist_gmres drives this instead.  Vectors passed to
+ *     solve / spmv / gmres are DEVICE pointers (FP64) in the tree's permuted ordering,
+ *     the ordering the reference's solve(x) / spmv(x) / dist_gmres work in.
+ *
+ *  2. Grouped batched kernels: LAPACK-semantics routines the reference calls per block
+ *     (inventory of call sites: geqrt src/ptree.cpp:187, tpqrt :206, tpmqrt :248,
+ *     larfb src/util.cpp:468-493, trsm src/util.cpp:233-241, geqp3 + choose_rank
+ *     src/util.cpp:501-512,830-851, larft :403-411).  Each call takes an array of task
+ *     descriptors in HOST memory whose matrix pointers are DEVICE pointers (column-major,
+ *     explicit leading dimensions); all tasks run in one launch, one workgroup per task.
+ *     These calls copy the descriptors, launch on `stream` and synchronize it.
+ *
+ * Precision: `prec` is 64 (double) or 32 (float) for every matrix pointer of a call.
+ * T factors are stored in 32-column blocks: T is 32 x k (ld 32), one 32 x 32 upper
+ * triangular block per panel of 32 reflectors (LAPACK dgeqrt / dtpqrt storage, NB = 32).
+ *
+ * Return values: 0 on success, < 0 on a bad argument (LAPACK info convention: -i names
+ * the offending argument), -100 on a HIP runtime failure; spaqr_last_error() describes it.
+ */
+#ifndef SPAQR_GPU_H
+#define SPAQR_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ ParTree facade */
+typedef struct spaqr_tree spaqr_tree;
+
+typedef struct {
+    int iters;        /* as dist_gmres returns: 0-based k of the converged step (summed over restarts) */
+    int cycles;       /* restart cycles run (1 without restart) */
+    double prec_res;  /* final preconditioned residual estimate |beta_{k+1}| / ||M^-1 r0|| */
+    double true_res;  /* ||b - A x|| / ||b|| computed in FP64 on the device */
+} spaqr_gmres_result;
+
+/* ParTree(int nlevels, int skip)                                     include/ptree.h:135 */
+spaqr_tree* spaqr_tree_create(int nlevels, int skip);
+void spaqr_tree_destroy(spaqr_tree* t);
+/* Tree::set_tol / set_skip / set_scale / set_square / set_order       src/tree.cpp:14-19 */
+int spaqr_tree_set_tol(spaqr_tree* t, double tol);
+int spaqr_tree_set_skip(spaqr_tree* t, int skip);
+int spaqr_tree_set_scale(spaqr_tree* t, int scale);
+int spaqr_tree_set_square(spaqr_tree* t, int square);
+int spaqr_tree_set_order(spaqr_tree* t, float order);
+/* ParTree::set_verbose                                                  src/ptree.cpp:22 */
+int spaqr_tree_set_verbose(spaqr_tree* t, int verbose);
+/* Tree::set_Xcoo: coordinates X (d x n, column-major)                    src/tree.cpp:20 */
+int spaqr_tree_set_Xcoo(spaqr_tree* t, const double* X, int d, int n);
+/* Factor precision: 64 (reference FP64 path) or 32 (FP32 fronts; GMRES stays FP64).
+   No counterpart in the reference.  Set before spaqr_tree_assemble. */
+int spaqr_tree_set_precision(spaqr_tree* t, int bits);
+/* ParTree::partition(SpMat& A): A is n x n CSC (host), sorted row indices. src/ptree.cpp:495 */
+int spaqr_tree_partition(spaqr_tree* t, int nrows, int ncols, const int* colptr, const int* rowind,
+                         const double* val);
+/* ParTree::assemble(SpMat& A): builds the blocks and uploads them to HBM. src/ptree.cpp:696 */
+int spaqr_tree_assemble(spaqr_tree* t);
+/* ParTree::factorize()                                                 src/ptree.cpp:873 */
+int spaqr_tree_factorize(spaqr_tree* t);
+/* ParTree::solve(VectorXd& x): x <- M^{-1} x, in place (device)       src/ptree.cpp:2788 */
+int spaqr_tree_solve(spaqr_tree* t, double* x_dev);
+/* ParTree::spmv(VectorXd x): y = A x (device)                         src/ptree.cpp:3212 */
+int spaqr_tree_spmv(spaqr_tree* t, const double* x_dev, double* y_dev);
+/* dist_gmres(A, b, M, iter, tol, verb=true): x solves A x = b (device vectors, permuted
+   ordering).  restart <= 0 is the reference's single cycle; restart > 0 restarts from the
+   current iterate until ||b - A x|| / ||b|| <= true_tol or iters steps.  hist (host) gets
+   the per-step preconditioned residual estimates.                         src/is.cpp:447 */
+int spaqr_gmres(spaqr_tree* t, const double* b_dev, double* x_dev, int iters, double tol, int restart,
+                double true_tol, spaqr_gmres_result* out, double* hist, int hist_cap, int* hist_len);
+/* Tree::rows() / cols() and ParTree::nrows_local / ncols_local (one rank)  src/tree.cpp:23 */
+int spaqr_tree_n(const spaqr_tree* t);
+/* column permutation (cperm, natural index of permuted position i)        include/tree.h:53 */
+int spaqr_tree_get_cperm(const spaqr_tree* t, int* out);
+/* cluster table, 13 ints per cluster in creation order: merge level, self (lvl, sep), l (lvl,
+   sep), r (lvl, sep), section, part, cstart, cols at creation, parent order, rank (-1 if the
+   cluster was not sparsified)                                            include/cluster.h:142 */
+int spaqr_tree_num_clusters(const spaqr_tree* t);
+int spaqr_tree_get_clusters(const spaqr_tree* t, int* out);
+/* per-level statistics, 8 doubles per level: interiors, interfaces, sparsified, max front,
+   colour layers, sum of k before, sum of ranks after, algorithmic flops              */
+int spaqr_tree_level_stats(const spaqr_tree* t, double* out, int max_levels);
+/* upper triangle of the factored diagonal block of an interior (k x k col-major); returns k */
+int spaqr_tree_diag_R(spaqr_tree* t, int order, double* out, int maxn);
+double spaqr_tree_factor_ms(const spaqr_tree* t);
+/* the HIP stream (hipStream_t) every call of this tree is ordered on */
+void* spaqr_tree_stream(spaqr_tree* t);
+const char* spaqr_last_error(const spaqr_tree* t);
+/* wait for every call issued on this tree's stream */
+int spaqr_tree_sync(spaqr_tree* t);
+/* Kernel-class timing with HIP events on the tree's stream (no reference counterpart).
+   Classes, in order: 0 geqrt (interior diagonal QR), 1 larfb (Q^T on interior rows),
+   2 tpqrt chains, 3 tpmqrt chains (trailing updates), 4 scaling QR, 5 scaling apply,
+   6 RRQR gather, 7 geqp3, 8 sparsify apply + merge, 9 fills, 10 preconditioner solve,
+   11 SpMV, 12 GMRES orthogonalisation.  Per class 5 doubles: milliseconds, launches,
+   tasks, algorithmic flops, algorithmic bytes.  Timings are folded in at each sync. */
+int spaqr_tree_set_profiling(spaqr_tree* t, int on);
+int spaqr_tree_profile(spaqr_tree* t, double* out, int max_classes);
+int spaqr_tree_profile_reset(spaqr_tree* t);
+
+/* ------------------------------------------------------------------ device memory helpers
+   For callers without a GPU runtime of their own (cgo, JNI, ctypes).  No reference
+   counterpart: the reference keeps everything in host memory. */
+void* spaqr_device_alloc(size_t bytes);
+int spaqr_device_free(void* p);
+int spaqr_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes);
+int spaqr_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes);
+int spaqr_device_synchronize(void);
+
+/* ------------------------------------------------------------------ batched kernels */
+/* dgeqrt(m, n, nb=32, A, lda, T, 32): A <- V, R; T is 32 x min(m,n).    LAPACKE_dgeqrt */
+typedef struct { void* A; void* T; int m, n, lda, pad; } spaqr_geqrt_desc;
+int spaqr_geqrt_batched(int prec, int count, const spaqr_geqrt_desc* d, void* stream);
+
+/* C (m x n) <- Q^T C with Q from a geqrt (V m x k, T 32 x k); then, if R != NULL,
+   C <- C R^{-1} (R upper k2 x k2, n == k2).  dlarfb('L','T','F','C') / cblas_dtrsm(Right,
+   Upper, NoTrans, NonUnit).  V may be NULL (trsm only). */
+typedef struct { const void* V; const void* T; void* C; const void* R; int m, n, k, ldv, ldc, k2, ldr, pad; } spaqr_larfb_desc;
+int spaqr_larfb_batched(int prec, int count, const spaqr_larfb_desc* d, void* stream);
+
+/* dtpqrt(M=rows, N=k, L=0, NB=32) chained: for each link j of a task, in order,
+   [R; B_j] <- QR, V_j in B_j, T_j (32 x k).  One task per interior front.              */
+typedef struct { void* B; void* T; int rows, ldb; } spaqr_tpqrt_link;
+typedef struct { void* R; int k, ldr, link0, nlinks; } spaqr_tpqrt_desc;
+int spaqr_tpqrt_batched(int prec, int count, const spaqr_tpqrt_desc* d, int nlinks, const spaqr_tpqrt_link* links,
+                        void* stream);
+
+/* dtpmqrt('L','T', M=rows, N=ncols, K=k, L=0, NB=32) chained over the links of a task:
+   [Atop (k x ncols, lda); B_j(:, col0:col0+ncols)] <- Q_j^T [...].                      */
+typedef struct { const void* V; const void* T; void* B; int rows, ldv, ldb, pad; } spaqr_tpmqrt_link;
+typedef struct { void* Atop; int k, lda, ncols, link0, nlinks, col0; } spaqr_tpmqrt_desc;
+int spaqr_tpmqrt_batched(int prec, int count, const spaqr_tpmqrt_desc* d, int nlinks, const spaqr_tpmqrt_link* links,
+                         void* stream);
+
+/* dgeqp3 + choose_rank(|R_ii|, tol) on C (k x w, ld k), stopped at the rank: rank_out (device
+   int), V = C(:, 0:rank) in Vout (ld k, k x min(k,w) capacity), tau, T (32 x rank), plus
+   scratch vn1, vn2 (w each) and jpvt (w ints, 0-based pivots).  C is overwritten.
+   tie_rel = 0 pivots on LAPACK's argmax (first maximum); tie_rel > 0 picks the lowest index
+   whose partial norm is >= (1 - tie_rel) * max (the tree path uses 1e-10). */
+typedef struct { void* C; void* Vout; void* tau; void* T; void* vn1; void* vn2; int* jpvt; int* rank_out; int k, w, pad0, pad1; } spaqr_geqp3_desc;
+int spaqr_geqp3_batched(int prec, int count, const spaqr_geqp3_desc* d, double tol, double tie_rel, void* stream);
+
+/* Sparsification apply + scatter (src/ptree.cpp:1867-1897 then :330): A (mc x nc) <- A Q_n
+   (Vn != NULL, first rn reflectors of Vn kn x rn), A[0:kc, 0:rn] <- Q_c^T A (Vc != NULL,
+   first rc reflectors), dst(i, j) = A(i, j) for i < rc, j < rn; ident != 0 writes I_rc. */
+typedef struct {
+    void* A; const void* Vn; const void* Tn; const void* Vc; const void* Tc; void* dst;
+    int mc, nc, lda, kn, rn, kc, rc, ldvn, ldvc, ldd, ident, pad;
+} spaqr_spars_desc;
+int spaqr_spars_apply_batched(int prec, int count, const spaqr_spars_desc* d, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPAQR_GPU_H */

@@ -329,11 +329,19 @@ inline void ormqr_ln_vec(int m, int k, const double* V, int ldv, const double* t
     }
 }
 
+inline long& geqp3_near_ties() {
+    static long count = 0;
+    return count;
+}
+
 // dgeqp3 (column-pivoted QR; the dlaqp2 step applied to every column).  jpvt is 0-based
 // on return (the reference decrements LAPACK's 1-based pivots, src/util.cpp:510-511).
 // Pivot = first index of the largest partial column norm; norms are downdated with
 // LAPACK's cancellation guard (recompute when temp2 <= sqrt(eps)).
-inline void geqp3(int m, int n, double* A, int lda, int* jpvt, double* tau) {
+// tie_rel > 0 replaces the argmax by "lowest index whose norm is >= (1 - tie_rel) * max":
+// identical to LAPACK whenever the best column wins by more than tie_rel, and independent of
+// rounding order when two columns tie in exact arithmetic (symmetric grids produce such ties).
+inline void geqp3(int m, int n, double* A, int lda, int* jpvt, double* tau, double tie_rel = 0.0) {
     if (m == 0 || n == 0) return;
     int k = std::min(m, n);
     std::vector<double> vn1(n), vn2(n);
@@ -343,11 +351,22 @@ inline void geqp3(int m, int n, double* A, int lda, int* jpvt, double* tau) {
         vn1[j] = vn2[j] = std::sqrt(s);
         jpvt[j] = j;
     }
-    const double tol3z = std::sqrt(2.220446049250313e-16);
+    const double tol3z = std::sqrt(1.1102230246251565e-16);  // sqrt(dlamch('E'))
     for (int i = 0; i < k; ++i) {
         int pvt = i;
         for (int j = i + 1; j < n; ++j)
             if (std::fabs(vn1[j]) > std::fabs(vn1[pvt])) pvt = j;
+        if (tie_rel > 0.0) {
+            const double thr = std::fabs(vn1[pvt]) * (1.0 - tie_rel);
+            for (int j = i; j < n; ++j)
+                if (std::fabs(vn1[j]) >= thr) { pvt = j; break; }
+        }
+        {
+            // diagnostic: count pivot steps whose winner is within 1e-12 of a runner-up
+            double best = std::fabs(vn1[pvt]);
+            for (int j = i; j < n; ++j)
+                if (j != pvt && best > 0 && std::fabs(vn1[j]) >= best * (1.0 - 1e-12)) { geqp3_near_ties()++; break; }
+        }
         if (pvt != i) {
             for (int r = 0; r < m; ++r) std::swap(A[r + (size_t)pvt * lda], A[r + (size_t)i * lda]);
             std::swap(jpvt[pvt], jpvt[i]);

@@ -57,6 +57,10 @@ def lib():
         L.orc_larfb_lt.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
         L.orc_larfb_rn.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
         L.orc_geqp3.argtypes = [C.c_int, C.c_int, _dp, _ip, _dp]
+        L.orc_geqp3_tie.argtypes = [C.c_int, C.c_int, _dp, _ip, _dp, C.c_double]
+        L.orc_geqp3_near_ties.restype = C.c_long
+        L.orc_geqp3_near_ties.argtypes = [C.c_int]
+        L.orc_gemqrt_lt.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
         L.orc_choose_rank.argtypes = [_dp, C.c_int, C.c_double]
         L.orc_choose_rank.restype = C.c_int
         L.orc_trsm_right_upper.argtypes = [C.c_int, C.c_int, _dp, _dp]

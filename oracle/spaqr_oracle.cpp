@@ -39,6 +39,10 @@
 
 namespace orc {
 
+// Relative band inside which geqp3 pivot candidates count as tied (lowest index wins); the
+// HIP kernel uses the same rule, so FP64 runs pick the same pivots despite rounding order.
+constexpr double kPivotTieRel = 1e-10;
+
 // ------------------------------------------------------------------ IDs
 struct SepID {
     int lvl = -1, sep = 0;
@@ -651,7 +655,7 @@ struct Tree {
             std::vector<double> tau(mn, 0.0);
             int rank = 0;
             if (w > 0 && k > 0) {
-                geqp3(k, w, C.data(), k, jpvt.data(), tau.data());
+                geqp3(k, w, C.data(), k, jpvt.data(), tau.data(), kPivotTieRel);
                 std::vector<double> rii(mn);
                 for (int i = 0; i < mn; ++i) rii[i] = C(i, i);
                 rank = choose_rank(rii.data(), mn, tol);
@@ -1023,7 +1027,19 @@ void orc_larfb_rn(int m, int n, int k, const double* V, const double* T, double*
     larfb_right_notrans(m, n, k, V, n, T, k, C, m);
 }
 void orc_geqp3(int m, int n, double* A, int* jpvt, double* tau) { geqp3(m, n, A, m, jpvt, tau); }
+void orc_geqp3_tie(int m, int n, double* A, int* jpvt, double* tau, double tie_rel) {
+    geqp3(m, n, A, m, jpvt, tau, tie_rel);
+}
+// C (m x n) <- Q^T C with Q from geqrt(m, k, nb) (V m x k, T nb x k)
+void orc_gemqrt_lt(int m, int n, int k, int nb, const double* V, const double* T, double* C) {
+    gemqrt_lt(m, n, k, nb, V, m, T, nb, C, m);
+}
 int orc_choose_rank(const double* s, int n, double tol) { return choose_rank(s, n, tol); }
+long orc_geqp3_near_ties(int reset) {
+    long v = geqp3_near_ties();
+    if (reset) geqp3_near_ties() = 0;
+    return v;
+}
 void orc_trsm_right_upper(int m, int n, const double* R, double* B) { trsm_right_upper(m, n, R, n, B, m); }
 void orc_trsv_upper(int n, const double* R, double* x) { trsv_upper(n, R, n, x); }
 void orc_ormqr_lt(int m, int k, const double* V, const double* tau, double* x) { ormqr_lt_vec(m, k, V, m, tau, x); }

@@ -1,0 +1,301 @@
+// C ABI (include/spaqr_gpu.h) over the level planner and the batched kernels.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/spaqr_gpu.h"
+#include "kernels.hpp"
+#include "launch.hpp"
+#include "tree.hpp"
+
+using namespace spaqr;
+
+struct spaqr_tree {
+    Tree* t;
+};
+
+static_assert(sizeof(spaqr_geqrt_desc) == sizeof(GeqrtTask<double>), "geqrt desc layout");
+static_assert(sizeof(spaqr_larfb_desc) == sizeof(LarfbTask<double>), "larfb desc layout");
+static_assert(sizeof(spaqr_tpqrt_desc) == sizeof(TpqrtTask<double>), "tpqrt desc layout");
+static_assert(sizeof(spaqr_tpqrt_link) == sizeof(TpqrtLink<double>), "tpqrt link layout");
+static_assert(sizeof(spaqr_tpmqrt_desc) == sizeof(TpmqrtTask<double>), "tpmqrt desc layout");
+static_assert(sizeof(spaqr_tpmqrt_link) == sizeof(TpmqrtLink<double>), "tpmqrt link layout");
+static_assert(sizeof(spaqr_geqp3_desc) == sizeof(Geqp3Task<double>), "geqp3 desc layout");
+static_assert(sizeof(spaqr_spars_desc) == sizeof(SparsApplyTask<double>), "spars desc layout");
+
+static thread_local std::string g_err;
+
+// batched-kernel call: upload descriptors, launch, synchronize, free
+template <typename F>
+static int run_batched(const void* host, size_t bytes, const void* host2, size_t bytes2, void* stream, F launch) {
+    hipStream_t s = (hipStream_t)stream;
+    void* d = nullptr;
+    void* d2 = nullptr;
+    if (hipMalloc(&d, bytes ? bytes : 8) != hipSuccess) { g_err = "hipMalloc failed"; return -100; }
+    if (bytes2 && hipMalloc(&d2, bytes2) != hipSuccess) { (void)hipFree(d); g_err = "hipMalloc failed"; return -100; }
+    hipError_t e = hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && bytes2) e = hipMemcpyAsync(d2, host2, bytes2, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        launch(d, d2, s);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d);
+    if (d2) (void)hipFree(d2);
+    if (e != hipSuccess) { g_err = hipGetErrorString(e); return -100; }
+    return 0;
+}
+
+
+extern "C" {
+
+spaqr_tree* spaqr_tree_create(int nlevels, int skip) {
+    if (nlevels < 1 || skip < 0) { g_err = "nlevels must be >= 1 and skip >= 0"; return nullptr; }
+    try {
+        spaqr_tree* h = new spaqr_tree;
+        h->t = new Tree(nlevels, skip);
+        return h;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void spaqr_tree_destroy(spaqr_tree* h) {
+    if (!h) return;
+    delete h->t;
+    delete h;
+}
+
+#define TREE_OR_FAIL(h)                 \
+    if (!h || !h->t) {                  \
+        g_err = "null tree handle";     \
+        return -1;                      \
+    }
+
+int spaqr_tree_set_tol(spaqr_tree* h, double tol) { TREE_OR_FAIL(h); if (!(tol >= 0)) return -2; h->t->tol = tol; return 0; }
+int spaqr_tree_set_skip(spaqr_tree* h, int s) { TREE_OR_FAIL(h); if (s < 0) return -2; h->t->skip = s; return 0; }
+int spaqr_tree_set_scale(spaqr_tree* h, int s) {
+    TREE_OR_FAIL(h);
+    if (s != 1) { h->t->last_error = "ParTree::factorize requires scale == 1 (src/ptree.cpp:875)"; return -2; }
+    h->t->scale = s;
+    return 0;
+}
+int spaqr_tree_set_square(spaqr_tree* h, int s) {
+    TREE_OR_FAIL(h);
+    if (s != 1) { h->t->last_error = "only square systems are supported"; return -2; }
+    h->t->square = s;
+    return 0;
+}
+int spaqr_tree_set_order(spaqr_tree* h, float o) {
+    TREE_OR_FAIL(h);
+    if (o != 1.0f) { h->t->last_error = "ParTree supports order 1 only (pspaQR.cpp:174-180)"; return -2; }
+    h->t->order = o;
+    return 0;
+}
+int spaqr_tree_set_verbose(spaqr_tree* h, int v) { TREE_OR_FAIL(h); h->t->verbose = v; return 0; }
+int spaqr_tree_set_precision(spaqr_tree* h, int bits) {
+    TREE_OR_FAIL(h);
+    if (bits != 64 && bits != 32) { h->t->last_error = "precision must be 64 or 32"; return -2; }
+    h->t->prec = bits;
+    return 0;
+}
+int spaqr_tree_set_Xcoo(spaqr_tree* h, const double* X, int d, int n) {
+    TREE_OR_FAIL(h);
+    if (!X || d <= 0 || n <= 0) return -2;
+    h->t->X.assign(X, X + (size_t)d * n);
+    h->t->xdim = d;
+    return 0;
+}
+int spaqr_tree_partition(spaqr_tree* h, int nrows, int ncols, const int* colptr, const int* rowind, const double* val) {
+    TREE_OR_FAIL(h);
+    if (nrows <= 0 || ncols <= 0 || !colptr || !rowind || !val) return -2;
+    HostCSC A;
+    A.nr = nrows;
+    A.nc = ncols;
+    A.colptr.assign(colptr, colptr + ncols + 1);
+    A.rowind.assign(rowind, rowind + colptr[ncols]);
+    A.val.assign(val, val + colptr[ncols]);
+    return h->t->partition(A);
+}
+int spaqr_tree_assemble(spaqr_tree* h) { TREE_OR_FAIL(h); return h->t->assemble(); }
+int spaqr_tree_factorize(spaqr_tree* h) { TREE_OR_FAIL(h); return h->t->factorize(); }
+int spaqr_tree_solve(spaqr_tree* h, double* x) { TREE_OR_FAIL(h); if (!x) return -2; return h->t->solve_dev(x); }
+int spaqr_tree_spmv(spaqr_tree* h, const double* x, double* y) {
+    TREE_OR_FAIL(h);
+    if (!x || !y) return -2;
+    return h->t->spmv_dev(x, y);
+}
+int spaqr_gmres(spaqr_tree* h, const double* b, double* x, int iters, double tol, int restart, double true_tol,
+                spaqr_gmres_result* out, double* hist, int hist_cap, int* hist_len) {
+    TREE_OR_FAIL(h);
+    if (!b || !x || iters <= 0) return -2;
+    GmresResult r;
+    int rc = h->t->gmres_dev(b, x, iters, tol, restart, true_tol, &r, hist, hist_cap, hist_len);
+    if (out) { out->iters = r.iters; out->cycles = r.cycles; out->prec_res = r.prec_res; out->true_res = r.true_res; }
+    return rc;
+}
+int spaqr_tree_n(const spaqr_tree* h) { return (h && h->t) ? h->t->n_unknowns() : -1; }
+int spaqr_tree_get_cperm(const spaqr_tree* h, int* out) {
+    TREE_OR_FAIL(h);
+    const auto& p = h->t->cperm();
+    std::memcpy(out, p.data(), sizeof(int) * p.size());
+    return 0;
+}
+int spaqr_tree_num_clusters(const spaqr_tree* h) { return (h && h->t) ? h->t->num_clusters() : -1; }
+int spaqr_tree_get_clusters(const spaqr_tree* h, int* out) {
+    TREE_OR_FAIL(h);
+    for (int o = 0; o < h->t->num_clusters(); ++o) {
+        const Cluster& c = h->t->cluster(o);
+        int* r = out + 13 * o;
+        r[0] = c.merge_level; r[1] = c.id.self.lvl; r[2] = c.id.self.sep;
+        r[3] = c.id.l.lvl; r[4] = c.id.l.sep; r[5] = c.id.r.lvl; r[6] = c.id.r.sep;
+        r[7] = c.id.section; r[8] = c.id.part; r[9] = c.cstart; r[10] = c.cols_org;
+        r[11] = c.parent; r[12] = c.sparsified ? c.rank : -1;
+    }
+    return 0;
+}
+int spaqr_tree_level_stats(const spaqr_tree* h, double* out, int max_levels) {
+    TREE_OR_FAIL(h);
+    const auto& s = h->t->level_stats();
+    int nl = std::min((int)s.size(), max_levels);
+    for (int i = 0; i < nl; ++i) {
+        double* r = out + 8 * i;
+        r[0] = s[i].n_interiors; r[1] = s[i].n_interfaces; r[2] = s[i].n_sparsified; r[3] = s[i].max_front;
+        r[4] = s[i].colors; r[5] = (double)s[i].cols_before; r[6] = (double)s[i].ranks_after; r[7] = s[i].flops;
+    }
+    return nl;
+}
+int spaqr_tree_diag_R(spaqr_tree* h, int order, double* out, int maxn) { TREE_OR_FAIL(h); return h->t->diag_R(order, out, maxn); }
+double spaqr_tree_factor_ms(const spaqr_tree* h) { return (h && h->t) ? h->t->t_factor_ms : -1.0; }
+void* spaqr_tree_stream(spaqr_tree* h) { return (h && h->t) ? (void*)h->t->stream() : nullptr; }
+const char* spaqr_last_error(const spaqr_tree* h) {
+    if (h && h->t && !h->t->last_error.empty()) return h->t->last_error.c_str();
+    return g_err.c_str();
+}
+
+int spaqr_tree_sync(spaqr_tree* h) {
+    TREE_OR_FAIL(h);
+    return h->t->sync();
+}
+int spaqr_tree_set_profiling(spaqr_tree* h, int on) {
+    TREE_OR_FAIL(h);
+    h->t->profiling = on != 0;
+    return 0;
+}
+int spaqr_tree_profile(spaqr_tree* h, double* out, int max_classes) {
+    TREE_OR_FAIL(h);
+    int rc = h->t->sync();
+    if (rc) return rc;
+    int nc = std::min((int)P_NCAT, max_classes);
+    for (int i = 0; i < nc; ++i) {
+        const ProfEntry& p = h->t->prof[i];
+        out[5 * i] = p.ms; out[5 * i + 1] = (double)p.launches; out[5 * i + 2] = (double)p.tasks;
+        out[5 * i + 3] = p.flops; out[5 * i + 4] = p.bytes;
+    }
+    return nc;
+}
+int spaqr_tree_profile_reset(spaqr_tree* h) {
+    TREE_OR_FAIL(h);
+    int rc = h->t->sync();
+    h->t->prof_reset();
+    return rc;
+}
+
+void* spaqr_device_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) { g_err = "hipMalloc failed"; return nullptr; }
+    return p;
+}
+int spaqr_device_free(void* p) { return hipFree(p) == hipSuccess ? 0 : -100; }
+int spaqr_memcpy_h2d(void* d, const void* h, size_t bytes) {
+    return hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : -100;
+}
+int spaqr_memcpy_d2h(void* h, const void* d, size_t bytes) {
+    return hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -100;
+}
+int spaqr_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -100; }
+
+// ---------------------------------------------------------------- batched kernels
+int spaqr_geqrt_batched(int prec, int count, const spaqr_geqrt_desc* d, void* stream) {
+    if (prec != 32 && prec != 64) return -1;
+    if (count < 0 || (count > 0 && !d)) return -2;
+    for (int i = 0; i < count; ++i)
+        if (d[i].m < 0 || d[i].n < 0 || d[i].lda < std::max(1, d[i].m)) return -3;
+    if (count == 0) return 0;
+    return run_batched(d, sizeof(*d) * count, nullptr, 0, stream, [&](void* dd, void*, hipStream_t s) {
+        if (prec == 64) launch_geqrt<double>((const GeqrtTask<double>*)dd, count, s);
+        else launch_geqrt<float>((const GeqrtTask<float>*)dd, count, s);
+    });
+}
+
+int spaqr_larfb_batched(int prec, int count, const spaqr_larfb_desc* d, void* stream) {
+    if (prec != 32 && prec != 64) return -1;
+    if (count < 0 || (count > 0 && !d)) return -2;
+    for (int i = 0; i < count; ++i)
+        if (d[i].m < 0 || d[i].n < 0 || d[i].k < 0 || d[i].ldc < std::max(1, d[i].m) || (d[i].R && d[i].n != d[i].k2))
+            return -3;
+    if (count == 0) return 0;
+    return run_batched(d, sizeof(*d) * count, nullptr, 0, stream, [&](void* dd, void*, hipStream_t s) {
+        if (prec == 64) launch_larfb<double>((const LarfbTask<double>*)dd, count, s);
+        else launch_larfb<float>((const LarfbTask<float>*)dd, count, s);
+    });
+}
+
+int spaqr_tpqrt_batched(int prec, int count, const spaqr_tpqrt_desc* d, int nlinks, const spaqr_tpqrt_link* links,
+                        void* stream) {
+    if (prec != 32 && prec != 64) return -1;
+    if (count < 0 || (count > 0 && !d)) return -2;
+    for (int i = 0; i < count; ++i)
+        if (d[i].link0 < 0 || d[i].nlinks < 0 || d[i].link0 + d[i].nlinks > nlinks) return -3;
+    if (count == 0) return 0;
+    return run_batched(d, sizeof(*d) * count, links, sizeof(*links) * nlinks, stream,
+                       [&](void* dd, void* dl, hipStream_t s) {
+                           if (prec == 64)
+                               launch_tpqrt<double>((const TpqrtTask<double>*)dd, (const TpqrtLink<double>*)dl, count, s);
+                           else
+                               launch_tpqrt<float>((const TpqrtTask<float>*)dd, (const TpqrtLink<float>*)dl, count, s);
+                       });
+}
+
+int spaqr_tpmqrt_batched(int prec, int count, const spaqr_tpmqrt_desc* d, int nlinks, const spaqr_tpmqrt_link* links,
+                         void* stream) {
+    if (prec != 32 && prec != 64) return -1;
+    if (count < 0 || (count > 0 && !d)) return -2;
+    for (int i = 0; i < count; ++i)
+        if (d[i].link0 < 0 || d[i].nlinks < 0 || d[i].link0 + d[i].nlinks > nlinks) return -3;
+    if (count == 0) return 0;
+    return run_batched(d, sizeof(*d) * count, links, sizeof(*links) * nlinks, stream,
+                       [&](void* dd, void* dl, hipStream_t s) {
+                           if (prec == 64)
+                               launch_tpmqrt<double>((const TpmqrtTask<double>*)dd, (const TpmqrtLink<double>*)dl, count, s);
+                           else
+                               launch_tpmqrt<float>((const TpmqrtTask<float>*)dd, (const TpmqrtLink<float>*)dl, count, s);
+                       });
+}
+
+int spaqr_geqp3_batched(int prec, int count, const spaqr_geqp3_desc* d, double tol, double tie_rel, void* stream) {
+    if (prec != 32 && prec != 64) return -1;
+    if (count < 0 || (count > 0 && !d)) return -2;
+    if (!(tol >= 0)) return -4;
+    if (!(tie_rel >= 0 && tie_rel < 1)) return -5;
+    if (count == 0) return 0;
+    return run_batched(d, sizeof(*d) * count, nullptr, 0, stream, [&](void* dd, void*, hipStream_t s) {
+        if (prec == 64) launch_geqp3<double>((const Geqp3Task<double>*)dd, count, tol, tie_rel, s);
+        else launch_geqp3<float>((const Geqp3Task<float>*)dd, count, tol, tie_rel, s);
+    });
+}
+
+int spaqr_spars_apply_batched(int prec, int count, const spaqr_spars_desc* d, void* stream) {
+    if (prec != 32 && prec != 64) return -1;
+    if (count < 0 || (count > 0 && !d)) return -2;
+    if (count == 0) return 0;
+    return run_batched(d, sizeof(*d) * count, nullptr, 0, stream, [&](void* dd, void*, hipStream_t s) {
+        if (prec == 64) launch_spars_apply<double>((const SparsApplyTask<double>*)dd, count, s);
+        else launch_spars_apply<float>((const SparsApplyTask<float>*)dd, count, s);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,986 @@
+// gfx950 kernels for the spaQR per-front hot path (see kernels.hpp for the task layouts).
+//
+// Workgroup-level building blocks:
+//   wg_panel_qr   Householder QR of a <= 32-column panel (geqrt2 / tpqrt2 with L = 0),
+//                 producing the panel's T; 8 lanes of one wavefront own one column, so the
+//                 column dot products reduce with three intra-wave shuffles.
+//   wg_apply_lt   H^T [Atop; C] with a block reflector of <= 32 columns (larfb 'L','T' /
+//                 tprfb 'L','T'): W = Atop + V^T C staged through LDS in 64-row chunks,
+//                 W <- T^T W, C -= V W with V rows held in registers.
+//   wg_apply_rn   C H (larfb 'R','N') for the sparsification's column transform.
+//   wg_trsm_ru    C R^{-1} (trsm Right/Upper/NoTrans/NonUnit), 32-column blocked.
+// Each batched kernel walks its descriptor (a front, a chain of neighbours, a block) with
+// one 256-thread workgroup, so every front of a level is factored in one grouped launch.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace spaqr {
+
+template <typename S>
+struct ApplySmem {
+    S Vs[32 * 65];
+    S Cs[32 * 65];
+    S W[32 * 33];
+    S Ts[32 * 33];
+};
+
+template <typename S>
+struct PanelSmem {
+    S Ts[32 * 33];
+    S d[32];
+    S t[32];
+    S tau, scal;
+};
+
+template <typename S>
+__device__ __forceinline__ S wave_sum(S v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------------------
+// Panel QR.  PENT = false: the panel is A (m x nbp, ld ldr == ldb, Rt == B): reflector j
+// is [1; A(j+1:m, j)] acting on rows j..m-1.  PENT = true: the panel is [R (nbp x nbp
+// upper, ldr); B (m x nbp, ldb)], reflector j is [e_j; B(:, j)] (dtpqrt2 with L = 0).
+// The nbp x nbp T of the panel is written to Tout (ld ldt).
+template <typename S, bool PENT>
+__device__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* Tout, int ldt, PanelSmem<S>& sm) {
+    const int tid = threadIdx.x;
+    const int l = tid >> 3, g = tid & 7;
+    for (int e = tid; e < 32 * 33; e += WG) sm.Ts[e] = S(0);
+    __syncthreads();
+    for (int j = 0; j < nbp; ++j) {
+        const int r0 = PENT ? 0 : j + 1;
+        S d = S(0);
+        if (l < nbp) {
+            const S* bj = B + (size_t)j * ldb;
+            const S* bl = B + (size_t)l * ldb;
+            for (int r = r0 + g; r < m; r += 8) d += bj[r] * bl[r];
+        }
+        d += __shfl_xor(d, 1, 64);
+        d += __shfl_xor(d, 2, 64);
+        d += __shfl_xor(d, 4, 64);
+        if (g == 0) sm.d[l] = d;
+        __syncthreads();
+        if (tid == 0) {
+            S alpha = Rt[j + (size_t)j * ldr];
+            S ss = sm.d[j];
+            S tau, scal, beta;
+            if (ss == S(0)) {
+                tau = S(0); scal = S(1); beta = alpha;
+            } else {
+                beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+                tau = (beta - alpha) / beta;
+                scal = S(1) / (alpha - beta);
+            }
+            Rt[j + (size_t)j * ldr] = beta;
+            sm.tau = tau;
+            sm.scal = scal;
+        }
+        __syncthreads();
+        const S tau = sm.tau, scal = sm.scal;
+        if (l > j && l < nbp) {
+            S wl = (Rt[j + (size_t)l * ldr] + scal * sm.d[l]) * tau;
+            const S ws = wl * scal;
+            const S* bj = B + (size_t)j * ldb;
+            S* bl = B + (size_t)l * ldb;
+            for (int r = r0 + g; r < m; r += 8) bl[r] -= bj[r] * ws;
+            if (g == 0) Rt[j + (size_t)l * ldr] -= wl;
+        }
+        if (tid < j) {
+            S tv = scal * sm.d[tid];
+            if (!PENT) tv += Rt[j + (size_t)tid * ldr];
+            sm.t[tid] = tv;
+        }
+        __syncthreads();
+        if (l == j) {
+            S* bj = B + (size_t)j * ldb;
+            for (int r = r0 + g; r < m; r += 8) bj[r] *= scal;
+        }
+        if (tid < j) {
+            S s = S(0);
+            for (int q = tid; q < j; ++q) s += sm.Ts[q * 33 + tid] * sm.t[q];
+            sm.Ts[j * 33 + tid] = -tau * s;
+        } else if (tid == j) {
+            sm.Ts[j * 33 + j] = tau;
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < nbp * nbp; e += WG) {
+        int i = e % nbp, jj = e / nbp;
+        Tout[i + (size_t)jj * ldt] = (i <= jj) ? sm.Ts[jj * 33 + i] : S(0);
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------------
+// [Atop; C] <- H^T [Atop; C] (PENT, V is the m x kb lower block of [I; V]) or
+// C <- H^T C (unit lower V, m x kb, implicit ones / zeros above).  kb <= 32.
+template <typename S, bool PENT>
+__device__ void wg_apply_lt(int m, int kb, const S* __restrict__ V, int ldv, const S* __restrict__ T, int ldt,
+                            S* Atop, int lda, S* C, int ldc, int ncols, ApplySmem<S>& sm) {
+    const int tid = threadIdx.x;
+    __syncthreads();
+    for (int e = tid; e < 32 * 32; e += WG) {
+        int i = e & 31, j = e >> 5;
+        sm.Ts[j * 33 + i] = (i <= j && j < kb) ? T[i + (size_t)j * ldt] : S(0);
+    }
+    const int wi = tid & 31, wq = tid >> 5;
+    const int rr = tid & 63, cg = tid >> 6;
+    for (int c0 = 0; c0 < ncols; c0 += 32) {
+        const int nc = min(32, ncols - c0);
+        S acc[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int j = wq * 4 + q;
+            acc[q] = (PENT && wi < kb && j < nc) ? Atop[wi + (size_t)(c0 + j) * lda] : S(0);
+        }
+        for (int r0 = 0; r0 < m; r0 += 64) {
+            const int rb = min(64, m - r0);
+            __syncthreads();
+            for (int e = tid; e < 64 * 32; e += WG) {
+                int r = e & 63, i = e >> 6;
+                S v = S(0), cv = S(0);
+                if (r < rb) {
+                    int R = r0 + r;
+                    if (i < kb) {
+                        if (PENT) v = V[R + (size_t)i * ldv];
+                        else v = (R > i) ? V[R + (size_t)i * ldv] : (R == i ? S(1) : S(0));
+                    }
+                    if (i < nc) cv = C[R + (size_t)(c0 + i) * ldc];
+                }
+                sm.Vs[i * 65 + r] = v;
+                sm.Cs[i * 65 + r] = cv;
+            }
+            __syncthreads();
+            for (int r = 0; r < rb; ++r) {
+                S v = sm.Vs[wi * 65 + r];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] += v * sm.Cs[(wq * 4 + q) * 65 + r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sm.W[(wq * 4 + q) * 33 + wi] = acc[q];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            S s = S(0);
+            for (int l2 = 0; l2 <= wi; ++l2) s += sm.Ts[wi * 33 + l2] * sm.W[(wq * 4 + q) * 33 + l2];
+            acc[q] = s;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sm.W[(wq * 4 + q) * 33 + wi] = (wi < kb) ? acc[q] : S(0);
+        __syncthreads();
+        if (PENT) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                int j = wq * 4 + q;
+                if (wi < kb && j < nc) Atop[wi + (size_t)(c0 + j) * lda] -= sm.W[j * 33 + wi];
+            }
+        }
+        for (int r0 = 0; r0 < m; r0 += 64) {
+            const int R = r0 + rr;
+            if (R < m) {
+                S vrow[32];
+#pragma unroll
+                for (int i = 0; i < 32; ++i) {
+                    S v = S(0);
+                    if (i < kb) {
+                        if (PENT) v = V[R + (size_t)i * ldv];
+                        else v = (R > i) ? V[R + (size_t)i * ldv] : (R == i ? S(1) : S(0));
+                    }
+                    vrow[i] = v;
+                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    int j = cg * 8 + q;
+                    if (j < nc) {
+                        S s = S(0);
+#pragma unroll
+                        for (int i = 0; i < 32; ++i) s += vrow[i] * sm.W[j * 33 + i];
+                        C[R + (size_t)(c0 + j) * ldc] -= s;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------------
+// C (m x n) <- C H, H = I - V T V^T, V (n x kb unit lower, ldv), kb <= 32.
+template <typename S>
+__device__ void wg_apply_rn(int m, int n, int kb, const S* __restrict__ V, int ldv, const S* __restrict__ T, int ldt,
+                            S* C, int ldc, ApplySmem<S>& sm) {
+    const int tid = threadIdx.x;
+    __syncthreads();
+    for (int e = tid; e < 32 * 32; e += WG) {
+        int i = e & 31, j = e >> 5;
+        sm.Ts[j * 33 + i] = (i <= j && j < kb) ? T[i + (size_t)j * ldt] : S(0);
+    }
+    const int rr = tid & 63, cg = tid >> 6;  // row in a 64-row tile, group of 8 reflectors
+    for (int r0 = 0; r0 < m; r0 += 64) {
+        const int R = r0 + rr;
+        S acc[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] = S(0);
+        // W = C V  (staging V chunks of 64 rows: Vs[i * 65 + l])
+        for (int l0 = 0; l0 < n; l0 += 64) {
+            const int lb = min(64, n - l0);
+            __syncthreads();
+            for (int e = tid; e < 64 * 32; e += WG) {
+                int l = e & 63, i = e >> 6;
+                S v = S(0);
+                if (l < lb && i < kb) {
+                    int L = l0 + l;
+                    v = (L > i) ? V[L + (size_t)i * ldv] : (L == i ? S(1) : S(0));
+                }
+                sm.Vs[i * 65 + l] = v;
+            }
+            __syncthreads();
+            if (R < m) {
+                for (int l = 0; l < lb; ++l) {
+                    S c = C[R + (size_t)(l0 + l) * ldc];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) acc[q] += c * sm.Vs[(cg * 8 + q) * 65 + l];
+                }
+            }
+        }
+        // W <- W T : stage W rows in Cs (Cs[i * 65 + rr])
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sm.Cs[(cg * 8 + q) * 65 + rr] = acc[q];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            int i = cg * 8 + q;
+            S s = S(0);
+            for (int l = 0; l <= i; ++l) s += sm.Cs[l * 65 + rr] * sm.Ts[i * 33 + l];
+            acc[q] = (i < kb) ? s : S(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sm.Cs[(cg * 8 + q) * 65 + rr] = acc[q];
+        __syncthreads();
+        // C -= W V^T : thread (rr, cg) covers columns l = cg, cg + 4, ...
+        if (R < m) {
+            S w[32];
+#pragma unroll
+            for (int i = 0; i < 32; ++i) w[i] = sm.Cs[i * 65 + rr];
+            for (int l = cg; l < n; l += 4) {
+                S s = S(0);
+#pragma unroll
+                for (int i = 0; i < 32; ++i) {
+                    if (i < kb) {
+                        S v = (l > i) ? V[l + (size_t)i * ldv] : (l == i ? S(1) : S(0));
+                        s += w[i] * v;
+                    }
+                }
+                C[R + (size_t)l * ldc] -= s;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------------
+// B (m x n) <- B R^{-1}, R upper (ldr).  32-column blocks: B_j -= B_{<j} R_{<j,j} then a
+// per-row triangular solve with the 32 x 32 diagonal block in LDS.
+template <typename S>
+__device__ void wg_trsm_ru(int m, int n, const S* __restrict__ R, int ldr, S* B, int ldb, ApplySmem<S>& sm) {
+    const int tid = threadIdx.x;
+    const int rr = tid & 63, cg = tid >> 6;
+    for (int j0 = 0; j0 < n; j0 += 32) {
+        const int jb = min(32, n - j0);
+        if (j0 > 0) {
+            for (int r0 = 0; r0 < m; r0 += 64) {
+                const int Rw = r0 + rr;
+                S acc[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) acc[q] = S(0);
+                for (int l0 = 0; l0 < j0; l0 += 32) {
+                    __syncthreads();
+                    for (int e = tid; e < 32 * 32; e += WG) {
+                        int l = e & 31, q = e >> 5;
+                        sm.W[q * 33 + l] = (l0 + l < j0 && q < jb) ? R[(l0 + l) + (size_t)(j0 + q) * ldr] : S(0);
+                    }
+                    __syncthreads();
+                    if (Rw < m) {
+                        const int lb = min(32, j0 - l0);
+                        for (int l = 0; l < lb; ++l) {
+                            S b = B[Rw + (size_t)(l0 + l) * ldb];
+#pragma unroll
+                            for (int q = 0; q < 8; ++q) acc[q] += b * sm.W[(cg * 8 + q) * 33 + l];
+                        }
+                    }
+                }
+                if (Rw < m) {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (cg * 8 + q < jb) B[Rw + (size_t)(j0 + cg * 8 + q) * ldb] -= acc[q];
+                }
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < 32 * 32; e += WG) {
+            int i = e & 31, q = e >> 5;
+            sm.Ts[q * 33 + i] = (i < jb && q < jb) ? R[(j0 + i) + (size_t)(j0 + q) * ldr] : S(0);
+        }
+        __syncthreads();
+        for (int r = tid; r < m; r += WG) {
+            S x[32];
+#pragma unroll
+            for (int q = 0; q < 32; ++q) x[q] = (q < jb) ? B[r + (size_t)(j0 + q) * ldb] : S(0);
+#pragma unroll
+            for (int q = 0; q < 32; ++q) {
+                if (q < jb) {
+                    S s = x[q];
+#pragma unroll
+                    for (int l = 0; l < q; ++l) s -= x[l] * sm.Ts[q * 33 + l];
+                    x[q] = s / sm.Ts[q * 33 + q];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 32; ++q)
+                if (q < jb) B[r + (size_t)(j0 + q) * ldb] = x[q];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Blocked T (NB x k) of the unit-lower reflectors V (m x k, ldv) with scalars tau.
+template <typename S>
+__device__ void wg_larft_blocked(int m, int k, const S* V, int ldv, const S* tau, S* Tout, PanelSmem<S>& sm) {
+    const int tid = threadIdx.x;
+    const int l = tid >> 3, g = tid & 7;
+    for (int p0 = 0; p0 < k; p0 += NB) {
+        const int nbp = min(NB, k - p0);
+        __syncthreads();
+        for (int e = tid; e < 32 * 33; e += WG) sm.Ts[e] = S(0);
+        __syncthreads();
+        for (int q = 0; q < nbp; ++q) {
+            // G(i, q) for i < q: V(p0+q, p0+i) + sum_{r > p0+q} V(r, p0+i) V(r, p0+q); thread column l = i
+            S d = S(0);
+            if (l < q) {
+                const S* vi = V + (size_t)(p0 + l) * ldv;
+                const S* vq = V + (size_t)(p0 + q) * ldv;
+                for (int r = p0 + q + 1 + g; r < m; r += 8) d += vi[r] * vq[r];
+            }
+            d += __shfl_xor(d, 1, 64);
+            d += __shfl_xor(d, 2, 64);
+            d += __shfl_xor(d, 4, 64);
+            if (g == 0 && l < q) sm.t[l] = d + V[(p0 + q) + (size_t)(p0 + l) * ldv];
+            __syncthreads();
+            const S tq = tau[p0 + q];
+            if (tid < q) {
+                S s = S(0);
+                for (int c = tid; c < q; ++c) s += sm.Ts[c * 33 + tid] * sm.t[c];
+                sm.Ts[q * 33 + tid] = (tq == S(0)) ? S(0) : -tq * s;
+            } else if (tid == q) {
+                sm.Ts[q * 33 + q] = tq;
+            }
+            __syncthreads();
+        }
+        for (int e = tid; e < nbp * nbp; e += WG) {
+            int i = e % nbp, jj = e / nbp;
+            Tout[i + (size_t)(p0 + jj) * NB] = (i <= jj) ? sm.Ts[jj * 33 + i] : S(0);
+        }
+    }
+    __syncthreads();
+}
+
+// =======================================================================================
+// Batched kernels
+// =======================================================================================
+
+template <typename S>
+__global__ __launch_bounds__(WG) void k_geqrt(const GeqrtTask<S>* __restrict__ tasks) {
+    __shared__ ApplySmem<S> asm_;
+    __shared__ PanelSmem<S> psm;
+    const GeqrtTask<S> t = tasks[blockIdx.x];
+    const int kmin = min(t.m, t.n);
+    for (int p0 = 0; p0 < kmin; p0 += NB) {
+        const int nbp = min(NB, kmin - p0);
+        S* Ap = t.A + p0 + (size_t)p0 * t.lda;
+        wg_panel_qr<S, false>(t.m - p0, nbp, Ap, t.lda, Ap, t.lda, t.T + (size_t)p0 * NB, NB, psm);
+        if (p0 + nbp < t.n)
+            wg_apply_lt<S, false>(t.m - p0, nbp, Ap, t.lda, t.T + (size_t)p0 * NB, NB, nullptr, 0,
+                                  Ap + (size_t)nbp * t.lda, t.lda, t.n - p0 - nbp, asm_);
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(WG) void k_larfb(const LarfbTask<S>* __restrict__ tasks) {
+    __shared__ ApplySmem<S> asm_;
+    const LarfbTask<S> t = tasks[blockIdx.x];
+    if (t.V) {
+        for (int p0 = 0; p0 < t.k; p0 += NB) {
+            const int nbp = min(NB, t.k - p0);
+            wg_apply_lt<S, false>(t.m - p0, nbp, t.V + p0 + (size_t)p0 * t.ldv, t.ldv, t.T + (size_t)p0 * NB, NB,
+                                  nullptr, 0, t.C + p0, t.ldc, t.n, asm_);
+        }
+    }
+    if (t.R) wg_trsm_ru<S>(t.m, t.k2, t.R, t.ldr, t.C, t.ldc, asm_);
+}
+
+template <typename S>
+__global__ __launch_bounds__(WG) void k_tpqrt(const TpqrtTask<S>* __restrict__ tasks,
+                                              const TpqrtLink<S>* __restrict__ links) {
+    __shared__ ApplySmem<S> asm_;
+    __shared__ PanelSmem<S> psm;
+    const TpqrtTask<S> t = tasks[blockIdx.x];
+    for (int li = 0; li < t.nlinks; ++li) {
+        const TpqrtLink<S> L = links[t.link0 + li];
+        for (int p0 = 0; p0 < t.k; p0 += NB) {
+            const int nbp = min(NB, t.k - p0);
+            S* Rp = t.R + p0 + (size_t)p0 * t.ldr;
+            S* Bp = L.B + (size_t)p0 * L.ldb;
+            wg_panel_qr<S, true>(L.rows, nbp, Rp, t.ldr, Bp, L.ldb, L.T + (size_t)p0 * NB, NB, psm);
+            if (p0 + nbp < t.k)
+                wg_apply_lt<S, true>(L.rows, nbp, Bp, L.ldb, L.T + (size_t)p0 * NB, NB, Rp + (size_t)nbp * t.ldr,
+                                     t.ldr, Bp + (size_t)nbp * L.ldb, L.ldb, t.k - p0 - nbp, asm_);
+        }
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(WG) void k_tpmqrt(const TpmqrtTask<S>* __restrict__ tasks,
+                                               const TpmqrtLink<S>* __restrict__ links) {
+    __shared__ ApplySmem<S> asm_;
+    const TpmqrtTask<S> t = tasks[blockIdx.x];
+    for (int li = 0; li < t.nlinks; ++li) {
+        const TpmqrtLink<S> L = links[t.link0 + li];
+        for (int p0 = 0; p0 < t.k; p0 += NB) {
+            const int nbp = min(NB, t.k - p0);
+            wg_apply_lt<S, true>(L.rows, nbp, L.V + (size_t)p0 * L.ldv, L.ldv, L.T + (size_t)p0 * NB, NB,
+                                 t.Atop + p0, t.lda, L.B + (size_t)t.col0 * L.ldb, L.ldb, t.ncols, asm_);
+        }
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(WG) void k_copy(const CopyTask<S>* __restrict__ tasks) {
+    const CopyTask<S> t = tasks[blockIdx.x];
+    const int total = t.rows * t.cols;
+    for (int e = threadIdx.x; e < total; e += WG) {
+        int i = e % t.rows, j = e / t.rows;
+        S v = t.trans ? t.src[j + (size_t)i * t.lds] : t.src[i + (size_t)j * t.lds];
+        t.dst[i + (size_t)j * t.ldd] = v;
+    }
+}
+
+// Column-pivoted QR (dlaqp2 steps) of C (k x w, ld k), stopped at the first |R_jj|/|R_00| <
+// tol (choose_rank, src/util.cpp:830-851, counts exactly the leading columns that pass).
+template <typename S>
+__global__ __launch_bounds__(WG) void k_geqp3(const Geqp3Task<S>* __restrict__ tasks, double tol, double tie_rel) {
+    __shared__ PanelSmem<S> psm;
+    __shared__ S red_v[4];
+    __shared__ int red_i[4];
+    __shared__ int sh_pvt, sh_stop;
+    __shared__ S sh_tau, sh_scal, sh_r00, sh_best;
+    const Geqp3Task<S> t = tasks[blockIdx.x];
+    const int k = t.k, w = t.w, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    S* C = t.C;
+    const int kmax = min(k, w);
+    int rank;
+    if (tol >= 1.0 || kmax == 0) {
+        rank = 0;
+    } else {
+        for (int l = wv; l < w; l += 4) {
+            S s = S(0);
+            for (int r = lane; r < k; r += 64) { S v = C[r + (size_t)l * k]; s += v * v; }
+            s = wave_sum(s);
+            if (lane == 0) { S nv = sqrt(s); t.vn1[l] = nv; t.vn2[l] = nv; t.jpvt[l] = l; }
+        }
+        __syncthreads();
+        const S tol3z = sqrt(S(sizeof(S) == 8 ? 1.1102230246251565e-16 : 5.9604644775390625e-08));
+        rank = kmax;
+        for (int j = 0; j < kmax; ++j) {
+            S best = S(-1);
+            int bi = w;
+            for (int l = j + tid; l < w; l += WG) {
+                S v = t.vn1[l];
+                if (v > best) { best = v; bi = l; }
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                S ov = __shfl_xor(best, o, 64);
+                int oi = __shfl_xor(bi, o, 64);
+                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+            }
+            if (lane == 0) { red_v[wv] = best; red_i[wv] = bi; }
+            __syncthreads();
+            if (tid == 0) {
+                S b = red_v[0];
+                int i0 = red_i[0];
+                for (int q = 1; q < 4; ++q)
+                    if (red_v[q] > b || (red_v[q] == b && red_i[q] < i0)) { b = red_v[q]; i0 = red_i[q]; }
+                sh_pvt = i0;
+                sh_best = b;
+            }
+            __syncthreads();
+            if (tie_rel > 0.0) {
+                // lowest index inside the tie band below the maximum
+                const S thr = sh_best * (S)(1.0 - tie_rel);
+                int li = w;
+                for (int l = j + tid; l < w; l += WG)
+                    if (t.vn1[l] >= thr) { li = l; break; }
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) li = min(li, __shfl_xor(li, o, 64));
+                if (lane == 0) red_i[wv] = li;
+                __syncthreads();
+                if (tid == 0) sh_pvt = min(min(red_i[0], red_i[1]), min(red_i[2], red_i[3]));
+                __syncthreads();
+            }
+            const int pvt = sh_pvt;
+            if (pvt != j) {
+                for (int r = tid; r < k; r += WG) {
+                    S a = C[r + (size_t)pvt * k];
+                    C[r + (size_t)pvt * k] = C[r + (size_t)j * k];
+                    C[r + (size_t)j * k] = a;
+                }
+                if (tid == 0) {
+                    int ip = t.jpvt[pvt]; t.jpvt[pvt] = t.jpvt[j]; t.jpvt[j] = ip;
+                    t.vn1[pvt] = t.vn1[j];
+                    t.vn2[pvt] = t.vn2[j];
+                }
+            }
+            __syncthreads();
+            // Householder of C(j:k, j)
+            S ss = S(0);
+            for (int r = j + 1 + tid; r < k; r += WG) { S v = C[r + (size_t)j * k]; ss += v * v; }
+            ss = wave_sum(ss);
+            if (lane == 0) red_v[wv] = ss;
+            __syncthreads();
+            if (tid == 0) {
+                S s4 = red_v[0] + red_v[1] + red_v[2] + red_v[3];
+                S alpha = C[j + (size_t)j * k];
+                S tau, scal, beta;
+                if (s4 == S(0)) { tau = S(0); scal = S(1); beta = alpha; }
+                else {
+                    beta = -copysign(sqrt(alpha * alpha + s4), alpha);
+                    tau = (beta - alpha) / beta;
+                    scal = S(1) / (alpha - beta);
+                }
+                int stop = 0;
+                if (j == 0) sh_r00 = fabs(beta);
+                else if (tol > 0.0 && !((double)(fabs(beta) / sh_r00) >= tol)) stop = 1;
+                if (!stop) { C[j + (size_t)j * k] = beta; t.tau[j] = tau; }
+                sh_stop = stop;
+                sh_tau = tau;
+                sh_scal = scal;
+            }
+            __syncthreads();
+            if (sh_stop) { rank = j; break; }
+            const S tau = sh_tau, scal = sh_scal;
+            for (int r = j + 1 + tid; r < k; r += WG) C[r + (size_t)j * k] *= scal;
+            __syncthreads();
+            // apply H_j to the trailing columns and downdate their norms (one wave per column)
+            const S* vj = C + (size_t)j * k;
+            for (int l = j + 1 + wv; l < w; l += 4) {
+                S* cl = C + (size_t)l * k;
+                S dot = S(0);
+                for (int r = j + 1 + lane; r < k; r += 64) dot += vj[r] * cl[r];
+                dot = wave_sum(dot);
+                const S cjl = cl[j];
+                const S wl = (cjl + dot) * tau;
+                const S rjl = cjl - wl;
+                S nrm = S(0);
+                for (int r = j + 1 + lane; r < k; r += 64) {
+                    S v = cl[r] - vj[r] * wl;
+                    cl[r] = v;
+                    nrm += v * v;
+                }
+                nrm = wave_sum(nrm);
+                if (lane == 0) {
+                    cl[j] = rjl;
+                    S v1 = t.vn1[l];
+                    if (v1 != S(0)) {
+                        S temp = fabs(rjl) / v1;
+                        temp = S(1) - temp * temp;
+                        temp = temp > S(0) ? temp : S(0);
+                        S q = v1 / t.vn2[l];
+                        S temp2 = temp * q * q;
+                        if (temp2 <= tol3z) {
+                            S nv = (j + 1 < k) ? sqrt(nrm) : S(0);
+                            t.vn1[l] = nv;
+                            t.vn2[l] = nv;
+                        } else {
+                            t.vn1[l] = v1 * sqrt(temp);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) *t.rank_out = rank;
+    // V = C(:, 0:rank) (unit lower used), T blocked
+    for (int e = tid; e < k * rank; e += WG) t.Vout[e] = C[e];
+    __syncthreads();
+    if (rank > 0) wg_larft_blocked<S>(k, rank, t.Vout, k, t.tau, t.T, psm);
+}
+
+// Sparsify apply + scatter into the parent block (src/ptree.cpp:1867-1897, 289-340).
+template <typename S>
+__global__ __launch_bounds__(WG) void k_spars_apply(const SparsApplyTask<S>* __restrict__ tasks) {
+    __shared__ ApplySmem<S> asm_;
+    const SparsApplyTask<S> t = tasks[blockIdx.x];
+    if (t.ident) {
+        for (int e = threadIdx.x; e < t.rc * t.rc; e += WG) {
+            int i = e % t.rc, j = e / t.rc;
+            t.dst[i + (size_t)j * t.ldd] = (i == j) ? S(1) : S(0);
+        }
+        return;
+    }
+    if (t.Vn) {
+        for (int p0 = 0; p0 < t.rn; p0 += NB) {
+            const int nbp = min(NB, t.rn - p0);
+            wg_apply_rn<S>(t.mc, t.kn - p0, nbp, t.Vn + p0 + (size_t)p0 * t.ldvn, t.ldvn, t.Tn + (size_t)p0 * NB, NB,
+                           t.A + (size_t)p0 * t.lda, t.lda, asm_);
+        }
+    }
+    if (t.Vc) {
+        for (int p0 = 0; p0 < t.rc; p0 += NB) {
+            const int nbp = min(NB, t.rc - p0);
+            wg_apply_lt<S, false>(t.kc - p0, nbp, t.Vc + p0 + (size_t)p0 * t.ldvc, t.ldvc, t.Tc + (size_t)p0 * NB, NB,
+                                  nullptr, 0, t.A + p0, t.lda, t.rn, asm_);
+        }
+    }
+    for (int e = threadIdx.x; e < t.rc * t.rn; e += WG) {
+        int i = e % t.rc, j = e / t.rc;
+        t.dst[i + (size_t)j * t.ldd] = t.A[i + (size_t)j * t.lda];
+    }
+}
+
+template <typename S>
+__global__ void k_fill(S* p, size_t n, S v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+// =======================================================================================
+// Preconditioner apply (vectors)
+// =======================================================================================
+
+// x <- H^T x (trans) or H x, H = I - V T V^T, unit-lower V (m x kb), kb <= 32.
+// PENT: [xt (kb); x (m)] with V the lower block of [I; V] (trans only).
+template <typename S, bool PENT>
+__device__ void wg_vec_apply(int m, int kb, const S* __restrict__ V, int ldv, const S* __restrict__ T, int ldt,
+                             S* xt, S* x, bool trans, PanelSmem<S>& sm) {
+    const int tid = threadIdx.x;
+    const int l = tid >> 3, g = tid & 7;
+    S d = S(0);
+    if (l < kb) {
+        const S* vl = V + (size_t)l * ldv;
+        if (PENT) {
+            for (int r = g; r < m; r += 8) d += vl[r] * x[r];
+        } else {
+            for (int r = l + 1 + g; r < m; r += 8) d += vl[r] * x[r];
+        }
+    }
+    d += __shfl_xor(d, 1, 64);
+    d += __shfl_xor(d, 2, 64);
+    d += __shfl_xor(d, 4, 64);
+    if (g == 0 && l < kb) sm.d[l] = d + (PENT ? xt[l] : x[l]);
+    __syncthreads();
+    if (tid < kb) {
+        S s = S(0);
+        if (trans) {
+            for (int q = 0; q <= tid; ++q) s += T[q + (size_t)tid * ldt] * sm.d[q];
+        } else {
+            for (int q = tid; q < kb; ++q) s += T[tid + (size_t)q * ldt] * sm.d[q];
+        }
+        sm.t[tid] = s;
+    }
+    __syncthreads();
+    if (PENT && tid < kb) xt[tid] -= sm.t[tid];
+    for (int r = tid; r < m; r += WG) {
+        S s = S(0);
+        if (PENT) {
+            for (int i = 0; i < kb; ++i) s += V[r + (size_t)i * ldv] * sm.t[i];
+        } else {
+            int lim = min(r, kb - 1);
+            for (int i = 0; i <= lim; ++i) s += ((r == i) ? S(1) : V[r + (size_t)i * ldv]) * sm.t[i];
+        }
+        x[r] -= s;
+    }
+    __syncthreads();
+}
+
+template <typename S>
+__global__ __launch_bounds__(WG) void k_vec_q(const VecQTask<S>* __restrict__ tasks) {
+    __shared__ PanelSmem<S> sm;
+    const VecQTask<S> t = tasks[blockIdx.x];
+    if (t.trans) {
+        for (int p0 = 0; p0 < t.k; p0 += NB)
+            wg_vec_apply<S, false>(t.m - p0, min(NB, t.k - p0), t.V + p0 + (size_t)p0 * t.ldv, t.ldv,
+                                   t.T + (size_t)p0 * NB, NB, nullptr, t.x + p0, true, sm);
+    } else {
+        int last = ((t.k - 1) / NB) * NB;
+        for (int p0 = last; p0 >= 0; p0 -= NB)
+            wg_vec_apply<S, false>(t.m - p0, min(NB, t.k - p0), t.V + p0 + (size_t)p0 * t.ldv, t.ldv,
+                                   t.T + (size_t)p0 * NB, NB, nullptr, t.x + p0, false, sm);
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(WG) void k_vec_chain(const VecChainTask<S>* __restrict__ tasks,
+                                                  const VecChainLink<S>* __restrict__ links) {
+    __shared__ PanelSmem<S> sm;
+    const VecChainTask<S> t = tasks[blockIdx.x];
+    for (int li = 0; li < t.nlinks; ++li) {
+        const VecChainLink<S> L = links[t.link0 + li];
+        for (int p0 = 0; p0 < t.k; p0 += NB)
+            wg_vec_apply<S, true>(L.rows, min(NB, t.k - p0), L.V + (size_t)p0 * L.ldv, L.ldv, L.T + (size_t)p0 * NB,
+                                  NB, t.xc + p0, L.xn, true, sm);
+    }
+}
+
+// x[0:k] <- R^{-1} x[0:k], R upper (ldr); one workgroup, column-oriented back substitution.
+template <typename S>
+__device__ void wg_trsv_upper(int k, const S* __restrict__ R, int ldr, S* x) {
+    __shared__ S xj_sh;
+    const int tid = threadIdx.x;
+    for (int j = k - 1; j >= 0; --j) {
+        if (tid == 0) { S v = x[j] / R[j + (size_t)j * ldr]; x[j] = v; xj_sh = v; }
+        __syncthreads();
+        const S xj = xj_sh;
+        for (int i = tid; i < j; i += WG) x[i] -= R[i + (size_t)j * ldr] * xj;
+        __syncthreads();
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(WG) void k_vec_spars(const VecSparsTask<S>* __restrict__ tasks, int fwd) {
+    __shared__ PanelSmem<S> sm;
+    const VecSparsTask<S> t = tasks[blockIdx.x];
+    if (fwd) {
+        for (int p0 = 0; p0 < t.k; p0 += NB)
+            wg_vec_apply<S, false>(t.ldq - p0, min(NB, t.k - p0), t.Qs + p0 + (size_t)p0 * t.ldq, t.ldq,
+                                   t.Ts + (size_t)p0 * NB, NB, nullptr, t.x + p0, true, sm);
+        for (int p0 = 0; p0 < t.r; p0 += NB)
+            wg_vec_apply<S, false>(t.k - p0, min(NB, t.r - p0), t.Qsp + p0 + (size_t)p0 * t.k, t.k,
+                                   t.Tsp + (size_t)p0 * NB, NB, nullptr, t.x + p0, true, sm);
+    } else {
+        if (t.r > 0) {
+            int last = ((t.r - 1) / NB) * NB;
+            for (int p0 = last; p0 >= 0; p0 -= NB)
+                wg_vec_apply<S, false>(t.k - p0, min(NB, t.r - p0), t.Qsp + p0 + (size_t)p0 * t.k, t.k,
+                                       t.Tsp + (size_t)p0 * NB, NB, nullptr, t.x + p0, false, sm);
+        }
+        wg_trsv_upper<S>(t.k, t.Qs, t.ldq, t.x);
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(WG) void k_vec_trsv(const VecTrsvTask<S>* __restrict__ tasks,
+                                                 const VecGemvLink<S>* __restrict__ links) {
+    const VecTrsvTask<S> t = tasks[blockIdx.x];
+    for (int li = 0; li < t.nlinks; ++li) {
+        const VecGemvLink<S> L = links[t.link0 + li];
+        for (int i = threadIdx.x; i < t.k; i += WG) {
+            S s = S(0);
+            for (int j = 0; j < L.s; ++j) s += L.A[i + (size_t)j * L.lda] * L.y[j];
+            t.x[i] -= s;
+        }
+        __syncthreads();
+    }
+    wg_trsv_upper<S>(t.k, t.R, t.ldr, t.x);
+}
+
+template <typename S>
+__global__ __launch_bounds__(WG) void k_vec_copy(const VecCopyTask<S>* __restrict__ tasks) {
+    const VecCopyTask<S> t = tasks[blockIdx.x];
+    for (int i = threadIdx.x; i < t.n; i += WG) t.dst[i] = t.src[i];
+}
+
+// =======================================================================================
+// FP64 Krylov pieces (GMRES stays FP64 in every mode)
+// =======================================================================================
+
+__global__ __launch_bounds__(WG) void k_spmv_csr(int n, const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                 const double* __restrict__ val, const double* __restrict__ x,
+                                                 double* __restrict__ y) {
+    int row = blockIdx.x * WG + threadIdx.x;
+    if (row >= n) return;
+    double s = 0.0;
+    for (int k = rowptr[row]; k < rowptr[row + 1]; ++k) s += val[k] * x[col[k]];
+    y[row] = s;
+}
+
+// partial[b] = sum over this block's slice of a .* b; fixed grid => deterministic
+__global__ __launch_bounds__(WG) void k_dot_partial(int n, const double* __restrict__ a, const double* __restrict__ b,
+                                                    double* __restrict__ partial) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) s += a[i] * b[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(WG) void k_dot_final(int nb, const double* __restrict__ partial, double* out, int mode) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += WG) s += partial[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = red[0] + red[1] + red[2] + red[3];
+        *out = mode ? sqrt(v) : v;
+    }
+}
+
+// y <- y - (*h) x
+__global__ __launch_bounds__(WG) void k_axpy_dev(int n, const double* __restrict__ h, const double* __restrict__ x,
+                                                 double* __restrict__ y) {
+    const double a = *h;
+    for (int i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) y[i] -= a * x[i];
+}
+
+// y <- x / (*h)
+__global__ __launch_bounds__(WG) void k_scal_dev(int n, const double* __restrict__ h, const double* __restrict__ x,
+                                                 double* __restrict__ y) {
+    const double a = *h;
+    for (int i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) y[i] = x[i] / a;
+}
+
+// y <- a x + b y (host scalars)
+__global__ __launch_bounds__(WG) void k_axpby(int n, double a, const double* __restrict__ x, double b,
+                                              double* __restrict__ y) {
+    for (int i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) y[i] = a * x[i] + b * y[i];
+}
+
+template <typename S>
+__global__ __launch_bounds__(WG) void k_convert_d2s(int n, const double* __restrict__ x, S* __restrict__ y) {
+    for (int i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) y[i] = (S)x[i];
+}
+template <typename S>
+__global__ __launch_bounds__(WG) void k_convert_s2d(int n, const S* __restrict__ x, double* __restrict__ y) {
+    for (int i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) y[i] = (double)x[i];
+}
+
+}  // namespace spaqr
+
+// =======================================================================================
+// Host-side launchers (C++ linkage, used by the planner and the C ABI)
+// =======================================================================================
+#include "launch.hpp"
+
+namespace spaqr {
+
+template <typename S>
+void launch_geqrt(const GeqrtTask<S>* d, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_geqrt<S>, dim3(n), dim3(WG), 0, s, d);
+}
+template <typename S>
+void launch_larfb(const LarfbTask<S>* d, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_larfb<S>, dim3(n), dim3(WG), 0, s, d);
+}
+template <typename S>
+void launch_tpqrt(const TpqrtTask<S>* d, const TpqrtLink<S>* l, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_tpqrt<S>, dim3(n), dim3(WG), 0, s, d, l);
+}
+template <typename S>
+void launch_tpmqrt(const TpmqrtTask<S>* d, const TpmqrtLink<S>* l, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_tpmqrt<S>, dim3(n), dim3(WG), 0, s, d, l);
+}
+template <typename S>
+void launch_copy(const CopyTask<S>* d, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_copy<S>, dim3(n), dim3(WG), 0, s, d);
+}
+template <typename S>
+void launch_geqp3(const Geqp3Task<S>* d, int n, double tol, double tie_rel, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_geqp3<S>, dim3(n), dim3(WG), 0, s, d, tol, tie_rel);
+}
+template <typename S>
+void launch_spars_apply(const SparsApplyTask<S>* d, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_spars_apply<S>, dim3(n), dim3(WG), 0, s, d);
+}
+template <typename S>
+void launch_fill(S* p, size_t n, S v, hipStream_t s) {
+    if (n == 0) return;
+    size_t blocks = (n + WG - 1) / WG;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_fill<S>, dim3((unsigned)blocks), dim3(WG), 0, s, p, n, v);
+}
+template <typename S>
+void launch_vec_q(const VecQTask<S>* d, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_vec_q<S>, dim3(n), dim3(WG), 0, s, d);
+}
+template <typename S>
+void launch_vec_chain(const VecChainTask<S>* d, const VecChainLink<S>* l, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_vec_chain<S>, dim3(n), dim3(WG), 0, s, d, l);
+}
+template <typename S>
+void launch_vec_spars(const VecSparsTask<S>* d, int n, int fwd, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_vec_spars<S>, dim3(n), dim3(WG), 0, s, d, fwd);
+}
+template <typename S>
+void launch_vec_trsv(const VecTrsvTask<S>* d, const VecGemvLink<S>* l, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_vec_trsv<S>, dim3(n), dim3(WG), 0, s, d, l);
+}
+template <typename S>
+void launch_vec_copy(const VecCopyTask<S>* d, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_vec_copy<S>, dim3(n), dim3(WG), 0, s, d);
+}
+template <typename S>
+void launch_convert_d2s(int n, const double* x, S* y, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_convert_d2s<S>, dim3(std::min((n + WG - 1) / WG, 4096)), dim3(WG), 0, s, n, x, y);
+}
+template <typename S>
+void launch_convert_s2d(int n, const S* x, double* y, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_convert_s2d<S>, dim3(std::min((n + WG - 1) / WG, 4096)), dim3(WG), 0, s, n, x, y);
+}
+
+void launch_spmv(int n, const int* rowptr, const int* col, const double* val, const double* x, double* y,
+                 hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_spmv_csr, dim3((n + WG - 1) / WG), dim3(WG), 0, s, n, rowptr, col, val, x, y);
+}
+void launch_dot(int n, const double* a, const double* b, double* partial, int nblocks, double* out, int sqrt_mode,
+                hipStream_t s) {
+    hipLaunchKernelGGL(k_dot_partial, dim3(nblocks), dim3(WG), 0, s, n, a, b, partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(WG), 0, s, nblocks, partial, out, sqrt_mode);
+}
+void launch_axpy_dev(int n, const double* h, const double* x, double* y, hipStream_t s) {
+    hipLaunchKernelGGL(k_axpy_dev, dim3(std::min((n + WG - 1) / WG, 2048)), dim3(WG), 0, s, n, h, x, y);
+}
+void launch_scal_dev(int n, const double* h, const double* x, double* y, hipStream_t s) {
+    hipLaunchKernelGGL(k_scal_dev, dim3(std::min((n + WG - 1) / WG, 2048)), dim3(WG), 0, s, n, h, x, y);
+}
+void launch_axpby(int n, double a, const double* x, double b, double* y, hipStream_t s) {
+    hipLaunchKernelGGL(k_axpby, dim3(std::min((n + WG - 1) / WG, 2048)), dim3(WG), 0, s, n, a, x, b, y);
+}
+
+#define SPAQR_INSTANTIATE(S)                                                                          \
+    template void launch_geqrt<S>(const GeqrtTask<S>*, int, hipStream_t);                            \
+    template void launch_larfb<S>(const LarfbTask<S>*, int, hipStream_t);                            \
+    template void launch_tpqrt<S>(const TpqrtTask<S>*, const TpqrtLink<S>*, int, hipStream_t);       \
+    template void launch_tpmqrt<S>(const TpmqrtTask<S>*, const TpmqrtLink<S>*, int, hipStream_t);    \
+    template void launch_copy<S>(const CopyTask<S>*, int, hipStream_t);                              \
+    template void launch_geqp3<S>(const Geqp3Task<S>*, int, double, double, hipStream_t);                    \
+    template void launch_spars_apply<S>(const SparsApplyTask<S>*, int, hipStream_t);                 \
+    template void launch_fill<S>(S*, size_t, S, hipStream_t);                                        \
+    template void launch_vec_q<S>(const VecQTask<S>*, int, hipStream_t);                             \
+    template void launch_vec_chain<S>(const VecChainTask<S>*, const VecChainLink<S>*, int, hipStream_t); \
+    template void launch_vec_spars<S>(const VecSparsTask<S>*, int, int, hipStream_t);                \
+    template void launch_vec_trsv<S>(const VecTrsvTask<S>*, const VecGemvLink<S>*, int, hipStream_t); \
+    template void launch_vec_copy<S>(const VecCopyTask<S>*, int, hipStream_t);                       \
+    template void launch_convert_d2s<S>(int, const double*, S*, hipStream_t);                        \
+    template void launch_convert_s2d<S>(int, const S*, double*, hipStream_t);
+
+SPAQR_INSTANTIATE(double)
+SPAQR_INSTANTIATE(float)
+
+}  // namespace spaqr

@@ -1,0 +1,68 @@
+// Batched, grouped gfx950 kernels for the spaQR per-front hot path.
+//
+// Every kernel takes a device array of task descriptors and runs one 256-thread workgroup
+// (four 64-wide wavefronts) per descriptor, so all fronts / blocks of one tree level, of
+// uneven sizes, go into one launch.  Storage is column-major; every T factor is kept in
+// 32-column blocks (T is NB x k, one NB x NB upper-triangular block per panel), the layout
+// LAPACK's dgeqrt/dtpqrt use with NB = 32.
+#pragma once
+#include <cstdint>
+#include <hip/hip_runtime.h>
+
+namespace spaqr {
+
+constexpr int NB = 32;    // reflector block (panel) width
+constexpr int WG = 256;   // threads per workgroup
+
+// geqrt of A (m x n, lda), T (NB x n) out.  Also used for the interface scaling QR.
+template <typename S> struct GeqrtTask { S* A; S* T; int m, n, lda, pad; };
+
+// C (m x n, ldc) <- Q^T C, Q from a geqrt (V m x k in Vd with ldv, T NB x k).
+// Optionally followed by C <- C R^{-1} (R upper k2 x k2 in Rd, ldr) when Rd != nullptr.
+template <typename S> struct LarfbTask {
+    const S* V; const S* T; S* C; const S* R;
+    int m, n, k, ldv, ldc, k2, ldr, pad;
+};
+
+// tpqrt chain of one interior: R (k x k upper, ldr) against each B_j (rows_j x k).
+template <typename S> struct TpqrtLink { S* B; S* T; int rows, ldb; };
+template <typename S> struct TpqrtTask { S* R; int k, ldr, link0, nlinks; };
+
+// tpmqrt chain of one (interior c, column block m, column tile starting at col0):
+// [Atop (k x ncols); B_j(:, col0:col0+ncols)] <- Q_j^T [...] for each link j in order
+// (Atop already points at the tile).
+template <typename S> struct TpmqrtLink { const S* V; const S* T; S* B; int rows, ldv, ldb, pad; };
+template <typename S> struct TpmqrtTask { S* Atop; int k, lda, ncols, link0, nlinks, col0; };
+
+// geqp3 (column-pivoted QR with early stop at the sparsification tolerance) of the
+// gathered C (k x w, ld k).  Outputs rank, V (k x rank in Vout, ld k), tau, T (NB x rank).
+template <typename S> struct Geqp3Task {
+    S* C; S* Vout; S* tau; S* T; S* vn1; S* vn2; int* jpvt; int* rank_out;
+    int k, w, pad0, pad1;
+};
+
+// Gather/scatter of a sub-block: dst(i, j) = src(i, j) or src(j, i) (transpose).
+template <typename S> struct CopyTask { const S* src; S* dst; int rows, cols, lds, ldd, trans, pad; };
+
+// Sparsify apply of one interface-interface block A (mc x nc, lda):
+// A <- A Q_n (if Vn), A[0:mc, 0:rn] <- Q_c^T A[0:mc, 0:rn] (if Vc),
+// then dst(rpos + i, cpos + j) = A(i, j) for i < rc, j < rn (ldd), or the identity if ident.
+template <typename S> struct SparsApplyTask {
+    S* A; const S* Vn; const S* Tn; const S* Vc; const S* Tc; S* dst;
+    int mc, nc, lda, kn, rn, kc, rc, ldvn, ldvc, ldd, ident, pad;
+};
+
+// ---- preconditioner apply (vectors)
+// x (m) <- Q^T x (trans = 1) or Q x (trans = 0), Q from V (m x k, ldv) with blocked T.
+template <typename S> struct VecQTask { const S* V; const S* T; S* x; int m, k, ldv, trans; };
+// [xc (k); xn_j (rows_j)] <- Q_j^T [...] along the chain of one interior.
+template <typename S> struct VecChainLink { const S* V; const S* T; S* xn; int rows, ldv; };
+template <typename S> struct VecChainTask { S* xc; int k, link0, nlinks, pad; };
+// Sparsified interface forward/backward: Qs (m x k, ldq) + Ts, Qsp (k x r, ld k) + Tsp.
+template <typename S> struct VecSparsTask { const S* Qs; const S* Ts; const S* Qsp; const S* Tsp; S* x; int k, ldq, r, pad; };
+// x[0:k] -= sum_j A_j[0:k, 0:s_j] y_j, then x[0:k] <- R^{-1} x[0:k].
+template <typename S> struct VecGemvLink { const S* A; const S* y; int lda, s; };
+template <typename S> struct VecTrsvTask { const S* R; S* x; int k, ldr, link0, nlinks; };
+template <typename S> struct VecCopyTask { const S* src; S* dst; int n, pad; };
+
+}  // namespace spaqr

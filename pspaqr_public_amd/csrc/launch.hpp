@@ -1,0 +1,36 @@
+// Host-side launchers for the batched kernels (defined in kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+
+#include "kernels.hpp"
+
+namespace spaqr {
+
+template <typename S> void launch_geqrt(const GeqrtTask<S>* d, int n, hipStream_t s);
+template <typename S> void launch_larfb(const LarfbTask<S>* d, int n, hipStream_t s);
+template <typename S> void launch_tpqrt(const TpqrtTask<S>* d, const TpqrtLink<S>* l, int n, hipStream_t s);
+template <typename S> void launch_tpmqrt(const TpmqrtTask<S>* d, const TpmqrtLink<S>* l, int n, hipStream_t s);
+template <typename S> void launch_copy(const CopyTask<S>* d, int n, hipStream_t s);
+template <typename S> void launch_geqp3(const Geqp3Task<S>* d, int n, double tol, double tie_rel, hipStream_t s);
+template <typename S> void launch_spars_apply(const SparsApplyTask<S>* d, int n, hipStream_t s);
+template <typename S> void launch_fill(S* p, size_t n, S v, hipStream_t s);
+template <typename S> void launch_vec_q(const VecQTask<S>* d, int n, hipStream_t s);
+template <typename S> void launch_vec_chain(const VecChainTask<S>* d, const VecChainLink<S>* l, int n, hipStream_t s);
+template <typename S> void launch_vec_spars(const VecSparsTask<S>* d, int n, int fwd, hipStream_t s);
+template <typename S> void launch_vec_trsv(const VecTrsvTask<S>* d, const VecGemvLink<S>* l, int n, hipStream_t s);
+template <typename S> void launch_vec_copy(const VecCopyTask<S>* d, int n, hipStream_t s);
+template <typename S> void launch_convert_d2s(int n, const double* x, S* y, hipStream_t s);
+template <typename S> void launch_convert_s2d(int n, const S* x, double* y, hipStream_t s);
+
+void launch_spmv(int n, const int* rowptr, const int* col, const double* val, const double* x, double* y,
+                 hipStream_t s);
+void launch_dot(int n, const double* a, const double* b, double* partial, int nblocks, double* out, int sqrt_mode,
+                hipStream_t s);
+void launch_axpy_dev(int n, const double* h, const double* x, double* y, hipStream_t s);
+void launch_scal_dev(int n, const double* h, const double* x, double* y, hipStream_t s);
+void launch_axpby(int n, double a, const double* x, double b, double* y, hipStream_t s);
+
+}  // namespace spaqr

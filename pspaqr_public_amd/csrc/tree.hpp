@@ -1,0 +1,197 @@
+// Level planner for the MI355X spaQR path: the host side of ParTree (include/ptree.h:56-161
+// in the reference) restated for one GPU.  The host keeps the symbolic structure (ordering,
+// cluster hierarchy, fill-in, merge) in flat arrays; every numeric operation of a tree
+// level is issued as a few grouped batched launches over device descriptor tables, and the
+// factors stay resident in HBM for the preconditioner solve and GMRES.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace spaqr {
+
+struct SepID {
+    int lvl = -1, sep = 0;
+    SepID() {}
+    SepID(int l, int s) : lvl(l), sep(s) {}
+    bool operator==(const SepID& o) const { return lvl == o.lvl && sep == o.sep; }
+    bool operator!=(const SepID& o) const { return !(*this == o); }
+    bool operator<(const SepID& o) const { return lvl < o.lvl || (lvl == o.lvl && sep < o.sep); }
+};
+
+struct ClusterID {
+    SepID self, l, r;
+    int part = 2, section = 0;
+    bool operator==(const ClusterID& o) const {
+        return self == o.self && l == o.l && r == o.r && part == o.part && section == o.section;
+    }
+    bool operator!=(const ClusterID& o) const { return !(*this == o); }
+    bool operator<(const ClusterID& o) const;
+};
+
+struct HostCSC {
+    int nr = 0, nc = 0;
+    std::vector<int> colptr, rowind;
+    std::vector<double> val;
+};
+
+struct Cluster {
+    int order = -1, merge_level = 0;
+    ClusterID id;
+    int cstart = 0;
+    int rows = 0, cols = 0, rows_org = 0, cols_org = 0, rank = 0;
+    int parent = -1;
+    std::vector<int> children;
+    int rpos = -1, cpos = -1;
+    bool eliminated = false;
+    bool sparsified = false;
+    std::vector<int> out, in, sout, sin;  // edge ids
+    int self_edge = -1;
+    std::vector<int> dist2;               // sorted cluster orders
+    // device factors (type-erased; element type is the tree's precision)
+    void* T = nullptr;                    // NB x cols, diagonal geqrt (interiors)
+    void* Qs = nullptr;                   // rows x cols, scaling QR (sparsified interfaces)
+    void* Ts = nullptr;                   // NB x cols
+    void* Qsp = nullptr;                  // cols x rank, sparsification reflectors
+    void* Tsp = nullptr;                  // NB x cols
+    void* tausp = nullptr;
+    int color = 0;
+    size_t xoff = 0;                      // offset of this cluster's segment in the solve workspace
+    int lvl() const { return id.self.lvl; }
+};
+
+struct Edge {
+    int n1, n2;        // block A(n2, n1): rows(n2) x cols(n1)
+    int rows, cols;
+    void* A = nullptr; // device, column-major, ld = rows
+    void* T = nullptr; // tpqrt T (NB x cols(n1)) when n1 is an interior and n2 != n1
+    bool spars = false;
+    bool alive = true;
+};
+
+struct LevelStats {
+    int level = 0;
+    int n_interiors = 0, n_interfaces = 0, n_sparsified = 0;
+    int max_front = 0;
+    int colors = 0;
+    long long cols_before = 0, ranks_after = 0;
+    double flops = 0.0;
+};
+
+// Kernel classes timed with HIP events on the tree's stream when profiling is on.
+enum ProfCat {
+    P_GEQRT = 0, P_LARFB, P_TPQRT, P_TPMQRT, P_SGEQRT, P_SAPPLY, P_GATHER, P_GEQP3, P_SPARS, P_FILL,
+    P_SOLVE, P_SPMV, P_KRYLOV, P_NCAT
+};
+struct ProfEntry {
+    double ms = 0, flops = 0, bytes = 0;
+    long launches = 0, tasks = 0;
+};
+
+struct GmresResult {
+    int iters = 0;        // 0-based k of the last cycle plus earlier cycles' steps (as dist_gmres counts)
+    int cycles = 0;
+    double prec_res = 0;  // last preconditioned relative residual estimate
+    double true_res = 0;  // ||b - A x|| / ||b|| in FP64
+};
+
+class Tree {
+public:
+    Tree(int nlevels, int skip);
+    ~Tree();
+
+    // configuration (include/tree.h:137-143; include/ptree.h:123-125)
+    double tol = 0.1;
+    int nlevels, skip;
+    int scale = 1, square = 1, verbose = 0;
+    float order = 1.0f;
+    int prec = 64;  // 64: FP64 factors; 32: FP32 factors (GMRES stays FP64)
+    std::vector<double> X;  // coordinates, d x n
+    int xdim = 0;
+
+    int partition(const HostCSC& A);
+    int assemble();
+    int factorize();
+    int solve_dev(double* x);                       // preconditioner apply, in place, device vector
+    int spmv_dev(const double* x, double* y);       // y = A_perm x, device vectors
+    int gmres_dev(const double* b, double* x, int iters, double tol, int restart, double true_tol,
+                  GmresResult* res, double* hist, int hist_cap, int* hist_len);
+    int n_unknowns() const { return n; }
+    const std::vector<int>& cperm() const { return cperm_; }
+    int num_clusters() const { return (int)cl.size(); }
+    const Cluster& cluster(int o) const { return cl[o]; }
+    const std::vector<LevelStats>& level_stats() const { return stats; }
+    double factor_flops() const;
+    int diag_R(int order, double* out, int maxn);  // upper triangle of A(c,c) after factorize
+    hipStream_t stream() const { return stream_; }
+    bool profiling = false;
+    ProfEntry prof[P_NCAT];
+    void prof_reset();
+    int sync();  // waits for the stream and folds finished event timings into prof
+    std::string last_error;
+    double t_factor_ms = 0.0;
+
+private:
+    int n = 0;
+    HostCSC A0, App;
+    std::vector<int> cperm_;
+    std::vector<Cluster> cl0;                 // post-assembly structure, restored by factorize()
+    std::vector<Edge> ed0;
+    void* d_blocks0 = nullptr;                // pristine assembled blocks (factor precision)
+    std::vector<ClusterID> cpermed;
+    std::vector<Cluster> cl;                  // indexed by order
+    std::vector<Edge> ed;
+    std::vector<std::vector<int>> bottoms, interiors, interfaces;
+    int current_bottom = 0;
+    bool factorized = false;
+    std::vector<LevelStats> stats;
+    hipStream_t stream_ = nullptr;
+
+    // device memory
+    struct Arena {
+        std::vector<void*> chunks;
+        char* cur = nullptr;
+        size_t left = 0, total = 0;
+        void* alloc(size_t bytes);
+        void release();
+    } arena, desc_arena;
+    std::vector<double> h_blocks0;  // assembled level-0 blocks (FP64, host)
+    std::vector<size_t> h_block_off;
+    int* d_rowptr = nullptr;
+    int* d_col = nullptr;
+    double* d_val = nullptr;
+    long long d_nnz = 0;
+    double* d_scratch = nullptr;  // dot partials + scalars
+    int dot_blocks = 512;
+
+    // solve schedule
+    struct SolveSchedule;
+    SolveSchedule* sched = nullptr;
+    void* xw = nullptr;  // solve workspace (precision of the factors)
+    size_t xw_len = 0;
+
+    bool want_sparsify(const Cluster& c) const;
+    int new_edge(int c1, int c2);
+    void sort_out(Cluster& c);
+    int find_out(const Cluster& c, int n2) const;
+    void fillin();
+    template <typename S> int factorize_t();
+    template <typename S> int build_solve_schedule();
+    template <typename S> int solve_t(double* x);
+    void* upload(const void* h, size_t bytes);
+    std::vector<std::vector<char>> host_keep;  // sources of in-flight async uploads
+    void sync_stream();
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_pending;
+    template <class F> void timed(int cat, long launches, long tasks, F&& f);
+    void prof_add(int cat, double flops, double bytes) {
+        prof[cat].flops += flops;
+        prof[cat].bytes += bytes;
+    }
+    template <typename T> T* upload_vec(const std::vector<T>& v) {
+        return v.empty() ? nullptr : (T*)upload(v.data(), v.size() * sizeof(T));
+    }
+};
+
+}  // namespace spaqr

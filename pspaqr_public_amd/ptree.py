@@ -1,0 +1,166 @@
+"""Python mirror of the reference's ParTree driver API over the HIP library.
+
+Method names and argument meaning follow include/ptree.h:123-159 and include/tree.h:137-143
+(set_tol, set_skip, set_Xcoo, partition, assemble, factorize, solve, spmv) plus
+dist_gmres (src/is.cpp:447-527).  Vectors are numpy arrays in the permuted ordering the
+reference's solve/spmv/dist_gmres work in; they are copied to HBM for the call.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import DeviceBuffer, GmresResult, check, lib
+
+
+def linspace_nd(n: int, d: int) -> np.ndarray:
+    """Tensor-grid coordinates (d x n^d, column-major), id = sum_i x_i n^(d-1-i)
+    (src/util.cpp:901-930)."""
+    grids = np.meshgrid(*[np.arange(n, dtype=np.float64)] * d, indexing="ij")
+    X = np.stack([g.ravel() for g in grids], axis=0)
+    return np.asfortranarray(X)
+
+
+class ParTree:
+    def __init__(self, nlevels: int, skip: int = 0):
+        self._h = lib().spaqr_tree_create(nlevels, skip)
+        if not self._h:
+            raise RuntimeError("spaqr_tree_create failed: %s" % lib().spaqr_last_error(None).decode())
+        self.nlevels = nlevels
+        self._n = 0
+        self._keep = []
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().spaqr_tree_destroy(h)
+            self._h = None
+
+    # ---- setters (include/tree.h:137-143, include/ptree.h:123-125)
+    def set_tol(self, tol: float):
+        check(lib().spaqr_tree_set_tol(self._h, float(tol)), self._h, "set_tol")
+
+    def set_skip(self, s: int):
+        check(lib().spaqr_tree_set_skip(self._h, int(s)), self._h, "set_skip")
+
+    def set_scale(self, s: int):
+        check(lib().spaqr_tree_set_scale(self._h, int(s)), self._h, "set_scale")
+
+    def set_square(self, s: int):
+        check(lib().spaqr_tree_set_square(self._h, int(s)), self._h, "set_square")
+
+    def set_order(self, o: float):
+        check(lib().spaqr_tree_set_order(self._h, float(o)), self._h, "set_order")
+
+    def set_verbose(self, v: int):
+        check(lib().spaqr_tree_set_verbose(self._h, int(v)), self._h, "set_verbose")
+
+    def set_precision(self, bits: int):
+        check(lib().spaqr_tree_set_precision(self._h, int(bits)), self._h, "set_precision")
+
+    def set_Xcoo(self, X: np.ndarray):
+        X = np.asfortranarray(X, dtype=np.float64)
+        self._X = X
+        check(lib().spaqr_tree_set_Xcoo(self._h, X.ctypes.data, X.shape[0], X.shape[1]), self._h, "set_Xcoo")
+
+    # ---- partition / assemble / factorize
+    def partition(self, A):
+        A = A.tocsc()
+        A.sort_indices()
+        self._n = A.shape[1]
+        cp = np.ascontiguousarray(A.indptr, dtype=np.int32)
+        ri = np.ascontiguousarray(A.indices, dtype=np.int32)
+        va = np.ascontiguousarray(A.data, dtype=np.float64)
+        self._keep = [cp, ri, va]
+        check(lib().spaqr_tree_partition(self._h, A.shape[0], A.shape[1], cp.ctypes.data, ri.ctypes.data,
+                                         va.ctypes.data), self._h, "partition")
+
+    def assemble(self, A=None):
+        check(lib().spaqr_tree_assemble(self._h), self._h, "assemble")
+        self._bx = DeviceBuffer(8 * self._n)
+        self._by = DeviceBuffer(8 * self._n)
+
+    def factorize(self) -> int:
+        check(lib().spaqr_tree_factorize(self._h), self._h, "factorize")
+        return 0
+
+    # ---- solve / spmv / gmres (numpy in, numpy out)
+    def solve(self, x: np.ndarray) -> np.ndarray:
+        self._bx.upload(np.asarray(x, dtype=np.float64))
+        check(lib().spaqr_tree_solve(self._h, self._bx.ptr), self._h, "solve")
+        check(lib().spaqr_tree_sync(self._h), self._h, "sync")
+        return self._bx.download(np.float64, self._n)
+
+    def spmv(self, x: np.ndarray) -> np.ndarray:
+        self._bx.upload(np.asarray(x, dtype=np.float64))
+        check(lib().spaqr_tree_spmv(self._h, self._bx.ptr, self._by.ptr), self._h, "spmv")
+        check(lib().spaqr_tree_sync(self._h), self._h, "sync")
+        return self._by.download(np.float64, self._n)
+
+    def gmres(self, b: np.ndarray, iters: int = 300, tol: float = 1e-12, restart: int = 0, true_tol: float = 1e-12):
+        """dist_gmres(A, b, M, iters, tol, verb=true); returns (x, result, history)."""
+        self._bx.upload(np.asarray(b, dtype=np.float64))
+        res = GmresResult()
+        hist = np.zeros(iters + 16)
+        hl = C.c_int(0)
+        check(lib().spaqr_gmres(self._h, self._bx.ptr, self._by.ptr, iters, tol, restart, true_tol, C.byref(res),
+                                hist.ctypes.data, hist.size, C.byref(hl)), self._h, "gmres")
+        x = self._by.download(np.float64, self._n)
+        return x, res, hist[: hl.value]
+
+    def gmres_device(self, b_ptr, x_ptr, iters=300, tol=1e-12, restart=0, true_tol=1e-12):
+        res = GmresResult()
+        check(lib().spaqr_gmres(self._h, b_ptr, x_ptr, iters, tol, restart, true_tol, C.byref(res), None, 0, None),
+              self._h, "gmres")
+        return res
+
+    # ---- introspection
+    @property
+    def n(self) -> int:
+        return self._n
+
+    def cperm(self) -> np.ndarray:
+        p = np.zeros(self._n, dtype=np.int32)
+        check(lib().spaqr_tree_get_cperm(self._h, p.ctypes.data), self._h, "get_cperm")
+        return p
+
+    def clusters(self) -> np.ndarray:
+        nc = lib().spaqr_tree_num_clusters(self._h)
+        out = np.zeros(13 * nc, dtype=np.int32)
+        check(lib().spaqr_tree_get_clusters(self._h, out.ctypes.data), self._h, "get_clusters")
+        return out.reshape(nc, 13)
+
+    def level_stats(self) -> np.ndarray:
+        out = np.zeros(8 * self.nlevels)
+        nl = lib().spaqr_tree_level_stats(self._h, out.ctypes.data, self.nlevels)
+        return out[: 8 * nl].reshape(nl, 8)
+
+    def diag_R(self, order: int, maxn: int = 1 << 22):
+        buf = np.zeros(maxn)
+        k = lib().spaqr_tree_diag_R(self._h, order, buf.ctypes.data, maxn)
+        if k < 0:
+            return None
+        return buf[: k * k].reshape((k, k), order="F")
+
+    def factor_ms(self) -> float:
+        return lib().spaqr_tree_factor_ms(self._h)
+
+    @property
+    def handle(self):
+        return self._h
+
+
+def build_tree(problem, precision: int = 64, nlevels=None, tol=None, skip=None) -> ParTree:
+    """The reference driver's set-up sequence (pspaQR.cpp:183-198) for a Problem."""
+    nl = problem.nlevels if nlevels is None else nlevels
+    t = ParTree(nl, problem.skip if skip is None else skip)
+    t.set_scale(1)
+    t.set_tol(problem.tol if tol is None else tol)
+    t.set_order(1)
+    t.set_square(1)
+    t.set_precision(precision)
+    t.set_Xcoo(linspace_nd(problem.n_side, problem.dim))
+    t.partition(problem.A)
+    t.assemble(problem.A)
+    return t

@@ -1,0 +1,115 @@
+"""GPU parity of the whole path (partition, assemble, factorize, solve, GMRES) against the
+oracle on the same inputs.
+
+Bars (FP64): identical permutation and cluster table, identical sparsification ranks,
+diagonal R factors of every interior within 1e-12 relative after row-sign normalisation,
+preconditioner apply within 1e-10, the same GMRES iteration count and a true relative
+residual <= 1e-12.  FP32 fronts: restarted FP64 GMRES still reaches 1e-12.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleTree
+from pspaqr_public_amd.problems import make_problem
+from pspaqr_public_amd.ptree import build_tree
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(1, None), (2, 12), (3, 12), (2, 16), (3, 16), (5, 16)]
+
+
+def _pair(cfg, side, tol=None, skip=None):
+    P = make_problem(cfg, side)
+    t = tol if tol is not None else P.tol
+    s = skip if skip is not None else P.skip
+    O = OracleTree(P.A, P.n_side, P.dim, P.nlevels, t, s)
+    O.factorize()
+    T = build_tree(P, 64, tol=t, skip=s)
+    T.factorize()
+    return P, O, T
+
+
+def _sign_norm_rel(Rg, Ro):
+    d = np.sign(np.diag(Rg)) * np.sign(np.diag(Ro))
+    d[d == 0] = 1
+    return np.linalg.norm(d[:, None] * Rg - Ro) / max(np.linalg.norm(Ro), 1e-300)
+
+
+@pytest.mark.parametrize("cfg,side", CASES)
+def test_structure_and_ranks(cfg, side):
+    P, O, T = _pair(cfg, side)
+    assert np.array_equal(T.cperm(), O.cperm())
+    cg, co = T.clusters(), O.clusters()
+    assert cg.shape == co.shape
+    assert np.array_equal(cg[:, :12], co[:, :12])
+    # sparsification ranks, interface by interface
+    assert np.array_equal(cg[:, 12], co[:, 12]), np.nonzero(cg[:, 12] != co[:, 12])
+
+
+@pytest.mark.parametrize("cfg,side", CASES)
+def test_diag_R_parity(cfg, side):
+    P, O, T = _pair(cfg, side)
+    cl = O.clusters()
+    worst = 0.0
+    for order in range(cl.shape[0]):
+        if cl[order, 0] != cl[order, 1]:  # interiors only (merge level == separator level)
+            continue
+        Ro = O.diag_R(order)
+        Rg = T.diag_R(order)
+        assert Rg is not None and Ro is not None
+        worst = max(worst, _sign_norm_rel(Rg, Ro))
+    assert worst < 1e-12, worst
+
+
+@pytest.mark.parametrize("cfg,side", CASES)
+def test_solve_and_gmres_parity(cfg, side):
+    P, O, T = _pair(cfg, side)
+    cp = O.cperm()
+    b = P.b[cp]
+    xo = O.solve(b)
+    xg = T.solve(b)
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
+    assert np.linalg.norm(T.spmv(b) - O.spmv(b)) / np.linalg.norm(O.spmv(b)) < 1e-14
+    xo, ito, _ = O.gmres(b, iters=300, tol=1e-12)
+    xg, res, _ = T.gmres(b, iters=300, tol=1e-12)
+    assert res.iters == ito
+    r = b - O.spmv(xg)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-12
+    assert res.true_res <= 1e-12
+
+
+@pytest.mark.parametrize("cfg,side", [(1, 32), (2, 12), (3, 12)])
+def test_exact_factorization_tol0(cfg, side):
+    """tol = 0 (skip = nlevels - 1, pspaQR.cpp:150): no sparsification, the solve is the
+    exact inverse."""
+    P = make_problem(cfg, side)
+    T = build_tree(P, 64, tol=0.0, skip=P.nlevels - 1)
+    T.factorize()
+    cp = T.cperm()
+    b = P.b[cp]
+    x = T.solve(b)
+    r = b - T.spmv(x)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-13
+
+
+@pytest.mark.parametrize("cfg,side", [(1, None), (3, 16)])
+def test_fp32_fronts_reach_fp64_residual(cfg, side):
+    P = make_problem(cfg, side)
+    T = build_tree(P, 32)
+    T.factorize()
+    b = P.b[T.cperm()]
+    x, res, _ = T.gmres(b, iters=300, tol=1e-12, restart=50, true_tol=1e-12)
+    r = b - T.spmv(x)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-12
+    assert res.true_res <= 1e-12
+
+
+def test_refactorize_is_repeatable():
+    P = make_problem(2, 12)
+    T = build_tree(P, 64)
+    T.factorize()
+    b = P.b[T.cperm()]
+    x1 = T.solve(b)
+    T.factorize()
+    x2 = T.solve(b)
+    assert np.array_equal(x1, x2)
