@@ -51,13 +51,24 @@ struct Mat {
 // alpha is overwritten by beta, x by the reflector tail v(2:n); returns tau.
 // (LAPACK dlarfg; the safe-minimum rescaling loop is not needed for the matrices on this
 // path and is omitted, both here and in the HIP kernels.)
+constexpr double kAlphaSignBand = 1e-10;
+
+inline long& larfg_tiny_alpha() {
+    static long count = 0;
+    return count;
+}
+
 inline double larfg(int n, double& alpha, double* x, int incx) {
     if (n <= 1) return 0.0;
     double ss = 0.0;
     for (int i = 0; i < n - 1; ++i) ss += x[(size_t)i * incx] * x[(size_t)i * incx];
     if (ss == 0.0) return 0.0;
     double xnorm = std::sqrt(ss);
-    double beta = -std::copysign(std::hypot(alpha, xnorm), alpha);
+    // An alpha at rounding-noise level carries no sign information: treat it as >= 0 so the
+    // reflector does not flip with the summation order (the HIP kernels use the same rule).
+    const bool tiny = std::fabs(alpha) <= kAlphaSignBand * xnorm;
+    if (tiny) larfg_tiny_alpha()++;
+    double beta = tiny ? -std::hypot(alpha, xnorm) : -std::copysign(std::hypot(alpha, xnorm), alpha);
     double tau = (beta - alpha) / beta;
     double scal = 1.0 / (alpha - beta);
     for (int i = 0; i < n - 1; ++i) x[(size_t)i * incx] *= scal;
@@ -333,6 +344,10 @@ inline long& geqp3_near_ties() {
     static long count = 0;
     return count;
 }
+inline double& geqp3_near_tie_band() {
+    static double band = 1e-12;
+    return band;
+}
 
 // dgeqp3 (column-pivoted QR; the dlaqp2 step applied to every column).  jpvt is 0-based
 // on return (the reference decrements LAPACK's 1-based pivots, src/util.cpp:510-511).
@@ -365,7 +380,7 @@ inline void geqp3(int m, int n, double* A, int lda, int* jpvt, double* tau, doub
             // diagnostic: count pivot steps whose winner is within 1e-12 of a runner-up
             double best = std::fabs(vn1[pvt]);
             for (int j = i; j < n; ++j)
-                if (j != pvt && best > 0 && std::fabs(vn1[j]) >= best * (1.0 - 1e-12)) { geqp3_near_ties()++; break; }
+                if (j != pvt && best > 0 && std::fabs(vn1[j]) >= best * (1.0 - geqp3_near_tie_band())) { geqp3_near_ties()++; break; }
         }
         if (pvt != i) {
             for (int r = 0; r < m; ++r) std::swap(A[r + (size_t)pvt * lda], A[r + (size_t)i * lda]);

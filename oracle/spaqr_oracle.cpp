@@ -1035,6 +1035,12 @@ void orc_gemqrt_lt(int m, int n, int k, int nb, const double* V, const double* T
     gemqrt_lt(m, n, k, nb, V, m, T, nb, C, m);
 }
 int orc_choose_rank(const double* s, int n, double tol) { return choose_rank(s, n, tol); }
+void orc_set_near_tie_band(double b) { geqp3_near_tie_band() = b; }
+long orc_larfg_tiny_alpha(int reset) {
+    long v = larfg_tiny_alpha();
+    if (reset) larfg_tiny_alpha() = 0;
+    return v;
+}
 long orc_geqp3_near_ties(int reset) {
     long v = geqp3_near_ties();
     if (reset) geqp3_near_ties() = 0;

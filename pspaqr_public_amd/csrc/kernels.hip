@@ -35,6 +35,16 @@ struct PanelSmem {
     S tau, scal;
 };
 
+// sign(alpha) * ||(alpha, x)|| for a Householder reflector, with an alpha at rounding-noise
+// level (|alpha| <= band * ||x||) treated as non-negative: its sign carries no information and
+// would otherwise make the reflector flip with the summation order.
+template <typename S>
+__device__ __forceinline__ S householder_norm_signed(S alpha, S ss) {
+    const S nrm = sqrt(alpha * alpha + ss);
+    const S band = sizeof(S) == 8 ? S(1e-10) : S(1e-5);
+    return (fabs(alpha) <= band * sqrt(ss)) ? nrm : copysign(nrm, alpha);
+}
+
 template <typename S>
 __device__ __forceinline__ S wave_sum(S v) {
 #pragma unroll
@@ -73,7 +83,7 @@ __device__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* To
             if (ss == S(0)) {
                 tau = S(0); scal = S(1); beta = alpha;
             } else {
-                beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+                beta = -householder_norm_signed(alpha, ss);
                 tau = (beta - alpha) / beta;
                 scal = S(1) / (alpha - beta);
             }
@@ -565,7 +575,7 @@ __global__ __launch_bounds__(WG) void k_geqp3(const Geqp3Task<S>* __restrict__ t
                 S tau, scal, beta;
                 if (s4 == S(0)) { tau = S(0); scal = S(1); beta = alpha; }
                 else {
-                    beta = -copysign(sqrt(alpha * alpha + s4), alpha);
+                    beta = -householder_norm_signed(alpha, s4);
                     tau = (beta - alpha) / beta;
                     scal = S(1) / (alpha - beta);
                 }
