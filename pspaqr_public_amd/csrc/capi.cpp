@@ -8,6 +8,7 @@
 #include "../../include/spaqr_gpu.h"
 #include "kernels.hpp"
 #include "launch.hpp"
+#include "rrqr.hpp"
 #include "tree.hpp"
 
 using namespace spaqr;
@@ -282,10 +283,46 @@ int spaqr_geqp3_batched(int prec, int count, const spaqr_geqp3_desc* d, double t
     if (!(tol >= 0)) return -4;
     if (!(tie_rel >= 0 && tie_rel < 1)) return -5;
     if (count == 0) return 0;
-    return run_batched(d, sizeof(*d) * count, nullptr, 0, stream, [&](void* dd, void*, hipStream_t s) {
-        if (prec == 64) launch_geqp3<double>((const Geqp3Task<double>*)dd, count, tol, tie_rel, s);
-        else launch_geqp3<float>((const Geqp3Task<float>*)dd, count, tol, tie_rel, s);
-    });
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<void*> owned;
+    std::vector<std::vector<char>> keep;
+    RrqrAlloc mem{[&](size_t b) {
+                      void* p = nullptr;
+                      if (hipMalloc(&p, b ? b : 8) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+                      owned.push_back(p);
+                      return p;
+                  },
+                  [&](const void* h, size_t b) {
+                      void* p = nullptr;
+                      if (hipMalloc(&p, b ? b : 8) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+                      owned.push_back(p);
+                      keep.emplace_back((const char*)h, (const char*)h + b);
+                      (void)hipMemcpyAsync(p, keep.back().data(), b, hipMemcpyHostToDevice, s);
+                      return p;
+                  }};
+    int rc = 0;
+    try {
+        const int zero = 0;
+        int* d_err = (int*)mem.upload(&zero, sizeof(int));
+        if (prec == 64) {
+            std::vector<Geqp3Task<double>> v((const Geqp3Task<double>*)d, (const Geqp3Task<double>*)d + count);
+            geqp3_grouped<double>(v, tol, tie_rel, s, mem, d_err);
+        } else {
+            std::vector<Geqp3Task<float>> v((const Geqp3Task<float>*)d, (const Geqp3Task<float>*)d + count);
+            geqp3_grouped<float>(v, tol, tie_rel, s, mem, d_err);
+        }
+        int herr = 0;
+        hipError_t e = hipMemcpyAsync(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); rc = -100; }
+        else if (herr) { g_err = "cooperative geqp3: barrier wait exceeded its spin limit"; rc = -100; }
+    } catch (std::exception& ex) {
+        g_err = ex.what();
+        (void)hipStreamSynchronize(s);
+        rc = -100;
+    }
+    for (void* p : owned) (void)hipFree(p);
+    return rc;
 }
 
 int spaqr_spars_apply_batched(int prec, int count, const spaqr_spars_desc* d, void* stream) {

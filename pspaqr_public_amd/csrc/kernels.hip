@@ -13,6 +13,7 @@
 // one 256-thread workgroup, so every front of a level is factored in one grouped launch.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdint>
 
 #include "kernels.hpp"
@@ -992,5 +993,341 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
 
 SPAQR_INSTANTIATE(double)
 SPAQR_INSTANTIATE(float)
+
+}  // namespace spaqr
+
+// =======================================================================================
+// Cooperative column-pivoted QR.  One interface's C (k x w) is split into column slices held
+// in the LDS of P workgroups (co-resident: cooperative launch).  Per pivot step every block
+// publishes its best candidate column (local maximum norm, lowest LAPACK position inside
+// the tie band) to HBM, the P blocks of the interface meet at one barrier, and each block
+// then picks the global pivot and forms the Householder reflector itself from the published
+// column (identical arithmetic in every block), so only one barrier per step is needed and
+// the trailing update never leaves LDS.  Column swaps are bookkeeping on LAPACK positions.
+// Barrier waits are bounded: a wait that exceeds the spin limit raises *err and returns.
+// =======================================================================================
+namespace spaqr {
+
+constexpr long kBarrierSpinLimit = 1L << 24;
+
+__device__ __forceinline__ void task_barrier(int* bar, int P, int& passed, int* err) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(bar, 1);
+        passed++;
+        const int target = P * passed;
+        long spins = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++spins > kBarrierSpinLimit) { atomicOr(err, 1); break; }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __threadfence();
+    } else {
+        passed++;
+    }
+    __syncthreads();
+}
+
+template <typename S>
+__device__ __forceinline__ S ld_coherent(const S* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename S>
+__device__ __forceinline__ void st_coherent(S* p, S v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// record layout inside a slot (k + 4 values): M_b, v_cand, pos_cand, unused, column[k]
+template <typename S>
+__global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __restrict__ tasks,
+                                                   const int* __restrict__ blk2task, int* err, double tol,
+                                                   double tie_rel) {
+    extern __shared__ __align__(16) unsigned char dyn_smem[];
+    __shared__ S red_v[4];
+    __shared__ int red_i[4];
+    __shared__ S sh_s[4];
+    __shared__ int sh_i[4];
+    __shared__ PanelSmem<S> psm;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const Geqp3CoopTask<S> t = tasks[blk2task[blockIdx.x]];
+    const int k = t.k, w = t.w, P = t.P, cpb = t.cpb;
+    const int p = blockIdx.x - t.b0;
+    const int q0 = p * cpb;
+    const int ncol = max(0, min(w, q0 + cpb) - q0);
+    const int rec = k + 4;
+    S* slice = reinterpret_cast<S*>(dyn_smem);    // k x cpb, ld k
+    S* vn1 = slice + (size_t)k * cpb;
+    S* vn2 = vn1 + cpb;
+    S* vbuf = vn2 + cpb;                          // k: the pivot column, then the reflector
+    int* pos = reinterpret_cast<int*>(vbuf + k);  // LAPACK position of each owned column; -1 once pivoted
+    const S tol3z = sqrt(S(sizeof(S) == 8 ? 1.1102230246251565e-16 : 5.9604644775390625e-08));
+    int passed = 0;
+
+    for (int e = tid; e < k * ncol; e += WG) slice[e] = t.C[(size_t)q0 * k + e];
+    __syncthreads();
+    for (int c = wv; c < ncol; c += 4) {
+        S s = S(0);
+        for (int r = lane; r < k; r += 64) { S v = slice[r + (size_t)c * k]; s += v * v; }
+        s = wave_sum(s);
+        if (lane == 0) { vn1[c] = sqrt(s); vn2[c] = vn1[c]; pos[c] = q0 + c; }
+    }
+    __syncthreads();
+
+    int rank = t.kmax;
+    S r00 = S(0);
+    for (int j = 0; j < t.kmax; ++j) {
+        // ---- local maximum and the lowest-position candidate inside the local band
+        S mb = S(-1);
+        for (int c = tid; c < ncol; c += WG)
+            if (pos[c] >= 0 && vn1[c] > mb) mb = vn1[c];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) mb = fmax(mb, __shfl_xor(mb, o, 64));
+        if (lane == 0) red_v[wv] = mb;
+        __syncthreads();
+        mb = fmax(fmax(red_v[0], red_v[1]), fmax(red_v[2], red_v[3]));
+        __syncthreads();
+        S thr = (tie_rel > 0.0) ? mb * (S)(1.0 - tie_rel) : mb;
+        int cand = INT_MAX, cc = -1;
+        for (int c = tid; c < ncol; c += WG)
+            if (pos[c] >= 0 && vn1[c] >= thr && pos[c] < cand) { cand = pos[c]; cc = c; }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            int oc = __shfl_xor(cand, o, 64), occ = __shfl_xor(cc, o, 64);
+            if (oc < cand) { cand = oc; cc = occ; }
+        }
+        if (lane == 0) { red_i[wv] = cand; sh_i[wv] = cc; }
+        __syncthreads();
+        if (tid == 0) {
+            int b = 0;
+            for (int q = 1; q < 4; ++q)
+                if (red_i[q] < red_i[b]) b = q;
+            sh_i[3] = sh_i[b];  // candidate column (local index) or -1
+        }
+        __syncthreads();
+        int ccol = sh_i[3];
+        int pv_col = -1;  // local column of the pivot when this block owns it
+        if (P == 1) {
+            if (ccol < 0) { rank = j; break; }  // no finite candidate left (NaN input)
+            pv_col = ccol;
+            for (int r = tid; r < k; r += WG) vbuf[r] = slice[r + (size_t)ccol * k];
+            if (tid == 0) sh_i[0] = pos[ccol];
+            __syncthreads();
+        } else {
+            // publish: slot = parity of the barriers passed so far
+            S* slot = t.xch + (size_t)(passed & 1) * P * rec;
+            S* my = slot + (size_t)p * rec;
+            if (tid == 0) {
+                st_coherent(my + 0, mb);
+                st_coherent(my + 1, ccol >= 0 ? vn1[ccol] : S(-1));
+                st_coherent(my + 2, ccol >= 0 ? (S)pos[ccol] : S(-1));
+            }
+            if (ccol >= 0)
+                for (int r = tid; r < k; r += WG) st_coherent(my + 4 + r, slice[r + (size_t)ccol * k]);
+            task_barrier(t.bar, P, passed, err);
+            // global maximum, band, ambiguity
+            S M = S(-1);
+            for (int q = tid; q < P; q += WG) M = fmax(M, ld_coherent(slot + (size_t)q * rec));
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) M = fmax(M, __shfl_xor(M, o, 64));
+            if (lane == 0) red_v[wv] = M;
+            __syncthreads();
+            M = fmax(fmax(red_v[0], red_v[1]), fmax(red_v[2], red_v[3]));
+            __syncthreads();
+            const S gthr = (tie_rel > 0.0) ? M * (S)(1.0 - tie_rel) : M;
+            int best = INT_MAX, bq = -1, amb = 0;
+            for (int q = tid; q < P; q += WG) {
+                const S* rq = slot + (size_t)q * rec;
+                S mq = ld_coherent(rq), vq = ld_coherent(rq + 1);
+                int pq = (int)ld_coherent(rq + 2);
+                if (vq >= gthr && pq >= 0 && pq < best) { best = pq; bq = q; }
+                if (mq >= gthr && !(vq >= gthr)) amb = 1;
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                int ob = __shfl_xor(best, o, 64), oq = __shfl_xor(bq, o, 64);
+                if (ob < best) { best = ob; bq = oq; }
+                amb |= __shfl_xor(amb, o, 64);
+            }
+            if (lane == 0) { red_i[wv] = best; sh_i[wv] = bq; red_v[wv] = (S)amb; }
+            __syncthreads();
+            if (tid == 0) {
+                int b = 0;
+                for (int q = 1; q < 4; ++q)
+                    if (red_i[q] < red_i[b]) b = q;
+                sh_i[2] = sh_i[b];
+                sh_i[3] = (red_v[0] + red_v[1] + red_v[2] + red_v[3]) > S(0) ? 1 : 0;
+            }
+            __syncthreads();
+            int winq = sh_i[2];
+            const S* wslot = slot;
+            if (sh_i[3]) {
+                // a block's band candidate fell outside the global band: second round with gthr
+                __syncthreads();
+                int c2 = INT_MAX, cc2 = -1;
+                for (int c = tid; c < ncol; c += WG)
+                    if (pos[c] >= 0 && vn1[c] >= gthr && pos[c] < c2) { c2 = pos[c]; cc2 = c; }
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    int oc = __shfl_xor(c2, o, 64), occ = __shfl_xor(cc2, o, 64);
+                    if (oc < c2) { c2 = oc; cc2 = occ; }
+                }
+                if (lane == 0) { red_i[wv] = c2; sh_i[wv] = cc2; }
+                __syncthreads();
+                if (tid == 0) {
+                    int b = 0;
+                    for (int q = 1; q < 4; ++q)
+                        if (red_i[q] < red_i[b]) b = q;
+                    sh_i[3] = sh_i[b];
+                }
+                __syncthreads();
+                ccol = sh_i[3];
+                S* slot2 = t.xch + (size_t)(passed & 1) * P * rec;
+                S* my2 = slot2 + (size_t)p * rec;
+                if (tid == 0) {
+                    st_coherent(my2 + 1, ccol >= 0 ? vn1[ccol] : S(-1));
+                    st_coherent(my2 + 2, ccol >= 0 ? (S)pos[ccol] : S(-1));
+                }
+                if (ccol >= 0)
+                    for (int r = tid; r < k; r += WG) st_coherent(my2 + 4 + r, slice[r + (size_t)ccol * k]);
+                task_barrier(t.bar, P, passed, err);
+                int b2 = INT_MAX, q2 = -1;
+                for (int q = tid; q < P; q += WG) {
+                    int pq = (int)ld_coherent(slot2 + (size_t)q * rec + 2);
+                    if (pq >= 0 && pq < b2) { b2 = pq; q2 = q; }
+                }
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    int ob = __shfl_xor(b2, o, 64), oq = __shfl_xor(q2, o, 64);
+                    if (ob < b2) { b2 = ob; q2 = oq; }
+                }
+                if (lane == 0) { red_i[wv] = b2; sh_i[wv] = q2; }
+                __syncthreads();
+                if (tid == 0) {
+                    int b = 0;
+                    for (int q = 1; q < 4; ++q)
+                        if (red_i[q] < red_i[b]) b = q;
+                    sh_i[2] = sh_i[b];
+                }
+                __syncthreads();
+                winq = sh_i[2];
+                wslot = slot2;
+            }
+            if (winq < 0) { rank = j; break; }  // no finite candidate in any block (NaN input)
+            const S* wr = wslot + (size_t)winq * rec;
+            for (int r = tid; r < k; r += WG) vbuf[r] = ld_coherent(wr + 4 + r);
+            if (tid == 0) sh_i[0] = (int)ld_coherent(wr + 2);
+            if (winq == p) pv_col = ccol;
+            __syncthreads();
+        }
+        const int pvpos = sh_i[0];
+        // ---- reflector from the pivot column (every block, identical arithmetic)
+        S ss = S(0);
+        for (int r = j + 1 + tid; r < k; r += WG) ss += vbuf[r] * vbuf[r];
+        ss = wave_sum(ss);
+        if (lane == 0) red_v[wv] = ss;
+        __syncthreads();
+        if (tid == 0) {
+            const S s4 = red_v[0] + red_v[1] + red_v[2] + red_v[3];
+            const S alpha = vbuf[j];
+            S tau, scal, beta;
+            if (s4 == S(0)) { tau = S(0); scal = S(1); beta = alpha; }
+            else {
+                beta = -householder_norm_signed(alpha, s4);
+                tau = (beta - alpha) / beta;
+                scal = S(1) / (alpha - beta);
+            }
+            sh_s[0] = tau;
+            sh_s[1] = scal;
+            sh_s[2] = beta;
+        }
+        __syncthreads();
+        const S tau = sh_s[0], scal = sh_s[1], beta = sh_s[2];
+        if (j == 0) r00 = fabs(beta);
+        else if (tol > 0.0 && !((double)(fabs(beta) / r00) >= tol)) { rank = j; break; }
+        for (int r = j + 1 + tid; r < k; r += WG) vbuf[r] *= scal;
+        __syncthreads();
+        // ---- the owner stores the finished column; positions are swapped by bookkeeping
+        if (pv_col >= 0) {
+            for (int r = tid; r < k; r += WG) t.Vout[r + (size_t)j * k] = (r == j) ? beta : vbuf[r];
+            if (tid == 0) {
+                t.tau[j] = tau;
+                if (t.jpvt_out) t.jpvt_out[j] = q0 + pv_col;
+                pos[pv_col] = -1;
+            }
+        }
+        for (int c = tid; c < ncol; c += WG)
+            if (pos[c] == j && j != pvpos) pos[c] = pvpos;
+        __syncthreads();
+        // ---- apply H_j to the owned unpivoted columns, downdate their norms
+        for (int c = wv; c < ncol; c += 4) {
+            if (pos[c] < 0) continue;
+            S* cl = slice + (size_t)c * k;
+            S dot = S(0);
+            for (int r = j + 1 + lane; r < k; r += 64) dot += vbuf[r] * cl[r];
+            dot = wave_sum(dot);
+            const S cjl = cl[j];
+            const S wl = (cjl + dot) * tau;
+            const S rjl = cjl - wl;
+            S nrm = S(0);
+            for (int r = j + 1 + lane; r < k; r += 64) {
+                S v = cl[r] - vbuf[r] * wl;
+                cl[r] = v;
+                nrm += v * v;
+            }
+            nrm = wave_sum(nrm);
+            if (lane == 0) {
+                cl[j] = rjl;
+                S v1 = vn1[c];
+                if (v1 != S(0)) {
+                    S temp = fabs(rjl) / v1;
+                    temp = S(1) - temp * temp;
+                    temp = temp > S(0) ? temp : S(0);
+                    S q = v1 / vn2[c];
+                    S temp2 = temp * q * q;
+                    if (temp2 <= tol3z) {
+                        S nv = (j + 1 < k) ? sqrt(nrm) : S(0);
+                        vn1[c] = nv;
+                        vn2[c] = nv;
+                    } else {
+                        vn1[c] = v1 * sqrt(temp);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // ---- T of the kept reflectors, by the first block once every owner has stored its columns
+    if (P > 1) task_barrier(t.bar, P, passed, err);
+    if (p == 0) {
+        if (tid == 0) *t.rank_out = rank;
+        if (rank > 0) wg_larft_blocked<S>(k, rank, t.Vout, k, t.tau, t.T, psm);
+    }
+}
+
+template <typename S>
+int geqp3_coop_max_blocks(size_t lds_bytes) {
+    int per_cu = 0, dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_geqp3_coop<S>, WG, lds_bytes) != hipSuccess) return 0;
+    return per_cu * ncu;
+}
+
+template <typename S>
+hipError_t launch_geqp3_coop(const Geqp3CoopTask<S>* d_tasks, const int* d_blk2task, int* d_err, int nblocks,
+                             double tol, double tie_rel, size_t lds_bytes, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    void* args[] = {(void*)&d_tasks, (void*)&d_blk2task, (void*)&d_err, (void*)&tol, (void*)&tie_rel};
+    return hipLaunchCooperativeKernel((const void*)k_geqp3_coop<S>, dim3(nblocks), dim3(WG), args,
+                                      (unsigned)lds_bytes, s);
+}
+
+template int geqp3_coop_max_blocks<double>(size_t);
+template int geqp3_coop_max_blocks<float>(size_t);
+template hipError_t launch_geqp3_coop<double>(const Geqp3CoopTask<double>*, const int*, int*, int, double, double,
+                                              size_t, hipStream_t);
+template hipError_t launch_geqp3_coop<float>(const Geqp3CoopTask<float>*, const int*, int*, int, double, double,
+                                             size_t, hipStream_t);
 
 }  // namespace spaqr

@@ -41,6 +41,16 @@ template <typename S> struct Geqp3Task {
     int k, w, pad0, pad1;
 };
 
+// Cooperative geqp3: one interface's C (k x w, ld k, in HBM) is split into column slices
+// of `cpb` columns held in the LDS of `P` workgroups (blocks b0 .. b0+P-1 of a cooperative
+// launch).  xch is per-task exchange scratch in HBM (geqp3_coop_xch_elems values), bar a
+// zero-initialised barrier counter.  Outputs as Geqp3Task (jpvt_out: first `rank` pivots).
+template <typename S> struct Geqp3CoopTask {
+    const S* C; S* Vout; S* tau; S* T; int* rank_out; int* jpvt_out; S* xch; int* bar;
+    int k, w, b0, P, cpb, kmax;
+};
+inline size_t geqp3_coop_xch_elems(int k, int P) { return 2 * (size_t)P * (k + 4); }
+
 // Gather/scatter of a sub-block: dst(i, j) = src(i, j) or src(j, i) (transpose).
 template <typename S> struct CopyTask { const S* src; S* dst; int rows, cols, lds, ldd, trans, pad; };
 

@@ -25,6 +25,11 @@ template <typename S> void launch_vec_copy(const VecCopyTask<S>* d, int n, hipSt
 template <typename S> void launch_convert_d2s(int n, const double* x, S* y, hipStream_t s);
 template <typename S> void launch_convert_s2d(int n, const S* x, double* y, hipStream_t s);
 
+template <typename S> int geqp3_coop_max_blocks(size_t lds_bytes);
+template <typename S>
+hipError_t launch_geqp3_coop(const Geqp3CoopTask<S>* d_tasks, const int* d_blk2task, int* d_err, int nblocks,
+                             double tol, double tie_rel, size_t lds_bytes, hipStream_t s);
+
 void launch_spmv(int n, const int* rowptr, const int* col, const double* val, const double* x, double* y,
                  hipStream_t s);
 void launch_dot(int n, const double* a, const double* b, double* partial, int nblocks, double* out, int sqrt_mode,

@@ -1,0 +1,89 @@
+// Grouped geqp3 driver (see rrqr.hpp).
+#include "rrqr.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+
+#include "launch.hpp"
+
+namespace spaqr {
+
+template <typename S>
+int geqp3_grouped(const std::vector<Geqp3Task<S>>& tasks, double tol, double tie_rel, hipStream_t stream,
+                  const RrqrAlloc& mem, int* d_err) {
+    static int max_blocks_cache[2] = {-1, -1};
+    int& maxb = max_blocks_cache[sizeof(S) == 8 ? 0 : 1];
+    if (maxb < 0) maxb = geqp3_coop_max_blocks<S>(kGeqp3CoopLds);
+    std::vector<Geqp3Task<S>> fallback;
+    struct Pending { std::vector<Geqp3CoopTask<S>> t; std::vector<int> b2t; int blocks = 0, jmax = 0, active = 0; };
+    std::vector<int> zeros;
+    std::vector<Pending> launches(1);
+    for (const auto& g : tasks) {
+        const int k = g.k, w = g.w;
+        const int kmax = (tol >= 1.0) ? 0 : std::min(k, w);
+        const size_t per_col = (size_t)k * sizeof(S) + 2 * sizeof(S) + sizeof(int);  // slice, vn1, vn2, pos
+        const size_t fixed = (size_t)k * sizeof(S) + 64;
+        int cpb = (kGeqp3CoopLds > fixed) ? (int)((kGeqp3CoopLds - fixed) / per_col) : 0;
+        if (w > 0) cpb = std::min(cpb, w);
+        const int P = (w > 0 && cpb > 0) ? (w + cpb - 1) / cpb : 1;
+        if (maxb <= 0 || cpb <= 0 || P > maxb) {
+            fallback.push_back(g);
+            continue;
+        }
+        if (launches.back().blocks + P > maxb) launches.emplace_back();
+        Pending& L = launches.back();
+        Geqp3CoopTask<S> c{};
+        c.C = g.C;
+        c.Vout = g.Vout;
+        c.tau = g.tau;
+        c.T = g.T;
+        c.rank_out = g.rank_out;
+        c.jpvt_out = g.jpvt;
+        c.xch = P > 1 ? (S*)mem.alloc(sizeof(S) * geqp3_coop_xch_elems(k, P)) : nullptr;
+        c.bar = nullptr;
+        c.k = k;
+        c.w = std::max(w, 1);
+        c.b0 = L.blocks;
+        c.P = P;
+        c.cpb = std::max(cpb, 1);
+        c.kmax = kmax;
+        if (w == 0) { c.w = 0; }
+        for (int b = 0; b < P; ++b) L.b2t.push_back((int)L.t.size());
+        L.t.push_back(c);
+        L.blocks += P;
+        L.jmax = std::max(L.jmax, kmax);
+        if (kmax > 0) L.active++;
+    }
+    static const bool dbg = getenv("SPAQR_DEBUG_RRQR") != nullptr;
+    if (dbg) {
+        int kmx = 0, wmx = 0, pmx = 0;
+        for (auto& L : launches)
+            for (auto& c : L.t) { kmx = std::max(kmx, c.k); wmx = std::max(wmx, c.w); pmx = std::max(pmx, c.P); }
+        fprintf(stderr, "[geqp3] tasks %zu maxb %d launches %zu fallback %zu  max k %d w %d P %d\n", tasks.size(), maxb,
+                launches.size(), fallback.size(), kmx, wmx, pmx);
+        for (auto& f : fallback) fprintf(stderr, "   fallback k %d w %d\n", f.k, f.w);
+    }
+    for (auto& L : launches) {
+        if (L.blocks == 0) continue;
+        zeros.assign(L.t.size(), 0);
+        int* d_bars = (int*)mem.upload(zeros.data(), zeros.size() * sizeof(int));
+        for (size_t i = 0; i < L.t.size(); ++i) L.t[i].bar = d_bars + i;
+        auto* d_t = (Geqp3CoopTask<S>*)mem.upload(L.t.data(), L.t.size() * sizeof(Geqp3CoopTask<S>));
+        auto* d_b = (int*)mem.upload(L.b2t.data(), L.b2t.size() * sizeof(int));
+        hipError_t e = launch_geqp3_coop<S>(d_t, d_b, d_err, L.blocks, tol, tie_rel, kGeqp3CoopLds, stream);
+        if (e != hipSuccess) throw std::runtime_error(std::string("cooperative geqp3 launch: ") + hipGetErrorString(e));
+    }
+    if (!fallback.empty()) {
+        auto* d = (Geqp3Task<S>*)mem.upload(fallback.data(), fallback.size() * sizeof(Geqp3Task<S>));
+        launch_geqp3<S>(d, (int)fallback.size(), tol, tie_rel, stream);
+    }
+    return (int)fallback.size();
+}
+
+template int geqp3_grouped<double>(const std::vector<Geqp3Task<double>>&, double, double, hipStream_t, const RrqrAlloc&, int*);
+template int geqp3_grouped<float>(const std::vector<Geqp3Task<float>>&, double, double, hipStream_t, const RrqrAlloc&, int*);
+
+}  // namespace spaqr

@@ -1,0 +1,28 @@
+// Grouped geqp3 driver: packs the interfaces of one level into cooperative launches whose
+// workgroups hold LDS-resident column slices (k_geqp3_coop); an interface too large for one
+// cooperative launch runs on the single-workgroup kernel (k_geqp3) instead.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <functional>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace spaqr {
+
+struct RrqrAlloc {
+    std::function<void*(size_t)> alloc;                  // device scratch, alive until the stream syncs
+    std::function<void*(const void*, size_t)> upload;    // async H2D copy of a host buffer, returns device ptr
+};
+
+// LDS budget of one cooperative geqp3 workgroup (column slice + norms + reflector)
+constexpr size_t kGeqp3CoopLds = 64 * 1024;
+
+// Returns the number of tasks that ran on the single-workgroup fallback.  *d_err (device int,
+// zero on entry) becomes non-zero if a cooperative barrier wait hit its spin limit.
+template <typename S>
+int geqp3_grouped(const std::vector<Geqp3Task<S>>& tasks, double tol, double tie_rel, hipStream_t stream,
+                  const RrqrAlloc& mem, int* d_err);
+
+}  // namespace spaqr

@@ -30,6 +30,7 @@
 
 #include "kernels.hpp"
 #include "launch.hpp"
+#include "rrqr.hpp"
 
 #define HIPCHK(x)                                                                                    \
     do {                                                                                             \
@@ -728,6 +729,7 @@ int Tree::factorize_t() {
         std::vector<Geqp3Task<S>> gq;
         std::vector<int> gq_cluster;
         int* d_ranks = nullptr;
+        int* d_rrqr_err = nullptr;
         const bool spars_level = (L >= skip && L < nlevels - 2 && tol != 0);
         if (spars_level) {
             std::vector<int> who;
@@ -775,7 +777,6 @@ int Tree::factorize_t() {
             GeqrtTask<S>* d_sgeq = upload_vec(sgeq);
             LarfbTask<S>* d_sapp = upload_vec(sapp);
             CopyTask<S>* d_gat = upload_vec(gat);
-            Geqp3Task<S>* d_gq = upload_vec(gq);
             long ntpm = 0;
             for (int q = 0; q < ncolors; ++q) ntpm += (long)tpm[q].size();
             if (!geq.empty()) timed(P_GEQRT, 1, (long)geq.size(), [&] { launch_geqrt<S>(d_geq, (int)geq.size(), stream_); });
@@ -792,13 +793,23 @@ int Tree::factorize_t() {
                 });
             if (!sapp.empty()) timed(P_SAPPLY, 1, (long)sapp.size(), [&] { launch_larfb<S>(d_sapp, (int)sapp.size(), stream_); });
             if (!gat.empty()) timed(P_GATHER, 1, (long)gat.size(), [&] { launch_copy<S>(d_gat, (int)gat.size(), stream_); });
-            if (!gq.empty()) timed(P_GEQP3, 1, (long)gq.size(), [&] { launch_geqp3<S>(d_gq, (int)gq.size(), tol, kPivotTieRel, stream_); });
+            if (!gq.empty()) {
+                RrqrAlloc mem{[&](size_t b) { return arena.alloc(b); },
+                              [&](const void* h, size_t b) { return upload(h, b); }};
+                const int zero = 0;
+                d_rrqr_err = (int*)upload(&zero, sizeof(int));
+                timed(P_GEQP3, 1, (long)gq.size(),
+                      [&] { geqp3_grouped<S>(gq, tol, kPivotTieRel, stream_, mem, d_rrqr_err); });
+            }
             HIPCHK(hipGetLastError());
         }
         if (!gq.empty()) {
             std::vector<int> ranks(gq.size());
+            int rrqr_err = 0;
             HIPCHK(hipMemcpyAsync(ranks.data(), d_ranks, sizeof(int) * ranks.size(), hipMemcpyDeviceToHost, stream_));
+            HIPCHK(hipMemcpyAsync(&rrqr_err, d_rrqr_err, sizeof(int), hipMemcpyDeviceToHost, stream_));
             sync_stream();
+            if (rrqr_err) throw std::runtime_error("cooperative geqp3: a barrier wait exceeded its spin limit");
             for (size_t q = 0; q < gq.size(); ++q) {
                 Cluster& C = cl[gq_cluster[q]];
                 const int k = C.cols;
