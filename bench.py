@@ -3,9 +3,10 @@
 
 One step = factorize the assembled matrix on the GPU + FP64 GMRES to ||b - A x|| / ||b|| <=
 1e-12, with the matrix blocks and b already resident in HBM.  The timed path is FP32 fronts
-(batched, one grouped launch per level phase) inside FP64 GMRES(50); the plain batched FP64
-path is timed on the same GPU in the same run and reported beside it, as is the oracle (the
-CPU restatement of the reference path) on a bounded sample.
+(batched, one grouped launch per level phase) inside FP64 flexible GMRES(50); the plain
+batched FP64 path (the reference's left-preconditioned GMRES) is timed on the same GPU in the
+same run and reported beside it, as is the oracle (the CPU restatement of the reference path)
+on a bounded sample.
 
 Prints one JSON line.  --gpus > 1 is refused: this path runs on one GPU.
 """
@@ -45,7 +46,7 @@ def profile(tree):
                              bytes=rows[i, 4]) for i in range(n)}
 
 
-def run_steps(tree, b_dev, x_dev, n, steps, restart, iters, tag):
+def run_steps(tree, b_dev, x_dev, n, steps, restart, iters, tag, flexible=False):
     from pspaqr_public_amd._lib import lib
 
     res = None
@@ -55,7 +56,8 @@ def run_steps(tree, b_dev, x_dev, n, steps, restart, iters, tag):
         f0 = time.perf_counter()
         tree.factorize()
         fact_s += time.perf_counter() - f0
-        res = tree.gmres_device(b_dev.ptr, x_dev.ptr, iters=iters, tol=1e-12, restart=restart, true_tol=1e-12)
+        res = tree.gmres_device(b_dev.ptr, x_dev.ptr, iters=iters, tol=1e-12, restart=restart, true_tol=1e-12,
+                                flexible=flexible)
     lib().spaqr_tree_sync(tree.handle)
     t1 = time.perf_counter()
     log("[%s] %d step(s): %.4f s/step (factor %.4f s), gmres iters %d cycles %d true relres %.3e" % (
@@ -121,9 +123,9 @@ def main():
     log("[bench] partition+assemble %.2f s" % (time.perf_counter() - t0))
     lib().spaqr_tree_set_profiling(T32.handle, 1)
     if args.warmup:
-        run_steps(T32, b32, x32, T32.n, args.warmup, restart, args.iters, "fp32 warmup")
+        run_steps(T32, b32, x32, T32.n, args.warmup, restart, args.iters, "fp32 warmup", True)
     lib().spaqr_tree_profile_reset(T32.handle)
-    step_s, fact_s, res32 = run_steps(T32, b32, x32, T32.n, args.steps, restart, args.iters, "fp32")
+    step_s, fact_s, res32 = run_steps(T32, b32, x32, T32.n, args.steps, restart, args.iters, "fp32", True)
     prof = profile(T32)
     stats = T32.level_stats()
     x = x32.download(np.float64, T32.n)
@@ -161,8 +163,8 @@ def main():
         Ts = build_tree(Ps, 32)
         bs = Ps.b[Ts.cperm()]
         bsd, xsd = DeviceBuffer.from_array(bs), DeviceBuffer(8 * Ts.n)
-        run_steps(Ts, bsd, xsd, Ts.n, 1, restart, args.iters, "fp32 sample warmup")
-        sg, _, _ = run_steps(Ts, bsd, xsd, Ts.n, 1, restart, args.iters, "fp32 sample")
+        run_steps(Ts, bsd, xsd, Ts.n, 1, restart, args.iters, "fp32 sample warmup", True)
+        sg, _, _ = run_steps(Ts, bsd, xsd, Ts.n, 1, restart, args.iters, "fp32 sample", True)
         sample_gpu = sg
         cpu = dict(value=cpu_s, unit="s (factor+GMRES to 1e-12 on the %s sample, n=%d)" % (Ps.name, Ps.A.shape[0]),
                    cores=1, kind="port",
@@ -188,7 +190,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "%s: 3D 7-point upwind convection-diffusion %d^3 (n=%d), tensor-grid ND -n %d -d 3, lvl %d, "
-                        "tol %g; FP32 fronts in grouped batched launches, FP64 GMRES(%d) to true relres 1e-12"
+                        "tol %g; FP32 fronts in grouped batched launches, FP64 flexible GMRES(%d) to true relres 1e-12"
                         % (P.name, P.n_side, P.A.shape[0], P.n_side, P.nlevels, P.tol, restart)
                         if args.config == 3 else "%s (n=%d, lvl %d, tol %g)" % (P.name, P.A.shape[0], P.nlevels, P.tol),
             "n": int(P.A.shape[0]),

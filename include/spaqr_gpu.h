@@ -77,6 +77,13 @@ int spaqr_tree_spmv(spaqr_tree* t, const double* x_dev, double* y_dev);
    the per-step preconditioned residual estimates.                         src/is.cpp:447 */
 int spaqr_gmres(spaqr_tree* t, const double* b_dev, double* x_dev, int iters, double tol, int restart,
                 double true_tol, spaqr_gmres_result* out, double* hist, int hist_cap, int* hist_len);
+/* Flexible (right-preconditioned) GMRES with the same arguments: z_k = M^{-1} q_k is stored
+   and the recurrence estimates the true residual ||b - A x|| / ||b||, which is what tol and
+   the hist entries refer to here.  For FP32 factors, whose rounding makes M^{-1} slightly
+   non-linear.  No counterpart in the reference (its dist_gmres is left-preconditioned,
+   src/is.cpp:447). */
+int spaqr_fgmres(spaqr_tree* t, const double* b_dev, double* x_dev, int iters, double tol, int restart,
+                 double true_tol, spaqr_gmres_result* out, double* hist, int hist_cap, int* hist_len);
 /* Tree::rows() / cols() and ParTree::nrows_local / ncols_local (one rank)  src/tree.cpp:23 */
 int spaqr_tree_n(const spaqr_tree* t);
 /* column permutation (cperm, natural index of permuted position i)        include/tree.h:53 */

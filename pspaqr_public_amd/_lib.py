@@ -76,6 +76,7 @@ EXPORTS = {
     "spaqr_tree_solve": (_i, [_vp, _vp]),
     "spaqr_tree_spmv": (_i, [_vp, _vp, _vp]),
     "spaqr_gmres": (_i, [_vp, _vp, _vp, _i, _d, _i, _d, C.POINTER(GmresResult), _vp, _i, C.POINTER(_i)]),
+    "spaqr_fgmres": (_i, [_vp, _vp, _vp, _i, _d, _i, _d, C.POINTER(GmresResult), _vp, _i, C.POINTER(_i)]),
     "spaqr_tree_n": (_i, [_vp]),
     "spaqr_tree_get_cperm": (_i, [_vp, _vp]),
     "spaqr_tree_num_clusters": (_i, [_vp]),

@@ -134,7 +134,16 @@ int spaqr_gmres(spaqr_tree* h, const double* b, double* x, int iters, double tol
     TREE_OR_FAIL(h);
     if (!b || !x || iters <= 0) return -2;
     GmresResult r;
-    int rc = h->t->gmres_dev(b, x, iters, tol, restart, true_tol, &r, hist, hist_cap, hist_len);
+    int rc = h->t->gmres_dev(b, x, iters, tol, restart, true_tol, &r, hist, hist_cap, hist_len, false);
+    if (out) { out->iters = r.iters; out->cycles = r.cycles; out->prec_res = r.prec_res; out->true_res = r.true_res; }
+    return rc;
+}
+int spaqr_fgmres(spaqr_tree* h, const double* b, double* x, int iters, double tol, int restart, double true_tol,
+                 spaqr_gmres_result* out, double* hist, int hist_cap, int* hist_len) {
+    TREE_OR_FAIL(h);
+    if (!b || !x || iters <= 0) return -2;
+    GmresResult r;
+    int rc = h->t->gmres_dev(b, x, iters, tol, restart, true_tol, &r, hist, hist_cap, hist_len, true);
     if (out) { out->iters = r.iters; out->cycles = r.cycles; out->prec_res = r.prec_res; out->true_res = r.true_res; }
     return rc;
 }

@@ -59,7 +59,7 @@ __device__ __forceinline__ S wave_sum(S v) {
 // upper, ldr); B (m x nbp, ldb)], reflector j is [e_j; B(:, j)] (dtpqrt2 with L = 0).
 // The nbp x nbp T of the panel is written to Tout (ld ldt).
 template <typename S, bool PENT>
-__device__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* Tout, int ldt, PanelSmem<S>& sm) {
+__device__ __forceinline__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* Tout, int ldt, PanelSmem<S>& sm) {
     const int tid = threadIdx.x;
     const int l = tid >> 3, g = tid & 7;
     for (int e = tid; e < 32 * 33; e += WG) sm.Ts[e] = S(0);
@@ -132,7 +132,7 @@ __device__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* To
 // [Atop; C] <- H^T [Atop; C] (PENT, V is the m x kb lower block of [I; V]) or
 // C <- H^T C (unit lower V, m x kb, implicit ones / zeros above).  kb <= 32.
 template <typename S, bool PENT>
-__device__ void wg_apply_lt(int m, int kb, const S* __restrict__ V, int ldv, const S* __restrict__ T, int ldt,
+__device__ __forceinline__ void wg_apply_lt(int m, int kb, const S* __restrict__ V, int ldv, const S* __restrict__ T, int ldt,
                             S* Atop, int lda, S* C, int ldc, int ncols, ApplySmem<S>& sm) {
     const int tid = threadIdx.x;
     __syncthreads();
@@ -227,7 +227,7 @@ __device__ void wg_apply_lt(int m, int kb, const S* __restrict__ V, int ldv, con
 // ---------------------------------------------------------------------------------------
 // C (m x n) <- C H, H = I - V T V^T, V (n x kb unit lower, ldv), kb <= 32.
 template <typename S>
-__device__ void wg_apply_rn(int m, int n, int kb, const S* __restrict__ V, int ldv, const S* __restrict__ T, int ldt,
+__device__ __forceinline__ void wg_apply_rn(int m, int n, int kb, const S* __restrict__ V, int ldv, const S* __restrict__ T, int ldt,
                             S* C, int ldc, ApplySmem<S>& sm) {
     const int tid = threadIdx.x;
     __syncthreads();
@@ -304,7 +304,7 @@ __device__ void wg_apply_rn(int m, int n, int kb, const S* __restrict__ V, int l
 // B (m x n) <- B R^{-1}, R upper (ldr).  32-column blocks: B_j -= B_{<j} R_{<j,j} then a
 // per-row triangular solve with the 32 x 32 diagonal block in LDS.
 template <typename S>
-__device__ void wg_trsm_ru(int m, int n, const S* __restrict__ R, int ldr, S* B, int ldb, ApplySmem<S>& sm) {
+__device__ __forceinline__ void wg_trsm_ru(int m, int n, const S* __restrict__ R, int ldr, S* B, int ldb, ApplySmem<S>& sm) {
     const int tid = threadIdx.x;
     const int rr = tid & 63, cg = tid >> 6;
     for (int j0 = 0; j0 < n; j0 += 32) {
@@ -368,7 +368,7 @@ __device__ void wg_trsm_ru(int m, int n, const S* __restrict__ R, int ldr, S* B,
 // ---------------------------------------------------------------------------------------
 // Blocked T (NB x k) of the unit-lower reflectors V (m x k, ldv) with scalars tau.
 template <typename S>
-__device__ void wg_larft_blocked(int m, int k, const S* V, int ldv, const S* tau, S* Tout, PanelSmem<S>& sm) {
+__device__ __forceinline__ void wg_larft_blocked(int m, int k, const S* V, int ldv, const S* tau, S* Tout, PanelSmem<S>& sm) {
     const int tid = threadIdx.x;
     const int l = tid >> 3, g = tid & 7;
     for (int p0 = 0; p0 < k; p0 += NB) {
@@ -682,130 +682,259 @@ __global__ void k_fill(S* p, size_t n, S v) {
 // Preconditioner apply (vectors)
 // =======================================================================================
 
-// x <- H^T x (trans) or H x, H = I - V T V^T, unit-lower V (m x kb), kb <= 32.
-// PENT: [xt (kb); x (m)] with V the lower block of [I; V] (trans only).
+// Every vector kernel below gives each thread a fixed set of absolute rows (row R belongs
+// to thread R mod WG) for the whole task, so a thread only ever reads back entries it wrote
+// itself; the threads exchange nothing but the 32 reflector coefficients of the current
+// panel, through LDS.
+
+template <typename S>
+struct VecSmem {
+    S red[4][32];   // per-wave partial column sums of one panel
+    S t[32];        // panel coefficients t = T^T d (or T d)
+    S xb[32];       // triangular-solve block
+    S T[32 * 33];   // the panel's T block, column l at T[l * 33]
+};
+
+// Sum the 32 per-lane values d[0..31] over the wave.  On return lane L holds the total of
+// column vec_col(L) (lanes L and L^1 hold the same column): a transposing butterfly, 32 shuffles.
+__device__ __forceinline__ int vec_col(int lane) {
+    return ((lane >> 5) & 1) * 16 + ((lane >> 4) & 1) * 8 + ((lane >> 3) & 1) * 4 + ((lane >> 2) & 1) * 2 +
+           ((lane >> 1) & 1);
+}
+
+template <int H, typename S>
+__device__ __forceinline__ void butterfly_step(S (&d)[32], int lane) {
+    const bool up = (lane & (2 * H)) != 0;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const S send = up ? d[i] : d[i + H];
+        const S keep = up ? d[i + H] : d[i];
+        d[i] = keep + __shfl_xor(send, 2 * H, 64);
+    }
+}
+
+template <typename S>
+__device__ __forceinline__ S wave_sum32(S (&d)[32]) {
+    const int lane = threadIdx.x & 63;
+    butterfly_step<16>(d, lane);
+    butterfly_step<8>(d, lane);
+    butterfly_step<4>(d, lane);
+    butterfly_step<2>(d, lane);
+    butterfly_step<1>(d, lane);
+    return d[0] + __shfl_xor(d[0], 1, 64);
+}
+
+// One panel (kb <= 32 reflectors) of a block reflector H = I - V T V^T applied to a vector:
+// x <- H^T x (trans) or H x.  T is the panel's upper-triangular kb x kb block (ld NB).
+//   PENT = false: V is unit lower-trapezoidal (geqrt layout), its column l acting on rows
+//     p0 + l .. m-1 of x (absolute rows; rows < p0 are untouched).
+//   PENT = true: V = [I; Vb] (tpqrt layout): xt[p0 .. p0+kb) is the top block and Vb
+//     (m rows) acts on x[0 .. m); xt entries are owned by lanes of wave 0.
+// The kernels are latency-bound, so every load is unconditional (columns past kb are
+// clamped to column kb-1 and weighted by zero): the 32 loads of a row issue together and the
+// wave waits once.
 template <typename S, bool PENT>
-__device__ void wg_vec_apply(int m, int kb, const S* __restrict__ V, int ldv, const S* __restrict__ T, int ldt,
-                             S* xt, S* x, bool trans, PanelSmem<S>& sm) {
-    const int tid = threadIdx.x;
-    const int l = tid >> 3, g = tid & 7;
-    S d = S(0);
-    if (l < kb) {
-        const S* vl = V + (size_t)l * ldv;
-        if (PENT) {
-            for (int r = g; r < m; r += 8) d += vl[r] * x[r];
-        } else {
-            for (int r = l + 1 + g; r < m; r += 8) d += vl[r] * x[r];
+__device__ __forceinline__ void wg_vec_panel(int m, int p0, int kb, const S* __restrict__ V, int ldv, const S* __restrict__ T,
+                             S* xt, S* x, bool trans, VecSmem<S>& sm) {
+    constexpr int VH = sizeof(S) == 8 ? 16 : 32;  // loads in flight per row (register budget)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the panel's T block into LDS (read in phase 2; the previous panel finished with it)
+    {
+        S tv[(NB * NB) / WG];
+#pragma unroll
+        for (int i = 0; i < (NB * NB) / WG; ++i) {
+            const int e = tid + i * WG, q = e & (NB - 1), l = min(e / NB, kb - 1);
+            tv[i] = T[q + (size_t)l * NB];
+        }
+#pragma unroll
+        for (int i = 0; i < (NB * NB) / WG; ++i) {
+            const int e = tid + i * WG, q = e & (NB - 1), l = e / NB;
+            sm.T[l * 33 + q] = (q < kb && l < kb) ? tv[i] : S(0);
         }
     }
-    d += __shfl_xor(d, 1, 64);
-    d += __shfl_xor(d, 2, 64);
-    d += __shfl_xor(d, 4, 64);
-    if (g == 0 && l < kb) sm.d[l] = d + (PENT ? xt[l] : x[l]);
+    S xtv = S(0);
+    if (PENT && wave == 0 && lane < 32) xtv = xt[p0 + min(lane, kb - 1)];
+    // phase 1: d_l = sum_R v(R, l) x[R] over this thread's rows.  Non-PENT: the kb rows
+    // p0 .. p0+kb-1 hold the unit diagonal and zeros above it; rows from p0+kb on are full.
+    S d[32];
+#pragma unroll
+    for (int l = 0; l < 32; ++l) d[l] = S(0);
+    const int r0 = PENT ? 0 : p0;  // first row the panel touches
+    const S* __restrict__ Vp = V + (size_t)(PENT ? 0 : p0) * ldv;  // uniform base, 32-bit offsets below
+    for (int R = r0 + ((tid - r0 % WG + WG) % WG); R < m; R += WG) {
+        const S xr = x[R];
+        const int i = R - r0;  // non-PENT: column l of V is zero above row l and one on it
+#pragma unroll
+        for (int h = 0; h < 32; h += VH) {
+            S v[VH];
+#pragma unroll
+            for (int l = 0; l < VH; ++l) v[l] = Vp[(unsigned)(R + min(h + l, kb - 1) * ldv)];
+#pragma unroll
+            for (int l = 0; l < VH; ++l)
+                d[h + l] += (PENT || h + l < i ? v[l] : (h + l == i ? S(1) : S(0))) * xr;
+        }
+    }
+    const S tot = wave_sum32<S>(d);
+    if ((lane & 1) == 0) sm.red[wave][vec_col(lane)] = tot;
     __syncthreads();
-    if (tid < kb) {
+    // phase 2 (wave 0): t = T^T d or T d; columns >= kb of d are ignored (T is zero there)
+    if (wave == 0) {
+        S dl = S(0);
+        if (lane < 32) dl = sm.red[0][lane] + sm.red[1][lane] + sm.red[2][lane] + sm.red[3][lane] + xtv;
+        const int lc = min(lane, 31);
+        S tl = S(0);
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            const S dq = __shfl(dl, q, 64);
+            const S tq = trans ? sm.T[lc * 33 + q] : sm.T[q * 33 + lc];
+            const bool use = trans ? (q <= lc) : (q >= lc);
+            tl += (use ? tq : S(0)) * dq;
+        }
+        if (lane < 32) sm.t[lane] = (lane < kb) ? tl : S(0);
+        if (PENT && lane < kb) xt[p0 + lane] = xtv - tl;
+    }
+    __syncthreads();
+    // phase 3: x[R] -= sum_l v(R, l) t_l  (t_l = 0 for l >= kb)
+    S t[32];
+#pragma unroll
+    for (int l = 0; l < 32; ++l) t[l] = sm.t[l];
+    for (int R = r0 + ((tid - r0 % WG + WG) % WG); R < m; R += WG) {
+        const int i = R - r0;
         S s = S(0);
-        if (trans) {
-            for (int q = 0; q <= tid; ++q) s += T[q + (size_t)tid * ldt] * sm.d[q];
-        } else {
-            for (int q = tid; q < kb; ++q) s += T[tid + (size_t)q * ldt] * sm.d[q];
+#pragma unroll
+        for (int h = 0; h < 32; h += VH) {
+            S v[VH];
+#pragma unroll
+            for (int l = 0; l < VH; ++l) v[l] = Vp[(unsigned)(R + min(h + l, kb - 1) * ldv)];
+#pragma unroll
+            for (int l = 0; l < VH; ++l) s += (PENT || h + l < i ? v[l] : (h + l == i ? S(1) : S(0))) * t[h + l];
         }
-        sm.t[tid] = s;
+        x[R] -= s;
     }
-    __syncthreads();
-    if (PENT && tid < kb) xt[tid] -= sm.t[tid];
-    for (int r = tid; r < m; r += WG) {
-        S s = S(0);
-        if (PENT) {
-            for (int i = 0; i < kb; ++i) s += V[r + (size_t)i * ldv] * sm.t[i];
-        } else {
-            int lim = min(r, kb - 1);
-            for (int i = 0; i <= lim; ++i) s += ((r == i) ? S(1) : V[r + (size_t)i * ldv]) * sm.t[i];
-        }
-        x[r] -= s;
+}
+
+// x <- Q^T x (trans) or Q x for Q = H_0 H_1 ... (geqrt layout, k reflectors over m rows,
+// blocked T with NB-column panels).
+template <typename S>
+__device__ __forceinline__ void wg_vec_q(int m, int k, const S* __restrict__ V, int ldv, const S* __restrict__ T, S* x, bool trans,
+                         VecSmem<S>& sm) {
+    if (k <= 0) return;
+    if (trans) {
+        for (int p0 = 0; p0 < k; p0 += NB)
+            wg_vec_panel<S, false>(m, p0, min(NB, k - p0), V, ldv, T + (size_t)p0 * NB, nullptr, x, true, sm);
+    } else {
+        for (int p0 = ((k - 1) / NB) * NB; p0 >= 0; p0 -= NB)
+            wg_vec_panel<S, false>(m, p0, min(NB, k - p0), V, ldv, T + (size_t)p0 * NB, nullptr, x, false, sm);
     }
-    __syncthreads();
 }
 
 template <typename S>
 __global__ __launch_bounds__(WG) void k_vec_q(const VecQTask<S>* __restrict__ tasks) {
-    __shared__ PanelSmem<S> sm;
+    __shared__ VecSmem<S> sm;
     const VecQTask<S> t = tasks[blockIdx.x];
-    if (t.trans) {
-        for (int p0 = 0; p0 < t.k; p0 += NB)
-            wg_vec_apply<S, false>(t.m - p0, min(NB, t.k - p0), t.V + p0 + (size_t)p0 * t.ldv, t.ldv,
-                                   t.T + (size_t)p0 * NB, NB, nullptr, t.x + p0, true, sm);
-    } else {
-        int last = ((t.k - 1) / NB) * NB;
-        for (int p0 = last; p0 >= 0; p0 -= NB)
-            wg_vec_apply<S, false>(t.m - p0, min(NB, t.k - p0), t.V + p0 + (size_t)p0 * t.ldv, t.ldv,
-                                   t.T + (size_t)p0 * NB, NB, nullptr, t.x + p0, false, sm);
-    }
+    wg_vec_q<S>(t.m, t.k, t.V, t.ldv, t.T, t.x, t.trans != 0, sm);
 }
 
 template <typename S>
 __global__ __launch_bounds__(WG) void k_vec_chain(const VecChainTask<S>* __restrict__ tasks,
                                                   const VecChainLink<S>* __restrict__ links) {
-    __shared__ PanelSmem<S> sm;
+    __shared__ VecSmem<S> sm;
     const VecChainTask<S> t = tasks[blockIdx.x];
     for (int li = 0; li < t.nlinks; ++li) {
         const VecChainLink<S> L = links[t.link0 + li];
         for (int p0 = 0; p0 < t.k; p0 += NB)
-            wg_vec_apply<S, true>(L.rows, min(NB, t.k - p0), L.V + (size_t)p0 * L.ldv, L.ldv, L.T + (size_t)p0 * NB,
-                                  NB, t.xc + p0, L.xn, true, sm);
+            wg_vec_panel<S, true>(L.rows, p0, min(NB, t.k - p0), L.V + (size_t)p0 * L.ldv, L.ldv,
+                                  L.T + (size_t)p0 * NB, t.xc, L.xn, true, sm);
     }
 }
 
-// x[0:k] <- R^{-1} x[0:k], R upper (ldr); one workgroup, column-oriented back substitution.
+// x[0:k] <- R^{-1} x[0:k], R upper triangular (ldr): 32-row blocks from the bottom; wave 0
+// solves the diagonal block with shuffles, then every thread updates its own rows above.
 template <typename S>
-__device__ void wg_trsv_upper(int k, const S* __restrict__ R, int ldr, S* x) {
-    __shared__ S xj_sh;
-    const int tid = threadIdx.x;
-    for (int j = k - 1; j >= 0; --j) {
-        if (tid == 0) { S v = x[j] / R[j + (size_t)j * ldr]; x[j] = v; xj_sh = v; }
+__device__ __forceinline__ void wg_trsv_upper(int k, const S* __restrict__ R, int ldr, S* x, VecSmem<S>& sm) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int p0 = ((k - 1) / NB) * NB; p0 >= 0 && k > 0; p0 -= NB) {
+        const int kb = min(NB, k - p0);
+        const S* __restrict__ Rc = R + (size_t)p0 * ldr;  // the block's columns
+        // wave 0 loads its rows of the diagonal block while the right-hand side is gathered
+        S row[32];
+        const int lr = min(lane, kb - 1);
+        if (wave == 0) {
+#pragma unroll
+            for (int j = 0; j < 32; ++j) row[j] = Rc[(unsigned)(p0 + lr + min(j, kb - 1) * ldr)];
+        }
+        for (int r = p0 + ((tid - p0 % WG + WG) % WG); r < p0 + kb; r += WG) sm.xb[r - p0] = x[r];
         __syncthreads();
-        const S xj = xj_sh;
-        for (int i = tid; i < j; i += WG) x[i] -= R[i + (size_t)j * ldr] * xj;
+        if (wave == 0) {
+            S xi = (lane < kb) ? sm.xb[lr] : S(0);
+#pragma unroll
+            for (int j = 31; j >= 0; --j) {
+                if (j < kb) {
+                    if (lane == j) xi = xi / row[j];
+                    const S xj = __shfl(xi, j, 64);
+                    if (lane < j) xi -= row[j] * xj;
+                }
+            }
+            if (lane < 32) sm.xb[lane] = (lane < kb) ? xi : S(0);
+        }
         __syncthreads();
+        for (int r = tid; r < p0 + kb; r += WG) {
+            if (r >= p0) {
+                x[r] = sm.xb[r - p0];
+            } else {
+                S v[32];
+#pragma unroll
+                for (int j = 0; j < 32; ++j) v[j] = Rc[(unsigned)(r + min(j, kb - 1) * ldr)];
+                S s = S(0);
+#pragma unroll
+                for (int j = 0; j < 32; ++j) s += v[j] * sm.xb[j];
+                x[r] -= s;
+            }
+        }
+        __syncthreads();  // sm.xb is rewritten by the next block
     }
 }
 
 template <typename S>
 __global__ __launch_bounds__(WG) void k_vec_spars(const VecSparsTask<S>* __restrict__ tasks, int fwd) {
-    __shared__ PanelSmem<S> sm;
+    __shared__ VecSmem<S> sm;
     const VecSparsTask<S> t = tasks[blockIdx.x];
     if (fwd) {
-        for (int p0 = 0; p0 < t.k; p0 += NB)
-            wg_vec_apply<S, false>(t.ldq - p0, min(NB, t.k - p0), t.Qs + p0 + (size_t)p0 * t.ldq, t.ldq,
-                                   t.Ts + (size_t)p0 * NB, NB, nullptr, t.x + p0, true, sm);
-        for (int p0 = 0; p0 < t.r; p0 += NB)
-            wg_vec_apply<S, false>(t.k - p0, min(NB, t.r - p0), t.Qsp + p0 + (size_t)p0 * t.k, t.k,
-                                   t.Tsp + (size_t)p0 * NB, NB, nullptr, t.x + p0, true, sm);
+        wg_vec_q<S>(t.ldq, t.k, t.Qs, t.ldq, t.Ts, t.x, true, sm);
+        __syncthreads();
+        wg_vec_q<S>(t.k, t.r, t.Qsp, t.k, t.Tsp, t.x, true, sm);
     } else {
-        if (t.r > 0) {
-            int last = ((t.r - 1) / NB) * NB;
-            for (int p0 = last; p0 >= 0; p0 -= NB)
-                wg_vec_apply<S, false>(t.k - p0, min(NB, t.r - p0), t.Qsp + p0 + (size_t)p0 * t.k, t.k,
-                                       t.Tsp + (size_t)p0 * NB, NB, nullptr, t.x + p0, false, sm);
-        }
-        wg_trsv_upper<S>(t.k, t.Qs, t.ldq, t.x);
+        wg_vec_q<S>(t.k, t.r, t.Qsp, t.k, t.Tsp, t.x, false, sm);
+        __syncthreads();
+        wg_trsv_upper<S>(t.k, t.Qs, t.ldq, t.x, sm);
     }
 }
 
 template <typename S>
 __global__ __launch_bounds__(WG) void k_vec_trsv(const VecTrsvTask<S>* __restrict__ tasks,
                                                  const VecGemvLink<S>* __restrict__ links) {
+    __shared__ VecSmem<S> sm;
     const VecTrsvTask<S> t = tasks[blockIdx.x];
     for (int li = 0; li < t.nlinks; ++li) {
         const VecGemvLink<S> L = links[t.link0 + li];
+        if (L.s <= 0) continue;
         for (int i = threadIdx.x; i < t.k; i += WG) {
             S s = S(0);
-            for (int j = 0; j < L.s; ++j) s += L.A[i + (size_t)j * L.lda] * L.y[j];
+            for (int j0 = 0; j0 < L.s; j0 += 16) {
+                S a[16], y[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int jj = min(j0 + j, L.s - 1);
+                    a[j] = L.A[(unsigned)(i + jj * L.lda)];
+                    y[j] = (j0 + j < L.s) ? L.y[jj] : S(0);
+                }
+#pragma unroll
+                for (int j = 0; j < 16; ++j) s += a[j] * y[j];
+            }
             t.x[i] -= s;
         }
-        __syncthreads();
     }
-    wg_trsv_upper<S>(t.k, t.R, t.ldr, t.x);
+    wg_trsv_upper<S>(t.k, t.R, t.ldr, t.x, sm);
 }
 
 template <typename S>
@@ -1323,6 +1452,21 @@ hipError_t launch_geqp3_coop(const Geqp3CoopTask<S>* d_tasks, const int* d_blk2t
                                       (unsigned)lds_bytes, s);
 }
 
+template <typename S>
+size_t geqp3_coop_lds_budget(int blocks_per_cu) {
+    int dev = 0, lds_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess)
+        return 0;
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, (const void*)k_geqp3_coop<S>) != hipSuccess) return 0;
+    const size_t per = (size_t)lds_cu / (size_t)std::max(blocks_per_cu, 1);
+    if (per <= fa.sharedSizeBytes + 1024) return 0;
+    return ((per - fa.sharedSizeBytes) / 1024) * 1024;  // dynamic bytes, whole KiB
+}
+
+template size_t geqp3_coop_lds_budget<double>(int);
+template size_t geqp3_coop_lds_budget<float>(int);
 template int geqp3_coop_max_blocks<double>(size_t);
 template int geqp3_coop_max_blocks<float>(size_t);
 template hipError_t launch_geqp3_coop<double>(const Geqp3CoopTask<double>*, const int*, int*, int, double, double,

@@ -25,6 +25,9 @@ template <typename S> void launch_vec_copy(const VecCopyTask<S>* d, int n, hipSt
 template <typename S> void launch_convert_d2s(int n, const double* x, S* y, hipStream_t s);
 template <typename S> void launch_convert_s2d(int n, const S* x, double* y, hipStream_t s);
 
+// dynamic LDS bytes per cooperative geqp3 workgroup such that blocks_per_cu of them fit on a CU
+template <typename S>
+size_t geqp3_coop_lds_budget(int blocks_per_cu);
 template <typename S> int geqp3_coop_max_blocks(size_t lds_bytes);
 template <typename S>
 hipError_t launch_geqp3_coop(const Geqp3CoopTask<S>* d_tasks, const int* d_blk2task, int* d_err, int nblocks,

@@ -13,20 +13,33 @@ namespace spaqr {
 
 template <typename S>
 int geqp3_grouped(const std::vector<Geqp3Task<S>>& tasks, double tol, double tie_rel, hipStream_t stream,
-                  const RrqrAlloc& mem, int* d_err) {
+                  const RrqrAlloc& mem, int* d_err, int* dispatches) {
     static int max_blocks_cache[2] = {-1, -1};
+    static size_t lds_cache[2] = {0, 0};
     int& maxb = max_blocks_cache[sizeof(S) == 8 ? 0 : 1];
-    if (maxb < 0) maxb = geqp3_coop_max_blocks<S>(kGeqp3CoopLds);
+    size_t& lds = lds_cache[sizeof(S) == 8 ? 0 : 1];
+    if (maxb < 0) {
+        lds = geqp3_coop_lds_budget<S>(kGeqp3CoopBlocksPerCU);
+        maxb = lds ? geqp3_coop_max_blocks<S>(lds) : 0;
+    }
     std::vector<Geqp3Task<S>> fallback;
     struct Pending { std::vector<Geqp3CoopTask<S>> t; std::vector<int> b2t; int blocks = 0, jmax = 0, active = 0; };
     std::vector<int> zeros;
     std::vector<Pending> launches(1);
-    for (const auto& g : tasks) {
+    // A launch lasts as long as its longest pivot sequence, so interfaces of similar height
+    // are packed together: largest first.
+    std::vector<int> order(tasks.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return std::min(tasks[a].k, tasks[a].w) > std::min(tasks[b].k, tasks[b].w);
+    });
+    for (int oi : order) {
+        const auto& g = tasks[oi];
         const int k = g.k, w = g.w;
         const int kmax = (tol >= 1.0) ? 0 : std::min(k, w);
         const size_t per_col = (size_t)k * sizeof(S) + 2 * sizeof(S) + sizeof(int);  // slice, vn1, vn2, pos
         const size_t fixed = (size_t)k * sizeof(S) + 64;
-        int cpb = (kGeqp3CoopLds > fixed) ? (int)((kGeqp3CoopLds - fixed) / per_col) : 0;
+        int cpb = (lds > fixed) ? (int)((lds - fixed) / per_col) : 0;
         if (w > 0) cpb = std::min(cpb, w);
         const int P = (w > 0 && cpb > 0) ? (w + cpb - 1) / cpb : 1;
         if (maxb <= 0 || cpb <= 0 || P > maxb) {
@@ -62,28 +75,39 @@ int geqp3_grouped(const std::vector<Geqp3Task<S>>& tasks, double tol, double tie
         int kmx = 0, wmx = 0, pmx = 0;
         for (auto& L : launches)
             for (auto& c : L.t) { kmx = std::max(kmx, c.k); wmx = std::max(wmx, c.w); pmx = std::max(pmx, c.P); }
-        fprintf(stderr, "[geqp3] tasks %zu maxb %d launches %zu fallback %zu  max k %d w %d P %d\n", tasks.size(), maxb,
-                launches.size(), fallback.size(), kmx, wmx, pmx);
+        fprintf(stderr, "[geqp3] tasks %zu maxb %d lds %zu launches %zu fallback %zu  max k %d w %d P %d\n",
+                tasks.size(), maxb, lds, launches.size(), fallback.size(), kmx, wmx, pmx);
         for (auto& f : fallback) fprintf(stderr, "   fallback k %d w %d\n", f.k, f.w);
     }
+    // all tables are uploaded before the first dispatch so that the launches run back to back
+    size_t ntask = 0;
+    for (auto& L : launches) ntask += L.t.size();
+    zeros.assign(std::max<size_t>(ntask, 1), 0);
+    int* d_bars = (int*)mem.upload(zeros.data(), zeros.size() * sizeof(int));
+    struct Ready { Geqp3CoopTask<S>* t; int* b2t; int blocks; };
+    std::vector<Ready> ready;
+    size_t bar_off = 0;
     for (auto& L : launches) {
         if (L.blocks == 0) continue;
-        zeros.assign(L.t.size(), 0);
-        int* d_bars = (int*)mem.upload(zeros.data(), zeros.size() * sizeof(int));
-        for (size_t i = 0; i < L.t.size(); ++i) L.t[i].bar = d_bars + i;
+        for (size_t i = 0; i < L.t.size(); ++i) L.t[i].bar = d_bars + bar_off + i;
+        bar_off += L.t.size();
         auto* d_t = (Geqp3CoopTask<S>*)mem.upload(L.t.data(), L.t.size() * sizeof(Geqp3CoopTask<S>));
         auto* d_b = (int*)mem.upload(L.b2t.data(), L.b2t.size() * sizeof(int));
-        hipError_t e = launch_geqp3_coop<S>(d_t, d_b, d_err, L.blocks, tol, tie_rel, kGeqp3CoopLds, stream);
+        ready.push_back({d_t, d_b, L.blocks});
+    }
+    Geqp3Task<S>* d_fb = fallback.empty()
+                             ? nullptr
+                             : (Geqp3Task<S>*)mem.upload(fallback.data(), fallback.size() * sizeof(Geqp3Task<S>));
+    for (const auto& r : ready) {
+        hipError_t e = launch_geqp3_coop<S>(r.t, r.b2t, d_err, r.blocks, tol, tie_rel, lds, stream);
         if (e != hipSuccess) throw std::runtime_error(std::string("cooperative geqp3 launch: ") + hipGetErrorString(e));
     }
-    if (!fallback.empty()) {
-        auto* d = (Geqp3Task<S>*)mem.upload(fallback.data(), fallback.size() * sizeof(Geqp3Task<S>));
-        launch_geqp3<S>(d, (int)fallback.size(), tol, tie_rel, stream);
-    }
+    if (d_fb) launch_geqp3<S>(d_fb, (int)fallback.size(), tol, tie_rel, stream);
+    if (dispatches) *dispatches = (int)ready.size() + (d_fb ? 1 : 0);
     return (int)fallback.size();
 }
 
-template int geqp3_grouped<double>(const std::vector<Geqp3Task<double>>&, double, double, hipStream_t, const RrqrAlloc&, int*);
-template int geqp3_grouped<float>(const std::vector<Geqp3Task<float>>&, double, double, hipStream_t, const RrqrAlloc&, int*);
+template int geqp3_grouped<double>(const std::vector<Geqp3Task<double>>&, double, double, hipStream_t, const RrqrAlloc&, int*, int*);
+template int geqp3_grouped<float>(const std::vector<Geqp3Task<float>>&, double, double, hipStream_t, const RrqrAlloc&, int*, int*);
 
 }  // namespace spaqr

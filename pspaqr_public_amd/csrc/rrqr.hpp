@@ -16,13 +16,15 @@ struct RrqrAlloc {
     std::function<void*(const void*, size_t)> upload;    // async H2D copy of a host buffer, returns device ptr
 };
 
-// LDS budget of one cooperative geqp3 workgroup (column slice + norms + reflector)
-constexpr size_t kGeqp3CoopLds = 64 * 1024;
+// Cooperative geqp3 workgroups resident per CU: each gets an equal share of the CU's LDS for
+// its column slice, norms and reflector.
+constexpr int kGeqp3CoopBlocksPerCU = 2;
 
 // Returns the number of tasks that ran on the single-workgroup fallback.  *d_err (device int,
 // zero on entry) becomes non-zero if a cooperative barrier wait hit its spin limit.
+// *dispatches (if non-null) receives the number of kernel launches issued.
 template <typename S>
 int geqp3_grouped(const std::vector<Geqp3Task<S>>& tasks, double tol, double tie_rel, hipStream_t stream,
-                  const RrqrAlloc& mem, int* d_err);
+                  const RrqrAlloc& mem, int* d_err, int* dispatches = nullptr);
 
 }  // namespace spaqr

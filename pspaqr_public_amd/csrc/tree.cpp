@@ -798,8 +798,10 @@ int Tree::factorize_t() {
                               [&](const void* h, size_t b) { return upload(h, b); }};
                 const int zero = 0;
                 d_rrqr_err = (int*)upload(&zero, sizeof(int));
-                timed(P_GEQP3, 1, (long)gq.size(),
-                      [&] { geqp3_grouped<S>(gq, tol, kPivotTieRel, stream_, mem, d_rrqr_err); });
+                int nd = 0;
+                timed(P_GEQP3, 0, (long)gq.size(),
+                      [&] { geqp3_grouped<S>(gq, tol, kPivotTieRel, stream_, mem, d_rrqr_err, &nd); });
+                if (profiling) prof[P_GEQP3].launches += nd;
             }
             HIPCHK(hipGetLastError());
         }
@@ -1087,13 +1089,23 @@ int Tree::diag_R(int order, double* out, int maxn) {
 // one small readback per iteration.  restart > 0 restarts from the current iterate (an
 // extension; the reference runs one cycle) until ||b - A x|| / ||b|| <= true_tol.
 int Tree::gmres_dev(const double* b, double* x, int iters, double tol_, int restart, double true_tol,
-                    GmresResult* res, double* hist, int hist_cap, int* hist_len) {
+                    GmresResult* res, double* hist, int hist_cap, int* hist_len, bool flexible) {
+    // Left-preconditioned (flexible == false): dist_gmres, src/is.cpp:447-527 -- Arnoldi on
+    // M^{-1}A with modified Gram-Schmidt and Givens rotations, x0 = 0, stop when the
+    // preconditioned residual estimate |beta_{k+1}| drops to tol * ||M^{-1} b||.
+    // Flexible (right-preconditioned) variant: z_k = M^{-1} q_k is kept, w = A z_k is
+    // orthogonalised, x = x0 + Z y, and |beta_{k+1}| estimates the TRUE residual ||b - A x||, so a
+    // preconditioner whose rounding makes it slightly non-linear (FP32 factors) does not stall
+    // the stopping test; it stops at |beta_{k+1}| <= tol * ||b||.
+    // Restarts (restart > 0) continue from the current iterate with the same absolute target.
     try {
         if (!factorized) throw std::runtime_error("factorize() first");
         const int m = (restart > 0) ? std::min(restart, iters) : iters;
         double* Q = nullptr;
+        double* Z = nullptr;
         double* w = nullptr;
         HIPCHK(hipMalloc(&Q, sizeof(double) * (size_t)n * (m + 1)));
+        if (flexible) HIPCHK(hipMalloc(&Z, sizeof(double) * (size_t)n * m));
         HIPCHK(hipMalloc(&w, sizeof(double) * (size_t)n));
         double* part = d_scratch;
         double* hdev = d_scratch + dot_blocks;  // up to 4096 scalars
@@ -1108,11 +1120,11 @@ int Tree::gmres_dev(const double* b, double* x, int iters, double tol_, int rest
         HIPCHK(hipMemcpyAsync(&bnorm, hdev, sizeof(double), hipMemcpyDeviceToHost, stream_));
         sync_stream();
         int total = 0, cycles = 0;
-        double last_err = 1.0, true_rel = 1.0;
+        double last_err = 1.0, true_rel = 1.0, ref_norm = -1.0;
         bool first = true;
         while (true) {
             cycles++;
-            // r = M^{-1} (b - A x)
+            // r = b - A x (left: r = M^{-1}(b - A x))
             if (first) {
                 HIPCHK(hipMemcpyAsync(Q, b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
             } else {
@@ -1120,13 +1132,14 @@ int Tree::gmres_dev(const double* b, double* x, int iters, double tol_, int rest
                 HIPCHK(hipMemcpyAsync(Q, b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
                 launch_axpby(n, -1.0, w, 1.0, Q, stream_);
             }
-            if (solve_dev(Q) != 0) throw std::runtime_error(last_error);
+            if (!flexible && solve_dev(Q) != 0) throw std::runtime_error(last_error);
             dot(Q, Q, hdev, 1);
             double r_norm = 0.0;
             HIPCHK(hipMemcpyAsync(&r_norm, hdev, sizeof(double), hipMemcpyDeviceToHost, stream_));
             sync_stream();
-            if (!(r_norm == r_norm)) throw std::runtime_error("GMRES: NaN preconditioned residual");
+            if (!(r_norm == r_norm)) throw std::runtime_error("GMRES: NaN residual");
             if (r_norm == 0.0) { true_rel = 0.0; break; }
+            if (ref_norm < 0) ref_norm = flexible ? bnorm : r_norm;
             launch_scal_dev(n, hdev, Q, Q, stream_);
             std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), beta(m + 1, 0.0);
             auto h = [&](int i, int j) -> double& { return H[i + (size_t)j * (m + 1)]; };
@@ -1136,8 +1149,15 @@ int Tree::gmres_dev(const double* b, double* x, int iters, double tol_, int rest
             for (; k < m; ++k) {
                 double* qk = Q + (size_t)k * n;
                 double* qn = Q + (size_t)(k + 1) * n;
-                spmv_dev(qk, qn);
-                if (solve_dev(qn) != 0) throw std::runtime_error(last_error);
+                if (flexible) {
+                    double* zk = Z + (size_t)k * n;
+                    HIPCHK(hipMemcpyAsync(zk, qk, sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
+                    if (solve_dev(zk) != 0) throw std::runtime_error(last_error);
+                    spmv_dev(zk, qn);
+                } else {
+                    spmv_dev(qk, qn);
+                    if (solve_dev(qn) != 0) throw std::runtime_error(last_error);
+                }
                 timed(P_KRYLOV, 2L * (k + 1) * 2 + 3, 1, [&] {
                     for (int i = 0; i <= k; ++i) {
                         dot(qn, Q + (size_t)i * n, hdev + i, 0);
@@ -1165,7 +1185,7 @@ int Tree::gmres_dev(const double* b, double* x, int iters, double tol_, int rest
                 h(k + 1, k) = 0.0;
                 beta[k + 1] = -sn[k] * beta[k];
                 beta[k] = cs[k] * beta[k];
-                last_err = std::fabs(beta[k + 1]) / r_norm;
+                last_err = std::fabs(beta[k + 1]) / ref_norm;
                 if (hist && nh < hist_cap) hist[nh++] = last_err;
                 if (last_err <= tol_) break;
             }
@@ -1176,7 +1196,8 @@ int Tree::gmres_dev(const double* b, double* x, int iters, double tol_, int rest
                 for (int j = i + 1; j <= k; ++j) y[i] -= h(i, j) * y[j];
                 y[i] /= h(i, i);
             }
-            for (int j = 0; j <= k; ++j) launch_axpby(n, y[j], Q + (size_t)j * n, 1.0, x, stream_);
+            const double* basis = flexible ? Z : Q;
+            for (int j = 0; j <= k; ++j) launch_axpby(n, y[j], basis + (size_t)j * n, 1.0, x, stream_);
             total += k + 1;
             first = false;
             // true residual
@@ -1192,6 +1213,7 @@ int Tree::gmres_dev(const double* b, double* x, int iters, double tol_, int rest
         }
         sync_stream();
         (void)hipFree(Q);
+        if (Z) (void)hipFree(Z);
         (void)hipFree(w);
         if (res) {
             res->iters = total - 1;

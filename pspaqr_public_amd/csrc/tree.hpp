@@ -117,7 +117,7 @@ public:
     int solve_dev(double* x);                       // preconditioner apply, in place, device vector
     int spmv_dev(const double* x, double* y);       // y = A_perm x, device vectors
     int gmres_dev(const double* b, double* x, int iters, double tol, int restart, double true_tol,
-                  GmresResult* res, double* hist, int hist_cap, int* hist_len);
+                  GmresResult* res, double* hist, int hist_cap, int* hist_len, bool flexible = false);
     int n_unknowns() const { return n; }
     const std::vector<int>& cperm() const { return cperm_; }
     int num_clusters() const { return (int)cl.size(); }

@@ -98,21 +98,25 @@ class ParTree:
         check(lib().spaqr_tree_sync(self._h), self._h, "sync")
         return self._by.download(np.float64, self._n)
 
-    def gmres(self, b: np.ndarray, iters: int = 300, tol: float = 1e-12, restart: int = 0, true_tol: float = 1e-12):
-        """dist_gmres(A, b, M, iters, tol, verb=true); returns (x, result, history)."""
+    def gmres(self, b: np.ndarray, iters: int = 300, tol: float = 1e-12, restart: int = 0, true_tol: float = 1e-12,
+              flexible: bool = False):
+        """dist_gmres(A, b, M, iters, tol, verb=true); returns (x, result, history).
+
+        flexible=True runs the right-preconditioned flexible variant (no reference counterpart)."""
         self._bx.upload(np.asarray(b, dtype=np.float64))
         res = GmresResult()
         hist = np.zeros(iters + 16)
         hl = C.c_int(0)
-        check(lib().spaqr_gmres(self._h, self._bx.ptr, self._by.ptr, iters, tol, restart, true_tol, C.byref(res),
-                                hist.ctypes.data, hist.size, C.byref(hl)), self._h, "gmres")
+        fn = lib().spaqr_fgmres if flexible else lib().spaqr_gmres
+        check(fn(self._h, self._bx.ptr, self._by.ptr, iters, tol, restart, true_tol, C.byref(res),
+                 hist.ctypes.data, hist.size, C.byref(hl)), self._h, "gmres")
         x = self._by.download(np.float64, self._n)
         return x, res, hist[: hl.value]
 
-    def gmres_device(self, b_ptr, x_ptr, iters=300, tol=1e-12, restart=0, true_tol=1e-12):
+    def gmres_device(self, b_ptr, x_ptr, iters=300, tol=1e-12, restart=0, true_tol=1e-12, flexible=False):
         res = GmresResult()
-        check(lib().spaqr_gmres(self._h, b_ptr, x_ptr, iters, tol, restart, true_tol, C.byref(res), None, 0, None),
-              self._h, "gmres")
+        fn = lib().spaqr_fgmres if flexible else lib().spaqr_gmres
+        check(fn(self._h, b_ptr, x_ptr, iters, tol, restart, true_tol, C.byref(res), None, 0, None), self._h, "gmres")
         return res
 
     # ---- introspection

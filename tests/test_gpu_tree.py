@@ -104,6 +104,30 @@ def test_fp32_fronts_reach_fp64_residual(cfg, side):
     assert res.true_res <= 1e-12
 
 
+@pytest.mark.parametrize("cfg,side", [(1, None), (3, 16)])
+def test_flexible_gmres(cfg, side):
+    """Right-preconditioned flexible GMRES (no reference counterpart): its recurrence estimates
+    the true residual, so with FP32 factors it stops at true relres <= 1e-12 within a couple of
+    steps of what the FP64 left-preconditioned path needs; with an exact factorization
+    (tol = 0) it converges in one step, as the left-preconditioned form does."""
+    P = make_problem(cfg, side)
+    T64 = build_tree(P, 64)
+    T64.factorize()
+    b = P.b[T64.cperm()]
+    _, r64, _ = T64.gmres(b, iters=300, tol=1e-12)
+    T32 = build_tree(P, 32)
+    T32.factorize()
+    x, res, _ = T32.gmres(b, iters=300, tol=1e-12, restart=50, true_tol=1e-12, flexible=True)
+    rel = np.linalg.norm(b - T32.spmv(x)) / np.linalg.norm(b)
+    assert rel <= 1e-12 and res.true_res <= 1e-12
+    assert res.iters <= r64.iters + 3, (res.iters, r64.iters)
+    Te = build_tree(P, 64, tol=0.0, skip=P.nlevels - 1)
+    Te.factorize()
+    x, res, _ = Te.gmres(b, iters=10, tol=1e-12, flexible=True)
+    assert res.iters == 0
+    assert np.linalg.norm(b - Te.spmv(x)) / np.linalg.norm(b) < 1e-13
+
+
 def test_refactorize_is_repeatable():
     P = make_problem(2, 12)
     T = build_tree(P, 64)
