@@ -46,11 +46,81 @@ __device__ __forceinline__ S householder_norm_signed(S alpha, S ss) {
     return (fabs(alpha) <= band * sqrt(ss)) ? nrm : copysign(nrm, alpha);
 }
 
+// ---- cross-lane reductions on DPP (ALU-speed lane moves) instead of LDS permutes.
+// dpp_ctrl codes: quad_perm [1,0,3,2] = 0xB1, quad_perm [2,3,0,1] = 0x4E,
+// row_half_mirror = 0x141, row_mirror = 0x140.  All lanes of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_mov(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float lane_read(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_read(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ int lane_read(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+// Sum over each row of 16 lanes; every lane of the row gets the same (bitwise) total.
+template <typename S>
+__device__ __forceinline__ S row16_sum(S v) {
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    v += dpp_mov<0x141>(v);
+    v += dpp_mov<0x140>(v);
+    return v;
+}
+template <typename S>
+__device__ __forceinline__ S row16_max(S v) {
+    v = max(v, dpp_mov<0xB1>(v));
+    v = max(v, dpp_mov<0x4E>(v));
+    v = max(v, dpp_mov<0x141>(v));
+    v = max(v, dpp_mov<0x140>(v));
+    return v;
+}
+template <typename S>
+__device__ __forceinline__ S row16_min(S v) {
+    v = min(v, dpp_mov<0xB1>(v));
+    v = min(v, dpp_mov<0x4E>(v));
+    v = min(v, dpp_mov<0x141>(v));
+    v = min(v, dpp_mov<0x140>(v));
+    return v;
+}
+// Sum over each group of 8 lanes (the same total in all 8).
+template <typename S>
+__device__ __forceinline__ S oct_sum(S v) {
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    v += dpp_mov<0x141>(v);
+    return v;
+}
+// Whole-wave reductions; the result is uniform across the wave.
 template <typename S>
 __device__ __forceinline__ S wave_sum(S v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v = row16_sum(v);
+    return (lane_read(v, 0) + lane_read(v, 16)) + (lane_read(v, 32) + lane_read(v, 48));
+}
+template <typename S>
+__device__ __forceinline__ S wave_max(S v) {
+    v = row16_max(v);
+    return max(max(lane_read(v, 0), lane_read(v, 16)), max(lane_read(v, 32), lane_read(v, 48)));
+}
+__device__ __forceinline__ int wave_min(int v) {
+    v = row16_min(v);
+    return min(min(lane_read(v, 0), lane_read(v, 16)), min(lane_read(v, 32), lane_read(v, 48)));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -72,9 +142,7 @@ __device__ __forceinline__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B
             const S* bl = B + (size_t)l * ldb;
             for (int r = r0 + g; r < m; r += 8) d += bj[r] * bl[r];
         }
-        d += __shfl_xor(d, 1, 64);
-        d += __shfl_xor(d, 2, 64);
-        d += __shfl_xor(d, 4, 64);
+        d = oct_sum(d);
         if (g == 0) sm.d[l] = d;
         __syncthreads();
         if (tid == 0) {
@@ -384,9 +452,7 @@ __device__ __forceinline__ void wg_larft_blocked(int m, int k, const S* V, int l
                 const S* vq = V + (size_t)(p0 + q) * ldv;
                 for (int r = p0 + q + 1 + g; r < m; r += 8) d += vi[r] * vq[r];
             }
-            d += __shfl_xor(d, 1, 64);
-            d += __shfl_xor(d, 2, 64);
-            d += __shfl_xor(d, 4, 64);
+            d = oct_sum(d);
             if (g == 0 && l < q) sm.t[l] = d + V[(p0 + q) + (size_t)(p0 + l) * ldv];
             __syncthreads();
             const S tq = tau[p0 + q];
@@ -1195,11 +1261,15 @@ __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __res
 
     for (int e = tid; e < k * ncol; e += WG) slice[e] = t.C[(size_t)q0 * k + e];
     __syncthreads();
-    for (int c = wv; c < ncol; c += 4) {
+    // 16 lanes (one DPP row) per column
+    const int grp = tid >> 4, gl = tid & 15;
+    for (int c0 = 0; c0 < ncol; c0 += WG / 16) {
+        const int c = c0 + grp;
         S s = S(0);
-        for (int r = lane; r < k; r += 64) { S v = slice[r + (size_t)c * k]; s += v * v; }
-        s = wave_sum(s);
-        if (lane == 0) { vn1[c] = sqrt(s); vn2[c] = vn1[c]; pos[c] = q0 + c; }
+        if (c < ncol)
+            for (int r = gl; r < k; r += 16) { S v = slice[r + (size_t)c * k]; s += v * v; }
+        s = row16_sum(s);
+        if (c < ncol && gl == 0) { vn1[c] = sqrt(s); vn2[c] = vn1[c]; pos[c] = q0 + c; }
     }
     __syncthreads();
 
@@ -1210,8 +1280,7 @@ __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __res
         S mb = S(-1);
         for (int c = tid; c < ncol; c += WG)
             if (pos[c] >= 0 && vn1[c] > mb) mb = vn1[c];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) mb = fmax(mb, __shfl_xor(mb, o, 64));
+        mb = wave_max(mb);
         if (lane == 0) red_v[wv] = mb;
         __syncthreads();
         mb = fmax(fmax(red_v[0], red_v[1]), fmax(red_v[2], red_v[3]));
@@ -1220,12 +1289,12 @@ __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __res
         int cand = INT_MAX, cc = -1;
         for (int c = tid; c < ncol; c += WG)
             if (pos[c] >= 0 && vn1[c] >= thr && pos[c] < cand) { cand = pos[c]; cc = c; }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            int oc = __shfl_xor(cand, o, 64), occ = __shfl_xor(cc, o, 64);
-            if (oc < cand) { cand = oc; cc = occ; }
+        {
+            const int wmin = wave_min(cand);
+            const int src = __ffsll((long long)__ballot(cand == wmin)) - 1;
+            const int wcc = lane_read(cc, src);
+            if (lane == 0) { red_i[wv] = wmin; sh_i[wv] = wcc; }
         }
-        if (lane == 0) { red_i[wv] = cand; sh_i[wv] = cc; }
         __syncthreads();
         if (tid == 0) {
             int b = 0;
@@ -1257,8 +1326,7 @@ __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __res
             // global maximum, band, ambiguity
             S M = S(-1);
             for (int q = tid; q < P; q += WG) M = fmax(M, ld_coherent(slot + (size_t)q * rec));
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) M = fmax(M, __shfl_xor(M, o, 64));
+            M = wave_max(M);
             if (lane == 0) red_v[wv] = M;
             __syncthreads();
             M = fmax(fmax(red_v[0], red_v[1]), fmax(red_v[2], red_v[3]));
@@ -1272,13 +1340,13 @@ __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __res
                 if (vq >= gthr && pq >= 0 && pq < best) { best = pq; bq = q; }
                 if (mq >= gthr && !(vq >= gthr)) amb = 1;
             }
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                int ob = __shfl_xor(best, o, 64), oq = __shfl_xor(bq, o, 64);
-                if (ob < best) { best = ob; bq = oq; }
-                amb |= __shfl_xor(amb, o, 64);
+            {
+                const int wb = wave_min(best);
+                const int src = __ffsll((long long)__ballot(best == wb)) - 1;
+                const int wq = lane_read(bq, src);
+                const int wamb = __any(amb) ? 1 : 0;
+                if (lane == 0) { red_i[wv] = wb; sh_i[wv] = wq; red_v[wv] = (S)wamb; }
             }
-            if (lane == 0) { red_i[wv] = best; sh_i[wv] = bq; red_v[wv] = (S)amb; }
             __syncthreads();
             if (tid == 0) {
                 int b = 0;
@@ -1296,12 +1364,12 @@ __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __res
                 int c2 = INT_MAX, cc2 = -1;
                 for (int c = tid; c < ncol; c += WG)
                     if (pos[c] >= 0 && vn1[c] >= gthr && pos[c] < c2) { c2 = pos[c]; cc2 = c; }
-#pragma unroll
-                for (int o = 32; o > 0; o >>= 1) {
-                    int oc = __shfl_xor(c2, o, 64), occ = __shfl_xor(cc2, o, 64);
-                    if (oc < c2) { c2 = oc; cc2 = occ; }
+                {
+                    const int wmin = wave_min(c2);
+                    const int src = __ffsll((long long)__ballot(c2 == wmin)) - 1;
+                    const int wcc = lane_read(cc2, src);
+                    if (lane == 0) { red_i[wv] = wmin; sh_i[wv] = wcc; }
                 }
-                if (lane == 0) { red_i[wv] = c2; sh_i[wv] = cc2; }
                 __syncthreads();
                 if (tid == 0) {
                     int b = 0;
@@ -1325,12 +1393,12 @@ __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __res
                     int pq = (int)ld_coherent(slot2 + (size_t)q * rec + 2);
                     if (pq >= 0 && pq < b2) { b2 = pq; q2 = q; }
                 }
-#pragma unroll
-                for (int o = 32; o > 0; o >>= 1) {
-                    int ob = __shfl_xor(b2, o, 64), oq = __shfl_xor(q2, o, 64);
-                    if (ob < b2) { b2 = ob; q2 = oq; }
+                {
+                    const int wb = wave_min(b2);
+                    const int src = __ffsll((long long)__ballot(b2 == wb)) - 1;
+                    const int wq = lane_read(q2, src);
+                    if (lane == 0) { red_i[wv] = wb; sh_i[wv] = wq; }
                 }
-                if (lane == 0) { red_i[wv] = b2; sh_i[wv] = q2; }
                 __syncthreads();
                 if (tid == 0) {
                     int b = 0;
@@ -1389,23 +1457,26 @@ __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __res
             if (pos[c] == j && j != pvpos) pos[c] = pvpos;
         __syncthreads();
         // ---- apply H_j to the owned unpivoted columns, downdate their norms
-        for (int c = wv; c < ncol; c += 4) {
-            if (pos[c] < 0) continue;
-            S* cl = slice + (size_t)c * k;
+        for (int c0 = 0; c0 < ncol; c0 += WG / 16) {
+            const int c = c0 + grp;
+            const bool act = (c < ncol) && pos[c] >= 0;  // uniform over the 16-lane row
+            S* cl = slice + (size_t)min(c, ncol - 1) * k;
             S dot = S(0);
-            for (int r = j + 1 + lane; r < k; r += 64) dot += vbuf[r] * cl[r];
-            dot = wave_sum(dot);
-            const S cjl = cl[j];
+            if (act)
+                for (int r = j + 1 + gl; r < k; r += 16) dot += vbuf[r] * cl[r];
+            dot = row16_sum(dot);
+            const S cjl = act ? cl[j] : S(0);
             const S wl = (cjl + dot) * tau;
             const S rjl = cjl - wl;
             S nrm = S(0);
-            for (int r = j + 1 + lane; r < k; r += 64) {
-                S v = cl[r] - vbuf[r] * wl;
-                cl[r] = v;
-                nrm += v * v;
-            }
-            nrm = wave_sum(nrm);
-            if (lane == 0) {
+            if (act)
+                for (int r = j + 1 + gl; r < k; r += 16) {
+                    S v = cl[r] - vbuf[r] * wl;
+                    cl[r] = v;
+                    nrm += v * v;
+                }
+            nrm = row16_sum(nrm);
+            if (act && gl == 0) {
                 cl[j] = rjl;
                 S v1 = vn1[c];
                 if (v1 != S(0)) {
