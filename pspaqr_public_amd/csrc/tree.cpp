@@ -147,12 +147,20 @@ double flops_geqrt(double m, double n) { return 2 * m * n * n - 2 * n * n * n / 
 void* Tree::Arena::alloc(size_t bytes) {
     bytes = (bytes + 255) & ~(size_t)255;
     if (bytes == 0) bytes = 256;
-    if (bytes > left) {
-        // reuse the next retained chunk if large enough, else allocate a new one
-        size_t want = std::max(bytes, (size_t)256 << 20);
+    while (bytes > left) {
+        // move on to the next retained chunk if it is large enough, else allocate a new one
+        if (next < chunks.size() && chunks[next].second >= bytes) {
+            cur = (char*)chunks[next].first;
+            left = chunks[next].second;
+            ++next;
+            break;
+        }
+        if (next < chunks.size()) { ++next; continue; }  // too small for this request: skip it
+        const size_t want = std::max(bytes, (size_t)256 << 20);
         void* p = nullptr;
         HIPCHK(hipMalloc(&p, want));
-        chunks.push_back(p);
+        chunks.push_back({p, want});
+        next = chunks.size();
         cur = (char*)p;
         left = want;
     }
@@ -163,12 +171,17 @@ void* Tree::Arena::alloc(size_t bytes) {
     return r;
 }
 
-void Tree::Arena::release() {
-    for (void* p : chunks) (void)hipFree(p);
-    chunks.clear();
+void Tree::Arena::reset() {
+    next = 0;
     cur = nullptr;
     left = 0;
     total = 0;
+}
+
+void Tree::Arena::release() {
+    for (auto& c : chunks) (void)hipFree(c.first);
+    chunks.clear();
+    reset();
 }
 
 void* Tree::upload(const void* h, size_t bytes) {
@@ -575,8 +588,9 @@ int Tree::factorize_t() {
     cl = cl0;
     ed = ed0;
     current_bottom = 0;
-    arena.release();
-    desc_arena.release();
+    sync_stream();  // earlier work on the retained chunks has finished before they are reused
+    arena.reset();
+    desc_arena.reset();
     stats.clear();
     delete sched;
     sched = nullptr;

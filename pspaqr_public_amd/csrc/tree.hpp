@@ -150,11 +150,15 @@ private:
     hipStream_t stream_ = nullptr;
 
     // device memory
+    // Bump allocator over retained 256 MiB (or larger) hipMalloc chunks: reset() rewinds it
+    // for the next factorization without returning memory, release() frees the chunks.
     struct Arena {
-        std::vector<void*> chunks;
+        std::vector<std::pair<void*, size_t>> chunks;
+        size_t next = 0;
         char* cur = nullptr;
         size_t left = 0, total = 0;
         void* alloc(size_t bytes);
+        void reset();
         void release();
     } arena, desc_arena;
     std::vector<double> h_blocks0;  // assembled level-0 blocks (FP64, host)
