@@ -3,7 +3,8 @@
 
 One step = factorize the assembled matrix on the GPU + FP64 GMRES to ||b - A x|| / ||b|| <=
 1e-12, with the matrix blocks and b already resident in HBM.  The timed path is FP32 fronts
-(batched, one grouped launch per level phase) inside FP64 flexible GMRES(50); the plain
+(batched, one grouped launch per level phase) with learned sparsification ranks (a sketch sized
+by the rank predictor, tail-tested, geqp3 fallback) inside FP64 flexible GMRES(50); the plain
 batched FP64 path (the reference's left-preconditioned GMRES) is timed on the same GPU in the
 same run and reported beside it, as is the oracle (the CPU restatement of the reference path)
 on a bounded sample.
@@ -28,8 +29,9 @@ sys.path.insert(0, ROOT)
 METRIC = "time to rel. residual 1e-12 (factor+GMRES), FP32/batched/learned vs FP64 plain"
 PEAK = {"hbm": 8000.0, "fp64": 78.6, "fp32": 157.3}  # GB/s, TFLOP/s (MI355X dense)
 CLASSES = ["geqrt", "larfb", "tpqrt_chain", "tpmqrt_chain", "scale_geqrt", "scale_apply", "rrqr_gather", "geqp3",
-           "sparsify_apply_merge", "fill", "precond_solve", "spmv", "gmres_orth"]
-MFMA_CLASSES = {"geqrt", "larfb", "tpqrt_chain", "tpmqrt_chain", "scale_geqrt", "scale_apply", "sparsify_apply_merge"}
+           "sparsify_apply_merge", "fill", "precond_solve", "spmv", "gmres_orth", "rank_sketch"]
+MFMA_CLASSES = {"geqrt", "larfb", "tpqrt_chain", "tpmqrt_chain", "scale_geqrt", "scale_apply", "sparsify_apply_merge",
+                "rank_sketch"}
 
 
 def log(*a):
@@ -65,6 +67,14 @@ def run_steps(tree, b_dev, x_dev, n, steps, restart, iters, tag, flexible=False)
     return (t1 - t0) / max(steps, 1), fact_s / max(steps, 1), res
 
 
+def set_rank_mode(tree, mode):
+    if mode:
+        tree.set_rank_mode(mode)
+        if mode == 1:
+            from pspaqr_public_amd.rank_model import load_weights
+            tree.set_rank_model(load_weights())
+
+
 def cpu_baseline(cfg, side):
     """Oracle (CPU FP64 restatement of the reference path, one thread) on a bounded sample."""
     from oracle.oracle import OracleTree
@@ -95,6 +105,9 @@ def main():
     ap.add_argument("--cpu-side", type=int, default=32, help="grid side of the CPU-baseline sample")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--rank-mode", type=int, default=1,
+                    help="FP32 path's sparsification ranks: 1 learned sketch (default), 2 sketch without the "
+                         "predictor, 0 geqp3 as the reference")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this path (one rank's whole share of the tree) runs on a single "
@@ -110,8 +123,9 @@ def main():
     restart = P.meta.get("restart", 50) or 50
     log("[bench] %s n=%d nnz=%d levels=%d tol=%g" % (P.name, P.A.shape[0], P.A.nnz, P.nlevels, P.tol))
 
-    def prepare(prec):
+    def prepare(prec, rank_mode=0):
         t = build_tree(P, prec)
+        set_rank_mode(t, rank_mode)
         b = P.b[t.cperm()]
         bd = DeviceBuffer.from_array(b)
         xd = DeviceBuffer(8 * t.n)
@@ -119,13 +133,15 @@ def main():
 
     # ---- FP32 fronts (the timed path)
     t0 = time.perf_counter()
-    T32, b, b32, x32 = prepare(32)
+    T32, b, b32, x32 = prepare(32, args.rank_mode)
     log("[bench] partition+assemble %.2f s" % (time.perf_counter() - t0))
     lib().spaqr_tree_set_profiling(T32.handle, 1)
     if args.warmup:
         run_steps(T32, b32, x32, T32.n, args.warmup, restart, args.iters, "fp32 warmup", True)
     lib().spaqr_tree_profile_reset(T32.handle)
+    T32.rank_stats_reset()
     step_s, fact_s, res32 = run_steps(T32, b32, x32, T32.n, args.steps, restart, args.iters, "fp32", True)
+    rstats = T32.rank_stats()
     prof = profile(T32)
     stats = T32.level_stats()
     x = x32.download(np.float64, T32.n)
@@ -161,6 +177,7 @@ def main():
     if not args.skip_cpu:
         Ps, cpu_s, cit, cres = cpu_baseline(args.config, args.cpu_side)
         Ts = build_tree(Ps, 32)
+        set_rank_mode(Ts, args.rank_mode)
         bs = Ps.b[Ts.cperm()]
         bsd, xsd = DeviceBuffer.from_array(bs), DeviceBuffer(8 * Ts.n)
         run_steps(Ts, bsd, xsd, Ts.n, 1, restart, args.iters, "fp32 sample warmup", True)
@@ -190,7 +207,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "%s: 3D 7-point upwind convection-diffusion %d^3 (n=%d), tensor-grid ND -n %d -d 3, lvl %d, "
-                        "tol %g; FP32 fronts in grouped batched launches, FP64 flexible GMRES(%d) to true relres 1e-12"
+                        "tol %g; FP32 fronts in grouped batched launches, learned sketch ranks, FP64 flexible GMRES(%d) to "
+                        "true relres 1e-12"
                         % (P.name, P.n_side, P.A.shape[0], P.n_side, P.nlevels, P.tol, restart)
                         if args.config == 3 else "%s (n=%d, lvl %d, tol %g)" % (P.name, P.A.shape[0], P.nlevels, P.tol),
             "n": int(P.A.shape[0]),
@@ -205,6 +223,8 @@ def main():
         "fp64_plain": fp64,
         "speedup_vs_fp64": (fp64["seconds"] / step_s) if fp64 else None,
         "factor_gflop": round(float(stats[:, 7].sum()) / 1e9, 3),
+        "rank_mode": args.rank_mode,
+        "rank_stats_per_step": {k: (v / max(args.steps, 1)) for k, v in rstats.items()},
         "roofline": roof,
         "cpu_baseline": cpu,
         "per_class": per_class,

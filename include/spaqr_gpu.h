@@ -108,11 +108,47 @@ int spaqr_tree_sync(spaqr_tree* t);
    Classes, in order: 0 geqrt (interior diagonal QR), 1 larfb (Q^T on interior rows),
    2 tpqrt chains, 3 tpmqrt chains (trailing updates), 4 scaling QR, 5 scaling apply,
    6 RRQR gather, 7 geqp3, 8 sparsify apply + merge, 9 fills, 10 preconditioner solve,
-   11 SpMV, 12 GMRES orthogonalisation.  Per class 5 doubles: milliseconds, launches,
+   11 SpMV, 12 GMRES orthogonalisation, 13 sketch / features / predictor / tail test of the
+   rank path.  Per class 5 doubles: milliseconds, launches,
    tasks, algorithmic flops, algorithmic bytes.  Timings are folded in at each sync. */
 int spaqr_tree_set_profiling(spaqr_tree* t, int on);
 int spaqr_tree_profile(spaqr_tree* t, double* out, int max_classes);
 int spaqr_tree_profile_reset(spaqr_tree* t);
+
+/* ------------------------------------------------------------------ rank path
+   The sparsification rank of each interface (src/ptree.cpp:373-439).  Mode 0 (default) is the
+   reference's geqp3 + choose_rank on the gathered C.  Mode 1 replaces it by a pivoted QR of a
+   Gaussian sketch C*Omega whose width comes from a learned rank predictor (MLP 32 -> 64 -> 64
+   -> 1); the chosen rank is the smallest whose tail passes the test geqp3 stops on
+   (every column of (Q^T C)[r:, :] has norm <= tol * max column norm of C), and an interface
+   whose sketch fails falls back to geqp3 on C.  Mode 2 is mode 1 with the sketch as wide as
+   min(k, w) (no predictor).  The closest reference analogue is the unused adaptive range
+   finder random_rrqr (src/util.cpp:643-715); none of these calls has a reference counterpart. */
+int spaqr_tree_set_rank_mode(spaqr_tree* t, int mode);
+/* predictor weights (host floats, spaqr_mlp_params() of them):
+   W1 (64 x 32, row-major) b1 (64) W2 (64 x 64) b2 (64) w3 (64) b3 */
+int spaqr_tree_set_rank_model(spaqr_tree* t, const float* w, int nparams);
+/* record (32 features, rank) of every sparsified interface wide enough to sketch */
+int spaqr_tree_set_rank_log(spaqr_tree* t, int on);
+/* copies up to cap entries: x (32 floats each), meta (5 ints each: rank, k, w, level, mode
+   used); returns the number of entries recorded */
+int spaqr_tree_rank_log(const spaqr_tree* t, float* x, int* meta, int cap);
+int spaqr_tree_rank_log_clear(spaqr_tree* t);
+/* 6 doubles: interfaces, sketched, fell back to geqp3, direct geqp3, sum of sketch widths,
+   sum of min(k, w) over the sketched interfaces (since the last reset) */
+int spaqr_tree_rank_stats(const spaqr_tree* t, double* out);
+int spaqr_tree_rank_stats_reset(spaqr_tree* t);
+
+/* The predictor on its own (host buffers; the GPU kernels run underneath and synchronize).
+   Features X are n x 32 row-major; targets y are rank fractions rank / min(k, w) in [0, 1].
+   predict: out = sigmoid(MLP(X)).  grad: full-batch gradient of the mean logistic loss
+   mean_i softplus(o_i) - y_i o_i.  train: `epochs` full-batch Adam steps (lr) from W, in
+   place; loss_hist (optional, epochs floats) gets the loss before each step. */
+int spaqr_mlp_params(void);
+int spaqr_mlp_features(void);
+int spaqr_mlp_predict(const float* X, int n, const float* W, float* out);
+int spaqr_mlp_grad(const float* X, const float* y, int n, const float* W, float* grad, float* loss);
+int spaqr_mlp_train(const float* X, const float* y, int n, float* W, int epochs, float lr, float* loss_hist);
 
 /* ------------------------------------------------------------------ device memory helpers
    For callers without a GPU runtime of their own (cgo, JNI, ctypes).  No reference

@@ -90,6 +90,18 @@ EXPORTS = {
     "spaqr_tree_set_profiling": (_i, [_vp, _i]),
     "spaqr_tree_profile": (_i, [_vp, _vp, _i]),
     "spaqr_tree_profile_reset": (_i, [_vp]),
+    "spaqr_tree_set_rank_mode": (_i, [_vp, _i]),
+    "spaqr_tree_set_rank_model": (_i, [_vp, _vp, _i]),
+    "spaqr_tree_set_rank_log": (_i, [_vp, _i]),
+    "spaqr_tree_rank_log": (_i, [_vp, _vp, _vp, _i]),
+    "spaqr_tree_rank_log_clear": (_i, [_vp]),
+    "spaqr_tree_rank_stats": (_i, [_vp, _vp]),
+    "spaqr_tree_rank_stats_reset": (_i, [_vp]),
+    "spaqr_mlp_params": (_i, []),
+    "spaqr_mlp_features": (_i, []),
+    "spaqr_mlp_predict": (_i, [_vp, _i, _vp, _vp]),
+    "spaqr_mlp_grad": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "spaqr_mlp_train": (_i, [_vp, _vp, _i, _vp, _i, C.c_float, _vp]),
     "spaqr_device_alloc": (_vp, [C.c_size_t]),
     "spaqr_device_free": (_i, [_vp]),
     "spaqr_memcpy_h2d": (_i, [_vp, _vp, C.c_size_t]),

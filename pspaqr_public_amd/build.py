@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libspaqr_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SPAQR_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "rrqr.cpp", "tree.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "learned.hip", "rrqr.cpp", "tree.cpp", "tree_rank.cpp", "capi.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,

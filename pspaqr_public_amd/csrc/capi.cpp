@@ -8,6 +8,7 @@
 #include "../../include/spaqr_gpu.h"
 #include "kernels.hpp"
 #include "launch.hpp"
+#include "learned.hpp"
 #include "rrqr.hpp"
 #include "tree.hpp"
 
@@ -212,6 +213,99 @@ int spaqr_tree_profile_reset(spaqr_tree* h) {
     int rc = h->t->sync();
     h->t->prof_reset();
     return rc;
+}
+
+// ---------------------------------------------------------------- rank path
+int spaqr_tree_set_rank_mode(spaqr_tree* h, int mode) {
+    TREE_OR_FAIL(h);
+    if (mode < 0 || mode > 2) { g_err = "rank mode must be 0, 1 or 2"; return -2; }
+    h->t->rank_mode = mode;
+    return 0;
+}
+int spaqr_tree_set_rank_model(spaqr_tree* h, const float* w, int nparams) {
+    TREE_OR_FAIL(h);
+    if (!w) return -2;
+    if (nparams != kMlpParams) { g_err = "rank model: wrong parameter count"; return -3; }
+    try {
+        return h->t->set_rank_model(w);
+    } catch (std::exception& e) {
+        h->t->last_error = e.what();
+        return -100;
+    }
+}
+int spaqr_tree_set_rank_log(spaqr_tree* h, int on) {
+    TREE_OR_FAIL(h);
+    h->t->rank_log = on != 0;
+    return 0;
+}
+int spaqr_tree_rank_log(const spaqr_tree* h, float* x, int* meta, int cap) {
+    if (!h || !h->t) return -1;
+    const auto& L = h->t->rlog;
+    const int n = (int)L.rank.size();
+    const int m = std::min(n, std::max(cap, 0));
+    for (int i = 0; i < m; ++i) {
+        if (x) std::memcpy(x + (size_t)i * kRankFeatures, L.x.data() + (size_t)i * kRankFeatures, sizeof(float) * kRankFeatures);
+        if (meta) {
+            meta[5 * i] = L.rank[i];
+            meta[5 * i + 1] = L.k[i];
+            meta[5 * i + 2] = L.w[i];
+            meta[5 * i + 3] = L.level[i];
+            meta[5 * i + 4] = L.mode[i];
+        }
+    }
+    return n;
+}
+int spaqr_tree_rank_log_clear(spaqr_tree* h) {
+    TREE_OR_FAIL(h);
+    h->t->rlog = Tree::RankLog();
+    return 0;
+}
+int spaqr_tree_rank_stats(const spaqr_tree* h, double* out) {
+    if (!h || !h->t) return -1;
+    if (!out) return -2;
+    const auto& r = h->t->rstats;
+    out[0] = (double)r.tasks;
+    out[1] = (double)r.sketched;
+    out[2] = (double)r.fallback;
+    out[3] = (double)r.direct;
+    out[4] = r.sketch_cols;
+    out[5] = r.full_cols;
+    return 0;
+}
+int spaqr_tree_rank_stats_reset(spaqr_tree* h) {
+    TREE_OR_FAIL(h);
+    h->t->rstats = Tree::RankStats();
+    return 0;
+}
+int spaqr_mlp_params(void) { return kMlpParams; }
+int spaqr_mlp_features(void) { return kRankFeatures; }
+int spaqr_mlp_predict(const float* X, int n, const float* W, float* out) {
+    if (n < 0 || (n > 0 && (!X || !out)) || !W) return -2;
+    if (n == 0) return 0;
+    try {
+        return mlp_predict_host(X, n, W, out);
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -100;
+    }
+}
+int spaqr_mlp_grad(const float* X, const float* y, int n, const float* W, float* grad, float* loss) {
+    if (n <= 0 || !X || !y || !W || !grad || !loss) return -2;
+    try {
+        return mlp_grad_host(X, y, n, W, grad, loss);
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -100;
+    }
+}
+int spaqr_mlp_train(const float* X, const float* y, int n, float* W, int epochs, float lr, float* loss_hist) {
+    if (n <= 0 || !X || !y || !W || epochs < 0 || !(lr > 0.f)) return -2;
+    try {
+        return mlp_train_host(X, y, n, W, epochs, lr, loss_hist);
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -100;
+    }
 }
 
 void* spaqr_device_alloc(size_t bytes) {

@@ -1,0 +1,107 @@
+// Cross-lane reductions for 64-wide gfx950 wavefronts built on DPP lane moves (ALU-speed)
+// and readlane, instead of LDS permutes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace spaqr {
+
+// ---- cross-lane reductions on DPP (ALU-speed lane moves) instead of LDS permutes.
+// dpp_ctrl codes: quad_perm [1,0,3,2] = 0xB1, quad_perm [2,3,0,1] = 0x4E,
+// row_half_mirror = 0x141, row_mirror = 0x140.  All lanes of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_mov(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float lane_read(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_read(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ int lane_read(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+// Sum over each row of 16 lanes; every lane of the row gets the same (bitwise) total.
+template <typename S>
+__device__ __forceinline__ S row16_sum(S v) {
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    v += dpp_mov<0x141>(v);
+    v += dpp_mov<0x140>(v);
+    return v;
+}
+template <typename S>
+__device__ __forceinline__ S row16_max(S v) {
+    v = max(v, dpp_mov<0xB1>(v));
+    v = max(v, dpp_mov<0x4E>(v));
+    v = max(v, dpp_mov<0x141>(v));
+    v = max(v, dpp_mov<0x140>(v));
+    return v;
+}
+template <typename S>
+__device__ __forceinline__ S row16_min(S v) {
+    v = min(v, dpp_mov<0xB1>(v));
+    v = min(v, dpp_mov<0x4E>(v));
+    v = min(v, dpp_mov<0x141>(v));
+    v = min(v, dpp_mov<0x140>(v));
+    return v;
+}
+// Sum over each group of 8 lanes (the same total in all 8).
+template <typename S>
+__device__ __forceinline__ S oct_sum(S v) {
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    v += dpp_mov<0x141>(v);
+    return v;
+}
+// Whole-wave reductions; the result is uniform across the wave.
+template <typename S>
+__device__ __forceinline__ S wave_sum(S v) {
+    v = row16_sum(v);
+    return (lane_read(v, 0) + lane_read(v, 16)) + (lane_read(v, 32) + lane_read(v, 48));
+}
+template <typename S>
+__device__ __forceinline__ S wave_max(S v) {
+    v = row16_max(v);
+    return max(max(lane_read(v, 0), lane_read(v, 16)), max(lane_read(v, 32), lane_read(v, 48)));
+}
+__device__ __forceinline__ int wave_min(int v) {
+    v = row16_min(v);
+    return min(min(lane_read(v, 0), lane_read(v, 16)), min(lane_read(v, 32), lane_read(v, 48)));
+}
+
+// Row-of-16 shift right by N lanes with zero fill (inclusive-scan building block).
+template <int N>
+__device__ __forceinline__ float dpp_shr(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x110 + N, 0xF, 0xF, true));
+}
+template <int N>
+__device__ __forceinline__ double dpp_shr(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x110 + N, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x110 + N, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// Inclusive prefix sum within each row of 16 lanes.
+template <typename S>
+__device__ __forceinline__ S row16_scan(S v) {
+    v += dpp_shr<1>(v);
+    v += dpp_shr<2>(v);
+    v += dpp_shr<4>(v);
+    v += dpp_shr<8>(v);
+    return v;
+}
+
+}  // namespace spaqr

@@ -16,6 +16,7 @@
 #include <climits>
 #include <cstdint>
 
+#include "dpp.hpp"
 #include "kernels.hpp"
 
 namespace spaqr {
@@ -46,82 +47,6 @@ __device__ __forceinline__ S householder_norm_signed(S alpha, S ss) {
     return (fabs(alpha) <= band * sqrt(ss)) ? nrm : copysign(nrm, alpha);
 }
 
-// ---- cross-lane reductions on DPP (ALU-speed lane moves) instead of LDS permutes.
-// dpp_ctrl codes: quad_perm [1,0,3,2] = 0xB1, quad_perm [2,3,0,1] = 0x4E,
-// row_half_mirror = 0x141, row_mirror = 0x140.  All lanes of the wave must be active.
-template <int CTRL>
-__device__ __forceinline__ float dpp_mov(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_mov(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-template <int CTRL>
-__device__ __forceinline__ int dpp_mov(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
-}
-__device__ __forceinline__ float lane_read(float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-__device__ __forceinline__ double lane_read(double v, int l) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-__device__ __forceinline__ int lane_read(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
-
-// Sum over each row of 16 lanes; every lane of the row gets the same (bitwise) total.
-template <typename S>
-__device__ __forceinline__ S row16_sum(S v) {
-    v += dpp_mov<0xB1>(v);
-    v += dpp_mov<0x4E>(v);
-    v += dpp_mov<0x141>(v);
-    v += dpp_mov<0x140>(v);
-    return v;
-}
-template <typename S>
-__device__ __forceinline__ S row16_max(S v) {
-    v = max(v, dpp_mov<0xB1>(v));
-    v = max(v, dpp_mov<0x4E>(v));
-    v = max(v, dpp_mov<0x141>(v));
-    v = max(v, dpp_mov<0x140>(v));
-    return v;
-}
-template <typename S>
-__device__ __forceinline__ S row16_min(S v) {
-    v = min(v, dpp_mov<0xB1>(v));
-    v = min(v, dpp_mov<0x4E>(v));
-    v = min(v, dpp_mov<0x141>(v));
-    v = min(v, dpp_mov<0x140>(v));
-    return v;
-}
-// Sum over each group of 8 lanes (the same total in all 8).
-template <typename S>
-__device__ __forceinline__ S oct_sum(S v) {
-    v += dpp_mov<0xB1>(v);
-    v += dpp_mov<0x4E>(v);
-    v += dpp_mov<0x141>(v);
-    return v;
-}
-// Whole-wave reductions; the result is uniform across the wave.
-template <typename S>
-__device__ __forceinline__ S wave_sum(S v) {
-    v = row16_sum(v);
-    return (lane_read(v, 0) + lane_read(v, 16)) + (lane_read(v, 32) + lane_read(v, 48));
-}
-template <typename S>
-__device__ __forceinline__ S wave_max(S v) {
-    v = row16_max(v);
-    return max(max(lane_read(v, 0), lane_read(v, 16)), max(lane_read(v, 32), lane_read(v, 48)));
-}
-__device__ __forceinline__ int wave_min(int v) {
-    v = row16_min(v);
-    return min(min(lane_read(v, 0), lane_read(v, 16)), min(lane_read(v, 32), lane_read(v, 48)));
-}
 
 // ---------------------------------------------------------------------------------------
 // Panel QR.  PENT = false: the panel is A (m x nbp, ld ldr == ldb, Rt == B): reflector j
@@ -202,11 +127,14 @@ __device__ __forceinline__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B
 template <typename S, bool PENT>
 __device__ __forceinline__ void wg_apply_lt(int m, int kb, const S* __restrict__ V, int ldv, const S* __restrict__ T, int ldt,
                             S* Atop, int lda, S* C, int ldc, int ncols, ApplySmem<S>& sm) {
+    // Loads are unconditional (indices clamped into the operands, values masked after), so a
+    // thread's loads issue back to back instead of one memory latency each.
     const int tid = threadIdx.x;
+    if (m <= 0 || kb <= 0) return;
     __syncthreads();
     for (int e = tid; e < 32 * 32; e += WG) {
         int i = e & 31, j = e >> 5;
-        sm.Ts[j * 33 + i] = (i <= j && j < kb) ? T[i + (size_t)j * ldt] : S(0);
+        sm.Ts[j * 33 + i] = (i <= j && j < kb) ? T[i + (size_t)min(j, kb - 1) * ldt] : S(0);
     }
     const int wi = tid & 31, wq = tid >> 5;
     const int rr = tid & 63, cg = tid >> 6;
@@ -216,24 +144,31 @@ __device__ __forceinline__ void wg_apply_lt(int m, int kb, const S* __restrict__
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             int j = wq * 4 + q;
-            acc[q] = (PENT && wi < kb && j < nc) ? Atop[wi + (size_t)(c0 + j) * lda] : S(0);
+            acc[q] = S(0);
+            if (PENT) {
+                const S a = Atop[min(wi, kb - 1) + (size_t)(c0 + min(j, nc - 1)) * lda];
+                acc[q] = (wi < kb && j < nc) ? a : S(0);
+            }
         }
         for (int r0 = 0; r0 < m; r0 += 64) {
             const int rb = min(64, m - r0);
             __syncthreads();
-            for (int e = tid; e < 64 * 32; e += WG) {
-                int r = e & 63, i = e >> 6;
-                S v = S(0), cv = S(0);
-                if (r < rb) {
-                    int R = r0 + r;
-                    if (i < kb) {
-                        if (PENT) v = V[R + (size_t)i * ldv];
-                        else v = (R > i) ? V[R + (size_t)i * ldv] : (R == i ? S(1) : S(0));
-                    }
-                    if (i < nc) cv = C[R + (size_t)(c0 + i) * ldc];
-                }
+            S vv[8], cv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int e = tid + q * WG, r = e & 63, i = e >> 6;
+                const int R = r0 + min(r, rb - 1);
+                vv[q] = V[R + (size_t)min(i, kb - 1) * ldv];
+                cv[q] = C[R + (size_t)(c0 + min(i, nc - 1)) * ldc];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int e = tid + q * WG, r = e & 63, i = e >> 6;
+                const int R = r0 + r;
+                S v = S(0);
+                if (r < rb && i < kb) v = PENT ? vv[q] : ((R > i) ? vv[q] : (R == i ? S(1) : S(0)));
                 sm.Vs[i * 65 + r] = v;
-                sm.Cs[i * 65 + r] = cv;
+                sm.Cs[i * 65 + r] = (r < rb && i < nc) ? cv[q] : S(0);
             }
             __syncthreads();
             for (int r = 0; r < rb; ++r) {
@@ -268,23 +203,20 @@ __device__ __forceinline__ void wg_apply_lt(int m, int kb, const S* __restrict__
             if (R < m) {
                 S vrow[32];
 #pragma unroll
-                for (int i = 0; i < 32; ++i) {
-                    S v = S(0);
-                    if (i < kb) {
-                        if (PENT) v = V[R + (size_t)i * ldv];
-                        else v = (R > i) ? V[R + (size_t)i * ldv] : (R == i ? S(1) : S(0));
-                    }
-                    vrow[i] = v;
-                }
+                for (int i = 0; i < 32; ++i) vrow[i] = V[R + (size_t)min(i, kb - 1) * ldv];
+#pragma unroll
+                for (int i = 0; i < 32; ++i)
+                    vrow[i] = (i < kb) ? (PENT ? vrow[i] : ((R > i) ? vrow[i] : (R == i ? S(1) : S(0)))) : S(0);
+                S cold[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) cold[q] = C[R + (size_t)(c0 + min(cg * 8 + q, nc - 1)) * ldc];
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     int j = cg * 8 + q;
-                    if (j < nc) {
-                        S s = S(0);
+                    S s = S(0);
 #pragma unroll
-                        for (int i = 0; i < 32; ++i) s += vrow[i] * sm.W[j * 33 + i];
-                        C[R + (size_t)(c0 + j) * ldc] -= s;
-                    }
+                    for (int i = 0; i < 32; ++i) s += vrow[i] * sm.W[j * 33 + i];
+                    if (j < nc) C[R + (size_t)(c0 + j) * ldc] = cold[q] - s;
                 }
             }
         }
@@ -297,37 +229,51 @@ __device__ __forceinline__ void wg_apply_lt(int m, int kb, const S* __restrict__
 template <typename S>
 __device__ __forceinline__ void wg_apply_rn(int m, int n, int kb, const S* __restrict__ V, int ldv, const S* __restrict__ T, int ldt,
                             S* C, int ldc, ApplySmem<S>& sm) {
+    // Two sweeps over 64-column chunks of V staged in LDS (unit lower-trapezoidal, n x kb):
+    // W = C V for this 64-row tile, W <- W T, then C -= W V^T.  All global loads are
+    // unconditional (clamped, masked), issued in groups of 16 per thread.
     const int tid = threadIdx.x;
+    if (m <= 0 || n <= 0 || kb <= 0) return;
     __syncthreads();
     for (int e = tid; e < 32 * 32; e += WG) {
         int i = e & 31, j = e >> 5;
-        sm.Ts[j * 33 + i] = (i <= j && j < kb) ? T[i + (size_t)j * ldt] : S(0);
+        sm.Ts[j * 33 + i] = (i <= j && j < kb) ? T[i + (size_t)min(j, kb - 1) * ldt] : S(0);
     }
     const int rr = tid & 63, cg = tid >> 6;  // row in a 64-row tile, group of 8 reflectors
+    auto stage_v = [&](int l0, int lb) {
+        S vv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + q * WG, l = e & 63, i = e >> 6;
+            vv[q] = V[(l0 + min(l, lb - 1)) + (size_t)min(i, kb - 1) * ldv];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + q * WG, l = e & 63, i = e >> 6, L = l0 + l;
+            S v = S(0);
+            if (l < lb && i < kb) v = (L > i) ? vv[q] : (L == i ? S(1) : S(0));
+            sm.Vs[i * 65 + l] = v;
+        }
+    };
     for (int r0 = 0; r0 < m; r0 += 64) {
-        const int R = r0 + rr;
+        const int R = r0 + rr, Rc = min(R, m - 1);
         S acc[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) acc[q] = S(0);
-        // W = C V  (staging V chunks of 64 rows: Vs[i * 65 + l])
         for (int l0 = 0; l0 < n; l0 += 64) {
             const int lb = min(64, n - l0);
             __syncthreads();
-            for (int e = tid; e < 64 * 32; e += WG) {
-                int l = e & 63, i = e >> 6;
-                S v = S(0);
-                if (l < lb && i < kb) {
-                    int L = l0 + l;
-                    v = (L > i) ? V[L + (size_t)i * ldv] : (L == i ? S(1) : S(0));
-                }
-                sm.Vs[i * 65 + l] = v;
-            }
+            stage_v(l0, lb);
             __syncthreads();
-            if (R < m) {
-                for (int l = 0; l < lb; ++l) {
-                    S c = C[R + (size_t)(l0 + l) * ldc];
+            for (int lq = 0; lq < lb; lq += 8) {
+                S c[8];
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) acc[q] += c * sm.Vs[(cg * 8 + q) * 65 + l];
+                for (int u = 0; u < 8; ++u) c[u] = C[Rc + (size_t)(l0 + min(lq + u, lb - 1)) * ldc];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const S cu = (lq + u < lb) ? c[u] : S(0);
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) acc[q] += cu * sm.Vs[(cg * 8 + q) * 65 + min(lq + u, 63)];
                 }
             }
         }
@@ -347,21 +293,27 @@ __device__ __forceinline__ void wg_apply_rn(int m, int n, int kb, const S* __res
 #pragma unroll
         for (int q = 0; q < 8; ++q) sm.Cs[(cg * 8 + q) * 65 + rr] = acc[q];
         __syncthreads();
-        // C -= W V^T : thread (rr, cg) covers columns l = cg, cg + 4, ...
-        if (R < m) {
-            S w[32];
+        // C -= W V^T over the same chunks: thread (rr, cg) covers chunk columns cg, cg + 4, ...
+        S w[32];
 #pragma unroll
-            for (int i = 0; i < 32; ++i) w[i] = sm.Cs[i * 65 + rr];
-            for (int l = cg; l < n; l += 4) {
-                S s = S(0);
+        for (int i = 0; i < 32; ++i) w[i] = sm.Cs[i * 65 + rr];
+        for (int l0 = 0; l0 < n; l0 += 64) {
+            const int lb = min(64, n - l0);
+            __syncthreads();
+            stage_v(l0, lb);
+            __syncthreads();
+            for (int uh = 0; uh < 16; uh += 8) {
+                S cold[8];
 #pragma unroll
-                for (int i = 0; i < 32; ++i) {
-                    if (i < kb) {
-                        S v = (l > i) ? V[l + (size_t)i * ldv] : (l == i ? S(1) : S(0));
-                        s += w[i] * v;
-                    }
+                for (int u = 0; u < 8; ++u) cold[u] = C[Rc + (size_t)(l0 + min(cg + 4 * (uh + u), lb - 1)) * ldc];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int l = cg + 4 * (uh + u);
+                    S sacc = S(0);
+#pragma unroll
+                    for (int i = 0; i < 32; ++i) sacc += w[i] * sm.Vs[i * 65 + l];
+                    if (R < m && l < lb) C[R + (size_t)(l0 + l) * ldc] = cold[u] - sacc;
                 }
-                C[R + (size_t)l * ldc] -= s;
             }
         }
     }
@@ -387,13 +339,18 @@ __device__ __forceinline__ void wg_trsm_ru(int m, int n, const S* __restrict__ R
                     __syncthreads();
                     for (int e = tid; e < 32 * 32; e += WG) {
                         int l = e & 31, q = e >> 5;
-                        sm.W[q * 33 + l] = (l0 + l < j0 && q < jb) ? R[(l0 + l) + (size_t)(j0 + q) * ldr] : S(0);
+                        const S rv = R[min(l0 + l, j0 - 1) + (size_t)(j0 + min(q, jb - 1)) * ldr];
+                        sm.W[q * 33 + l] = (l0 + l < j0 && q < jb) ? rv : S(0);
                     }
                     __syncthreads();
                     if (Rw < m) {
                         const int lb = min(32, j0 - l0);
-                        for (int l = 0; l < lb; ++l) {
-                            S b = B[Rw + (size_t)(l0 + l) * ldb];
+                        S bv[32];
+#pragma unroll
+                        for (int l = 0; l < 32; ++l) bv[l] = B[Rw + (size_t)(l0 + min(l, lb - 1)) * ldb];
+#pragma unroll
+                        for (int l = 0; l < 32; ++l) {
+                            const S b = (l < lb) ? bv[l] : S(0);
 #pragma unroll
                             for (int q = 0; q < 8; ++q) acc[q] += b * sm.W[(cg * 8 + q) * 33 + l];
                         }
@@ -409,13 +366,16 @@ __device__ __forceinline__ void wg_trsm_ru(int m, int n, const S* __restrict__ R
         __syncthreads();
         for (int e = tid; e < 32 * 32; e += WG) {
             int i = e & 31, q = e >> 5;
-            sm.Ts[q * 33 + i] = (i < jb && q < jb) ? R[(j0 + i) + (size_t)(j0 + q) * ldr] : S(0);
+            const S rv = R[(j0 + min(i, jb - 1)) + (size_t)(j0 + min(q, jb - 1)) * ldr];
+            sm.Ts[q * 33 + i] = (i < jb && q < jb) ? rv : S(0);
         }
         __syncthreads();
         for (int r = tid; r < m; r += WG) {
             S x[32];
 #pragma unroll
-            for (int q = 0; q < 32; ++q) x[q] = (q < jb) ? B[r + (size_t)(j0 + q) * ldb] : S(0);
+            for (int q = 0; q < 32; ++q) x[q] = B[r + (size_t)(j0 + min(q, jb - 1)) * ldb];
+#pragma unroll
+            for (int q = 0; q < 32; ++q) x[q] = (q < jb) ? x[q] : S(0);
 #pragma unroll
             for (int q = 0; q < 32; ++q) {
                 if (q < jb) {

@@ -1,0 +1,415 @@
+// gfx950 kernels of the randomized / learned sparsification (see learned.hpp) and of the rank
+// predictor's training (forward, backward, Adam) -- all one-workgroup-per-task batched launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+
+#include "dpp.hpp"
+#include "learned.hpp"
+
+namespace spaqr {
+
+// ---------------------------------------------------------------------------------------
+// Gaussian test matrix.  Entry (i, j) is a function of (i, j, seed) only -- splitmix64
+// (S. Vigna's public-domain mixer) of the packed index gives two uniforms, Box-Muller gives
+// the normal -- so a larger matrix regenerated later agrees with a smaller one on the overlap.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+template <typename S>
+__global__ __launch_bounds__(WG) void k_gauss(S* om, int rows, int cols, int ld, uint64_t seed) {
+    const size_t total = (size_t)rows * cols;
+    for (size_t e = (size_t)blockIdx.x * WG + threadIdx.x; e < total; e += (size_t)gridDim.x * WG) {
+        const int i = (int)(e % rows), j = (int)(e / rows);
+        const uint64_t h1 = splitmix64(seed ^ (((uint64_t)j << 32) | (uint32_t)i));
+        const uint64_t h2 = splitmix64(h1);
+        const double u1 = 1.0 - (double)(h1 >> 11) * 0x1.0p-53;  // (0, 1]
+        const double u2 = (double)(h2 >> 11) * 0x1.0p-53;        // [0, 1)
+        om[i + (size_t)j * ld] = (S)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Column norms (16 lanes per column).
+// ---------------------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(WG) void k_colnorm(const ColNormTask<S>* __restrict__ tasks) {
+    __shared__ S red[4];
+    const ColNormTask<S> t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, grp = tid >> 4, gl = tid & 15;
+    S mx = S(0);
+    for (int c0 = 0; c0 < t.w; c0 += WG / 16) {
+        const int c = c0 + grp;
+        S s = S(0);
+        if (c < t.w)
+            for (int r = gl; r < t.k; r += 16) {
+                const S v = t.C[r + (size_t)c * t.k];
+                s += v * v;
+            }
+        s = row16_sum(s);
+        if (c < t.w) {
+            if (gl == 0) t.cn[c] = s;
+            mx = max(mx, s);
+        }
+    }
+    mx = wave_max(mx);
+    if (lane == 0) red[wv] = mx;
+    __syncthreads();
+    if (tid == 0) *t.cmax2 = max(max(red[0], red[1]), max(red[2], red[3]));
+}
+
+// ---------------------------------------------------------------------------------------
+// Sketch Y = C Omega: 64 x 64 output tile per workgroup, w consumed in chunks of 32 through
+// LDS.  Lane l of wave q owns row r0 + l and columns c0 + 16q .. c0 + 16q + 15.  Loads are
+// clamped in range and zero-weighted, so they issue unconditionally.
+// ---------------------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(WG) void k_sketch(const SketchTask<S>* __restrict__ tasks,
+                                               const SketchTile* __restrict__ tiles, const S* __restrict__ om,
+                                               int ldo) {
+    __shared__ S Cs[32][65];
+    __shared__ S Os[64][33];
+    const SketchTile tl = tiles[blockIdx.x];
+    const SketchTask<S> t = tasks[tl.task];
+    const int tid = threadIdx.x, row = tid & 63, wv = tid >> 6;
+    S acc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = S(0);
+    for (int w0 = 0; w0 < t.w; w0 += 32) {
+        S cv[8], ov[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + q * WG, rr = e & 63, cc = e >> 6;
+            const int gr = min(tl.r0 + rr, t.k - 1), gc = min(w0 + cc, t.w - 1);
+            cv[q] = t.C[gr + (size_t)gc * t.k];
+            const int i = e & 31, j = e >> 5;
+            ov[q] = om[min(w0 + i, t.w - 1) + (size_t)min(tl.c0 + j, t.s - 1) * ldo];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + q * WG, rr = e & 63, cc = e >> 6;
+            Cs[cc][rr] = (w0 + cc < t.w) ? cv[q] : S(0);
+            const int i = e & 31, j = e >> 5;
+            Os[j][i] = (w0 + i < t.w) ? ov[q] : S(0);
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int i = 0; i < 32; ++i) {
+            const S c = Cs[i][row];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) acc[j] += c * Os[wv * 16 + j][i];
+        }
+        __syncthreads();
+    }
+    const int r = tl.r0 + row;
+    if (r < t.k) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int c = tl.c0 + wv * 16 + j;
+            if (c < t.s) t.Y[r + (size_t)c * t.k] = acc[j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Rank features of one interface.
+//   x[0..15]  decay of the R diagonal of the QR'd 16-column sketch: 1 + log10(|R_jj|/|R_00|)/8
+//   x[16..26] log sizes, aspect, level, tolerance, neighbour counts, share of columns above
+//             the tolerance, column-norm concentration and effective-rank ratio
+// ---------------------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(WG) void k_rank_features(const FeatTask<S>* __restrict__ tasks, double tol) {
+    __shared__ double red_s[4], red_c[4];
+    const FeatTask<S> t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const double cmax2 = (double)*t.cmax2;
+    const double thr2 = tol * tol * cmax2;
+    double sum = 0.0, cnt = 0.0;
+    for (int j = tid; j < t.w; j += WG) {
+        const double c = (double)t.cn[j];
+        sum += c;
+        cnt += (c >= thr2 && c > 0.0) ? 1.0 : 0.0;
+    }
+    sum = wave_sum(sum);
+    cnt = wave_sum(cnt);
+    if (lane == 0) { red_s[wv] = sum; red_c[wv] = cnt; }
+    __syncthreads();
+    if (tid < kRankFeatures) {
+        const double tot = (red_s[0] + red_s[1]) + (red_s[2] + red_s[3]);
+        const double count = (red_c[0] + red_c[1]) + (red_c[2] + red_c[3]);
+        const int kw = min(t.k, t.w), nd = min(min(kSketchFeatureCols, t.k), t.w);
+        const double r0 = nd > 0 ? fabs((double)t.Y16[0]) : 0.0;
+        double f = 0.0;
+        if (tid < kSketchFeatureCols) {
+            if (tid < nd && r0 > 0.0) {
+                const double rj = fabs((double)t.Y16[tid + (size_t)tid * t.k]);
+                f = fmin(1.0, fmax(0.0, 1.0 + log10(rj / r0 + 1e-30) / 8.0));
+            }
+        } else {
+            switch (tid) {
+                case 16: f = log2(t.k + 1.0) / 11.0; break;
+                case 17: f = log2(t.w + 1.0) / 14.0; break;
+                case 18: f = (double)kw / (double)max(max(t.k, t.w), 1); break;
+                case 19: f = t.level / 16.0; break;
+                case 20: f = tol > 0.0 ? -log10(tol) / 8.0 : 1.0; break;
+                case 21: f = t.nsout / 32.0; break;
+                case 22: f = t.nsin / 32.0; break;
+                case 23: f = t.w > 0 ? count / t.w : 0.0; break;
+                case 24: f = kw > 0 ? fmin(1.0, count / kw) : 0.0; break;
+                case 25: f = tot > 0.0 ? cmax2 / tot : 0.0; break;
+                case 26: f = (cmax2 > 0.0 && kw > 0) ? fmin(1.0, tot / cmax2 / kw) : 0.0; break;
+                default: f = 0.0;
+            }
+        }
+        t.x[tid] = (float)f;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Tail selection on B = Q^T C: per column (16 lanes) the smallest r <= rY whose tail
+// ||B(r:k, j)||^2 <= tol^2 max_j ||C(:, j)||^2, by an inclusive DPP scan of the squared
+// entries in 16-row chunks; the interface's rank is the largest of these, or -1 when some
+// column's tail stays above the threshold at rY.  Sums in FP64.
+// ---------------------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(WG) void k_tail_select(const TailTask<S>* __restrict__ tasks, double tol) {
+    __shared__ int red[4];
+    const TailTask<S> t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, grp = tid >> 4, gl = tid & 15;
+    const int rY = *t.rank;
+    const double thr2 = tol * tol * (double)*t.cmax2;
+    int need = 0;
+    for (int c0 = 0; c0 < t.w; c0 += WG / 16) {
+        const int c = c0 + grp;
+        const S* col = t.B + (size_t)min(c, t.w - 1) * t.k;
+        double tot = 0.0;
+        if (c < t.w)
+            for (int r = gl; r < t.k; r += 16) {
+                const double v = (double)col[r];
+                tot += v * v;
+            }
+        tot = row16_sum(tot);
+        int rc = 0;
+        if (c < t.w && tot > thr2) {
+            rc = INT_MAX;
+            double run = 0.0;
+            for (int b = 0; b <= rY; b += 16) {
+                const int i = b + gl;
+                const double v = (i < rY && i < t.k) ? (double)col[i] : 0.0;
+                const double sq = v * v;
+                const double incl = row16_scan(sq);
+                const double tail = tot - (run + incl - sq);  // rows i .. k-1
+                const unsigned long long bal = __ballot(i <= rY && tail <= thr2);
+                const unsigned grpbits = (unsigned)((bal >> (lane & 48)) & 0xFFFFull);
+                if (grpbits) {
+                    rc = b + __ffs(grpbits) - 1;
+                    break;
+                }
+                run += __shfl(incl, (lane & 48) | 15, 64);
+            }
+        }
+        need = max(need, rc);
+    }
+    const int wn = max(max(lane_read(need, 0), lane_read(need, 16)), max(lane_read(need, 32), lane_read(need, 48)));
+    if (lane == 0) red[wv] = wn;
+    __syncthreads();
+    if (tid == 0) {
+        const int r = max(max(red[0], red[1]), max(red[2], red[3]));
+        *t.rank = (r == INT_MAX) ? -1 : r;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// MLP: prediction (one thread per sample, weights in LDS).
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_mlp_predict(const float* __restrict__ X, int n, const float* __restrict__ W,
+                                                    float* __restrict__ out) {
+    __shared__ float w[kMlpParams];
+    for (int p = threadIdx.x; p < kMlpParams; p += WG) w[p] = W[p];
+    __syncthreads();
+    const int i = blockIdx.x * WG + threadIdx.x;
+    if (i >= n) return;
+    float x[kRankFeatures];
+#pragma unroll
+    for (int f = 0; f < kRankFeatures; ++f) x[f] = X[(size_t)i * kRankFeatures + f];
+    float h1[kMlpHidden];
+#pragma unroll
+    for (int h = 0; h < kMlpHidden; ++h) {
+        float a = w[kMlpOffB1 + h];
+#pragma unroll
+        for (int f = 0; f < kRankFeatures; ++f) a += w[kMlpOffW1 + h * kRankFeatures + f] * x[f];
+        h1[h] = tanhf(a);
+    }
+    float o = w[kMlpOffB3];
+    for (int h = 0; h < kMlpHidden; ++h) {
+        float a = w[kMlpOffB2 + h];
+#pragma unroll
+        for (int g = 0; g < kMlpHidden; ++g) a += w[kMlpOffW2 + h * kMlpHidden + g] * h1[g];
+        o += w[kMlpOffW3 + h] * tanhf(a);
+    }
+    out[i] = 1.0f / (1.0f + expf(-o));
+}
+
+// ---------------------------------------------------------------------------------------
+// MLP: gradient of L = mean_i [softplus(o_i) - y_i o_i] (logistic loss on rank fractions).
+// Each workgroup walks its chunk of samples in order; threads 0..63 carry one hidden unit
+// through the forward and backward sweep and every thread accumulates a fixed set of
+// parameter gradients in registers, written once per workgroup.
+// ---------------------------------------------------------------------------------------
+constexpr int kParamsPerThread = (kMlpParams + WG - 1) / WG;  // 25
+
+int mlp_grad_blocks(int n) { return std::max(1, std::min(256, (n + 31) / 32)); }
+
+__global__ __launch_bounds__(WG) void k_mlp_grad(const float* __restrict__ X, const float* __restrict__ y, int n,
+                                                 const float* __restrict__ W, float* __restrict__ partial) {
+    __shared__ float w[kMlpParams];
+    __shared__ float xs[kRankFeatures], h1[kMlpHidden], h2[kMlpHidden], da1[kMlpHidden], da2[kMlpHidden];
+    __shared__ float dos;
+    const int tid = threadIdx.x;
+    for (int p = tid; p < kMlpParams; p += WG) w[p] = W[p];
+    const int per = (n + gridDim.x - 1) / gridDim.x;
+    const int s0 = blockIdx.x * per, s1 = min(n, s0 + per);
+    float g[kParamsPerThread];
+#pragma unroll
+    for (int q = 0; q < kParamsPerThread; ++q) g[q] = 0.f;
+    float loss = 0.f;
+    const float inv_n = 1.0f / (float)n;
+    __syncthreads();
+    for (int sm = s0; sm < s1; ++sm) {
+        if (tid < kRankFeatures) xs[tid] = X[(size_t)sm * kRankFeatures + tid];
+        __syncthreads();
+        if (tid < kMlpHidden) {
+            float a = w[kMlpOffB1 + tid];
+            for (int f = 0; f < kRankFeatures; ++f) a += w[kMlpOffW1 + tid * kRankFeatures + f] * xs[f];
+            h1[tid] = tanhf(a);
+        }
+        __syncthreads();
+        if (tid < kMlpHidden) {  // wave 0 exactly
+            float a = w[kMlpOffB2 + tid];
+            for (int g2 = 0; g2 < kMlpHidden; ++g2) a += w[kMlpOffW2 + tid * kMlpHidden + g2] * h1[g2];
+            const float hh = tanhf(a);
+            h2[tid] = hh;
+            const float o = wave_sum(w[kMlpOffW3 + tid] * hh) + w[kMlpOffB3];
+            const float p = 1.0f / (1.0f + expf(-o));
+            const float yt = y[sm];
+            const float d_o = (p - yt) * inv_n;
+            da2[tid] = d_o * w[kMlpOffW3 + tid] * (1.0f - hh * hh);
+            if (tid == 0) {
+                dos = d_o;
+                loss += (fmaxf(o, 0.f) - o * yt + log1pf(expf(-fabsf(o)))) * inv_n;
+            }
+        }
+        __syncthreads();
+        if (tid < kMlpHidden) {
+            float d = 0.f;
+            for (int h = 0; h < kMlpHidden; ++h) d += w[kMlpOffW2 + h * kMlpHidden + tid] * da2[h];
+            da1[tid] = d * (1.0f - h1[tid] * h1[tid]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kParamsPerThread; ++q) {
+            const int p = tid + q * WG;
+            float c = 0.f;
+            if (p < kMlpOffB1) c = da1[p / kRankFeatures] * xs[p % kRankFeatures];
+            else if (p < kMlpOffW2) c = da1[p - kMlpOffB1];
+            else if (p < kMlpOffB2) { const int e = p - kMlpOffW2; c = da2[e / kMlpHidden] * h1[e % kMlpHidden]; }
+            else if (p < kMlpOffW3) c = da2[p - kMlpOffB2];
+            else if (p < kMlpOffB3) c = dos * h2[p - kMlpOffW3];
+            else if (p == kMlpOffB3) c = dos;
+            g[q] += c;
+        }
+        __syncthreads();
+    }
+    float* out = partial + (size_t)blockIdx.x * (kMlpParams + 1);
+#pragma unroll
+    for (int q = 0; q < kParamsPerThread; ++q) {
+        const int p = tid + q * WG;
+        if (p < kMlpParams) out[p] = g[q];
+    }
+    if (tid == 0) out[kMlpParams] = loss;
+}
+
+// grad[p] = sum_b partial[b][p] in block order; loss likewise; optional Adam step
+// (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected).
+__global__ __launch_bounds__(WG) void k_mlp_reduce(const float* __restrict__ partial, int nblocks, float* grad,
+                                                   float* loss, float* W, float* m, float* v, int step, float lr,
+                                                   int adam) {
+    const int p = blockIdx.x * WG + threadIdx.x;
+    if (p > kMlpParams) return;
+    float s = 0.f;
+    for (int b = 0; b < nblocks; ++b) s += partial[(size_t)b * (kMlpParams + 1) + p];
+    if (p == kMlpParams) {
+        if (loss) *loss = s;
+        return;
+    }
+    if (grad) grad[p] = s;
+    if (adam) {
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float mm = b1 * m[p] + (1.f - b1) * s;
+        const float vv = b2 * v[p] + (1.f - b2) * s * s;
+        m[p] = mm;
+        v[p] = vv;
+        const float mh = mm / (1.f - powf(b1, (float)step));
+        const float vh = vv / (1.f - powf(b2, (float)step));
+        W[p] -= lr * mh / (sqrtf(vh) + 1e-8f);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------------
+template <typename S>
+void launch_gauss(S* om, int rows, int cols, int ld, uint64_t seed, hipStream_t s) {
+    const size_t total = (size_t)rows * cols;
+    if (total == 0) return;
+    const int blocks = (int)std::min<size_t>((total + WG - 1) / WG, 4096);
+    hipLaunchKernelGGL(k_gauss<S>, dim3(blocks), dim3(WG), 0, s, om, rows, cols, ld, seed);
+}
+template <typename S>
+void launch_colnorm(const ColNormTask<S>* d, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_colnorm<S>, dim3(n), dim3(WG), 0, s, d);
+}
+template <typename S>
+void launch_sketch(const SketchTask<S>* d, const SketchTile* tiles, int ntiles, const S* om, int ldo, hipStream_t s) {
+    if (ntiles > 0) hipLaunchKernelGGL(k_sketch<S>, dim3(ntiles), dim3(WG), 0, s, d, tiles, om, ldo);
+}
+template <typename S>
+void launch_rank_features(const FeatTask<S>* d, int n, double tol, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_rank_features<S>, dim3(n), dim3(WG), 0, s, d, tol);
+}
+template <typename S>
+void launch_tail_select(const TailTask<S>* d, int n, double tol, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_tail_select<S>, dim3(n), dim3(WG), 0, s, d, tol);
+}
+void launch_mlp_predict(const float* X, int n, const float* W, float* out, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_mlp_predict, dim3((n + WG - 1) / WG), dim3(WG), 0, s, X, n, W, out);
+}
+void launch_mlp_grad(const float* X, const float* y, int n, const float* W, float* partial, int nblocks,
+                     hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_mlp_grad, dim3(nblocks), dim3(WG), 0, s, X, y, n, W, partial);
+}
+void launch_mlp_reduce(const float* partial, int nblocks, float* grad, float* loss, float* W, float* m, float* v,
+                       int step, float lr, int adam, hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + 1 + WG - 1) / WG), dim3(WG), 0, s, partial, nblocks, grad,
+                       loss, W, m, v, step, lr, adam);
+}
+
+#define SPAQR_INST(S)                                                                                       \
+    template void launch_gauss<S>(S*, int, int, int, uint64_t, hipStream_t);                                 \
+    template void launch_colnorm<S>(const ColNormTask<S>*, int, hipStream_t);                                \
+    template void launch_sketch<S>(const SketchTask<S>*, const SketchTile*, int, const S*, int, hipStream_t); \
+    template void launch_rank_features<S>(const FeatTask<S>*, int, double, hipStream_t);                     \
+    template void launch_tail_select<S>(const TailTask<S>*, int, double, hipStream_t);
+SPAQR_INST(float)
+SPAQR_INST(double)
+#undef SPAQR_INST
+
+}  // namespace spaqr

@@ -1,0 +1,71 @@
+// Randomized interface sparsification with a learned sketch size.
+//
+// The reference's RRQR (src/ptree.cpp:373-439) runs a full column-pivoted QR of the gathered
+// C = [A(n,c)^T | A(c,m)_top] (k x w).  Here, for FP32 fronts, the rank-revealing QR runs on a
+// Gaussian sketch Y = C Omega (k x s) instead, with s taken from a small MLP's rank prediction
+// (the closest reference analogue is the unused adaptive range finder random_rrqr,
+// src/util.cpp:643-715).  Every result is judged by the criterion geqp3 itself stops on:
+// with Q from the sketch, the chosen rank r is the smallest for which every column of
+// (Q^T C)[r:k, :] has norm <= tol * max_j ||C(:, j)||; an interface that fails falls back to
+// geqp3 on C.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace spaqr {
+
+constexpr int kRankFeatures = 32;
+constexpr int kMlpHidden = 64;
+// W1 (64 x 32, row-major), b1 (64), W2 (64 x 64), b2 (64), w3 (64), b3 (1)
+constexpr int kMlpOffW1 = 0;
+constexpr int kMlpOffB1 = kMlpOffW1 + kMlpHidden * kRankFeatures;
+constexpr int kMlpOffW2 = kMlpOffB1 + kMlpHidden;
+constexpr int kMlpOffB2 = kMlpOffW2 + kMlpHidden * kMlpHidden;
+constexpr int kMlpOffW3 = kMlpOffB2 + kMlpHidden;
+constexpr int kMlpOffB3 = kMlpOffW3 + kMlpHidden;
+constexpr int kMlpParams = kMlpOffB3 + 1;  // 6337
+constexpr int kSketchFeatureCols = 16;
+
+// Column norms: cn[j] = ||C(:, j)||^2 (w values) and cmax2 = max_j cn[j].
+template <typename S> struct ColNormTask { const S* C; S* cn; S* cmax2; int k, w; };
+// Sketch Y (k x s, ld k) = C (k x w, ld k) * Omega(0:w, 0:s); one 64 x 64 tile per workgroup.
+template <typename S> struct SketchTask { const S* C; S* Y; int k, w, s, pad; };
+struct SketchTile { int task, r0, c0, pad; };
+// Features of one interface (32 floats) from the QR'd 16-column sketch Y16 (its R diagonal),
+// the column norms and the tree context.
+template <typename S> struct FeatTask {
+    const S* Y16; const S* cn; const S* cmax2; float* x;
+    int k, w, nsout, nsin, level, pad;
+};
+// B = Q^T C (k x w) with Q from the sketch's first rY reflectors: rank <- smallest r <= rY
+// whose tail column norms are all <= tol * max column norm of the original C, or -1.
+template <typename S> struct TailTask { const S* B; const S* cmax2; int* rank; int k, w; };
+
+template <typename S> void launch_gauss(S* om, int rows, int cols, int ld, uint64_t seed, hipStream_t s);
+template <typename S> void launch_colnorm(const ColNormTask<S>* d, int n, hipStream_t s);
+template <typename S>
+void launch_sketch(const SketchTask<S>* d, const SketchTile* tiles, int ntiles, const S* om, int ldo, hipStream_t s);
+template <typename S> void launch_rank_features(const FeatTask<S>* d, int n, double tol, hipStream_t s);
+template <typename S> void launch_tail_select(const TailTask<S>* d, int n, double tol, hipStream_t s);
+
+// MLP (kRankFeatures -> 64 -> 64 -> 1, tanh, sigmoid output): predicted rank fractions.
+void launch_mlp_predict(const float* X, int n, const float* W, float* out, hipStream_t s);
+// One full-batch gradient of the mean logistic loss; partial sums per workgroup are reduced
+// in a fixed order, so the result is deterministic.
+int mlp_grad_blocks(int n);
+void launch_mlp_grad(const float* X, const float* y, int n, const float* W, float* partial, int nblocks,
+                     hipStream_t s);
+// grad = sum of partials (and loss); with adam != 0 also one Adam step on W (m, v moments).
+void launch_mlp_reduce(const float* partial, int nblocks, float* grad, float* loss, float* W, float* m, float* v,
+                       int step, float lr, int adam, hipStream_t s);
+
+// Host-memory wrappers (allocate, copy, run, copy back, synchronize) for the C ABI.
+int mlp_predict_host(const float* X, int n, const float* W, float* out);
+int mlp_grad_host(const float* X, const float* y, int n, const float* W, float* grad, float* loss);
+int mlp_train_host(const float* X, const float* y, int n, float* W, int epochs, float lr, float* loss_hist);
+
+}  // namespace spaqr

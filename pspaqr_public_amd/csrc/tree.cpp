@@ -32,18 +32,9 @@
 #include "launch.hpp"
 #include "rrqr.hpp"
 
-#define HIPCHK(x)                                                                                    \
-    do {                                                                                             \
-        hipError_t _e = (x);                                                                         \
-        if (_e != hipSuccess)                                                                        \
-            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(_e) + " at " #x); \
-    } while (0)
+#include "tree_impl.hpp"
 
 namespace spaqr {
-
-// geqp3 pivot candidates within this relative band of the largest norm count as tied and
-// the lowest index wins (LAPACK's plain argmax lets rounding order decide exact ties).
-constexpr double kPivotTieRel = 1e-10;
 
 bool ClusterID::operator<(const ClusterID& o) const {  // include/cluster.h:128-134
     if (self != o.self) return self < o.self;
@@ -204,23 +195,6 @@ void Tree::sync_stream() {
     prof_pending.clear();
 }
 
-template <class F>
-void Tree::timed(int cat, long launches, long tasks, F&& f) {
-    if (!profiling) {
-        f();
-        return;
-    }
-    hipEvent_t a, b;
-    HIPCHK(hipEventCreate(&a));
-    HIPCHK(hipEventCreate(&b));
-    HIPCHK(hipEventRecord(a, stream_));
-    f();
-    HIPCHK(hipEventRecord(b, stream_));
-    prof_pending.push_back({cat, {a, b}});
-    prof[cat].launches += launches;
-    prof[cat].tasks += tasks;
-}
-
 void Tree::prof_reset() {
     for (auto& p : prof) p = ProfEntry();
 }
@@ -268,6 +242,8 @@ Tree::~Tree() {
     if (d_scratch) (void)hipFree(d_scratch);
     if (d_blocks0) (void)hipFree(d_blocks0);
     if (xw) (void)hipFree(xw);
+    if (d_mlp_w) (void)hipFree(d_mlp_w);
+    if (d_omega) (void)hipFree(d_omega);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -740,8 +716,10 @@ int Tree::factorize_t() {
             }
         // ---- RRQR descriptors (src/ptree.cpp:373-439)
         std::vector<CopyTask<S>> gat;
+        std::vector<int> gat_begin;  // first gather copy of each RRQR task
         std::vector<Geqp3Task<S>> gq;
         std::vector<int> gq_cluster;
+        std::vector<char> used_sketch;
         int* d_ranks = nullptr;
         int* d_rrqr_err = nullptr;
         const bool spars_level = (L >= skip && L < nlevels - 2 && tol != 0);
@@ -758,6 +736,7 @@ int Tree::factorize_t() {
                 for (int e : C.sin) w += ed[e].cols;
                 const int kmax = std::max(1, std::min(k, w));
                 S* Cb = (S*)arena.alloc((size_t)k * std::max(w, 1) * sizeof(S));
+                gat_begin.push_back((int)gat.size());
                 int is = 0;
                 for (int e : C.sout) {
                     gat.push_back({Ap(e), Cb + (size_t)is * k, k, ed[e].rows, ed[e].rows, k, 1, 0});
@@ -774,7 +753,7 @@ int Tree::factorize_t() {
                 int* jp = (int*)arena.alloc((size_t)std::max(w, 1) * sizeof(int));
                 gq.push_back({Cb, (S*)C.Qsp, (S*)C.tausp, (S*)C.Tsp, vn, vn + std::max(w, 1), jp, d_ranks + q, k, w, 0, 0});
                 prof_add(P_GATHER, 0.0, SZ * 2.0 * k * w);
-                prof_add(P_GEQP3, 0.0, SZ * 3.0 * k * w);
+                if (rank_mode == 0) prof_add(P_GEQP3, 0.0, SZ * 3.0 * k * w);
                 gq_cluster.push_back(who[q]);
             }
         }
@@ -812,10 +791,16 @@ int Tree::factorize_t() {
                               [&](const void* h, size_t b) { return upload(h, b); }};
                 const int zero = 0;
                 d_rrqr_err = (int*)upload(&zero, sizeof(int));
-                int nd = 0;
-                timed(P_GEQP3, 0, (long)gq.size(),
-                      [&] { geqp3_grouped<S>(gq, tol, kPivotTieRel, stream_, mem, d_rrqr_err, &nd); });
-                if (profiling) prof[P_GEQP3].launches += nd;
+                gat_begin.push_back((int)gat.size());
+                used_sketch.assign(gq.size(), 0);
+                if (rank_mode == 0 && !rank_log) {
+                    int nd = 0;
+                    timed(P_GEQP3, 0, (long)gq.size(),
+                          [&] { geqp3_grouped<S>(gq, tol, kPivotTieRel, stream_, mem, d_rrqr_err, &nd); });
+                    if (profiling) prof[P_GEQP3].launches += nd;
+                } else {
+                    st.flops += rrqr_level<S>(gq, gat, gat_begin, gq_cluster, L, d_ranks, d_rrqr_err, used_sketch);
+                }
             }
             HIPCHK(hipGetLastError());
         }
@@ -833,10 +818,11 @@ int Tree::factorize_t() {
                 st.cols_before += k;
                 st.ranks_after += ranks[q];
                 int r = ranks[q];
-                for (int j = 0; j < r; ++j) {
-                    st.flops += 4.0 * (k - j) * (w - j);
-                    prof_add(P_GEQP3, 4.0 * (k - j) * (w - j), 0.0);
-                }
+                if (!used_sketch[q])
+                    for (int j = 0; j < r; ++j) {
+                        st.flops += 4.0 * (k - j) * (w - j);
+                        prof_add(P_GEQP3, 4.0 * (k - j) * (w - j), 0.0);
+                    }
                 C.sparsified = true;
                 C.rank = r;
                 C.rows = C.cols = r;  // reset_size (src/ptree.cpp:435-438)

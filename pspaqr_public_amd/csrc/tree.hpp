@@ -10,6 +10,8 @@
 #include <string>
 #include <vector>
 
+#include "kernels.hpp"
+
 namespace spaqr {
 
 struct SepID {
@@ -83,7 +85,7 @@ struct LevelStats {
 // Kernel classes timed with HIP events on the tree's stream when profiling is on.
 enum ProfCat {
     P_GEQRT = 0, P_LARFB, P_TPQRT, P_TPMQRT, P_SGEQRT, P_SAPPLY, P_GATHER, P_GEQP3, P_SPARS, P_FILL,
-    P_SOLVE, P_SPMV, P_KRYLOV, P_NCAT
+    P_SOLVE, P_SPMV, P_KRYLOV, P_RSKETCH, P_NCAT
 };
 struct ProfEntry {
     double ms = 0, flops = 0, bytes = 0;
@@ -126,6 +128,20 @@ public:
     double factor_flops() const;
     int diag_R(int order, double* out, int maxn);  // upper triangle of A(c,c) after factorize
     hipStream_t stream() const { return stream_; }
+    // Sparsification rank path.  0: geqp3 on the gathered C (the reference method); 1: pivoted
+    // QR of a Gaussian sketch whose width comes from the rank predictor, judged by the tail
+    // test, geqp3 fallback; 2: the same with the sketch as wide as min(k, w) (no predictor).
+    int rank_mode = 0;
+    bool rank_log = false;     // record (features, rank) of every sparsified interface
+    int set_rank_model(const float* w);  // kMlpParams predictor weights (host)
+    struct RankLog {
+        std::vector<float> x;  // kRankFeatures per entry
+        std::vector<int> rank, k, w, level, mode;
+    } rlog;
+    struct RankStats {
+        long tasks = 0, sketched = 0, fallback = 0, direct = 0;
+        double sketch_cols = 0, full_cols = 0, rank_sum = 0;
+    } rstats;
     bool profiling = false;
     ProfEntry prof[P_NCAT];
     void prof_reset();
@@ -169,6 +185,17 @@ private:
     long long d_nnz = 0;
     double* d_scratch = nullptr;  // dot partials + scalars
     int dot_blocks = 512;
+
+    // rank path state
+    float* d_mlp_w = nullptr;
+    bool has_model = false;
+    void* d_omega = nullptr;
+    int om_rows = 0, om_cols = 0, om_elem = 0;
+    template <typename S>
+    double rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTask<S>>& gat,
+                    const std::vector<int>& gat_begin, const std::vector<int>& gq_cluster, int L, int* d_ranks,
+                    int* d_err, std::vector<char>& used_sketch);
+    template <typename S> const S* omega(int rows, int cols);
 
     // solve schedule
     struct SolveSchedule;

@@ -1,0 +1,308 @@
+// Sparsification rank path of the level planner: the reference's geqp3 on the gathered C
+// (src/ptree.cpp:373-439), or the randomized path of learned.hpp -- 16-column sketch
+// features, the predictor's sketch width, pivoted QR of the sketch, the tail test and the
+// per-interface geqp3 fallback -- plus the host side of the predictor's training.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "launch.hpp"
+#include "learned.hpp"
+#include "rrqr.hpp"
+#include "tree_impl.hpp"
+
+namespace spaqr {
+
+namespace {
+constexpr double kSketchGrow = 1.25;      // sketch width = grow * predicted rank + pad
+constexpr int kSketchPad = 8;
+constexpr double kSketchTolFactor = 0.25;  // the sketch's QR stops at this fraction of tol
+constexpr uint64_t kSketchSeed = 0x5eed5ca1eull;
+}  // namespace
+
+int Tree::set_rank_model(const float* w) {
+    if (!d_mlp_w) HIPCHK(hipMalloc(&d_mlp_w, sizeof(float) * kMlpParams));
+    HIPCHK(hipMemcpy(d_mlp_w, w, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+    has_model = true;
+    return 0;
+}
+
+template <typename S>
+const S* Tree::omega(int rows, int cols) {
+    rows = std::max(rows, 1);
+    cols = std::max(cols, 1);
+    if (!d_omega || rows > om_rows || cols > om_cols || om_elem != (int)sizeof(S)) {
+        sync_stream();  // the previous matrix may still be read by queued work
+        if (d_omega) (void)hipFree(d_omega);
+        d_omega = nullptr;
+        const int r2 = std::max(rows, om_elem == (int)sizeof(S) ? om_rows : 0);
+        const int c2 = std::max(cols, om_elem == (int)sizeof(S) ? om_cols : 0);
+        HIPCHK(hipMalloc(&d_omega, sizeof(S) * (size_t)r2 * c2));
+        launch_gauss<S>((S*)d_omega, r2, c2, r2, kSketchSeed, stream_);
+        om_rows = r2;
+        om_cols = c2;
+        om_elem = (int)sizeof(S);
+    }
+    return (const S*)d_omega;
+}
+
+template <typename S>
+double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTask<S>>& gat,
+                        const std::vector<int>& gat_begin, const std::vector<int>& gq_cluster, int L, int* d_ranks,
+                        int* d_err, std::vector<char>& used_sketch) {
+    const int nt = (int)gq.size();
+    const double SZ = sizeof(S);
+    double flops = 0.0;
+    RrqrAlloc mem{[&](size_t b) { return arena.alloc(b); }, [&](const void* h, size_t b) { return upload(h, b); }};
+    const bool sketch_mode = rank_mode == 1 || rank_mode == 2;
+    if (rank_mode == 1 && !has_model)
+        throw std::runtime_error("rank mode 1 needs predictor weights (spaqr_tree_set_rank_model)");
+    const bool feats = rank_log || rank_mode == 1;
+    std::vector<int> elig;  // interfaces wide enough for a 16-column sketch
+    std::vector<char> is_elig(nt, 0);
+    if (tol > 0.0 && tol < 1.0)
+        for (int q = 0; q < nt; ++q)
+            if (std::min(gq[q].k, gq[q].w) > kSketchFeatureCols) {
+                elig.push_back(q);
+                is_elig[q] = 1;
+            }
+    const int ne = (int)elig.size();
+    int wmax = 1;
+    for (int q : elig) wmax = std::max(wmax, gq[q].w);
+    S* d_cmax2 = (S*)arena.alloc(sizeof(S) * std::max(nt, 1));
+    float* d_X = nullptr;
+    if (ne && (feats || sketch_mode)) {
+        std::vector<ColNormTask<S>> cn;
+        for (int q : elig) cn.push_back({gq[q].C, gq[q].vn1, d_cmax2 + q, gq[q].k, gq[q].w});
+        auto* d_cn = upload_vec(cn);
+        timed(P_RSKETCH, 1, ne, [&] { launch_colnorm<S>(d_cn, ne, stream_); });
+        for (int q : elig) prof_add(P_RSKETCH, 2.0 * gq[q].k * gq[q].w, SZ * gq[q].k * gq[q].w);
+    }
+    if (ne && feats) {
+        const S* om = omega<S>(wmax, kSketchFeatureCols);
+        std::vector<SketchTask<S>> sk;
+        std::vector<SketchTile> tiles;
+        std::vector<GeqrtTask<S>> qr;
+        std::vector<FeatTask<S>> ft;
+        d_X = (float*)arena.alloc(sizeof(float) * kRankFeatures * ne);
+        for (int i = 0; i < ne; ++i) {
+            const int q = elig[i], k = gq[q].k, w = gq[q].w;
+            S* Y16 = (S*)arena.alloc(sizeof(S) * (size_t)k * kSketchFeatureCols);
+            S* T16 = (S*)arena.alloc(sizeof(S) * NB * kSketchFeatureCols);
+            for (int r0 = 0; r0 < k; r0 += 64) tiles.push_back({(int)sk.size(), r0, 0, 0});
+            sk.push_back({gq[q].C, Y16, k, w, kSketchFeatureCols, 0});
+            qr.push_back({Y16, T16, k, std::min(kSketchFeatureCols, k), k, 0});
+            const Cluster& C = cl[gq_cluster[q]];
+            ft.push_back({Y16, gq[q].vn1, d_cmax2 + q, d_X + (size_t)i * kRankFeatures, k, w, (int)C.sout.size(),
+                          (int)C.sin.size(), L, 0});
+            prof_add(P_RSKETCH, 2.0 * k * w * kSketchFeatureCols, SZ * ((double)k * w + 2.0 * k * kSketchFeatureCols));
+        }
+        auto* d_sk = upload_vec(sk);
+        auto* d_tl = upload_vec(tiles);
+        auto* d_qr = upload_vec(qr);
+        auto* d_ft = upload_vec(ft);
+        timed(P_RSKETCH, 3, ne, [&] {
+            launch_sketch<S>(d_sk, d_tl, (int)tiles.size(), om, om_rows, stream_);
+            launch_geqrt<S>(d_qr, ne, stream_);
+            launch_rank_features<S>(d_ft, ne, tol, stream_);
+        });
+    }
+    std::vector<int> rk(nt, -1), ry(nt, 0), s_of(nt, 0);
+    std::vector<char> applied(nt, 0);
+    if (ne && sketch_mode) {
+        if (rank_mode == 1) {
+            float* d_frac = (float*)arena.alloc(sizeof(float) * ne);
+            timed(P_RSKETCH, 1, ne, [&] { launch_mlp_predict(d_X, ne, d_mlp_w, d_frac, stream_); });
+            std::vector<float> frac(ne);
+            HIPCHK(hipMemcpyAsync(frac.data(), d_frac, sizeof(float) * ne, hipMemcpyDeviceToHost, stream_));
+            sync_stream();
+            for (int i = 0; i < ne; ++i) {
+                const int q = elig[i], kw = std::min(gq[q].k, gq[q].w);
+                const double f = std::isfinite(frac[i]) ? std::min(1.0f, std::max(0.0f, frac[i])) : 1.0;
+                s_of[q] = std::max(1, std::min(kw, (int)std::ceil(kSketchGrow * f * kw) + kSketchPad));
+            }
+        } else {
+            for (int q : elig) s_of[q] = std::min(gq[q].k, gq[q].w);
+        }
+        int smax = 1;
+        for (int q : elig) smax = std::max(smax, s_of[q]);
+        const S* om = omega<S>(wmax, smax);
+        std::vector<SketchTask<S>> sk;
+        std::vector<SketchTile> tiles;
+        std::vector<Geqp3Task<S>> gy;
+        for (int q : elig) {
+            const int k = gq[q].k, w = gq[q].w, s = s_of[q];
+            S* Y = (S*)arena.alloc(sizeof(S) * (size_t)k * s);
+            for (int r0 = 0; r0 < k; r0 += 64)
+                for (int c0 = 0; c0 < s; c0 += 64) tiles.push_back({(int)sk.size(), r0, c0, 0});
+            sk.push_back({gq[q].C, Y, k, w, s, 0});
+            Geqp3Task<S> g = gq[q];
+            g.C = Y;
+            g.w = s;
+            gy.push_back(g);
+            flops += 2.0 * k * w * s;
+            prof_add(P_RSKETCH, 2.0 * k * w * s, SZ * ((double)k * w + (double)k * s));
+            prof_add(P_GEQP3, 0.0, SZ * 3.0 * k * s);
+            rstats.sketch_cols += s;
+            rstats.full_cols += std::min(k, w);
+        }
+        auto* d_sk = upload_vec(sk);
+        auto* d_tl = upload_vec(tiles);
+        timed(P_RSKETCH, 1, ne, [&] { launch_sketch<S>(d_sk, d_tl, (int)tiles.size(), om, om_rows, stream_); });
+        int nd = 0;
+        timed(P_GEQP3, 0, ne,
+              [&] { geqp3_grouped<S>(gy, tol * kSketchTolFactor, kPivotTieRel, stream_, mem, d_err, &nd); });
+        if (profiling) prof[P_GEQP3].launches += nd;
+        HIPCHK(hipMemcpyAsync(ry.data(), d_ranks, sizeof(int) * nt, hipMemcpyDeviceToHost, stream_));
+        sync_stream();
+        std::vector<LarfbTask<S>> lb;
+        std::vector<TailTask<S>> tt;
+        for (int q : elig) {
+            const int k = gq[q].k, w = gq[q].w, r = ry[q];
+            for (int j = 0; j < r; ++j) flops += 4.0 * (k - j) * (s_of[q] - j);
+            if (r <= 0) continue;
+            lb.push_back({gq[q].Vout, gq[q].T, gq[q].C, nullptr, k, w, r, k, k, 0, 0, 0});
+            tt.push_back({gq[q].C, d_cmax2 + q, d_ranks + q, k, w});
+            applied[q] = 1;
+            flops += 4.0 * k * w * r - 2.0 * w * (double)r * r;
+            prof_add(P_RSKETCH, 4.0 * k * w * r - 2.0 * w * (double)r * r, SZ * ((double)k * r + 2.0 * k * w));
+        }
+        if (!lb.empty()) {
+            auto* d_lb = upload_vec(lb);
+            auto* d_tt = upload_vec(tt);
+            timed(P_RSKETCH, 2, (long)lb.size(), [&] {
+                launch_larfb<S>(d_lb, (int)lb.size(), stream_);
+                launch_tail_select<S>(d_tt, (int)tt.size(), tol, stream_);
+            });
+        }
+        HIPCHK(hipMemcpyAsync(rk.data(), d_ranks, sizeof(int) * nt, hipMemcpyDeviceToHost, stream_));
+        sync_stream();
+    }
+    // ---- decide: sketch result, or geqp3 on C
+    std::vector<int> fb;
+    std::vector<int> fixed(nt, 0);
+    bool patch = false;
+    for (int q = 0; q < nt; ++q) {
+        if (sketch_mode && is_elig[q] && applied[q] && rk[q] >= 0) {
+            used_sketch[q] = 1;
+            fixed[q] = std::max(rk[q], 1);  // choose_rank keeps at least one column
+            if (rk[q] == 0) patch = true;
+        } else {
+            fb.push_back(q);
+        }
+    }
+    if (patch) {
+        for (int q = 0; q < nt; ++q)
+            if (!used_sketch[q]) fixed[q] = 0;
+        int* src = (int*)upload(fixed.data(), sizeof(int) * nt);
+        HIPCHK(hipMemcpyAsync(d_ranks, src, sizeof(int) * nt, hipMemcpyDeviceToDevice, stream_));
+    }
+    if (!fb.empty()) {
+        std::vector<CopyTask<S>> regather;
+        std::vector<Geqp3Task<S>> gf;
+        for (int q : fb) {
+            if (applied[q])
+                for (int g = gat_begin[q]; g < gat_begin[q + 1]; ++g) regather.push_back(gat[g]);
+            gf.push_back(gq[q]);
+            prof_add(P_GEQP3, 0.0, SZ * 3.0 * gq[q].k * gq[q].w);
+        }
+        if (!regather.empty()) {
+            auto* d_rg = upload_vec(regather);
+            timed(P_GATHER, 1, (long)regather.size(), [&] { launch_copy<S>(d_rg, (int)regather.size(), stream_); });
+        }
+        int nd = 0;
+        timed(P_GEQP3, 0, (long)gf.size(),
+              [&] { geqp3_grouped<S>(gf, tol, kPivotTieRel, stream_, mem, d_err, &nd); });
+        if (profiling) prof[P_GEQP3].launches += nd;
+    }
+    rstats.tasks += nt;
+    for (int q = 0; q < nt; ++q) {
+        if (used_sketch[q]) rstats.sketched++;
+        else if (sketch_mode && is_elig[q]) rstats.fallback++;
+        else rstats.direct++;
+    }
+    if (rank_log && ne) {
+        std::vector<float> X((size_t)ne * kRankFeatures);
+        std::vector<int> fin(nt);
+        HIPCHK(hipMemcpyAsync(X.data(), d_X, sizeof(float) * X.size(), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipMemcpyAsync(fin.data(), d_ranks, sizeof(int) * nt, hipMemcpyDeviceToHost, stream_));
+        sync_stream();
+        for (int i = 0; i < ne; ++i) {
+            const int q = elig[i];
+            rlog.x.insert(rlog.x.end(), X.begin() + (size_t)i * kRankFeatures, X.begin() + (size_t)(i + 1) * kRankFeatures);
+            rlog.rank.push_back(fin[q]);
+            rlog.k.push_back(gq[q].k);
+            rlog.w.push_back(gq[q].w);
+            rlog.level.push_back(L);
+            rlog.mode.push_back(used_sketch[q] ? rank_mode : 0);
+        }
+    }
+    return flops;
+}
+
+template const float* Tree::omega<float>(int, int);
+template const double* Tree::omega<double>(int, int);
+template double Tree::rrqr_level<float>(std::vector<Geqp3Task<float>>&, const std::vector<CopyTask<float>>&,
+                                        const std::vector<int>&, const std::vector<int>&, int, int*, int*,
+                                        std::vector<char>&);
+template double Tree::rrqr_level<double>(std::vector<Geqp3Task<double>>&, const std::vector<CopyTask<double>>&,
+                                         const std::vector<int>&, const std::vector<int>&, int, int*, int*,
+                                         std::vector<char>&);
+
+// ---------------------------------------------------------------------------- predictor host side
+namespace {
+struct DevBuf {
+    void* p = nullptr;
+    explicit DevBuf(size_t bytes) { HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 4))); }
+    ~DevBuf() { (void)hipFree(p); }
+    float* f() const { return (float*)p; }
+};
+}  // namespace
+
+int mlp_predict_host(const float* X, int n, const float* W, float* out) {
+    DevBuf dX(sizeof(float) * (size_t)n * kRankFeatures), dW(sizeof(float) * kMlpParams), dO(sizeof(float) * n);
+    HIPCHK(hipMemcpy(dX.p, X, sizeof(float) * (size_t)n * kRankFeatures, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dW.p, W, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+    launch_mlp_predict(dX.f(), n, dW.f(), dO.f(), nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(out, dO.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int mlp_grad_host(const float* X, const float* y, int n, const float* W, float* grad, float* loss) {
+    const int nb = mlp_grad_blocks(n);
+    DevBuf dX(sizeof(float) * (size_t)n * kRankFeatures), dy(sizeof(float) * n), dW(sizeof(float) * kMlpParams),
+        dP(sizeof(float) * (size_t)nb * (kMlpParams + 1)), dG(sizeof(float) * kMlpParams), dL(sizeof(float));
+    HIPCHK(hipMemcpy(dX.p, X, sizeof(float) * (size_t)n * kRankFeatures, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dy.p, y, sizeof(float) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dW.p, W, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+    launch_mlp_grad(dX.f(), dy.f(), n, dW.f(), dP.f(), nb, nullptr);
+    launch_mlp_reduce(dP.f(), nb, dG.f(), dL.f(), nullptr, nullptr, nullptr, 0, 0.f, 0, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(grad, dG.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(loss, dL.p, sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int mlp_train_host(const float* X, const float* y, int n, float* W, int epochs, float lr, float* loss_hist) {
+    const int nb = mlp_grad_blocks(n);
+    DevBuf dX(sizeof(float) * (size_t)n * kRankFeatures), dy(sizeof(float) * n), dW(sizeof(float) * kMlpParams),
+        dP(sizeof(float) * (size_t)nb * (kMlpParams + 1)), dm(sizeof(float) * kMlpParams),
+        dv(sizeof(float) * kMlpParams), dL(sizeof(float) * std::max(epochs, 1));
+    HIPCHK(hipMemcpy(dX.p, X, sizeof(float) * (size_t)n * kRankFeatures, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dy.p, y, sizeof(float) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dW.p, W, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(dm.p, 0, sizeof(float) * kMlpParams));
+    HIPCHK(hipMemset(dv.p, 0, sizeof(float) * kMlpParams));
+    for (int e = 0; e < epochs; ++e) {
+        launch_mlp_grad(dX.f(), dy.f(), n, dW.f(), dP.f(), nb, nullptr);
+        launch_mlp_reduce(dP.f(), nb, nullptr, dL.f() + e, dW.f(), dm.f(), dv.f(), e + 1, lr, 1, nullptr);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(W, dW.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+    if (loss_hist && epochs > 0) HIPCHK(hipMemcpy(loss_hist, dL.p, sizeof(float) * epochs, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // namespace spaqr

@@ -119,6 +119,39 @@ class ParTree:
         check(fn(self._h, b_ptr, x_ptr, iters, tol, restart, true_tol, C.byref(res), None, 0, None), self._h, "gmres")
         return res
 
+    # ---- rank path (no reference counterpart; see include/spaqr_gpu.h)
+    def set_rank_mode(self, mode: int):
+        check(lib().spaqr_tree_set_rank_mode(self._h, int(mode)), self._h, "set_rank_mode")
+
+    def set_rank_model(self, weights: np.ndarray):
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        check(lib().spaqr_tree_set_rank_model(self._h, w.ctypes.data, int(w.size)), self._h, "set_rank_model")
+
+    def set_rank_log(self, on: bool = True):
+        check(lib().spaqr_tree_set_rank_log(self._h, int(bool(on))), self._h, "set_rank_log")
+
+    def rank_log(self):
+        """(X [n x 32] float32, meta [n x 5] int32: rank, k, w, level, mode used)."""
+        n = lib().spaqr_tree_rank_log(self._h, None, None, 0)
+        nf = lib().spaqr_mlp_features()
+        X = np.zeros((n, nf), dtype=np.float32)
+        meta = np.zeros((n, 5), dtype=np.int32)
+        if n:
+            lib().spaqr_tree_rank_log(self._h, X.ctypes.data, meta.ctypes.data, n)
+        return X, meta
+
+    def rank_log_clear(self):
+        check(lib().spaqr_tree_rank_log_clear(self._h), self._h, "rank_log_clear")
+
+    def rank_stats(self) -> dict:
+        out = np.zeros(6)
+        check(lib().spaqr_tree_rank_stats(self._h, out.ctypes.data), self._h, "rank_stats")
+        return dict(interfaces=int(out[0]), sketched=int(out[1]), fallback=int(out[2]), direct=int(out[3]),
+                    sketch_cols=float(out[4]), full_cols=float(out[5]))
+
+    def rank_stats_reset(self):
+        check(lib().spaqr_tree_rank_stats_reset(self._h), self._h, "rank_stats_reset")
+
     # ---- introspection
     @property
     def n(self) -> int:
