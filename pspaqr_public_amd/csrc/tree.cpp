@@ -760,7 +760,11 @@ int Tree::factorize_t() {
         // ---- upload and launch
         {
             GeqrtTask<S>* d_geq = upload_vec(geq);
-            LarfbTask<S>* d_lfb = upload_vec(lfb);
+            std::vector<LarfbTask<S>> lfb_c, lfb_t, sapp_c, sapp_t;
+            split_larfb(lfb, lfb_c, lfb_t);
+            split_larfb(sapp, sapp_c, sapp_t);
+            LarfbTask<S>* d_lfb = upload_vec(lfb_c);
+            LarfbTask<S>* d_lfb_t = upload_vec(lfb_t);
             TpqrtTask<S>* d_tpq = upload_vec(tpq);
             TpqrtLink<S>* d_tpql = upload_vec(tpql);
             TpmqrtLink<S>* d_tpml = upload_vec(tpml);
@@ -768,12 +772,17 @@ int Tree::factorize_t() {
             for (int q = 0; q < ncolors; ++q) d_tpm[q] = upload_vec(tpm[q]);
             CopyTask<S>* d_scopy = upload_vec(scopy);
             GeqrtTask<S>* d_sgeq = upload_vec(sgeq);
-            LarfbTask<S>* d_sapp = upload_vec(sapp);
+            LarfbTask<S>* d_sapp = upload_vec(sapp_c);
+            LarfbTask<S>* d_sapp_t = upload_vec(sapp_t);
             CopyTask<S>* d_gat = upload_vec(gat);
             long ntpm = 0;
             for (int q = 0; q < ncolors; ++q) ntpm += (long)tpm[q].size();
             if (!geq.empty()) timed(P_GEQRT, 1, (long)geq.size(), [&] { launch_geqrt<S>(d_geq, (int)geq.size(), stream_); });
-            if (!lfb.empty()) timed(P_LARFB, 1, (long)lfb.size(), [&] { launch_larfb<S>(d_lfb, (int)lfb.size(), stream_); });
+            if (!lfb.empty())
+                timed(P_LARFB, 1 + !lfb_t.empty(), (long)lfb.size(), [&] {
+                    launch_larfb<S>(d_lfb, (int)lfb_c.size(), stream_);
+                    launch_larfb<S>(d_lfb_t, (int)lfb_t.size(), stream_);
+                });
             if (!tpq.empty()) timed(P_TPQRT, 1, (long)tpq.size(), [&] { launch_tpqrt<S>(d_tpq, d_tpql, (int)tpq.size(), stream_); });
             if (ntpm)
                 timed(P_TPMQRT, ncolors, ntpm, [&] {
@@ -784,7 +793,11 @@ int Tree::factorize_t() {
                     launch_copy<S>(d_scopy, (int)scopy.size(), stream_);
                     launch_geqrt<S>(d_sgeq, (int)sgeq.size(), stream_);
                 });
-            if (!sapp.empty()) timed(P_SAPPLY, 1, (long)sapp.size(), [&] { launch_larfb<S>(d_sapp, (int)sapp.size(), stream_); });
+            if (!sapp.empty())
+                timed(P_SAPPLY, 1 + !sapp_t.empty(), (long)sapp.size(), [&] {
+                    launch_larfb<S>(d_sapp, (int)sapp_c.size(), stream_);
+                    launch_larfb<S>(d_sapp_t, (int)sapp_t.size(), stream_);
+                });
             if (!gat.empty()) timed(P_GATHER, 1, (long)gat.size(), [&] { launch_copy<S>(d_gat, (int)gat.size(), stream_); });
             if (!gq.empty()) {
                 RrqrAlloc mem{[&](size_t b) { return arena.alloc(b); },

@@ -2,8 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "tree.hpp"
 
@@ -19,6 +21,35 @@ namespace spaqr {
 // geqp3 pivot candidates within this relative band of the largest norm count as tied and
 // the lowest index wins (LAPACK's plain argmax lets rounding order decide exact ties).
 constexpr double kPivotTieRel = 1e-10;
+
+// Splits larfb work so that a large block is spread over many workgroups: Q^T C acts on each
+// column independently, so tasks are cut into column chunks; a fused trailing C R^{-1}
+// mixes columns, so it is moved to a second list of row chunks (V = nullptr: trsm only),
+// to run after the first.
+template <typename S>
+void split_larfb(const std::vector<LarfbTask<S>>& in, std::vector<LarfbTask<S>>& cols_out,
+                 std::vector<LarfbTask<S>>& trsm_out, int col_chunk = 128, int row_chunk = 256) {
+    for (const auto& t : in) {
+        if (t.V)
+            for (int c0 = 0; c0 < t.n; c0 += col_chunk) {
+                LarfbTask<S> u = t;
+                u.C = t.C + (size_t)c0 * t.ldc;
+                u.n = std::min(col_chunk, t.n - c0);
+                u.R = nullptr;
+                u.k2 = 0;
+                cols_out.push_back(u);
+            }
+        if (t.R)
+            for (int r0 = 0; r0 < t.m; r0 += row_chunk) {
+                LarfbTask<S> u = t;
+                u.V = nullptr;
+                u.T = nullptr;
+                u.C = t.C + r0;
+                u.m = std::min(row_chunk, t.m - r0);
+                trsm_out.push_back(u);
+            }
+    }
+}
 
 template <class F>
 void Tree::timed(int cat, long launches, long tasks, F&& f) {

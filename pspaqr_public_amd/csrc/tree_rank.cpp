@@ -169,10 +169,12 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
             prof_add(P_RSKETCH, 4.0 * k * w * r - 2.0 * w * (double)r * r, SZ * ((double)k * r + 2.0 * k * w));
         }
         if (!lb.empty()) {
-            auto* d_lb = upload_vec(lb);
+            std::vector<LarfbTask<S>> lb_c, lb_t;
+            split_larfb(lb, lb_c, lb_t);
+            auto* d_lb = upload_vec(lb_c);
             auto* d_tt = upload_vec(tt);
             timed(P_RSKETCH, 2, (long)lb.size(), [&] {
-                launch_larfb<S>(d_lb, (int)lb.size(), stream_);
+                launch_larfb<S>(d_lb, (int)lb_c.size(), stream_);
                 launch_tail_select<S>(d_tt, (int)tt.size(), tol, stream_);
             });
         }
