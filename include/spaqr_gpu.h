@@ -194,7 +194,8 @@ int spaqr_geqp3_batched(int prec, int count, const spaqr_geqp3_desc* d, double t
 
 /* Sparsification apply + scatter (src/ptree.cpp:1867-1897 then :330): A (mc x nc) <- A Q_n
    (Vn != NULL, first rn reflectors of Vn kn x rn), A[0:kc, 0:rn] <- Q_c^T A (Vc != NULL,
-   first rc reflectors), dst(i, j) = A(i, j) for i < rc, j < rn; ident != 0 writes I_rc. */
+   first rc reflectors), dst(i, j) = A(i, j) for i < rc, j < rn; ident != 0 writes I_rc.
+   pad = 0 runs every phase; 1 only A <- A Q_n; 2 only the left transform and the scatter. */
 typedef struct {
     void* A; const void* Vn; const void* Tn; const void* Vc; const void* Tc; void* dst;
     int mc, nc, lda, kn, rn, kc, rc, ldvn, ldvc, ldd, ident, pad;

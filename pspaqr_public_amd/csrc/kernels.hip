@@ -677,13 +677,15 @@ __global__ __launch_bounds__(WG) void k_spars_apply(const SparsApplyTask<S>* __r
         }
         return;
     }
-    if (t.Vn) {
+    const bool right = t.pad == 0 || t.pad == 1, left = t.pad == 0 || t.pad == 2;
+    if (t.Vn && right) {
         for (int p0 = 0; p0 < t.rn; p0 += NB) {
             const int nbp = min(NB, t.rn - p0);
             wg_apply_rn<S>(t.mc, t.kn - p0, nbp, t.Vn + p0 + (size_t)p0 * t.ldvn, t.ldvn, t.Tn + (size_t)p0 * NB, NB,
                            t.A + (size_t)p0 * t.lda, t.lda, asm_);
         }
     }
+    if (!left) return;
     if (t.Vc) {
         for (int p0 = 0; p0 < t.rc; p0 += NB) {
             const int nbp = min(NB, t.rc - p0);

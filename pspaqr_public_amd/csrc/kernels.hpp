@@ -57,6 +57,7 @@ template <typename S> struct CopyTask { const S* src; S* dst; int rows, cols, ld
 // Sparsify apply of one interface-interface block A (mc x nc, lda):
 // A <- A Q_n (if Vn), A[0:mc, 0:rn] <- Q_c^T A[0:mc, 0:rn] (if Vc),
 // then dst(rpos + i, cpos + j) = A(i, j) for i < rc, j < rn (ldd), or the identity if ident.
+// pad selects the phases: 0 all, 1 the right apply only, 2 the left apply and the scatter.
 template <typename S> struct SparsApplyTask {
     S* A; const S* Vn; const S* Tn; const S* Vc; const S* Tc; S* dst;
     int mc, nc, lda, kn, rn, kc, rc, ldvn, ldvc, ldd, ident, pad;

@@ -922,9 +922,39 @@ int Tree::factorize_t() {
                 spa.push_back(t);
             }
         }
-        SparsApplyTask<S>* d_spa = upload_vec(spa);
+        // Two launches so that large blocks spread over many workgroups: A Q_n acts on each row
+        // independently (row chunks), Q_c^T A and the scatter on each column (column chunks).
+        std::vector<SparsApplyTask<S>> spa1, spa2;
+        constexpr int kSparsRows = 256, kSparsCols = 128;
+        for (const auto& t : spa) {
+            if (t.ident) {
+                spa2.push_back(t);
+                continue;
+            }
+            if (t.Vn)
+                for (int r0 = 0; r0 < t.mc; r0 += kSparsRows) {
+                    SparsApplyTask<S> u = t;
+                    u.A = t.A + r0;
+                    u.mc = std::min(kSparsRows, t.mc - r0);
+                    u.pad = 1;
+                    spa1.push_back(u);
+                }
+            for (int c0 = 0; c0 < t.rn; c0 += kSparsCols) {
+                SparsApplyTask<S> u = t;
+                u.A = t.A + (size_t)c0 * t.lda;
+                u.dst = t.dst + (size_t)c0 * t.ldd;
+                u.rn = std::min(kSparsCols, t.rn - c0);
+                u.pad = 2;
+                spa2.push_back(u);
+            }
+        }
+        SparsApplyTask<S>* d_spa1 = upload_vec(spa1);
+        SparsApplyTask<S>* d_spa2 = upload_vec(spa2);
         if (!spa.empty())
-            timed(P_SPARS, 1, (long)spa.size(), [&] { launch_spars_apply<S>(d_spa, (int)spa.size(), stream_); });
+            timed(P_SPARS, 2, (long)spa.size(), [&] {
+                launch_spars_apply<S>(d_spa1, (int)spa1.size(), stream_);
+                launch_spars_apply<S>(d_spa2, (int)spa2.size(), stream_);
+            });
         HIPCHK(hipGetLastError());
     }
     sync_stream();
