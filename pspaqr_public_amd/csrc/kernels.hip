@@ -28,6 +28,20 @@ struct ApplySmem {
     S W[32 * 33];
     S Ts[32 * 33];
 };
+// FP32 applies run on MFMA: a 64-row x 128-column C tile and per-wave W blocks.
+template <>
+struct ApplySmem<float> {
+    float Vs[32 * 65];
+    float Cs[128 * 65];
+    float W[32 * 33];
+    float Ts[32 * 33];
+    float Ws[4 * 32 * 33];
+};
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+// v_mfma_f32_32x32x2f32: lane l supplies A(l % 32, l / 32) and B(l / 32, l % 32); accumulator
+// entry r of lane l is D(mfma_row(l, r), l % 32).
+__device__ __forceinline__ int mfma_row(int l, int r) { return (r >> 2) * 8 + (l >> 5) * 4 + (r & 3); }
 
 template <typename S>
 struct PanelSmem {
@@ -124,6 +138,129 @@ __device__ __forceinline__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B
 // ---------------------------------------------------------------------------------------
 // [Atop; C] <- H^T [Atop; C] (PENT, V is the m x kb lower block of [I; V]) or
 // C <- H^T C (unit lower V, m x kb, implicit ones / zeros above).  kb <= 32.
+// H^T [Atop; C] (or H^T C) on MFMA, FP32: for each 128-column chunk of C (wave w owns its
+// 32-column slice), W = Atop + V^T C accumulates over 64-row tiles of V and C staged in LDS,
+// W <- T^T W, Atop -= W, then C -= V W tile by tile.  Every MFMA is a 32x32x2 step.
+template <bool PENT>
+__device__ __forceinline__ void wg_apply_lt_mfma(int m, int kb, const float* __restrict__ V, int ldv,
+                                                 const float* __restrict__ T, int ldt, float* Atop, int lda, float* C,
+                                                 int ldc, int ncols, ApplySmem<float>& sm) {
+    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+    const int li = l & 31, lk = l >> 5;
+    float* Ws = sm.Ws + w * (32 * 33);
+    __syncthreads();
+    for (int e = tid; e < 32 * 32; e += WG) {  // Ts[i * 33 + k] = T(k, i), upper
+        const int k = e & 31, i = e >> 5;
+        const float tv = T[min(k, kb - 1) + (size_t)min(i, kb - 1) * ldt];
+        sm.Ts[i * 33 + k] = (k <= i && i < kb) ? tv : 0.f;
+    }
+    auto stage = [&](int r0, int rb, int c0, int nc) {
+        float vv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + q * WG, r = e & 63, i = e >> 6;
+            vv[q] = V[(r0 + min(r, rb - 1)) + (size_t)min(i, kb - 1) * ldv];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + q * WG, r = e & 63, i = e >> 6, R = r0 + r;
+            float v = 0.f;
+            if (r < rb && i < kb) v = PENT ? vv[q] : ((R > i) ? vv[q] : (R == i ? 1.f : 0.f));
+            sm.Vs[i * 65 + r] = v;
+        }
+#pragma unroll 4
+        for (int q0 = 0; q0 < 32; q0 += 8) {
+            float cv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int e = tid + (q0 + q) * WG, r = e & 63, j = e >> 6;
+                cv[q] = C[(r0 + min(r, rb - 1)) + (size_t)(c0 + min(j, nc - 1)) * ldc];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int e = tid + (q0 + q) * WG, r = e & 63, j = e >> 6;
+                sm.Cs[j * 65 + r] = (r < rb && j < nc) ? cv[q] : 0.f;
+            }
+        }
+    };
+    for (int c0 = 0; c0 < ncols; c0 += 128) {
+        const int nc = min(128, ncols - c0);
+        const int jc = 32 * w + li;  // this lane's column inside the chunk
+        floatx16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            float a = 0.f;
+            if (PENT) {
+                const int i = mfma_row(l, r);
+                const float av = Atop[min(i, kb - 1) + (size_t)(c0 + min(jc, nc - 1)) * lda];
+                a = (i < kb && jc < nc) ? av : 0.f;
+            }
+            acc[r] = a;
+        }
+        for (int r0 = 0; r0 < m; r0 += 64) {
+            const int rb = min(64, m - r0);
+            __syncthreads();
+            stage(r0, rb, c0, nc);
+            __syncthreads();
+#pragma unroll 8
+            for (int kk = 0; kk < 64; kk += 2) {
+                const float a = sm.Vs[li * 65 + kk + lk];
+                const float b = sm.Cs[jc * 65 + kk + lk];
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+            }
+        }
+        // W <- T^T W (W through the wave's LDS block, column-major by j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Ws[li * 33 + mfma_row(l, r)] = acc[r];
+        __syncthreads();
+        floatx16 w2;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) w2[r] = 0.f;
+#pragma unroll
+        for (int kk = 0; kk < 32; kk += 2) {
+            const float a = sm.Ts[li * 33 + kk + lk];
+            const float b = Ws[li * 33 + kk + lk];
+            w2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, w2, 0, 0, 0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = mfma_row(l, r);
+            const float v = (i < kb) ? w2[r] : 0.f;
+            Ws[li * 33 + i] = v;
+            if (PENT && i < kb && jc < nc) Atop[i + (size_t)(c0 + jc) * lda] -= v;
+        }
+        // C -= V W, tile by tile
+        for (int r0 = 0; r0 < m; r0 += 64) {
+            const int rb = min(64, m - r0);
+            __syncthreads();
+            stage(r0, rb, c0, nc);
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                floatx16 d;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) d[r] = 0.f;
+#pragma unroll
+                for (int kk = 0; kk < 32; kk += 2) {
+                    const float a = sm.Vs[(kk + lk) * 65 + h * 32 + li];
+                    const float b = Ws[li * 33 + kk + lk];
+                    d = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, d, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 16; ++r) sm.Cs[jc * 65 + h * 32 + mfma_row(l, r)] -= d[r];
+            }
+            __syncthreads();
+#pragma unroll 4
+            for (int q = 0; q < 32; ++q) {
+                const int e = tid + q * WG, r = e & 63, j = e >> 6;
+                if (r < rb && j < nc) C[(r0 + r) + (size_t)(c0 + j) * ldc] = sm.Cs[j * 65 + r];
+            }
+        }
+    }
+    __syncthreads();
+}
+
 template <typename S, bool PENT>
 __device__ __forceinline__ void wg_apply_lt(int m, int kb, const S* __restrict__ V, int ldv, const S* __restrict__ T, int ldt,
                             S* Atop, int lda, S* C, int ldc, int ncols, ApplySmem<S>& sm) {
@@ -131,6 +268,10 @@ __device__ __forceinline__ void wg_apply_lt(int m, int kb, const S* __restrict__
     // thread's loads issue back to back instead of one memory latency each.
     const int tid = threadIdx.x;
     if (m <= 0 || kb <= 0) return;
+    if constexpr (sizeof(S) == 4) {
+        wg_apply_lt_mfma<PENT>(m, kb, V, ldv, T, ldt, Atop, lda, C, ldc, ncols, sm);
+        return;
+    }
     __syncthreads();
     for (int e = tid; e < 32 * 32; e += WG) {
         int i = e & 31, j = e >> 5;
