@@ -104,4 +104,33 @@ __device__ __forceinline__ S row16_scan(S v) {
     return v;
 }
 
+// Sum the 32 per-lane values d[0..31] over the wave.  On return lane L holds the total of
+// column vec_col(L) (lanes L and L^1 hold the same column): a transposing butterfly, 32 shuffles.
+__device__ __forceinline__ int vec_col(int lane) {
+    return ((lane >> 5) & 1) * 16 + ((lane >> 4) & 1) * 8 + ((lane >> 3) & 1) * 4 + ((lane >> 2) & 1) * 2 +
+           ((lane >> 1) & 1);
+}
+
+template <int H, typename S>
+__device__ __forceinline__ void butterfly_step(S (&d)[32], int lane) {
+    const bool up = (lane & (2 * H)) != 0;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const S send = up ? d[i] : d[i + H];
+        const S keep = up ? d[i + H] : d[i];
+        d[i] = keep + __shfl_xor(send, 2 * H, 64);
+    }
+}
+
+template <typename S>
+__device__ __forceinline__ S wave_sum32(S (&d)[32]) {
+    const int lane = threadIdx.x & 63;
+    butterfly_step<16>(d, lane);
+    butterfly_step<8>(d, lane);
+    butterfly_step<4>(d, lane);
+    butterfly_step<2>(d, lane);
+    butterfly_step<1>(d, lane);
+    return d[0] + __shfl_xor(d[0], 1, 64);
+}
+
 }  // namespace spaqr

@@ -49,6 +49,8 @@ struct PanelSmem {
     S d[32];
     S t[32];
     S tau, scal;
+    S red[2][4][32];  // per-step column sums of the register-resident panel QR (double-buffered)
+    S rowj[2][32];
 };
 
 // sign(alpha) * ||(alpha, x)|| for a Householder reflector, with an alpha at rounding-noise
@@ -68,7 +70,8 @@ __device__ __forceinline__ S householder_norm_signed(S alpha, S ss) {
 // upper, ldr); B (m x nbp, ldb)], reflector j is [e_j; B(:, j)] (dtpqrt2 with L = 0).
 // The nbp x nbp T of the panel is written to Tout (ld ldt).
 template <typename S, bool PENT>
-__device__ __forceinline__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* Tout, int ldt, PanelSmem<S>& sm) {
+__device__ __forceinline__ void wg_panel_qr_mem(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* Tout, int ldt,
+                                                PanelSmem<S>& sm) {
     const int tid = threadIdx.x;
     const int l = tid >> 3, g = tid & 7;
     for (int e = tid; e < 32 * 33; e += WG) sm.Ts[e] = S(0);
@@ -133,6 +136,143 @@ __device__ __forceinline__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B
         Tout[i + (size_t)jj * ldt] = (i <= jj) ? sm.Ts[jj * 33 + i] : S(0);
     }
     __syncthreads();
+}
+
+// Register-resident panel QR: thread t holds the whole rows t, t + 256, ... (NR of them) of
+// the m x nbp panel for all nbp steps, so a step is one 32-column dot-product reduction
+// (transposing butterfly + LDS, one barrier) and register updates; nothing goes back to
+// memory until the end.  Same reflectors, R and T as wg_panel_qr_mem.
+template <typename S, bool PENT, int NR>
+__device__ __forceinline__ void wg_panel_qr_reg(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* Tout, int ldt,
+                                                PanelSmem<S>& sm) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    S a[NR][32];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        const int R = tid + i * WG, Rc = min(R, m - 1);
+#pragma unroll
+        for (int l = 0; l < 32; ++l) a[i][l] = B[Rc + (size_t)min(l, nbp - 1) * ldb];
+#pragma unroll
+        for (int l = 0; l < 32; ++l) a[i][l] = (R < m && l < nbp) ? a[i][l] : S(0);
+    }
+    for (int e = tid; e < 32 * 33; e += WG) sm.Ts[e] = S(0);
+    __syncthreads();
+    for (int j = 0; j < nbp; ++j) {
+        const int par = j & 1;
+        // PENT: row j of the separate triangle, read before the barrier (written after it)
+        S rj[32];
+        if (PENT) {
+#pragma unroll
+            for (int l = 0; l < 32; ++l) rj[l] = Rt[j + (size_t)min(l, nbp - 1) * ldr];
+        }
+        S vj[NR];
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            S v = S(0);
+#pragma unroll
+            for (int l = 0; l < 32; ++l) v = (l == j) ? a[i][l] : v;
+            vj[i] = v;
+        }
+        S d[32];
+#pragma unroll
+        for (int l = 0; l < 32; ++l) d[l] = S(0);
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const int R = tid + i * WG;
+            const S x = (R < m && (PENT || R > j)) ? vj[i] : S(0);
+#pragma unroll
+            for (int l = 0; l < 32; ++l) d[l] += x * a[i][l];
+        }
+        if (!PENT && tid == j) {
+#pragma unroll
+            for (int l = 0; l < 32; ++l) sm.rowj[par][l] = a[0][l];
+        }
+        const S tot = wave_sum32<S>(d);
+        if ((lane & 1) == 0) sm.red[par][wave][vec_col(lane)] = tot;
+        __syncthreads();
+        S dl[32];
+#pragma unroll
+        for (int l = 0; l < 32; ++l)
+            dl[l] = (sm.red[par][0][l] + sm.red[par][1][l]) + (sm.red[par][2][l] + sm.red[par][3][l]);
+        if (!PENT) {
+#pragma unroll
+            for (int l = 0; l < 32; ++l) rj[l] = sm.rowj[par][l];
+        }
+        S alpha = S(0), ss = S(0);
+#pragma unroll
+        for (int l = 0; l < 32; ++l) {
+            alpha = (l == j) ? rj[l] : alpha;
+            ss = (l == j) ? dl[l] : ss;
+        }
+        S tau, scal, beta;
+        if (ss == S(0)) {
+            tau = S(0); scal = S(1); beta = alpha;
+        } else {
+            beta = -householder_norm_signed(alpha, ss);
+            tau = (beta - alpha) / beta;
+            scal = S(1) / (alpha - beta);
+        }
+        S wl[32];
+#pragma unroll
+        for (int l = 0; l < 32; ++l) wl[l] = (l > j && l < nbp) ? (rj[l] + scal * dl[l]) * tau : S(0);
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const int R = tid + i * WG;
+            if (R < m && (PENT || R > j)) {
+                const S vs = vj[i] * scal;
+#pragma unroll
+                for (int l = 0; l < 32; ++l) a[i][l] = (l == j) ? vs : a[i][l] - vs * wl[l];
+            } else if (!PENT && R == j) {
+#pragma unroll
+                for (int l = 0; l < 32; ++l) a[i][l] = (l == j) ? beta : a[i][l] - wl[l];
+            }
+        }
+        if (PENT) {
+            S mine = S(0);
+#pragma unroll
+            for (int l = 0; l < 32; ++l) mine = (l == tid) ? rj[l] - wl[l] : mine;
+            if (tid == j) Rt[j + (size_t)j * ldr] = beta;
+            else if (tid > j && tid < nbp) Rt[j + (size_t)tid * ldr] = mine;
+        }
+        // column j of T: T(q, j) = -tau sum_{p=q}^{j-1} T(q, p) t_p, t_p = v_p^T v_j
+        if (tid < j) {
+            S s = S(0);
+#pragma unroll
+            for (int q = 0; q < 32; ++q) {
+                if (q >= tid && q < j) {
+                    const S tq = scal * dl[q] + (PENT ? S(0) : rj[q]);
+                    s += sm.Ts[q * 33 + tid] * tq;
+                }
+            }
+            sm.Ts[j * 33 + tid] = -tau * s;
+        } else if (tid == j) {
+            sm.Ts[j * 33 + j] = tau;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        const int R = tid + i * WG;
+        if (R < m) {
+#pragma unroll
+            for (int l = 0; l < 32; ++l)
+                if (l < nbp) B[R + (size_t)l * ldb] = a[i][l];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < nbp * nbp; e += WG) {
+        int i = e % nbp, jj = e / nbp;
+        Tout[i + (size_t)jj * ldt] = (i <= jj) ? sm.Ts[jj * 33 + i] : S(0);
+    }
+    __syncthreads();
+}
+
+template <typename S, bool PENT>
+__device__ __forceinline__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* Tout, int ldt,
+                                            PanelSmem<S>& sm) {
+    if (m <= WG) wg_panel_qr_reg<S, PENT, 1>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm);
+    else if (sizeof(S) == 4 && m <= 2 * WG) wg_panel_qr_reg<S, PENT, 2>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm);
+    else if (sizeof(S) == 4 && m <= 4 * WG) wg_panel_qr_reg<S, PENT, 4>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm);
+    else wg_panel_qr_mem<S, PENT>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -594,6 +734,21 @@ __global__ __launch_bounds__(WG) void k_geqrt(const GeqrtTask<S>* __restrict__ t
     }
 }
 
+template <typename S, bool PENT>
+__global__ __launch_bounds__(WG) void k_panel_qr(const PanelTask<S>* __restrict__ tasks) {
+    __shared__ PanelSmem<S> psm;
+    const PanelTask<S> t = tasks[blockIdx.x];
+    if (PENT) wg_panel_qr<S, true>(t.m, t.nbp, t.A, t.lda, t.B, t.ldb, t.T, NB, psm);
+    else wg_panel_qr<S, false>(t.m, t.nbp, t.A, t.lda, t.A, t.lda, t.T, NB, psm);
+}
+
+template <typename S, bool PENT>
+__global__ __launch_bounds__(WG) void k_panel_apply(const PanelApplyTask<S>* __restrict__ tasks) {
+    __shared__ ApplySmem<S> asm_;
+    const PanelApplyTask<S> t = tasks[blockIdx.x];
+    wg_apply_lt<S, PENT>(t.m, t.kb, t.V, t.ldv, t.T, NB, t.Atop, t.lda, t.C, t.ldc, t.ncols, asm_);
+}
+
 template <typename S>
 __global__ __launch_bounds__(WG) void k_larfb(const LarfbTask<S>* __restrict__ tasks) {
     __shared__ ApplySmem<S> asm_;
@@ -863,35 +1018,6 @@ struct VecSmem {
     S xb[32];       // triangular-solve block
     S T[32 * 33];   // the panel's T block, column l at T[l * 33]
 };
-
-// Sum the 32 per-lane values d[0..31] over the wave.  On return lane L holds the total of
-// column vec_col(L) (lanes L and L^1 hold the same column): a transposing butterfly, 32 shuffles.
-__device__ __forceinline__ int vec_col(int lane) {
-    return ((lane >> 5) & 1) * 16 + ((lane >> 4) & 1) * 8 + ((lane >> 3) & 1) * 4 + ((lane >> 2) & 1) * 2 +
-           ((lane >> 1) & 1);
-}
-
-template <int H, typename S>
-__device__ __forceinline__ void butterfly_step(S (&d)[32], int lane) {
-    const bool up = (lane & (2 * H)) != 0;
-#pragma unroll
-    for (int i = 0; i < H; ++i) {
-        const S send = up ? d[i] : d[i + H];
-        const S keep = up ? d[i + H] : d[i];
-        d[i] = keep + __shfl_xor(send, 2 * H, 64);
-    }
-}
-
-template <typename S>
-__device__ __forceinline__ S wave_sum32(S (&d)[32]) {
-    const int lane = threadIdx.x & 63;
-    butterfly_step<16>(d, lane);
-    butterfly_step<8>(d, lane);
-    butterfly_step<4>(d, lane);
-    butterfly_step<2>(d, lane);
-    butterfly_step<1>(d, lane);
-    return d[0] + __shfl_xor(d[0], 1, 64);
-}
 
 // One panel (kb <= 32 reflectors) of a block reflector H = I - V T V^T applied to a vector:
 // x <- H^T x (trans) or H x.  T is the panel's upper-triangular kb x kb block (ld NB).
@@ -1194,6 +1320,18 @@ void launch_geqrt(const GeqrtTask<S>* d, int n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_geqrt<S>, dim3(n), dim3(WG), 0, s, d);
 }
 template <typename S>
+void launch_panel_qr(const PanelTask<S>* d, int n, bool pent, hipStream_t s) {
+    if (n <= 0) return;
+    if (pent) hipLaunchKernelGGL((k_panel_qr<S, true>), dim3(n), dim3(WG), 0, s, d);
+    else hipLaunchKernelGGL((k_panel_qr<S, false>), dim3(n), dim3(WG), 0, s, d);
+}
+template <typename S>
+void launch_panel_apply(const PanelApplyTask<S>* d, int n, bool pent, hipStream_t s) {
+    if (n <= 0) return;
+    if (pent) hipLaunchKernelGGL((k_panel_apply<S, true>), dim3(n), dim3(WG), 0, s, d);
+    else hipLaunchKernelGGL((k_panel_apply<S, false>), dim3(n), dim3(WG), 0, s, d);
+}
+template <typename S>
 void launch_larfb(const LarfbTask<S>* d, int n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_larfb<S>, dim3(n), dim3(WG), 0, s, d);
 }
@@ -1275,6 +1413,8 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
 #define SPAQR_INSTANTIATE(S)                                                                          \
     template void launch_geqrt<S>(const GeqrtTask<S>*, int, hipStream_t);                            \
     template void launch_larfb<S>(const LarfbTask<S>*, int, hipStream_t);                            \
+    template void launch_panel_qr<S>(const PanelTask<S>*, int, bool, hipStream_t);                    \
+    template void launch_panel_apply<S>(const PanelApplyTask<S>*, int, bool, hipStream_t);           \
     template void launch_tpqrt<S>(const TpqrtTask<S>*, const TpqrtLink<S>*, int, hipStream_t);       \
     template void launch_tpmqrt<S>(const TpmqrtTask<S>*, const TpmqrtLink<S>*, int, hipStream_t);    \
     template void launch_copy<S>(const CopyTask<S>*, int, hipStream_t);                              \

@@ -51,6 +51,15 @@ template <typename S> struct Geqp3CoopTask {
 };
 inline size_t geqp3_coop_xch_elems(int k, int P) { return 2 * (size_t)P * (k + 4); }
 
+// One panel step of a large front, split in two launches: the panel QR (one workgroup per
+// front) and the trailing update over column chunks (many workgroups per front).
+//   PENT = false (geqrt): the panel is A (m x nbp, lda) from its diagonal entry, T out (ld NB).
+//   PENT = true (tpqrt link): the panel is [R (nbp x nbp, ldr = lda); B (m x nbp, ldb)].
+template <typename S> struct PanelTask { S* A; S* B; S* T; int m, nbp, lda, ldb; };
+// [Atop (kb rows, lda); C (m rows, ldc)] <- H^T [...] over ncols columns (Atop unused when
+// PENT = false); V (m x kb, ldv) and T (kb x kb, ld NB) are the panel's reflectors.
+template <typename S> struct PanelApplyTask { const S* V; const S* T; S* Atop; S* C; int m, kb, ldv, lda, ldc, ncols; };
+
 // Gather/scatter of a sub-block: dst(i, j) = src(i, j) or src(j, i) (transpose).
 template <typename S> struct CopyTask { const S* src; S* dst; int rows, cols, lds, ldd, trans, pad; };
 

@@ -11,6 +11,8 @@ namespace spaqr {
 
 template <typename S> void launch_geqrt(const GeqrtTask<S>* d, int n, hipStream_t s);
 template <typename S> void launch_larfb(const LarfbTask<S>* d, int n, hipStream_t s);
+template <typename S> void launch_panel_qr(const PanelTask<S>* d, int n, bool pent, hipStream_t s);
+template <typename S> void launch_panel_apply(const PanelApplyTask<S>* d, int n, bool pent, hipStream_t s);
 template <typename S> void launch_tpqrt(const TpqrtTask<S>* d, const TpqrtLink<S>* l, int n, hipStream_t s);
 template <typename S> void launch_tpmqrt(const TpmqrtTask<S>* d, const TpmqrtLink<S>* l, int n, hipStream_t s);
 template <typename S> void launch_copy(const CopyTask<S>* d, int n, hipStream_t s);

@@ -23,6 +23,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <set>
@@ -537,17 +538,27 @@ int Tree::assemble() {
 // edges are appended to ed (their device blocks are allocated by the caller).
 void Tree::fillin() {
     for (int c : interiors[current_bottom]) {
-        std::set<int> d2(cl[c].dist2.begin(), cl[c].dist2.end());
-        for (int e : cl[c].in) d2.insert(ed[e].n1);
-        cl[c].dist2.assign(d2.begin(), d2.end());
+        std::vector<int>& d2 = cl[c].dist2;
+        for (int e : cl[c].in) d2.push_back(ed[e].n1);
+        std::sort(d2.begin(), d2.end());
+        d2.erase(std::unique(d2.begin(), d2.end()), d2.end());
     }
+    // edge existence through a stamp array: mark[n2] == stamp when nn already has an edge to n2
+    std::vector<int> mark(cl.size(), -1);
+    int stamp = 0;
     for (int c : interiors[current_bottom]) {
-        std::vector<int> outs = cl[c].out;
-        for (int nn : cl[c].dist2)
+        const std::vector<int> outs = cl[c].out;
+        for (int nn : cl[c].dist2) {
+            ++stamp;
+            for (int e : cl[nn].out) mark[ed[e].n2] = stamp;
             for (int e : outs) {
-                int nbr = ed[e].n2;
-                if (find_out(cl[nn], nbr) < 0) new_edge(nn, nbr);
+                const int nbr = ed[e].n2;
+                if (mark[nbr] != stamp) {
+                    new_edge(nn, nbr);
+                    mark[nbr] = stamp;
+                }
             }
+        }
     }
 }
 
@@ -576,6 +587,15 @@ int Tree::factorize_t() {
         HIPCHK(hipMemcpyAsync(base, d_blocks0, h_blocks0.size() * sizeof(S), hipMemcpyDeviceToDevice, stream_));
         for (size_t i = 0; i < ed.size(); ++i) ed[i].A = base + h_block_off[i];
     }
+    // host-phase timers (SPAQR_DEBUG_HOST=1 prints them per factorization)
+    static const bool dbg_host = getenv("SPAQR_DEBUG_HOST") != nullptr;
+    double hms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    auto hclock = [] { return std::chrono::steady_clock::now(); };
+    auto hacc = [&](int i, std::chrono::steady_clock::time_point& t) {
+        auto n = std::chrono::steady_clock::now();
+        hms[i] += std::chrono::duration<double, std::milli>(n - t).count();
+        t = n;
+    };
     auto zalloc = [&](size_t nel) -> S* {
         S* p = (S*)arena.alloc(std::max<size_t>(nel, 1) * sizeof(S));
         launch_fill<S>(p, nel, S(0), stream_);
@@ -586,11 +606,13 @@ int Tree::factorize_t() {
     auto Ap = [&](int id) -> S* { return (S*)ed[id].A; };
 
     for (int L = 0; L < nlevels; ++L) {
+        auto ht = hclock();
         current_bottom = L;
         LevelStats st;
         st.level = L;
         const size_t ne_before = ed.size();
         fillin();
+        hacc(0, ht);
         {
             size_t tot = 0;
             for (size_t i = ne_before; i < ed.size(); ++i) tot += (size_t)ed[i].rows * ed[i].cols;
@@ -641,13 +663,13 @@ int Tree::factorize_t() {
             tpq.push_back(tk);
         }
         // colour layers for the shared-block updates (order-consistent)
-        std::map<int, int> last_color;
+        std::vector<int> last_color(cl.size(), -1);
         int ncolors = 0;
         for (int c : ints) {
             int col = 0;
             for (size_t j = 1; j < cl[c].out.size(); ++j) {
-                auto it = last_color.find(ed[cl[c].out[j]].n2);
-                if (it != last_color.end()) col = std::max(col, it->second + 1);
+                const int lc = last_color[ed[cl[c].out[j]].n2];
+                if (lc >= 0) col = std::max(col, lc + 1);
             }
             cl[c].color = col;
             for (size_t j = 1; j < cl[c].out.size(); ++j) last_color[ed[cl[c].out[j]].n2] = col;
@@ -657,21 +679,24 @@ int Tree::factorize_t() {
         std::vector<std::vector<TpmqrtTask<S>>> tpm(ncolors);
         std::vector<TpmqrtLink<S>> tpml;
         const int TILE = 64;
+        std::vector<int> mslot(cl.size(), -1);  // out edge of m by target
         for (int c : ints) {
             Cluster& C = cl[c];
             for (int e : C.in) {
                 Edge& cm = E(e);
                 const int m = cm.n1;
                 const int link0 = (int)tpml.size();
+                for (int f : cl[m].out) mslot[ed[f].n2] = f;
                 for (size_t j = 1; j < C.out.size(); ++j) {
                     Edge& cn = E(C.out[j]);
-                    int mn = find_out(cl[m], cn.n2);
+                    int mn = mslot[cn.n2];
                     if (mn < 0) throw std::runtime_error("missing fill-in block");
                     tpml.push_back({Ap(C.out[j]), (const S*)cn.T, Ap(mn), cn.rows, cn.rows, ed[mn].rows, 0});
                     st.flops += 4.0 * cn.rows * C.cols * cm.cols;
                     prof_add(P_TPMQRT, 4.0 * cn.rows * C.cols * cm.cols,
                              SZ * ((double)cn.rows * C.cols + NB * C.cols + 2.0 * C.cols * cm.cols + 2.0 * cn.rows * cm.cols));
                 }
+                for (int f : cl[m].out) mslot[ed[f].n2] = -1;
                 const int nl = (int)tpml.size() - link0;
                 if (nl == 0) continue;
                 for (int c0 = 0; c0 < cm.cols; c0 += TILE)
@@ -757,42 +782,68 @@ int Tree::factorize_t() {
                 gq_cluster.push_back(who[q]);
             }
         }
+        hacc(1, ht);
         // ---- upload and launch
         {
-            GeqrtTask<S>* d_geq = upload_vec(geq);
+            std::vector<GeqrtTask<S>> geq_small, geq_big, sgeq_small, sgeq_big;
+            for (const auto& t : geq) (std::min(t.m, t.n) >= kBigFront ? geq_big : geq_small).push_back(t);
+            for (const auto& t : sgeq) (std::min(t.m, t.n) >= kBigFront ? sgeq_big : sgeq_small).push_back(t);
+            std::vector<TpqrtTask<S>> tpq_small, tpq_big;
+            // tpqrt chains stay on the one-workgroup kernel: with the register-resident panel QR
+            // the host-driven panel loop (tpqrt_big) costs more in launch gaps than it gains
+            for (const auto& t : tpq) tpq_small.push_back(t);
+            GeqrtTask<S>* d_geq = upload_vec(geq_small);
             std::vector<LarfbTask<S>> lfb_c, lfb_t, sapp_c, sapp_t;
             split_larfb(lfb, lfb_c, lfb_t);
             split_larfb(sapp, sapp_c, sapp_t);
             LarfbTask<S>* d_lfb = upload_vec(lfb_c);
             LarfbTask<S>* d_lfb_t = upload_vec(lfb_t);
-            TpqrtTask<S>* d_tpq = upload_vec(tpq);
+            TpqrtTask<S>* d_tpq = upload_vec(tpq_small);
             TpqrtLink<S>* d_tpql = upload_vec(tpql);
             TpmqrtLink<S>* d_tpml = upload_vec(tpml);
             std::vector<TpmqrtTask<S>*> d_tpm(ncolors);
             for (int q = 0; q < ncolors; ++q) d_tpm[q] = upload_vec(tpm[q]);
             CopyTask<S>* d_scopy = upload_vec(scopy);
-            GeqrtTask<S>* d_sgeq = upload_vec(sgeq);
+            GeqrtTask<S>* d_sgeq = upload_vec(sgeq_small);
             LarfbTask<S>* d_sapp = upload_vec(sapp_c);
             LarfbTask<S>* d_sapp_t = upload_vec(sapp_t);
             CopyTask<S>* d_gat = upload_vec(gat);
             long ntpm = 0;
             for (int q = 0; q < ncolors; ++q) ntpm += (long)tpm[q].size();
-            if (!geq.empty()) timed(P_GEQRT, 1, (long)geq.size(), [&] { launch_geqrt<S>(d_geq, (int)geq.size(), stream_); });
+            if (!geq.empty()) {
+                int nl = 0;
+                timed(P_GEQRT, 0, (long)geq.size(), [&] {
+                    launch_geqrt<S>(d_geq, (int)geq_small.size(), stream_);
+                    nl = !geq_small.empty() + geqrt_big<S>(geq_big);
+                });
+                if (profiling) prof[P_GEQRT].launches += nl;
+            }
             if (!lfb.empty())
                 timed(P_LARFB, 1 + !lfb_t.empty(), (long)lfb.size(), [&] {
                     launch_larfb<S>(d_lfb, (int)lfb_c.size(), stream_);
                     launch_larfb<S>(d_lfb_t, (int)lfb_t.size(), stream_);
                 });
-            if (!tpq.empty()) timed(P_TPQRT, 1, (long)tpq.size(), [&] { launch_tpqrt<S>(d_tpq, d_tpql, (int)tpq.size(), stream_); });
+            if (!tpq.empty()) {
+                int nl = 0;
+                timed(P_TPQRT, 0, (long)tpq.size(), [&] {
+                    launch_tpqrt<S>(d_tpq, d_tpql, (int)tpq_small.size(), stream_);
+                    nl = !tpq_small.empty() + tpqrt_big<S>(tpq_big, tpql);
+                });
+                if (profiling) prof[P_TPQRT].launches += nl;
+            }
             if (ntpm)
                 timed(P_TPMQRT, ncolors, ntpm, [&] {
                     for (int q = 0; q < ncolors; ++q) launch_tpmqrt<S>(d_tpm[q], d_tpml, (int)tpm[q].size(), stream_);
                 });
-            if (!sgeq.empty())
-                timed(P_SGEQRT, 2, (long)sgeq.size(), [&] {
+            if (!sgeq.empty()) {
+                int nl = 0;
+                timed(P_SGEQRT, 0, (long)sgeq.size(), [&] {
                     launch_copy<S>(d_scopy, (int)scopy.size(), stream_);
-                    launch_geqrt<S>(d_sgeq, (int)sgeq.size(), stream_);
+                    launch_geqrt<S>(d_sgeq, (int)sgeq_small.size(), stream_);
+                    nl = 1 + !sgeq_small.empty() + geqrt_big<S>(sgeq_big);
                 });
+                if (profiling) prof[P_SGEQRT].launches += nl;
+            }
             if (!sapp.empty())
                 timed(P_SAPPLY, 1 + !sapp_t.empty(), (long)sapp.size(), [&] {
                     launch_larfb<S>(d_sapp, (int)sapp_c.size(), stream_);
@@ -842,6 +893,7 @@ int Tree::factorize_t() {
             }
             st.n_sparsified = (int)gq.size();
         }
+        hacc(2, ht);
         stats.push_back(st);
         if (L == nlevels - 1) break;
 
@@ -862,18 +914,25 @@ int Tree::factorize_t() {
             cl[p].rank = cs;
         }
         const size_t ne_merge = ed.size();
-        for (int p : bottoms[Lp]) {
-            std::set<int> nbrs;
-            std::set<int> d2(cl[p].dist2.begin(), cl[p].dist2.end());
-            for (int ch : cl[p].children) {
-                for (int e : cl[ch].out)
-                    if (!cl[ed[e].n2].eliminated) nbrs.insert(cl[ed[e].n2].parent);
-                for (int d : cl[ch].dist2)
-                    if (cl[d].lvl() >= Lp) d2.insert(cl[d].parent);
+        {
+            std::vector<int> nbrs, d2;
+            for (int p : bottoms[Lp]) {
+                nbrs.clear();
+                d2.assign(cl[p].dist2.begin(), cl[p].dist2.end());
+                for (int ch : cl[p].children) {
+                    for (int e : cl[ch].out)
+                        if (!cl[ed[e].n2].eliminated) nbrs.push_back(cl[ed[e].n2].parent);
+                    for (int d : cl[ch].dist2)
+                        if (cl[d].lvl() >= Lp) d2.push_back(cl[d].parent);
+                }
+                std::sort(d2.begin(), d2.end());
+                d2.erase(std::unique(d2.begin(), d2.end()), d2.end());
+                cl[p].dist2 = d2;
+                std::sort(nbrs.begin(), nbrs.end());
+                nbrs.erase(std::unique(nbrs.begin(), nbrs.end()), nbrs.end());
+                for (int nn : nbrs) new_edge(p, nn);
+                sort_out(cl[p]);
             }
-            cl[p].dist2.assign(d2.begin(), d2.end());
-            for (int nn : nbrs) new_edge(p, nn);
-            sort_out(cl[p]);
         }
         {
             size_t tot = 0;
@@ -885,16 +944,25 @@ int Tree::factorize_t() {
                 off += (size_t)ed[i].rows * ed[i].cols;
             }
         }
+        hacc(3, ht);
         std::vector<SparsApplyTask<S>> spa;
+        std::vector<int> slot(cl.size(), -1);  // parent's out edge by target, for the parent being filled
+        int slot_owner = -1;
         for (int so : itf) {
             const Cluster& SO = cl[so];
             const int pso = SO.parent;
+            if (pso != slot_owner) {
+                if (slot_owner >= 0)
+                    for (int e : cl[slot_owner].out) slot[ed[e].n2] = -1;
+                for (int e : cl[pso].out) slot[ed[e].n2] = e;
+                slot_owner = pso;
+            }
             for (int e : SO.out) {
                 const Edge& eo = ed[e];
                 const int no = eo.n2;
                 if (cl[no].eliminated) continue;
                 const Cluster& NO = cl[no];
-                int f = find_out(cl[pso], NO.parent);
+                int f = slot[NO.parent];
                 if (f < 0) throw std::runtime_error("merge: parent edge missing");
                 S* dst = Ap(f) + NO.rpos + (size_t)SO.cpos * ed[f].rows;
                 SparsApplyTask<S> t{};
@@ -956,13 +1024,104 @@ int Tree::factorize_t() {
                 launch_spars_apply<S>(d_spa2, (int)spa2.size(), stream_);
             });
         HIPCHK(hipGetLastError());
+        hacc(4, ht);
     }
+    auto ht2 = hclock();
     sync_stream();
+    hacc(5, ht2);
     factorized = true;
     int rc = build_solve_schedule<S>();
     sync_stream();
+    hacc(6, ht2);
+    if (dbg_host)
+        fprintf(stderr, "[host] fillin %.1f  descr %.1f  launch+ranks %.1f  merge %.1f  spars %.1f  tail-sync %.1f  solve-sched %.1f ms\n",
+                hms[0], hms[1], hms[2], hms[3], hms[4], hms[5], hms[6]);
     t_factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
+}
+
+// ---------------------------------------------------------------------------- large fronts
+// A front of k >= kBigFront columns would keep one workgroup busy for tens of milliseconds
+// (its panels are sequential).  Its panel loop runs on the host instead: per 32-column panel
+// one launch factors the panel of every large front (one workgroup each) and one launch
+// applies it to the trailing columns in 128-column chunks across many workgroups.  The
+// arithmetic per column is that of the one-workgroup kernels.
+
+template <typename S>
+int Tree::geqrt_big(const std::vector<GeqrtTask<S>>& big) {
+    if (big.empty()) return 0;
+    int kmax = 0;
+    for (const auto& t : big) kmax = std::max(kmax, std::min(t.m, t.n));
+    // every step's descriptors are built and uploaded at once; step s launches its slice
+    std::vector<PanelTask<S>> pq;
+    std::vector<PanelApplyTask<S>> pa;
+    std::vector<int> qoff, aoff;
+    for (int p0 = 0; p0 < kmax; p0 += NB) {
+        qoff.push_back((int)pq.size());
+        aoff.push_back((int)pa.size());
+        for (const auto& t : big) {
+            const int kmin = std::min(t.m, t.n);
+            if (p0 >= kmin) continue;
+            const int nbp = std::min(NB, kmin - p0);
+            S* Ap = t.A + p0 + (size_t)p0 * t.lda;
+            pq.push_back({Ap, Ap, t.T + (size_t)p0 * NB, t.m - p0, nbp, t.lda, t.lda});
+            for (int c0 = p0 + nbp; c0 < t.n; c0 += 128)
+                pa.push_back({Ap, t.T + (size_t)p0 * NB, nullptr, t.A + p0 + (size_t)c0 * t.lda, t.m - p0, nbp, t.lda, 0,
+                              t.lda, std::min(128, t.n - c0)});
+        }
+    }
+    qoff.push_back((int)pq.size());
+    aoff.push_back((int)pa.size());
+    auto* dq = upload_vec(pq);
+    auto* da = upload_vec(pa);
+    int launches = 0;
+    for (size_t st = 0; st + 1 < qoff.size(); ++st) {
+        launch_panel_qr<S>(dq + qoff[st], qoff[st + 1] - qoff[st], false, stream_);
+        launch_panel_apply<S>(da + aoff[st], aoff[st + 1] - aoff[st], false, stream_);
+        launches += 1 + (aoff[st + 1] > aoff[st]);
+    }
+    return launches;
+}
+
+template <typename S>
+int Tree::tpqrt_big(const std::vector<TpqrtTask<S>>& big, const std::vector<TpqrtLink<S>>& links) {
+    if (big.empty()) return 0;
+    int maxl = 0;
+    for (const auto& t : big) maxl = std::max(maxl, t.nlinks);
+    std::vector<PanelTask<S>> pq;
+    std::vector<PanelApplyTask<S>> pa;
+    std::vector<int> qoff, aoff;
+    for (int li = 0; li < maxl; ++li) {
+        int kmax = 0;
+        for (const auto& t : big)
+            if (t.nlinks > li) kmax = std::max(kmax, t.k);
+        for (int p0 = 0; p0 < kmax; p0 += NB) {
+            qoff.push_back((int)pq.size());
+            aoff.push_back((int)pa.size());
+            for (const auto& t : big) {
+                if (t.nlinks <= li || p0 >= t.k) continue;
+                const TpqrtLink<S>& L = links[t.link0 + li];
+                const int nbp = std::min(NB, t.k - p0);
+                S* Rp = t.R + p0 + (size_t)p0 * t.ldr;
+                S* Bp = L.B + (size_t)p0 * L.ldb;
+                pq.push_back({Rp, Bp, L.T + (size_t)p0 * NB, L.rows, nbp, t.ldr, L.ldb});
+                for (int c0 = p0 + nbp; c0 < t.k; c0 += 128)
+                    pa.push_back({Bp, L.T + (size_t)p0 * NB, t.R + p0 + (size_t)c0 * t.ldr, L.B + (size_t)c0 * L.ldb,
+                                  L.rows, nbp, L.ldb, t.ldr, L.ldb, std::min(128, t.k - c0)});
+            }
+        }
+    }
+    qoff.push_back((int)pq.size());
+    aoff.push_back((int)pa.size());
+    auto* dq = upload_vec(pq);
+    auto* da = upload_vec(pa);
+    int launches = 0;
+    for (size_t st = 0; st + 1 < qoff.size(); ++st) {
+        launch_panel_qr<S>(dq + qoff[st], qoff[st + 1] - qoff[st], true, stream_);
+        launch_panel_apply<S>(da + aoff[st], aoff[st + 1] - aoff[st], true, stream_);
+        launches += 1 + (aoff[st + 1] > aoff[st]);
+    }
+    return launches;
 }
 
 int Tree::factorize() {

@@ -196,6 +196,9 @@ private:
                     const std::vector<int>& gat_begin, const std::vector<int>& gq_cluster, int L, int* d_ranks,
                     int* d_err, std::vector<char>& used_sketch);
     template <typename S> const S* omega(int rows, int cols);
+    template <typename S> int geqrt_big(const std::vector<GeqrtTask<S>>& big);
+    template <typename S>
+    int tpqrt_big(const std::vector<TpqrtTask<S>>& big, const std::vector<TpqrtLink<S>>& links);
 
     // solve schedule
     struct SolveSchedule;

@@ -22,6 +22,10 @@ namespace spaqr {
 // the lowest index wins (LAPACK's plain argmax lets rounding order decide exact ties).
 constexpr double kPivotTieRel = 1e-10;
 
+// Fronts with at least this many columns run their panel loop from the host (geqrt_big,
+// tpqrt_big) so that the trailing updates spread over many workgroups.
+constexpr int kBigFront = 160;
+
 // Splits larfb work so that a large block is spread over many workgroups: Q^T C acts on each
 // column independently, so tasks are cut into column chunks; a fused trailing C R^{-1}
 // mixes columns, so it is moved to a second list of row chunks (V = nullptr: trsm only),
