@@ -1431,6 +1431,7 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
 
 SPAQR_INSTANTIATE(double)
 SPAQR_INSTANTIATE(float)
+template void launch_fill<int>(int*, size_t, int, hipStream_t);
 
 }  // namespace spaqr
 

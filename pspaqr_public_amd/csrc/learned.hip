@@ -12,6 +12,8 @@
 
 namespace spaqr {
 
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
 // ---------------------------------------------------------------------------------------
 // Gaussian test matrix.  Entry (i, j) is a function of (i, j, seed) only -- splitmix64
 // (S. Vigna's public-domain mixer) of the packed index gives two uniforms, Box-Muller gives
@@ -63,13 +65,21 @@ __global__ __launch_bounds__(WG) void k_colnorm(const ColNormTask<S>* __restrict
     mx = wave_max(mx);
     if (lane == 0) red[wv] = mx;
     __syncthreads();
-    if (tid == 0) *t.cmax2 = max(max(red[0], red[1]), max(red[2], red[3]));
+    if (tid == 0) {
+        const S v = max(max(red[0], red[1]), max(red[2], red[3]));
+        if constexpr (sizeof(S) == 8)
+            atomicMax((unsigned long long*)t.cmax2, (unsigned long long)__double_as_longlong((double)v));
+        else
+            atomicMax((unsigned int*)t.cmax2, (unsigned int)__float_as_int((float)v));
+    }
 }
 
 // ---------------------------------------------------------------------------------------
 // Sketch Y = C Omega: 64 x 64 output tile per workgroup, w consumed in chunks of 32 through
-// LDS.  Lane l of wave q owns row r0 + l and columns c0 + 16q .. c0 + 16q + 15.  Loads are
-// clamped in range and zero-weighted, so they issue unconditionally.
+// LDS; FP32 on MFMA (v_mfma_f32_32x32x2f32, one 32 x 32 block per wave: accumulator entry q of
+// lane l is row 8(q/4) + 4(l/32) + q%4, column l%32), FP64 on VALU (lane l of wave q owns row
+// r0 + l and columns c0 + 16q ..).  Loads are clamped in range and zero-weighted, so they issue
+// unconditionally.
 // ---------------------------------------------------------------------------------------
 template <typename S>
 __global__ __launch_bounds__(WG) void k_sketch(const SketchTask<S>* __restrict__ tasks,
@@ -101,20 +111,40 @@ __global__ __launch_bounds__(WG) void k_sketch(const SketchTask<S>* __restrict__
             Os[j][i] = (w0 + i < t.w) ? ov[q] : S(0);
         }
         __syncthreads();
-#pragma unroll 8
-        for (int i = 0; i < 32; ++i) {
-            const S c = Cs[i][row];
+        if constexpr (sizeof(S) == 4) {
+            // MFMA 32x32x2: wave (wm, wn) owns the 32 x 32 block (rows 32 wm.., columns 32 wn..)
+            const int li = tid & 31, lk = (tid >> 5) & 1, wm = wv & 1, wn = wv >> 1;
+            floatx16& m = *reinterpret_cast<floatx16*>(acc);
 #pragma unroll
-            for (int j = 0; j < 16; ++j) acc[j] += c * Os[wv * 16 + j][i];
+            for (int kk = 0; kk < 32; kk += 2)
+                m = __builtin_amdgcn_mfma_f32_32x32x2f32((float)Cs[kk + lk][32 * wm + li],
+                                                         (float)Os[32 * wn + li][kk + lk], m, 0, 0, 0);
+        } else {
+#pragma unroll 8
+            for (int i = 0; i < 32; ++i) {
+                const S c = Cs[i][row];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) acc[j] += c * Os[wv * 16 + j][i];
+            }
         }
         __syncthreads();
     }
-    const int r = tl.r0 + row;
-    if (r < t.k) {
+    if constexpr (sizeof(S) == 4) {
+        const int li = tid & 31, wm = wv & 1, wn = wv >> 1, l = tid & 63;
+        const int c = tl.c0 + 32 * wn + li;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int c = tl.c0 + wv * 16 + j;
-            if (c < t.s) t.Y[r + (size_t)c * t.k] = acc[j];
+        for (int q = 0; q < 16; ++q) {
+            const int r = tl.r0 + 32 * wm + (q >> 2) * 8 + (l >> 5) * 4 + (q & 3);
+            if (r < t.k && c < t.s) t.Y[r + (size_t)c * t.k] = acc[q];
+        }
+    } else {
+        const int r = tl.r0 + row;
+        if (r < t.k) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int c = tl.c0 + wv * 16 + j;
+                if (c < t.s) t.Y[r + (size_t)c * t.k] = acc[j];
+            }
         }
     }
 }
@@ -184,7 +214,7 @@ __global__ __launch_bounds__(WG) void k_tail_select(const TailTask<S>* __restric
     __shared__ int red[4];
     const TailTask<S> t = tasks[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, grp = tid >> 4, gl = tid & 15;
-    const int rY = *t.rank;
+    const int rY = *t.ry;
     const double thr2 = tol * tol * (double)*t.cmax2;
     int need = 0;
     for (int c0 = 0; c0 < t.w; c0 += WG / 16) {
@@ -221,10 +251,7 @@ __global__ __launch_bounds__(WG) void k_tail_select(const TailTask<S>* __restric
     const int wn = max(max(lane_read(need, 0), lane_read(need, 16)), max(lane_read(need, 32), lane_read(need, 48)));
     if (lane == 0) red[wv] = wn;
     __syncthreads();
-    if (tid == 0) {
-        const int r = max(max(red[0], red[1]), max(red[2], red[3]));
-        *t.rank = (r == INT_MAX) ? -1 : r;
-    }
+    if (tid == 0) atomicMax(t.rank, max(max(red[0], red[1]), max(red[2], red[3])));
 }
 
 // ---------------------------------------------------------------------------------------

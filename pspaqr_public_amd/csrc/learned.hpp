@@ -30,7 +30,9 @@ constexpr int kMlpOffB3 = kMlpOffW3 + kMlpHidden;
 constexpr int kMlpParams = kMlpOffB3 + 1;  // 6337
 constexpr int kSketchFeatureCols = 16;
 
-// Column norms: cn[j] = ||C(:, j)||^2 (w values) and cmax2 = max_j cn[j].
+// Column norms: cn[j] = ||C(:, j)||^2 (w values); *cmax2 = max(*cmax2, max_j cn[j]) through an
+// atomic on the bit pattern (non-negative), so one interface may be split over column chunks
+// and *cmax2 must start at zero.
 template <typename S> struct ColNormTask { const S* C; S* cn; S* cmax2; int k, w; };
 // Sketch Y (k x s, ld k) = C (k x w, ld k) * Omega(0:w, 0:s); one 64 x 64 tile per workgroup.
 template <typename S> struct SketchTask { const S* C; S* Y; int k, w, s, pad; };
@@ -41,9 +43,11 @@ template <typename S> struct FeatTask {
     const S* Y16; const S* cn; const S* cmax2; float* x;
     int k, w, nsout, nsin, level, pad;
 };
-// B = Q^T C (k x w) with Q from the sketch's first rY reflectors: rank <- smallest r <= rY
-// whose tail column norms are all <= tol * max column norm of the original C, or -1.
-template <typename S> struct TailTask { const S* B; const S* cmax2; int* rank; int k, w; };
+// B = Q^T C (k x w) with Q from the sketch's first rY = *ry reflectors: *rank <- max(*rank, the
+// smallest r <= rY whose tail column norms over these w columns are all <= tol * max column
+// norm of the original C), INT_MAX if some column fails; an interface may be split over
+// column chunks (atomic max) and *rank must start at zero.
+template <typename S> struct TailTask { const S* B; const S* cmax2; const int* ry; int* rank; int k, w; };
 
 template <typename S> void launch_gauss(S* om, int rows, int cols, int ld, uint64_t seed, hipStream_t s);
 template <typename S> void launch_colnorm(const ColNormTask<S>* d, int n, hipStream_t s);

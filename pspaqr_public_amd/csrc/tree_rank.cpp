@@ -3,6 +3,7 @@
 // features, the predictor's sketch width, pivoted QR of the sketch, the tail test and the
 // per-interface geqp3 fallback -- plus the host side of the predictor's training.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -72,11 +73,18 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
     for (int q : elig) wmax = std::max(wmax, gq[q].w);
     S* d_cmax2 = (S*)arena.alloc(sizeof(S) * std::max(nt, 1));
     float* d_X = nullptr;
+    constexpr int kColChunk = 1024;  // columns per workgroup of the column-norm and tail kernels
     if (ne && (feats || sketch_mode)) {
         std::vector<ColNormTask<S>> cn;
-        for (int q : elig) cn.push_back({gq[q].C, gq[q].vn1, d_cmax2 + q, gq[q].k, gq[q].w});
+        for (int q : elig)
+            for (int c0 = 0; c0 < gq[q].w; c0 += kColChunk)
+                cn.push_back({gq[q].C + (size_t)c0 * gq[q].k, gq[q].vn1 + c0, d_cmax2 + q, gq[q].k,
+                              std::min(kColChunk, gq[q].w - c0)});
         auto* d_cn = upload_vec(cn);
-        timed(P_RSKETCH, 1, ne, [&] { launch_colnorm<S>(d_cn, ne, stream_); });
+        timed(P_RSKETCH, 2, ne, [&] {
+            launch_fill<S>(d_cmax2, (size_t)nt, S(0), stream_);
+            launch_colnorm<S>(d_cn, (int)cn.size(), stream_);
+        });
         for (int q : elig) prof_add(P_RSKETCH, 2.0 * gq[q].k * gq[q].w, SZ * gq[q].k * gq[q].w);
     }
     if (ne && feats) {
@@ -158,12 +166,15 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
         sync_stream();
         std::vector<LarfbTask<S>> lb;
         std::vector<TailTask<S>> tt;
+        int* d_tail = (int*)arena.alloc(sizeof(int) * std::max(nt, 1));
         for (int q : elig) {
             const int k = gq[q].k, w = gq[q].w, r = ry[q];
             for (int j = 0; j < r; ++j) flops += 4.0 * (k - j) * (s_of[q] - j);
             if (r <= 0) continue;
             lb.push_back({gq[q].Vout, gq[q].T, gq[q].C, nullptr, k, w, r, k, k, 0, 0, 0});
-            tt.push_back({gq[q].C, d_cmax2 + q, d_ranks + q, k, w});
+            for (int c0 = 0; c0 < w; c0 += kColChunk)
+                tt.push_back({gq[q].C + (size_t)c0 * k, d_cmax2 + q, d_ranks + q, d_tail + q, k,
+                              std::min(kColChunk, w - c0)});
             applied[q] = 1;
             flops += 4.0 * k * w * r - 2.0 * w * (double)r * r;
             prof_add(P_RSKETCH, 4.0 * k * w * r - 2.0 * w * (double)r * r, SZ * ((double)k * r + 2.0 * k * w));
@@ -173,30 +184,31 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
             split_larfb(lb, lb_c, lb_t);
             auto* d_lb = upload_vec(lb_c);
             auto* d_tt = upload_vec(tt);
-            timed(P_RSKETCH, 2, (long)lb.size(), [&] {
+            timed(P_RSKETCH, 3, (long)lb.size(), [&] {
+                launch_fill<int>(d_tail, (size_t)nt, 0, stream_);
                 launch_larfb<S>(d_lb, (int)lb_c.size(), stream_);
                 launch_tail_select<S>(d_tt, (int)tt.size(), tol, stream_);
             });
+            HIPCHK(hipMemcpyAsync(rk.data(), d_tail, sizeof(int) * nt, hipMemcpyDeviceToHost, stream_));
+            sync_stream();
+            for (int q = 0; q < nt; ++q)
+                if (rk[q] == INT_MAX) rk[q] = -1;
         }
-        HIPCHK(hipMemcpyAsync(rk.data(), d_ranks, sizeof(int) * nt, hipMemcpyDeviceToHost, stream_));
-        sync_stream();
     }
     // ---- decide: sketch result, or geqp3 on C
     std::vector<int> fb;
     std::vector<int> fixed(nt, 0);
-    bool patch = false;
+    bool any_sketch = false;
     for (int q = 0; q < nt; ++q) {
         if (sketch_mode && is_elig[q] && applied[q] && rk[q] >= 0) {
             used_sketch[q] = 1;
             fixed[q] = std::max(rk[q], 1);  // choose_rank keeps at least one column
-            if (rk[q] == 0) patch = true;
+            any_sketch = true;
         } else {
             fb.push_back(q);
         }
     }
-    if (patch) {
-        for (int q = 0; q < nt; ++q)
-            if (!used_sketch[q]) fixed[q] = 0;
+    if (any_sketch) {  // accepted sketch ranks; the fallback geqp3 below writes the others
         int* src = (int*)upload(fixed.data(), sizeof(int) * nt);
         HIPCHK(hipMemcpyAsync(d_ranks, src, sizeof(int) * nt, hipMemcpyDeviceToDevice, stream_));
     }
