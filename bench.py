@@ -108,7 +108,20 @@ def main():
     ap.add_argument("--rank-mode", type=int, default=1,
                     help="FP32 path's sparsification ranks: 1 learned sketch (default), 2 sketch without the "
                          "predictor, 0 geqp3 as the reference")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device three times and exit: the reference dispatch "
+                         "that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(args.copy_calibration * (1 << 18), device="cuda")  # float32
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("copied 3 x %d MiB" % args.copy_calibration, flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this path (one rank's whole share of the tree) runs on a single "
               "MI355X; --gpus %d is not supported (see DESIGN.md for how it would shard)." % args.gpus,
@@ -156,6 +169,14 @@ def main():
         fp64 = dict(seconds=s64, factor_s=f64, iters=res64.iters, true_relres=res64.true_res)
         del T64
 
+    # HBM bytes per dispatch of the dominant class, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    # (calibrated on a device copy) kept beside the kernel-trace summaries
+    traffic_by_class = {}
+    tp = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            traffic_by_class = json.load(f).get("bytes_per_dispatch", {})
+
     # ---- roofline of the dominant kernel class (by time, over the timed steps)
     cls = max((c for c in prof if c not in ("precond_solve", "gmres_orth")), key=lambda c: prof[c]["ms"])
     p = prof[cls]
@@ -163,12 +184,12 @@ def main():
     if cls in MFMA_CLASSES:
         achieved = p["flops"] / (p["ms"] * 1e-3) / 1e12 if p["ms"] > 0 else 0.0
         roof = dict(bound="mfma", achieved=achieved, peak=PEAK["fp32"], unit="TFLOP/s", frac=achieved / PEAK["fp32"],
-                    traffic=None, kernel=cls, launches=p["launches"], tasks=p["tasks"],
+                    traffic=traffic_by_class.get(cls), kernel=cls, launches=p["launches"], tasks=p["tasks"],
                     avg_launch_ms=per_launch_ms, algorithmic_flops_per_launch=p["flops"] / max(p["launches"], 1))
     else:
         achieved = p["bytes"] / (p["ms"] * 1e-3) / 1e9 if p["ms"] > 0 else 0.0
         roof = dict(bound="hbm", achieved=achieved, peak=PEAK["hbm"], unit="GB/s", frac=achieved / PEAK["hbm"],
-                    traffic=None, kernel=cls, launches=p["launches"], tasks=p["tasks"],
+                    traffic=traffic_by_class.get(cls), kernel=cls, launches=p["launches"], tasks=p["tasks"],
                     avg_launch_ms=per_launch_ms, algorithmic_bytes_per_launch=p["bytes"] / max(p["launches"], 1))
 
     # ---- CPU baseline (oracle, bounded sample) and the GPU on the same sample
