@@ -134,8 +134,9 @@ int spaqr_tree_set_rank_log(spaqr_tree* t, int on);
    used); returns the number of entries recorded */
 int spaqr_tree_rank_log(const spaqr_tree* t, float* x, int* meta, int cap);
 int spaqr_tree_rank_log_clear(spaqr_tree* t);
-/* 6 doubles: interfaces, sketched, fell back to geqp3, direct geqp3, sum of sketch widths,
-   sum of min(k, w) over the sketched interfaces (since the last reset) */
+/* 7 doubles: interfaces, sketched, fell back to geqp3, direct geqp3, sum of sketch widths
+   (retries included), sum of min(k, w) over the sketched interfaces, sketches retried with a
+   doubled width (since the last reset) */
 int spaqr_tree_rank_stats(const spaqr_tree* t, double* out);
 int spaqr_tree_rank_stats_reset(spaqr_tree* t);
 

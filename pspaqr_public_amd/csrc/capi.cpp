@@ -270,6 +270,7 @@ int spaqr_tree_rank_stats(const spaqr_tree* h, double* out) {
     out[3] = (double)r.direct;
     out[4] = r.sketch_cols;
     out[5] = r.full_cols;
+    out[6] = (double)r.retried;
     return 0;
 }
 int spaqr_tree_rank_stats_reset(spaqr_tree* h) {

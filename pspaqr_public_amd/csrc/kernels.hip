@@ -82,6 +82,7 @@ __device__ __forceinline__ void wg_panel_qr_mem(int m, int nbp, S* Rt, int ldr, 
         if (l < nbp) {
             const S* bj = B + (size_t)j * ldb;
             const S* bl = B + (size_t)l * ldb;
+#pragma unroll 8
             for (int r = r0 + g; r < m; r += 8) d += bj[r] * bl[r];
         }
         d = oct_sum(d);
@@ -109,6 +110,7 @@ __device__ __forceinline__ void wg_panel_qr_mem(int m, int nbp, S* Rt, int ldr, 
             const S ws = wl * scal;
             const S* bj = B + (size_t)j * ldb;
             S* bl = B + (size_t)l * ldb;
+#pragma unroll 8
             for (int r = r0 + g; r < m; r += 8) bl[r] -= bj[r] * ws;
             if (g == 0) Rt[j + (size_t)l * ldr] -= wl;
         }
@@ -120,6 +122,7 @@ __device__ __forceinline__ void wg_panel_qr_mem(int m, int nbp, S* Rt, int ldr, 
         __syncthreads();
         if (l == j) {
             S* bj = B + (size_t)j * ldb;
+#pragma unroll 8
             for (int r = r0 + g; r < m; r += 8) bj[r] *= scal;
         }
         if (tid < j) {
@@ -1478,7 +1481,11 @@ __device__ __forceinline__ void st_coherent(S* p, S v) {
 }
 
 // record layout inside a slot (k + 4 values): M_b, v_cand, pos_cand, unused, column[k]
-template <typename S>
+// HBM = false: the workgroup's column slice lives in LDS for the whole factorization.
+// HBM = true: for interfaces whose C cannot be spread over the co-resident workgroups' LDS, each
+// workgroup updates its columns of the gathered C in place in HBM (only norms, positions and
+// the reflector stay in LDS); the row loops are unrolled so that loads overlap.
+template <typename S, bool HBM>
 __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __restrict__ tasks,
                                                    const int* __restrict__ blk2task, int* err, double tol,
                                                    double tie_rel) {
@@ -1495,15 +1502,16 @@ __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __res
     const int q0 = p * cpb;
     const int ncol = max(0, min(w, q0 + cpb) - q0);
     const int rec = k + 4;
-    S* slice = reinterpret_cast<S*>(dyn_smem);    // k x cpb, ld k
-    S* vn1 = slice + (size_t)k * cpb;
+    S* slice = HBM ? const_cast<S*>(t.C) + (size_t)q0 * k : reinterpret_cast<S*>(dyn_smem);  // k x cpb, ld k
+    S* vn1 = HBM ? reinterpret_cast<S*>(dyn_smem) : slice + (size_t)k * cpb;
     S* vn2 = vn1 + cpb;
     S* vbuf = vn2 + cpb;                          // k: the pivot column, then the reflector
     int* pos = reinterpret_cast<int*>(vbuf + k);  // LAPACK position of each owned column; -1 once pivoted
     const S tol3z = sqrt(S(sizeof(S) == 8 ? 1.1102230246251565e-16 : 5.9604644775390625e-08));
     int passed = 0;
 
-    for (int e = tid; e < k * ncol; e += WG) slice[e] = t.C[(size_t)q0 * k + e];
+    if (!HBM)
+        for (int e = tid; e < k * ncol; e += WG) slice[e] = t.C[(size_t)q0 * k + e];
     __syncthreads();
     // 16 lanes (one DPP row) per column
     const int grp = tid >> 4, gl = tid & 15;
@@ -1706,19 +1714,23 @@ __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __res
             const bool act = (c < ncol) && pos[c] >= 0;  // uniform over the 16-lane row
             S* cl = slice + (size_t)min(c, ncol - 1) * k;
             S dot = S(0);
-            if (act)
+            if (act) {
+#pragma unroll 8
                 for (int r = j + 1 + gl; r < k; r += 16) dot += vbuf[r] * cl[r];
+            }
             dot = row16_sum(dot);
             const S cjl = act ? cl[j] : S(0);
             const S wl = (cjl + dot) * tau;
             const S rjl = cjl - wl;
             S nrm = S(0);
-            if (act)
+            if (act) {
+#pragma unroll 8
                 for (int r = j + 1 + gl; r < k; r += 16) {
                     S v = cl[r] - vbuf[r] * wl;
                     cl[r] = v;
                     nrm += v * v;
                 }
+            }
             nrm = row16_sum(nrm);
             if (act && gl == 0) {
                 cl[j] = rjl;
@@ -1750,21 +1762,23 @@ __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __res
 }
 
 template <typename S>
-int geqp3_coop_max_blocks(size_t lds_bytes) {
+int geqp3_coop_max_blocks(size_t lds_bytes, bool hbm) {
     int per_cu = 0, dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_geqp3_coop<S>, WG, lds_bytes) != hipSuccess) return 0;
+    hipError_t e = hbm ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_geqp3_coop<S, true>, WG, lds_bytes)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_geqp3_coop<S, false>, WG, lds_bytes);
+    if (e != hipSuccess) return 0;
     return per_cu * ncu;
 }
 
 template <typename S>
 hipError_t launch_geqp3_coop(const Geqp3CoopTask<S>* d_tasks, const int* d_blk2task, int* d_err, int nblocks,
-                             double tol, double tie_rel, size_t lds_bytes, hipStream_t s) {
+                             double tol, double tie_rel, size_t lds_bytes, bool hbm, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     void* args[] = {(void*)&d_tasks, (void*)&d_blk2task, (void*)&d_err, (void*)&tol, (void*)&tie_rel};
-    return hipLaunchCooperativeKernel((const void*)k_geqp3_coop<S>, dim3(nblocks), dim3(WG), args,
-                                      (unsigned)lds_bytes, s);
+    const void* fn = hbm ? (const void*)k_geqp3_coop<S, true> : (const void*)k_geqp3_coop<S, false>;
+    return hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(WG), args, (unsigned)lds_bytes, s);
 }
 
 template <typename S>
@@ -1774,7 +1788,7 @@ size_t geqp3_coop_lds_budget(int blocks_per_cu) {
     if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess)
         return 0;
     hipFuncAttributes fa;
-    if (hipFuncGetAttributes(&fa, (const void*)k_geqp3_coop<S>) != hipSuccess) return 0;
+    if (hipFuncGetAttributes(&fa, (const void*)k_geqp3_coop<S, false>) != hipSuccess) return 0;
     const size_t per = (size_t)lds_cu / (size_t)std::max(blocks_per_cu, 1);
     if (per <= fa.sharedSizeBytes + 1024) return 0;
     return ((per - fa.sharedSizeBytes) / 1024) * 1024;  // dynamic bytes, whole KiB
@@ -1782,11 +1796,11 @@ size_t geqp3_coop_lds_budget(int blocks_per_cu) {
 
 template size_t geqp3_coop_lds_budget<double>(int);
 template size_t geqp3_coop_lds_budget<float>(int);
-template int geqp3_coop_max_blocks<double>(size_t);
-template int geqp3_coop_max_blocks<float>(size_t);
+template int geqp3_coop_max_blocks<double>(size_t, bool);
+template int geqp3_coop_max_blocks<float>(size_t, bool);
 template hipError_t launch_geqp3_coop<double>(const Geqp3CoopTask<double>*, const int*, int*, int, double, double,
-                                              size_t, hipStream_t);
+                                              size_t, bool, hipStream_t);
 template hipError_t launch_geqp3_coop<float>(const Geqp3CoopTask<float>*, const int*, int*, int, double, double,
-                                             size_t, hipStream_t);
+                                             size_t, bool, hipStream_t);
 
 }  // namespace spaqr

@@ -894,6 +894,9 @@ int Tree::factorize_t() {
             st.n_sparsified = (int)gq.size();
         }
         hacc(2, ht);
+        if (dbg_host)
+            fprintf(stderr, "[level %d] interiors %d interfaces %d max front %d  host %.1f ms so far, GPU+sync %.1f ms\n", L,
+                    st.n_interiors, st.n_interfaces, st.max_front, hms[0] + hms[1] + hms[3] + hms[4], hms[2]);
         stats.push_back(st);
         if (L == nlevels - 1) break;
 

@@ -139,7 +139,7 @@ public:
         std::vector<int> rank, k, w, level, mode;
     } rlog;
     struct RankStats {
-        long tasks = 0, sketched = 0, fallback = 0, direct = 0;
+        long tasks = 0, sketched = 0, fallback = 0, direct = 0, retried = 0;
         double sketch_cols = 0, full_cols = 0, rank_sum = 0;
     } rstats;
     bool profiling = false;

@@ -18,8 +18,12 @@ namespace spaqr {
 namespace {
 constexpr double kSketchGrow = 1.25;      // sketch width = grow * predicted rank + pad
 constexpr int kSketchPad = 8;
-constexpr double kSketchTolFactor = 0.25;  // the sketch's QR stops at this fraction of tol
+// The sketch's pivoted QR runs to its full width s: its own |R_jj| / |R_00| test is relative to
+// the sketch's largest column, which can sit far above the largest column of C that the tail
+// test measures against, so stopping it early only causes fallbacks.
+constexpr double kSketchTolFactor = 0.0;
 constexpr uint64_t kSketchSeed = 0x5eed5ca1eull;
+constexpr int kSketchRounds = 3;           // first sketch plus up to two doublings
 }  // namespace
 
 int Tree::set_rank_model(const float* w) {
@@ -133,66 +137,92 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
         } else {
             for (int q : elig) s_of[q] = std::min(gq[q].k, gq[q].w);
         }
-        int smax = 1;
-        for (int q : elig) smax = std::max(smax, s_of[q]);
-        const S* om = omega<S>(wmax, smax);
-        std::vector<SketchTask<S>> sk;
-        std::vector<SketchTile> tiles;
-        std::vector<Geqp3Task<S>> gy;
-        for (int q : elig) {
-            const int k = gq[q].k, w = gq[q].w, s = s_of[q];
-            S* Y = (S*)arena.alloc(sizeof(S) * (size_t)k * s);
-            for (int r0 = 0; r0 < k; r0 += 64)
-                for (int c0 = 0; c0 < s; c0 += 64) tiles.push_back({(int)sk.size(), r0, c0, 0});
-            sk.push_back({gq[q].C, Y, k, w, s, 0});
-            Geqp3Task<S> g = gq[q];
-            g.C = Y;
-            g.w = s;
-            gy.push_back(g);
-            flops += 2.0 * k * w * s;
-            prof_add(P_RSKETCH, 2.0 * k * w * s, SZ * ((double)k * w + (double)k * s));
-            prof_add(P_GEQP3, 0.0, SZ * 3.0 * k * s);
-            rstats.sketch_cols += s;
-            rstats.full_cols += std::min(k, w);
-        }
-        auto* d_sk = upload_vec(sk);
-        auto* d_tl = upload_vec(tiles);
-        timed(P_RSKETCH, 1, ne, [&] { launch_sketch<S>(d_sk, d_tl, (int)tiles.size(), om, om_rows, stream_); });
-        int nd = 0;
-        timed(P_GEQP3, 0, ne,
-              [&] { geqp3_grouped<S>(gy, tol * kSketchTolFactor, kPivotTieRel, stream_, mem, d_err, &nd); });
-        if (profiling) prof[P_GEQP3].launches += nd;
-        HIPCHK(hipMemcpyAsync(ry.data(), d_ranks, sizeof(int) * nt, hipMemcpyDeviceToHost, stream_));
-        sync_stream();
-        std::vector<LarfbTask<S>> lb;
-        std::vector<TailTask<S>> tt;
-        int* d_tail = (int*)arena.alloc(sizeof(int) * std::max(nt, 1));
-        for (int q : elig) {
-            const int k = gq[q].k, w = gq[q].w, r = ry[q];
-            for (int j = 0; j < r; ++j) flops += 4.0 * (k - j) * (s_of[q] - j);
-            if (r <= 0) continue;
-            lb.push_back({gq[q].Vout, gq[q].T, gq[q].C, nullptr, k, w, r, k, k, 0, 0, 0});
-            for (int c0 = 0; c0 < w; c0 += kColChunk)
-                tt.push_back({gq[q].C + (size_t)c0 * k, d_cmax2 + q, d_ranks + q, d_tail + q, k,
-                              std::min(kColChunk, w - c0)});
-            applied[q] = 1;
-            flops += 4.0 * k * w * r - 2.0 * w * (double)r * r;
-            prof_add(P_RSKETCH, 4.0 * k * w * r - 2.0 * w * (double)r * r, SZ * ((double)k * r + 2.0 * k * w));
-        }
-        if (!lb.empty()) {
-            std::vector<LarfbTask<S>> lb_c, lb_t;
-            split_larfb(lb, lb_c, lb_t);
-            auto* d_lb = upload_vec(lb_c);
-            auto* d_tt = upload_vec(tt);
-            timed(P_RSKETCH, 3, (long)lb.size(), [&] {
-                launch_fill<int>(d_tail, (size_t)nt, 0, stream_);
-                launch_larfb<S>(d_lb, (int)lb_c.size(), stream_);
-                launch_tail_select<S>(d_tt, (int)tt.size(), tol, stream_);
-            });
-            HIPCHK(hipMemcpyAsync(rk.data(), d_tail, sizeof(int) * nt, hipMemcpyDeviceToHost, stream_));
+        // Rounds: sketch, pivoted QR of the sketch, Q^T C and the tail test.  An interface whose
+        // tail test fails is gathered again and retried with a sketch twice as wide (a sketch as
+        // wide as min(k, w) spans the range of C, so the last round can only fail by rounding).
+        std::vector<int> todo = elig;
+        for (int round = 0; round < kSketchRounds && !todo.empty(); ++round) {
+            int smax = 1;
+            for (int q : todo) smax = std::max(smax, s_of[q]);
+            const S* om = omega<S>(wmax, smax);
+            if (round > 0) {  // C was overwritten by the previous round's Q^T C
+                std::vector<CopyTask<S>> regather;
+                for (int q : todo)
+                    for (int g = gat_begin[q]; g < gat_begin[q + 1]; ++g) regather.push_back(gat[g]);
+                auto* d_rg = upload_vec(regather);
+                timed(P_GATHER, 1, (long)regather.size(), [&] { launch_copy<S>(d_rg, (int)regather.size(), stream_); });
+            }
+            std::vector<SketchTask<S>> sk;
+            std::vector<SketchTile> tiles;
+            std::vector<Geqp3Task<S>> gy;
+            for (int q : todo) {
+                const int k = gq[q].k, w = gq[q].w, s = s_of[q];
+                S* Y = (S*)arena.alloc(sizeof(S) * (size_t)k * s);
+                for (int r0 = 0; r0 < k; r0 += 64)
+                    for (int c0 = 0; c0 < s; c0 += 64) tiles.push_back({(int)sk.size(), r0, c0, 0});
+                sk.push_back({gq[q].C, Y, k, w, s, 0});
+                Geqp3Task<S> g = gq[q];
+                g.C = Y;
+                g.w = s;
+                gy.push_back(g);
+                flops += 2.0 * k * w * s;
+                prof_add(P_RSKETCH, 2.0 * k * w * s, SZ * ((double)k * w + (double)k * s));
+                prof_add(P_GEQP3, 0.0, SZ * 3.0 * k * s);
+                rstats.sketch_cols += s;
+                if (round == 0) rstats.full_cols += std::min(k, w);
+            }
+            auto* d_sk = upload_vec(sk);
+            auto* d_tl = upload_vec(tiles);
+            timed(P_RSKETCH, 1, (long)todo.size(),
+                  [&] { launch_sketch<S>(d_sk, d_tl, (int)tiles.size(), om, om_rows, stream_); });
+            int nd = 0;
+            timed(P_GEQP3, 0, (long)todo.size(),
+                  [&] { geqp3_grouped<S>(gy, tol * kSketchTolFactor, kPivotTieRel, stream_, mem, d_err, &nd); });
+            if (profiling) prof[P_GEQP3].launches += nd;
+            HIPCHK(hipMemcpyAsync(ry.data(), d_ranks, sizeof(int) * nt, hipMemcpyDeviceToHost, stream_));
             sync_stream();
-            for (int q = 0; q < nt; ++q)
-                if (rk[q] == INT_MAX) rk[q] = -1;
+            std::vector<LarfbTask<S>> lb;
+            std::vector<TailTask<S>> tt;
+            int* d_tail = (int*)arena.alloc(sizeof(int) * std::max(nt, 1));
+            for (int q : todo) {
+                const int k = gq[q].k, w = gq[q].w, r = ry[q];
+                for (int j = 0; j < r; ++j) flops += 4.0 * (k - j) * (s_of[q] - j);
+                applied[q] = 0;
+                if (r <= 0) continue;
+                lb.push_back({gq[q].Vout, gq[q].T, gq[q].C, nullptr, k, w, r, k, k, 0, 0, 0});
+                for (int c0 = 0; c0 < w; c0 += kColChunk)
+                    tt.push_back({gq[q].C + (size_t)c0 * k, d_cmax2 + q, d_ranks + q, d_tail + q, k,
+                                  std::min(kColChunk, w - c0)});
+                applied[q] = 1;
+                flops += 4.0 * k * w * r - 2.0 * w * (double)r * r;
+                prof_add(P_RSKETCH, 4.0 * k * w * r - 2.0 * w * (double)r * r, SZ * ((double)k * r + 2.0 * k * w));
+            }
+            std::vector<int> again;
+            if (!lb.empty()) {
+                std::vector<LarfbTask<S>> lb_c, lb_t;
+                split_larfb(lb, lb_c, lb_t);
+                auto* d_lb = upload_vec(lb_c);
+                auto* d_tt = upload_vec(tt);
+                timed(P_RSKETCH, 3, (long)lb.size(), [&] {
+                    launch_fill<int>(d_tail, (size_t)nt, 0, stream_);
+                    launch_larfb<S>(d_lb, (int)lb_c.size(), stream_);
+                    launch_tail_select<S>(d_tt, (int)tt.size(), tol, stream_);
+                });
+                std::vector<int> got(nt, 0);
+                HIPCHK(hipMemcpyAsync(got.data(), d_tail, sizeof(int) * nt, hipMemcpyDeviceToHost, stream_));
+                sync_stream();
+                for (int q : todo) {
+                    if (!applied[q]) continue;
+                    rk[q] = (got[q] == INT_MAX) ? -1 : got[q];
+                    const int kw = std::min(gq[q].k, gq[q].w);
+                    if (rk[q] < 0 && s_of[q] < kw && round + 1 < kSketchRounds) {
+                        s_of[q] = std::min(kw, 2 * s_of[q]);
+                        again.push_back(q);
+                        rstats.retried++;
+                    }
+                }
+            }
+            todo.swap(again);
         }
     }
     // ---- decide: sketch result, or geqp3 on C

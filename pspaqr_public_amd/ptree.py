@@ -144,10 +144,10 @@ class ParTree:
         check(lib().spaqr_tree_rank_log_clear(self._h), self._h, "rank_log_clear")
 
     def rank_stats(self) -> dict:
-        out = np.zeros(6)
+        out = np.zeros(7)
         check(lib().spaqr_tree_rank_stats(self._h, out.ctypes.data), self._h, "rank_stats")
         return dict(interfaces=int(out[0]), sketched=int(out[1]), fallback=int(out[2]), direct=int(out[3]),
-                    sketch_cols=float(out[4]), full_cols=float(out[5]))
+                    sketch_cols=float(out[4]), full_cols=float(out[5]), retried=int(out[6]))
 
     def rank_stats_reset(self):
         check(lib().spaqr_tree_rank_stats_reset(self._h), self._h, "rank_stats_reset")

@@ -281,3 +281,30 @@ def test_geqp3_tie_band_matches_oracle():
     assert int(b["rank"].download(np.int32, 1)[0]) == r
     assert np.array_equal(b["jp"].download(np.int32, w)[:r], jp[:r])
     assert rel(unF(b["V"].download(np.float64, k * r), k, r), Cf[:, :r]) < 1e-12
+
+
+def test_geqp3_hbm_slices_match_oracle():
+    """An interface too tall for its column slices to sit in LDS (k = 6000 rows in FP64: one
+    column is 47 KB) runs the cooperative geqp3 with the slices left in HBM; same rank, pivots,
+    reflectors and T as the oracle."""
+    rng = np.random.default_rng(21)
+    k, w, tol = 6000, 300, 1e-2
+    U, _ = np.linalg.qr(rng.standard_normal((k, w)))
+    Vt, _ = np.linalg.qr(rng.standard_normal((w, w)))
+    Cm = (U * 0.93 ** np.arange(w)) @ Vt.T
+    c = F(Cm)
+    jp = np.zeros(w, dtype=np.int32)
+    tau = np.zeros(min(k, w))
+    olib().orc_geqp3(k, w, c, jp, tau)
+    Cf = unF(c, k, w)
+    r = olib().orc_choose_rank(np.ascontiguousarray(np.diag(Cf[:, :w])), w, tol)
+    assert 20 < r < w
+    b = dict(C=dbuf(F(Cm)), V=G.DeviceBuffer(8 * k * w), tau=G.DeviceBuffer(8 * w), T=G.DeviceBuffer(8 * NB * w),
+             vn=G.DeviceBuffer(16 * w), jp=G.DeviceBuffer(4 * w), rank=G.DeviceBuffer(4))
+    d = G.Geqp3Desc(b["C"].ptr, b["V"].ptr, b["tau"].ptr, b["T"].ptr, b["vn"].ptr, b["vn"].ptr + 8 * w, b["jp"].ptr,
+                    b["rank"].ptr, k, w, 0, 0)
+    assert G.lib().spaqr_geqp3_batched(64, 1, (G.Geqp3Desc * 1)(d), tol, 0.0, None) == 0
+    assert int(b["rank"].download(np.int32, 1)[0]) == r
+    assert np.array_equal(b["jp"].download(np.int32, w)[:r], jp[:r])
+    assert rel(unF(b["V"].download(np.float64, k * r), k, r), Cf[:, :r]) < 1e-11
+    assert rel(b["tau"].download(np.float64, r), tau[:r]) < 1e-11
