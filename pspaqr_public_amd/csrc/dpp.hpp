@@ -104,6 +104,15 @@ __device__ __forceinline__ S row16_scan(S v) {
     return v;
 }
 
+// Orders one wave's LDS writes before its later LDS reads by other lanes, without a
+// workgroup barrier: the wavefront-scope fences stop the compiler from moving LDS accesses
+// across the point (the hardware executes one wave's LDS instructions in order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Sum the 32 per-lane values d[0..31] over the wave.  On return lane L holds the total of
 // column vec_col(L) (lanes L and L^1 hold the same column): a transposing butterfly, 32 shuffles.
 __device__ __forceinline__ int vec_col(int lane) {

@@ -1146,6 +1146,234 @@ __global__ __launch_bounds__(WG) void k_vec_chain(const VecChainTask<S>* __restr
     }
 }
 
+// One link of a chain held in one wave's registers: lane R holds row R of Vb (all k <= 64
+// columns) and x_n[R]; lane j holds column j % 32 of panel j / 32's T block, masked to its
+// upper triangle (zero past k).
+template <typename S>
+struct ChainLinkRegs {
+    S v[64];
+    S tc[NB];
+    S xr;
+};
+
+// Global-memory view of a pointer read from a task descriptor (lets the compiler emit global
+// rather than flat loads).
+template <typename S>
+__device__ __forceinline__ const __attribute__((address_space(1))) S* gmem(const S* p) {
+    return (const __attribute__((address_space(1))) S*)p;
+}
+
+template <typename S>
+__device__ __forceinline__ void chain_link_load(const VecChainLink<S>& L, int k, int lane, ChainLinkRegs<S>& g) {
+    const int r = min(lane, L.rows - 1);
+    const auto V = gmem(L.V);
+#pragma unroll
+    for (int l = 0; l < 64; ++l) g.v[l] = V[(unsigned)(r + min(l, k - 1) * L.ldv)];
+    // every T load is issued; the mask multiplies (a select would let the compiler branch
+    // around each load and wait on it)
+    const int j = min(lane, k - 1), jc = lane & (NB - 1);
+    const auto Tj = gmem(L.T) + (size_t)j * NB;
+    const bool live = lane < k;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) g.tc[q] = Tj[q] * ((live && q <= jc) ? S(1) : S(0));
+    g.xr = gmem(L.xn)[r];
+}
+
+// x_c <- H^T x_c over a chain of tpqrt links (the forward solve's elimination of an interior
+// into its neighbours), one wave per chain, for chains whose k <= 64 and whose links have
+// <= 64 rows.  The wave uses no workgroup barrier: the 32 panel coefficients pass through
+// the wave's own LDS rows.  The next link's V, T and x_n are loaded before the current link
+// is applied (FP32; FP64 would spill, so there each link loads its own data first), so each
+// link costs one load latency at most.
+template <typename S>
+__global__ __launch_bounds__(WG) void k_vec_chain_wave(const VecChainTask<S>* __restrict__ tasks,
+                                                       const VecChainLink<S>* __restrict__ links, int n) {
+    __shared__ S sxc[4][64];  // x_c of the wave's chain
+    __shared__ S sd[4][NB];   // panel d = x_c(panel) + Vb^T x_n
+    __shared__ S st[4][NB];   // panel t = T^T d
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ti = blockIdx.x * 4 + wave;
+    if (ti >= n) return;  // whole waves leave; nothing below synchronises across waves
+    const VecChainTask<S> t = tasks[ti];
+    const int k = t.k;
+    sxc[wave][lane] = (lane < k) ? t.xc[min(lane, k - 1)] : S(0);  // zero past k: masked products stay finite
+    constexpr bool PF = sizeof(S) == 4;
+    ChainLinkRegs<S> cur, nxt;
+    if (PF) chain_link_load<S>(links[t.link0], k, lane, cur);
+    for (int li = 0; li < t.nlinks; ++li) {
+        const VecChainLink<S> L = links[t.link0 + li];
+        if (!PF) chain_link_load<S>(L, k, lane, cur);
+        if (PF && li + 1 < t.nlinks) chain_link_load<S>(links[t.link0 + li + 1], k, lane, nxt);
+        const bool rv = lane < L.rows;
+        S xr = rv ? cur.xr : S(0);
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            if (p * NB >= k) break;
+            S d[NB];
+#pragma unroll
+            for (int l = 0; l < NB; ++l) d[l] = cur.v[p * NB + l] * xr;
+            const S tot = wave_sum32<S>(d);
+            if ((lane & 1) == 0) sd[wave][vec_col(lane)] = tot + sxc[wave][p * NB + vec_col(lane)];
+            wave_sync();
+            if ((lane >> 5) == p) {
+                S tj = S(0);
+#pragma unroll
+                for (int q = 0; q < NB; ++q) tj += cur.tc[q] * sd[wave][q];
+                st[wave][lane & (NB - 1)] = tj;
+                if (lane < k) sxc[wave][lane] -= tj;
+            }
+            wave_sync();
+            S s = S(0);
+#pragma unroll
+            for (int l = 0; l < NB; ++l) s += cur.v[p * NB + l] * st[wave][l];
+            if (rv) xr -= s;  // lanes past the link's rows keep x = 0 for the next panel's sums
+            wave_sync();  // sd / st are rewritten by the next panel
+        }
+        if (rv) L.xn[lane] = xr;
+        if (PF && li + 1 < t.nlinks) cur = nxt;
+    }
+    wave_sync();
+    if (lane < k) t.xc[lane] = sxc[wave][lane];
+}
+
+// Chains too large for one wave (k <= kChainPipeK, links of up to 256 * NRW rows): one
+// workgroup per chain, the chain flattened into a stream of (link, panel) items.
+//  - Thread tid owns rows tid + 256 i (i < NRW) of every link, and keeps x_n for them in
+//    registers for the whole link.
+//  - Item i's V rows and T block are loaded NBUF - 1 items ahead into a ring of register
+//    buffers, so HBM latency overlaps the panels in between.
+//  - One workgroup barrier per panel: after it, every wave recomputes the 32 coefficients
+//    t = T^T d from the reduced partials itself (identical arithmetic in every wave) and keeps
+//    its own copy of x_c in LDS, so no second barrier is needed to publish t.  The partial
+//    sums and the T block are double-buffered in LDS: a wave can only be one barrier ahead.
+constexpr int kChainPipeK = 512;
+
+template <typename S, int NRW>
+struct ChainItemRegs {
+    S v[NRW][NB];       // V rows of the panel
+    S tb[NB * NB / WG];  // this thread's share of the panel's T block
+    S x[NRW];           // x_n rows (used at the link's first panel)
+};
+
+template <typename S>
+struct ChainPipeSmem {
+    S red[2][4][NB];       // per-wave partial column sums
+    S Ts[2][NB * (NB + 1)];  // the panel's T block, column l at l * 33 (masked to its triangle)
+    S dw[4][NB];           // per-wave full d
+    S tw[4][NB];           // per-wave t
+    S xc[4][kChainPipeK];  // per-wave copy of x_c
+};
+
+template <typename S, int NRW>
+__device__ __forceinline__ void chain_item_load(const VecChainLink<S>* __restrict__ links, int link0, int np, int k,
+                                                int item, ChainItemRegs<S, NRW>& g) {
+    const int li = item / np, p0 = (item - li * np) * NB, kb = min(NB, k - p0);
+    const VecChainLink<S> L = links[link0 + li];
+    const int tid = threadIdx.x;
+    const auto V = gmem(L.V) + (size_t)p0 * L.ldv;
+#pragma unroll
+    for (int i = 0; i < NRW; ++i) {
+        const int r = min(tid + i * WG, L.rows - 1);
+#pragma unroll
+        for (int l = 0; l < NB; ++l) g.v[i][l] = V[(unsigned)(r + min(l, kb - 1) * L.ldv)];
+        g.x[i] = gmem(L.xn)[r];
+    }
+    const auto T = gmem(L.T) + (size_t)p0 * NB;
+#pragma unroll
+    for (int e = 0; e < NB * NB / WG; ++e) {
+        const int idx = tid + e * WG, q = idx & (NB - 1), l = idx / NB;
+        g.tb[e] = T[q + min(l, kb - 1) * NB] * ((q <= l && l < kb) ? S(1) : S(0));
+    }
+}
+
+template <typename S, int NRW>
+__device__ __forceinline__ void chain_item_apply(const VecChainLink<S>* __restrict__ links, int link0, int np, int k,
+                                                 int item, const ChainItemRegs<S, NRW>& g, S (&x)[NRW],
+                                                 ChainPipeSmem<S>& sm) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, slot = item & 1;
+    const int li = item / np, pi = item - li * np, p0 = pi * NB;
+    const int rows = links[link0 + li].rows;
+    if (pi == 0) {
+#pragma unroll
+        for (int i = 0; i < NRW; ++i) x[i] = (tid + i * WG < rows) ? g.x[i] : S(0);
+    }
+#pragma unroll
+    for (int e = 0; e < NB * NB / WG; ++e) {
+        const int idx = tid + e * WG, q = idx & (NB - 1), l = idx / NB;
+        sm.Ts[slot][l * (NB + 1) + q] = g.tb[e];
+    }
+    S d[NB];
+#pragma unroll
+    for (int l = 0; l < NB; ++l) {
+        S a = S(0);
+#pragma unroll
+        for (int i = 0; i < NRW; ++i) a += g.v[i][l] * x[i];
+        d[l] = a;
+    }
+    const S tot = wave_sum32<S>(d);
+    if ((lane & 1) == 0) sm.red[slot][wave][vec_col(lane)] = tot;
+    __syncthreads();
+    const int j = lane & (NB - 1);
+    const S dj = ((sm.red[slot][0][j] + sm.red[slot][1][j]) + (sm.red[slot][2][j] + sm.red[slot][3][j])) +
+                 sm.xc[wave][p0 + j];
+    if (lane < NB) sm.dw[wave][j] = dj;
+    wave_sync();
+    S tj = S(0);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) tj += sm.Ts[slot][j * (NB + 1) + q] * sm.dw[wave][q];
+    if (lane < NB) {
+        sm.tw[wave][j] = tj;
+        sm.xc[wave][p0 + j] -= tj;  // t_j = 0 past the panel (T is masked there)
+    }
+    wave_sync();
+    S tl[NB];
+#pragma unroll
+    for (int l = 0; l < NB; ++l) tl[l] = sm.tw[wave][l];
+#pragma unroll
+    for (int i = 0; i < NRW; ++i) {
+        S s = S(0);
+#pragma unroll
+        for (int l = 0; l < NB; ++l) s += g.v[i][l] * tl[l];
+        if (tid + i * WG < rows) x[i] -= s;
+    }
+    if (pi == np - 1) {
+        S* xn = links[link0 + li].xn;
+#pragma unroll
+        for (int i = 0; i < NRW; ++i)
+            if (tid + i * WG < rows) xn[tid + i * WG] = x[i];
+    }
+    wave_sync();  // dw / tw are rewritten by the next item
+}
+
+template <typename S, int NRW, int NBUF>
+__global__ __launch_bounds__(WG) void k_vec_chain_pipe(const VecChainTask<S>* __restrict__ tasks,
+                                                       const VecChainLink<S>* __restrict__ links) {
+    __shared__ ChainPipeSmem<S> sm;
+    const VecChainTask<S> t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int k = t.k, np = (k + NB - 1) / NB, n = t.nlinks * np;
+    for (int i = lane; i < np * NB; i += 64) sm.xc[wave][i] = (i < k) ? t.xc[i] : S(0);
+    ChainItemRegs<S, NRW> buf[NBUF];
+#pragma unroll
+    for (int b = 0; b < NBUF; ++b)
+        if (b < n) chain_item_load<S, NRW>(links, t.link0, np, k, b, buf[b]);
+    S x[NRW];
+#pragma unroll
+    for (int i = 0; i < NRW; ++i) x[i] = S(0);
+    for (int i0 = 0; i0 < n; i0 += NBUF) {
+#pragma unroll
+        for (int b = 0; b < NBUF; ++b) {
+            const int item = i0 + b;
+            if (item < n) {
+                chain_item_apply<S, NRW>(links, t.link0, np, k, item, buf[b], x, sm);
+                if (item + NBUF < n) chain_item_load<S, NRW>(links, t.link0, np, k, item + NBUF, buf[b]);
+            }
+        }
+    }
+    if (wave == 0)
+        for (int i = lane; i < k; i += 64) t.xc[i] = sm.xc[0][i];
+}
+
 // x[0:k] <- R^{-1} x[0:k], R upper triangular (ldr): 32-row blocks from the bottom; wave 0
 // solves the diagonal block with shuffles, then every thread updates its own rows above.
 template <typename S>
@@ -1374,6 +1602,19 @@ void launch_vec_chain(const VecChainTask<S>* d, const VecChainLink<S>* l, int n,
     if (n > 0) hipLaunchKernelGGL(k_vec_chain<S>, dim3(n), dim3(WG), 0, s, d, l);
 }
 template <typename S>
+void launch_vec_chain_pipe(const VecChainTask<S>* d, const VecChainLink<S>* l, int n, int nrw, hipStream_t s) {
+    if (n <= 0) return;
+    constexpr bool F = sizeof(S) == 4;  // FP64 holds fewer items in flight (registers)
+    if (nrw <= 1)
+        hipLaunchKernelGGL((k_vec_chain_pipe<S, 1, F ? 3 : 2>), dim3(n), dim3(WG), 0, s, d, l);
+    else
+        hipLaunchKernelGGL((k_vec_chain_pipe<S, 2, F ? 2 : 1>), dim3(n), dim3(WG), 0, s, d, l);
+}
+template <typename S>
+void launch_vec_chain_wave(const VecChainTask<S>* d, const VecChainLink<S>* l, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_vec_chain_wave<S>, dim3((n + 3) / 4), dim3(WG), 0, s, d, l, n);
+}
+template <typename S>
 void launch_vec_spars(const VecSparsTask<S>* d, int n, int fwd, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_vec_spars<S>, dim3(n), dim3(WG), 0, s, d, fwd);
 }
@@ -1426,6 +1667,8 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
     template void launch_fill<S>(S*, size_t, S, hipStream_t);                                        \
     template void launch_vec_q<S>(const VecQTask<S>*, int, hipStream_t);                             \
     template void launch_vec_chain<S>(const VecChainTask<S>*, const VecChainLink<S>*, int, hipStream_t); \
+    template void launch_vec_chain_wave<S>(const VecChainTask<S>*, const VecChainLink<S>*, int, hipStream_t); \
+    template void launch_vec_chain_pipe<S>(const VecChainTask<S>*, const VecChainLink<S>*, int, int, hipStream_t); \
     template void launch_vec_spars<S>(const VecSparsTask<S>*, int, int, hipStream_t);                \
     template void launch_vec_trsv<S>(const VecTrsvTask<S>*, const VecGemvLink<S>*, int, hipStream_t); \
     template void launch_vec_copy<S>(const VecCopyTask<S>*, int, hipStream_t);                       \

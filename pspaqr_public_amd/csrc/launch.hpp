@@ -21,6 +21,12 @@ template <typename S> void launch_spars_apply(const SparsApplyTask<S>* d, int n,
 template <typename S> void launch_fill(S* p, size_t n, S v, hipStream_t s);
 template <typename S> void launch_vec_q(const VecQTask<S>* d, int n, hipStream_t s);
 template <typename S> void launch_vec_chain(const VecChainTask<S>* d, const VecChainLink<S>* l, int n, hipStream_t s);
+// Chains with k <= 64 whose links all have 1..64 rows: one wave per chain.
+// Chains with k <= 512 whose links have 1..256 * nrw rows (nrw = 1 or 2): one workgroup per
+// chain, panels pipelined.
+template <typename S>
+void launch_vec_chain_pipe(const VecChainTask<S>* d, const VecChainLink<S>* l, int n, int nrw, hipStream_t s);
+template <typename S> void launch_vec_chain_wave(const VecChainTask<S>* d, const VecChainLink<S>* l, int n, hipStream_t s);
 template <typename S> void launch_vec_spars(const VecSparsTask<S>* d, int n, int fwd, hipStream_t s);
 template <typename S> void launch_vec_trsv(const VecTrsvTask<S>* d, const VecGemvLink<S>* l, int n, hipStream_t s);
 template <typename S> void launch_vec_copy(const VecCopyTask<S>* d, int n, hipStream_t s);

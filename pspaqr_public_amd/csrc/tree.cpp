@@ -20,6 +20,8 @@
 #include "tree.hpp"
 
 #include <algorithm>
+#include <array>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -215,7 +217,9 @@ struct Tree::SolveSchedule {
     struct Level {
         void* merge_fwd = nullptr; int n_merge_fwd = 0;
         void* q = nullptr; int n_q = 0;
-        std::vector<std::pair<void*, int>> chain;  // per colour: tasks
+        // per colour, the chains by kernel: 0 one wave (k <= 64, rows <= 64), 1 / 2 pipelined
+        // workgroup (k <= 512, rows <= 256 / 512), 3 plain workgroup
+        std::vector<std::array<std::pair<void*, int>, 4>> chain;
         void* chain_links = nullptr;
         void* spars = nullptr; int n_spars = 0;
         void* trsv = nullptr; int n_trsv = 0;
@@ -1173,7 +1177,10 @@ int Tree::build_solve_schedule() {
         std::vector<VecQTask<S>> q;
         int ncol = 0;
         for (int c : interiors[L]) ncol = std::max(ncol, cl[c].color + 1);
-        std::vector<std::vector<VecChainTask<S>>> ch(ncol);
+        // largest k sent to the one-wave chain kernel (64; SPAQR_WAVE_CHAINS=0 turns it off)
+        static const int wave_chain_k = getenv("SPAQR_WAVE_CHAINS") ? std::min(64, atoi(getenv("SPAQR_WAVE_CHAINS"))) : 64;
+        static const int pipe_chain_k = getenv("SPAQR_PIPE_CHAINS") ? std::min(512, atoi(getenv("SPAQR_PIPE_CHAINS"))) : 512;
+        std::vector<std::array<std::vector<VecChainTask<S>>, 4>> ch(ncol);
         std::vector<VecChainLink<S>> links;
         std::vector<VecTrsvTask<S>> tr;
         std::vector<VecGemvLink<S>> tl;
@@ -1182,12 +1189,21 @@ int Tree::build_solve_schedule() {
             const Edge& es = ed[C.self_edge];
             q.push_back({(const S*)es.A, (const S*)C.T, X0 + C.xoff, es.rows, es.cols, es.rows, 1});
             VecChainTask<S> t{X0 + C.xoff, C.cols, (int)links.size(), 0, 0};
+            int rmin = INT_MAX, rmax = 0;
             for (size_t j = 1; j < C.out.size(); ++j) {
                 const Edge& eo = ed[C.out[j]];
                 links.push_back({(const S*)eo.A, (const S*)eo.T, X0 + cl[eo.n2].xoff, eo.rows, eo.rows});
+                rmin = std::min(rmin, eo.rows);
+                rmax = std::max(rmax, eo.rows);
             }
             t.nlinks = (int)links.size() - t.link0;
-            if (t.nlinks > 0) ch[C.color].push_back(t);
+            int kind = 3;
+            if (C.cols >= 1 && rmin >= 1) {
+                if (C.cols <= wave_chain_k && rmax <= 64) kind = 0;
+                else if (C.cols <= pipe_chain_k && rmax <= 256) kind = 1;
+                else if (C.cols <= pipe_chain_k && rmax <= 512) kind = 2;
+            }
+            if (t.nlinks > 0) ch[C.color][kind].push_back(t);
             VecTrsvTask<S> tt{(const S*)es.A, X0 + C.xoff, C.cols, es.rows, (int)tl.size(), 0};
             for (int e : C.in) {
                 const Edge& ei = ed[e];
@@ -1205,8 +1221,19 @@ int Tree::build_solve_schedule() {
         }
         lv.q = upload_vec(q);
         lv.n_q = (int)q.size();
+        if (getenv("SPAQR_DEBUG_HOST")) {
+            int cnt[4] = {0, 0, 0, 0}, kmax[4] = {0, 0, 0, 0};
+            for (int q = 0; q < ncol; ++q)
+                for (int c = 0; c < 4; ++c)
+                    for (auto& t : ch[q][c]) { ++cnt[c]; kmax[c] = std::max(kmax[c], t.k); }
+            fprintf(stderr, "[solve L%d] %d colours; chains (count/max k): wave %d/%d pipe1 %d/%d pipe2 %d/%d plain %d/%d\n",
+                    L, ncol, cnt[0], kmax[0], cnt[1], kmax[1], cnt[2], kmax[2], cnt[3], kmax[3]);
+        }
         lv.chain_links = upload_vec(links);
-        for (int k = 0; k < ncol; ++k) lv.chain.push_back({upload_vec(ch[k]), (int)ch[k].size()});
+        for (int k = 0; k < ncol; ++k) {
+            lv.chain.emplace_back();
+            for (int c = 0; c < 4; ++c) lv.chain.back()[c] = {upload_vec(ch[k][c]), (int)ch[k][c].size()};
+        }
         lv.spars = upload_vec(sp);
         lv.n_spars = (int)sp.size();
         lv.trsv = upload_vec(tr);
@@ -1224,8 +1251,13 @@ int Tree::solve_t(double* x) {
         auto& lv = sched->lv[L];
         if (L > 0) launch_vec_copy<S>((const VecCopyTask<S>*)lv.merge_fwd, lv.n_merge_fwd, stream_);
         launch_vec_q<S>((const VecQTask<S>*)lv.q, lv.n_q, stream_);
-        for (auto& c : lv.chain)
-            launch_vec_chain<S>((const VecChainTask<S>*)c.first, (const VecChainLink<S>*)lv.chain_links, c.second, stream_);
+        const auto links = (const VecChainLink<S>*)lv.chain_links;
+        for (auto& c : lv.chain) {
+            launch_vec_chain_wave<S>((const VecChainTask<S>*)c[0].first, links, c[0].second, stream_);
+            launch_vec_chain_pipe<S>((const VecChainTask<S>*)c[1].first, links, c[1].second, 1, stream_);
+            launch_vec_chain_pipe<S>((const VecChainTask<S>*)c[2].first, links, c[2].second, 2, stream_);
+            launch_vec_chain<S>((const VecChainTask<S>*)c[3].first, links, c[3].second, stream_);
+        }
         launch_vec_spars<S>((const VecSparsTask<S>*)lv.spars, lv.n_spars, 1, stream_);
     }
     for (int L = nlevels - 1; L >= 0; --L) {
