@@ -786,6 +786,129 @@ __global__ __launch_bounds__(WG) void k_tpqrt(const TpqrtTask<S>* __restrict__ t
     }
 }
 
+// tpqrt chain of a small interior (k <= 64, every link 0..64 rows) on one 64-thread
+// workgroup, column-major in the lanes: lane c holds column c of the link's B in registers,
+// R sits in LDS (step j reads and writes row j only).  A step needs no cross-lane reduction:
+// lane j forms the reflector from its own column, broadcasts it through LDS, and one
+// lane-local dot product per lane gives both the trailing update (columns > j) and the
+// panel's Gram column v_p^T v_j (columns < j) from which T is built at the panel's end.
+// Same reflectors (sign rule included), R, V and blocked T as the workgroup kernel.
+__device__ __forceinline__ int rpos(int j) { return j * 64 - (j * (j - 1)) / 2; }
+
+template <typename S>
+struct TpqrtWaveSmem {
+    S R[64 * 65 / 2];  // upper triangle of R, row by row: R(j, c) at rpos(j) + c - j
+    S v[64];       // reflector of the current step
+    S G[32 * 33];  // G(p, j) = v_p^T v_j of the current panel at j * 33 + p
+    S Ts[32 * 33]; // T(q, j) of the current panel at j * 33 + q
+    S tau[32];
+};
+
+template <typename S>
+__global__ __launch_bounds__(64) void k_tpqrt_wave(const TpqrtTask<S>* __restrict__ tasks,
+                                                   const TpqrtLink<S>* __restrict__ links) {
+    __shared__ TpqrtWaveSmem<S> sm;
+    const TpqrtTask<S> t = tasks[blockIdx.x];
+    const int c = threadIdx.x, k = t.k, cc = min(c, k - 1);
+    for (int j = 0; j <= c && j < k; ++j) sm.R[rpos(j) + c - j] = (c < k) ? t.R[j + (size_t)cc * t.ldr] : S(0);
+    for (int li = 0; li < t.nlinks; ++li) {
+        const TpqrtLink<S> L = links[t.link0 + li];
+        const int rows = L.rows;
+        S b[64];
+        if (rows > 0) {
+            const S* __restrict__ Bc = L.B + (size_t)cc * L.ldb;
+#pragma unroll
+            for (int i = 0; i < 64; ++i) b[i] = Bc[min(i, rows - 1)];
+#pragma unroll
+            for (int i = 0; i < 64; ++i) b[i] = (i < rows && c < k) ? b[i] : S(0);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 64; ++i) b[i] = S(0);
+        }
+        for (int p0 = 0; p0 < k; p0 += NB) {
+            const int kb = min(NB, k - p0);
+            for (int jj = 0; jj < kb; ++jj) {
+                const int j = p0 + jj;
+                wave_sync();  // the previous step's readers of sm.v are done
+                if (c == j) {
+                    S s0 = S(0), s1 = S(0), s2 = S(0), s3 = S(0);
+#pragma unroll
+                    for (int i = 0; i < 64; i += 4) {
+                        s0 += b[i] * b[i];
+                        s1 += b[i + 1] * b[i + 1];
+                        s2 += b[i + 2] * b[i + 2];
+                        s3 += b[i + 3] * b[i + 3];
+                    }
+                    const S ss = (s0 + s1) + (s2 + s3);
+                    const S alpha = sm.R[rpos(j)];
+                    S tau, scal, beta;
+                    if (ss == S(0)) {
+                        tau = S(0); scal = S(1); beta = alpha;
+                    } else {
+                        beta = -householder_norm_signed(alpha, ss);
+                        tau = (beta - alpha) / beta;
+                        scal = S(1) / (alpha - beta);
+                    }
+#pragma unroll
+                    for (int i = 0; i < 64; ++i) {
+                        b[i] *= scal;
+                        sm.v[i] = b[i];
+                    }
+                    sm.R[rpos(j)] = beta;
+                    sm.tau[jj] = tau;
+                }
+                wave_sync();
+                const S tau = sm.tau[jj];
+                S d0 = S(0), d1 = S(0), d2 = S(0), d3 = S(0);
+#pragma unroll
+                for (int i = 0; i < 64; i += 4) {
+                    d0 += b[i] * sm.v[i];
+                    d1 += b[i + 1] * sm.v[i + 1];
+                    d2 += b[i + 2] * sm.v[i + 2];
+                    d3 += b[i + 3] * sm.v[i + 3];
+                }
+                const S d = (d0 + d1) + (d2 + d3);
+                if (c > j && c < k) {
+                    const S w = tau * (sm.R[rpos(j) + c - j] + d);
+                    sm.R[rpos(j) + c - j] -= w;
+#pragma unroll
+                    for (int i = 0; i < 64; ++i) b[i] -= w * sm.v[i];
+                } else if (c >= p0 && c < j) {
+                    sm.G[jj * 33 + (c - p0)] = d;
+                }
+            }
+            wave_sync();
+            // the panel's T: lane q builds row q, T(q, j) = -tau_j sum_{p=q}^{j-1} T(q, p) G(p, j);
+            // a row depends on nothing but itself, so the lanes run independently
+            const int q = c;
+            if (q < kb) {
+                S* Tq = L.T + (size_t)p0 * NB + q;
+                for (int jj = 0; jj < kb; ++jj) {
+                    S v = S(0);
+                    if (q == jj) {
+                        v = sm.tau[jj];
+                    } else if (q < jj) {
+                        S s = S(0);
+                        for (int p = q; p < jj; ++p) s += sm.Ts[p * 33 + q] * sm.G[jj * 33 + p];
+                        v = -sm.tau[jj] * s;
+                    }
+                    sm.Ts[jj * 33 + q] = v;
+                    Tq[(size_t)jj * NB] = v;
+                }
+            }
+        }
+        if (rows > 0 && c < k) {
+            S* Bc = L.B + (size_t)c * L.ldb;
+#pragma unroll
+            for (int i = 0; i < 64; ++i)
+                if (i < rows) Bc[i] = b[i];
+        }
+    }
+    wave_sync();
+    for (int j = 0; j <= min(c, k - 1); ++j)
+        if (c < k) t.R[j + (size_t)c * t.ldr] = sm.R[rpos(j) + c - j];
+}
+
 template <typename S>
 __global__ __launch_bounds__(WG) void k_tpmqrt(const TpmqrtTask<S>* __restrict__ tasks,
                                                const TpmqrtLink<S>* __restrict__ links) {
@@ -1571,6 +1694,10 @@ void launch_tpqrt(const TpqrtTask<S>* d, const TpqrtLink<S>* l, int n, hipStream
     if (n > 0) hipLaunchKernelGGL(k_tpqrt<S>, dim3(n), dim3(WG), 0, s, d, l);
 }
 template <typename S>
+void launch_tpqrt_wave(const TpqrtTask<S>* d, const TpqrtLink<S>* l, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_tpqrt_wave<S>, dim3(n), dim3(64), 0, s, d, l);
+}
+template <typename S>
 void launch_tpmqrt(const TpmqrtTask<S>* d, const TpmqrtLink<S>* l, int n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_tpmqrt<S>, dim3(n), dim3(WG), 0, s, d, l);
 }
@@ -1660,6 +1787,7 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
     template void launch_panel_qr<S>(const PanelTask<S>*, int, bool, hipStream_t);                    \
     template void launch_panel_apply<S>(const PanelApplyTask<S>*, int, bool, hipStream_t);           \
     template void launch_tpqrt<S>(const TpqrtTask<S>*, const TpqrtLink<S>*, int, hipStream_t);       \
+    template void launch_tpqrt_wave<S>(const TpqrtTask<S>*, const TpqrtLink<S>*, int, hipStream_t);       \
     template void launch_tpmqrt<S>(const TpmqrtTask<S>*, const TpmqrtLink<S>*, int, hipStream_t);    \
     template void launch_copy<S>(const CopyTask<S>*, int, hipStream_t);                              \
     template void launch_geqp3<S>(const Geqp3Task<S>*, int, double, double, hipStream_t);                    \
@@ -1721,6 +1849,33 @@ __device__ __forceinline__ S ld_coherent(const S* p) {
 template <typename S>
 __device__ __forceinline__ void st_coherent(S* p, S v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// dst[0:n] = src[0:n] (coherent reads), eight loads in flight per thread before any wait: a
+// plain loop waits on every load, and a published column of a few thousand rows would cost
+// that many serial memory round trips per pivot step.
+template <typename S>
+__device__ __forceinline__ void coherent_read(const S* src, S* dst, int n) {
+    for (int r0 = threadIdx.x; r0 < n; r0 += 8 * WG) {
+        S v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = ld_coherent(src + min(r0 + u * WG, n - 1));
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (r0 + u * WG < n) dst[r0 + u * WG] = v[u];
+    }
+}
+// dst[0:n] = src[0:n] (coherent writes), the source read eight rows at a time.
+template <typename S>
+__device__ __forceinline__ void coherent_write(const S* src, S* dst, int n) {
+    for (int r0 = threadIdx.x; r0 < n; r0 += 8 * WG) {
+        S v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = src[min(r0 + u * WG, n - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (r0 + u * WG < n) st_coherent(dst + r0 + u * WG, v[u]);
+    }
 }
 
 // record layout inside a slot (k + 4 values): M_b, v_cand, pos_cand, unused, column[k]
@@ -1815,12 +1970,24 @@ __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __res
                 st_coherent(my + 1, ccol >= 0 ? vn1[ccol] : S(-1));
                 st_coherent(my + 2, ccol >= 0 ? (S)pos[ccol] : S(-1));
             }
-            if (ccol >= 0)
-                for (int r = tid; r < k; r += WG) st_coherent(my + 4 + r, slice[r + (size_t)ccol * k]);
+            if (ccol >= 0) coherent_write(slice + (size_t)ccol * k, my + 4, k);
             task_barrier(t.bar, P, passed, err);
-            // global maximum, band, ambiguity
+            // global maximum, band, ambiguity; the records of q = tid and tid + WG (P <= 2 WG, the
+            // co-resident limit) are read once, all three fields in one round trip
+            S mrec[2], vrec[2];
+            int prec[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const S* rq = slot + (size_t)min(tid + h * WG, P - 1) * rec;
+                mrec[h] = ld_coherent(rq);
+                vrec[h] = ld_coherent(rq + 1);
+                prec[h] = (int)ld_coherent(rq + 2);
+            }
             S M = S(-1);
-            for (int q = tid; q < P; q += WG) M = fmax(M, ld_coherent(slot + (size_t)q * rec));
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                if (tid + h * WG < P) M = fmax(M, mrec[h]);
+            for (int q = tid + 2 * WG; q < P; q += WG) M = fmax(M, ld_coherent(slot + (size_t)q * rec));
             M = wave_max(M);
             if (lane == 0) red_v[wv] = M;
             __syncthreads();
@@ -1829,9 +1996,19 @@ __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __res
             const S gthr = (tie_rel > 0.0) ? M * (S)(1.0 - tie_rel) : M;
             int best = INT_MAX, bq = -1, amb = 0;
             for (int q = tid; q < P; q += WG) {
-                const S* rq = slot + (size_t)q * rec;
-                S mq = ld_coherent(rq), vq = ld_coherent(rq + 1);
-                int pq = (int)ld_coherent(rq + 2);
+                S mq, vq;
+                int pq;
+                if (q < tid + 2 * WG) {
+                    const int h = (q - tid) / WG;
+                    mq = h ? mrec[1] : mrec[0];
+                    vq = h ? vrec[1] : vrec[0];
+                    pq = h ? prec[1] : prec[0];
+                } else {
+                    const S* rq = slot + (size_t)q * rec;
+                    mq = ld_coherent(rq);
+                    vq = ld_coherent(rq + 1);
+                    pq = (int)ld_coherent(rq + 2);
+                }
                 if (vq >= gthr && pq >= 0 && pq < best) { best = pq; bq = q; }
                 if (mq >= gthr && !(vq >= gthr)) amb = 1;
             }
@@ -1880,8 +2057,7 @@ __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __res
                     st_coherent(my2 + 1, ccol >= 0 ? vn1[ccol] : S(-1));
                     st_coherent(my2 + 2, ccol >= 0 ? (S)pos[ccol] : S(-1));
                 }
-                if (ccol >= 0)
-                    for (int r = tid; r < k; r += WG) st_coherent(my2 + 4 + r, slice[r + (size_t)ccol * k]);
+                if (ccol >= 0) coherent_write(slice + (size_t)ccol * k, my2 + 4, k);
                 task_barrier(t.bar, P, passed, err);
                 int b2 = INT_MAX, q2 = -1;
                 for (int q = tid; q < P; q += WG) {
@@ -1907,7 +2083,7 @@ __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __res
             }
             if (winq < 0) { rank = j; break; }  // no finite candidate in any block (NaN input)
             const S* wr = wslot + (size_t)winq * rec;
-            for (int r = tid; r < k; r += WG) vbuf[r] = ld_coherent(wr + 4 + r);
+            coherent_read(wr + 4, vbuf, k);
             if (tid == 0) sh_i[0] = (int)ld_coherent(wr + 2);
             if (winq == p) pv_col = ccol;
             __syncthreads();

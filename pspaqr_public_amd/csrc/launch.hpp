@@ -14,6 +14,8 @@ template <typename S> void launch_larfb(const LarfbTask<S>* d, int n, hipStream_
 template <typename S> void launch_panel_qr(const PanelTask<S>* d, int n, bool pent, hipStream_t s);
 template <typename S> void launch_panel_apply(const PanelApplyTask<S>* d, int n, bool pent, hipStream_t s);
 template <typename S> void launch_tpqrt(const TpqrtTask<S>* d, const TpqrtLink<S>* l, int n, hipStream_t s);
+// Chains with k <= 64 and links of at most 64 rows: one 64-thread workgroup per chain.
+template <typename S> void launch_tpqrt_wave(const TpqrtTask<S>* d, const TpqrtLink<S>* l, int n, hipStream_t s);
 template <typename S> void launch_tpmqrt(const TpmqrtTask<S>* d, const TpmqrtLink<S>* l, int n, hipStream_t s);
 template <typename S> void launch_copy(const CopyTask<S>* d, int n, hipStream_t s);
 template <typename S> void launch_geqp3(const Geqp3Task<S>* d, int n, double tol, double tie_rel, hipStream_t s);

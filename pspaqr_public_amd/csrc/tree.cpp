@@ -178,16 +178,48 @@ void Tree::Arena::release() {
     reset();
 }
 
+char* Tree::PinnedArena::alloc(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (bytes == 0) bytes = 256;
+    while (bytes > left) {
+        if (next < chunks.size() && chunks[next].second >= bytes) {
+            cur = chunks[next].first;
+            left = chunks[next].second;
+            ++next;
+            break;
+        }
+        if (next < chunks.size()) { ++next; continue; }
+        const size_t want = std::max(bytes, (size_t)64 << 20);
+        void* p = nullptr;
+        HIPCHK(hipHostMalloc(&p, want, hipHostMallocDefault));
+        chunks.push_back({(char*)p, want});
+        next = chunks.size();
+        cur = (char*)p;
+        left = want;
+    }
+    char* r = cur;
+    cur += bytes;
+    left -= bytes;
+    return r;
+}
+
+void Tree::PinnedArena::release() {
+    for (auto& c : chunks) (void)hipHostFree(c.first);
+    chunks.clear();
+    reset();
+}
+
 void* Tree::upload(const void* h, size_t bytes) {
     void* d = desc_arena.alloc(bytes);
-    host_keep.emplace_back((const char*)h, (const char*)h + bytes);
-    HIPCHK(hipMemcpyAsync(d, host_keep.back().data(), bytes, hipMemcpyHostToDevice, stream_));
+    char* p = staging.alloc(bytes);
+    memcpy(p, h, bytes);
+    HIPCHK(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, stream_));
     return d;
 }
 
 void Tree::sync_stream() {
     HIPCHK(hipStreamSynchronize(stream_));
-    host_keep.clear();
+    staging.reset();  // every upload queued so far has completed
     for (auto& p : prof_pending) {
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, p.second.first, p.second.second));
@@ -241,6 +273,7 @@ Tree::~Tree() {
     delete sched;
     arena.release();
     desc_arena.release();
+    staging.release();
     if (d_rowptr) (void)hipFree(d_rowptr);
     if (d_col) (void)hipFree(d_col);
     if (d_val) (void)hipFree(d_val);
@@ -593,7 +626,7 @@ int Tree::factorize_t() {
     }
     // host-phase timers (SPAQR_DEBUG_HOST=1 prints them per factorization)
     static const bool dbg_host = getenv("SPAQR_DEBUG_HOST") != nullptr;
-    double hms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double hms[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     auto hclock = [] { return std::chrono::steady_clock::now(); };
     auto hacc = [&](int i, std::chrono::steady_clock::time_point& t) {
         auto n = std::chrono::steady_clock::now();
@@ -792,10 +825,16 @@ int Tree::factorize_t() {
             std::vector<GeqrtTask<S>> geq_small, geq_big, sgeq_small, sgeq_big;
             for (const auto& t : geq) (std::min(t.m, t.n) >= kBigFront ? geq_big : geq_small).push_back(t);
             for (const auto& t : sgeq) (std::min(t.m, t.n) >= kBigFront ? sgeq_big : sgeq_small).push_back(t);
-            std::vector<TpqrtTask<S>> tpq_small, tpq_big;
-            // tpqrt chains stay on the one-workgroup kernel: with the register-resident panel QR
-            // the host-driven panel loop (tpqrt_big) costs more in launch gaps than it gains
-            for (const auto& t : tpq) tpq_small.push_back(t);
+            std::vector<TpqrtTask<S>> tpq_wave, tpq_small, tpq_big;
+            // small chains (k <= 64, links of <= 64 rows) go one per wave; the rest stay on the
+            // one-workgroup kernel: with the register-resident panel QR the host-driven panel
+            // loop (tpqrt_big) costs more in launch gaps than it gains
+            static const bool tpqrt_waves = getenv("SPAQR_TPQRT_WAVE") == nullptr || atoi(getenv("SPAQR_TPQRT_WAVE")) != 0;
+            for (const auto& t : tpq) {
+                bool small = tpqrt_waves && t.k <= 64;
+                for (int j = 0; j < t.nlinks && small; ++j) small = tpql[t.link0 + j].rows <= 64;
+                (small ? tpq_wave : tpq_small).push_back(t);
+            }
             GeqrtTask<S>* d_geq = upload_vec(geq_small);
             std::vector<LarfbTask<S>> lfb_c, lfb_t, sapp_c, sapp_t;
             split_larfb(lfb, lfb_c, lfb_t);
@@ -803,6 +842,7 @@ int Tree::factorize_t() {
             LarfbTask<S>* d_lfb = upload_vec(lfb_c);
             LarfbTask<S>* d_lfb_t = upload_vec(lfb_t);
             TpqrtTask<S>* d_tpq = upload_vec(tpq_small);
+            TpqrtTask<S>* d_tpqw = upload_vec(tpq_wave);
             TpqrtLink<S>* d_tpql = upload_vec(tpql);
             TpmqrtLink<S>* d_tpml = upload_vec(tpml);
             std::vector<TpmqrtTask<S>*> d_tpm(ncolors);
@@ -830,8 +870,9 @@ int Tree::factorize_t() {
             if (!tpq.empty()) {
                 int nl = 0;
                 timed(P_TPQRT, 0, (long)tpq.size(), [&] {
+                    launch_tpqrt_wave<S>(d_tpqw, d_tpql, (int)tpq_wave.size(), stream_);
                     launch_tpqrt<S>(d_tpq, d_tpql, (int)tpq_small.size(), stream_);
-                    nl = !tpq_small.empty() + tpqrt_big<S>(tpq_big, tpql);
+                    nl = !tpq_wave.empty() + !tpq_small.empty() + tpqrt_big<S>(tpq_big, tpql);
                 });
                 if (profiling) prof[P_TPQRT].launches += nl;
             }
@@ -906,6 +947,39 @@ int Tree::factorize_t() {
 
         // ---- merge (src/ptree.cpp:1940-2123)
         for (int c : ints) cl[c].eliminated = true;
+        // The sparsify apply's right transform (A <- A Q_n, in place on the interface blocks, row
+        // chunks) needs nothing from the merge: it is queued first and runs while the host merges.
+        constexpr int kSparsRows = 256, kSparsCols = 128;
+        {
+            std::vector<SparsApplyTask<S>> spa1;
+            for (int so : itf) {
+                const Cluster& SO = cl[so];
+                if (!SO.sparsified) continue;
+                for (int e : SO.out) {
+                    const Edge& eo = ed[e];
+                    const int no = eo.n2;
+                    if (no == so || cl[no].eliminated) continue;
+                    SparsApplyTask<S> u{};
+                    u.Vn = (const S*)SO.Qsp;
+                    u.Tn = (const S*)SO.Tsp;
+                    u.ldvn = eo.cols;
+                    u.nc = eo.cols;
+                    u.lda = eo.rows;
+                    u.kn = eo.cols;
+                    u.kc = eo.rows;
+                    u.rn = SO.cols;
+                    u.pad = 1;
+                    for (int r0 = 0; r0 < eo.rows; r0 += kSparsRows) {
+                        u.A = Ap(e) + r0;
+                        u.mc = std::min(kSparsRows, eo.rows - r0);
+                        spa1.push_back(u);
+                    }
+                }
+            }
+            SparsApplyTask<S>* d_spa1 = upload_vec(spa1);
+            if (!spa1.empty())
+                timed(P_SPARS, 1, 0, [&] { launch_spars_apply<S>(d_spa1, (int)spa1.size(), stream_); });
+        }
         const int Lp = L + 1;
         current_bottom = Lp;
         for (int p : bottoms[Lp]) {
@@ -922,25 +996,35 @@ int Tree::factorize_t() {
         }
         const size_t ne_merge = ed.size();
         {
-            std::vector<int> nbrs, d2;
+            // de-duplicate through stamp arrays (stamp = the parent being merged), then sort
+            // only the distinct entries
+            std::vector<int> nbrs, d2, seen_n(cl.size(), -1), seen_d(cl.size(), -1);
             for (int p : bottoms[Lp]) {
                 nbrs.clear();
-                d2.assign(cl[p].dist2.begin(), cl[p].dist2.end());
+                d2.clear();
+                for (int d : cl[p].dist2)
+                    if (seen_d[d] != p) { seen_d[d] = p; d2.push_back(d); }
                 for (int ch : cl[p].children) {
-                    for (int e : cl[ch].out)
-                        if (!cl[ed[e].n2].eliminated) nbrs.push_back(cl[ed[e].n2].parent);
-                    for (int d : cl[ch].dist2)
-                        if (cl[d].lvl() >= Lp) d2.push_back(cl[d].parent);
+                    for (int e : cl[ch].out) {
+                        const int n2 = ed[e].n2;
+                        if (cl[n2].eliminated) continue;
+                        const int q = cl[n2].parent;
+                        if (seen_n[q] != p) { seen_n[q] = p; nbrs.push_back(q); }
+                    }
+                    for (int d : cl[ch].dist2) {
+                        if (cl[d].lvl() < Lp) continue;
+                        const int q = cl[d].parent;
+                        if (seen_d[q] != p) { seen_d[q] = p; d2.push_back(q); }
+                    }
                 }
                 std::sort(d2.begin(), d2.end());
-                d2.erase(std::unique(d2.begin(), d2.end()), d2.end());
                 cl[p].dist2 = d2;
                 std::sort(nbrs.begin(), nbrs.end());
-                nbrs.erase(std::unique(nbrs.begin(), nbrs.end()), nbrs.end());
                 for (int nn : nbrs) new_edge(p, nn);
                 sort_out(cl[p]);
             }
         }
+        hacc(3, ht);
         {
             size_t tot = 0;
             for (size_t i = ne_merge; i < ed.size(); ++i) tot += (size_t)ed[i].rows * ed[i].cols;
@@ -951,7 +1035,7 @@ int Tree::factorize_t() {
                 off += (size_t)ed[i].rows * ed[i].cols;
             }
         }
-        hacc(3, ht);
+        hacc(7, ht);
         std::vector<SparsApplyTask<S>> spa;
         std::vector<int> slot(cl.size(), -1);  // parent's out edge by target, for the parent being filled
         int slot_owner = -1;
@@ -997,23 +1081,15 @@ int Tree::factorize_t() {
                 spa.push_back(t);
             }
         }
-        // Two launches so that large blocks spread over many workgroups: A Q_n acts on each row
-        // independently (row chunks), Q_c^T A and the scatter on each column (column chunks).
-        std::vector<SparsApplyTask<S>> spa1, spa2;
-        constexpr int kSparsRows = 256, kSparsCols = 128;
+        hacc(8, ht);
+        // The left transform Q_c^T A and the scatter act on each column: column chunks, so that
+        // large blocks spread over many workgroups (the right transform was queued before the merge).
+        std::vector<SparsApplyTask<S>> spa2;
         for (const auto& t : spa) {
             if (t.ident) {
                 spa2.push_back(t);
                 continue;
             }
-            if (t.Vn)
-                for (int r0 = 0; r0 < t.mc; r0 += kSparsRows) {
-                    SparsApplyTask<S> u = t;
-                    u.A = t.A + r0;
-                    u.mc = std::min(kSparsRows, t.mc - r0);
-                    u.pad = 1;
-                    spa1.push_back(u);
-                }
             for (int c0 = 0; c0 < t.rn; c0 += kSparsCols) {
                 SparsApplyTask<S> u = t;
                 u.A = t.A + (size_t)c0 * t.lda;
@@ -1023,13 +1099,10 @@ int Tree::factorize_t() {
                 spa2.push_back(u);
             }
         }
-        SparsApplyTask<S>* d_spa1 = upload_vec(spa1);
         SparsApplyTask<S>* d_spa2 = upload_vec(spa2);
+        hacc(9, ht);
         if (!spa.empty())
-            timed(P_SPARS, 2, (long)spa.size(), [&] {
-                launch_spars_apply<S>(d_spa1, (int)spa1.size(), stream_);
-                launch_spars_apply<S>(d_spa2, (int)spa2.size(), stream_);
-            });
+            timed(P_SPARS, 1, (long)spa.size(), [&] { launch_spars_apply<S>(d_spa2, (int)spa2.size(), stream_); });
         HIPCHK(hipGetLastError());
         hacc(4, ht);
     }
@@ -1041,8 +1114,10 @@ int Tree::factorize_t() {
     sync_stream();
     hacc(6, ht2);
     if (dbg_host)
-        fprintf(stderr, "[host] fillin %.1f  descr %.1f  launch+ranks %.1f  merge %.1f  spars %.1f  tail-sync %.1f  solve-sched %.1f ms\n",
-                hms[0], hms[1], hms[2], hms[3], hms[4], hms[5], hms[6]);
+        fprintf(stderr,
+                "[host] fillin %.1f  descr %.1f  launch+ranks %.1f  merge %.1f (+blocks %.1f)  spars build %.1f split+upload "
+                "%.1f launch %.1f  tail-sync %.1f  solve-sched %.1f ms\n",
+                hms[0], hms[1], hms[2], hms[3], hms[7], hms[8], hms[9], hms[4], hms[5], hms[6]);
     t_factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
 }

@@ -215,7 +215,17 @@ private:
     template <typename S> int build_solve_schedule();
     template <typename S> int solve_t(double* x);
     void* upload(const void* h, size_t bytes);
-    std::vector<std::vector<char>> host_keep;  // sources of in-flight async uploads
+    // pinned staging for descriptor uploads: one host copy, then an asynchronous DMA; the
+    // chunks are reused after every stream synchronisation
+    struct PinnedArena {
+        std::vector<std::pair<char*, size_t>> chunks;
+        size_t next = 0;
+        char* cur = nullptr;
+        size_t left = 0;
+        char* alloc(size_t bytes);
+        void reset() { next = 0; cur = nullptr; left = 0; }
+        void release();
+    } staging;
     void sync_stream();
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_pending;
     template <class F> void timed(int cat, long launches, long tasks, F&& f);
