@@ -18,9 +18,11 @@
 //   spmv          ParTree::spmv (src/ptree.cpp:3212-3262)
 //   GMRES         dist_gmres (src/is.cpp:447-527) and LUSolve (src/util.cpp:1019-1032)
 //
-// Orders the reference leaves to its task scheduler or to pointer hashing are fixed here
-// to cluster order: edgesIn lists, fill-in pair order, the order in which interiors of one
-// level update a shared block (and so order_of_trans), and the solve's reductions.
+// Orders the reference leaves to its task scheduler or to pointer hashing are fixed here:
+// edgesIn lists, fill-in pair order and the solve's reductions follow cluster order; the
+// interiors of one level update shared blocks (ssrfb, and the solve's order_of_trans) colour
+// by colour, with the DSatur colouring of dsatur_colours below and cluster order inside a
+// colour (a colour's interiors share no out-neighbour, so that inner order cannot matter).
 // The RRQR is full geqp3 + choose_rank (the reference's non-MKL branch,
 // src/ptree.cpp:411-416).  Only square systems (rperm == cperm, c2r_count == 1).
 #include <algorithm>
@@ -28,11 +30,13 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <set>
 #include <stdexcept>
+#include <tuple>
 #include <vector>
 
 #include "dense.hpp"
@@ -294,6 +298,48 @@ static std::vector<int> init_permutation(int nlevels, const std::vector<ClusterI
 }
 
 // ------------------------------------------------------------------ clusters and edges
+// Colours for the interiors of one level (index = position in cluster order; nb[i] = the
+// orders of interior i's out-neighbours other than itself).  Two interiors conflict when they
+// share an out-neighbour.  DSatur: repeatedly colour the uncoloured interior with the most
+// distinct colours among its conflicts (then the most conflicts, then the lowest index) with
+// the smallest colour none of its conflicts holds.
+static std::vector<int> dsatur_colours(const std::vector<std::vector<int>>& nb) {
+    const int V = (int)nb.size();
+    std::map<int, std::vector<int>> sharing;
+    for (int i = 0; i < V; ++i)
+        for (int n : nb[i]) sharing[n].push_back(i);
+    std::vector<std::vector<int>> adj(V);
+    for (auto& kv : sharing)
+        for (int a : kv.second)
+            for (int b : kv.second)
+                if (a != b) adj[a].push_back(b);
+    for (auto& a : adj) {
+        std::sort(a.begin(), a.end());
+        a.erase(std::unique(a.begin(), a.end()), a.end());
+    }
+    std::vector<int> col(V, -1), sat(V, 0);
+    std::vector<std::vector<char>> seen(V);
+    std::set<std::tuple<int, int, int>> queue;  // (-saturation, -conflicts, index)
+    for (int i = 0; i < V; ++i) queue.insert(std::make_tuple(0, -(int)adj[i].size(), i));
+    while (!queue.empty()) {
+        const int i = std::get<2>(*queue.begin());
+        queue.erase(queue.begin());
+        int c = 0;
+        while (c < (int)seen[i].size() && seen[i][c]) ++c;
+        col[i] = c;
+        for (int j : adj[i]) {
+            if (col[j] >= 0) continue;
+            if ((int)seen[j].size() <= c) seen[j].resize(c + 1, 0);
+            if (seen[j][c]) continue;
+            queue.erase(std::make_tuple(-sat[j], -(int)adj[j].size(), j));
+            seen[j][c] = 1;
+            ++sat[j];
+            queue.insert(std::make_tuple(-sat[j], -(int)adj[j].size(), j));
+        }
+    }
+    return col;
+}
+
 struct Cluster;
 struct Edge {  // include/edge.h:14-58: block A(n2, n1), rows(n2) x cols(n1)
     Cluster* n1;
@@ -358,6 +404,7 @@ struct Tree {
     std::vector<int> cperm;
     std::vector<ClusterID> cpermed;
     std::vector<std::vector<Cluster*>> bottoms, interiors, interfaces;
+    std::vector<std::vector<Cluster*>> update_order;  // per level: interiors by (colour, cluster order)
     std::vector<std::unique_ptr<Cluster>> owned_c;
     std::vector<std::unique_ptr<Edge>> owned_e;
     int max_order = 0;
@@ -577,8 +624,25 @@ struct Tree {
             }
         }
         // ssrfb_edges: for every m in edgesIn(c), the chain over n in edgesOut(c)
-        // (src/ptree.cpp:223-266, 1366-1406); interiors in cluster order.
-        for (Cluster* c : ints) {
+        // (src/ptree.cpp:223-266, 1366-1406); interiors colour by colour.
+        {
+            std::vector<std::vector<int>> nb(ints.size());
+            for (size_t i = 0; i < ints.size(); ++i)
+                for (size_t j = 1; j < ints[i]->edgesOut.size(); ++j) nb[i].push_back(ints[i]->edgesOut[j]->n2->order);
+            const std::vector<int> col = dsatur_colours(nb);
+            std::vector<int> idx(ints.size());
+            for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return col[a] < col[b]; });
+            if ((int)update_order.size() <= current_bottom) update_order.resize(current_bottom + 1);
+            update_order[current_bottom].clear();
+            for (int i : idx) update_order[current_bottom].push_back(ints[i]);
+            if (getenv("SPAQR_ORACLE_COLOURS")) {
+                int nc = 0;
+                for (int c : col) nc = std::max(nc, c + 1);
+                fprintf(stderr, "[oracle] level %d: %zu interiors, %d colours\n", current_bottom, ints.size(), nc);
+            }
+        }
+        for (Cluster* c : update_order[current_bottom]) {
             int k = c->cols();
             int nb = std::min(32, k);
             for (Edge* cm : c->edgesIn) {
@@ -759,7 +823,7 @@ struct Tree {
                 int m = c->eself->A.r, k = c->eself->A.c;
                 gemqrt_lt(m, 1, k, k, c->eself->A.data(), m, c->T.data(), k, c->x.data(), m);
             }
-            for (Cluster* c : interiors[L]) {  // fwd_tsqrt :2160-2186, order_of_trans = cluster order
+            for (Cluster* c : update_order[L]) {  // fwd_tsqrt :2160-2186, order_of_trans = colour order
                 int k = c->cols(), nb = std::min(32, k);
                 for (size_t j = 1; j < c->edgesOut.size(); ++j) {
                     Edge* e = c->edgesOut[j];

@@ -30,6 +30,7 @@
 #include <map>
 #include <set>
 #include <stdexcept>
+#include <tuple>
 
 #include "kernels.hpp"
 #include "launch.hpp"
@@ -299,6 +300,59 @@ int Tree::find_out(const Cluster& c, int n2) const {
 
 void Tree::sort_out(Cluster& c) {
     std::sort(c.out.begin(), c.out.end(), [this](int a, int b) { return ed[a].n2 < ed[b].n2; });
+}
+
+// Colours of one level's interiors (two interiors conflict when they share an out-neighbour):
+// DSatur -- the uncoloured interior with the most distinct colours among its conflicts goes
+// next (ties: more conflicts, then earlier in cluster order) and takes the smallest colour no
+// conflict holds.  Interiors then update shared blocks colour by colour.  Returns the number
+// of colours.
+int Tree::colour_interiors(const std::vector<int>& ints) {
+    const int V = (int)ints.size();
+    std::vector<int> slot(cl.size(), -1);  // interior -> index
+    for (int i = 0; i < V; ++i) slot[ints[i]] = i;
+    // interiors by out-neighbour, then conflicts as sorted unique lists
+    std::vector<std::pair<int, int>> by_nb;
+    for (int i = 0; i < V; ++i)
+        for (size_t j = 1; j < cl[ints[i]].out.size(); ++j) by_nb.push_back({ed[cl[ints[i]].out[j]].n2, i});
+    std::sort(by_nb.begin(), by_nb.end());
+    std::vector<std::vector<int>> adj(V);
+    for (size_t a = 0; a < by_nb.size();) {
+        size_t b = a;
+        while (b < by_nb.size() && by_nb[b].first == by_nb[a].first) ++b;
+        for (size_t x = a; x < b; ++x)
+            for (size_t y = a; y < b; ++y)
+                if (x != y) adj[by_nb[x].second].push_back(by_nb[y].second);
+        a = b;
+    }
+    for (auto& a : adj) {
+        std::sort(a.begin(), a.end());
+        a.erase(std::unique(a.begin(), a.end()), a.end());
+    }
+    std::vector<int> col(V, -1), sat(V, 0);
+    std::vector<std::vector<char>> seen(V);
+    std::set<std::tuple<int, int, int>> next;  // (-saturation, -conflicts, index)
+    for (int i = 0; i < V; ++i) next.insert(std::make_tuple(0, -(int)adj[i].size(), i));
+    int ncol = 0;
+    while (!next.empty()) {
+        const int i = std::get<2>(*next.begin());
+        next.erase(next.begin());
+        int c = 0;
+        while (c < (int)seen[i].size() && seen[i][c]) ++c;
+        col[i] = c;
+        ncol = std::max(ncol, c + 1);
+        for (int j : adj[i]) {
+            if (col[j] >= 0) continue;
+            if ((int)seen[j].size() <= c) seen[j].resize(c + 1, 0);
+            if (seen[j][c]) continue;
+            next.erase(std::make_tuple(-sat[j], -(int)adj[j].size(), j));
+            seen[j][c] = 1;
+            ++sat[j];
+            next.insert(std::make_tuple(-sat[j], -(int)adj[j].size(), j));
+        }
+    }
+    for (int i = 0; i < V; ++i) cl[ints[i]].color = col[i];
+    return ncol;
 }
 
 int Tree::new_edge(int c1, int c2) {  // new_edgeOut, src/ptree.cpp:101-120
@@ -699,19 +753,9 @@ int Tree::factorize_t() {
             tk.nlinks = (int)tpql.size() - tk.link0;
             tpq.push_back(tk);
         }
-        // colour layers for the shared-block updates (order-consistent)
-        std::vector<int> last_color(cl.size(), -1);
-        int ncolors = 0;
-        for (int c : ints) {
-            int col = 0;
-            for (size_t j = 1; j < cl[c].out.size(); ++j) {
-                const int lc = last_color[ed[cl[c].out[j]].n2];
-                if (lc >= 0) col = std::max(col, lc + 1);
-            }
-            cl[c].color = col;
-            for (size_t j = 1; j < cl[c].out.size(); ++j) last_color[ed[cl[c].out[j]].n2] = col;
-            ncolors = std::max(ncolors, col + 1);
-        }
+        // colour layers for the shared-block updates: interiors that share an out-neighbour get
+        // different colours, and the colours run in order (the update order of the level)
+        int ncolors = colour_interiors(ints);
         st.colors = ncolors;
         std::vector<std::vector<TpmqrtTask<S>>> tpm(ncolors);
         std::vector<TpmqrtLink<S>> tpml;

@@ -208,6 +208,7 @@ private:
 
     bool want_sparsify(const Cluster& c) const;
     int new_edge(int c1, int c2);
+    int colour_interiors(const std::vector<int>& ints);
     void sort_out(Cluster& c);
     int find_out(const Cluster& c, int n2) const;
     void fillin();
