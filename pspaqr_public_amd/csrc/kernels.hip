@@ -1121,6 +1121,133 @@ __global__ __launch_bounds__(WG) void k_spars_apply(const SparsApplyTask<S>* __r
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Small blocks (at most 64 x 64): one wave per task, four tasks per workgroup, no LDS.  The
+// block sits in registers -- a lane per column when reflectors act on columns (Q^T C), a lane
+// per row when they act on rows (C Q, C R^{-1}) -- so every step is lane-local; the reflector
+// entries are the same for all lanes (wave-uniform reads).  Reflectors are applied one at a
+// time with tau_l = T(l mod 32, l), the diagonal of the blocked T: the same transformation as
+// the blocked kernels.  MR = register length (16, 32 or 64) >= the block's reflector dimension.
+
+// x (MR entries, rows 0..m-1 live) <- H_{k-1} ... H_0 x for the geqrt reflectors V (m x k, ldv).
+// Reflector l is loaded once per wave (lane r holds V(r, l), the next column already in
+// flight) and its entries reach every lane through readlane.
+template <typename S, int MR>
+__device__ __forceinline__ void lane_apply_qt(S (&x)[MR], int m, int k, const S* __restrict__ V, int ldv,
+                                              const S* __restrict__ T) {
+    const int lane = threadIdx.x & 63, rl = min(lane, m - 1);
+    S vn = (k > 0) ? V[rl] : S(0);
+    for (int l = 0; l < k; ++l) {
+        const S vc = vn;
+        if (l + 1 < k) vn = V[rl + (size_t)(l + 1) * ldv];
+        const S e = (lane == l) ? S(1) : ((lane > l && lane < m) ? vc : S(0));
+        S w0 = S(0), w1 = S(0);
+#pragma unroll
+        for (int r = 0; r < MR; r += 2) {
+            w0 += lane_read(e, r) * x[r];
+            w1 += lane_read(e, r + 1) * x[r + 1];
+        }
+        const S w = (w0 + w1) * T[(l & (NB - 1)) + (size_t)l * NB];
+#pragma unroll
+        for (int r = 0; r < MR; ++r) x[r] -= w * lane_read(e, r);
+    }
+}
+
+template <typename S, int MR, bool TRSM>
+__global__ __launch_bounds__(WG) void k_larfb_small(const LarfbTask<S>* __restrict__ tasks, int n) {
+    const int lane = threadIdx.x & 63, ti = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (ti >= n) return;
+    const LarfbTask<S> t = tasks[ti];
+    if (!TRSM) {
+        // C (m x n) <- Q^T C: lane j owns column j
+        const int j = lane, jc = min(j, t.n - 1);
+        S* __restrict__ Cj = t.C + (size_t)jc * t.ldc;
+        S x[MR];
+#pragma unroll
+        for (int r = 0; r < MR; ++r) x[r] = Cj[min(r, t.m - 1)];
+#pragma unroll
+        for (int r = 0; r < MR; ++r) x[r] = (r < t.m) ? x[r] : S(0);
+        lane_apply_qt<S, MR>(x, t.m, t.k, t.V, t.ldv, t.T);
+        if (j < t.n) {
+#pragma unroll
+            for (int r = 0; r < MR; ++r)
+                if (r < t.m) Cj[r] = x[r];
+        }
+    } else {
+        // C (m x k2) <- C R^{-1}: lane i owns row i, forward substitution along the row
+        const int i = lane, ic = min(i, t.m - 1);
+        S x[MR];
+#pragma unroll
+        for (int c = 0; c < MR; ++c) x[c] = t.C[ic + (size_t)min(c, t.k2 - 1) * t.ldc];
+        // column c of R: lane l holds R(l, c), broadcast by readlane
+        const int ll = min(lane, t.k2 - 1);
+        S rn = t.R[ll];
+#pragma unroll
+        for (int c = 0; c < MR; ++c) {
+            if (c < t.k2) {
+                const S rc = rn;
+                if (c + 1 < t.k2) rn = t.R[ll + (size_t)(c + 1) * t.ldr];
+                S s = x[c];
+#pragma unroll
+                for (int l = 0; l < c; ++l) s -= x[l] * lane_read(rc, l);
+                x[c] = s / lane_read(rc, c);
+            }
+        }
+        if (i < t.m) {
+#pragma unroll
+            for (int c = 0; c < MR; ++c)
+                if (c < t.k2) t.C[i + (size_t)c * t.ldc] = x[c];
+        }
+    }
+}
+
+template <typename S, int MR>
+__global__ __launch_bounds__(WG) void k_spars_small(const SparsApplyTask<S>* __restrict__ tasks, int n) {
+    const int lane = threadIdx.x & 63, ti = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (ti >= n) return;
+    const SparsApplyTask<S> t = tasks[ti];
+    if (t.ident) {
+        if (lane < t.rc)
+            for (int i = 0; i < t.rc; ++i) t.dst[i + (size_t)lane * t.ldd] = (i == lane) ? S(1) : S(0);
+        return;
+    }
+    if (t.pad == 1) {
+        // A (mc x kn) <- A Q_n: lane i owns row i; reflector l acts on columns l..kn-1
+        const int i = lane, ic = min(i, t.mc - 1);
+        S x[MR];
+#pragma unroll
+        for (int c = 0; c < MR; ++c) x[c] = t.A[ic + (size_t)min(c, t.kn - 1) * t.lda];
+#pragma unroll
+        for (int c = 0; c < MR; ++c) x[c] = (c < t.kn) ? x[c] : S(0);
+        lane_apply_qt<S, MR>(x, t.kn, t.rn, t.Vn, t.ldvn, t.Tn);
+        if (i < t.mc) {
+#pragma unroll
+            for (int c = 0; c < MR; ++c)
+                if (c < t.kn) t.A[i + (size_t)c * t.lda] = x[c];
+        }
+        return;
+    }
+    // A[0:kc, j] <- Q_c^T A[0:kc, j] for the rn columns, then dst(0:rc, j) = A(0:rc, j)
+    const int j = lane, jc = min(j, t.rn - 1);
+    S* __restrict__ Aj = t.A + (size_t)jc * t.lda;
+    S x[MR];
+#pragma unroll
+    for (int r = 0; r < MR; ++r) x[r] = Aj[min(r, t.kc - 1)];
+#pragma unroll
+    for (int r = 0; r < MR; ++r) x[r] = (r < t.kc) ? x[r] : S(0);
+    if (t.Vc) lane_apply_qt<S, MR>(x, t.kc, t.rc, t.Vc, t.ldvc, t.Tc);
+    if (j < t.rn) {
+        if (t.Vc) {
+#pragma unroll
+            for (int r = 0; r < MR; ++r)
+                if (r < t.kc) Aj[r] = x[r];
+        }
+#pragma unroll
+        for (int r = 0; r < MR; ++r)
+            if (r < t.rc) t.dst[r + (size_t)j * t.ldd] = x[r];
+    }
+}
+
 template <typename S>
 __global__ void k_fill(S* p, size_t n, S v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1714,6 +1841,28 @@ void launch_spars_apply(const SparsApplyTask<S>* d, int n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_spars_apply<S>, dim3(n), dim3(WG), 0, s, d);
 }
 template <typename S>
+void launch_larfb_small(const LarfbTask<S>* d, int n, int mr, bool trsm, hipStream_t s) {
+    if (n <= 0) return;
+    const dim3 g((n + 3) / 4);
+    if (trsm) {
+        if (mr <= 16) hipLaunchKernelGGL((k_larfb_small<S, 16, true>), g, dim3(WG), 0, s, d, n);
+        else if (mr <= 32) hipLaunchKernelGGL((k_larfb_small<S, 32, true>), g, dim3(WG), 0, s, d, n);
+        else hipLaunchKernelGGL((k_larfb_small<S, 64, true>), g, dim3(WG), 0, s, d, n);
+    } else {
+        if (mr <= 16) hipLaunchKernelGGL((k_larfb_small<S, 16, false>), g, dim3(WG), 0, s, d, n);
+        else if (mr <= 32) hipLaunchKernelGGL((k_larfb_small<S, 32, false>), g, dim3(WG), 0, s, d, n);
+        else hipLaunchKernelGGL((k_larfb_small<S, 64, false>), g, dim3(WG), 0, s, d, n);
+    }
+}
+template <typename S>
+void launch_spars_small(const SparsApplyTask<S>* d, int n, int mr, hipStream_t s) {
+    if (n <= 0) return;
+    const dim3 g((n + 3) / 4);
+    if (mr <= 16) hipLaunchKernelGGL((k_spars_small<S, 16>), g, dim3(WG), 0, s, d, n);
+    else if (mr <= 32) hipLaunchKernelGGL((k_spars_small<S, 32>), g, dim3(WG), 0, s, d, n);
+    else hipLaunchKernelGGL((k_spars_small<S, 64>), g, dim3(WG), 0, s, d, n);
+}
+template <typename S>
 void launch_fill(S* p, size_t n, S v, hipStream_t s) {
     if (n == 0) return;
     size_t blocks = (n + WG - 1) / WG;
@@ -1792,6 +1941,8 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
     template void launch_copy<S>(const CopyTask<S>*, int, hipStream_t);                              \
     template void launch_geqp3<S>(const Geqp3Task<S>*, int, double, double, hipStream_t);                    \
     template void launch_spars_apply<S>(const SparsApplyTask<S>*, int, hipStream_t);                 \
+    template void launch_larfb_small<S>(const LarfbTask<S>*, int, int, bool, hipStream_t);            \
+    template void launch_spars_small<S>(const SparsApplyTask<S>*, int, int, hipStream_t);             \
     template void launch_fill<S>(S*, size_t, S, hipStream_t);                                        \
     template void launch_vec_q<S>(const VecQTask<S>*, int, hipStream_t);                             \
     template void launch_vec_chain<S>(const VecChainTask<S>*, const VecChainLink<S>*, int, hipStream_t); \

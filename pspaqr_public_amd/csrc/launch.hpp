@@ -20,6 +20,11 @@ template <typename S> void launch_tpmqrt(const TpmqrtTask<S>* d, const TpmqrtLin
 template <typename S> void launch_copy(const CopyTask<S>* d, int n, hipStream_t s);
 template <typename S> void launch_geqp3(const Geqp3Task<S>* d, int n, double tol, double tie_rel, hipStream_t s);
 template <typename S> void launch_spars_apply(const SparsApplyTask<S>* d, int n, hipStream_t s);
+// Blocks of at most 64 x 64 (mr = register length class 16 / 32 / 64 >= the reflector
+// dimension): one wave per task.  Larfb tasks are Q^T C column chunks (trsm = false) or
+// C R^{-1} row chunks (trsm = true).
+template <typename S> void launch_larfb_small(const LarfbTask<S>* d, int n, int mr, bool trsm, hipStream_t s);
+template <typename S> void launch_spars_small(const SparsApplyTask<S>* d, int n, int mr, hipStream_t s);
 template <typename S> void launch_fill(S* p, size_t n, S v, hipStream_t s);
 template <typename S> void launch_vec_q(const VecQTask<S>* d, int n, hipStream_t s);
 template <typename S> void launch_vec_chain(const VecChainTask<S>* d, const VecChainLink<S>* l, int n, hipStream_t s);

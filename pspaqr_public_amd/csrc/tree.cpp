@@ -695,6 +695,60 @@ int Tree::factorize_t() {
     auto E = [&](int id) -> Edge& { return ed[id]; };
     const double SZ = (double)sizeof(S);
     auto Ap = [&](int id) -> S* { return (S*)ed[id].A; };
+    // Blocks of at most 64 x 64 go to the one-wave kernels (SPAQR_SMALL_TASKS=0 keeps every
+    // task on the workgroup kernels); the rest stay on the workgroup kernels.
+    static const bool small_tasks = getenv("SPAQR_SMALL_TASKS") == nullptr || atoi(getenv("SPAQR_SMALL_TASKS")) != 0;
+    struct LarfbBatch {
+        LarfbTask<S>* d[4] = {nullptr, nullptr, nullptr, nullptr};  // buckets 16 / 32 / 64, workgroup
+        int n[4] = {0, 0, 0, 0};
+        bool trsm = false;
+        int launches() const { return (n[0] > 0) + (n[1] > 0) + (n[2] > 0) + (n[3] > 0); }
+    };
+    auto prep_larfb = [&](const std::vector<LarfbTask<S>>& v, bool trsm) {
+        LarfbBatch b;
+        b.trsm = trsm;
+        std::vector<LarfbTask<S>> sm[3], big;
+        if (small_tasks)
+            split_small(v, [&](const LarfbTask<S>& t) {
+                if (trsm) return (t.m <= 64 && t.k2 <= 64) ? t.k2 : -1;
+                return (t.m <= 64 && t.n <= 64) ? t.m : -1;
+            }, sm, big);
+        else
+            big = v;
+        for (int q = 0; q < 3; ++q) { b.d[q] = upload_vec(sm[q]); b.n[q] = (int)sm[q].size(); }
+        b.d[3] = upload_vec(big);
+        b.n[3] = (int)big.size();
+        return b;
+    };
+    auto run_larfb = [&](const LarfbBatch& b) {
+        for (int q = 0; q < 3; ++q) launch_larfb_small<S>(b.d[q], b.n[q], 16 << q, b.trsm, stream_);
+        launch_larfb<S>(b.d[3], b.n[3], stream_);
+    };
+    struct SparsBatch {
+        SparsApplyTask<S>* d[4] = {nullptr, nullptr, nullptr, nullptr};
+        int n[4] = {0, 0, 0, 0};
+        int launches() const { return (n[0] > 0) + (n[1] > 0) + (n[2] > 0) + (n[3] > 0); }
+    };
+    auto prep_spars = [&](const std::vector<SparsApplyTask<S>>& v) {
+        SparsBatch b;
+        std::vector<SparsApplyTask<S>> sm[3], big;
+        if (small_tasks)
+            split_small(v, [&](const SparsApplyTask<S>& t) {
+                if (t.ident) return t.rc <= 64 ? 16 : -1;
+                if (t.pad == 1) return (t.mc <= 64 && t.kn <= 64) ? t.kn : -1;
+                return (t.rn <= 64 && t.kc <= 64) ? t.kc : -1;
+            }, sm, big);
+        else
+            big = v;
+        for (int q = 0; q < 3; ++q) { b.d[q] = upload_vec(sm[q]); b.n[q] = (int)sm[q].size(); }
+        b.d[3] = upload_vec(big);
+        b.n[3] = (int)big.size();
+        return b;
+    };
+    auto run_spars = [&](const SparsBatch& b) {
+        for (int q = 0; q < 3; ++q) launch_spars_small<S>(b.d[q], b.n[q], 16 << q, stream_);
+        launch_spars_apply<S>(b.d[3], b.n[3], stream_);
+    };
 
     for (int L = 0; L < nlevels; ++L) {
         auto ht = hclock();
@@ -883,8 +937,7 @@ int Tree::factorize_t() {
             std::vector<LarfbTask<S>> lfb_c, lfb_t, sapp_c, sapp_t;
             split_larfb(lfb, lfb_c, lfb_t);
             split_larfb(sapp, sapp_c, sapp_t);
-            LarfbTask<S>* d_lfb = upload_vec(lfb_c);
-            LarfbTask<S>* d_lfb_t = upload_vec(lfb_t);
+            const LarfbBatch lfb_b = prep_larfb(lfb_c, false), lfb_tb = prep_larfb(lfb_t, true);
             TpqrtTask<S>* d_tpq = upload_vec(tpq_small);
             TpqrtTask<S>* d_tpqw = upload_vec(tpq_wave);
             TpqrtLink<S>* d_tpql = upload_vec(tpql);
@@ -893,8 +946,7 @@ int Tree::factorize_t() {
             for (int q = 0; q < ncolors; ++q) d_tpm[q] = upload_vec(tpm[q]);
             CopyTask<S>* d_scopy = upload_vec(scopy);
             GeqrtTask<S>* d_sgeq = upload_vec(sgeq_small);
-            LarfbTask<S>* d_sapp = upload_vec(sapp_c);
-            LarfbTask<S>* d_sapp_t = upload_vec(sapp_t);
+            const LarfbBatch sapp_b = prep_larfb(sapp_c, false), sapp_tb = prep_larfb(sapp_t, true);
             CopyTask<S>* d_gat = upload_vec(gat);
             long ntpm = 0;
             for (int q = 0; q < ncolors; ++q) ntpm += (long)tpm[q].size();
@@ -907,9 +959,9 @@ int Tree::factorize_t() {
                 if (profiling) prof[P_GEQRT].launches += nl;
             }
             if (!lfb.empty())
-                timed(P_LARFB, 1 + !lfb_t.empty(), (long)lfb.size(), [&] {
-                    launch_larfb<S>(d_lfb, (int)lfb_c.size(), stream_);
-                    launch_larfb<S>(d_lfb_t, (int)lfb_t.size(), stream_);
+                timed(P_LARFB, lfb_b.launches() + lfb_tb.launches(), (long)lfb.size(), [&] {
+                    run_larfb(lfb_b);
+                    run_larfb(lfb_tb);
                 });
             if (!tpq.empty()) {
                 int nl = 0;
@@ -934,9 +986,9 @@ int Tree::factorize_t() {
                 if (profiling) prof[P_SGEQRT].launches += nl;
             }
             if (!sapp.empty())
-                timed(P_SAPPLY, 1 + !sapp_t.empty(), (long)sapp.size(), [&] {
-                    launch_larfb<S>(d_sapp, (int)sapp_c.size(), stream_);
-                    launch_larfb<S>(d_sapp_t, (int)sapp_t.size(), stream_);
+                timed(P_SAPPLY, sapp_b.launches() + sapp_tb.launches(), (long)sapp.size(), [&] {
+                    run_larfb(sapp_b);
+                    run_larfb(sapp_tb);
                 });
             if (!gat.empty()) timed(P_GATHER, 1, (long)gat.size(), [&] { launch_copy<S>(d_gat, (int)gat.size(), stream_); });
             if (!gq.empty()) {
@@ -1020,9 +1072,8 @@ int Tree::factorize_t() {
                     }
                 }
             }
-            SparsApplyTask<S>* d_spa1 = upload_vec(spa1);
-            if (!spa1.empty())
-                timed(P_SPARS, 1, 0, [&] { launch_spars_apply<S>(d_spa1, (int)spa1.size(), stream_); });
+            const SparsBatch sb1 = prep_spars(spa1);
+            if (!spa1.empty()) timed(P_SPARS, sb1.launches(), 0, [&] { run_spars(sb1); });
         }
         const int Lp = L + 1;
         current_bottom = Lp;
@@ -1143,10 +1194,9 @@ int Tree::factorize_t() {
                 spa2.push_back(u);
             }
         }
-        SparsApplyTask<S>* d_spa2 = upload_vec(spa2);
+        const SparsBatch sb2 = prep_spars(spa2);
         hacc(9, ht);
-        if (!spa.empty())
-            timed(P_SPARS, 1, (long)spa.size(), [&] { launch_spars_apply<S>(d_spa2, (int)spa2.size(), stream_); });
+        if (!spa.empty()) timed(P_SPARS, sb2.launches(), (long)spa.size(), [&] { run_spars(sb2); });
         HIPCHK(hipGetLastError());
         hacc(4, ht);
     }

@@ -55,6 +55,18 @@ void split_larfb(const std::vector<LarfbTask<S>>& in, std::vector<LarfbTask<S>>&
     }
 }
 
+// Splits off the tasks whose block fits one wave: dim(t) is the block's register length
+// (<= 64) or -1 to keep the task on the workgroup kernels; small tasks are bucketed by the
+// register length classes 16 / 32 / 64 of the one-wave kernels.
+template <typename T, typename D>
+void split_small(const std::vector<T>& in, D dim, std::vector<T> (&small)[3], std::vector<T>& big) {
+    for (const auto& t : in) {
+        const int d = dim(t);
+        if (d < 0) big.push_back(t);
+        else small[d <= 16 ? 0 : (d <= 32 ? 1 : 2)].push_back(t);
+    }
+}
+
 template <class F>
 void Tree::timed(int cat, long launches, long tasks, F&& f) {
     if (!profiling) {
