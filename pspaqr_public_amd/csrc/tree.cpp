@@ -213,7 +213,16 @@ void Tree::PinnedArena::release() {
 void* Tree::upload(const void* h, size_t bytes) {
     void* d = desc_arena.alloc(bytes);
     char* p = staging.alloc(bytes);
-    memcpy(p, h, bytes);
+    constexpr size_t kPiece = (size_t)1 << 20;
+    if (bytes >= 8 * kPiece) {  // large descriptor lists: copy 1 MiB pieces on several threads
+        const int pieces = (int)((bytes + kPiece - 1) / kPiece);
+        parallel_ranges(pieces, 1, [&](int b, int e, int) {
+            const size_t lo = (size_t)b * kPiece, hi = std::min(bytes, (size_t)e * kPiece);
+            memcpy(p + lo, (const char*)h + lo, hi - lo);
+        });
+    } else {
+        memcpy(p, h, bytes);
+    }
     HIPCHK(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, stream_));
     return d;
 }
@@ -1091,31 +1100,41 @@ int Tree::factorize_t() {
         }
         const size_t ne_merge = ed.size();
         {
-            // de-duplicate through stamp arrays (stamp = the parent being merged), then sort
-            // only the distinct entries
-            std::vector<int> nbrs, d2, seen_n(cl.size(), -1), seen_d(cl.size(), -1);
-            for (int p : bottoms[Lp]) {
-                nbrs.clear();
-                d2.clear();
-                for (int d : cl[p].dist2)
-                    if (seen_d[d] != p) { seen_d[d] = p; d2.push_back(d); }
-                for (int ch : cl[p].children) {
-                    for (int e : cl[ch].out) {
-                        const int n2 = ed[e].n2;
-                        if (cl[n2].eliminated) continue;
-                        const int q = cl[n2].parent;
-                        if (seen_n[q] != p) { seen_n[q] = p; nbrs.push_back(q); }
+            // Per parent: the merged neighbour and distance-2 lists (reads of the children only,
+            // so the parents run in parallel; duplicates dropped through per-thread stamp arrays,
+            // stamp = the parent), then the new edges in parent order.
+            const auto& bl = bottoms[Lp];
+            const int np = (int)bl.size();
+            std::vector<std::vector<int>> pd2(np), pnb(np);
+            parallel_ranges(np, 64, [&](int b, int e, int) {
+                std::vector<int> seen_n(cl.size(), -1), seen_d(cl.size(), -1);
+                for (int i = b; i < e; ++i) {
+                    const int p = bl[i];
+                    std::vector<int>& d2 = pd2[i];
+                    std::vector<int>& nbrs = pnb[i];
+                    for (int d : cl[p].dist2)
+                        if (seen_d[d] != p) { seen_d[d] = p; d2.push_back(d); }
+                    for (int ch : cl[p].children) {
+                        for (int e2 : cl[ch].out) {
+                            const int n2 = ed[e2].n2;
+                            if (cl[n2].eliminated) continue;
+                            const int q = cl[n2].parent;
+                            if (seen_n[q] != p) { seen_n[q] = p; nbrs.push_back(q); }
+                        }
+                        for (int d : cl[ch].dist2) {
+                            if (cl[d].lvl() < Lp) continue;
+                            const int q = cl[d].parent;
+                            if (seen_d[q] != p) { seen_d[q] = p; d2.push_back(q); }
+                        }
                     }
-                    for (int d : cl[ch].dist2) {
-                        if (cl[d].lvl() < Lp) continue;
-                        const int q = cl[d].parent;
-                        if (seen_d[q] != p) { seen_d[q] = p; d2.push_back(q); }
-                    }
+                    std::sort(d2.begin(), d2.end());
+                    std::sort(nbrs.begin(), nbrs.end());
                 }
-                std::sort(d2.begin(), d2.end());
-                cl[p].dist2 = d2;
-                std::sort(nbrs.begin(), nbrs.end());
-                for (int nn : nbrs) new_edge(p, nn);
+            });
+            for (int i = 0; i < np; ++i) {
+                const int p = bl[i];
+                cl[p].dist2 = std::move(pd2[i]);
+                for (int nn : pnb[i]) new_edge(p, nn);
                 sort_out(cl[p]);
             }
         }
@@ -1131,72 +1150,84 @@ int Tree::factorize_t() {
             }
         }
         hacc(7, ht);
-        std::vector<SparsApplyTask<S>> spa;
-        std::vector<int> slot(cl.size(), -1);  // parent's out edge by target, for the parent being filled
-        int slot_owner = -1;
-        for (int so : itf) {
-            const Cluster& SO = cl[so];
-            const int pso = SO.parent;
-            if (pso != slot_owner) {
-                if (slot_owner >= 0)
-                    for (int e : cl[slot_owner].out) slot[ed[e].n2] = -1;
-                for (int e : cl[pso].out) slot[ed[e].n2] = e;
-                slot_owner = pso;
-            }
-            for (int e : SO.out) {
-                const Edge& eo = ed[e];
-                const int no = eo.n2;
-                if (cl[no].eliminated) continue;
-                const Cluster& NO = cl[no];
-                int f = slot[NO.parent];
-                if (f < 0) throw std::runtime_error("merge: parent edge missing");
-                S* dst = Ap(f) + NO.rpos + (size_t)SO.cpos * ed[f].rows;
-                SparsApplyTask<S> t{};
-                t.A = Ap(e);
-                t.dst = dst;
-                t.ldd = ed[f].rows;
-                t.mc = eo.rows;
-                t.nc = eo.cols;
-                t.lda = eo.rows;
-                t.kn = eo.cols;
-                t.kc = eo.rows;
-                t.rn = SO.cols;
-                t.rc = NO.rows;
-                if (no == so) {
-                    t.ident = SO.sparsified ? 1 : 0;
-                } else {
+        // Sparsify-apply tasks of every interface block, built in parallel over the interfaces
+        // (per-thread lists, concatenated in interface order).  The left transform Q_c^T A and
+        // the scatter act on each column: column chunks, so that large blocks spread over many
+        // workgroups (the right transform was queued before the merge).
+        struct SparsPart {
+            std::vector<SparsApplyTask<S>> tasks;
+            long blocks = 0;
+            double flops = 0.0, pflops = 0.0, pbytes = 0.0;
+        };
+        std::vector<SparsPart> parts(host_threads());
+        const int nparts = parallel_ranges((int)itf.size(), 256, [&](int b, int e, int slot) {
+            SparsPart& P = parts[slot];
+            for (int ii = b; ii < e; ++ii) {
+                const int so = itf[ii];
+                const Cluster& SO = cl[so];
+                const std::vector<int>& pout = cl[SO.parent].out;  // sorted by target
+                for (int e2 : SO.out) {
+                    const Edge& eo = ed[e2];
+                    const int no = eo.n2;
+                    if (cl[no].eliminated) continue;
+                    const Cluster& NO = cl[no];
+                    // the parent's out edge to NO's parent
+                    auto it = std::lower_bound(pout.begin(), pout.end(), NO.parent,
+                                               [&](int x, int target) { return ed[x].n2 < target; });
+                    if (it == pout.end() || ed[*it].n2 != NO.parent) throw std::runtime_error("merge: parent edge missing");
+                    const int f = *it;
+                    SparsApplyTask<S> t{};
+                    t.A = Ap(e2);
+                    t.dst = Ap(f) + NO.rpos + (size_t)SO.cpos * ed[f].rows;
+                    t.ldd = ed[f].rows;
+                    t.mc = eo.rows;
+                    t.nc = eo.cols;
+                    t.lda = eo.rows;
+                    t.kn = eo.cols;
+                    t.kc = eo.rows;
+                    t.rn = SO.cols;
+                    t.rc = NO.rows;
+                    ++P.blocks;
+                    if (no == so) {
+                        t.ident = SO.sparsified ? 1 : 0;
+                        P.tasks.push_back(t);
+                        continue;
+                    }
                     if (SO.sparsified) { t.Vn = (const S*)SO.Qsp; t.Tn = (const S*)SO.Tsp; t.ldvn = eo.cols; }
                     if (NO.sparsified) { t.Vc = (const S*)NO.Qsp; t.Tc = (const S*)NO.Tsp; t.ldvc = eo.rows; }
-                    if (SO.sparsified) stats.back().flops += 4.0 * eo.rows * eo.cols * SO.cols;
-                    if (NO.sparsified) stats.back().flops += 4.0 * eo.rows * SO.cols * NO.cols;
-                    prof_add(P_SPARS, (SO.sparsified ? 4.0 * eo.rows * eo.cols * SO.cols : 0.0) +
-                                          (NO.sparsified ? 4.0 * eo.rows * SO.cols * NO.cols : 0.0),
-                             SZ * (2.0 * eo.rows * eo.cols + (double)NO.rows * SO.cols));
+                    const double fl = (SO.sparsified ? 4.0 * eo.rows * eo.cols * SO.cols : 0.0) +
+                                      (NO.sparsified ? 4.0 * eo.rows * SO.cols * NO.cols : 0.0);
+                    P.flops += fl;
+                    P.pflops += fl;
+                    P.pbytes += SZ * (2.0 * eo.rows * eo.cols + (double)NO.rows * SO.cols);
+                    for (int c0 = 0; c0 < t.rn; c0 += kSparsCols) {
+                        SparsApplyTask<S> u = t;
+                        u.A = t.A + (size_t)c0 * t.lda;
+                        u.dst = t.dst + (size_t)c0 * t.ldd;
+                        u.rn = std::min(kSparsCols, t.rn - c0);
+                        u.pad = 2;
+                        P.tasks.push_back(u);
+                    }
                 }
-                spa.push_back(t);
             }
+        });
+        long nspa = 0;
+        std::vector<SparsApplyTask<S>> spa2;
+        {
+            size_t tot = 0;
+            for (int q = 0; q < nparts; ++q) tot += parts[q].tasks.size();
+            spa2.reserve(tot);
+        }
+        for (int q = 0; q < nparts; ++q) {
+            nspa += parts[q].blocks;
+            stats.back().flops += parts[q].flops;
+            prof_add(P_SPARS, parts[q].pflops, parts[q].pbytes);
+            spa2.insert(spa2.end(), parts[q].tasks.begin(), parts[q].tasks.end());
         }
         hacc(8, ht);
-        // The left transform Q_c^T A and the scatter act on each column: column chunks, so that
-        // large blocks spread over many workgroups (the right transform was queued before the merge).
-        std::vector<SparsApplyTask<S>> spa2;
-        for (const auto& t : spa) {
-            if (t.ident) {
-                spa2.push_back(t);
-                continue;
-            }
-            for (int c0 = 0; c0 < t.rn; c0 += kSparsCols) {
-                SparsApplyTask<S> u = t;
-                u.A = t.A + (size_t)c0 * t.lda;
-                u.dst = t.dst + (size_t)c0 * t.ldd;
-                u.rn = std::min(kSparsCols, t.rn - c0);
-                u.pad = 2;
-                spa2.push_back(u);
-            }
-        }
         const SparsBatch sb2 = prep_spars(spa2);
         hacc(9, ht);
-        if (!spa.empty()) timed(P_SPARS, sb2.launches(), (long)spa.size(), [&] { run_spars(sb2); });
+        if (nspa) timed(P_SPARS, sb2.launches(), nspa, [&] { run_spars(sb2); });
         HIPCHK(hipGetLastError());
         hacc(4, ht);
     }

@@ -3,6 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <exception>
+#include <thread>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -53,6 +56,47 @@ void split_larfb(const std::vector<LarfbTask<S>>& in, std::vector<LarfbTask<S>>&
                 trsm_out.push_back(u);
             }
     }
+}
+
+// Host threads for the planner loops: OMP_NUM_THREADS when set, else the hardware count,
+// at most 16.
+inline int host_threads() {
+    static const int n = [] {
+        const char* e = getenv("OMP_NUM_THREADS");
+        int v = e ? atoi(e) : 0;
+        if (v <= 0) v = (int)std::thread::hardware_concurrency();
+        return std::max(1, std::min(v, 16));
+    }();
+    return n;
+}
+
+// Runs f(begin, end, slot) over [0, n) cut into contiguous ranges (at least min_per items
+// each) on up to host_threads() threads.  Slot t gets the t-th range, so per-slot results
+// concatenated in slot order reproduce the sequential order.  Returns the number of slots;
+// an exception in any range is rethrown after all have finished.
+template <class F>
+int parallel_ranges(int n, int min_per, F&& f) {
+    const int T = std::min(host_threads(), std::max(1, n / std::max(1, min_per)));
+    if (T <= 1) {
+        f(0, n, 0);
+        return 1;
+    }
+    std::vector<std::exception_ptr> err(T);
+    auto run = [&](int t) {
+        try {
+            f((int)((long long)n * t / T), (int)((long long)n * (t + 1) / T), t);
+        } catch (...) {
+            err[t] = std::current_exception();
+        }
+    };
+    std::vector<std::thread> th;
+    th.reserve(T - 1);
+    for (int t = 1; t < T; ++t) th.emplace_back(run, t);
+    run(0);
+    for (auto& x : th) x.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+    return T;
 }
 
 // Splits off the tasks whose block fits one wave: dim(t) is the block's register length
