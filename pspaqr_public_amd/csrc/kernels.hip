@@ -909,6 +909,57 @@ __global__ __launch_bounds__(64) void k_tpqrt_wave(const TpqrtTask<S>* __restric
         if (c < k) t.R[j + (size_t)c * t.ldr] = sm.R[rpos(j) + c - j];
 }
 
+// Progress counters between the workgroups of one cooperative chain.  The producer makes its
+// stores visible (agent-scope release by every thread, then the counter store); a consumer
+// spins on the counter (bounded: past the limit it raises *err and goes on, so the kernel
+// always ends) and every thread then acquires before reading the producer's data.
+constexpr long kChainSpinLimit = 1L << 24;
+
+__device__ __forceinline__ void chain_wait(const int* c, int target, int* err) {
+    if (threadIdx.x == 0) {
+        long spins = 0;
+        while (__hip_atomic_load(c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++spins > kChainSpinLimit) { atomicOr(err, 1); break; }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+__device__ __forceinline__ void chain_signal(int* c, int v) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(c, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// tpqrt chain with one workgroup per 32-column panel.  For each link, workgroup p applies the
+// block reflectors of panels 0 .. p-1 of that link to its own columns (each after the panel's
+// owner has published it), factors its own panel [R_pp; B(:, panel p)] and publishes it.
+// Links pipeline across the panels: workgroup 0 starts link l + 1 while later panels still
+// work on link l.  Same reflectors, R and T as the one-workgroup kernel.
+template <typename S>
+__global__ __launch_bounds__(WG) void k_tpqrt_coop(const TpqrtCoopTask<S>* __restrict__ tasks,
+                                                   const int* __restrict__ blk2task,
+                                                   const TpqrtLink<S>* __restrict__ links, int* err) {
+    __shared__ ApplySmem<S> asm_;
+    __shared__ PanelSmem<S> psm;
+    const TpqrtCoopTask<S> t = tasks[blk2task[blockIdx.x]];
+    const int p = blockIdx.x - t.b0, p0 = p * NB, kb = min(NB, t.k - p0);
+    for (int li = 0; li < t.nlinks; ++li) {
+        const TpqrtLink<S> L = links[t.link0 + li];
+        S* Bp = L.B + (size_t)p0 * L.ldb;
+        for (int q = 0; q < p; ++q) {
+            chain_wait(t.cnt + q, li + 1, err);
+            const int q0 = q * NB;
+            wg_apply_lt<S, true>(L.rows, NB, L.B + (size_t)q0 * L.ldb, L.ldb, L.T + (size_t)q0 * NB, NB,
+                                 t.R + q0 + (size_t)p0 * t.ldr, t.ldr, Bp, L.ldb, kb, asm_);
+        }
+        wg_panel_qr<S, true>(L.rows, kb, t.R + p0 + (size_t)p0 * t.ldr, t.ldr, Bp, L.ldb, L.T + (size_t)p0 * NB, NB, psm);
+        chain_signal(t.cnt + p, li + 1);
+    }
+}
+
 template <typename S>
 __global__ __launch_bounds__(WG) void k_tpmqrt(const TpmqrtTask<S>* __restrict__ tasks,
                                                const TpmqrtLink<S>* __restrict__ links) {
@@ -1821,6 +1872,19 @@ void launch_tpqrt(const TpqrtTask<S>* d, const TpqrtLink<S>* l, int n, hipStream
     if (n > 0) hipLaunchKernelGGL(k_tpqrt<S>, dim3(n), dim3(WG), 0, s, d, l);
 }
 template <typename S>
+void launch_tpqrt_coop(const TpqrtCoopTask<S>* d, const int* blk2task, int nblocks, const TpqrtLink<S>* l, int* err,
+                       hipStream_t s) {
+    if (nblocks > 0) hipLaunchKernelGGL(k_tpqrt_coop<S>, dim3(nblocks), dim3(WG), 0, s, d, blk2task, l, err);
+}
+template <typename S>
+int tpqrt_coop_max_blocks() {
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tpqrt_coop<S>, WG, 0) != hipSuccess) return 0;
+    return per_cu * cus;
+}
+template <typename S>
 void launch_tpqrt_wave(const TpqrtTask<S>* d, const TpqrtLink<S>* l, int n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_tpqrt_wave<S>, dim3(n), dim3(64), 0, s, d, l);
 }
@@ -1937,6 +2001,8 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
     template void launch_panel_apply<S>(const PanelApplyTask<S>*, int, bool, hipStream_t);           \
     template void launch_tpqrt<S>(const TpqrtTask<S>*, const TpqrtLink<S>*, int, hipStream_t);       \
     template void launch_tpqrt_wave<S>(const TpqrtTask<S>*, const TpqrtLink<S>*, int, hipStream_t);       \
+    template void launch_tpqrt_coop<S>(const TpqrtCoopTask<S>*, const int*, int, const TpqrtLink<S>*, int*, hipStream_t); \
+    template int tpqrt_coop_max_blocks<S>();                                                          \
     template void launch_tpmqrt<S>(const TpmqrtTask<S>*, const TpmqrtLink<S>*, int, hipStream_t);    \
     template void launch_copy<S>(const CopyTask<S>*, int, hipStream_t);                              \
     template void launch_geqp3<S>(const Geqp3Task<S>*, int, double, double, hipStream_t);                    \

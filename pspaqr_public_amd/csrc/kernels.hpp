@@ -28,6 +28,11 @@ template <typename S> struct LarfbTask {
 template <typename S> struct TpqrtLink { S* B; S* T; int rows, ldb; };
 template <typename S> struct TpqrtTask { S* R; int k, ldr, link0, nlinks; };
 
+// Cooperative tpqrt chain: workgroups b0 .. b0 + ceil(k/32) - 1 of the launch, workgroup p
+// owning column panel p of R and of every link's B; cnt points at the chain's ceil(k/32)
+// zero-initialised progress counters (links finished per panel).
+template <typename S> struct TpqrtCoopTask { S* R; int* cnt; int k, ldr, link0, nlinks, b0, pad; };
+
 // tpmqrt chain of one (interior c, column block m, column tile starting at col0):
 // [Atop (k x ncols); B_j(:, col0:col0+ncols)] <- Q_j^T [...] for each link j in order
 // (Atop already points at the tile).

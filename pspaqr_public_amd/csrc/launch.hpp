@@ -16,6 +16,12 @@ template <typename S> void launch_panel_apply(const PanelApplyTask<S>* d, int n,
 template <typename S> void launch_tpqrt(const TpqrtTask<S>* d, const TpqrtLink<S>* l, int n, hipStream_t s);
 // Chains with k <= 64 and links of at most 64 rows: one 64-thread workgroup per chain.
 template <typename S> void launch_tpqrt_wave(const TpqrtTask<S>* d, const TpqrtLink<S>* l, int n, hipStream_t s);
+// Cooperative chains (one workgroup per 32-column panel; blk2task maps each of the nblocks
+// workgroups to its chain).  A launch must not exceed tpqrt_coop_max_blocks() workgroups.
+template <typename S>
+void launch_tpqrt_coop(const TpqrtCoopTask<S>* d, const int* blk2task, int nblocks, const TpqrtLink<S>* l, int* err,
+                       hipStream_t s);
+template <typename S> int tpqrt_coop_max_blocks();
 template <typename S> void launch_tpmqrt(const TpmqrtTask<S>* d, const TpmqrtLink<S>* l, int n, hipStream_t s);
 template <typename S> void launch_copy(const CopyTask<S>* d, int n, hipStream_t s);
 template <typename S> void launch_geqp3(const Geqp3Task<S>* d, int n, double tol, double tie_rel, hipStream_t s);

@@ -687,6 +687,8 @@ int Tree::factorize_t() {
         HIPCHK(hipMemcpyAsync(base, d_blocks0, h_blocks0.size() * sizeof(S), hipMemcpyDeviceToDevice, stream_));
         for (size_t i = 0; i < ed.size(); ++i) ed[i].A = base + h_block_off[i];
     }
+    d_chain_err = (int*)arena.alloc(sizeof(int));
+    launch_fill<int>(d_chain_err, 1, 0, stream_);
     // host-phase timers (SPAQR_DEBUG_HOST=1 prints them per factorization)
     static const bool dbg_host = getenv("SPAQR_DEBUG_HOST") != nullptr;
     double hms[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -818,8 +820,10 @@ int Tree::factorize_t() {
         }
         // colour layers for the shared-block updates: interiors that share an out-neighbour get
         // different colours, and the colours run in order (the update order of the level)
+        hacc(1, ht);
         int ncolors = colour_interiors(ints);
         st.colors = ncolors;
+        hacc(10, ht);
         std::vector<std::vector<TpmqrtTask<S>>> tpm(ncolors);
         std::vector<TpmqrtLink<S>> tpml;
         const int TILE = 64;
@@ -848,6 +852,7 @@ int Tree::factorize_t() {
                                             link0, nl, c0});
             }
         }
+        hacc(11, ht);
         // ---- scaling descriptors (src/ptree.cpp:269-286, 1621-1633)
         std::vector<CopyTask<S>> scopy;
         std::vector<GeqrtTask<S>> sgeq;
@@ -932,7 +937,8 @@ int Tree::factorize_t() {
             std::vector<GeqrtTask<S>> geq_small, geq_big, sgeq_small, sgeq_big;
             for (const auto& t : geq) (std::min(t.m, t.n) >= kBigFront ? geq_big : geq_small).push_back(t);
             for (const auto& t : sgeq) (std::min(t.m, t.n) >= kBigFront ? sgeq_big : sgeq_small).push_back(t);
-            std::vector<TpqrtTask<S>> tpq_wave, tpq_small, tpq_big;
+            std::vector<TpqrtTask<S>> tpq_wave, tpq_small, tpq_big, tpq_coop;
+            static const bool tpqrt_coop_on = getenv("SPAQR_TPQRT_COOP") == nullptr || atoi(getenv("SPAQR_TPQRT_COOP")) != 0;
             // small chains (k <= 64, links of <= 64 rows) go one per wave; the rest stay on the
             // one-workgroup kernel: with the register-resident panel QR the host-driven panel
             // loop (tpqrt_big) costs more in launch gaps than it gains
@@ -940,7 +946,9 @@ int Tree::factorize_t() {
             for (const auto& t : tpq) {
                 bool small = tpqrt_waves && t.k <= 64;
                 for (int j = 0; j < t.nlinks && small; ++j) small = tpql[t.link0 + j].rows <= 64;
-                (small ? tpq_wave : tpq_small).push_back(t);
+                if (small) tpq_wave.push_back(t);
+                else if (tpqrt_coop_on && t.k > NB && t.nlinks > 1) tpq_coop.push_back(t);  // several panels to pipeline
+                else tpq_small.push_back(t);
             }
             GeqrtTask<S>* d_geq = upload_vec(geq_small);
             std::vector<LarfbTask<S>> lfb_c, lfb_t, sapp_c, sapp_t;
@@ -977,7 +985,8 @@ int Tree::factorize_t() {
                 timed(P_TPQRT, 0, (long)tpq.size(), [&] {
                     launch_tpqrt_wave<S>(d_tpqw, d_tpql, (int)tpq_wave.size(), stream_);
                     launch_tpqrt<S>(d_tpq, d_tpql, (int)tpq_small.size(), stream_);
-                    nl = !tpq_wave.empty() + !tpq_small.empty() + tpqrt_big<S>(tpq_big, tpql);
+                    nl = !tpq_wave.empty() + !tpq_small.empty() + tpqrt_big<S>(tpq_big, tpql) +
+                         tpqrt_coop<S>(tpq_coop, d_tpql);
                 });
                 if (profiling) prof[P_TPQRT].launches += nl;
             }
@@ -1232,7 +1241,12 @@ int Tree::factorize_t() {
         hacc(4, ht);
     }
     auto ht2 = hclock();
-    sync_stream();
+    {
+        int chain_err = 0;
+        HIPCHK(hipMemcpyAsync(&chain_err, d_chain_err, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        sync_stream();
+        if (chain_err) throw std::runtime_error("cooperative tpqrt: a wait exceeded its spin limit");
+    }
     hacc(5, ht2);
     factorized = true;
     int rc = build_solve_schedule<S>();
@@ -1240,9 +1254,9 @@ int Tree::factorize_t() {
     hacc(6, ht2);
     if (dbg_host)
         fprintf(stderr,
-                "[host] fillin %.1f  descr %.1f  launch+ranks %.1f  merge %.1f (+blocks %.1f)  spars build %.1f split+upload "
-                "%.1f launch %.1f  tail-sync %.1f  solve-sched %.1f ms\n",
-                hms[0], hms[1], hms[2], hms[3], hms[7], hms[8], hms[9], hms[4], hms[5], hms[6]);
+                "[host] fillin %.1f  descr %.1f (colour %.1f tpmqrt %.1f)  launch+ranks %.1f  merge %.1f (+blocks %.1f)  spars "
+                "build %.1f split+upload %.1f launch %.1f  tail-sync %.1f  solve-sched %.1f ms\n",
+                hms[0], hms[1], hms[10], hms[11], hms[2], hms[3], hms[7], hms[8], hms[9], hms[4], hms[5], hms[6]);
     t_factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
 }
@@ -1327,6 +1341,39 @@ int Tree::tpqrt_big(const std::vector<TpqrtTask<S>>& big, const std::vector<Tpqr
         launch_panel_qr<S>(dq + qoff[st], qoff[st + 1] - qoff[st], true, stream_);
         launch_panel_apply<S>(da + aoff[st], aoff[st + 1] - aoff[st], true, stream_);
         launches += 1 + (aoff[st + 1] > aoff[st]);
+    }
+    return launches;
+}
+
+// Cooperative tpqrt chains, packed into launches of at most the co-resident number of
+// workgroups (a workgroup waits only on lower-numbered workgroups of its own chain).
+template <typename S>
+int Tree::tpqrt_coop(const std::vector<TpqrtTask<S>>& chains, const TpqrtLink<S>* d_links) {
+    if (chains.empty()) return 0;
+    static const int maxb = tpqrt_coop_max_blocks<S>();
+    if (maxb <= 0) throw std::runtime_error("cooperative tpqrt: no occupancy");
+    int panels = 0;
+    for (const auto& t : chains) panels += (t.k + NB - 1) / NB;
+    int* cnt = (int*)arena.alloc(sizeof(int) * panels);
+    launch_fill<int>(cnt, (size_t)panels, 0, stream_);
+    int launches = 0, coff = 0;
+    for (size_t i = 0; i < chains.size();) {
+        std::vector<TpqrtCoopTask<S>> tk;
+        std::vector<int> b2t;
+        int nb = 0;
+        while (i < chains.size()) {
+            const TpqrtTask<S>& c = chains[i];
+            const int P = (c.k + NB - 1) / NB;
+            if (P > maxb) throw std::runtime_error("cooperative tpqrt: chain wider than the co-resident limit");
+            if (nb + P > maxb) break;
+            tk.push_back({c.R, cnt + coff, c.k, c.ldr, c.link0, c.nlinks, nb, 0});
+            for (int q = 0; q < P; ++q) b2t.push_back((int)tk.size() - 1);
+            nb += P;
+            coff += P;
+            ++i;
+        }
+        launch_tpqrt_coop<S>(upload_vec(tk), upload_vec(b2t), nb, d_links, d_chain_err, stream_);
+        ++launches;
     }
     return launches;
 }

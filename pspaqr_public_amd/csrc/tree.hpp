@@ -199,6 +199,8 @@ private:
     template <typename S> int geqrt_big(const std::vector<GeqrtTask<S>>& big);
     template <typename S>
     int tpqrt_big(const std::vector<TpqrtTask<S>>& big, const std::vector<TpqrtLink<S>>& links);
+    template <typename S> int tpqrt_coop(const std::vector<TpqrtTask<S>>& chains, const TpqrtLink<S>* d_links);
+    int* d_chain_err = nullptr;  // raised by a cooperative chain whose wait exceeded its spin limit
 
     // solve schedule
     struct SolveSchedule;
