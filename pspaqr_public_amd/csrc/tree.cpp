@@ -133,9 +133,6 @@ HostCSC permute_rc(const HostCSC& A, const std::vector<int>& perm) {  // A(perm,
     return P;
 }
 
-template <typename S>
-double flops_geqrt(double m, double n) { return 2 * m * n * n - 2 * n * n * n / 3; }
-
 }  // namespace
 
 // ============================================================================ arena
@@ -1303,6 +1300,8 @@ int Tree::geqrt_big(const std::vector<GeqrtTask<S>>& big) {
     }
     return launches;
 }
+template int Tree::geqrt_big<float>(const std::vector<GeqrtTask<float>>&);
+template int Tree::geqrt_big<double>(const std::vector<GeqrtTask<double>>&);
 
 template <typename S>
 int Tree::tpqrt_big(const std::vector<TpqrtTask<S>>& big, const std::vector<TpqrtLink<S>>& links) {

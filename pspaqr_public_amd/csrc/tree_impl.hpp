@@ -58,6 +58,10 @@ void split_larfb(const std::vector<LarfbTask<S>>& in, std::vector<LarfbTask<S>>&
     }
 }
 
+// Flops of an m x n Householder QR.
+template <typename S>
+inline double flops_geqrt(double m, double n) { return 2 * m * n * n - 2 * n * n * n / 3; }
+
 // Host threads for the planner loops: OMP_NUM_THREADS when set, else the hardware count,
 // at most 16.
 inline int host_threads() {

@@ -24,6 +24,11 @@ constexpr int kSketchPad = 8;
 constexpr double kSketchTolFactor = 0.0;
 constexpr uint64_t kSketchSeed = 0x5eed5ca1eull;
 constexpr int kSketchRounds = 3;           // first sketch plus up to two doublings
+// Sketches at least this wide are QR'd without pivoting (blocked Householder on MFMA): any
+// prefix of a Gaussian sketch spans the dominant range, so the tail test finds a rank a few
+// columns above the pivoted one, while the pivoted QR would pay one cross-workgroup barrier
+// per column.  SPAQR_SKETCH_UNPIVOTED overrides the width.
+constexpr int kSketchUnpivoted = 96;
 }  // namespace
 
 int Tree::set_rank_model(const float* w) {
@@ -152,22 +157,33 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
                 auto* d_rg = upload_vec(regather);
                 timed(P_GATHER, 1, (long)regather.size(), [&] { launch_copy<S>(d_rg, (int)regather.size(), stream_); });
             }
+            static const int unpiv_min = getenv("SPAQR_SKETCH_UNPIVOTED") ? atoi(getenv("SPAQR_SKETCH_UNPIVOTED"))
+                                                                            : kSketchUnpivoted;
             std::vector<SketchTask<S>> sk;
             std::vector<SketchTile> tiles;
             std::vector<Geqp3Task<S>> gy;
+            std::vector<GeqrtTask<S>> uy_small, uy_big;  // unpivoted sketch QRs, in place in Vout
+            std::vector<int> ry0(nt, 0);
             for (int q : todo) {
                 const int k = gq[q].k, w = gq[q].w, s = s_of[q];
-                S* Y = (S*)arena.alloc(sizeof(S) * (size_t)k * s);
+                const bool unpiv = unpiv_min > 0 && s >= unpiv_min;
+                S* Y = unpiv ? gq[q].Vout : (S*)arena.alloc(sizeof(S) * (size_t)k * s);
                 for (int r0 = 0; r0 < k; r0 += 64)
                     for (int c0 = 0; c0 < s; c0 += 64) tiles.push_back({(int)sk.size(), r0, c0, 0});
                 sk.push_back({gq[q].C, Y, k, w, s, 0});
-                Geqp3Task<S> g = gq[q];
-                g.C = Y;
-                g.w = s;
-                gy.push_back(g);
+                if (unpiv) {
+                    (std::min(k, s) >= kBigFront ? uy_big : uy_small).push_back({Y, gq[q].T, k, s, k, 0});
+                    ry0[q] = std::min(k, s);
+                    prof_add(P_GEQP3, flops_geqrt<S>(k, s), SZ * (2.0 * k * s + NB * s));
+                } else {
+                    Geqp3Task<S> g = gq[q];
+                    g.C = Y;
+                    g.w = s;
+                    gy.push_back(g);
+                    prof_add(P_GEQP3, 0.0, SZ * 3.0 * k * s);
+                }
                 flops += 2.0 * k * w * s;
                 prof_add(P_RSKETCH, 2.0 * k * w * s, SZ * ((double)k * w + (double)k * s));
-                prof_add(P_GEQP3, 0.0, SZ * 3.0 * k * s);
                 rstats.sketch_cols += s;
                 if (round == 0) rstats.full_cols += std::min(k, w);
             }
@@ -175,9 +191,19 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
             auto* d_tl = upload_vec(tiles);
             timed(P_RSKETCH, 1, (long)todo.size(),
                   [&] { launch_sketch<S>(d_sk, d_tl, (int)tiles.size(), om, om_rows, stream_); });
+            // rank bounds of the unpivoted QRs (full width); the pivoted QRs write their own
+            if (!uy_small.empty() || !uy_big.empty())
+                HIPCHK(hipMemcpyAsync(d_ranks, upload(ry0.data(), sizeof(int) * nt), sizeof(int) * nt,
+                                      hipMemcpyDeviceToDevice, stream_));
             int nd = 0;
-            timed(P_GEQP3, 0, (long)todo.size(),
-                  [&] { geqp3_grouped<S>(gy, tol * kSketchTolFactor, kPivotTieRel, stream_, mem, d_err, &nd); });
+            timed(P_GEQP3, 0, (long)todo.size(), [&] {
+                auto* d_uy = upload_vec(uy_small);
+                launch_geqrt<S>(d_uy, (int)uy_small.size(), stream_);
+                nd = !uy_small.empty() + geqrt_big<S>(uy_big);
+                int ndp = 0;
+                if (!gy.empty()) geqp3_grouped<S>(gy, tol * kSketchTolFactor, kPivotTieRel, stream_, mem, d_err, &ndp);
+                nd += ndp;
+            });
             if (profiling) prof[P_GEQP3].launches += nd;
             HIPCHK(hipMemcpyAsync(ry.data(), d_ranks, sizeof(int) * nt, hipMemcpyDeviceToHost, stream_));
             sync_stream();
