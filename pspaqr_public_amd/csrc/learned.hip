@@ -42,6 +42,24 @@ __global__ __launch_bounds__(WG) void k_gauss(S* om, int rows, int cols, int ld,
 // ---------------------------------------------------------------------------------------
 // Column norms (16 lanes per column).
 // ---------------------------------------------------------------------------------------
+// Sum of squares of col[gl], col[gl + 16], ... (rows < k) accumulated in A, eight loads in
+// flight per lane (a plain loop waits on every load).
+template <typename A, typename S>
+__device__ __forceinline__ A col_sumsq16(const S* __restrict__ col, int k, int gl) {
+    A acc[4] = {A(0), A(0), A(0), A(0)};
+    for (int r0 = gl; r0 < k; r0 += 16 * 8) {
+        S v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = col[min(r0 + 16 * u, k - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const A x = (r0 + 16 * u < k) ? (A)v[u] : A(0);
+            acc[u & 3] += x * x;
+        }
+    }
+    return (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
 template <typename S>
 __global__ __launch_bounds__(WG) void k_colnorm(const ColNormTask<S>* __restrict__ tasks) {
     __shared__ S red[4];
@@ -50,12 +68,7 @@ __global__ __launch_bounds__(WG) void k_colnorm(const ColNormTask<S>* __restrict
     S mx = S(0);
     for (int c0 = 0; c0 < t.w; c0 += WG / 16) {
         const int c = c0 + grp;
-        S s = S(0);
-        if (c < t.w)
-            for (int r = gl; r < t.k; r += 16) {
-                const S v = t.C[r + (size_t)c * t.k];
-                s += v * v;
-            }
+        S s = col_sumsq16<S>(t.C + (size_t)min(c, t.w - 1) * t.k, t.k, gl);
         s = row16_sum(s);
         if (c < t.w) {
             if (gl == 0) t.cn[c] = s;
@@ -93,8 +106,9 @@ __global__ __launch_bounds__(WG) void k_sketch(const SketchTask<S>* __restrict__
     S acc[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) acc[j] = S(0);
-    for (int w0 = 0; w0 < t.w; w0 += 32) {
-        S cv[8], ov[8];
+    // the next 32-column step of C and Omega is loaded into registers while this one computes
+    S cv[8], ov[8];
+    auto load_step = [&](int w0) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int e = tid + q * WG, rr = e & 63, cc = e >> 6;
@@ -103,6 +117,9 @@ __global__ __launch_bounds__(WG) void k_sketch(const SketchTask<S>* __restrict__
             const int i = e & 31, j = e >> 5;
             ov[q] = om[min(w0 + i, t.w - 1) + (size_t)min(tl.c0 + j, t.s - 1) * ldo];
         }
+    };
+    load_step(0);
+    for (int w0 = 0; w0 < t.w; w0 += 32) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int e = tid + q * WG, rr = e & 63, cc = e >> 6;
@@ -111,6 +128,7 @@ __global__ __launch_bounds__(WG) void k_sketch(const SketchTask<S>* __restrict__
             Os[j][i] = (w0 + i < t.w) ? ov[q] : S(0);
         }
         __syncthreads();
+        if (w0 + 32 < t.w) load_step(w0 + 32);
         if constexpr (sizeof(S) == 4) {
             // MFMA 32x32x2: wave (wm, wn) owns the 32 x 32 block (rows 32 wm.., columns 32 wn..)
             const int li = tid & 31, lk = (tid >> 5) & 1, wm = wv & 1, wn = wv >> 1;
@@ -220,20 +238,18 @@ __global__ __launch_bounds__(WG) void k_tail_select(const TailTask<S>* __restric
     for (int c0 = 0; c0 < t.w; c0 += WG / 16) {
         const int c = c0 + grp;
         const S* col = t.B + (size_t)min(c, t.w - 1) * t.k;
-        double tot = 0.0;
-        if (c < t.w)
-            for (int r = gl; r < t.k; r += 16) {
-                const double v = (double)col[r];
-                tot += v * v;
-            }
+        double tot = col_sumsq16<double>(col, t.k, gl);
         tot = row16_sum(tot);
         int rc = 0;
         if (c < t.w && tot > thr2) {
             rc = INT_MAX;
             double run = 0.0;
+            S vnext = col[min(gl, t.k - 1)];  // the next chunk's entry is loaded one chunk ahead
             for (int b = 0; b <= rY; b += 16) {
                 const int i = b + gl;
-                const double v = (i < rY && i < t.k) ? (double)col[i] : 0.0;
+                const S vcur = vnext;
+                vnext = col[min(i + 16, t.k - 1)];
+                const double v = (i < rY && i < t.k) ? (double)vcur : 0.0;
                 const double sq = v * v;
                 const double incl = row16_scan(sq);
                 const double tail = tot - (run + incl - sq);  // rows i .. k-1
