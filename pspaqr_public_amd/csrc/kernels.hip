@@ -284,7 +284,9 @@ __device__ __forceinline__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B
 // H^T [Atop; C] (or H^T C) on MFMA, FP32: for each 128-column chunk of C (wave w owns its
 // 32-column slice), W = Atop + V^T C accumulates over 64-row tiles of V and C staged in LDS,
 // W <- T^T W, Atop -= W, then C -= V W tile by tile.  Every MFMA is a 32x32x2 step.
-template <bool PENT>
+// TRC = true applies to the transpose of the array given: X = C^T with X(r, j) at C[j + r ldc]
+// (C X-columns are C rows), which turns the right apply C <- C H into X <- H^T X.
+template <bool PENT, bool TRC = false>
 __device__ __forceinline__ void wg_apply_lt_mfma(int m, int kb, const float* __restrict__ V, int ldv,
                                                  const float* __restrict__ T, int ldt, float* Atop, int lda, float* C,
                                                  int ldc, int ncols, ApplySmem<float>& sm) {
@@ -316,12 +318,15 @@ __device__ __forceinline__ void wg_apply_lt_mfma(int m, int kb, const float* __r
             float cv[8];
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const int e = tid + (q0 + q) * WG, r = e & 63, j = e >> 6;
-                cv[q] = C[(r0 + min(r, rb - 1)) + (size_t)(c0 + min(j, nc - 1)) * ldc];
+                const int e = tid + (q0 + q) * WG;
+                const int r = TRC ? (e >> 7) : (e & 63), j = TRC ? (e & 127) : (e >> 6);  // coalesced either way
+                const int rr = r0 + min(r, rb - 1), jj = c0 + min(j, nc - 1);
+                cv[q] = TRC ? C[jj + (size_t)rr * ldc] : C[rr + (size_t)jj * ldc];
             }
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const int e = tid + (q0 + q) * WG, r = e & 63, j = e >> 6;
+                const int e = tid + (q0 + q) * WG;
+                const int r = TRC ? (e >> 7) : (e & 63), j = TRC ? (e & 127) : (e >> 6);
                 sm.Cs[j * 65 + r] = (r < rb && j < nc) ? cv[q] : 0.f;
             }
         }
@@ -396,8 +401,12 @@ __device__ __forceinline__ void wg_apply_lt_mfma(int m, int kb, const float* __r
             __syncthreads();
 #pragma unroll 4
             for (int q = 0; q < 32; ++q) {
-                const int e = tid + q * WG, r = e & 63, j = e >> 6;
-                if (r < rb && j < nc) C[(r0 + r) + (size_t)(c0 + j) * ldc] = sm.Cs[j * 65 + r];
+                const int e = tid + q * WG;
+                const int r = TRC ? (e >> 7) : (e & 63), j = TRC ? (e & 127) : (e >> 6);
+                if (r < rb && j < nc) {
+                    if (TRC) C[(c0 + j) + (size_t)(r0 + r) * ldc] = sm.Cs[j * 65 + r];
+                    else C[(r0 + r) + (size_t)(c0 + j) * ldc] = sm.Cs[j * 65 + r];
+                }
             }
         }
     }
@@ -518,6 +527,10 @@ __device__ __forceinline__ void wg_apply_rn(int m, int n, int kb, const S* __res
     // unconditional (clamped, masked), issued in groups of 16 per thread.
     const int tid = threadIdx.x;
     if (m <= 0 || n <= 0 || kb <= 0) return;
+    if constexpr (sizeof(S) == 4) {  // FP32: C H = (H^T C^T)^T on MFMA
+        wg_apply_lt_mfma<false, true>(n, kb, V, ldv, T, ldt, nullptr, 0, C, ldc, m, sm);
+        return;
+    }
     __syncthreads();
     for (int e = tid; e < 32 * 32; e += WG) {
         int i = e & 31, j = e >> 5;
