@@ -113,33 +113,43 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Sum the 32 per-lane values d[0..31] over the wave.  On return lane L holds the total of
-// column vec_col(L) (lanes L and L^1 hold the same column): a transposing butterfly, 32 shuffles.
+// Sum the 32 per-lane values d[0..31] over the wave: a transposing butterfly.  On return
+// lane L holds the total of column vec_col(L); lanes L and L ^ 32 hold the same column and
+// vec_owner(L) (L < 32) picks one of them.  The four steps inside a 16-lane row pair lanes
+// with DPP moves (row_mirror, row_half_mirror, quad_perm), which flip the step's bit plus
+// only bits that are still to be reduced, so the partners always hold the same columns; the
+// two cross-row steps (16, 32) use shuffles.  30 DPP moves and 2 shuffles in all.
 __device__ __forceinline__ int vec_col(int lane) {
-    return ((lane >> 5) & 1) * 16 + ((lane >> 4) & 1) * 8 + ((lane >> 3) & 1) * 4 + ((lane >> 2) & 1) * 2 +
-           ((lane >> 1) & 1);
+    return ((lane >> 3) & 1) * 16 + ((lane >> 2) & 1) * 8 + ((lane >> 1) & 1) * 4 + (lane & 1) * 2 +
+           ((lane >> 4) & 1);
 }
+__device__ __forceinline__ bool vec_owner(int lane) { return lane < 32; }
 
-template <int H, typename S>
-__device__ __forceinline__ void butterfly_step(S (&d)[32], int lane) {
-    const bool up = (lane & (2 * H)) != 0;
+template <int H, int BIT, int CTRL, typename S>
+__device__ __forceinline__ void butterfly_dpp(S (&d)[32], int lane) {
+    const bool up = ((lane >> BIT) & 1) != 0;
 #pragma unroll
     for (int i = 0; i < H; ++i) {
         const S send = up ? d[i] : d[i + H];
         const S keep = up ? d[i + H] : d[i];
-        d[i] = keep + __shfl_xor(send, 2 * H, 64);
+        d[i] = keep + dpp_mov<CTRL>(send);
     }
 }
 
 template <typename S>
 __device__ __forceinline__ S wave_sum32(S (&d)[32]) {
     const int lane = threadIdx.x & 63;
-    butterfly_step<16>(d, lane);
-    butterfly_step<8>(d, lane);
-    butterfly_step<4>(d, lane);
-    butterfly_step<2>(d, lane);
-    butterfly_step<1>(d, lane);
-    return d[0] + __shfl_xor(d[0], 1, 64);
+    butterfly_dpp<16, 3, 0x140>(d, lane);  // row_mirror: flips bits 0-3
+    butterfly_dpp<8, 2, 0x141>(d, lane);   // row_half_mirror: flips bits 0-2
+    butterfly_dpp<4, 1, 0x4E>(d, lane);    // quad_perm [2,3,0,1]: xor 2
+    butterfly_dpp<2, 0, 0xB1>(d, lane);    // quad_perm [1,0,3,2]: xor 1
+    {
+        const bool up = ((lane >> 4) & 1) != 0;
+        const S send = up ? d[0] : d[1];
+        const S keep = up ? d[1] : d[0];
+        d[0] = keep + __shfl_xor(send, 16, 64);
+    }
+    return d[0] + __shfl_xor(d[0], 32, 64);
 }
 
 }  // namespace spaqr

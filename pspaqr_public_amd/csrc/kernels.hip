@@ -191,7 +191,7 @@ __device__ __forceinline__ void wg_panel_qr_reg(int m, int nbp, S* Rt, int ldr, 
             for (int l = 0; l < 32; ++l) sm.rowj[par][l] = a[0][l];
         }
         const S tot = wave_sum32<S>(d);
-        if ((lane & 1) == 0) sm.red[par][wave][vec_col(lane)] = tot;
+        if (vec_owner(lane)) sm.red[par][wave][vec_col(lane)] = tot;
         __syncthreads();
         S dl[32];
 #pragma unroll
@@ -626,7 +626,41 @@ __device__ __forceinline__ void wg_trsm_ru(int m, int n, const S* __restrict__ R
     const int rr = tid & 63, cg = tid >> 6;
     for (int j0 = 0; j0 < n; j0 += 32) {
         const int jb = min(32, n - j0);
-        if (j0 > 0) {
+        if (sizeof(S) == 4 && j0 > 0) {
+            // B_j -= B_<j R_<j,j on MFMA: wave w owns rows r0 + 32 w .. of a 128-row tile; the
+            // 32 x 32 block of R is staged in LDS per 32-row step of the reduction
+            const int l = tid & 63, li = l & 31, lk = l >> 5, w = tid >> 6;
+            for (int r0 = 0; r0 < m; r0 += 128) {
+                const int row = r0 + 32 * w + li, rowc = min(row, m - 1);
+                floatx16 acc;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+                for (int l0 = 0; l0 < j0; l0 += 32) {
+                    __syncthreads();
+                    for (int e = tid; e < 32 * 32; e += WG) {
+                        const int ll = e & 31, q = e >> 5;
+                        const S rv = R[min(l0 + ll, j0 - 1) + (size_t)(j0 + min(q, jb - 1)) * ldr];
+                        sm.W[q * 33 + ll] = (l0 + ll < j0 && q < jb) ? rv : S(0);
+                    }
+                    __syncthreads();
+                    float av[16];
+#pragma unroll
+                    for (int kk = 0; kk < 32; kk += 2) {
+                        const int c = l0 + kk + lk;
+                        const float v = (float)B[rowc + (size_t)min(c, j0 - 1) * ldb];
+                        av[kk / 2] = (c < j0 && row < m) ? v : 0.f;
+                    }
+#pragma unroll
+                    for (int kk = 0; kk < 32; kk += 2)
+                        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk / 2], (float)sm.W[li * 33 + kk + lk], acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int i = r0 + 32 * w + mfma_row(l, r);
+                    if (i < m && li < jb) B[i + (size_t)(j0 + li) * ldb] -= (S)acc[r];
+                }
+            }
+        } else if (j0 > 0) {
             for (int r0 = 0; r0 < m; r0 += 64) {
                 const int Rw = r0 + rr;
                 S acc[8];
@@ -816,6 +850,105 @@ struct TpqrtWaveSmem {
     S Ts[32 * 33]; // T(q, j) of the current panel at j * 33 + q
     S tau[32];
 };
+
+// geqrt of a small block (m, n <= 64) on one 64-thread workgroup, column-major in the lanes:
+// lane c holds column c (MR >= m rows) in registers.  Step j: lane j forms the reflector
+// from its own column (alpha = row j, the rows below it), and writes it to LDS as a full
+// MR-vector with zeros above row j and 1 on it, so that every lane's dot product and update
+// run over fixed register indices; the same dot product gives the panel's Gram column
+// v_p^T v_j for the columns already reflected, from which T is built at each panel's end.
+// Same reflectors (sign rule included), R, V and blocked T as the workgroup kernel.
+template <typename S>
+struct GeqrtWaveSmem {
+    S v[64];       // reflector of the current step (zero above its row, one on it)
+    S G[32 * 33];  // G(p, j) = v_p^T v_j of the current panel at j * 33 + p
+    S Ts[32 * 33]; // T(q, j) of the current panel at j * 33 + q
+    S tau[32];
+};
+
+template <typename S, int MR>
+__global__ __launch_bounds__(64) void k_geqrt_wave(const GeqrtTask<S>* __restrict__ tasks) {
+    __shared__ GeqrtWaveSmem<S> sm;
+    const GeqrtTask<S> t = tasks[blockIdx.x];
+    const int c = threadIdx.x, m = t.m, n = t.n, kmin = min(m, n), cc = min(c, n - 1);
+    S* __restrict__ Ac = t.A + (size_t)cc * t.lda;
+    S a[MR];
+#pragma unroll
+    for (int r = 0; r < MR; ++r) a[r] = Ac[min(r, m - 1)];
+#pragma unroll
+    for (int r = 0; r < MR; ++r) a[r] = (r < m && c < n) ? a[r] : S(0);
+    for (int p0 = 0; p0 < kmin; p0 += NB) {
+        const int kb = min(NB, kmin - p0);
+        for (int jj = 0; jj < kb; ++jj) {
+            const int j = p0 + jj;
+            wave_sync();  // the previous step's readers of sm.v are done
+            if (c == j) {
+                S alpha = S(0), s0 = S(0), s1 = S(0);
+#pragma unroll
+                for (int r = 0; r < MR; ++r) {
+                    alpha = (r == j) ? a[r] : alpha;
+                    const S x = (r > j) ? a[r] : S(0);
+                    if (r & 1) s1 += x * x; else s0 += x * x;
+                }
+                const S ss = s0 + s1;
+                S tau, scal, beta;
+                if (ss == S(0)) {
+                    tau = S(0); scal = S(1); beta = alpha;
+                } else {
+                    beta = -householder_norm_signed(alpha, ss);
+                    tau = (beta - alpha) / beta;
+                    scal = S(1) / (alpha - beta);
+                }
+#pragma unroll
+                for (int r = 0; r < MR; ++r) {
+                    const S v = (r > j) ? a[r] * scal : (r == j ? S(1) : S(0));
+                    sm.v[r] = v;
+                    a[r] = (r > j) ? v : (r == j ? beta : a[r]);
+                }
+                sm.tau[jj] = tau;
+            }
+            wave_sync();
+            const S tau = sm.tau[jj];
+            S d0 = S(0), d1 = S(0);
+#pragma unroll
+            for (int r = 0; r < MR; r += 2) {
+                d0 += a[r] * sm.v[r];
+                d1 += a[r + 1] * sm.v[r + 1];
+            }
+            const S d = d0 + d1;
+            if (c > j && c < n) {
+                const S w = tau * d;
+#pragma unroll
+                for (int r = 0; r < MR; ++r) a[r] -= w * sm.v[r];
+            } else if (c >= p0 && c < j) {
+                sm.G[jj * 33 + (c - p0)] = d;
+            }
+        }
+        wave_sync();
+        // the panel's T: lane q builds row q, T(q, j) = -tau_j sum_{p=q}^{j-1} T(q, p) G(p, j)
+        const int q = c;
+        if (q < kb) {
+            S* Tq = t.T + (size_t)p0 * NB + q;
+            for (int jj = 0; jj < kb; ++jj) {
+                S v = S(0);
+                if (q == jj) {
+                    v = sm.tau[jj];
+                } else if (q < jj) {
+                    S s = S(0);
+                    for (int p = q; p < jj; ++p) s += sm.Ts[p * 33 + q] * sm.G[jj * 33 + p];
+                    v = -sm.tau[jj] * s;
+                }
+                sm.Ts[jj * 33 + q] = v;
+                Tq[(size_t)jj * NB] = v;
+            }
+        }
+    }
+    if (c < n) {
+#pragma unroll
+        for (int r = 0; r < MR; ++r)
+            if (r < m) Ac[r] = a[r];
+    }
+}
 
 template <typename S>
 __global__ __launch_bounds__(64) void k_tpqrt_wave(const TpqrtTask<S>* __restrict__ tasks,
@@ -1387,7 +1520,7 @@ __device__ __forceinline__ void wg_vec_panel(int m, int p0, int kb, const S* __r
         }
     }
     const S tot = wave_sum32<S>(d);
-    if ((lane & 1) == 0) sm.red[wave][vec_col(lane)] = tot;
+    if (vec_owner(lane)) sm.red[wave][vec_col(lane)] = tot;
     __syncthreads();
     // phase 2 (wave 0): t = T^T d or T d; columns >= kb of d are ignored (T is zero there)
     if (wave == 0) {
@@ -1527,7 +1660,7 @@ __global__ __launch_bounds__(WG) void k_vec_chain_wave(const VecChainTask<S>* __
 #pragma unroll
             for (int l = 0; l < NB; ++l) d[l] = cur.v[p * NB + l] * xr;
             const S tot = wave_sum32<S>(d);
-            if ((lane & 1) == 0) sd[wave][vec_col(lane)] = tot + sxc[wave][p * NB + vec_col(lane)];
+            if (vec_owner(lane)) sd[wave][vec_col(lane)] = tot + sxc[wave][p * NB + vec_col(lane)];
             wave_sync();
             if ((lane >> 5) == p) {
                 S tj = S(0);
@@ -1625,7 +1758,7 @@ __device__ __forceinline__ void chain_item_apply(const VecChainLink<S>* __restri
         d[l] = a;
     }
     const S tot = wave_sum32<S>(d);
-    if ((lane & 1) == 0) sm.red[slot][wave][vec_col(lane)] = tot;
+    if (vec_owner(lane)) sm.red[slot][wave][vec_col(lane)] = tot;
     __syncthreads();
     const int j = lane & (NB - 1);
     const S dj = ((sm.red[slot][0][j] + sm.red[slot][1][j]) + (sm.red[slot][2][j] + sm.red[slot][3][j])) +
@@ -1885,6 +2018,13 @@ void launch_tpqrt(const TpqrtTask<S>* d, const TpqrtLink<S>* l, int n, hipStream
     if (n > 0) hipLaunchKernelGGL(k_tpqrt<S>, dim3(n), dim3(WG), 0, s, d, l);
 }
 template <typename S>
+void launch_geqrt_wave(const GeqrtTask<S>* d, int n, int mr, hipStream_t s) {
+    if (n <= 0) return;
+    if (mr <= 16) hipLaunchKernelGGL((k_geqrt_wave<S, 16>), dim3(n), dim3(64), 0, s, d);
+    else if (mr <= 32) hipLaunchKernelGGL((k_geqrt_wave<S, 32>), dim3(n), dim3(64), 0, s, d);
+    else hipLaunchKernelGGL((k_geqrt_wave<S, 64>), dim3(n), dim3(64), 0, s, d);
+}
+template <typename S>
 void launch_tpqrt_coop(const TpqrtCoopTask<S>* d, const int* blk2task, int nblocks, const TpqrtLink<S>* l, int* err,
                        hipStream_t s) {
     if (nblocks > 0) hipLaunchKernelGGL(k_tpqrt_coop<S>, dim3(nblocks), dim3(WG), 0, s, d, blk2task, l, err);
@@ -2015,6 +2155,7 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
     template void launch_tpqrt<S>(const TpqrtTask<S>*, const TpqrtLink<S>*, int, hipStream_t);       \
     template void launch_tpqrt_wave<S>(const TpqrtTask<S>*, const TpqrtLink<S>*, int, hipStream_t);       \
     template void launch_tpqrt_coop<S>(const TpqrtCoopTask<S>*, const int*, int, const TpqrtLink<S>*, int*, hipStream_t); \
+    template void launch_geqrt_wave<S>(const GeqrtTask<S>*, int, int, hipStream_t);                    \
     template int tpqrt_coop_max_blocks<S>();                                                          \
     template void launch_tpmqrt<S>(const TpmqrtTask<S>*, const TpmqrtLink<S>*, int, hipStream_t);    \
     template void launch_copy<S>(const CopyTask<S>*, int, hipStream_t);                              \

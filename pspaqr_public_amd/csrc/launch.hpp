@@ -10,6 +10,9 @@
 namespace spaqr {
 
 template <typename S> void launch_geqrt(const GeqrtTask<S>* d, int n, hipStream_t s);
+// Blocks of at most 64 x 64: one 64-thread workgroup each; mr = register length class
+// 16 / 32 / 64 >= the block's rows.
+template <typename S> void launch_geqrt_wave(const GeqrtTask<S>* d, int n, int mr, hipStream_t s);
 template <typename S> void launch_larfb(const LarfbTask<S>* d, int n, hipStream_t s);
 template <typename S> void launch_panel_qr(const PanelTask<S>* d, int n, bool pent, hipStream_t s);
 template <typename S> void launch_panel_apply(const PanelApplyTask<S>* d, int n, bool pent, hipStream_t s);

@@ -728,6 +728,27 @@ int Tree::factorize_t() {
         b.n[3] = (int)big.size();
         return b;
     };
+    struct GeqrtBatch {
+        GeqrtTask<S>* d[4] = {nullptr, nullptr, nullptr, nullptr};  // buckets 16 / 32 / 64, workgroup
+        int n[4] = {0, 0, 0, 0};
+        int launches() const { return (n[0] > 0) + (n[1] > 0) + (n[2] > 0) + (n[3] > 0); }
+    };
+    auto prep_geqrt = [&](const std::vector<GeqrtTask<S>>& v) {
+        GeqrtBatch b;
+        std::vector<GeqrtTask<S>> sm[3], big;
+        if (small_tasks)
+            split_small(v, [&](const GeqrtTask<S>& t) { return (t.m <= 64 && t.n <= 64) ? t.m : -1; }, sm, big);
+        else
+            big = v;
+        for (int q = 0; q < 3; ++q) { b.d[q] = upload_vec(sm[q]); b.n[q] = (int)sm[q].size(); }
+        b.d[3] = upload_vec(big);
+        b.n[3] = (int)big.size();
+        return b;
+    };
+    auto run_geqrt = [&](const GeqrtBatch& b) {
+        for (int q = 0; q < 3; ++q) launch_geqrt_wave<S>(b.d[q], b.n[q], 16 << q, stream_);
+        launch_geqrt<S>(b.d[3], b.n[3], stream_);
+    };
     auto run_larfb = [&](const LarfbBatch& b) {
         for (int q = 0; q < 3; ++q) launch_larfb_small<S>(b.d[q], b.n[q], 16 << q, b.trsm, stream_);
         launch_larfb<S>(b.d[3], b.n[3], stream_);
@@ -947,7 +968,7 @@ int Tree::factorize_t() {
                 else if (tpqrt_coop_on && t.k > NB && t.nlinks > 1) tpq_coop.push_back(t);  // several panels to pipeline
                 else tpq_small.push_back(t);
             }
-            GeqrtTask<S>* d_geq = upload_vec(geq_small);
+            const GeqrtBatch geq_b = prep_geqrt(geq_small), sgeq_b = prep_geqrt(sgeq_small);
             std::vector<LarfbTask<S>> lfb_c, lfb_t, sapp_c, sapp_t;
             split_larfb(lfb, lfb_c, lfb_t);
             split_larfb(sapp, sapp_c, sapp_t);
@@ -959,7 +980,7 @@ int Tree::factorize_t() {
             std::vector<TpmqrtTask<S>*> d_tpm(ncolors);
             for (int q = 0; q < ncolors; ++q) d_tpm[q] = upload_vec(tpm[q]);
             CopyTask<S>* d_scopy = upload_vec(scopy);
-            GeqrtTask<S>* d_sgeq = upload_vec(sgeq_small);
+
             const LarfbBatch sapp_b = prep_larfb(sapp_c, false), sapp_tb = prep_larfb(sapp_t, true);
             CopyTask<S>* d_gat = upload_vec(gat);
             long ntpm = 0;
@@ -967,8 +988,8 @@ int Tree::factorize_t() {
             if (!geq.empty()) {
                 int nl = 0;
                 timed(P_GEQRT, 0, (long)geq.size(), [&] {
-                    launch_geqrt<S>(d_geq, (int)geq_small.size(), stream_);
-                    nl = !geq_small.empty() + geqrt_big<S>(geq_big);
+                    run_geqrt(geq_b);
+                    nl = geq_b.launches() + geqrt_big<S>(geq_big);
                 });
                 if (profiling) prof[P_GEQRT].launches += nl;
             }
@@ -995,8 +1016,8 @@ int Tree::factorize_t() {
                 int nl = 0;
                 timed(P_SGEQRT, 0, (long)sgeq.size(), [&] {
                     launch_copy<S>(d_scopy, (int)scopy.size(), stream_);
-                    launch_geqrt<S>(d_sgeq, (int)sgeq_small.size(), stream_);
-                    nl = 1 + !sgeq_small.empty() + geqrt_big<S>(sgeq_big);
+                    run_geqrt(sgeq_b);
+                    nl = 1 + sgeq_b.launches() + geqrt_big<S>(sgeq_big);
                 });
                 if (profiling) prof[P_SGEQRT].launches += nl;
             }
