@@ -1106,6 +1106,71 @@ __global__ __launch_bounds__(WG) void k_tpqrt_coop(const TpqrtCoopTask<S>* __res
     }
 }
 
+// tpmqrt chain of a small task (k <= 64, links of at most MR <= 64 rows, tile of <= 64
+// columns) on one 64-thread workgroup: lane c owns column c of the tile -- its column of
+// B_n in registers, its column of Atop in LDS -- and the k reflectors of each link are
+// applied one at a time (tau_j = T(j mod 32, j)), the reflector broadcast through LDS with
+// the next one's load already in flight.  Every step is lane-local; the same transformation
+// as the blocked workgroup kernel.
+template <typename S, int MR>
+__global__ __launch_bounds__(64) void k_tpmqrt_wave(const TpmqrtTask<S>* __restrict__ tasks,
+                                                    const TpmqrtLink<S>* __restrict__ links) {
+    __shared__ S At[64 * 65];  // Atop(j, c) at j * 65 + c
+    __shared__ S vb[MR];       // reflector j (rows of B_n)
+    const TpmqrtTask<S> t = tasks[blockIdx.x];
+    const int c = threadIdx.x, k = t.k, nc = t.ncols, cc = min(c, nc - 1);
+    {
+        const S* __restrict__ Ac = t.Atop + (size_t)cc * t.lda;
+        for (int j0 = 0; j0 < k; j0 += 8) {
+            S x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = Ac[min(j0 + u, k - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (j0 + u < k) At[(j0 + u) * 65 + c] = x[u];
+        }
+    }
+    for (int li = 0; li < t.nlinks; ++li) {
+        const TpmqrtLink<S> L = links[t.link0 + li];
+        const int rows = L.rows, rl = min(c, rows - 1);
+        S* __restrict__ Bc = L.B + (size_t)(t.col0 + cc) * L.ldb;
+        S b[MR];
+#pragma unroll
+        for (int r = 0; r < MR; ++r) b[r] = Bc[min(r, rows - 1)];
+#pragma unroll
+        for (int r = 0; r < MR; ++r) b[r] = (r < rows) ? b[r] : S(0);
+        const S* __restrict__ V = L.V;
+        S vn = V[rl];
+        for (int j = 0; j < k; ++j) {
+            const S vc = vn;
+            if (j + 1 < k) vn = V[rl + (size_t)(j + 1) * L.ldv];
+            wave_sync();  // the previous step's readers of vb are done
+            if (c < MR) vb[c] = (c < rows) ? vc : S(0);
+            wave_sync();
+            const S tau = L.T[(j & (NB - 1)) + (size_t)j * NB];
+            S d0 = At[j * 65 + c], d1 = S(0);
+#pragma unroll
+            for (int r = 0; r < MR; r += 2) {
+                d0 += vb[r] * b[r];
+                d1 += vb[r + 1] * b[r + 1];
+            }
+            const S w = tau * (d0 + d1);
+            At[j * 65 + c] -= w;
+#pragma unroll
+            for (int r = 0; r < MR; ++r) b[r] -= w * vb[r];
+        }
+        if (c < nc) {
+#pragma unroll
+            for (int r = 0; r < MR; ++r)
+                if (r < rows) Bc[r] = b[r];
+        }
+    }
+    if (c < nc) {
+        S* __restrict__ Ac = t.Atop + (size_t)c * t.lda;
+        for (int j = 0; j < k; ++j) Ac[j] = At[j * 65 + c];
+    }
+}
+
 template <typename S>
 __global__ __launch_bounds__(WG) void k_tpmqrt(const TpmqrtTask<S>* __restrict__ tasks,
                                                const TpmqrtLink<S>* __restrict__ links) {
@@ -2018,6 +2083,13 @@ void launch_tpqrt(const TpqrtTask<S>* d, const TpqrtLink<S>* l, int n, hipStream
     if (n > 0) hipLaunchKernelGGL(k_tpqrt<S>, dim3(n), dim3(WG), 0, s, d, l);
 }
 template <typename S>
+void launch_tpmqrt_wave(const TpmqrtTask<S>* d, const TpmqrtLink<S>* l, int n, int mr, hipStream_t s) {
+    if (n <= 0) return;
+    if (mr <= 16) hipLaunchKernelGGL((k_tpmqrt_wave<S, 16>), dim3(n), dim3(64), 0, s, d, l);
+    else if (mr <= 32) hipLaunchKernelGGL((k_tpmqrt_wave<S, 32>), dim3(n), dim3(64), 0, s, d, l);
+    else hipLaunchKernelGGL((k_tpmqrt_wave<S, 64>), dim3(n), dim3(64), 0, s, d, l);
+}
+template <typename S>
 void launch_geqrt_wave(const GeqrtTask<S>* d, int n, int mr, hipStream_t s) {
     if (n <= 0) return;
     if (mr <= 16) hipLaunchKernelGGL((k_geqrt_wave<S, 16>), dim3(n), dim3(64), 0, s, d);
@@ -2156,6 +2228,7 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
     template void launch_tpqrt_wave<S>(const TpqrtTask<S>*, const TpqrtLink<S>*, int, hipStream_t);       \
     template void launch_tpqrt_coop<S>(const TpqrtCoopTask<S>*, const int*, int, const TpqrtLink<S>*, int*, hipStream_t); \
     template void launch_geqrt_wave<S>(const GeqrtTask<S>*, int, int, hipStream_t);                    \
+    template void launch_tpmqrt_wave<S>(const TpmqrtTask<S>*, const TpmqrtLink<S>*, int, int, hipStream_t); \
     template int tpqrt_coop_max_blocks<S>();                                                          \
     template void launch_tpmqrt<S>(const TpmqrtTask<S>*, const TpmqrtLink<S>*, int, hipStream_t);    \
     template void launch_copy<S>(const CopyTask<S>*, int, hipStream_t);                              \

@@ -26,6 +26,10 @@ void launch_tpqrt_coop(const TpqrtCoopTask<S>* d, const int* blk2task, int nbloc
                        hipStream_t s);
 template <typename S> int tpqrt_coop_max_blocks();
 template <typename S> void launch_tpmqrt(const TpmqrtTask<S>* d, const TpmqrtLink<S>* l, int n, hipStream_t s);
+// Tasks with k <= 64, tiles of <= 64 columns and links of 1..64 rows (mr = register length
+// class 16 / 32 / 64 >= the longest link): one 64-thread workgroup each.
+template <typename S>
+void launch_tpmqrt_wave(const TpmqrtTask<S>* d, const TpmqrtLink<S>* l, int n, int mr, hipStream_t s);
 template <typename S> void launch_copy(const CopyTask<S>* d, int n, hipStream_t s);
 template <typename S> void launch_geqp3(const Geqp3Task<S>* d, int n, double tol, double tie_rel, hipStream_t s);
 template <typename S> void launch_spars_apply(const SparsApplyTask<S>* d, int n, hipStream_t s);

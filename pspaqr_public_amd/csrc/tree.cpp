@@ -977,8 +977,27 @@ int Tree::factorize_t() {
             TpqrtTask<S>* d_tpqw = upload_vec(tpq_wave);
             TpqrtLink<S>* d_tpql = upload_vec(tpql);
             TpmqrtLink<S>* d_tpml = upload_vec(tpml);
-            std::vector<TpmqrtTask<S>*> d_tpm(ncolors);
-            for (int q = 0; q < ncolors; ++q) d_tpm[q] = upload_vec(tpm[q]);
+            // per colour: one-wave buckets (k <= 64, links of 1..64 rows) and the workgroup rest
+            std::vector<std::array<TpmqrtTask<S>*, 4>> d_tpm(ncolors);
+            std::vector<std::array<int, 4>> n_tpm(ncolors);
+            for (int q = 0; q < ncolors; ++q) {
+                std::vector<TpmqrtTask<S>> sm[3], big;
+                if (small_tasks)
+                    split_small(tpm[q], [&](const TpmqrtTask<S>& t) {
+                        if (t.k > 64 || t.ncols > 64) return -1;
+                        int rmax = 0, rmin = INT_MAX;
+                        for (int j = 0; j < t.nlinks; ++j) {
+                            rmax = std::max(rmax, tpml[t.link0 + j].rows);
+                            rmin = std::min(rmin, tpml[t.link0 + j].rows);
+                        }
+                        return (rmin >= 1 && rmax <= 64) ? rmax : -1;
+                    }, sm, big);
+                else
+                    big = tpm[q];
+                for (int b = 0; b < 3; ++b) { d_tpm[q][b] = upload_vec(sm[b]); n_tpm[q][b] = (int)sm[b].size(); }
+                d_tpm[q][3] = upload_vec(big);
+                n_tpm[q][3] = (int)big.size();
+            }
             CopyTask<S>* d_scopy = upload_vec(scopy);
 
             const LarfbBatch sapp_b = prep_larfb(sapp_c, false), sapp_tb = prep_larfb(sapp_t, true);
@@ -1010,7 +1029,10 @@ int Tree::factorize_t() {
             }
             if (ntpm)
                 timed(P_TPMQRT, ncolors, ntpm, [&] {
-                    for (int q = 0; q < ncolors; ++q) launch_tpmqrt<S>(d_tpm[q], d_tpml, (int)tpm[q].size(), stream_);
+                    for (int q = 0; q < ncolors; ++q) {
+                        for (int b = 0; b < 3; ++b) launch_tpmqrt_wave<S>(d_tpm[q][b], d_tpml, n_tpm[q][b], 16 << b, stream_);
+                        launch_tpmqrt<S>(d_tpm[q][3], d_tpml, n_tpm[q][3], stream_);
+                    }
                 });
             if (!sgeq.empty()) {
                 int nl = 0;
