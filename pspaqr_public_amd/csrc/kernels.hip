@@ -15,6 +15,7 @@
 
 #include <climits>
 #include <cstdint>
+#include <type_traits>
 
 #include "dpp.hpp"
 #include "kernels.hpp"
@@ -43,15 +44,53 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 // entry r of lane l is D(mfma_row(l, r), l % 32).
 __device__ __forceinline__ int mfma_row(int l, int r) { return (r >> 2) * 8 + (l >> 5) * 4 + (r & 3); }
 
+// Shared memory of the workgroup panel QR.  `mem`: wg_panel_qr_mem, wg_larft_blocked and the
+// row-layout register panel (FP64); `col`: the column-layout register panel (FP32).
 template <typename S>
-struct PanelSmem {
+struct MemPanelSmem {
     S Ts[32 * 33];
     S d[32];
     S t[32];
     S tau, scal;
-    S red[2][4][32];  // per-step column sums of the register-resident panel QR (double-buffered)
+    S red[2][4][32];  // per-step column sums of the row-layout register panel (double-buffered)
     S rowj[2][32];
 };
+template <typename S>
+struct alignas(16) ColPanelSmem {
+    static constexpr int kSegMax = 128;  // register rows per thread, at most
+    S top[32 * 33];         // the top nbp x nbp block, row-major: top[r * 33 + c]
+    S G[32 * 32];           // G(p, j) = v_p^T v_j at j * 32 + p
+    S red[2][8][32];        // per-step partial dot products [parity][segment][column]
+    S rowj[2][32];          // row j of the top block at step j
+    S tau[32], scl[32];     // tau_j and the scale 1 / (alpha_j - beta_j) of reflector j
+    S T[32 * 32];           // the panel's T, T(q, j) at j * 32 + q
+    S xb[8][kSegMax];       // column j's register rows, per segment
+};
+template <typename S>
+union PanelSmemCol {
+    MemPanelSmem<S> mem;
+    ColPanelSmem<S> col;
+};
+template <typename S>
+struct PanelSmemRow {
+    MemPanelSmem<S> mem;
+};
+template <typename S>
+using PanelSmem = typename std::conditional<sizeof(S) == 4, PanelSmemCol<S>, PanelSmemRow<S>>::type;
+// Kernels that alternate panels and applies.  FP32: one union (every panel and apply starts
+// and ends with a barrier, so the phases never overlap); FP64: side by side.
+template <typename S>
+union ApplyPanelSmemU {
+    ApplySmem<S> ap;
+    PanelSmem<S> pn;
+};
+template <typename S>
+struct ApplyPanelSmemS {
+    ApplySmem<S> ap;
+    PanelSmem<S> pn;
+};
+template <typename S>
+using ApplyPanelSmem = typename std::conditional<sizeof(S) == 4, ApplyPanelSmemU<S>, ApplyPanelSmemS<S>>::type;
 
 // sign(alpha) * ||(alpha, x)|| for a Householder reflector, with an alpha at rounding-noise
 // level (|alpha| <= band * ||x||) treated as non-negative: its sign carries no information and
@@ -71,7 +110,7 @@ __device__ __forceinline__ S householder_norm_signed(S alpha, S ss) {
 // The nbp x nbp T of the panel is written to Tout (ld ldt).
 template <typename S, bool PENT>
 __device__ __forceinline__ void wg_panel_qr_mem(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* Tout, int ldt,
-                                                PanelSmem<S>& sm) {
+                                                MemPanelSmem<S>& sm) {
     const int tid = threadIdx.x;
     const int l = tid >> 3, g = tid & 7;
     for (int e = tid; e < 32 * 33; e += WG) sm.Ts[e] = S(0);
@@ -147,7 +186,7 @@ __device__ __forceinline__ void wg_panel_qr_mem(int m, int nbp, S* Rt, int ldr, 
 // memory until the end.  Same reflectors, R and T as wg_panel_qr_mem.
 template <typename S, bool PENT, int NR>
 __device__ __forceinline__ void wg_panel_qr_reg(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* Tout, int ldt,
-                                                PanelSmem<S>& sm) {
+                                                MemPanelSmem<S>& sm) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     S a[NR][32];
 #pragma unroll
@@ -269,13 +308,184 @@ __device__ __forceinline__ void wg_panel_qr_reg(int m, int nbp, S* Rt, int ldr, 
     __syncthreads();
 }
 
+// f(i, xs[i]) for i = 0 .. SR-1 in 16-row chunks, the next chunk's LDS loads issued before
+// the current chunk's work: one chunk of lookahead.  Without the scheduling fences the
+// compiler issues all SR loads first, and long segments then spill.
+template <typename S, int SR, typename F>
+__device__ __forceinline__ void col_chunks(const S* xs, F&& f) {
+    constexpr int CH = SR < 16 ? SR : 16, NCH = SR / CH;
+    S x[2][CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) x[0][u] = xs[u];
+#pragma unroll
+    for (int k = 0; k < NCH; ++k) {
+        if (k + 1 < NCH) {
+#pragma unroll
+            for (int u = 0; u < CH; ++u) x[(k + 1) & 1][u] = xs[(k + 1) * CH + u];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < CH; ++u) f(k * CH + u, x[k & 1][u]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// Register-resident panel QR with the columns across the lanes.  Thread (g, c) -- column
+// c = lane % 32, segment g = 2 * wave + lane / 32 -- holds SR consecutive rows of column c,
+// rows base + g SR ... (base = nbp for the square panel, whose top nbp x nbp block sits in
+// LDS; base = 0 for the pentagonal one, whose top block is the separate triangle R, also
+// staged in LDS).  Top row r is handled by segment r % 8.  Step j:
+//   - column j's lanes publish their register rows to their segment (LDS, same wave);
+//   - every thread forms its column's partial dot product with column j (fixed register
+//     indices, plus its four top rows), one barrier, eight partials summed per column;
+//   - each thread derives the reflector (all threads alike) and updates its rows with one FMA.
+// Reflector columns stay unscaled until the panel's end (scl[j] keeps 1 / (alpha - beta)),
+// so column j needs no case of its own.  The same dot products give the Gram column
+// G(p, j) = v_p^T v_j of the finished columns, from which wave 0 builds T at the end.
+// Same reflectors (sign rule included), R, V and T as wg_panel_qr_mem.
+template <typename S, bool PENT, int SR>
+__device__ __forceinline__ void wg_panel_qr_col(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* Tout, int ldt,
+                                                ColPanelSmem<S>& sm) {
+    static_assert(SR % 4 == 0 && SR <= ColPanelSmem<S>::kSegMax, "segment length");
+    const int tid = threadIdx.x, lane = tid & 63, c = lane & 31;
+    const int g = 2 * (tid >> 6) + (lane >> 5);
+    int r0 = (PENT ? 0 : nbp) + g * SR;  // this thread's first register row
+    // opaque per call: otherwise the row indices r0 + i of every segment length are hoisted out
+    // of the callers' link and panel loops and stay live (spilled) for the whole kernel
+    asm volatile("" : "+v"(r0));
+    const int cc = min(c, nbp - 1);
+    __syncthreads();  // the LDS may be shared with the phase before
+    for (int e = tid; e < 32 * 32; e += WG) {
+        const int r = e & 31, q = e >> 5;
+        const S v = Rt[min(r, nbp - 1) + (size_t)min(q, nbp - 1) * ldr];
+        sm.top[r * 33 + q] = (r < nbp && q < nbp && (!PENT || r <= q)) ? v : S(0);
+        sm.G[e] = S(0);
+    }
+    S a[SR];
+#pragma unroll
+    for (int i = 0; i < SR; ++i) a[i] = B[min(r0 + i, m - 1) + (size_t)cc * ldb];
+#pragma unroll
+    for (int i = 0; i < SR; ++i) a[i] = (r0 + i < m && c < nbp) ? a[i] : S(0);
+    __syncthreads();
+    S* xs = sm.xb[g];
+    for (int j = 0; j < nbp; ++j) {
+        const int par = j & 1;
+        wave_sync();  // this wave's reads of the previous step's column are done
+        if (c == j) {
+#pragma unroll
+            for (int i = 0; i < SR; ++i) xs[i] = a[i];
+        }
+        if ((j & 7) == g) sm.rowj[par][c] = sm.top[j * 33 + c];
+        wave_sync();
+        S d0 = S(0), d1 = S(0);
+        col_chunks<S, SR>(xs, [&](int i, S x) {
+            if (i & 1) d1 += x * a[i]; else d0 += x * a[i];
+        });
+        S d = d0 + d1;
+        if (!PENT) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = g + 8 * q;
+                const S x = sm.top[r * 33 + j], y = sm.top[r * 33 + c];
+                d += (r > j) ? x * y : S(0);
+            }
+        }
+        sm.red[par][g][c] = d;
+        __syncthreads();
+        S ss, dc;
+        {
+            S p[8], q[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) { p[u] = sm.red[par][u][j]; q[u] = sm.red[par][u][c]; }
+            ss = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+            dc = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
+        }
+        const S alpha = sm.rowj[par][j], rowc = sm.rowj[par][c];
+        S tau, scal, beta;
+        if (ss == S(0)) {
+            tau = S(0); scal = S(1); beta = alpha;
+        } else {
+            beta = -householder_norm_signed(alpha, ss);
+            tau = (beta - alpha) / beta;
+            scal = S(1) / (alpha - beta);
+        }
+        // w_c = tau v_j^T a_c for the trailing columns; v_j = [e_j; scal x_j]
+        const S vta = (PENT ? S(0) : rowc) + scal * dc;
+        const S w = (c > j && c < nbp) ? (rowc + scal * dc) * tau : S(0);
+        const S sw = scal * w;
+        if (g == 0 && c < j) sm.G[j * 32 + c] = sm.scl[c] * vta;  // column c is still unscaled
+        if (tid == 0) { sm.tau[j] = tau; sm.scl[j] = scal; }
+        col_chunks<S, SR>(xs, [&](int i, S x) { a[i] -= x * sw; });
+        if (!PENT) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = g + 8 * q;
+                if (r > j) sm.top[r * 33 + c] -= sm.top[r * 33 + j] * sw;
+                else if (r == j) sm.top[r * 33 + c] = (c == j) ? beta : sm.top[r * 33 + c] - w;
+            }
+        } else if ((j & 7) == g) {
+            sm.top[j * 33 + c] = (c == j) ? beta : sm.top[j * 33 + c] - w;
+        }
+    }
+    __syncthreads();
+    // T, row q per lane of wave 0: T(q, j) = -tau_j sum_{p=q}^{j-1} T(q, p) G(p, j), T(j, j) = tau_j.
+    // Each T(q, j) goes to LDS as it is formed (an unconditional store keeps the arithmetic in
+    // place; otherwise it is sunk into the final stores and all 496 G loads stay live).
+    if (tid < 32) {
+        const int q = tid;
+        S Tq[32];
+#pragma unroll
+        for (int jj = 0; jj < 32; ++jj) {
+            S s = S(0);
+#pragma unroll
+            for (int p = 0; p < 32; ++p)  // fixed trip count: both loops unroll fully, Tq stays in registers
+                if (p < jj) s += Tq[p] * sm.G[jj * 32 + p];
+            const S tj = sm.tau[jj];
+            Tq[jj] = (q < jj) ? -tj * s : (q == jj ? tj : S(0));
+            sm.T[jj * 32 + q] = Tq[jj];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < 32 * 32; e += WG) {
+        const int q = e & 31, jj = e >> 5;
+        if (q < nbp && jj < nbp) Tout[q + (size_t)jj * ldt] = sm.T[e];
+    }
+    // V below the diagonal (scaled now), R on and above it
+    if (c < nbp) {
+        const S sc = sm.scl[c];
+#pragma unroll
+        for (int i = 0; i < SR; ++i)
+            if (r0 + i < m) B[r0 + i + (size_t)c * ldb] = a[i] * sc;
+    }
+    for (int e = tid; e < 32 * 32; e += WG) {
+        const int r = e & 31, q = e >> 5;
+        if (r < nbp && q < nbp) {
+            const S v = sm.top[r * 33 + q];
+            if (r <= q) Rt[r + (size_t)q * ldr] = v;
+            else if (!PENT) Rt[r + (size_t)q * ldr] = v * sm.scl[q];
+        }
+    }
+    __syncthreads();
+}
+
 template <typename S, bool PENT>
 __device__ __forceinline__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* Tout, int ldt,
                                             PanelSmem<S>& sm) {
-    if (m <= WG) wg_panel_qr_reg<S, PENT, 1>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm);
-    else if (sizeof(S) == 4 && m <= 2 * WG) wg_panel_qr_reg<S, PENT, 2>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm);
-    else if (sizeof(S) == 4 && m <= 4 * WG) wg_panel_qr_reg<S, PENT, 4>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm);
-    else wg_panel_qr_mem<S, PENT>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm);
+    if constexpr (sizeof(S) == 4) {
+        const int rows = PENT ? m : m - nbp;  // rows below the top block
+        if (m <= 0) wg_panel_qr_mem<S, PENT>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.mem);  // reads no row of B
+        else if (rows <= 64) wg_panel_qr_col<S, PENT, 8>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
+        else if (rows <= 128) wg_panel_qr_col<S, PENT, 16>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
+        else if (rows <= 256) wg_panel_qr_col<S, PENT, 32>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
+        else if (rows <= 512) wg_panel_qr_col<S, PENT, 64>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
+        else if (rows <= 1024) wg_panel_qr_col<S, PENT, 128>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
+        else wg_panel_qr_mem<S, PENT>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.mem);
+    } else {
+        // FP64 keeps the row-layout register panel (one row per thread, m <= 256)
+        if (m <= WG) wg_panel_qr_reg<S, PENT, 1>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.mem);
+        else wg_panel_qr_mem<S, PENT>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.mem);
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -727,7 +937,7 @@ __device__ __forceinline__ void wg_trsm_ru(int m, int n, const S* __restrict__ R
 // ---------------------------------------------------------------------------------------
 // Blocked T (NB x k) of the unit-lower reflectors V (m x k, ldv) with scalars tau.
 template <typename S>
-__device__ __forceinline__ void wg_larft_blocked(int m, int k, const S* V, int ldv, const S* tau, S* Tout, PanelSmem<S>& sm) {
+__device__ __forceinline__ void wg_larft_blocked(int m, int k, const S* V, int ldv, const S* tau, S* Tout, MemPanelSmem<S>& sm) {
     const int tid = threadIdx.x;
     const int l = tid >> 3, g = tid & 7;
     for (int p0 = 0; p0 < k; p0 += NB) {
@@ -770,8 +980,9 @@ __device__ __forceinline__ void wg_larft_blocked(int m, int k, const S* V, int l
 
 template <typename S>
 __global__ __launch_bounds__(WG) void k_geqrt(const GeqrtTask<S>* __restrict__ tasks) {
-    __shared__ ApplySmem<S> asm_;
-    __shared__ PanelSmem<S> psm;
+    __shared__ ApplyPanelSmem<S> sm_;
+    ApplySmem<S>& asm_ = sm_.ap;
+    PanelSmem<S>& psm = sm_.pn;
     const GeqrtTask<S> t = tasks[blockIdx.x];
     const int kmin = min(t.m, t.n);
     for (int p0 = 0; p0 < kmin; p0 += NB) {
@@ -816,8 +1027,9 @@ __global__ __launch_bounds__(WG) void k_larfb(const LarfbTask<S>* __restrict__ t
 template <typename S>
 __global__ __launch_bounds__(WG) void k_tpqrt(const TpqrtTask<S>* __restrict__ tasks,
                                               const TpqrtLink<S>* __restrict__ links) {
-    __shared__ ApplySmem<S> asm_;
-    __shared__ PanelSmem<S> psm;
+    __shared__ ApplyPanelSmem<S> sm_;
+    ApplySmem<S>& asm_ = sm_.ap;
+    PanelSmem<S>& psm = sm_.pn;
     const TpqrtTask<S> t = tasks[blockIdx.x];
     for (int li = 0; li < t.nlinks; ++li) {
         const TpqrtLink<S> L = links[t.link0 + li];
@@ -1088,8 +1300,9 @@ template <typename S>
 __global__ __launch_bounds__(WG) void k_tpqrt_coop(const TpqrtCoopTask<S>* __restrict__ tasks,
                                                    const int* __restrict__ blk2task,
                                                    const TpqrtLink<S>* __restrict__ links, int* err) {
-    __shared__ ApplySmem<S> asm_;
-    __shared__ PanelSmem<S> psm;
+    __shared__ ApplyPanelSmem<S> sm_;
+    ApplySmem<S>& asm_ = sm_.ap;
+    PanelSmem<S>& psm = sm_.pn;
     const TpqrtCoopTask<S> t = tasks[blk2task[blockIdx.x]];
     const int p = blockIdx.x - t.b0, p0 = p * NB, kb = min(NB, t.k - p0);
     for (int li = 0; li < t.nlinks; ++li) {
@@ -1346,7 +1559,7 @@ __global__ __launch_bounds__(WG) void k_geqp3(const Geqp3Task<S>* __restrict__ t
     // V = C(:, 0:rank) (unit lower used), T blocked
     for (int e = tid; e < k * rank; e += WG) t.Vout[e] = C[e];
     __syncthreads();
-    if (rank > 0) wg_larft_blocked<S>(k, rank, t.Vout, k, t.tau, t.T, psm);
+    if (rank > 0) wg_larft_blocked<S>(k, rank, t.Vout, k, t.tau, t.T, psm.mem);
 }
 
 // Sparsify apply + scatter into the parent block (src/ptree.cpp:1867-1897, 289-340).
@@ -2620,7 +2833,7 @@ __global__ __launch_bounds__(WG) void k_geqp3_coop(const Geqp3CoopTask<S>* __res
     if (P > 1) task_barrier(t.bar, P, passed, err);
     if (p == 0) {
         if (tid == 0) *t.rank_out = rank;
-        if (rank > 0) wg_larft_blocked<S>(k, rank, t.Vout, k, t.tau, t.T, psm);
+        if (rank > 0) wg_larft_blocked<S>(k, rank, t.Vout, k, t.tau, t.T, psm.mem);
     }
 }
 
