@@ -17,7 +17,10 @@ LIB = os.path.join(HERE, "libspaqr_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SPAQR_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "learned.hip", "rrqr.cpp", "tree.cpp", "tree_rank.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "resident.hip", "learned.hip", "rrqr.cpp", "tree.cpp", "tree_rank.cpp", "capi.cpp"]
+
+# per-file flags: the resident-accumulator applies keep MFMA accumulators in VGPRs
+EXTRA = {"resident.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
@@ -29,7 +32,7 @@ COMMON = [
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC] + COMMON + EXTRA.get(src, []) + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))

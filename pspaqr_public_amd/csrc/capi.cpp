@@ -11,6 +11,7 @@
 #include "learned.hpp"
 #include "rrqr.hpp"
 #include "tree.hpp"
+#include "tree_impl.hpp"
 
 using namespace spaqr;
 
@@ -343,10 +344,32 @@ int spaqr_larfb_batched(int prec, int count, const spaqr_larfb_desc* d, void* st
         if (d[i].m < 0 || d[i].n < 0 || d[i].k < 0 || d[i].ldc < std::max(1, d[i].m) || (d[i].R && d[i].n != d[i].k2))
             return -3;
     if (count == 0) return 0;
-    return run_batched(d, sizeof(*d) * count, nullptr, 0, stream, [&](void* dd, void*, hipStream_t s) {
-        if (prec == 64) launch_larfb<double>((const LarfbTask<double>*)dd, count, s);
-        else launch_larfb<float>((const LarfbTask<float>*)dd, count, s);
-    });
+    if (prec == 64)
+        return run_batched(d, sizeof(*d) * count, nullptr, 0, stream, [&](void* dd, void*, hipStream_t s) {
+            launch_larfb<double>((const LarfbTask<double>*)dd, count, s);
+        });
+    // FP32: blocks the C-resident kernel takes (Q^T C only, m <= 512, n <= 128) go to it
+    std::vector<LarfbTask<float>> res[3], rest;
+    for (int i = 0; i < count; ++i) {
+        LarfbTask<float> t;
+        static_assert(sizeof(t) == sizeof(d[i]), "larfb desc layout");
+        memcpy(&t, &d[i], sizeof(t));
+        const int q = larfb_res_bucket(t);
+        (q < 0 ? rest : res[q]).push_back(t);
+    }
+    for (int q = 0; q < 3; ++q) {
+        if (res[q].empty()) continue;
+        const int n = (int)res[q].size();
+        const int rc = run_batched(res[q].data(), sizeof(LarfbTask<float>) * n, nullptr, 0, stream,
+                                   [&](void* dd, void*, hipStream_t s) {
+                                       launch_larfb_res((const LarfbTask<float>*)dd, n, 4 << q, s);
+                                   });
+        if (rc) return rc;
+    }
+    if (rest.empty()) return 0;
+    const int n = (int)rest.size();
+    return run_batched(rest.data(), sizeof(LarfbTask<float>) * n, nullptr, 0, stream,
+                       [&](void* dd, void*, hipStream_t s) { launch_larfb<float>((const LarfbTask<float>*)dd, n, s); });
 }
 
 int spaqr_tpqrt_batched(int prec, int count, const spaqr_tpqrt_desc* d, int nlinks, const spaqr_tpqrt_link* links,

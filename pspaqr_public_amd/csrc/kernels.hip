@@ -19,6 +19,7 @@
 
 #include "dpp.hpp"
 #include "kernels.hpp"
+#include "mfma.hpp"
 
 namespace spaqr {
 
@@ -39,10 +40,6 @@ struct ApplySmem<float> {
     float Ws[4 * 32 * 33];
 };
 
-typedef float floatx16 __attribute__((ext_vector_type(16)));
-// v_mfma_f32_32x32x2f32: lane l supplies A(l % 32, l / 32) and B(l / 32, l % 32); accumulator
-// entry r of lane l is D(mfma_row(l, r), l % 32).
-__device__ __forceinline__ int mfma_row(int l, int r) { return (r >> 2) * 8 + (l >> 5) * 4 + (r & 3); }
 
 // Shared memory of the workgroup panel QR.  `mem`: wg_panel_qr_mem, wg_larft_blocked and the
 // row-layout register panel (FP64); `col`: the column-layout register panel (FP32).

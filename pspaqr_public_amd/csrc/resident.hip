@@ -1,0 +1,190 @@
+// gfx950 block-reflector applies with the updated block resident in MFMA accumulators
+// (FP32).  Built with the MFMA accumulators in VGPRs (-amdgpu-mfma-vgpr-form): a resident
+// tile is both an accumulator (C -= V W) and a B operand (W = V^T C), and with AGPR
+// accumulators the compiler keeps a second, VGPR copy of every tile.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "launch.hpp"
+#include "mfma.hpp"
+
+namespace spaqr {
+
+// ---------------------------------------------------------------------------------------
+// FP32 block-reflector applies with C resident in MFMA accumulators.  Wave w owns the
+// 32-column slab 32 w .. 32 w + 31 of C as MT 32 x 32 accumulator tiles (entry r of lane l
+// in tile t is C(32 t + mfma_row(l, r), 32 w + l % 32)) for all panels of the reflector, so
+// per panel only V_p (staged once in LDS, shared by the waves) and T_p move, and C is read
+// and written once per task instead of twice per panel.  Each product's contraction index is
+// permuted to the accumulator layout, so the three products chain register to register: in
+// step r the lanes of half h = l / 32 contribute index mfma_row(32 h, r), and the A operand
+// is read from LDS at that index:
+//   W  = V_p^T C  : A(i, .) = V_p(row 32 t + mfma_row(l, r), i),  B = C tile entry r
+//   W2 = T_p^T W  : A(i, .) = T_p(mfma_row(l, r), i),             B = W entry r
+//   C -= V_p W2   : A(i, .) = V_p(row 32 t + i, mfma_row(l, r)),  B = W2 entry r
+template <int MT>
+struct ResSmem {
+    float Vs[32 * MT * 33];  // V_p rows p0 .. p0 + 32 MT - 1, row-major with 33-float rows
+    float Ts[32 * 33];       // T_p(q, i) at q * 33 + i
+};
+
+// Stage panel p0 (kb <= 32 columns) of V rows p0 .. m-1 into sm.Vs (unit lower: ones on the
+// panel diagonal, zeros above; zeros past row m) and T_p into sm.Ts (zeros outside the upper
+// kb x kb triangle).
+template <int MT, bool UNIT>
+__device__ __forceinline__ void res_stage(const float* __restrict__ V, int ldv, const float* __restrict__ T, int m,
+                                          int p0, int kb, ResSmem<MT>& sm) {
+    // RS row slots (consecutive threads, consecutive rows: coalesced), CG column groups, RPT
+    // rows per thread; columns go 8 at a time so few loads are live next to the accumulators
+    constexpr int RS = 32 * MT < WG ? 32 * MT : WG, CG = WG / RS, RPT = 32 * MT / RS, CPG = 32 / CG;
+    const int tid = threadIdx.x, rows = m - p0, lr0 = tid % RS, cg = tid / RS;
+    const float* __restrict__ Vp = V + p0;
+#pragma unroll
+    for (int j0 = 0; j0 < CPG; j0 += 8) {
+        float v[8][RPT];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+            const int j = cg * CPG + j0 + jj;
+            const float* __restrict__ col = Vp + (size_t)(p0 + min(j, kb - 1)) * ldv;
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) v[jj][q] = col[min(lr0 + q * RS, rows - 1)];
+        }
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+            const int j = cg * CPG + j0 + jj;
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int lr = lr0 + q * RS;
+                float x = (lr < rows && j < kb) ? v[jj][q] : 0.f;
+                if (UNIT) x = (lr > j) ? x : ((lr == j && j < kb) ? 1.f : 0.f);
+                sm.Vs[lr * 33 + j] = x;
+            }
+        }
+    }
+    for (int e = tid; e < 32 * 32; e += WG) {
+        const int q = e & 31, i = e >> 5;
+        const float tv = T[min(q, kb - 1) + (size_t)min(i, kb - 1) * NB];
+        sm.Ts[q * 33 + i] = (q <= i && i < kb) ? tv : 0.f;
+    }
+}
+
+// Resident tiles from / to one column (Cc, rows 0 .. m-1) of C, a tile at a time: full tiles
+// with immediate offsets, the partial one clamped (loads) or masked (stores).
+template <int MT>
+__device__ __forceinline__ void res_load(floatx16 (&acc)[MT], const float* __restrict__ Cc, int m, bool ok) {
+    const int l = threadIdx.x & 63;
+    const float* __restrict__ ch = Cc + 4 * (l >> 5);
+#pragma unroll
+    for (int tt = 0; tt < MT; ++tt) {
+        if (32 * tt + 32 <= m) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[tt][r] = ch[32 * tt + (r >> 2) * 8 + (r & 3)];
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = 32 * tt + mfma_row(l, r);
+                const float v = Cc[min(row, m - 1)];
+                acc[tt][r] = (row < m) ? v : 0.f;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[tt][r] = ok ? acc[tt][r] : 0.f;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+template <int MT>
+__device__ __forceinline__ void res_store(const floatx16 (&acc)[MT], float* __restrict__ Cc, int m) {
+    const int l = threadIdx.x & 63;
+    float* __restrict__ ch = Cc + 4 * (l >> 5);
+#pragma unroll
+    for (int tt = 0; tt < MT; ++tt) {
+        if (32 * tt + 32 <= m) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) ch[32 * tt + (r >> 2) * 8 + (r & 3)] = acc[tt][r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = 32 * tt + mfma_row(l, r);
+                if (row < m) Cc[row] = acc[tt][r];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// One panel on this wave's resident tiles t0 .. ntiles-1 (rows from p0 = 32 t0); Wtop, if
+// given, is added to W = V_p^T C first (the pentagonal top rows) and receives W2 = T^T W.
+template <int MT>
+__device__ __forceinline__ void res_panel(floatx16 (&acc)[MT], int t0, int ntiles, const ResSmem<MT>& sm,
+                                          floatx16* Wtop) {
+    const int l = threadIdx.x & 63, li = l & 31;
+    // one accumulation chain: a 32x32x2 FP32 MFMA holds the SIMD for as long as its latency
+    floatx16 W;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) W[r] = Wtop ? (*Wtop)[r] : 0.f;
+#pragma unroll
+    for (int tt = 0; tt < MT; ++tt) {
+        if (tt >= t0 && tt < ntiles) {
+            const float* vt = sm.Vs + (size_t)(32 * (tt - t0)) * 33;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const float a = vt[mfma_row(l, r) * 33 + li];
+                W = __builtin_amdgcn_mfma_f32_32x32x2f32(a, acc[tt][r], W, 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);  // one tile's operands at a time (else all are hoisted)
+    }
+    floatx16 W2;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) W2[r] = 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const float a = sm.Ts[mfma_row(l, r) * 33 + li];
+        W2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, W[r], W2, 0, 0, 0);
+    }
+    if (Wtop) *Wtop = W2;
+#pragma unroll
+    for (int tt = 0; tt < MT; ++tt) {
+        if (tt >= t0 && tt < ntiles) {
+            const float* vt = sm.Vs + (size_t)(32 * (tt - t0)) * 33;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const float a = -vt[li * 33 + mfma_row(l, r)];
+                acc[tt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, W2[r], acc[tt], 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// C <- Q^T C (larfb 'L','T', Q from V m x k unit lower with blocked T), C m x n with
+// m <= 32 MT and n <= 128, FP32.  Same transformation as k_larfb without a trsm.
+template <int MT>
+__global__ __launch_bounds__(WG) void k_larfb_res(const LarfbTask<float>* __restrict__ tasks) {
+    __shared__ ResSmem<MT> sm;
+    const LarfbTask<float> t = tasks[blockIdx.x];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6, li = l & 31;
+    const int m = t.m, n = t.n, k = t.k, ntiles = (m + 31) >> 5;
+    const int col = 32 * w + li;
+    const bool active = 32 * w < n;  // this wave has columns (uniform)
+    float* __restrict__ Cc = t.C + (size_t)min(col, n - 1) * t.ldc;
+    floatx16 acc[MT];
+    res_load<MT>(acc, Cc, m, col < n);
+    for (int p0 = 0; p0 < k; p0 += 32) {
+        const int kb = min(32, k - p0);
+        __syncthreads();  // the previous panel's readers of sm are done
+        res_stage<MT, true>(t.V, t.ldv, t.T + (size_t)p0 * NB, m, p0, kb, sm);
+        __syncthreads();
+        if (active) res_panel<MT>(acc, p0 >> 5, ntiles, sm, nullptr);
+    }
+    if (col < n) res_store<MT>(acc, Cc, m);
+}
+
+void launch_larfb_res(const LarfbTask<float>* d, int n, int mt, hipStream_t s) {
+    if (n <= 0) return;
+    if (mt == 4) hipLaunchKernelGGL(k_larfb_res<4>, dim3(n), dim3(WG), 0, s, d);
+    else if (mt == 8) hipLaunchKernelGGL(k_larfb_res<8>, dim3(n), dim3(WG), 0, s, d);
+    else hipLaunchKernelGGL(k_larfb_res<16>, dim3(n), dim3(WG), 0, s, d);
+}
+
+}  // namespace spaqr

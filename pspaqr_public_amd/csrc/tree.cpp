@@ -30,6 +30,7 @@
 #include <map>
 #include <set>
 #include <stdexcept>
+#include <string>
 #include <tuple>
 
 #include "kernels.hpp"
@@ -706,11 +707,26 @@ int Tree::factorize_t() {
     // Blocks of at most 64 x 64 go to the one-wave kernels (SPAQR_SMALL_TASKS=0 keeps every
     // task on the workgroup kernels); the rest stay on the workgroup kernels.
     static const bool small_tasks = getenv("SPAQR_SMALL_TASKS") == nullptr || atoi(getenv("SPAQR_SMALL_TASKS")) != 0;
+    // SPAQR_DEBUG_SHAPES=1: flop-weighted shapes of the workgroup-kernel apply tasks, printed at the end
+    static const bool dbg_shapes = getenv("SPAQR_DEBUG_SHAPES") != nullptr;
+    std::map<std::string, std::array<double, 2>> shapes;  // key -> {tasks, flops}
+    auto bucket = [](int v) { int b = 16; while (b < v) b *= 2; return b; };
+    auto shape_add = [&](const char* cls, int k, int m, int n, double fl) {
+        char key[96];
+        snprintf(key, sizeof key, "%-7s k<=%-5d m<=%-5d n<=%-4d", cls, bucket(k), bucket(m), bucket(n));
+        auto& e = shapes[key];
+        e[0] += 1;
+        e[1] += fl;
+    };
     struct LarfbBatch {
         LarfbTask<S>* d[4] = {nullptr, nullptr, nullptr, nullptr};  // buckets 16 / 32 / 64, workgroup
         int n[4] = {0, 0, 0, 0};
+        LarfbTask<S>* dr[3] = {nullptr, nullptr, nullptr};  // C-resident FP32 buckets, m <= 128 / 256 / 512
+        int nr[3] = {0, 0, 0};
         bool trsm = false;
-        int launches() const { return (n[0] > 0) + (n[1] > 0) + (n[2] > 0) + (n[3] > 0); }
+        int launches() const {
+            return (n[0] > 0) + (n[1] > 0) + (n[2] > 0) + (n[3] > 0) + (nr[0] > 0) + (nr[1] > 0) + (nr[2] > 0);
+        }
     };
     auto prep_larfb = [&](const std::vector<LarfbTask<S>>& v, bool trsm) {
         LarfbBatch b;
@@ -723,9 +739,17 @@ int Tree::factorize_t() {
             }, sm, big);
         else
             big = v;
+        std::vector<LarfbTask<S>> res[3], rest;
+        for (const auto& t : big) {
+            const int q = larfb_res_bucket(t);
+            (q < 0 ? rest : res[q]).push_back(t);
+        }
         for (int q = 0; q < 3; ++q) { b.d[q] = upload_vec(sm[q]); b.n[q] = (int)sm[q].size(); }
-        b.d[3] = upload_vec(big);
-        b.n[3] = (int)big.size();
+        for (int q = 0; q < 3; ++q) { b.dr[q] = upload_vec(res[q]); b.nr[q] = (int)res[q].size(); }
+        b.d[3] = upload_vec(rest);
+        b.n[3] = (int)rest.size();
+        if (dbg_shapes)
+            for (const auto& t : big) shape_add(trsm ? "larfb+t" : "larfb", t.V ? t.k : t.k2, t.m, t.n, 4.0 * t.m * t.n * t.k);
         return b;
     };
     struct GeqrtBatch {
@@ -751,6 +775,8 @@ int Tree::factorize_t() {
     };
     auto run_larfb = [&](const LarfbBatch& b) {
         for (int q = 0; q < 3; ++q) launch_larfb_small<S>(b.d[q], b.n[q], 16 << q, b.trsm, stream_);
+        if constexpr (sizeof(S) == 4)
+            for (int q = 0; q < 3; ++q) launch_larfb_res(b.dr[q], b.nr[q], 4 << q, stream_);
         launch_larfb<S>(b.d[3], b.n[3], stream_);
     };
     struct SparsBatch {
@@ -997,6 +1023,12 @@ int Tree::factorize_t() {
                 for (int b = 0; b < 3; ++b) { d_tpm[q][b] = upload_vec(sm[b]); n_tpm[q][b] = (int)sm[b].size(); }
                 d_tpm[q][3] = upload_vec(big);
                 n_tpm[q][3] = (int)big.size();
+                if (dbg_shapes)
+                    for (const auto& t : big) {
+                        int rows = 0;
+                        for (int j = 0; j < t.nlinks; ++j) rows += tpml[t.link0 + j].rows;
+                        shape_add("tpmqrt", t.k, rows / std::max(t.nlinks, 1), t.ncols, 4.0 * t.k * t.ncols * rows);
+                    }
             }
             CopyTask<S>* d_scopy = upload_vec(scopy);
 
@@ -1297,6 +1329,14 @@ int Tree::factorize_t() {
                 "[host] fillin %.1f  descr %.1f (colour %.1f tpmqrt %.1f)  launch+ranks %.1f  merge %.1f (+blocks %.1f)  spars "
                 "build %.1f split+upload %.1f launch %.1f  tail-sync %.1f  solve-sched %.1f ms\n",
                 hms[0], hms[1], hms[10], hms[11], hms[2], hms[3], hms[7], hms[8], hms[9], hms[4], hms[5], hms[6]);
+    if (dbg_shapes) {
+        double tot = 0;
+        for (auto& e : shapes) tot += e.second[1];
+        for (auto& e : shapes)
+            if (e.second[1] > 0.005 * tot)
+                fprintf(stderr, "[shape] %s tasks %8.0f  GF %8.2f  (%4.1f%%)\n", e.first.c_str(), e.second[0],
+                        e.second[1] * 1e-9, 100.0 * e.second[1] / tot);
+    }
     t_factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
 }

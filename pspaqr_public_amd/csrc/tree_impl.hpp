@@ -58,6 +58,16 @@ void split_larfb(const std::vector<LarfbTask<S>>& in, std::vector<LarfbTask<S>>&
     }
 }
 
+// Bucket of the C-resident FP32 larfb kernel for a task (0: m <= 128, 1: m <= 256, 2: m <= 512),
+// or -1 when the task stays on the workgroup kernel (FP64, a trsm, m > 512 or n > 128).
+// SPAQR_LARFB_RES=0 keeps every task on the workgroup kernel.
+template <typename S>
+inline int larfb_res_bucket(const LarfbTask<S>& t) {
+    static const bool on = getenv("SPAQR_LARFB_RES") == nullptr || atoi(getenv("SPAQR_LARFB_RES")) != 0;
+    if (!on || sizeof(S) != 4 || !t.V || t.R || t.n > 128 || t.m > 512) return -1;
+    return t.m <= 128 ? 0 : (t.m <= 256 ? 1 : 2);
+}
+
 // Flops of an m x n Householder QR.
 template <typename S>
 inline double flops_geqrt(double m, double n) { return 2 * m * n * n - 2 * n * n * n / 3; }

@@ -63,7 +63,8 @@ def test_geqrt_grouped(prec):
 def test_larfb_and_trsm(prec):
     rng = np.random.default_rng(2)
     dt = np.float64 if prec == 64 else np.float32
-    cases = [(50, 20, 50), (70, 45, 33), (8, 3, 8), (128, 65, 128)]
+    # FP32 blocks of m <= 512 and n <= 128 run on the C-resident kernel (m <= 128 / 256 / 512 buckets)
+    cases = [(50, 20, 50), (70, 45, 33), (8, 3, 8), (128, 65, 128), (200, 128, 150), (257, 33, 100), (500, 70, 481)]
     keep, descs, refs = [], [], []
     for (m, n, k) in cases:
         A = rng.standard_normal((m, k))
