@@ -395,13 +395,38 @@ int spaqr_tpmqrt_batched(int prec, int count, const spaqr_tpmqrt_desc* d, int nl
     for (int i = 0; i < count; ++i)
         if (d[i].link0 < 0 || d[i].nlinks < 0 || d[i].link0 + d[i].nlinks > nlinks) return -3;
     if (count == 0) return 0;
-    return run_batched(d, sizeof(*d) * count, links, sizeof(*links) * nlinks, stream,
-                       [&](void* dd, void* dl, hipStream_t s) {
-                           if (prec == 64)
+    if (prec == 64)
+        return run_batched(d, sizeof(*d) * count, links, sizeof(*links) * nlinks, stream,
+                           [&](void* dd, void* dl, hipStream_t s) {
                                launch_tpmqrt<double>((const TpmqrtTask<double>*)dd, (const TpmqrtLink<double>*)dl, count, s);
-                           else
-                               launch_tpmqrt<float>((const TpmqrtTask<float>*)dd, (const TpmqrtLink<float>*)dl, count, s);
-                       });
+                           });
+    // FP32: chains the C-resident kernel takes (tiles <= 64 columns, links <= 512 rows) go to it
+    std::vector<TpmqrtLink<float>> hl(nlinks);
+    static_assert(sizeof(TpmqrtLink<float>) == sizeof(spaqr_tpmqrt_link), "tpmqrt link layout");
+    if (nlinks) memcpy(hl.data(), links, sizeof(TpmqrtLink<float>) * nlinks);
+    std::vector<TpmqrtTask<float>> res[3], rest;
+    for (int i = 0; i < count; ++i) {
+        TpmqrtTask<float> t;
+        static_assert(sizeof(t) == sizeof(d[i]), "tpmqrt desc layout");
+        memcpy(&t, &d[i], sizeof(t));
+        const int q = tpmqrt_res_bucket(t, hl.data());
+        (q < 0 ? rest : res[q]).push_back(t);
+    }
+    for (int q = 0; q < 4; ++q) {
+        std::vector<TpmqrtTask<float>>& v = q < 3 ? res[q] : rest;
+        if (v.empty()) continue;
+        const int n = (int)v.size();
+        const int rc = run_batched(v.data(), sizeof(TpmqrtTask<float>) * n, links, sizeof(*links) * nlinks, stream,
+                                   [&](void* dd, void* dl, hipStream_t s) {
+                                       if (q < 3)
+                                           launch_tpmqrt_res((const TpmqrtTask<float>*)dd, (const TpmqrtLink<float>*)dl, n,
+                                                             4 << q, s);
+                                       else
+                                           launch_tpmqrt<float>((const TpmqrtTask<float>*)dd, (const TpmqrtLink<float>*)dl, n, s);
+                                   });
+        if (rc) return rc;
+    }
+    return 0;
 }
 
 int spaqr_geqp3_batched(int prec, int count, const spaqr_geqp3_desc* d, double tol, double tie_rel, void* stream) {

@@ -16,6 +16,8 @@ template <typename S> void launch_geqrt_wave(const GeqrtTask<S>* d, int n, int m
 template <typename S> void launch_larfb(const LarfbTask<S>* d, int n, hipStream_t s);
 // FP32 Q^T C with C resident in MFMA accumulators: m <= 32 mt (mt = 4, 8, 16), n <= 128, no trsm.
 void launch_larfb_res(const LarfbTask<float>* d, int n, int mt, hipStream_t s);
+// FP32 tpmqrt chains with each link's tile resident in MFMA accumulators: ncols <= 64, link rows <= 32 mt.
+void launch_tpmqrt_res(const TpmqrtTask<float>* d, const TpmqrtLink<float>* l, int n, int mt, hipStream_t s);
 template <typename S> void launch_panel_qr(const PanelTask<S>* d, int n, bool pent, hipStream_t s);
 template <typename S> void launch_panel_apply(const PanelApplyTask<S>* d, int n, bool pent, hipStream_t s);
 template <typename S> void launch_tpqrt(const TpqrtTask<S>* d, const TpqrtLink<S>* l, int n, hipStream_t s);

@@ -28,29 +28,30 @@ struct ResSmem {
     float Ts[32 * 33];       // T_p(q, i) at q * 33 + i
 };
 
-// Stage panel p0 (kb <= 32 columns) of V rows p0 .. m-1 into sm.Vs (unit lower: ones on the
-// panel diagonal, zeros above; zeros past row m) and T_p into sm.Ts (zeros outside the upper
-// kb x kb triangle).
-template <int MT, bool UNIT>
-__device__ __forceinline__ void res_stage(const float* __restrict__ V, int ldv, const float* __restrict__ T, int m,
-                                          int p0, int kb, ResSmem<MT>& sm) {
+// Stage one panel of V -- Vp points at its first row and column, `rows` rows, kb <= 32
+// columns -- into sm.Vs (UNIT: unit lower, ones on the diagonal and zeros above; zeros past
+// the rows and columns) and its T block (T, ld NB) into sm.Ts (zeros outside the upper kb x kb
+// triangle).  NT threads.
+template <int MT, bool UNIT, int NT>
+__device__ __forceinline__ void res_stage(const float* __restrict__ Vp, int ldv, const float* __restrict__ T, int rows,
+                                          int kb, ResSmem<MT>& sm) {
     // RS row slots (consecutive threads, consecutive rows: coalesced), CG column groups, RPT
     // rows per thread; columns go 8 at a time so few loads are live next to the accumulators
-    constexpr int RS = 32 * MT < WG ? 32 * MT : WG, CG = WG / RS, RPT = 32 * MT / RS, CPG = 32 / CG;
-    const int tid = threadIdx.x, rows = m - p0, lr0 = tid % RS, cg = tid / RS;
-    const float* __restrict__ Vp = V + p0;
+    constexpr int RS = 32 * MT < NT ? 32 * MT : NT, CG = NT / RS, RPT = 32 * MT / RS, CPG = 32 / CG;
+    constexpr int JG = RPT >= 8 ? 1 : 8 / RPT;  // columns per group: 8 loads in flight per thread
+    const int tid = threadIdx.x, lr0 = tid % RS, cg = tid / RS;
 #pragma unroll
-    for (int j0 = 0; j0 < CPG; j0 += 8) {
-        float v[8][RPT];
+    for (int j0 = 0; j0 < CPG; j0 += JG) {
+        float v[JG][RPT];
 #pragma unroll
-        for (int jj = 0; jj < 8; ++jj) {
+        for (int jj = 0; jj < JG; ++jj) {
             const int j = cg * CPG + j0 + jj;
-            const float* __restrict__ col = Vp + (size_t)(p0 + min(j, kb - 1)) * ldv;
+            const float* __restrict__ col = Vp + (size_t)min(j, kb - 1) * ldv;
 #pragma unroll
             for (int q = 0; q < RPT; ++q) v[jj][q] = col[min(lr0 + q * RS, rows - 1)];
         }
 #pragma unroll
-        for (int jj = 0; jj < 8; ++jj) {
+        for (int jj = 0; jj < JG; ++jj) {
             const int j = cg * CPG + j0 + jj;
 #pragma unroll
             for (int q = 0; q < RPT; ++q) {
@@ -61,7 +62,7 @@ __device__ __forceinline__ void res_stage(const float* __restrict__ V, int ldv, 
             }
         }
     }
-    for (int e = tid; e < 32 * 32; e += WG) {
+    for (int e = tid; e < 32 * 32; e += NT) {
         const int q = e & 31, i = e >> 5;
         const float tv = T[min(q, kb - 1) + (size_t)min(i, kb - 1) * NB];
         sm.Ts[q * 33 + i] = (q <= i && i < kb) ? tv : 0.f;
@@ -72,7 +73,8 @@ __device__ __forceinline__ void res_stage(const float* __restrict__ V, int ldv, 
 // with immediate offsets, the partial one clamped (loads) or masked (stores).
 template <int MT>
 __device__ __forceinline__ void res_load(floatx16 (&acc)[MT], const float* __restrict__ Cc, int m, bool ok) {
-    const int l = threadIdx.x & 63;
+    int l = threadIdx.x & 63;
+    asm volatile("" : "+v"(l));  // opaque per call: keeps the per-lane row offsets out of callers' loops
     const float* __restrict__ ch = Cc + 4 * (l >> 5);
 #pragma unroll
     for (int tt = 0; tt < MT; ++tt) {
@@ -94,7 +96,8 @@ __device__ __forceinline__ void res_load(floatx16 (&acc)[MT], const float* __res
 }
 template <int MT>
 __device__ __forceinline__ void res_store(const floatx16 (&acc)[MT], float* __restrict__ Cc, int m) {
-    const int l = threadIdx.x & 63;
+    int l = threadIdx.x & 63;
+    asm volatile("" : "+v"(l));
     float* __restrict__ ch = Cc + 4 * (l >> 5);
 #pragma unroll
     for (int tt = 0; tt < MT; ++tt) {
@@ -112,8 +115,9 @@ __device__ __forceinline__ void res_store(const floatx16 (&acc)[MT], float* __re
     }
 }
 
-// One panel on this wave's resident tiles t0 .. ntiles-1 (rows from p0 = 32 t0); Wtop, if
-// given, is added to W = V_p^T C first (the pentagonal top rows) and receives W2 = T^T W.
+// One panel on this wave's resident tiles t0 .. ntiles-1 (sm.Vs row 0 is tile t0's row 0);
+// Wtop, if given, holds the pentagonal top rows A (added to W = A + V_p^T C first) and
+// receives A - W2.
 template <int MT>
 __device__ __forceinline__ void res_panel(floatx16 (&acc)[MT], int t0, int ntiles, const ResSmem<MT>& sm,
                                           floatx16* Wtop) {
@@ -142,7 +146,7 @@ __device__ __forceinline__ void res_panel(floatx16 (&acc)[MT], int t0, int ntile
         const float a = sm.Ts[mfma_row(l, r) * 33 + li];
         W2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, W[r], W2, 0, 0, 0);
     }
-    if (Wtop) *Wtop = W2;
+    if (Wtop) *Wtop -= W2;
 #pragma unroll
     for (int tt = 0; tt < MT; ++tt) {
         if (tt >= t0 && tt < ntiles) {
@@ -173,11 +177,59 @@ __global__ __launch_bounds__(WG) void k_larfb_res(const LarfbTask<float>* __rest
     for (int p0 = 0; p0 < k; p0 += 32) {
         const int kb = min(32, k - p0);
         __syncthreads();  // the previous panel's readers of sm are done
-        res_stage<MT, true>(t.V, t.ldv, t.T + (size_t)p0 * NB, m, p0, kb, sm);
+        res_stage<MT, true, WG>(t.V + p0 + (size_t)p0 * t.ldv, t.ldv, t.T + (size_t)p0 * NB, m - p0, kb, sm);
         __syncthreads();
         if (active) res_panel<MT>(acc, p0 >> 5, ntiles, sm, nullptr);
     }
     if (col < n) res_store<MT>(acc, Cc, m);
+}
+
+// tpmqrt chain (PENT) of one column tile (ncols <= 64) on two waves: for each link j,
+// [Atop; B_j] <- Q_j^T [Atop; B_j] with B_j's tile (rows_j <= 32 MT) resident in the
+// accumulators for all of the link's panels; Atop's 32 rows of a panel are read into the W
+// accumulator and written back as Atop - W2.  Same transformation as k_tpmqrt.
+constexpr int kResTpmThreads = 128;
+template <int MT>
+__global__ __launch_bounds__(kResTpmThreads) void k_tpmqrt_res(const TpmqrtTask<float>* __restrict__ tasks,
+                                                               const TpmqrtLink<float>* __restrict__ links) {
+    __shared__ ResSmem<MT> sm;
+    const TpmqrtTask<float> t = tasks[blockIdx.x];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6, li = l & 31;
+    const int k = t.k, nc = t.ncols, col = 32 * w + li, cc = min(col, nc - 1);
+    const bool active = 32 * w < nc;  // this wave has columns (uniform)
+    float* __restrict__ Acol = t.Atop + (size_t)cc * t.lda;
+    floatx16 acc[MT];
+    for (int j = 0; j < t.nlinks; ++j) {
+        const TpmqrtLink<float> L = links[t.link0 + j];
+        const int rows = L.rows, ntiles = (rows + 31) >> 5;
+        if (rows <= 0) continue;  // an empty link's reflectors are the identity
+        float* __restrict__ Cc = L.B + (size_t)(t.col0 + cc) * L.ldb;
+        if (active) res_load<MT>(acc, Cc, rows, col < nc);
+        for (int p0 = 0; p0 < k; p0 += 32) {
+            const int kb = min(32, k - p0);
+            __syncthreads();  // the previous panel's readers of sm are done
+            res_stage<MT, false, kResTpmThreads>(L.V + (size_t)p0 * L.ldv, L.ldv, L.T + (size_t)p0 * NB, rows, kb, sm);
+            __syncthreads();
+            if (active) {
+                floatx16 top;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int q = mfma_row(l, r);
+                    const float v = Acol[p0 + min(q, kb - 1)];
+                    top[r] = (q < kb && col < nc) ? v : 0.f;
+                }
+                res_panel<MT>(acc, 0, ntiles, sm, &top);
+                if (col < nc) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int q = mfma_row(l, r);
+                        if (q < kb) Acol[p0 + q] = top[r];
+                    }
+                }
+            }
+        }
+        if (active && col < nc) res_store<MT>(acc, Cc, rows);
+    }
 }
 
 void launch_larfb_res(const LarfbTask<float>* d, int n, int mt, hipStream_t s) {
@@ -185,6 +237,13 @@ void launch_larfb_res(const LarfbTask<float>* d, int n, int mt, hipStream_t s) {
     if (mt == 4) hipLaunchKernelGGL(k_larfb_res<4>, dim3(n), dim3(WG), 0, s, d);
     else if (mt == 8) hipLaunchKernelGGL(k_larfb_res<8>, dim3(n), dim3(WG), 0, s, d);
     else hipLaunchKernelGGL(k_larfb_res<16>, dim3(n), dim3(WG), 0, s, d);
+}
+void launch_tpmqrt_res(const TpmqrtTask<float>* d, const TpmqrtLink<float>* l, int n, int mt, hipStream_t s) {
+    if (n <= 0) return;
+    const dim3 b(kResTpmThreads);
+    if (mt == 4) hipLaunchKernelGGL(k_tpmqrt_res<4>, dim3(n), b, 0, s, d, l);
+    else if (mt == 8) hipLaunchKernelGGL(k_tpmqrt_res<8>, dim3(n), b, 0, s, d, l);
+    else hipLaunchKernelGGL(k_tpmqrt_res<16>, dim3(n), b, 0, s, d, l);
 }
 
 }  // namespace spaqr

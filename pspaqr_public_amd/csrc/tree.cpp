@@ -1006,6 +1006,8 @@ int Tree::factorize_t() {
             // per colour: one-wave buckets (k <= 64, links of 1..64 rows) and the workgroup rest
             std::vector<std::array<TpmqrtTask<S>*, 4>> d_tpm(ncolors);
             std::vector<std::array<int, 4>> n_tpm(ncolors);
+            std::vector<std::array<TpmqrtTask<S>*, 3>> d_tpmr(ncolors);  // C-resident FP32 buckets
+            std::vector<std::array<int, 3>> n_tpmr(ncolors);
             for (int q = 0; q < ncolors; ++q) {
                 std::vector<TpmqrtTask<S>> sm[3], big;
                 if (small_tasks)
@@ -1020,9 +1022,15 @@ int Tree::factorize_t() {
                     }, sm, big);
                 else
                     big = tpm[q];
+                std::vector<TpmqrtTask<S>> res[3], rest;
+                for (const auto& t : big) {
+                    const int b = tpmqrt_res_bucket(t, tpml.data());
+                    (b < 0 ? rest : res[b]).push_back(t);
+                }
                 for (int b = 0; b < 3; ++b) { d_tpm[q][b] = upload_vec(sm[b]); n_tpm[q][b] = (int)sm[b].size(); }
-                d_tpm[q][3] = upload_vec(big);
-                n_tpm[q][3] = (int)big.size();
+                for (int b = 0; b < 3; ++b) { d_tpmr[q][b] = upload_vec(res[b]); n_tpmr[q][b] = (int)res[b].size(); }
+                d_tpm[q][3] = upload_vec(rest);
+                n_tpm[q][3] = (int)rest.size();
                 if (dbg_shapes)
                     for (const auto& t : big) {
                         int rows = 0;
@@ -1063,6 +1071,8 @@ int Tree::factorize_t() {
                 timed(P_TPMQRT, ncolors, ntpm, [&] {
                     for (int q = 0; q < ncolors; ++q) {
                         for (int b = 0; b < 3; ++b) launch_tpmqrt_wave<S>(d_tpm[q][b], d_tpml, n_tpm[q][b], 16 << b, stream_);
+                        if constexpr (sizeof(S) == 4)
+                            for (int b = 0; b < 3; ++b) launch_tpmqrt_res(d_tpmr[q][b], d_tpml, n_tpmr[q][b], 4 << b, stream_);
                         launch_tpmqrt<S>(d_tpm[q][3], d_tpml, n_tpm[q][3], stream_);
                     }
                 });

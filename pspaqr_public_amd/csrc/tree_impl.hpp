@@ -68,6 +68,19 @@ inline int larfb_res_bucket(const LarfbTask<S>& t) {
     return t.m <= 128 ? 0 : (t.m <= 256 ? 1 : 2);
 }
 
+// Bucket of the C-resident FP32 tpmqrt kernel for a chain task (0: links of <= 128 rows,
+// 1: <= 256, 2: <= 512), or -1 when it stays on the workgroup kernel (FP64, a tile wider than
+// 64 columns or a longer link).  SPAQR_TPMQRT_RES=0 keeps every task on the workgroup kernel.
+template <typename S>
+inline int tpmqrt_res_bucket(const TpmqrtTask<S>& t, const TpmqrtLink<S>* links) {
+    static const bool on = getenv("SPAQR_TPMQRT_RES") == nullptr || atoi(getenv("SPAQR_TPMQRT_RES")) != 0;
+    if (!on || sizeof(S) != 4 || t.ncols > 64) return -1;
+    int rmax = 0;
+    for (int j = 0; j < t.nlinks; ++j) rmax = std::max(rmax, links[t.link0 + j].rows);
+    if (rmax > 512) return -1;
+    return rmax <= 128 ? 0 : (rmax <= 256 ? 1 : 2);
+}
+
 // Flops of an m x n Householder QR.
 template <typename S>
 inline double flops_geqrt(double m, double n) { return 2 * m * n * n - 2 * n * n * n / 3; }

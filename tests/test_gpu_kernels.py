@@ -100,7 +100,8 @@ def test_tpqrt_tpmqrt_chain(prec):
     rng = np.random.default_rng(3)
     dt = np.float64 if prec == 64 else np.float32
     tol = 1e-12 if prec == 64 else 5e-4
-    fronts = [(45, [30, 7, 64]), (8, [5, 9]), (70, [100])]
+    # FP32 chains with links of <= 128 / 256 / 512 rows run on the C-resident kernel
+    fronts = [(45, [30, 7, 64]), (8, [5, 9]), (70, [100]), (40, [300, 20]), (66, [200, 150])]
     keep = []
     tq_desc, tq_links, tm_desc, tm_links = [], [], [], []
     expect_R, expect_B, expect_top, expect_C = [], [], [], []
