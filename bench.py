@@ -169,13 +169,15 @@ def main():
         fp64 = dict(seconds=s64, factor_s=f64, iters=res64.iters, true_relres=res64.true_res)
         del T64
 
-    # HBM bytes per dispatch of the dominant class, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
-    # (calibrated on a device copy) kept beside the kernel-trace summaries
-    traffic_by_class = {}
+    # HBM bytes per step of each launch class, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    # (calibrated on a device copy) kept beside the kernel-trace summaries; per launch below
+    bytes_per_step = {}
     tp = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if os.path.exists(tp):
         with open(tp) as f:
-            traffic_by_class = json.load(f).get("bytes_per_dispatch", {})
+            bytes_per_step = json.load(f).get("bytes_per_step", {})
+    traffic_by_class = {c: int(round(v * max(args.steps, 1) / max(prof[c]["launches"], 1)))
+                        for c, v in bytes_per_step.items() if c in prof and prof[c]["launches"]}
 
     # ---- roofline of the dominant kernel class (by time, over the timed steps)
     cls = max((c for c in prof if c not in ("precond_solve", "gmres_orth")), key=lambda c: prof[c]["ms"])

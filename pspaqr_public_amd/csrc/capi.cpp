@@ -481,10 +481,31 @@ int spaqr_spars_apply_batched(int prec, int count, const spaqr_spars_desc* d, vo
     if (prec != 32 && prec != 64) return -1;
     if (count < 0 || (count > 0 && !d)) return -2;
     if (count == 0) return 0;
-    return run_batched(d, sizeof(*d) * count, nullptr, 0, stream, [&](void* dd, void*, hipStream_t s) {
-        if (prec == 64) launch_spars_apply<double>((const SparsApplyTask<double>*)dd, count, s);
-        else launch_spars_apply<float>((const SparsApplyTask<float>*)dd, count, s);
-    });
+    if (prec == 64)
+        return run_batched(d, sizeof(*d) * count, nullptr, 0, stream, [&](void* dd, void*, hipStream_t s) {
+            launch_spars_apply<double>((const SparsApplyTask<double>*)dd, count, s);
+        });
+    // FP32: right-transform (pad 1) and left-transform (pad 2) tasks the C-resident kernels take
+    std::vector<SparsApplyTask<float>> res[6], rest;
+    for (int i = 0; i < count; ++i) {
+        SparsApplyTask<float> t;
+        static_assert(sizeof(t) == sizeof(d[i]), "spars desc layout");
+        memcpy(&t, &d[i], sizeof(t));
+        const int q = spars_res_bucket(t);
+        (q < 0 ? rest : res[q]).push_back(t);
+    }
+    for (int q = 0; q < 7; ++q) {
+        std::vector<SparsApplyTask<float>>& v = q < 6 ? res[q] : rest;
+        if (v.empty()) continue;
+        const int n = (int)v.size();
+        const int rc = run_batched(v.data(), sizeof(SparsApplyTask<float>) * n, nullptr, 0, stream,
+                                   [&](void* dd, void*, hipStream_t s) {
+                                       if (q < 6) launch_spars_res((const SparsApplyTask<float>*)dd, n, 4 << (q % 3), q < 3, s);
+                                       else launch_spars_apply<float>((const SparsApplyTask<float>*)dd, n, s);
+                                   });
+        if (rc) return rc;
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -69,23 +69,25 @@ __device__ __forceinline__ void res_stage(const float* __restrict__ Vp, int ldv,
     }
 }
 
-// Resident tiles from / to one column (Cc, rows 0 .. m-1) of C, a tile at a time: full tiles
-// with immediate offsets, the partial one clamped (loads) or masked (stores).
-template <int MT>
-__device__ __forceinline__ void res_load(floatx16 (&acc)[MT], const float* __restrict__ Cc, int m, bool ok) {
+// Resident tiles from / to one column of C, a tile at a time: entry `row` of this lane's
+// column is base[row] (TRC: base[row * ld], C stored transposed).  Full tiles use fixed
+// offsets; the partial one is clamped (loads) or masked (stores) at m rows.
+template <int MT, bool TRC>
+__device__ __forceinline__ void res_load(floatx16 (&acc)[MT], const float* __restrict__ base, int ld, int m, bool ok) {
     int l = threadIdx.x & 63;
     asm volatile("" : "+v"(l));  // opaque per call: keeps the per-lane row offsets out of callers' loops
-    const float* __restrict__ ch = Cc + 4 * (l >> 5);
+    const size_t rs = TRC ? (size_t)ld : 1;
+    const float* __restrict__ bh = base + (size_t)(4 * (l >> 5)) * rs;
 #pragma unroll
     for (int tt = 0; tt < MT; ++tt) {
         if (32 * tt + 32 <= m) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc[tt][r] = ch[32 * tt + (r >> 2) * 8 + (r & 3)];
+            for (int r = 0; r < 16; ++r) acc[tt][r] = bh[(size_t)(32 * tt + (r >> 2) * 8 + (r & 3)) * rs];
         } else {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int row = 32 * tt + mfma_row(l, r);
-                const float v = Cc[min(row, m - 1)];
+                const float v = base[(size_t)min(row, m - 1) * rs];
                 acc[tt][r] = (row < m) ? v : 0.f;
             }
         }
@@ -94,21 +96,22 @@ __device__ __forceinline__ void res_load(floatx16 (&acc)[MT], const float* __res
         __builtin_amdgcn_sched_barrier(0);
     }
 }
-template <int MT>
-__device__ __forceinline__ void res_store(const floatx16 (&acc)[MT], float* __restrict__ Cc, int m) {
+template <int MT, bool TRC>
+__device__ __forceinline__ void res_store(const floatx16 (&acc)[MT], float* __restrict__ base, int ld, int m) {
     int l = threadIdx.x & 63;
     asm volatile("" : "+v"(l));
-    float* __restrict__ ch = Cc + 4 * (l >> 5);
+    const size_t rs = TRC ? (size_t)ld : 1;
+    float* __restrict__ bh = base + (size_t)(4 * (l >> 5)) * rs;
 #pragma unroll
     for (int tt = 0; tt < MT; ++tt) {
         if (32 * tt + 32 <= m) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) ch[32 * tt + (r >> 2) * 8 + (r & 3)] = acc[tt][r];
-        } else {
+            for (int r = 0; r < 16; ++r) bh[(size_t)(32 * tt + (r >> 2) * 8 + (r & 3)) * rs] = acc[tt][r];
+        } else if (32 * tt < m) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int row = 32 * tt + mfma_row(l, r);
-                if (row < m) Cc[row] = acc[tt][r];
+                if (row < m) base[(size_t)row * rs] = acc[tt][r];
             }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -173,7 +176,7 @@ __global__ __launch_bounds__(WG) void k_larfb_res(const LarfbTask<float>* __rest
     const bool active = 32 * w < n;  // this wave has columns (uniform)
     float* __restrict__ Cc = t.C + (size_t)min(col, n - 1) * t.ldc;
     floatx16 acc[MT];
-    res_load<MT>(acc, Cc, m, col < n);
+    res_load<MT, false>(acc, Cc, 0, m, col < n);
     for (int p0 = 0; p0 < k; p0 += 32) {
         const int kb = min(32, k - p0);
         __syncthreads();  // the previous panel's readers of sm are done
@@ -181,7 +184,7 @@ __global__ __launch_bounds__(WG) void k_larfb_res(const LarfbTask<float>* __rest
         __syncthreads();
         if (active) res_panel<MT>(acc, p0 >> 5, ntiles, sm, nullptr);
     }
-    if (col < n) res_store<MT>(acc, Cc, m);
+    if (col < n) res_store<MT, false>(acc, Cc, 0, m);
 }
 
 // tpmqrt chain (PENT) of one column tile (ncols <= 64) on two waves: for each link j,
@@ -204,7 +207,7 @@ __global__ __launch_bounds__(kResTpmThreads) void k_tpmqrt_res(const TpmqrtTask<
         const int rows = L.rows, ntiles = (rows + 31) >> 5;
         if (rows <= 0) continue;  // an empty link's reflectors are the identity
         float* __restrict__ Cc = L.B + (size_t)(t.col0 + cc) * L.ldb;
-        if (active) res_load<MT>(acc, Cc, rows, col < nc);
+        if (active) res_load<MT, false>(acc, Cc, 0, rows, col < nc);
         for (int p0 = 0; p0 < k; p0 += 32) {
             const int kb = min(32, k - p0);
             __syncthreads();  // the previous panel's readers of sm are done
@@ -228,7 +231,40 @@ __global__ __launch_bounds__(kResTpmThreads) void k_tpmqrt_res(const TpmqrtTask<
                 }
             }
         }
-        if (active && col < nc) res_store<MT>(acc, Cc, rows);
+        if (active && col < nc) res_store<MT, false>(acc, Cc, 0, rows);
+    }
+}
+
+// Sparsify apply of one interface block chunk (FP32), C resident in the accumulators:
+//   RIGHT (pad 1): A <- A Q_n on a chunk of mc <= 128 rows, as Q_n^T A^T (C = A^T, kn <= 32 MT
+//                  rows, read and written transposed: coalesced over A's rows);
+//   left  (pad 2): A(:, chunk) <- Q_c^T A(:, chunk) on rn <= 128 columns, kc <= 32 MT rows, and
+//                  rows 0 .. rc-1 go straight to their place in the parent block (dst).
+// Same transformation as k_spars_apply for those phases.
+template <int MT, bool RIGHT>
+__global__ __launch_bounds__(WG) void k_spars_res(const SparsApplyTask<float>* __restrict__ tasks) {
+    __shared__ ResSmem<MT> sm;
+    const SparsApplyTask<float> t = tasks[blockIdx.x];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6, li = l & 31;
+    const float* __restrict__ V = RIGHT ? t.Vn : t.Vc;
+    const float* __restrict__ T = RIGHT ? t.Tn : t.Tc;
+    const int ldv = RIGHT ? t.ldvn : t.ldvc;
+    const int m = RIGHT ? t.kn : t.kc, k = RIGHT ? t.rn : t.rc, n = RIGHT ? t.mc : t.rn;
+    const int ntiles = (m + 31) >> 5, col = 32 * w + li, cc = min(col, n - 1);
+    const bool active = 32 * w < n;
+    float* __restrict__ base = RIGHT ? t.A + cc : t.A + (size_t)cc * t.lda;
+    floatx16 acc[MT];
+    if (active) res_load<MT, RIGHT>(acc, base, t.lda, m, col < n);
+    for (int p0 = 0; p0 < k; p0 += 32) {
+        const int kb = min(32, k - p0);
+        __syncthreads();
+        res_stage<MT, true, WG>(V + p0 + (size_t)p0 * ldv, ldv, T + (size_t)p0 * NB, m - p0, kb, sm);
+        __syncthreads();
+        if (active) res_panel<MT>(acc, p0 >> 5, ntiles, sm, nullptr);
+    }
+    if (active && col < n) {
+        if (RIGHT) res_store<MT, true>(acc, base, t.lda, m);
+        else res_store<MT, false>(acc, t.dst + (size_t)col * t.ldd, 0, t.rc);
     }
 }
 
@@ -237,6 +273,18 @@ void launch_larfb_res(const LarfbTask<float>* d, int n, int mt, hipStream_t s) {
     if (mt == 4) hipLaunchKernelGGL(k_larfb_res<4>, dim3(n), dim3(WG), 0, s, d);
     else if (mt == 8) hipLaunchKernelGGL(k_larfb_res<8>, dim3(n), dim3(WG), 0, s, d);
     else hipLaunchKernelGGL(k_larfb_res<16>, dim3(n), dim3(WG), 0, s, d);
+}
+void launch_spars_res(const SparsApplyTask<float>* d, int n, int mt, bool right, hipStream_t s) {
+    if (n <= 0) return;
+    if (right) {
+        if (mt == 4) hipLaunchKernelGGL((k_spars_res<4, true>), dim3(n), dim3(WG), 0, s, d);
+        else if (mt == 8) hipLaunchKernelGGL((k_spars_res<8, true>), dim3(n), dim3(WG), 0, s, d);
+        else hipLaunchKernelGGL((k_spars_res<16, true>), dim3(n), dim3(WG), 0, s, d);
+    } else {
+        if (mt == 4) hipLaunchKernelGGL((k_spars_res<4, false>), dim3(n), dim3(WG), 0, s, d);
+        else if (mt == 8) hipLaunchKernelGGL((k_spars_res<8, false>), dim3(n), dim3(WG), 0, s, d);
+        else hipLaunchKernelGGL((k_spars_res<16, false>), dim3(n), dim3(WG), 0, s, d);
+    }
 }
 void launch_tpmqrt_res(const TpmqrtTask<float>* d, const TpmqrtLink<float>* l, int n, int mt, hipStream_t s) {
     if (n <= 0) return;

@@ -782,7 +782,14 @@ int Tree::factorize_t() {
     struct SparsBatch {
         SparsApplyTask<S>* d[4] = {nullptr, nullptr, nullptr, nullptr};
         int n[4] = {0, 0, 0, 0};
-        int launches() const { return (n[0] > 0) + (n[1] > 0) + (n[2] > 0) + (n[3] > 0); }
+        SparsApplyTask<S>* dr[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // C-resident FP32
+        int nr[6] = {0, 0, 0, 0, 0, 0};
+        int launches() const {
+            int c = 0;
+            for (int q = 0; q < 4; ++q) c += n[q] > 0;
+            for (int q = 0; q < 6; ++q) c += nr[q] > 0;
+            return c;
+        }
     };
     auto prep_spars = [&](const std::vector<SparsApplyTask<S>>& v) {
         SparsBatch b;
@@ -795,13 +802,21 @@ int Tree::factorize_t() {
             }, sm, big);
         else
             big = v;
+        std::vector<SparsApplyTask<S>> res[6], rest;
+        for (const auto& t : big) {
+            const int q = spars_res_bucket(t);
+            (q < 0 ? rest : res[q]).push_back(t);
+        }
         for (int q = 0; q < 3; ++q) { b.d[q] = upload_vec(sm[q]); b.n[q] = (int)sm[q].size(); }
-        b.d[3] = upload_vec(big);
-        b.n[3] = (int)big.size();
+        for (int q = 0; q < 6; ++q) { b.dr[q] = upload_vec(res[q]); b.nr[q] = (int)res[q].size(); }
+        b.d[3] = upload_vec(rest);
+        b.n[3] = (int)rest.size();
         return b;
     };
     auto run_spars = [&](const SparsBatch& b) {
         for (int q = 0; q < 3; ++q) launch_spars_small<S>(b.d[q], b.n[q], 16 << q, stream_);
+        if constexpr (sizeof(S) == 4)
+            for (int q = 0; q < 6; ++q) launch_spars_res(b.dr[q], b.nr[q], 4 << (q % 3), q < 3, stream_);
         launch_spars_apply<S>(b.d[3], b.n[3], stream_);
     };
 
@@ -1145,7 +1160,8 @@ int Tree::factorize_t() {
         for (int c : ints) cl[c].eliminated = true;
         // The sparsify apply's right transform (A <- A Q_n, in place on the interface blocks, row
         // chunks) needs nothing from the merge: it is queued first and runs while the host merges.
-        constexpr int kSparsRows = 256, kSparsCols = 128;
+        // (FP32 row chunks of 128: one chunk per workgroup of the C-resident kernel)
+        constexpr int kSparsRows = sizeof(S) == 4 ? 128 : 256, kSparsCols = 128;
         {
             std::vector<SparsApplyTask<S>> spa1;
             for (int so : itf) {

@@ -81,6 +81,20 @@ inline int tpmqrt_res_bucket(const TpmqrtTask<S>& t, const TpmqrtLink<S>* links)
     return rmax <= 128 ? 0 : (rmax <= 256 ? 1 : 2);
 }
 
+// Bucket of the C-resident FP32 sparsify kernels for a task: 0..2 (block rows <= 128 / 256 / 512)
+// for the right transform (pad 1: kn rows of A^T, mc <= 128), 3..5 for the left transform with
+// its scatter (pad 2 with Q_c: kc rows, rn <= 128), -1 for the workgroup kernel.
+// SPAQR_SPARS_RES=0 keeps every task on the workgroup kernel.
+template <typename S>
+inline int spars_res_bucket(const SparsApplyTask<S>& t) {
+    static const bool on = getenv("SPAQR_SPARS_RES") == nullptr || atoi(getenv("SPAQR_SPARS_RES")) != 0;
+    if (!on || sizeof(S) != 4 || t.ident) return -1;
+    auto b = [](int m) { return m <= 128 ? 0 : (m <= 256 ? 1 : 2); };
+    if (t.pad == 1 && t.Vn && t.kn <= 512 && t.mc <= 128) return b(t.kn);
+    if (t.pad == 2 && t.Vc && t.kc <= 512 && t.rn <= 128) return 3 + b(t.kc);
+    return -1;
+}
+
 // Flops of an m x n Householder QR.
 template <typename S>
 inline double flops_geqrt(double m, double n) { return 2 * m * n * n - 2 * n * n * n / 3; }
