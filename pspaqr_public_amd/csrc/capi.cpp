@@ -348,21 +348,24 @@ int spaqr_larfb_batched(int prec, int count, const spaqr_larfb_desc* d, void* st
         return run_batched(d, sizeof(*d) * count, nullptr, 0, stream, [&](void* dd, void*, hipStream_t s) {
             launch_larfb<double>((const LarfbTask<double>*)dd, count, s);
         });
-    // FP32: blocks the C-resident kernel takes (Q^T C only, m <= 512, n <= 128) go to it
-    std::vector<LarfbTask<float>> res[3], rest;
+    // FP32: blocks the C-resident kernels take (Q^T C only: m <= 512, n <= 128; C R^{-1} only:
+    // m <= 128, k2 <= 512) go to them
+    std::vector<LarfbTask<float>> res[6], rest;
     for (int i = 0; i < count; ++i) {
         LarfbTask<float> t;
         static_assert(sizeof(t) == sizeof(d[i]), "larfb desc layout");
         memcpy(&t, &d[i], sizeof(t));
-        const int q = larfb_res_bucket(t);
+        int q = larfb_res_bucket(t);
+        if (q < 0 && trsm_res_bucket(t) >= 0) q = 3 + trsm_res_bucket(t);
         (q < 0 ? rest : res[q]).push_back(t);
     }
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < 6; ++q) {
         if (res[q].empty()) continue;
         const int n = (int)res[q].size();
         const int rc = run_batched(res[q].data(), sizeof(LarfbTask<float>) * n, nullptr, 0, stream,
                                    [&](void* dd, void*, hipStream_t s) {
-                                       launch_larfb_res((const LarfbTask<float>*)dd, n, 4 << q, s);
+                                       if (q < 3) launch_larfb_res((const LarfbTask<float>*)dd, n, 4 << q, s);
+                                       else launch_trsm_res((const LarfbTask<float>*)dd, n, 4 << (q - 3), s);
                                    });
         if (rc) return rc;
     }

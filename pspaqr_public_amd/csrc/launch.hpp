@@ -16,6 +16,8 @@ template <typename S> void launch_geqrt_wave(const GeqrtTask<S>* d, int n, int m
 template <typename S> void launch_larfb(const LarfbTask<S>* d, int n, hipStream_t s);
 // FP32 Q^T C with C resident in MFMA accumulators: m <= 32 mt (mt = 4, 8, 16), n <= 128, no trsm.
 void launch_larfb_res(const LarfbTask<float>* d, int n, int mt, hipStream_t s);
+// FP32 C R^{-1} with C^T resident in MFMA accumulators: k2 <= 32 mt (mt = 4, 8, 16), m <= 128, no Q^T.
+void launch_trsm_res(const LarfbTask<float>* d, int n, int mt, hipStream_t s);
 // FP32 sparsify apply phases with the block chunk resident in MFMA accumulators (right: pad 1,
 // kn <= 32 mt, mc <= 128; left: pad 2 with Vc, kc <= 32 mt, rn <= 128).
 void launch_spars_res(const SparsApplyTask<float>* d, int n, int mt, bool right, hipStream_t s);

@@ -268,11 +268,100 @@ __global__ __launch_bounds__(WG) void k_spars_res(const SparsApplyTask<float>* _
     }
 }
 
+// C <- C R^{-1} (trsm 'R','U','N','N'; R k2 x k2 upper, ldr) for a chunk of m <= 128 rows of C
+// (FP32), as the lower solve R^T X^T = C^T with C^T resident: wave w holds C rows 32 w .. as
+// accumulator columns (read and written transposed, coalesced over C's rows).  Per 32-row
+// block J: wave 0 inverts R_JJ (a column of the inverse per lane, in LDS), X_J = R_JJ^{-T} C_J
+// and C_K -= R_JK^T X_J for the later blocks, all on MFMA with B from the resident tiles.
+template <int MT>
+struct TrsmSmem {
+    static constexpr int kStride = 32 * MT + 4;  // strip row stride (16-byte rows)
+    float Rs[32 * kStride];                      // R(J0 + q, J0 + c) at q * kStride + c, zero below the diagonal
+    float Li[32 * 33];                           // R_JJ^{-1}(q, i) at q * 33 + i
+};
+
+template <int MT>
+__global__ __launch_bounds__(WG) void k_trsm_res(const LarfbTask<float>* __restrict__ tasks) {
+    __shared__ TrsmSmem<MT> sm;
+    constexpr int ST = TrsmSmem<MT>::kStride;
+    const LarfbTask<float> t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6, li = l & 31;
+    const int m = t.m, k2 = t.k2, nJ = (k2 + 31) >> 5;
+    const int row = 32 * w + li;
+    const bool active = 32 * w < m;
+    float* __restrict__ base = t.C + min(row, m - 1);
+    floatx16 acc[MT];
+    if (active) res_load<MT, true>(acc, base, t.ldc, k2, row < m);
+    for (int J = 0; J < nJ; ++J) {
+        const int J0 = 32 * J, wid = k2 - J0, kb = min(32, wid);
+        __syncthreads();  // the previous block's readers of sm are done
+        // strip R(J0 .. J0 + 31, J0 .. k2 - 1): q = tid % 32, columns tid / 32 + 8 i
+        {
+            const int q = tid & 31, c0 = tid >> 5;
+            const float* __restrict__ Rq = t.R + J0 + min(q, kb - 1) + (size_t)J0 * t.ldr;
+#pragma unroll 4
+            for (int c = c0; c < 32 * MT; c += 8) {
+                const float v = Rq[(size_t)min(c, wid - 1) * t.ldr];
+                sm.Rs[q * ST + c] = (q < kb && c < wid && c >= q) ? v : 0.f;
+            }
+        }
+        __syncthreads();
+        if (tid < 32) {  // column c of R_JJ^{-1} by back substitution (identity past kb)
+            const int c = tid;
+            float s[32];
+#pragma unroll
+            for (int j = 0; j < 32; ++j) s[j] = 0.f;
+#pragma unroll
+            for (int i = 31; i >= 0; --i) {
+                const float d = (i < kb) ? sm.Rs[i * ST + i] : 1.f;
+                const float x = ((i == c ? 1.f : 0.f) - s[i]) / d;
+#pragma unroll
+                for (int j = 0; j < 32; ++j)
+                    if (j < i) s[j] += sm.Rs[j * ST + i] * x;
+                sm.Li[i * 33 + c] = x;
+                __builtin_amdgcn_sched_barrier(0);  // this step's R loads only (else all 496 are hoisted)
+            }
+        }
+        __syncthreads();
+        if (active) {
+            floatx16 X;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) X[r] = 0.f;
+#pragma unroll
+            for (int JJ = 0; JJ < MT; ++JJ) {
+                if (JJ == J) {  // tile J by a static index: X(i, .) = sum_q R_JJ^{-1}(q, i) C_J(q, .)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        X = __builtin_amdgcn_mfma_f32_32x32x2f32(sm.Li[mfma_row(l, r) * 33 + li], acc[JJ][r], X, 0, 0, 0);
+                    acc[JJ] = X;
+                }
+            }
+#pragma unroll
+            for (int K = 0; K < MT; ++K) {
+                if (K > J && K < nJ) {  // C_K(i, .) -= sum_q R(J0 + q, K0 + i) X(q, .)
+                    const float* rk = sm.Rs + 32 * (K - J);
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        acc[K] = __builtin_amdgcn_mfma_f32_32x32x2f32(-rk[mfma_row(l, r) * ST + li], X[r], acc[K], 0, 0, 0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+    if (active && row < m) res_store<MT, true>(acc, base, t.ldc, k2);
+}
+
 void launch_larfb_res(const LarfbTask<float>* d, int n, int mt, hipStream_t s) {
     if (n <= 0) return;
     if (mt == 4) hipLaunchKernelGGL(k_larfb_res<4>, dim3(n), dim3(WG), 0, s, d);
     else if (mt == 8) hipLaunchKernelGGL(k_larfb_res<8>, dim3(n), dim3(WG), 0, s, d);
     else hipLaunchKernelGGL(k_larfb_res<16>, dim3(n), dim3(WG), 0, s, d);
+}
+void launch_trsm_res(const LarfbTask<float>* d, int n, int mt, hipStream_t s) {
+    if (n <= 0) return;
+    if (mt == 4) hipLaunchKernelGGL(k_trsm_res<4>, dim3(n), dim3(WG), 0, s, d);
+    else if (mt == 8) hipLaunchKernelGGL(k_trsm_res<8>, dim3(n), dim3(WG), 0, s, d);
+    else hipLaunchKernelGGL(k_trsm_res<16>, dim3(n), dim3(WG), 0, s, d);
 }
 void launch_spars_res(const SparsApplyTask<float>* d, int n, int mt, bool right, hipStream_t s) {
     if (n <= 0) return;

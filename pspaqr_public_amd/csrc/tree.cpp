@@ -721,7 +721,7 @@ int Tree::factorize_t() {
     struct LarfbBatch {
         LarfbTask<S>* d[4] = {nullptr, nullptr, nullptr, nullptr};  // buckets 16 / 32 / 64, workgroup
         int n[4] = {0, 0, 0, 0};
-        LarfbTask<S>* dr[3] = {nullptr, nullptr, nullptr};  // C-resident FP32 buckets, m <= 128 / 256 / 512
+        LarfbTask<S>* dr[3] = {nullptr, nullptr, nullptr};  // C-resident FP32 buckets (larfb: m, trsm: k2 <= 128 / 256 / 512)
         int nr[3] = {0, 0, 0};
         bool trsm = false;
         int launches() const {
@@ -741,7 +741,7 @@ int Tree::factorize_t() {
             big = v;
         std::vector<LarfbTask<S>> res[3], rest;
         for (const auto& t : big) {
-            const int q = larfb_res_bucket(t);
+            const int q = trsm ? trsm_res_bucket(t) : larfb_res_bucket(t);
             (q < 0 ? rest : res[q]).push_back(t);
         }
         for (int q = 0; q < 3; ++q) { b.d[q] = upload_vec(sm[q]); b.n[q] = (int)sm[q].size(); }
@@ -776,7 +776,10 @@ int Tree::factorize_t() {
     auto run_larfb = [&](const LarfbBatch& b) {
         for (int q = 0; q < 3; ++q) launch_larfb_small<S>(b.d[q], b.n[q], 16 << q, b.trsm, stream_);
         if constexpr (sizeof(S) == 4)
-            for (int q = 0; q < 3; ++q) launch_larfb_res(b.dr[q], b.nr[q], 4 << q, stream_);
+            for (int q = 0; q < 3; ++q) {
+                if (b.trsm) launch_trsm_res(b.dr[q], b.nr[q], 4 << q, stream_);
+                else launch_larfb_res(b.dr[q], b.nr[q], 4 << q, stream_);
+            }
         launch_larfb<S>(b.d[3], b.n[3], stream_);
     };
     struct SparsBatch {

@@ -35,7 +35,7 @@ constexpr int kBigFront = 160;
 // to run after the first.
 template <typename S>
 void split_larfb(const std::vector<LarfbTask<S>>& in, std::vector<LarfbTask<S>>& cols_out,
-                 std::vector<LarfbTask<S>>& trsm_out, int col_chunk = 128, int row_chunk = 256) {
+                 std::vector<LarfbTask<S>>& trsm_out, int col_chunk = 128, int row_chunk = sizeof(S) == 4 ? 128 : 256) {
     for (const auto& t : in) {
         if (t.V)
             for (int c0 = 0; c0 < t.n; c0 += col_chunk) {
@@ -93,6 +93,15 @@ inline int spars_res_bucket(const SparsApplyTask<S>& t) {
     if (t.pad == 1 && t.Vn && t.kn <= 512 && t.mc <= 128) return b(t.kn);
     if (t.pad == 2 && t.Vc && t.kc <= 512 && t.rn <= 128) return 3 + b(t.kc);
     return -1;
+}
+
+// Bucket of the C-resident FP32 trsm kernel for a C R^{-1}-only task (0: k2 <= 128, 1: <= 256,
+// 2: <= 512; rows m <= 128), or -1 for the workgroup kernel.  SPAQR_TRSM_RES=0 turns it off.
+template <typename S>
+inline int trsm_res_bucket(const LarfbTask<S>& t) {
+    static const bool on = getenv("SPAQR_TRSM_RES") == nullptr || atoi(getenv("SPAQR_TRSM_RES")) != 0;
+    if (!on || sizeof(S) != 4 || t.V || !t.R || t.m > 128 || t.k2 > 512) return -1;
+    return t.k2 <= 128 ? 0 : (t.k2 <= 256 ? 1 : 2);
 }
 
 // Flops of an m x n Householder QR.

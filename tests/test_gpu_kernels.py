@@ -77,8 +77,10 @@ def test_larfb_and_trsm(prec):
         descs.append(G.LarfbDesc(Vb.ptr, Tb.ptr, Cb.ptr, None, m, n, k, m, m, 0, 0, 0))
         refs.append(unF(c, m, n))
     # trsm only tasks: C (m x k2) <- C R^{-1}
-    for (m, k2) in [(40, 40), (17, 70), (100, 33)]:
-        R = np.triu(rng.standard_normal((k2, k2))) + 4 * np.eye(k2)
+    # FP32 chunks of m <= 128 rows with k2 <= 128 / 256 / 512 run on the C-resident trsm
+    for (m, k2) in [(40, 40), (17, 70), (100, 33), (128, 200), (77, 481), (130, 40)]:
+        # well conditioned at every size (condition number about 2), like a scaling QR's R
+        R = np.triu(rng.standard_normal((k2, k2)), 1) / np.sqrt(k2) + np.diag(3 + np.abs(rng.standard_normal(k2)))
         Cm = rng.standard_normal((m, k2))
         c = F(Cm)
         olib().orc_trsm_right_upper(m, k2, F(R), c)
