@@ -350,22 +350,25 @@ int spaqr_larfb_batched(int prec, int count, const spaqr_larfb_desc* d, void* st
         });
     // FP32: blocks the C-resident kernels take (Q^T C only: m <= 512, n <= 128; C R^{-1} only:
     // m <= 128, k2 <= 512) go to them
-    std::vector<LarfbTask<float>> res[6], rest;
+    // (buckets 0..3: Q^T C kernels by rows; 4..6: C R^{-1} by k2)
+    std::vector<LarfbTask<float>> res[7], rest;
     for (int i = 0; i < count; ++i) {
         LarfbTask<float> t;
         static_assert(sizeof(t) == sizeof(d[i]), "larfb desc layout");
         memcpy(&t, &d[i], sizeof(t));
         int q = larfb_res_bucket(t);
-        if (q < 0 && trsm_res_bucket(t) >= 0) q = 3 + trsm_res_bucket(t);
+        if (q < 0 && trsm_res_bucket(t) >= 0) q = 4 + trsm_res_bucket(t);
         (q < 0 ? rest : res[q]).push_back(t);
     }
-    for (int q = 0; q < 6; ++q) {
+    for (int q = 0; q < 7; ++q) {
         if (res[q].empty()) continue;
         const int n = (int)res[q].size();
         const int rc = run_batched(res[q].data(), sizeof(LarfbTask<float>) * n, nullptr, 0, stream,
                                    [&](void* dd, void*, hipStream_t s) {
-                                       if (q < 3) launch_larfb_res((const LarfbTask<float>*)dd, n, 4 << q, s);
-                                       else launch_trsm_res((const LarfbTask<float>*)dd, n, 4 << (q - 3), s);
+                                       const LarfbTask<float>* td = (const LarfbTask<float>*)dd;
+                                       if (q < 3) launch_larfb_res(td, n, 4 << q, s);
+                                       else if (q == 3) launch_larfb_res2(td, n, s);
+                                       else launch_trsm_res(td, n, 4 << (q - 4), s);
                                    });
         if (rc) return rc;
     }
@@ -407,7 +410,7 @@ int spaqr_tpmqrt_batched(int prec, int count, const spaqr_tpmqrt_desc* d, int nl
     std::vector<TpmqrtLink<float>> hl(nlinks);
     static_assert(sizeof(TpmqrtLink<float>) == sizeof(spaqr_tpmqrt_link), "tpmqrt link layout");
     if (nlinks) memcpy(hl.data(), links, sizeof(TpmqrtLink<float>) * nlinks);
-    std::vector<TpmqrtTask<float>> res[3], rest;
+    std::vector<TpmqrtTask<float>> res[4], rest;
     for (int i = 0; i < count; ++i) {
         TpmqrtTask<float> t;
         static_assert(sizeof(t) == sizeof(d[i]), "tpmqrt desc layout");
@@ -415,17 +418,17 @@ int spaqr_tpmqrt_batched(int prec, int count, const spaqr_tpmqrt_desc* d, int nl
         const int q = tpmqrt_res_bucket(t, hl.data());
         (q < 0 ? rest : res[q]).push_back(t);
     }
-    for (int q = 0; q < 4; ++q) {
-        std::vector<TpmqrtTask<float>>& v = q < 3 ? res[q] : rest;
+    for (int q = 0; q < 5; ++q) {
+        std::vector<TpmqrtTask<float>>& v = q < 4 ? res[q] : rest;
         if (v.empty()) continue;
         const int n = (int)v.size();
         const int rc = run_batched(v.data(), sizeof(TpmqrtTask<float>) * n, links, sizeof(*links) * nlinks, stream,
                                    [&](void* dd, void* dl, hipStream_t s) {
-                                       if (q < 3)
-                                           launch_tpmqrt_res((const TpmqrtTask<float>*)dd, (const TpmqrtLink<float>*)dl, n,
-                                                             4 << q, s);
-                                       else
-                                           launch_tpmqrt<float>((const TpmqrtTask<float>*)dd, (const TpmqrtLink<float>*)dl, n, s);
+                                       const TpmqrtTask<float>* td = (const TpmqrtTask<float>*)dd;
+                                       const TpmqrtLink<float>* tl = (const TpmqrtLink<float>*)dl;
+                                       if (q < 3) launch_tpmqrt_res(td, tl, n, 4 << q, s);
+                                       else if (q == 3) launch_tpmqrt_res2(td, tl, n, s);
+                                       else launch_tpmqrt<float>(td, tl, n, s);
                                    });
         if (rc) return rc;
     }

@@ -16,6 +16,10 @@ template <typename S> void launch_geqrt_wave(const GeqrtTask<S>* d, int n, int m
 template <typename S> void launch_larfb(const LarfbTask<S>* d, int n, hipStream_t s);
 // FP32 Q^T C with C resident in MFMA accumulators: m <= 32 mt (mt = 4, 8, 16), n <= 128, no trsm.
 void launch_larfb_res(const LarfbTask<float>* d, int n, int mt, hipStream_t s);
+// FP32 resident applies for 512 < rows <= 1024 (two waves per 32-column slab): larfb with n <= 64,
+// tpmqrt chains with tiles of <= 64 columns.
+void launch_larfb_res2(const LarfbTask<float>* d, int n, hipStream_t s);
+void launch_tpmqrt_res2(const TpmqrtTask<float>* d, const TpmqrtLink<float>* l, int n, hipStream_t s);
 // FP32 C R^{-1} with C^T resident in MFMA accumulators: k2 <= 32 mt (mt = 4, 8, 16), m <= 128, no Q^T.
 void launch_trsm_res(const LarfbTask<float>* d, int n, int mt, hipStream_t s);
 // FP32 sparsify apply phases with the block chunk resident in MFMA accumulators (right: pad 1,

@@ -118,42 +118,58 @@ __device__ __forceinline__ void res_store(const floatx16 (&acc)[MT], float* __re
     }
 }
 
-// One panel on this wave's resident tiles t0 .. ntiles-1 (sm.Vs row 0 is tile t0's row 0);
-// Wtop, if given, holds the pentagonal top rows A (added to W = A + V_p^T C first) and
-// receives A - W2.
-template <int MT>
-__device__ __forceinline__ void res_panel(floatx16 (&acc)[MT], int t0, int ntiles, const ResSmem<MT>& sm,
-                                          floatx16* Wtop) {
+// One panel on this wave's resident tiles: acc[tt] holds tile tb + tt of the block (rows
+// 32 (tb + tt) ..); tiles t0 .. ntiles-1 take part, and Vs row 0 is tile t0's row 0.  Wtop, if
+// given, holds the pentagonal top rows A (added to W = A + V_p^T C first) and receives A - W2.
+// XCH: two waves hold the same columns in two row ranges; each adds the other's partial W
+// through LDS (wx_mine, wx_other: 16 x 64 floats each), so both hold the same W -- every wave
+// of the workgroup must make the call (it has a barrier); `active` waves do the arithmetic.
+template <int MT, bool XCH>
+__device__ __forceinline__ void res_panel(floatx16 (&acc)[MT], int tb, int t0, int ntiles, const float* Vs,
+                                          const float* Ts, floatx16* Wtop, bool active, float* wx_mine,
+                                          const float* wx_other) {
     const int l = threadIdx.x & 63, li = l & 31;
     // one accumulation chain: a 32x32x2 FP32 MFMA holds the SIMD for as long as its latency
     floatx16 W;
 #pragma unroll
     for (int r = 0; r < 16; ++r) W[r] = Wtop ? (*Wtop)[r] : 0.f;
+    if (active) {
 #pragma unroll
-    for (int tt = 0; tt < MT; ++tt) {
-        if (tt >= t0 && tt < ntiles) {
-            const float* vt = sm.Vs + (size_t)(32 * (tt - t0)) * 33;
+        for (int tt = 0; tt < MT; ++tt) {
+            const int g = tb + tt;
+            if (g >= t0 && g < ntiles) {
+                const float* vt = Vs + (size_t)(32 * (g - t0)) * 33;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const float a = vt[mfma_row(l, r) * 33 + li];
-                W = __builtin_amdgcn_mfma_f32_32x32x2f32(a, acc[tt][r], W, 0, 0, 0);
+                for (int r = 0; r < 16; ++r) {
+                    const float a = vt[mfma_row(l, r) * 33 + li];
+                    W = __builtin_amdgcn_mfma_f32_32x32x2f32(a, acc[tt][r], W, 0, 0, 0);
+                }
             }
+            __builtin_amdgcn_sched_barrier(0);  // one tile's operands at a time (else all are hoisted)
         }
-        __builtin_amdgcn_sched_barrier(0);  // one tile's operands at a time (else all are hoisted)
     }
+    if constexpr (XCH) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) wx_mine[r * 64 + l] = W[r];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; ++r) W[r] += wx_other[r * 64 + l];  // a + b == b + a: both waves agree
+    }
+    if (!active) return;
     floatx16 W2;
 #pragma unroll
     for (int r = 0; r < 16; ++r) W2[r] = 0.f;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        const float a = sm.Ts[mfma_row(l, r) * 33 + li];
+        const float a = Ts[mfma_row(l, r) * 33 + li];
         W2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, W[r], W2, 0, 0, 0);
     }
     if (Wtop) *Wtop -= W2;
 #pragma unroll
     for (int tt = 0; tt < MT; ++tt) {
-        if (tt >= t0 && tt < ntiles) {
-            const float* vt = sm.Vs + (size_t)(32 * (tt - t0)) * 33;
+        const int g = tb + tt;
+        if (g >= t0 && g < ntiles) {
+            const float* vt = Vs + (size_t)(32 * (g - t0)) * 33;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const float a = -vt[li * 33 + mfma_row(l, r)];
@@ -182,7 +198,7 @@ __global__ __launch_bounds__(WG) void k_larfb_res(const LarfbTask<float>* __rest
         __syncthreads();  // the previous panel's readers of sm are done
         res_stage<MT, true, WG>(t.V + p0 + (size_t)p0 * t.ldv, t.ldv, t.T + (size_t)p0 * NB, m - p0, kb, sm);
         __syncthreads();
-        if (active) res_panel<MT>(acc, p0 >> 5, ntiles, sm, nullptr);
+        res_panel<MT, false>(acc, 0, p0 >> 5, ntiles, sm.Vs, sm.Ts, nullptr, active, nullptr, nullptr);
     }
     if (col < n) res_store<MT, false>(acc, Cc, 0, m);
 }
@@ -221,7 +237,7 @@ __global__ __launch_bounds__(kResTpmThreads) void k_tpmqrt_res(const TpmqrtTask<
                     const float v = Acol[p0 + min(q, kb - 1)];
                     top[r] = (q < kb && col < nc) ? v : 0.f;
                 }
-                res_panel<MT>(acc, 0, ntiles, sm, &top);
+                res_panel<MT, false>(acc, 0, 0, ntiles, sm.Vs, sm.Ts, &top, true, nullptr, nullptr);
                 if (col < nc) {
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
@@ -260,7 +276,7 @@ __global__ __launch_bounds__(WG) void k_spars_res(const SparsApplyTask<float>* _
         __syncthreads();
         res_stage<MT, true, WG>(V + p0 + (size_t)p0 * ldv, ldv, T + (size_t)p0 * NB, m - p0, kb, sm);
         __syncthreads();
-        if (active) res_panel<MT>(acc, p0 >> 5, ntiles, sm, nullptr);
+        res_panel<MT, false>(acc, 0, p0 >> 5, ntiles, sm.Vs, sm.Ts, nullptr, active, nullptr, nullptr);
     }
     if (active && col < n) {
         if (RIGHT) res_store<MT, true>(acc, base, t.lda, m);
@@ -351,11 +367,89 @@ __global__ __launch_bounds__(WG) void k_trsm_res(const LarfbTask<float>* __restr
     if (active && row < m) res_store<MT, true>(acc, base, t.ldc, k2);
 }
 
+// Blocks of 513 .. 1024 rows: four waves, two 32-column slabs (w % 2) times two 512-row halves
+// (w / 2); the halves of a slab sum their partial W through LDS once per panel.
+struct Res2Smem {
+    ResSmem<32> v;        // V_p rows for the whole block (1024 rows)
+    float Wx[4 * 1024];   // per-wave partial W
+};
+
+// larfb (Q^T C) with 512 < m <= 1024 and n <= 64.
+__global__ __launch_bounds__(WG) void k_larfb_res2(const LarfbTask<float>* __restrict__ tasks) {
+    __shared__ Res2Smem sm;
+    const LarfbTask<float> t = tasks[blockIdx.x];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6, li = l & 31, sl = w & 1, h = w >> 1;
+    const int m = t.m, n = t.n, k = t.k, ntiles = (m + 31) >> 5;
+    const int col = 32 * sl + li;
+    const bool active = 32 * sl < n;
+    float* __restrict__ Cc = t.C + (size_t)min(col, n - 1) * t.ldc + 512 * h;
+    floatx16 acc[16];
+    if (active) res_load<16, false>(acc, Cc, 0, m - 512 * h, col < n);
+    for (int p0 = 0; p0 < k; p0 += 32) {
+        const int kb = min(32, k - p0);
+        __syncthreads();
+        res_stage<32, true, WG>(t.V + p0 + (size_t)p0 * t.ldv, t.ldv, t.T + (size_t)p0 * NB, m - p0, kb, sm.v);
+        __syncthreads();
+        res_panel<16, true>(acc, 16 * h, p0 >> 5, ntiles, sm.v.Vs, sm.v.Ts, nullptr, active, sm.Wx + w * 1024,
+                            sm.Wx + (w ^ 2) * 1024);
+    }
+    if (active && col < n) res_store<16, false>(acc, Cc, 0, m - 512 * h);
+}
+
+// tpmqrt chain of a tile (ncols <= 64) whose links have up to 1024 rows; half 0 carries Atop.
+__global__ __launch_bounds__(WG) void k_tpmqrt_res2(const TpmqrtTask<float>* __restrict__ tasks,
+                                                    const TpmqrtLink<float>* __restrict__ links) {
+    __shared__ Res2Smem sm;
+    const TpmqrtTask<float> t = tasks[blockIdx.x];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6, li = l & 31, sl = w & 1, h = w >> 1;
+    const int k = t.k, nc = t.ncols, col = 32 * sl + li, cc = min(col, nc - 1);
+    const bool active = 32 * sl < nc;
+    float* __restrict__ Acol = t.Atop + (size_t)cc * t.lda;
+    floatx16 acc[16];
+    for (int j = 0; j < t.nlinks; ++j) {
+        const TpmqrtLink<float> L = links[t.link0 + j];
+        const int rows = L.rows, ntiles = (rows + 31) >> 5;
+        if (rows <= 0) continue;  // an empty link's reflectors are the identity
+        const int rh = rows - 512 * h;  // this half's rows (<= 0: none)
+        float* __restrict__ Cc = L.B + (size_t)(t.col0 + cc) * L.ldb + 512 * h;
+        if (active && rh > 0) res_load<16, false>(acc, Cc, 0, rh, col < nc);
+        for (int p0 = 0; p0 < k; p0 += 32) {
+            const int kb = min(32, k - p0);
+            __syncthreads();
+            res_stage<32, false, WG>(L.V + (size_t)p0 * L.ldv, L.ldv, L.T + (size_t)p0 * NB, rows, kb, sm.v);
+            __syncthreads();
+            floatx16 top;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int q = mfma_row(l, r);
+                const float v = (h == 0) ? Acol[p0 + min(q, kb - 1)] : 0.f;
+                top[r] = (q < kb && col < nc) ? v : 0.f;
+            }
+            res_panel<16, true>(acc, 16 * h, 0, ntiles, sm.v.Vs, sm.v.Ts, &top, active, sm.Wx + w * 1024,
+                                sm.Wx + (w ^ 2) * 1024);
+            if (h == 0 && active && col < nc) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int q = mfma_row(l, r);
+                    if (q < kb) Acol[p0 + q] = top[r];
+                }
+            }
+        }
+        if (active && col < nc && rh > 0) res_store<16, false>(acc, Cc, 0, rh);
+    }
+}
+
 void launch_larfb_res(const LarfbTask<float>* d, int n, int mt, hipStream_t s) {
     if (n <= 0) return;
     if (mt == 4) hipLaunchKernelGGL(k_larfb_res<4>, dim3(n), dim3(WG), 0, s, d);
     else if (mt == 8) hipLaunchKernelGGL(k_larfb_res<8>, dim3(n), dim3(WG), 0, s, d);
     else hipLaunchKernelGGL(k_larfb_res<16>, dim3(n), dim3(WG), 0, s, d);
+}
+void launch_larfb_res2(const LarfbTask<float>* d, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_larfb_res2, dim3(n), dim3(WG), 0, s, d);
+}
+void launch_tpmqrt_res2(const TpmqrtTask<float>* d, const TpmqrtLink<float>* l, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_tpmqrt_res2, dim3(n), dim3(WG), 0, s, d, l);
 }
 void launch_trsm_res(const LarfbTask<float>* d, int n, int mt, hipStream_t s) {
     if (n <= 0) return;

@@ -721,11 +721,11 @@ int Tree::factorize_t() {
     struct LarfbBatch {
         LarfbTask<S>* d[4] = {nullptr, nullptr, nullptr, nullptr};  // buckets 16 / 32 / 64, workgroup
         int n[4] = {0, 0, 0, 0};
-        LarfbTask<S>* dr[3] = {nullptr, nullptr, nullptr};  // C-resident FP32 buckets (larfb: m, trsm: k2 <= 128 / 256 / 512)
-        int nr[3] = {0, 0, 0};
+        LarfbTask<S>* dr[4] = {nullptr, nullptr, nullptr, nullptr};  // C-resident FP32 (larfb: m, trsm: k2 buckets)
+        int nr[4] = {0, 0, 0, 0};
         bool trsm = false;
         int launches() const {
-            return (n[0] > 0) + (n[1] > 0) + (n[2] > 0) + (n[3] > 0) + (nr[0] > 0) + (nr[1] > 0) + (nr[2] > 0);
+            return (n[0] > 0) + (n[1] > 0) + (n[2] > 0) + (n[3] > 0) + (nr[0] > 0) + (nr[1] > 0) + (nr[2] > 0) + (nr[3] > 0);
         }
     };
     auto prep_larfb = [&](const std::vector<LarfbTask<S>>& v, bool trsm) {
@@ -739,13 +739,13 @@ int Tree::factorize_t() {
             }, sm, big);
         else
             big = v;
-        std::vector<LarfbTask<S>> res[3], rest;
+        std::vector<LarfbTask<S>> res[4], rest;
         for (const auto& t : big) {
             const int q = trsm ? trsm_res_bucket(t) : larfb_res_bucket(t);
             (q < 0 ? rest : res[q]).push_back(t);
         }
         for (int q = 0; q < 3; ++q) { b.d[q] = upload_vec(sm[q]); b.n[q] = (int)sm[q].size(); }
-        for (int q = 0; q < 3; ++q) { b.dr[q] = upload_vec(res[q]); b.nr[q] = (int)res[q].size(); }
+        for (int q = 0; q < 4; ++q) { b.dr[q] = upload_vec(res[q]); b.nr[q] = (int)res[q].size(); }
         b.d[3] = upload_vec(rest);
         b.n[3] = (int)rest.size();
         if (dbg_shapes)
@@ -780,6 +780,7 @@ int Tree::factorize_t() {
                 if (b.trsm) launch_trsm_res(b.dr[q], b.nr[q], 4 << q, stream_);
                 else launch_larfb_res(b.dr[q], b.nr[q], 4 << q, stream_);
             }
+        if constexpr (sizeof(S) == 4) launch_larfb_res2(b.dr[3], b.nr[3], stream_);
         launch_larfb<S>(b.d[3], b.n[3], stream_);
     };
     struct SparsBatch {
@@ -1024,8 +1025,8 @@ int Tree::factorize_t() {
             // per colour: one-wave buckets (k <= 64, links of 1..64 rows) and the workgroup rest
             std::vector<std::array<TpmqrtTask<S>*, 4>> d_tpm(ncolors);
             std::vector<std::array<int, 4>> n_tpm(ncolors);
-            std::vector<std::array<TpmqrtTask<S>*, 3>> d_tpmr(ncolors);  // C-resident FP32 buckets
-            std::vector<std::array<int, 3>> n_tpmr(ncolors);
+            std::vector<std::array<TpmqrtTask<S>*, 4>> d_tpmr(ncolors);  // C-resident FP32 buckets
+            std::vector<std::array<int, 4>> n_tpmr(ncolors);
             for (int q = 0; q < ncolors; ++q) {
                 std::vector<TpmqrtTask<S>> sm[3], big;
                 if (small_tasks)
@@ -1040,13 +1041,13 @@ int Tree::factorize_t() {
                     }, sm, big);
                 else
                     big = tpm[q];
-                std::vector<TpmqrtTask<S>> res[3], rest;
+                std::vector<TpmqrtTask<S>> res[4], rest;
                 for (const auto& t : big) {
                     const int b = tpmqrt_res_bucket(t, tpml.data());
                     (b < 0 ? rest : res[b]).push_back(t);
                 }
                 for (int b = 0; b < 3; ++b) { d_tpm[q][b] = upload_vec(sm[b]); n_tpm[q][b] = (int)sm[b].size(); }
-                for (int b = 0; b < 3; ++b) { d_tpmr[q][b] = upload_vec(res[b]); n_tpmr[q][b] = (int)res[b].size(); }
+                for (int b = 0; b < 4; ++b) { d_tpmr[q][b] = upload_vec(res[b]); n_tpmr[q][b] = (int)res[b].size(); }
                 d_tpm[q][3] = upload_vec(rest);
                 n_tpm[q][3] = (int)rest.size();
                 if (dbg_shapes)
@@ -1089,8 +1090,10 @@ int Tree::factorize_t() {
                 timed(P_TPMQRT, ncolors, ntpm, [&] {
                     for (int q = 0; q < ncolors; ++q) {
                         for (int b = 0; b < 3; ++b) launch_tpmqrt_wave<S>(d_tpm[q][b], d_tpml, n_tpm[q][b], 16 << b, stream_);
-                        if constexpr (sizeof(S) == 4)
+                        if constexpr (sizeof(S) == 4) {
                             for (int b = 0; b < 3; ++b) launch_tpmqrt_res(d_tpmr[q][b], d_tpml, n_tpmr[q][b], 4 << b, stream_);
+                            launch_tpmqrt_res2(d_tpmr[q][3], d_tpml, n_tpmr[q][3], stream_);
+                        }
                         launch_tpmqrt<S>(d_tpm[q][3], d_tpml, n_tpm[q][3], stream_);
                     }
                 });

@@ -37,11 +37,13 @@ template <typename S>
 void split_larfb(const std::vector<LarfbTask<S>>& in, std::vector<LarfbTask<S>>& cols_out,
                  std::vector<LarfbTask<S>>& trsm_out, int col_chunk = 128, int row_chunk = sizeof(S) == 4 ? 128 : 256) {
     for (const auto& t : in) {
+        // (FP32 blocks of more than 512 rows: 64-column chunks, the two-half resident kernel's width)
+        const int cw = (sizeof(S) == 4 && t.m > 512) ? std::min(col_chunk, 64) : col_chunk;
         if (t.V)
-            for (int c0 = 0; c0 < t.n; c0 += col_chunk) {
+            for (int c0 = 0; c0 < t.n; c0 += cw) {
                 LarfbTask<S> u = t;
                 u.C = t.C + (size_t)c0 * t.ldc;
-                u.n = std::min(col_chunk, t.n - c0);
+                u.n = std::min(cw, t.n - c0);
                 u.R = nullptr;
                 u.k2 = 0;
                 cols_out.push_back(u);
@@ -58,27 +60,28 @@ void split_larfb(const std::vector<LarfbTask<S>>& in, std::vector<LarfbTask<S>>&
     }
 }
 
-// Bucket of the C-resident FP32 larfb kernel for a task (0: m <= 128, 1: m <= 256, 2: m <= 512),
-// or -1 when the task stays on the workgroup kernel (FP64, a trsm, m > 512 or n > 128).
-// SPAQR_LARFB_RES=0 keeps every task on the workgroup kernel.
+// Bucket of the C-resident FP32 larfb kernels for a task (0: m <= 128, 1: m <= 256, 2: m <= 512,
+// all with n <= 128; 3: m <= 1024 with n <= 64), or -1 when the task stays on the workgroup
+// kernel (FP64, a trsm, larger blocks).  SPAQR_LARFB_RES=0 keeps every task on the workgroup kernel.
 template <typename S>
 inline int larfb_res_bucket(const LarfbTask<S>& t) {
     static const bool on = getenv("SPAQR_LARFB_RES") == nullptr || atoi(getenv("SPAQR_LARFB_RES")) != 0;
-    if (!on || sizeof(S) != 4 || !t.V || t.R || t.n > 128 || t.m > 512) return -1;
+    if (!on || sizeof(S) != 4 || !t.V || t.R || t.n > 128 || t.m > 1024) return -1;
+    if (t.m > 512) return t.n <= 64 ? 3 : -1;
     return t.m <= 128 ? 0 : (t.m <= 256 ? 1 : 2);
 }
 
-// Bucket of the C-resident FP32 tpmqrt kernel for a chain task (0: links of <= 128 rows,
-// 1: <= 256, 2: <= 512), or -1 when it stays on the workgroup kernel (FP64, a tile wider than
-// 64 columns or a longer link).  SPAQR_TPMQRT_RES=0 keeps every task on the workgroup kernel.
+// Bucket of the C-resident FP32 tpmqrt kernels for a chain task (0: links of <= 128 rows,
+// 1: <= 256, 2: <= 512, 3: <= 1024), or -1 when it stays on the workgroup kernel (FP64, a tile
+// wider than 64 columns or a longer link).  SPAQR_TPMQRT_RES=0 keeps every task on the workgroup kernel.
 template <typename S>
 inline int tpmqrt_res_bucket(const TpmqrtTask<S>& t, const TpmqrtLink<S>* links) {
     static const bool on = getenv("SPAQR_TPMQRT_RES") == nullptr || atoi(getenv("SPAQR_TPMQRT_RES")) != 0;
     if (!on || sizeof(S) != 4 || t.ncols > 64) return -1;
     int rmax = 0;
     for (int j = 0; j < t.nlinks; ++j) rmax = std::max(rmax, links[t.link0 + j].rows);
-    if (rmax > 512) return -1;
-    return rmax <= 128 ? 0 : (rmax <= 256 ? 1 : 2);
+    if (rmax > 1024) return -1;
+    return rmax <= 128 ? 0 : (rmax <= 256 ? 1 : (rmax <= 512 ? 2 : 3));
 }
 
 // Bucket of the C-resident FP32 sparsify kernels for a task: 0..2 (block rows <= 128 / 256 / 512)

@@ -225,20 +225,22 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
             }
             std::vector<int> again;
             if (!lb.empty()) {
-                std::vector<LarfbTask<S>> lb_c, lb_t, lb_res[3], lb_rest;
+                std::vector<LarfbTask<S>> lb_c, lb_t, lb_res[4], lb_rest;
                 split_larfb(lb, lb_c, lb_t);
                 for (const auto& u : lb_c) {
                     const int q = larfb_res_bucket(u);
                     (q < 0 ? lb_rest : lb_res[q]).push_back(u);
                 }
                 auto* d_lb = upload_vec(lb_rest);
-                LarfbTask<S>* d_lbr[3];
-                for (int q = 0; q < 3; ++q) d_lbr[q] = upload_vec(lb_res[q]);
+                LarfbTask<S>* d_lbr[4];
+                for (int q = 0; q < 4; ++q) d_lbr[q] = upload_vec(lb_res[q]);
                 auto* d_tt = upload_vec(tt);
                 timed(P_RSKETCH, 3, (long)lb.size(), [&] {
                     launch_fill<int>(d_tail, (size_t)nt, 0, stream_);
-                    if constexpr (sizeof(S) == 4)
+                    if constexpr (sizeof(S) == 4) {
                         for (int q = 0; q < 3; ++q) launch_larfb_res(d_lbr[q], (int)lb_res[q].size(), 4 << q, stream_);
+                        launch_larfb_res2(d_lbr[3], (int)lb_res[3].size(), stream_);
+                    }
                     launch_larfb<S>(d_lb, (int)lb_rest.size(), stream_);
                     launch_tail_select<S>(d_tt, (int)tt.size(), tol, stream_);
                 });

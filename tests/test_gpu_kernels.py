@@ -64,7 +64,8 @@ def test_larfb_and_trsm(prec):
     rng = np.random.default_rng(2)
     dt = np.float64 if prec == 64 else np.float32
     # FP32 blocks of m <= 512 and n <= 128 run on the C-resident kernel (m <= 128 / 256 / 512 buckets)
-    cases = [(50, 20, 50), (70, 45, 33), (8, 3, 8), (128, 65, 128), (200, 128, 150), (257, 33, 100), (500, 70, 481)]
+    cases = [(50, 20, 50), (70, 45, 33), (8, 3, 8), (128, 65, 128), (200, 128, 150), (257, 33, 100), (500, 70, 481),
+             (700, 64, 300), (1000, 40, 96)]  # the last two: 512 < m <= 1024, n <= 64 (two-half kernel)
     keep, descs, refs = [], [], []
     for (m, n, k) in cases:
         A = rng.standard_normal((m, k))
@@ -103,7 +104,7 @@ def test_tpqrt_tpmqrt_chain(prec):
     dt = np.float64 if prec == 64 else np.float32
     tol = 1e-12 if prec == 64 else 5e-4
     # FP32 chains with links of <= 128 / 256 / 512 rows run on the C-resident kernel
-    fronts = [(45, [30, 7, 64]), (8, [5, 9]), (70, [100]), (40, [300, 20]), (66, [200, 150])]
+    fronts = [(45, [30, 7, 64]), (8, [5, 9]), (70, [100]), (40, [300, 20]), (66, [200, 150]), (50, [700, 30, 1000])]
     keep = []
     tq_desc, tq_links, tm_desc, tm_links = [], [], [], []
     expect_R, expect_B, expect_top, expect_C = [], [], [], []
