@@ -1736,9 +1736,9 @@ __global__ void k_fill(S* p, size_t n, S v) {
 // itself; the threads exchange nothing but the 32 reflector coefficients of the current
 // panel, through LDS.
 
-template <typename S>
+template <typename S, int NT = WG>
 struct VecSmem {
-    S red[4][32];   // per-wave partial column sums of one panel
+    S red[NT / 64][32];  // per-wave partial column sums of one panel
     S t[32];        // panel coefficients t = T^T d (or T d)
     S xb[32];       // triangular-solve block
     S T[32 * 33];   // the panel's T block, column l at T[l * 33]
@@ -1753,22 +1753,22 @@ struct VecSmem {
 // The kernels are latency-bound, so every load is unconditional (columns past kb are
 // clamped to column kb-1 and weighted by zero): the 32 loads of a row issue together and the
 // wave waits once.
-template <typename S, bool PENT>
+template <typename S, bool PENT, int NT = WG>
 __device__ __forceinline__ void wg_vec_panel(int m, int p0, int kb, const S* __restrict__ V, int ldv, const S* __restrict__ T,
-                             S* xt, S* x, bool trans, VecSmem<S>& sm) {
+                             S* xt, S* x, bool trans, VecSmem<S, NT>& sm) {
     constexpr int VH = sizeof(S) == 8 ? 16 : 32;  // loads in flight per row (register budget)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // the panel's T block into LDS (read in phase 2; the previous panel finished with it)
     {
-        S tv[(NB * NB) / WG];
+        S tv[(NB * NB) / NT];
 #pragma unroll
-        for (int i = 0; i < (NB * NB) / WG; ++i) {
-            const int e = tid + i * WG, q = e & (NB - 1), l = min(e / NB, kb - 1);
+        for (int i = 0; i < (NB * NB) / NT; ++i) {
+            const int e = tid + i * NT, q = e & (NB - 1), l = min(e / NB, kb - 1);
             tv[i] = T[q + (size_t)l * NB];
         }
 #pragma unroll
-        for (int i = 0; i < (NB * NB) / WG; ++i) {
-            const int e = tid + i * WG, q = e & (NB - 1), l = e / NB;
+        for (int i = 0; i < (NB * NB) / NT; ++i) {
+            const int e = tid + i * NT, q = e & (NB - 1), l = e / NB;
             sm.T[l * 33 + q] = (q < kb && l < kb) ? tv[i] : S(0);
         }
     }
@@ -1781,7 +1781,7 @@ __device__ __forceinline__ void wg_vec_panel(int m, int p0, int kb, const S* __r
     for (int l = 0; l < 32; ++l) d[l] = S(0);
     const int r0 = PENT ? 0 : p0;  // first row the panel touches
     const S* __restrict__ Vp = V + (size_t)(PENT ? 0 : p0) * ldv;  // uniform base, 32-bit offsets below
-    for (int R = r0 + ((tid - r0 % WG + WG) % WG); R < m; R += WG) {
+    for (int R = r0 + ((tid - r0 % NT + NT) % NT); R < m; R += NT) {
         const S xr = x[R];
         const int i = R - r0;  // non-PENT: column l of V is zero above row l and one on it
 #pragma unroll
@@ -1800,7 +1800,11 @@ __device__ __forceinline__ void wg_vec_panel(int m, int p0, int kb, const S* __r
     // phase 2 (wave 0): t = T^T d or T d; columns >= kb of d are ignored (T is zero there)
     if (wave == 0) {
         S dl = S(0);
-        if (lane < 32) dl = sm.red[0][lane] + sm.red[1][lane] + sm.red[2][lane] + sm.red[3][lane] + xtv;
+        if (lane < 32) {
+#pragma unroll
+            for (int w = 0; w < NT / 64; ++w) dl += sm.red[w][lane];
+            dl += xtv;
+        }
         const int lc = min(lane, 31);
         S tl = S(0);
 #pragma unroll
@@ -1818,7 +1822,7 @@ __device__ __forceinline__ void wg_vec_panel(int m, int p0, int kb, const S* __r
     S t[32];
 #pragma unroll
     for (int l = 0; l < 32; ++l) t[l] = sm.t[l];
-    for (int R = r0 + ((tid - r0 % WG + WG) % WG); R < m; R += WG) {
+    for (int R = r0 + ((tid - r0 % NT + NT) % NT); R < m; R += NT) {
         const int i = R - r0;
         S s = S(0);
 #pragma unroll
@@ -1835,35 +1839,35 @@ __device__ __forceinline__ void wg_vec_panel(int m, int p0, int kb, const S* __r
 
 // x <- Q^T x (trans) or Q x for Q = H_0 H_1 ... (geqrt layout, k reflectors over m rows,
 // blocked T with NB-column panels).
-template <typename S>
+template <typename S, int NT = WG>
 __device__ __forceinline__ void wg_vec_q(int m, int k, const S* __restrict__ V, int ldv, const S* __restrict__ T, S* x, bool trans,
-                         VecSmem<S>& sm) {
+                         VecSmem<S, NT>& sm) {
     if (k <= 0) return;
     if (trans) {
         for (int p0 = 0; p0 < k; p0 += NB)
-            wg_vec_panel<S, false>(m, p0, min(NB, k - p0), V, ldv, T + (size_t)p0 * NB, nullptr, x, true, sm);
+            wg_vec_panel<S, false, NT>(m, p0, min(NB, k - p0), V, ldv, T + (size_t)p0 * NB, nullptr, x, true, sm);
     } else {
         for (int p0 = ((k - 1) / NB) * NB; p0 >= 0; p0 -= NB)
-            wg_vec_panel<S, false>(m, p0, min(NB, k - p0), V, ldv, T + (size_t)p0 * NB, nullptr, x, false, sm);
+            wg_vec_panel<S, false, NT>(m, p0, min(NB, k - p0), V, ldv, T + (size_t)p0 * NB, nullptr, x, false, sm);
     }
 }
 
-template <typename S>
-__global__ __launch_bounds__(WG) void k_vec_q(const VecQTask<S>* __restrict__ tasks) {
-    __shared__ VecSmem<S> sm;
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void k_vec_q(const VecQTask<S>* __restrict__ tasks) {
+    __shared__ VecSmem<S, NT> sm;
     const VecQTask<S> t = tasks[blockIdx.x];
-    wg_vec_q<S>(t.m, t.k, t.V, t.ldv, t.T, t.x, t.trans != 0, sm);
+    wg_vec_q<S, NT>(t.m, t.k, t.V, t.ldv, t.T, t.x, t.trans != 0, sm);
 }
 
-template <typename S>
-__global__ __launch_bounds__(WG) void k_vec_chain(const VecChainTask<S>* __restrict__ tasks,
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void k_vec_chain(const VecChainTask<S>* __restrict__ tasks,
                                                   const VecChainLink<S>* __restrict__ links) {
-    __shared__ VecSmem<S> sm;
+    __shared__ VecSmem<S, NT> sm;
     const VecChainTask<S> t = tasks[blockIdx.x];
     for (int li = 0; li < t.nlinks; ++li) {
         const VecChainLink<S> L = links[t.link0 + li];
         for (int p0 = 0; p0 < t.k; p0 += NB)
-            wg_vec_panel<S, true>(L.rows, p0, min(NB, t.k - p0), L.V + (size_t)p0 * L.ldv, L.ldv,
+            wg_vec_panel<S, true, NT>(L.rows, p0, min(NB, t.k - p0), L.V + (size_t)p0 * L.ldv, L.ldv,
                                   L.T + (size_t)p0 * NB, t.xc, L.xn, true, sm);
     }
 }
@@ -2098,8 +2102,8 @@ __global__ __launch_bounds__(WG) void k_vec_chain_pipe(const VecChainTask<S>* __
 
 // x[0:k] <- R^{-1} x[0:k], R upper triangular (ldr): 32-row blocks from the bottom; wave 0
 // solves the diagonal block with shuffles, then every thread updates its own rows above.
-template <typename S>
-__device__ __forceinline__ void wg_trsv_upper(int k, const S* __restrict__ R, int ldr, S* x, VecSmem<S>& sm) {
+template <typename S, int NT = WG>
+__device__ __forceinline__ void wg_trsv_upper(int k, const S* __restrict__ R, int ldr, S* x, VecSmem<S, NT>& sm) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int p0 = ((k - 1) / NB) * NB; p0 >= 0 && k > 0; p0 -= NB) {
         const int kb = min(NB, k - p0);
@@ -2111,7 +2115,7 @@ __device__ __forceinline__ void wg_trsv_upper(int k, const S* __restrict__ R, in
 #pragma unroll
             for (int j = 0; j < 32; ++j) row[j] = Rc[(unsigned)(p0 + lr + min(j, kb - 1) * ldr)];
         }
-        for (int r = p0 + ((tid - p0 % WG + WG) % WG); r < p0 + kb; r += WG) sm.xb[r - p0] = x[r];
+        for (int r = p0 + ((tid - p0 % NT + NT) % NT); r < p0 + kb; r += NT) sm.xb[r - p0] = x[r];
         __syncthreads();
         if (wave == 0) {
             S xi = (lane < kb) ? sm.xb[lr] : S(0);
@@ -2126,7 +2130,7 @@ __device__ __forceinline__ void wg_trsv_upper(int k, const S* __restrict__ R, in
             if (lane < 32) sm.xb[lane] = (lane < kb) ? xi : S(0);
         }
         __syncthreads();
-        for (int r = tid; r < p0 + kb; r += WG) {
+        for (int r = tid; r < p0 + kb; r += NT) {
             if (r >= p0) {
                 x[r] = sm.xb[r - p0];
             } else {
@@ -2143,30 +2147,30 @@ __device__ __forceinline__ void wg_trsv_upper(int k, const S* __restrict__ R, in
     }
 }
 
-template <typename S>
-__global__ __launch_bounds__(WG) void k_vec_spars(const VecSparsTask<S>* __restrict__ tasks, int fwd) {
-    __shared__ VecSmem<S> sm;
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void k_vec_spars(const VecSparsTask<S>* __restrict__ tasks, int fwd) {
+    __shared__ VecSmem<S, NT> sm;
     const VecSparsTask<S> t = tasks[blockIdx.x];
     if (fwd) {
-        wg_vec_q<S>(t.ldq, t.k, t.Qs, t.ldq, t.Ts, t.x, true, sm);
+        wg_vec_q<S, NT>(t.ldq, t.k, t.Qs, t.ldq, t.Ts, t.x, true, sm);
         __syncthreads();
-        wg_vec_q<S>(t.k, t.r, t.Qsp, t.k, t.Tsp, t.x, true, sm);
+        wg_vec_q<S, NT>(t.k, t.r, t.Qsp, t.k, t.Tsp, t.x, true, sm);
     } else {
-        wg_vec_q<S>(t.k, t.r, t.Qsp, t.k, t.Tsp, t.x, false, sm);
+        wg_vec_q<S, NT>(t.k, t.r, t.Qsp, t.k, t.Tsp, t.x, false, sm);
         __syncthreads();
-        wg_trsv_upper<S>(t.k, t.Qs, t.ldq, t.x, sm);
+        wg_trsv_upper<S, NT>(t.k, t.Qs, t.ldq, t.x, sm);
     }
 }
 
-template <typename S>
-__global__ __launch_bounds__(WG) void k_vec_trsv(const VecTrsvTask<S>* __restrict__ tasks,
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void k_vec_trsv(const VecTrsvTask<S>* __restrict__ tasks,
                                                  const VecGemvLink<S>* __restrict__ links) {
-    __shared__ VecSmem<S> sm;
+    __shared__ VecSmem<S, NT> sm;
     const VecTrsvTask<S> t = tasks[blockIdx.x];
     for (int li = 0; li < t.nlinks; ++li) {
         const VecGemvLink<S> L = links[t.link0 + li];
         if (L.s <= 0) continue;
-        for (int i = threadIdx.x; i < t.k; i += WG) {
+        for (int i = threadIdx.x; i < t.k; i += NT) {
             S s = S(0);
             for (int j0 = 0; j0 < L.s; j0 += 16) {
                 S a[16], y[16];
@@ -2182,7 +2186,7 @@ __global__ __launch_bounds__(WG) void k_vec_trsv(const VecTrsvTask<S>* __restric
             t.x[i] -= s;
         }
     }
-    wg_trsv_upper<S>(t.k, t.R, t.ldr, t.x, sm);
+    wg_trsv_upper<S, NT>(t.k, t.R, t.ldr, t.x, sm);
 }
 
 template <typename S>
@@ -2368,13 +2372,30 @@ void launch_fill(S* p, size_t n, S v, hipStream_t s) {
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(k_fill<S>, dim3((unsigned)blocks), dim3(WG), 0, s, p, n, v);
 }
+// Launches of at most kVecWideTasks tasks (the top levels' few, large fronts) run FP32 tasks on
+// 1024-thread workgroups: a lone task then has 16 waves of loads in flight on its CU.
+constexpr int kVecWideTasks = 64;
 template <typename S>
 void launch_vec_q(const VecQTask<S>* d, int n, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_vec_q<S>, dim3(n), dim3(WG), 0, s, d);
+    if (n <= 0) return;
+    if constexpr (sizeof(S) == 4) {
+        if (n <= kVecWideTasks) {
+            hipLaunchKernelGGL((k_vec_q<S, 1024>), dim3(n), dim3(1024), 0, s, d);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_vec_q<S, WG>), dim3(n), dim3(WG), 0, s, d);
 }
 template <typename S>
 void launch_vec_chain(const VecChainTask<S>* d, const VecChainLink<S>* l, int n, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_vec_chain<S>, dim3(n), dim3(WG), 0, s, d, l);
+    if (n <= 0) return;
+    if constexpr (sizeof(S) == 4) {
+        if (n <= kVecWideTasks) {
+            hipLaunchKernelGGL((k_vec_chain<S, 1024>), dim3(n), dim3(1024), 0, s, d, l);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_vec_chain<S, WG>), dim3(n), dim3(WG), 0, s, d, l);
 }
 template <typename S>
 void launch_vec_chain_pipe(const VecChainTask<S>* d, const VecChainLink<S>* l, int n, int nrw, hipStream_t s) {
@@ -2391,11 +2412,25 @@ void launch_vec_chain_wave(const VecChainTask<S>* d, const VecChainLink<S>* l, i
 }
 template <typename S>
 void launch_vec_spars(const VecSparsTask<S>* d, int n, int fwd, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_vec_spars<S>, dim3(n), dim3(WG), 0, s, d, fwd);
+    if (n <= 0) return;
+    if constexpr (sizeof(S) == 4) {
+        if (n <= kVecWideTasks) {
+            hipLaunchKernelGGL((k_vec_spars<S, 1024>), dim3(n), dim3(1024), 0, s, d, fwd);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_vec_spars<S, WG>), dim3(n), dim3(WG), 0, s, d, fwd);
 }
 template <typename S>
 void launch_vec_trsv(const VecTrsvTask<S>* d, const VecGemvLink<S>* l, int n, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_vec_trsv<S>, dim3(n), dim3(WG), 0, s, d, l);
+    if (n <= 0) return;
+    if constexpr (sizeof(S) == 4) {
+        if (n <= kVecWideTasks) {
+            hipLaunchKernelGGL((k_vec_trsv<S, 1024>), dim3(n), dim3(1024), 0, s, d, l);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_vec_trsv<S, WG>), dim3(n), dim3(WG), 0, s, d, l);
 }
 template <typename S>
 void launch_vec_copy(const VecCopyTask<S>* d, int n, hipStream_t s) {
