@@ -881,9 +881,61 @@ int Tree::factorize_t() {
             tk.nlinks = (int)tpql.size() - tk.link0;
             tpq.push_back(tk);
         }
+        hacc(1, ht);
+        // ---- upload and launch the elimination's panel work; the trailing-update and scaling
+        // descriptors are built while it runs
+        TpqrtLink<S>* d_tpql = nullptr;
+        {
+            std::vector<GeqrtTask<S>> geq_small, geq_big;
+            for (const auto& t : geq) (std::min(t.m, t.n) >= kBigFront ? geq_big : geq_small).push_back(t);
+            std::vector<TpqrtTask<S>> tpq_wave, tpq_small, tpq_big, tpq_coop;
+            static const bool tpqrt_coop_on = getenv("SPAQR_TPQRT_COOP") == nullptr || atoi(getenv("SPAQR_TPQRT_COOP")) != 0;
+            // small chains (k <= 64, links of <= 64 rows) go one per wave; the rest stay on the
+            // one-workgroup kernel: with the register-resident panel QR the host-driven panel
+            // loop (tpqrt_big) costs more in launch gaps than it gains
+            static const bool tpqrt_waves = getenv("SPAQR_TPQRT_WAVE") == nullptr || atoi(getenv("SPAQR_TPQRT_WAVE")) != 0;
+            for (const auto& t : tpq) {
+                bool small = tpqrt_waves && t.k <= 64;
+                for (int j = 0; j < t.nlinks && small; ++j) small = tpql[t.link0 + j].rows <= 64;
+                if (small) tpq_wave.push_back(t);
+                else if (tpqrt_coop_on && t.k > NB && t.nlinks > 1) tpq_coop.push_back(t);  // several panels to pipeline
+                else tpq_small.push_back(t);
+            }
+            const GeqrtBatch geq_b = prep_geqrt(geq_small);
+            std::vector<LarfbTask<S>> lfb_c, lfb_t;
+            split_larfb(lfb, lfb_c, lfb_t);
+            const LarfbBatch lfb_b = prep_larfb(lfb_c, false), lfb_tb = prep_larfb(lfb_t, true);
+            TpqrtTask<S>* d_tpq = upload_vec(tpq_small);
+            TpqrtTask<S>* d_tpqw = upload_vec(tpq_wave);
+            d_tpql = upload_vec(tpql);
+            if (!geq.empty()) {
+                int nl = 0;
+                timed(P_GEQRT, 0, (long)geq.size(), [&] {
+                    run_geqrt(geq_b);
+                    nl = geq_b.launches() + geqrt_big<S>(geq_big);
+                });
+                if (profiling) prof[P_GEQRT].launches += nl;
+            }
+            if (!lfb.empty())
+                timed(P_LARFB, lfb_b.launches() + lfb_tb.launches(), (long)lfb.size(), [&] {
+                    run_larfb(lfb_b);
+                    run_larfb(lfb_tb);
+                });
+            if (!tpq.empty()) {
+                int nl = 0;
+                timed(P_TPQRT, 0, (long)tpq.size(), [&] {
+                    launch_tpqrt_wave<S>(d_tpqw, d_tpql, (int)tpq_wave.size(), stream_);
+                    launch_tpqrt<S>(d_tpq, d_tpql, (int)tpq_small.size(), stream_);
+                    nl = !tpq_wave.empty() + !tpq_small.empty() + tpqrt_big<S>(tpq_big, tpql) +
+                         tpqrt_coop<S>(tpq_coop, d_tpql);
+                });
+                if (profiling) prof[P_TPQRT].launches += nl;
+            }
+            HIPCHK(hipGetLastError());
+        }
+        hacc(2, ht);
         // colour layers for the shared-block updates: interiors that share an out-neighbour get
         // different colours, and the colours run in order (the update order of the level)
-        hacc(1, ht);
         int ncolors = colour_interiors(ints);
         st.colors = ncolors;
         hacc(10, ht);
@@ -916,6 +968,60 @@ int Tree::factorize_t() {
             }
         }
         hacc(11, ht);
+        hacc(1, ht);
+        {
+            TpmqrtLink<S>* d_tpml = upload_vec(tpml);
+            // per colour: one-wave buckets (k <= 64, links of 1..64 rows) and the workgroup rest
+            std::vector<std::array<TpmqrtTask<S>*, 4>> d_tpm(ncolors);
+            std::vector<std::array<int, 4>> n_tpm(ncolors);
+            std::vector<std::array<TpmqrtTask<S>*, 4>> d_tpmr(ncolors);  // C-resident FP32 buckets
+            std::vector<std::array<int, 4>> n_tpmr(ncolors);
+            for (int q = 0; q < ncolors; ++q) {
+                std::vector<TpmqrtTask<S>> sm[3], big;
+                if (small_tasks)
+                    split_small(tpm[q], [&](const TpmqrtTask<S>& t) {
+                        if (t.k > 64 || t.ncols > 64) return -1;
+                        int rmax = 0, rmin = INT_MAX;
+                        for (int j = 0; j < t.nlinks; ++j) {
+                            rmax = std::max(rmax, tpml[t.link0 + j].rows);
+                            rmin = std::min(rmin, tpml[t.link0 + j].rows);
+                        }
+                        return (rmin >= 1 && rmax <= 64) ? rmax : -1;
+                    }, sm, big);
+                else
+                    big = tpm[q];
+                std::vector<TpmqrtTask<S>> res[4], rest;
+                for (const auto& t : big) {
+                    const int b = tpmqrt_res_bucket(t, tpml.data());
+                    (b < 0 ? rest : res[b]).push_back(t);
+                }
+                for (int b = 0; b < 3; ++b) { d_tpm[q][b] = upload_vec(sm[b]); n_tpm[q][b] = (int)sm[b].size(); }
+                for (int b = 0; b < 4; ++b) { d_tpmr[q][b] = upload_vec(res[b]); n_tpmr[q][b] = (int)res[b].size(); }
+                d_tpm[q][3] = upload_vec(rest);
+                n_tpm[q][3] = (int)rest.size();
+                if (dbg_shapes)
+                    for (const auto& t : big) {
+                        int rows = 0;
+                        for (int j = 0; j < t.nlinks; ++j) rows += tpml[t.link0 + j].rows;
+                        shape_add("tpmqrt", t.k, rows / std::max(t.nlinks, 1), t.ncols, 4.0 * t.k * t.ncols * rows);
+                    }
+            }
+            long ntpm = 0;
+            for (int q = 0; q < ncolors; ++q) ntpm += (long)tpm[q].size();
+            if (ntpm)
+                timed(P_TPMQRT, ncolors, ntpm, [&] {
+                    for (int q = 0; q < ncolors; ++q) {
+                        for (int b = 0; b < 3; ++b) launch_tpmqrt_wave<S>(d_tpm[q][b], d_tpml, n_tpm[q][b], 16 << b, stream_);
+                        if constexpr (sizeof(S) == 4) {
+                            for (int b = 0; b < 3; ++b) launch_tpmqrt_res(d_tpmr[q][b], d_tpml, n_tpmr[q][b], 4 << b, stream_);
+                            launch_tpmqrt_res2(d_tpmr[q][3], d_tpml, n_tpmr[q][3], stream_);
+                        }
+                        launch_tpmqrt<S>(d_tpm[q][3], d_tpml, n_tpm[q][3], stream_);
+                    }
+                });
+            HIPCHK(hipGetLastError());
+        }
+        hacc(2, ht);
         // ---- scaling descriptors (src/ptree.cpp:269-286, 1621-1633)
         std::vector<CopyTask<S>> scopy;
         std::vector<GeqrtTask<S>> sgeq;
@@ -995,108 +1101,15 @@ int Tree::factorize_t() {
             }
         }
         hacc(1, ht);
-        // ---- upload and launch
         {
-            std::vector<GeqrtTask<S>> geq_small, geq_big, sgeq_small, sgeq_big;
-            for (const auto& t : geq) (std::min(t.m, t.n) >= kBigFront ? geq_big : geq_small).push_back(t);
+            std::vector<GeqrtTask<S>> sgeq_small, sgeq_big;
             for (const auto& t : sgeq) (std::min(t.m, t.n) >= kBigFront ? sgeq_big : sgeq_small).push_back(t);
-            std::vector<TpqrtTask<S>> tpq_wave, tpq_small, tpq_big, tpq_coop;
-            static const bool tpqrt_coop_on = getenv("SPAQR_TPQRT_COOP") == nullptr || atoi(getenv("SPAQR_TPQRT_COOP")) != 0;
-            // small chains (k <= 64, links of <= 64 rows) go one per wave; the rest stay on the
-            // one-workgroup kernel: with the register-resident panel QR the host-driven panel
-            // loop (tpqrt_big) costs more in launch gaps than it gains
-            static const bool tpqrt_waves = getenv("SPAQR_TPQRT_WAVE") == nullptr || atoi(getenv("SPAQR_TPQRT_WAVE")) != 0;
-            for (const auto& t : tpq) {
-                bool small = tpqrt_waves && t.k <= 64;
-                for (int j = 0; j < t.nlinks && small; ++j) small = tpql[t.link0 + j].rows <= 64;
-                if (small) tpq_wave.push_back(t);
-                else if (tpqrt_coop_on && t.k > NB && t.nlinks > 1) tpq_coop.push_back(t);  // several panels to pipeline
-                else tpq_small.push_back(t);
-            }
-            const GeqrtBatch geq_b = prep_geqrt(geq_small), sgeq_b = prep_geqrt(sgeq_small);
-            std::vector<LarfbTask<S>> lfb_c, lfb_t, sapp_c, sapp_t;
-            split_larfb(lfb, lfb_c, lfb_t);
+            const GeqrtBatch sgeq_b = prep_geqrt(sgeq_small);
+            std::vector<LarfbTask<S>> sapp_c, sapp_t;
             split_larfb(sapp, sapp_c, sapp_t);
-            const LarfbBatch lfb_b = prep_larfb(lfb_c, false), lfb_tb = prep_larfb(lfb_t, true);
-            TpqrtTask<S>* d_tpq = upload_vec(tpq_small);
-            TpqrtTask<S>* d_tpqw = upload_vec(tpq_wave);
-            TpqrtLink<S>* d_tpql = upload_vec(tpql);
-            TpmqrtLink<S>* d_tpml = upload_vec(tpml);
-            // per colour: one-wave buckets (k <= 64, links of 1..64 rows) and the workgroup rest
-            std::vector<std::array<TpmqrtTask<S>*, 4>> d_tpm(ncolors);
-            std::vector<std::array<int, 4>> n_tpm(ncolors);
-            std::vector<std::array<TpmqrtTask<S>*, 4>> d_tpmr(ncolors);  // C-resident FP32 buckets
-            std::vector<std::array<int, 4>> n_tpmr(ncolors);
-            for (int q = 0; q < ncolors; ++q) {
-                std::vector<TpmqrtTask<S>> sm[3], big;
-                if (small_tasks)
-                    split_small(tpm[q], [&](const TpmqrtTask<S>& t) {
-                        if (t.k > 64 || t.ncols > 64) return -1;
-                        int rmax = 0, rmin = INT_MAX;
-                        for (int j = 0; j < t.nlinks; ++j) {
-                            rmax = std::max(rmax, tpml[t.link0 + j].rows);
-                            rmin = std::min(rmin, tpml[t.link0 + j].rows);
-                        }
-                        return (rmin >= 1 && rmax <= 64) ? rmax : -1;
-                    }, sm, big);
-                else
-                    big = tpm[q];
-                std::vector<TpmqrtTask<S>> res[4], rest;
-                for (const auto& t : big) {
-                    const int b = tpmqrt_res_bucket(t, tpml.data());
-                    (b < 0 ? rest : res[b]).push_back(t);
-                }
-                for (int b = 0; b < 3; ++b) { d_tpm[q][b] = upload_vec(sm[b]); n_tpm[q][b] = (int)sm[b].size(); }
-                for (int b = 0; b < 4; ++b) { d_tpmr[q][b] = upload_vec(res[b]); n_tpmr[q][b] = (int)res[b].size(); }
-                d_tpm[q][3] = upload_vec(rest);
-                n_tpm[q][3] = (int)rest.size();
-                if (dbg_shapes)
-                    for (const auto& t : big) {
-                        int rows = 0;
-                        for (int j = 0; j < t.nlinks; ++j) rows += tpml[t.link0 + j].rows;
-                        shape_add("tpmqrt", t.k, rows / std::max(t.nlinks, 1), t.ncols, 4.0 * t.k * t.ncols * rows);
-                    }
-            }
             CopyTask<S>* d_scopy = upload_vec(scopy);
-
             const LarfbBatch sapp_b = prep_larfb(sapp_c, false), sapp_tb = prep_larfb(sapp_t, true);
             CopyTask<S>* d_gat = upload_vec(gat);
-            long ntpm = 0;
-            for (int q = 0; q < ncolors; ++q) ntpm += (long)tpm[q].size();
-            if (!geq.empty()) {
-                int nl = 0;
-                timed(P_GEQRT, 0, (long)geq.size(), [&] {
-                    run_geqrt(geq_b);
-                    nl = geq_b.launches() + geqrt_big<S>(geq_big);
-                });
-                if (profiling) prof[P_GEQRT].launches += nl;
-            }
-            if (!lfb.empty())
-                timed(P_LARFB, lfb_b.launches() + lfb_tb.launches(), (long)lfb.size(), [&] {
-                    run_larfb(lfb_b);
-                    run_larfb(lfb_tb);
-                });
-            if (!tpq.empty()) {
-                int nl = 0;
-                timed(P_TPQRT, 0, (long)tpq.size(), [&] {
-                    launch_tpqrt_wave<S>(d_tpqw, d_tpql, (int)tpq_wave.size(), stream_);
-                    launch_tpqrt<S>(d_tpq, d_tpql, (int)tpq_small.size(), stream_);
-                    nl = !tpq_wave.empty() + !tpq_small.empty() + tpqrt_big<S>(tpq_big, tpql) +
-                         tpqrt_coop<S>(tpq_coop, d_tpql);
-                });
-                if (profiling) prof[P_TPQRT].launches += nl;
-            }
-            if (ntpm)
-                timed(P_TPMQRT, ncolors, ntpm, [&] {
-                    for (int q = 0; q < ncolors; ++q) {
-                        for (int b = 0; b < 3; ++b) launch_tpmqrt_wave<S>(d_tpm[q][b], d_tpml, n_tpm[q][b], 16 << b, stream_);
-                        if constexpr (sizeof(S) == 4) {
-                            for (int b = 0; b < 3; ++b) launch_tpmqrt_res(d_tpmr[q][b], d_tpml, n_tpmr[q][b], 4 << b, stream_);
-                            launch_tpmqrt_res2(d_tpmr[q][3], d_tpml, n_tpmr[q][3], stream_);
-                        }
-                        launch_tpmqrt<S>(d_tpm[q][3], d_tpml, n_tpm[q][3], stream_);
-                    }
-                });
             if (!sgeq.empty()) {
                 int nl = 0;
                 timed(P_SGEQRT, 0, (long)sgeq.size(), [&] {
