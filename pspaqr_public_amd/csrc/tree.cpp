@@ -689,7 +689,8 @@ int Tree::factorize_t() {
     launch_fill<int>(d_chain_err, 1, 0, stream_);
     // host-phase timers (SPAQR_DEBUG_HOST=1 prints them per factorization)
     static const bool dbg_host = getenv("SPAQR_DEBUG_HOST") != nullptr;
-    double hms[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double hms[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    hms[15] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();  // tables
     auto hclock = [] { return std::chrono::steady_clock::now(); };
     auto hacc = [&](int i, std::chrono::steady_clock::time_point& t) {
         auto n = std::chrono::steady_clock::now();
@@ -795,26 +796,51 @@ int Tree::factorize_t() {
             return c;
         }
     };
-    auto prep_spars = [&](const std::vector<SparsApplyTask<S>>& v) {
-        SparsBatch b;
-        std::vector<SparsApplyTask<S>> sm[3], big;
-        if (small_tasks)
-            split_small(v, [&](const SparsApplyTask<S>& t) {
-                if (t.ident) return t.rc <= 64 ? 16 : -1;
-                if (t.pad == 1) return (t.mc <= 64 && t.kn <= 64) ? t.kn : -1;
-                return (t.rn <= 64 && t.kc <= 64) ? t.kc : -1;
-            }, sm, big);
-        else
-            big = v;
-        std::vector<SparsApplyTask<S>> res[6], rest;
-        for (const auto& t : big) {
-            const int q = spars_res_bucket(t);
-            (q < 0 ? rest : res[q]).push_back(t);
+    // Sparsify tasks go straight from the builders' per-thread lists into their launch buckets
+    // (0-2 one-wave by size, 3-8 C-resident, 9 workgroup) and from there, each bucket once,
+    // into pinned staging and one upload.
+    auto spars_bucket = [&](const SparsApplyTask<S>& t) -> int {
+        if (small_tasks) {
+            int d = -1;
+            if (t.ident) d = t.rc <= 64 ? 16 : -1;
+            else if (t.pad == 1) d = (t.mc <= 64 && t.kn <= 64) ? t.kn : -1;
+            else d = (t.rn <= 64 && t.kc <= 64) ? t.kc : -1;
+            if (d >= 0) return d <= 16 ? 0 : (d <= 32 ? 1 : 2);
         }
-        for (int q = 0; q < 3; ++q) { b.d[q] = upload_vec(sm[q]); b.n[q] = (int)sm[q].size(); }
-        for (int q = 0; q < 6; ++q) { b.dr[q] = upload_vec(res[q]); b.nr[q] = (int)res[q].size(); }
-        b.d[3] = upload_vec(rest);
-        b.n[3] = (int)rest.size();
+        const int q = spars_res_bucket(t);
+        return q < 0 ? 9 : 3 + q;
+    };
+    // parts[i][q]: builder slot i's tasks of bucket q; one threaded pass copies every slot's
+    // buckets into place (one thread start per batch), then each bucket is one upload
+    using SparsBuckets = std::array<std::vector<SparsApplyTask<S>>, 10>;
+    auto prep_spars_parts = [&](const std::vector<SparsBuckets>& parts) {
+        const int np = (int)parts.size();
+        SparsBatch b;
+        std::array<std::vector<size_t>, 10> off;
+        std::array<SparsApplyTask<S>*, 10> hb{}, db{};
+        std::array<size_t, 10> nq{};
+        for (int q = 0; q < 10; ++q) {
+            off[q].assign(np + 1, 0);
+            for (int i = 0; i < np; ++i) off[q][i + 1] = off[q][i] + parts[i][q].size();
+            nq[q] = off[q][np];
+            if (nq[q]) {
+                const size_t bytes = nq[q] * sizeof(SparsApplyTask<S>);
+                db[q] = (SparsApplyTask<S>*)desc_arena.alloc(bytes);
+                hb[q] = (SparsApplyTask<S>*)staging.alloc(bytes);
+            }
+        }
+        parallel_ranges(np, 1, [&](int b0, int e0, int) {
+            for (int i = b0; i < e0; ++i)
+                for (int q = 0; q < 10; ++q)
+                    if (!parts[i][q].empty())
+                        memcpy(hb[q] + off[q][i], parts[i][q].data(), parts[i][q].size() * sizeof(SparsApplyTask<S>));
+        });
+        for (int q = 0; q < 10; ++q) {
+            if (nq[q]) HIPCHK(hipMemcpyAsync(db[q], hb[q], nq[q] * sizeof(SparsApplyTask<S>), hipMemcpyHostToDevice, stream_));
+            if (q < 3) { b.d[q] = db[q]; b.n[q] = (int)nq[q]; }
+            else if (q < 9) { b.dr[q - 3] = db[q]; b.nr[q - 3] = (int)nq[q]; }
+            else { b.d[3] = db[q]; b.n[3] = (int)nq[q]; }
+        }
         return b;
     };
     auto run_spars = [&](const SparsBatch& b) {
@@ -1182,34 +1208,45 @@ int Tree::factorize_t() {
         // (FP32 row chunks of 128: one chunk per workgroup of the C-resident kernel)
         constexpr int kSparsRows = sizeof(S) == 4 ? 128 : 256, kSparsCols = 128;
         {
-            std::vector<SparsApplyTask<S>> spa1;
-            for (int so : itf) {
-                const Cluster& SO = cl[so];
-                if (!SO.sparsified) continue;
-                for (int e : SO.out) {
-                    const Edge& eo = ed[e];
-                    const int no = eo.n2;
-                    if (no == so || cl[no].eliminated) continue;
-                    SparsApplyTask<S> u{};
-                    u.Vn = (const S*)SO.Qsp;
-                    u.Tn = (const S*)SO.Tsp;
-                    u.ldvn = eo.cols;
-                    u.nc = eo.cols;
-                    u.lda = eo.rows;
-                    u.kn = eo.cols;
-                    u.kc = eo.rows;
-                    u.rn = SO.cols;
-                    u.pad = 1;
-                    for (int r0 = 0; r0 < eo.rows; r0 += kSparsRows) {
-                        u.A = Ap(e) + r0;
-                        u.mc = std::min(kSparsRows, eo.rows - r0);
-                        spa1.push_back(u);
+            std::vector<SparsBuckets> spa1(host_threads());
+            const int np1 = parallel_ranges((int)itf.size(), 256, [&](int b, int e, int slot) {
+                SparsBuckets& out = spa1[slot];
+                for (int ii = b; ii < e; ++ii) {
+                    const int so = itf[ii];
+                    const Cluster& SO = cl[so];
+                    if (!SO.sparsified) continue;
+                    for (int e2 : SO.out) {
+                        const Edge& eo = ed[e2];
+                        const int no = eo.n2;
+                        if (no == so || cl[no].eliminated) continue;
+                        SparsApplyTask<S> u{};
+                        u.Vn = (const S*)SO.Qsp;
+                        u.Tn = (const S*)SO.Tsp;
+                        u.ldvn = eo.cols;
+                        u.nc = eo.cols;
+                        u.lda = eo.rows;
+                        u.kn = eo.cols;
+                        u.kc = eo.rows;
+                        u.rn = SO.cols;
+                        u.pad = 1;
+                        for (int r0 = 0; r0 < eo.rows; r0 += kSparsRows) {
+                            u.A = Ap(e2) + r0;
+                            u.mc = std::min(kSparsRows, eo.rows - r0);
+                            out[spars_bucket(u)].push_back(u);
+                        }
                     }
                 }
+            });
+            spa1.resize(np1);
+            size_t n1 = 0;
+            for (const auto& v : spa1)
+                for (const auto& w : v) n1 += w.size();
+            if (n1) {
+                const SparsBatch sb1 = prep_spars_parts(spa1);
+                timed(P_SPARS, sb1.launches(), 0, [&] { run_spars(sb1); });
             }
-            const SparsBatch sb1 = prep_spars(spa1);
-            if (!spa1.empty()) timed(P_SPARS, sb1.launches(), 0, [&] { run_spars(sb1); });
         }
+        hacc(12, ht);
         const int Lp = L + 1;
         current_bottom = Lp;
         for (int p : bottoms[Lp]) {
@@ -1225,6 +1262,7 @@ int Tree::factorize_t() {
             cl[p].rank = cs;
         }
         const size_t ne_merge = ed.size();
+        hacc(13, ht);
         {
             // Per parent: the merged neighbour and distance-2 lists (reads of the children only,
             // so the parents run in parallel; duplicates dropped through per-thread stamp arrays,
@@ -1257,6 +1295,7 @@ int Tree::factorize_t() {
                     std::sort(nbrs.begin(), nbrs.end());
                 }
             });
+            hacc(14, ht);
             for (int i = 0; i < np; ++i) {
                 const int p = bl[i];
                 cl[p].dist2 = std::move(pd2[i]);
@@ -1281,7 +1320,7 @@ int Tree::factorize_t() {
         // the scatter act on each column: column chunks, so that large blocks spread over many
         // workgroups (the right transform was queued before the merge).
         struct SparsPart {
-            std::vector<SparsApplyTask<S>> tasks;
+            SparsBuckets tasks;
             long blocks = 0;
             double flops = 0.0, pflops = 0.0, pbytes = 0.0;
         };
@@ -1316,7 +1355,7 @@ int Tree::factorize_t() {
                     ++P.blocks;
                     if (no == so) {
                         t.ident = SO.sparsified ? 1 : 0;
-                        P.tasks.push_back(t);
+                        P.tasks[spars_bucket(t)].push_back(t);
                         continue;
                     }
                     if (SO.sparsified) { t.Vn = (const S*)SO.Qsp; t.Tn = (const S*)SO.Tsp; t.ldvn = eo.cols; }
@@ -1332,26 +1371,21 @@ int Tree::factorize_t() {
                         u.dst = t.dst + (size_t)c0 * t.ldd;
                         u.rn = std::min(kSparsCols, t.rn - c0);
                         u.pad = 2;
-                        P.tasks.push_back(u);
+                        P.tasks[spars_bucket(u)].push_back(u);
                     }
                 }
             }
         });
         long nspa = 0;
-        std::vector<SparsApplyTask<S>> spa2;
-        {
-            size_t tot = 0;
-            for (int q = 0; q < nparts; ++q) tot += parts[q].tasks.size();
-            spa2.reserve(tot);
-        }
+        std::vector<SparsBuckets> spa2(nparts);
         for (int q = 0; q < nparts; ++q) {
             nspa += parts[q].blocks;
             stats.back().flops += parts[q].flops;
             prof_add(P_SPARS, parts[q].pflops, parts[q].pbytes);
-            spa2.insert(spa2.end(), parts[q].tasks.begin(), parts[q].tasks.end());
+            spa2[q] = std::move(parts[q].tasks);
         }
         hacc(8, ht);
-        const SparsBatch sb2 = prep_spars(spa2);
+        const SparsBatch sb2 = prep_spars_parts(spa2);
         hacc(9, ht);
         if (nspa) timed(P_SPARS, sb2.launches(), nspa, [&] { run_spars(sb2); });
         HIPCHK(hipGetLastError());
@@ -1371,9 +1405,11 @@ int Tree::factorize_t() {
     hacc(6, ht2);
     if (dbg_host)
         fprintf(stderr,
-                "[host] fillin %.1f  descr %.1f (colour %.1f tpmqrt %.1f)  launch+ranks %.1f  merge %.1f (+blocks %.1f)  spars "
-                "build %.1f split+upload %.1f launch %.1f  tail-sync %.1f  solve-sched %.1f ms\n",
-                hms[0], hms[1], hms[10], hms[11], hms[2], hms[3], hms[7], hms[8], hms[9], hms[4], hms[5], hms[6]);
+                "[host] tables %.1f  fillin %.1f  descr %.1f (colour %.1f tpmqrt %.1f)  launch+ranks %.1f  merge: spa1 %.1f "
+                "parents %.1f nbrs %.1f edges %.1f (+blocks %.1f)  spars build %.1f split+upload %.1f launch %.1f  tail-sync "
+                "%.1f  solve-sched %.1f ms\n",
+                hms[15], hms[0], hms[1], hms[10], hms[11], hms[2], hms[12], hms[13], hms[14], hms[3], hms[7], hms[8], hms[9],
+                hms[4], hms[5], hms[6]);
     if (dbg_shapes) {
         double tot = 0;
         for (auto& e : shapes) tot += e.second[1];

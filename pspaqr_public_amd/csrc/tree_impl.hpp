@@ -3,6 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <cstdlib>
 #include <exception>
 #include <thread>
@@ -123,6 +126,74 @@ inline int host_threads() {
     return n;
 }
 
+// Persistent workers for parallel_ranges: a planner level makes several short parallel passes,
+// and starting threads for each cost more than the passes.  run(T, job) runs job(t) for t in
+// [1, T) on the workers and job(0) on the caller, and returns when all have finished; callers
+// are serialized.
+class HostPool {
+  public:
+    static HostPool& get() {
+        static HostPool pool(host_threads() - 1);
+        return pool;
+    }
+    void run(int T, const std::function<void(int)>& job) {
+        std::lock_guard<std::mutex> serial(call_);
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            job_ = &job;
+            T_ = T;
+            pending_ = T - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        job(0);
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+        job_ = nullptr;
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+            ++gen_;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+
+  private:
+    explicit HostPool(int workers) {
+        for (int w = 1; w <= workers; ++w) th_.emplace_back([this, w] { loop(w); });
+    }
+    void loop(int w) {
+        unsigned long long seen = 0;
+        for (;;) {
+            const std::function<void(int)>* job;
+            int T;
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+                if (stop_) return;
+                job = job_;
+                T = T_;
+            }
+            if (w < T) {
+                (*job)(w);
+                std::lock_guard<std::mutex> lk(m_);
+                if (--pending_ == 0) done_.notify_one();
+            }
+        }
+    }
+    std::mutex call_, m_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)>* job_ = nullptr;
+    int T_ = 0, pending_ = 0;
+    unsigned long long gen_ = 0;
+    bool stop_ = false;
+    std::vector<std::thread> th_;
+};
+
 // Runs f(begin, end, slot) over [0, n) cut into contiguous ranges (at least min_per items
 // each) on up to host_threads() threads.  Slot t gets the t-th range, so per-slot results
 // concatenated in slot order reproduce the sequential order.  Returns the number of slots;
@@ -135,18 +206,14 @@ int parallel_ranges(int n, int min_per, F&& f) {
         return 1;
     }
     std::vector<std::exception_ptr> err(T);
-    auto run = [&](int t) {
+    const std::function<void(int)> run = [&](int t) {
         try {
             f((int)((long long)n * t / T), (int)((long long)n * (t + 1) / T), t);
         } catch (...) {
             err[t] = std::current_exception();
         }
     };
-    std::vector<std::thread> th;
-    th.reserve(T - 1);
-    for (int t = 1; t < T; ++t) th.emplace_back(run, t);
-    run(0);
-    for (auto& x : th) x.join();
+    HostPool::get().run(T, run);
     for (auto& e : err)
         if (e) std::rethrow_exception(e);
     return T;
