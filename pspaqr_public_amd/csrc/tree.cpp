@@ -689,7 +689,7 @@ int Tree::factorize_t() {
     launch_fill<int>(d_chain_err, 1, 0, stream_);
     // host-phase timers (SPAQR_DEBUG_HOST=1 prints them per factorization)
     static const bool dbg_host = getenv("SPAQR_DEBUG_HOST") != nullptr;
-    double hms[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double hms[24] = {0};
     hms[15] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();  // tables
     auto hclock = [] { return std::chrono::steady_clock::now(); };
     auto hacc = [&](int i, std::chrono::steady_clock::time_point& t) {
@@ -729,28 +729,55 @@ int Tree::factorize_t() {
             return (n[0] > 0) + (n[1] > 0) + (n[2] > 0) + (n[3] > 0) + (nr[0] > 0) + (nr[1] > 0) + (nr[2] > 0) + (nr[3] > 0);
         }
     };
-    auto prep_larfb = [&](const std::vector<LarfbTask<S>>& v, bool trsm) {
+    // larfb chunks go straight into launch buckets: 0-2 one-wave by size, 3-6 C-resident, 7
+    // workgroup; Q^T C and C R^{-1} chunks into separate bucket sets (two launches)
+    using LarfbBuckets = std::array<std::vector<LarfbTask<S>>, 8>;
+    auto larfb_bucket = [&](const LarfbTask<S>& t, bool trsm) -> int {
+        if (small_tasks) {
+            const int d = trsm ? ((t.m <= 64 && t.k2 <= 64) ? t.k2 : -1) : ((t.m <= 64 && t.n <= 64) ? t.m : -1);
+            if (d >= 0) return d <= 16 ? 0 : (d <= 32 ? 1 : 2);
+        }
+        const int q = trsm ? trsm_res_bucket(t) : larfb_res_bucket(t);
+        return q < 0 ? 7 : 3 + q;
+    };
+    // the chunks of one task, as split_larfb cuts them, filed by bucket
+    auto file_larfb = [&](const LarfbTask<S>& t, LarfbBuckets& cols, LarfbBuckets& tr) {
+        if (t.V) {
+            const int cw = (sizeof(S) == 4 && t.m > 512) ? 64 : 128;
+            for (int c0 = 0; c0 < t.n; c0 += cw) {
+                LarfbTask<S> u = t;
+                u.C = t.C + (size_t)c0 * t.ldc;
+                u.n = std::min(cw, t.n - c0);
+                u.R = nullptr;
+                u.k2 = 0;
+                cols[larfb_bucket(u, false)].push_back(u);
+            }
+        }
+        if (t.R) {
+            const int rc = sizeof(S) == 4 ? 128 : 256;
+            for (int r0 = 0; r0 < t.m; r0 += rc) {
+                LarfbTask<S> u = t;
+                u.V = nullptr;
+                u.T = nullptr;
+                u.C = t.C + r0;
+                u.m = std::min(rc, t.m - r0);
+                tr[larfb_bucket(u, true)].push_back(u);
+            }
+        }
+    };
+    auto prep_larfb_parts = [&](const std::vector<LarfbBuckets>& parts, bool trsm) {
+        const auto u = upload_buckets(parts);
         LarfbBatch b;
         b.trsm = trsm;
-        std::vector<LarfbTask<S>> sm[3], big;
-        if (small_tasks)
-            split_small(v, [&](const LarfbTask<S>& t) {
-                if (trsm) return (t.m <= 64 && t.k2 <= 64) ? t.k2 : -1;
-                return (t.m <= 64 && t.n <= 64) ? t.m : -1;
-            }, sm, big);
-        else
-            big = v;
-        std::vector<LarfbTask<S>> res[4], rest;
-        for (const auto& t : big) {
-            const int q = trsm ? trsm_res_bucket(t) : larfb_res_bucket(t);
-            (q < 0 ? rest : res[q]).push_back(t);
-        }
-        for (int q = 0; q < 3; ++q) { b.d[q] = upload_vec(sm[q]); b.n[q] = (int)sm[q].size(); }
-        for (int q = 0; q < 4; ++q) { b.dr[q] = upload_vec(res[q]); b.nr[q] = (int)res[q].size(); }
-        b.d[3] = upload_vec(rest);
-        b.n[3] = (int)rest.size();
+        for (int q = 0; q < 3; ++q) { b.d[q] = u[q].first; b.n[q] = u[q].second; }
+        for (int q = 0; q < 4; ++q) { b.dr[q] = u[3 + q].first; b.nr[q] = u[3 + q].second; }
+        b.d[3] = u[7].first;
+        b.n[3] = u[7].second;
         if (dbg_shapes)
-            for (const auto& t : big) shape_add(trsm ? "larfb+t" : "larfb", t.V ? t.k : t.k2, t.m, t.n, 4.0 * t.m * t.n * t.k);
+            for (const auto& pb : parts)
+                for (int q = 3; q < 8; ++q)
+                    for (const auto& t : pb[q])
+                        shape_add(trsm ? "larfb+t" : "larfb", t.V ? t.k : t.k2, t.m, t.n, 4.0 * t.m * t.n * t.k);
         return b;
     };
     struct GeqrtBatch {
@@ -814,33 +841,12 @@ int Tree::factorize_t() {
     // buckets into place (one thread start per batch), then each bucket is one upload
     using SparsBuckets = std::array<std::vector<SparsApplyTask<S>>, 10>;
     auto prep_spars_parts = [&](const std::vector<SparsBuckets>& parts) {
-        const int np = (int)parts.size();
+        const auto u = upload_buckets(parts);
         SparsBatch b;
-        std::array<std::vector<size_t>, 10> off;
-        std::array<SparsApplyTask<S>*, 10> hb{}, db{};
-        std::array<size_t, 10> nq{};
-        for (int q = 0; q < 10; ++q) {
-            off[q].assign(np + 1, 0);
-            for (int i = 0; i < np; ++i) off[q][i + 1] = off[q][i] + parts[i][q].size();
-            nq[q] = off[q][np];
-            if (nq[q]) {
-                const size_t bytes = nq[q] * sizeof(SparsApplyTask<S>);
-                db[q] = (SparsApplyTask<S>*)desc_arena.alloc(bytes);
-                hb[q] = (SparsApplyTask<S>*)staging.alloc(bytes);
-            }
-        }
-        parallel_ranges(np, 1, [&](int b0, int e0, int) {
-            for (int i = b0; i < e0; ++i)
-                for (int q = 0; q < 10; ++q)
-                    if (!parts[i][q].empty())
-                        memcpy(hb[q] + off[q][i], parts[i][q].data(), parts[i][q].size() * sizeof(SparsApplyTask<S>));
-        });
-        for (int q = 0; q < 10; ++q) {
-            if (nq[q]) HIPCHK(hipMemcpyAsync(db[q], hb[q], nq[q] * sizeof(SparsApplyTask<S>), hipMemcpyHostToDevice, stream_));
-            if (q < 3) { b.d[q] = db[q]; b.n[q] = (int)nq[q]; }
-            else if (q < 9) { b.dr[q - 3] = db[q]; b.nr[q - 3] = (int)nq[q]; }
-            else { b.d[3] = db[q]; b.n[3] = (int)nq[q]; }
-        }
+        for (int q = 0; q < 3; ++q) { b.d[q] = u[q].first; b.n[q] = u[q].second; }
+        for (int q = 0; q < 6; ++q) { b.dr[q] = u[3 + q].first; b.nr[q] = u[3 + q].second; }
+        b.d[3] = u[9].first;
+        b.n[3] = u[9].second;
         return b;
     };
     auto run_spars = [&](const SparsBatch& b) {
@@ -928,9 +934,9 @@ int Tree::factorize_t() {
                 else tpq_small.push_back(t);
             }
             const GeqrtBatch geq_b = prep_geqrt(geq_small);
-            std::vector<LarfbTask<S>> lfb_c, lfb_t;
-            split_larfb(lfb, lfb_c, lfb_t);
-            const LarfbBatch lfb_b = prep_larfb(lfb_c, false), lfb_tb = prep_larfb(lfb_t, true);
+            std::vector<LarfbBuckets> lfb_c(1), lfb_t(1);
+            for (const auto& t : lfb) file_larfb(t, lfb_c[0], lfb_t[0]);
+            const LarfbBatch lfb_b = prep_larfb_parts(lfb_c, false), lfb_tb = prep_larfb_parts(lfb_t, true);
             TpqrtTask<S>* d_tpq = upload_vec(tpq_small);
             TpqrtTask<S>* d_tpqw = upload_vec(tpq_wave);
             d_tpql = upload_vec(tpql);
@@ -1051,7 +1057,6 @@ int Tree::factorize_t() {
         // ---- scaling descriptors (src/ptree.cpp:269-286, 1621-1633)
         std::vector<CopyTask<S>> scopy;
         std::vector<GeqrtTask<S>> sgeq;
-        std::vector<LarfbTask<S>> sapp;
         for (int c : itf) {
             Cluster& C = cl[c];
             if (!want_sparsify(C)) continue;
@@ -1063,26 +1068,53 @@ int Tree::factorize_t() {
             st.flops += flops_geqrt<S>(es.rows, es.cols);
             prof_add(P_SGEQRT, flops_geqrt<S>(es.rows, es.cols), SZ * (4.0 * es.rows * es.cols + NB * es.cols));
         }
-        for (int c : itf)
-            for (int e : cl[c].sout) {
-                Edge& ee = E(e);
-                const Cluster& N2 = cl[ee.n2];
-                const Cluster& N1 = cl[c];
-                const bool w2 = want_sparsify(N2), w1 = want_sparsify(N1);
-                if (!w2 && !w1) continue;
-                const int ldq2 = w2 ? E(N2.self_edge).rows : 0;
-                const int ldq1 = w1 ? E(N1.self_edge).rows : 0;
-                sapp.push_back({w2 ? (const S*)N2.Qs : nullptr, w2 ? (const S*)N2.Ts : nullptr, Ap(e),
-                                w1 ? (const S*)N1.Qs : nullptr, ee.rows, ee.cols, N2.cols, ldq2, ee.rows, N1.cols, ldq1, 0});
-                if (w2) st.flops += 4.0 * ee.rows * ee.cols * N2.cols - 2.0 * ee.cols * N2.cols * N2.cols;
-                if (w1) st.flops += (double)ee.rows * ee.cols * ee.cols;
-                if (w2)
-                    prof_add(P_SAPPLY, 4.0 * ee.rows * ee.cols * N2.cols - 2.0 * ee.cols * N2.cols * N2.cols,
-                             SZ * ((double)ee.rows * N2.cols + NB * N2.cols + 2.0 * ee.rows * ee.cols));
-                if (w1)
-                    prof_add(P_SAPPLY, (double)ee.rows * ee.cols * ee.cols,
-                             SZ * (0.5 * ee.cols * ee.cols + 2.0 * ee.rows * ee.cols));
+        hacc(18, ht);
+        // the scaling applies of every interface block, built over the interfaces in parallel
+        struct SappPart {
+            LarfbBuckets c, t;
+            long n = 0;
+            double flops = 0.0, pflops = 0.0, pbytes = 0.0;
+        };
+        std::vector<SappPart> sparts(host_threads());
+        const int nsp = parallel_ranges((int)itf.size(), 64, [&](int b, int e, int slot) {
+            SappPart& P = sparts[slot];
+            for (int ii = b; ii < e; ++ii) {
+                const int c = itf[ii];
+                for (int e2 : cl[c].sout) {
+                    const Edge& ee = ed[e2];
+                    const Cluster& N2 = cl[ee.n2];
+                    const Cluster& N1 = cl[c];
+                    const bool w2 = want_sparsify(N2), w1 = want_sparsify(N1);
+                    if (!w2 && !w1) continue;
+                    const int ldq2 = w2 ? ed[N2.self_edge].rows : 0;
+                    const int ldq1 = w1 ? ed[N1.self_edge].rows : 0;
+                    file_larfb({w2 ? (const S*)N2.Qs : nullptr, w2 ? (const S*)N2.Ts : nullptr, Ap(e2),
+                                w1 ? (const S*)N1.Qs : nullptr, ee.rows, ee.cols, N2.cols, ldq2, ee.rows, N1.cols, ldq1, 0},
+                               P.c, P.t);
+                    ++P.n;
+                    if (w2) {
+                        const double f = 4.0 * ee.rows * ee.cols * N2.cols - 2.0 * ee.cols * N2.cols * N2.cols;
+                        P.flops += f;
+                        P.pflops += f;
+                        P.pbytes += SZ * ((double)ee.rows * N2.cols + NB * N2.cols + 2.0 * ee.rows * ee.cols);
+                    }
+                    if (w1) {
+                        const double f = (double)ee.rows * ee.cols * ee.cols;
+                        P.flops += f;
+                        P.pflops += f;
+                        P.pbytes += SZ * (0.5 * ee.cols * ee.cols + 2.0 * ee.rows * ee.cols);
+                    }
+                }
             }
+        });
+        sparts.resize(nsp);
+        long nsapp = 0;
+        for (const auto& P : sparts) {
+            nsapp += P.n;
+            st.flops += P.flops;
+            prof_add(P_SAPPLY, P.pflops, P.pbytes);
+        }
+        hacc(19, ht);
         // ---- RRQR descriptors (src/ptree.cpp:373-439)
         std::vector<CopyTask<S>> gat;
         std::vector<int> gat_begin;  // first gather copy of each RRQR task
@@ -1126,16 +1158,20 @@ int Tree::factorize_t() {
                 gq_cluster.push_back(who[q]);
             }
         }
-        hacc(1, ht);
+        hacc(16, ht);
         {
             std::vector<GeqrtTask<S>> sgeq_small, sgeq_big;
             for (const auto& t : sgeq) (std::min(t.m, t.n) >= kBigFront ? sgeq_big : sgeq_small).push_back(t);
             const GeqrtBatch sgeq_b = prep_geqrt(sgeq_small);
-            std::vector<LarfbTask<S>> sapp_c, sapp_t;
-            split_larfb(sapp, sapp_c, sapp_t);
+            std::vector<LarfbBuckets> sapp_c(nsp), sapp_t(nsp);
+            for (int i = 0; i < nsp; ++i) {
+                sapp_c[i] = std::move(sparts[i].c);
+                sapp_t[i] = std::move(sparts[i].t);
+            }
             CopyTask<S>* d_scopy = upload_vec(scopy);
-            const LarfbBatch sapp_b = prep_larfb(sapp_c, false), sapp_tb = prep_larfb(sapp_t, true);
+            const LarfbBatch sapp_b = prep_larfb_parts(sapp_c, false), sapp_tb = prep_larfb_parts(sapp_t, true);
             CopyTask<S>* d_gat = upload_vec(gat);
+            hacc(17, ht);
             if (!sgeq.empty()) {
                 int nl = 0;
                 timed(P_SGEQRT, 0, (long)sgeq.size(), [&] {
@@ -1145,8 +1181,8 @@ int Tree::factorize_t() {
                 });
                 if (profiling) prof[P_SGEQRT].launches += nl;
             }
-            if (!sapp.empty())
-                timed(P_SAPPLY, sapp_b.launches() + sapp_tb.launches(), (long)sapp.size(), [&] {
+            if (nsapp)
+                timed(P_SAPPLY, sapp_b.launches() + sapp_tb.launches(), nsapp, [&] {
                     run_larfb(sapp_b);
                     run_larfb(sapp_tb);
                 });
@@ -1407,9 +1443,9 @@ int Tree::factorize_t() {
         fprintf(stderr,
                 "[host] tables %.1f  fillin %.1f  descr %.1f (colour %.1f tpmqrt %.1f)  launch+ranks %.1f  merge: spa1 %.1f "
                 "parents %.1f nbrs %.1f edges %.1f (+blocks %.1f)  spars build %.1f split+upload %.1f launch %.1f  tail-sync "
-                "%.1f  solve-sched %.1f ms\n",
+                "%.1f  solve-sched %.1f ms  [scaling %.1f sapp %.1f rrqr %.1f  prep %.1f]\n",
                 hms[15], hms[0], hms[1], hms[10], hms[11], hms[2], hms[12], hms[13], hms[14], hms[3], hms[7], hms[8], hms[9],
-                hms[4], hms[5], hms[6]);
+                hms[4], hms[5], hms[6], hms[18], hms[19], hms[16], hms[17]);
     if (dbg_shapes) {
         double tot = 0;
         for (auto& e : shapes) tot += e.second[1];

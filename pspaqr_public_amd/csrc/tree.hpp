@@ -8,6 +8,7 @@
 
 #include <cstdint>
 #include <string>
+#include <array>
 #include <vector>
 
 #include "kernels.hpp"
@@ -236,6 +237,10 @@ private:
         prof[cat].flops += flops;
         prof[cat].bytes += bytes;
     }
+    // Tasks filed by builder slot and launch bucket (parts[i][q]): each bucket's slots copied in
+    // slot order into one staging block (a single threaded pass) and uploaded once.
+    template <typename T, size_t NBK>
+    std::array<std::pair<T*, int>, NBK> upload_buckets(const std::vector<std::array<std::vector<T>, NBK>>& parts);
     template <typename T> T* upload_vec(const std::vector<T>& v) {
         return v.empty() ? nullptr : (T*)upload(v.data(), v.size() * sizeof(T));
     }

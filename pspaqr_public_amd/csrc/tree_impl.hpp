@@ -7,6 +7,7 @@
 #include <functional>
 #include <mutex>
 #include <cstdlib>
+#include <cstring>
 #include <exception>
 #include <thread>
 #include <stdexcept>
@@ -217,6 +218,34 @@ int parallel_ranges(int n, int min_per, F&& f) {
     for (auto& e : err)
         if (e) std::rethrow_exception(e);
     return T;
+}
+
+template <typename T, size_t NBK>
+std::array<std::pair<T*, int>, NBK> Tree::upload_buckets(const std::vector<std::array<std::vector<T>, NBK>>& parts) {
+    const int np = (int)parts.size();
+    std::array<std::vector<size_t>, NBK> off;
+    std::array<T*, NBK> hb{}, db{};
+    std::array<size_t, NBK> nq{};
+    for (size_t q = 0; q < NBK; ++q) {
+        off[q].assign(np + 1, 0);
+        for (int i = 0; i < np; ++i) off[q][i + 1] = off[q][i] + parts[i][q].size();
+        nq[q] = off[q][np];
+        if (nq[q]) {
+            db[q] = (T*)desc_arena.alloc(nq[q] * sizeof(T));
+            hb[q] = (T*)staging.alloc(nq[q] * sizeof(T));
+        }
+    }
+    parallel_ranges(np, 1, [&](int b, int e, int) {
+        for (int i = b; i < e; ++i)
+            for (size_t q = 0; q < NBK; ++q)
+                if (!parts[i][q].empty()) memcpy(hb[q] + off[q][i], parts[i][q].data(), parts[i][q].size() * sizeof(T));
+    });
+    std::array<std::pair<T*, int>, NBK> out;
+    for (size_t q = 0; q < NBK; ++q) {
+        if (nq[q]) HIPCHK(hipMemcpyAsync(db[q], hb[q], nq[q] * sizeof(T), hipMemcpyHostToDevice, stream_));
+        out[q] = {db[q], (int)nq[q]};
+    }
+    return out;
 }
 
 // Splits off the tasks whose block fits one wave: dim(t) is the block's register length
