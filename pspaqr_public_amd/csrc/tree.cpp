@@ -971,84 +971,101 @@ int Tree::factorize_t() {
         int ncolors = colour_interiors(ints);
         st.colors = ncolors;
         hacc(10, ht);
-        std::vector<std::vector<TpmqrtTask<S>>> tpm(ncolors);
-        std::vector<TpmqrtLink<S>> tpml;
+        // Trailing-update chains, built over the interiors in parallel: links into one array at
+        // per-slot offsets (counted first), tasks straight into their colour's launch buckets
+        // (0-2 one-wave by link rows, 3-6 C-resident, 7 workgroup), all in interior order.
         const int TILE = 64;
-        std::vector<int> mslot(cl.size(), -1);  // out edge of m by target
-        for (int c : ints) {
-            Cluster& C = cl[c];
-            for (int e : C.in) {
-                Edge& cm = E(e);
-                const int m = cm.n1;
-                const int link0 = (int)tpml.size();
-                for (int f : cl[m].out) mslot[ed[f].n2] = f;
-                for (size_t j = 1; j < C.out.size(); ++j) {
-                    Edge& cn = E(C.out[j]);
-                    int mn = mslot[cn.n2];
-                    if (mn < 0) throw std::runtime_error("missing fill-in block");
-                    tpml.push_back({Ap(C.out[j]), (const S*)cn.T, Ap(mn), cn.rows, cn.rows, ed[mn].rows, 0});
-                    st.flops += 4.0 * cn.rows * C.cols * cm.cols;
-                    prof_add(P_TPMQRT, 4.0 * cn.rows * C.cols * cm.cols,
-                             SZ * ((double)cn.rows * C.cols + NB * C.cols + 2.0 * C.cols * cm.cols + 2.0 * cn.rows * cm.cols));
-                }
-                for (int f : cl[m].out) mslot[ed[f].n2] = -1;
-                const int nl = (int)tpml.size() - link0;
-                if (nl == 0) continue;
-                for (int c0 = 0; c0 < cm.cols; c0 += TILE)
-                    tpm[C.color].push_back({Ap(e) + (size_t)c0 * cm.rows, C.cols, cm.rows, std::min(TILE, cm.cols - c0),
-                                            link0, nl, c0});
+        using TpmBuckets = std::array<std::vector<TpmqrtTask<S>>, 8>;
+        const int nint = (int)ints.size();
+        const int NS = std::min(host_threads(), std::max(1, nint / 64));
+        std::vector<long> lcount(NS + 1, 0);
+        parallel_ranges(nint, 64, [&](int b, int e, int slot) {
+            long n = 0;
+            for (int i = b; i < e; ++i) {
+                const Cluster& C = cl[ints[i]];
+                if (!C.out.empty()) n += (long)C.in.size() * (long)(C.out.size() - 1);
             }
+            lcount[slot + 1] = n;
+        });
+        for (int i = 0; i < NS; ++i) lcount[i + 1] += lcount[i];
+        std::vector<TpmqrtLink<S>> tpml((size_t)lcount[NS]);
+        std::vector<std::vector<TpmBuckets>> tpm_parts(ncolors, std::vector<TpmBuckets>(NS));
+        std::vector<std::array<double, 3>> tpm_acc(NS, std::array<double, 3>{0.0, 0.0, 0.0});  // flops, prof flops, bytes
+        parallel_ranges(nint, 64, [&](int b, int e, int slot) {
+            std::vector<int> mslot(cl.size(), -1);  // out edge of m by target
+            long li = lcount[slot];
+            auto& acc = tpm_acc[slot];
+            for (int i = b; i < e; ++i) {
+                const Cluster& C = cl[ints[i]];
+                for (int e2 : C.in) {
+                    const Edge& cm = ed[e2];
+                    const int m = cm.n1;
+                    const int link0 = (int)li;
+                    int rmax = 0, rmin = INT_MAX;
+                    for (int f : cl[m].out) mslot[ed[f].n2] = f;
+                    for (size_t j = 1; j < C.out.size(); ++j) {
+                        const Edge& cn = ed[C.out[j]];
+                        const int mn = mslot[cn.n2];
+                        if (mn < 0) throw std::runtime_error("missing fill-in block");
+                        tpml[li++] = {Ap(C.out[j]), (const S*)cn.T, Ap(mn), cn.rows, cn.rows, ed[mn].rows, 0};
+                        rmax = std::max(rmax, cn.rows);
+                        rmin = std::min(rmin, cn.rows);
+                        const double f = 4.0 * cn.rows * C.cols * cm.cols;
+                        acc[0] += f;
+                        acc[1] += f;
+                        acc[2] += SZ * ((double)cn.rows * C.cols + NB * C.cols + 2.0 * C.cols * cm.cols + 2.0 * cn.rows * cm.cols);
+                    }
+                    for (int f : cl[m].out) mslot[ed[f].n2] = -1;
+                    const int nl = (int)li - link0;
+                    if (nl == 0) continue;
+                    for (int c0 = 0; c0 < cm.cols; c0 += TILE) {
+                        const int nc = std::min(TILE, cm.cols - c0);
+                        int q = -1;
+                        if (small_tasks && C.cols <= 64 && nc <= 64 && rmin >= 1 && rmax <= 64)
+                            q = rmax <= 16 ? 0 : (rmax <= 32 ? 1 : 2);
+                        else {
+                            const int r = tpmqrt_res_bucket_r<S>(nc, rmax);
+                            q = r < 0 ? 7 : 3 + r;
+                        }
+                        tpm_parts[C.color][slot][q].push_back(
+                            {Ap(e2) + (size_t)c0 * cm.rows, C.cols, cm.rows, nc, link0, nl, c0});
+                    }
+                }
+            }
+        });
+        for (const auto& acc : tpm_acc) {
+            st.flops += acc[0];
+            prof_add(P_TPMQRT, acc[1], acc[2]);
         }
         hacc(11, ht);
         hacc(1, ht);
         {
             TpmqrtLink<S>* d_tpml = upload_vec(tpml);
-            // per colour: one-wave buckets (k <= 64, links of 1..64 rows) and the workgroup rest
-            std::vector<std::array<TpmqrtTask<S>*, 4>> d_tpm(ncolors);
-            std::vector<std::array<int, 4>> n_tpm(ncolors);
-            std::vector<std::array<TpmqrtTask<S>*, 4>> d_tpmr(ncolors);  // C-resident FP32 buckets
-            std::vector<std::array<int, 4>> n_tpmr(ncolors);
-            for (int q = 0; q < ncolors; ++q) {
-                std::vector<TpmqrtTask<S>> sm[3], big;
-                if (small_tasks)
-                    split_small(tpm[q], [&](const TpmqrtTask<S>& t) {
-                        if (t.k > 64 || t.ncols > 64) return -1;
-                        int rmax = 0, rmin = INT_MAX;
-                        for (int j = 0; j < t.nlinks; ++j) {
-                            rmax = std::max(rmax, tpml[t.link0 + j].rows);
-                            rmin = std::min(rmin, tpml[t.link0 + j].rows);
-                        }
-                        return (rmin >= 1 && rmax <= 64) ? rmax : -1;
-                    }, sm, big);
-                else
-                    big = tpm[q];
-                std::vector<TpmqrtTask<S>> res[4], rest;
-                for (const auto& t : big) {
-                    const int b = tpmqrt_res_bucket(t, tpml.data());
-                    (b < 0 ? rest : res[b]).push_back(t);
-                }
-                for (int b = 0; b < 3; ++b) { d_tpm[q][b] = upload_vec(sm[b]); n_tpm[q][b] = (int)sm[b].size(); }
-                for (int b = 0; b < 4; ++b) { d_tpmr[q][b] = upload_vec(res[b]); n_tpmr[q][b] = (int)res[b].size(); }
-                d_tpm[q][3] = upload_vec(rest);
-                n_tpm[q][3] = (int)rest.size();
-                if (dbg_shapes)
-                    for (const auto& t : big) {
-                        int rows = 0;
-                        for (int j = 0; j < t.nlinks; ++j) rows += tpml[t.link0 + j].rows;
-                        shape_add("tpmqrt", t.k, rows / std::max(t.nlinks, 1), t.ncols, 4.0 * t.k * t.ncols * rows);
-                    }
-            }
+            std::vector<std::array<std::pair<TpmqrtTask<S>*, int>, 8>> tb(ncolors);
             long ntpm = 0;
-            for (int q = 0; q < ncolors; ++q) ntpm += (long)tpm[q].size();
+            for (int q = 0; q < ncolors; ++q) {
+                tb[q] = upload_buckets(tpm_parts[q]);
+                for (const auto& x : tb[q]) ntpm += x.second;
+                if (dbg_shapes)
+                    for (const auto& pb : tpm_parts[q])
+                        for (int bq = 3; bq < 8; ++bq)
+                            for (const auto& t : pb[bq]) {
+                                int rows = 0;
+                                for (int j = 0; j < t.nlinks; ++j) rows += tpml[t.link0 + j].rows;
+                                shape_add("tpmqrt", t.k, rows / std::max(t.nlinks, 1), t.ncols, 4.0 * t.k * t.ncols * rows);
+                            }
+            }
+
             if (ntpm)
                 timed(P_TPMQRT, ncolors, ntpm, [&] {
                     for (int q = 0; q < ncolors; ++q) {
-                        for (int b = 0; b < 3; ++b) launch_tpmqrt_wave<S>(d_tpm[q][b], d_tpml, n_tpm[q][b], 16 << b, stream_);
+                        const auto& u = tb[q];
+                        for (int b = 0; b < 3; ++b) launch_tpmqrt_wave<S>(u[b].first, d_tpml, u[b].second, 16 << b, stream_);
                         if constexpr (sizeof(S) == 4) {
-                            for (int b = 0; b < 3; ++b) launch_tpmqrt_res(d_tpmr[q][b], d_tpml, n_tpmr[q][b], 4 << b, stream_);
-                            launch_tpmqrt_res2(d_tpmr[q][3], d_tpml, n_tpmr[q][3], stream_);
+                            for (int b = 0; b < 3; ++b) launch_tpmqrt_res(u[3 + b].first, d_tpml, u[3 + b].second, 4 << b, stream_);
+                            launch_tpmqrt_res2(u[6].first, d_tpml, u[6].second, stream_);
                         }
-                        launch_tpmqrt<S>(d_tpm[q][3], d_tpml, n_tpm[q][3], stream_);
+                        launch_tpmqrt<S>(u[7].first, d_tpml, u[7].second, stream_);
                     }
                 });
             HIPCHK(hipGetLastError());

@@ -79,13 +79,16 @@ inline int larfb_res_bucket(const LarfbTask<S>& t) {
 // 1: <= 256, 2: <= 512, 3: <= 1024), or -1 when it stays on the workgroup kernel (FP64, a tile
 // wider than 64 columns or a longer link).  SPAQR_TPMQRT_RES=0 keeps every task on the workgroup kernel.
 template <typename S>
-inline int tpmqrt_res_bucket(const TpmqrtTask<S>& t, const TpmqrtLink<S>* links) {
+inline int tpmqrt_res_bucket_r(int ncols, int rmax) {  // rmax: the longest link's rows
     static const bool on = getenv("SPAQR_TPMQRT_RES") == nullptr || atoi(getenv("SPAQR_TPMQRT_RES")) != 0;
-    if (!on || sizeof(S) != 4 || t.ncols > 64) return -1;
+    if (!on || sizeof(S) != 4 || ncols > 64 || rmax > 1024) return -1;
+    return rmax <= 128 ? 0 : (rmax <= 256 ? 1 : (rmax <= 512 ? 2 : 3));
+}
+template <typename S>
+inline int tpmqrt_res_bucket(const TpmqrtTask<S>& t, const TpmqrtLink<S>* links) {
     int rmax = 0;
     for (int j = 0; j < t.nlinks; ++j) rmax = std::max(rmax, links[t.link0 + j].rows);
-    if (rmax > 1024) return -1;
-    return rmax <= 128 ? 0 : (rmax <= 256 ? 1 : (rmax <= 512 ? 2 : 3));
+    return tpmqrt_res_bucket_r<S>(t.ncols, rmax);
 }
 
 // Bucket of the C-resident FP32 sparsify kernels for a task: 0..2 (block rows <= 128 / 256 / 512)
