@@ -1742,6 +1742,8 @@ struct VecSmem {
     S t[32];        // panel coefficients t = T^T d (or T d)
     S xb[32];       // triangular-solve block
     S T[32 * 33];   // the panel's T block, column l at T[l * 33]
+    S xb2[2][32];   // row-per-thread triangular solve: solved blocks, by block parity
+    S part[NT];     // row-per-thread products: partial sums of the column phases
 };
 
 // One panel (kb <= 32 reflectors) of a block reflector H = I - V T V^T applied to a vector:
@@ -2147,6 +2149,101 @@ __device__ __forceinline__ void wg_trsv_upper(int k, const S* __restrict__ R, in
     }
 }
 
+// Row-per-thread forms for k <= NT: thread r holds x[r] in a register for the whole task.
+
+// Returns sum_j A_j[r, 0:s_j] y_j over a task's links for thread r < k (0 elsewhere).  All
+// waves work: wave w takes the 64-row block w % RB (RB = ceil(k / 64)) and every CP-th
+// 32-column chunk of each link (CP = NW / RB column phases), so each wave keeps 32 coalesced
+// column loads in flight; the CP partial sums of a row meet in LDS in a fixed order.
+template <typename S, int NT>
+__device__ __forceinline__ S wg_gemv_links_r1(int k, const VecGemvLink<S>* __restrict__ links, int link0, int nlinks,
+                                              VecSmem<S, NT>& sm) {
+    constexpr int NW = NT / 64, U = 32;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int RB = (k + 63) >> 6, CP = NW / RB;
+    const int rb = wave % RB, cp = wave / RB;
+    const int row = rb * 64 + lane, rr = min(row, k - 1);
+    S acc = S(0);
+    if (cp < CP) {
+        for (int li = 0; li < nlinks; ++li) {
+            const VecGemvLink<S> L = links[link0 + li];
+            const auto A = gmem(L.A);
+            const auto Y = gmem(L.y);
+            for (int c0 = cp * U; c0 < L.s; c0 += CP * U) {
+                S a[U], y[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int c = min(c0 + u, L.s - 1);
+                    a[u] = A[(unsigned)(rr + c * L.lda)];
+                    y[u] = Y[c];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) acc += (c0 + u < L.s ? y[u] : S(0)) * a[u];
+            }
+        }
+        sm.part[cp * (RB * 64) + row] = acc;
+    }
+    __syncthreads();
+    S tot = S(0);
+    if (tid < k)
+        for (int c = 0; c < CP; ++c) tot += sm.part[c * (RB * 64) + tid];
+    return tot;
+}
+
+// x <- R^{-1} x for R upper triangular (k x k, ldr, k <= NT), x[r] in thread r's xr.
+// 32-row blocks from the bottom.  Each thread holds its row of the current block's 32
+// columns and loads the next block's while this one is solved.  The diagonal block's rows
+// p0 .. p0+31 lie in one wave, which solves it by lane reads (no LDS, no barrier); the rows
+// above subtract its solution, read from LDS (double-buffered by block parity, so one
+// barrier per block).
+template <typename S, int NT>
+__device__ __forceinline__ void wg_trsv_upper_r1(int k, const S* __restrict__ R, int ldr, S& xr, VecSmem<S, NT>& sm) {
+    if (k <= 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = min(tid, k - 1);
+    int p0 = ((k - 1) / NB) * NB;
+    S v[32], vn[32];
+    {
+        const int kb = k - p0;
+        const auto Rc = gmem(R) + (size_t)p0 * ldr;
+#pragma unroll
+        for (int j = 0; j < 32; ++j) v[j] = Rc[(unsigned)(r + min(j, kb - 1) * ldr)];
+    }
+    for (int par = 0; p0 >= 0; p0 -= NB, par ^= 1) {
+        const int kb = min(NB, k - p0);  // only the first (bottom) block is partial
+        if (p0 > 0) {
+            int ro = r;
+            asm volatile("" : "+v"(ro));  // opaque per block: keeps 32 row offsets out of the loop
+            const auto Rn = gmem(R) + (size_t)(p0 - NB) * ldr;
+#pragma unroll
+            for (int j = 0; j < 32; ++j) vn[j] = Rn[(unsigned)(ro + j * ldr)];
+        }
+        if (wave == (p0 >> 6)) {
+            const int base = p0 & 63;
+            const int jl = lane - base;  // this lane's row within the block
+            S xi = xr;
+#pragma unroll
+            for (int j = 31; j >= 0; --j) {
+                if (j < kb) {
+                    const S xj = lane_read(xi, base + j) / lane_read(v[j], base + j);
+                    xi = (jl == j) ? xj : ((jl >= 0 && jl < j) ? xi - v[j] * xj : xi);
+                }
+            }
+            if (jl >= 0 && jl < 32) sm.xb2[par][jl] = (jl < kb) ? xi : S(0);
+            if (jl >= 0 && jl < kb) xr = xi;
+        }
+        __syncthreads();
+        if (tid < p0) {
+            S s = S(0);
+#pragma unroll
+            for (int j = 0; j < 32; ++j) s += v[j] * sm.xb2[par][j];
+            xr -= s;
+        }
+#pragma unroll
+        for (int j = 0; j < 32; ++j) v[j] = vn[j];
+    }
+}
+
 template <typename S, int NT>
 __global__ __launch_bounds__(NT) void k_vec_spars(const VecSparsTask<S>* __restrict__ tasks, int fwd) {
     __shared__ VecSmem<S, NT> sm;
@@ -2158,7 +2255,14 @@ __global__ __launch_bounds__(NT) void k_vec_spars(const VecSparsTask<S>* __restr
     } else {
         wg_vec_q<S, NT>(t.k, t.r, t.Qsp, t.k, t.Tsp, t.x, false, sm);
         __syncthreads();
-        wg_trsv_upper<S, NT>(t.k, t.Qs, t.ldq, t.x, sm);
+        if (t.k <= NT) {
+            const int tid = threadIdx.x;
+            S xr = tid < t.k ? t.x[tid] : S(0);
+            wg_trsv_upper_r1<S, NT>(t.k, t.Qs, t.ldq, xr, sm);
+            if (tid < t.k) t.x[tid] = xr;
+        } else {
+            wg_trsv_upper<S, NT>(t.k, t.Qs, t.ldq, t.x, sm);
+        }
     }
 }
 
@@ -2167,6 +2271,14 @@ __global__ __launch_bounds__(NT) void k_vec_trsv(const VecTrsvTask<S>* __restric
                                                  const VecGemvLink<S>* __restrict__ links) {
     __shared__ VecSmem<S, NT> sm;
     const VecTrsvTask<S> t = tasks[blockIdx.x];
+    if (t.k <= NT) {
+        const int tid = threadIdx.x;
+        S xr = tid < t.k ? t.x[tid] : S(0);
+        if (t.nlinks > 0) xr -= wg_gemv_links_r1<S, NT>(t.k, links, t.link0, t.nlinks, sm);
+        wg_trsv_upper_r1<S, NT>(t.k, t.R, t.ldr, xr, sm);
+        if (tid < t.k) t.x[tid] = xr;
+        return;
+    }
     for (int li = 0; li < t.nlinks; ++li) {
         const VecGemvLink<S> L = links[t.link0 + li];
         if (L.s <= 0) continue;

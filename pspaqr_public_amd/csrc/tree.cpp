@@ -1682,6 +1682,24 @@ int Tree::build_solve_schedule() {
             sp.push_back({(const S*)C.Qs, (const S*)C.Ts, (const S*)C.Qsp, (const S*)C.Tsp, X0 + C.xoff, C.cols_org,
                           ed[C.self_edge].rows, C.rank, 0});
         }
+        if (getenv("SPAQR_DEBUG_SOLVE") && q.size() <= 64) {
+            fprintf(stderr, "[solve shapes L%d]", L);
+            for (auto& t : q) fprintf(stderr, " q(%d,%d)", t.m, t.k);
+            for (int k = 0; k < ncol; ++k)
+                for (int c = 0; c < 4; ++c)
+                    for (auto& t : ch[k][c]) {
+                        fprintf(stderr, " chain%d(k%d:", c, t.k);
+                        for (int j = 0; j < t.nlinks; ++j) fprintf(stderr, "%s%d", j ? "," : "", links[t.link0 + j].rows);
+                        fprintf(stderr, ")");
+                    }
+            for (auto& t : sp) fprintf(stderr, " spars(m%d,k%d,r%d)", t.ldq, t.k, t.r);
+            for (auto& t : tr) {
+                long s = 0;
+                for (int j = 0; j < t.nlinks; ++j) s += tl[t.link0 + j].s;
+                fprintf(stderr, " trsv(k%d,nl%d,s%ld)", t.k, t.nlinks, s);
+            }
+            fprintf(stderr, "\n");
+        }
         lv.q = upload_vec(q);
         lv.n_q = (int)q.size();
         if (getenv("SPAQR_DEBUG_HOST")) {
