@@ -1736,6 +1736,13 @@ __global__ void k_fill(S* p, size_t n, S v) {
 // itself; the threads exchange nothing but the 32 reflector coefficients of the current
 // panel, through LDS.
 
+// Global-memory view of a pointer read from a task descriptor (lets the compiler emit global
+// rather than flat loads).
+template <typename S>
+__device__ __forceinline__ const __attribute__((address_space(1))) S* gmem(const S* p) {
+    return (const __attribute__((address_space(1))) S*)p;
+}
+
 template <typename S, int NT = WG>
 struct VecSmem {
     S red[NT / 64][32];  // per-wave partial column sums of one panel
@@ -1811,7 +1818,7 @@ __device__ __forceinline__ void wg_vec_panel(int m, int p0, int kb, const S* __r
         S tl = S(0);
 #pragma unroll
         for (int q = 0; q < 32; ++q) {
-            const S dq = __shfl(dl, q, 64);
+            const S dq = lane_read(dl, q);
             const S tq = trans ? sm.T[lc * 33 + q] : sm.T[q * 33 + lc];
             const bool use = trans ? (q <= lc) : (q >= lc);
             tl += (use ? tq : S(0)) * dq;
@@ -1839,12 +1846,136 @@ __device__ __forceinline__ void wg_vec_panel(int m, int p0, int kb, const S* __r
     }
 }
 
+// Row-per-thread panel applies (every row count <= NT): thread R holds x[R] in a register
+// and its row of the current panel's 32 columns, and loads the next panel's row (and T
+// share) while the current panel is applied, so a panel costs two barriers and no exposed
+// load latency.
+template <typename S, int NT>
+struct VecRowPanel {
+    S v[32];
+    S tv[(NB * NB) / NT];  // this thread's share of the panel's T block
+};
+
+// Loads row r's coefficients for one panel (V: the panel's first column; columns past kb
+// clamped, T zero there) and this thread's share of the panel's T block (ld NB).  PENT: the
+// row as stored; else unit lower trapezoidal, i = r - p0 the row within the panel (v below
+// the diagonal, 1 on it, 0 above).  Dead rows get zeros.  Only the rows of the diagonal
+// block need masks, so the compares stay in that branch.
+template <typename S, bool PENT, int NT>
+__device__ __forceinline__ void vec_row_panel_load(const S* V, int ldv, const S* T, int kb, int r, int i, bool live,
+                                                   VecRowPanel<S, NT>& g) {
+    const int tid = threadIdx.x;
+    int ro = r;
+    asm volatile("" : "+v"(ro));  // opaque per call: keeps 32 row offsets out of callers' loops
+    asm volatile("" : "+v"(i));   // and the 32 diagonal masks
+    const auto Vp = gmem(V);
+    if (!live || (!PENT && i < 0)) {
+#pragma unroll
+        for (int l = 0; l < 32; ++l) g.v[l] = S(0);
+    } else {
+#pragma unroll
+        for (int l = 0; l < 32; ++l) g.v[l] = Vp[(unsigned)(ro + min(l, kb - 1) * ldv)];
+        if (!PENT && i < 32) {
+#pragma unroll
+            for (int l = 0; l < 32; ++l) g.v[l] = l < i ? g.v[l] : (l == i ? S(1) : S(0));
+        }
+    }
+    const auto Tp = gmem(T);
+#pragma unroll
+    for (int e0 = 0; e0 < (NB * NB) / NT; ++e0) {
+        const int e = tid + e0 * NT, q = e & (NB - 1), l = min(e / NB, kb - 1);
+        g.tv[e0] = Tp[q + l * NB];
+    }
+}
+
+// One panel of H = I - V T V^T applied as in wg_vec_panel, from registers (g.v: this
+// thread's coefficients, zero for rows the panel does not touch).  xt (PENT, LDS) is the
+// top block at p0.  prefetch() runs between the reduction and the first barrier.
+template <typename S, bool PENT, int NT, class F>
+__device__ __forceinline__ void vec_row_panel_apply(const VecRowPanel<S, NT>& g, int kb, S& xr, S* xt, bool trans,
+                                                    VecSmem<S, NT>& sm, F&& prefetch) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int e0 = 0; e0 < (NB * NB) / NT; ++e0) {
+        const int e = tid + e0 * NT, q = e & (NB - 1), l = e / NB;
+        sm.T[l * 33 + q] = (q < kb && l < kb) ? g.tv[e0] : S(0);
+    }
+    S d[32];
+#pragma unroll
+    for (int l = 0; l < 32; ++l) d[l] = g.v[l] * xr;
+    const S tot = wave_sum32<S>(d);
+    if (vec_owner(lane)) sm.red[wave][vec_col(lane)] = tot;
+    __builtin_amdgcn_sched_barrier(0);  // the next panel's loads go after the reduction (registers)
+    prefetch();
+    __syncthreads();
+    if (wave == 0) {
+        S dl = S(0), xtv = S(0);
+        if (lane < 32) {
+#pragma unroll
+            for (int w = 0; w < NT / 64; ++w) dl += sm.red[w][lane];
+            if (PENT) {
+                xtv = xt[min(lane, kb - 1)];
+                dl += xtv;
+            }
+        }
+        const int lc = min(lane, 31);
+        S tl = S(0);
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            const S dq = lane_read(dl, q);
+            const S tq = trans ? sm.T[lc * 33 + q] : sm.T[q * 33 + lc];
+            const bool use = trans ? (q <= lc) : (q >= lc);
+            tl += (use ? tq : S(0)) * dq;
+        }
+        if (lane < 32) sm.t[lane] = (lane < kb) ? tl : S(0);
+        if (PENT && lane < kb) xt[lane] = xtv - tl;
+    }
+    __syncthreads();
+    S s = S(0);
+#pragma unroll
+    for (int l = 0; l < 32; ++l) s += g.v[l] * sm.t[l];
+    xr -= s;
+}
+
+// x <- Q^T x (trans) or Q x for m <= NT rows (geqrt layout, as wg_vec_q).
+template <typename S, int NT>
+__device__ __forceinline__ void wg_vec_q_r1(int m, int k, const S* __restrict__ V, int ldv, const S* __restrict__ T, S* x,
+                                            bool trans, VecSmem<S, NT>& sm) {
+    if (k <= 0) return;
+    const int tid = threadIdx.x, r = min(tid, m - 1);
+    const bool live = tid < m;
+    S xr = live ? x[tid] : S(0);
+    const int np = (k + NB - 1) / NB;
+    VecRowPanel<S, NT> cur, nxt;
+    {
+        const int p0 = (trans ? 0 : np - 1) * NB;
+        vec_row_panel_load<S, false, NT>(V + (size_t)p0 * ldv, ldv, T + (size_t)p0 * NB, min(NB, k - p0), r, tid - p0, live,
+                                         cur);
+    }
+    for (int j = 0; j < np; ++j) {
+        const int kb = min(NB, k - (trans ? j : np - 1 - j) * NB);
+        vec_row_panel_apply<S, false, NT>(cur, kb, xr, nullptr, trans, sm, [&] {
+            if (j + 1 < np) {
+                const int pn = (trans ? j + 1 : np - 2 - j) * NB;
+                vec_row_panel_load<S, false, NT>(V + (size_t)pn * ldv, ldv, T + (size_t)pn * NB, min(NB, k - pn), r, tid - pn,
+                                                 live, nxt);
+            }
+        });
+        cur = nxt;
+    }
+    if (live) x[tid] = xr;
+}
+
 // x <- Q^T x (trans) or Q x for Q = H_0 H_1 ... (geqrt layout, k reflectors over m rows,
 // blocked T with NB-column panels).
 template <typename S, int NT = WG>
 __device__ __forceinline__ void wg_vec_q(int m, int k, const S* __restrict__ V, int ldv, const S* __restrict__ T, S* x, bool trans,
                          VecSmem<S, NT>& sm) {
     if (k <= 0) return;
+    if (m <= NT) {
+        wg_vec_q_r1<S, NT>(m, k, V, ldv, T, x, trans, sm);
+        return;
+    }
     if (trans) {
         for (int p0 = 0; p0 < k; p0 += NB)
             wg_vec_panel<S, false, NT>(m, p0, min(NB, k - p0), V, ldv, T + (size_t)p0 * NB, nullptr, x, true, sm);
@@ -1861,11 +1992,54 @@ __global__ __launch_bounds__(NT) void k_vec_q(const VecQTask<S>* __restrict__ ta
     wg_vec_q<S, NT>(t.m, t.k, t.V, t.ldv, t.T, t.x, t.trans != 0, sm);
 }
 
+constexpr int kChainTopMax = 1024;  // largest chain top block kept in LDS by k_vec_chain
+
 template <typename S, int NT>
 __global__ __launch_bounds__(NT) void k_vec_chain(const VecChainTask<S>* __restrict__ tasks,
                                                   const VecChainLink<S>* __restrict__ links) {
     __shared__ VecSmem<S, NT> sm;
+    __shared__ S xc[kChainTopMax];
     const VecChainTask<S> t = tasks[blockIdx.x];
+    if (t.k <= 0) return;
+    if (t.rmax <= NT && t.k <= kChainTopMax) {
+        // row-per-thread: the top block in LDS, each link's rows in registers, the next
+        // panel (next link's first panel and x rows included) loaded under the current one
+        const int tid = threadIdx.x, np = (t.k + NB - 1) / NB;
+        for (int i = tid; i < t.k; i += NT) xc[i] = t.xc[i];
+        VecChainLink<S> L = links[t.link0];
+        bool live = tid < L.rows;
+        S xr = live ? L.xn[tid] : S(0), xnext = S(0);
+        VecRowPanel<S, NT> cur, nxt;
+        vec_row_panel_load<S, true, NT>(L.V, L.ldv, L.T, min(NB, t.k), min(tid, L.rows - 1), 0, live, cur);
+        for (int li = 0; li < t.nlinks; ++li) {
+            const bool more = li + 1 < t.nlinks;
+            const VecChainLink<S> Ln = links[t.link0 + (more ? li + 1 : li)];
+            for (int pi = 0; pi < np; ++pi) {
+                const int p0 = pi * NB, kb = min(NB, t.k - p0);
+                vec_row_panel_apply<S, true, NT>(cur, kb, xr, xc + p0, true, sm, [&] {
+                    if (pi + 1 < np) {
+                        const int pn = p0 + NB;
+                        vec_row_panel_load<S, true, NT>(L.V + (size_t)pn * L.ldv, L.ldv, L.T + (size_t)pn * NB,
+                                                        min(NB, t.k - pn), min(tid, L.rows - 1), 0, live, nxt);
+                    } else if (more) {
+                        const bool ln = tid < Ln.rows;
+                        vec_row_panel_load<S, true, NT>(Ln.V, Ln.ldv, Ln.T, min(NB, t.k), min(tid, Ln.rows - 1), 0, ln, nxt);
+                        xnext = ln ? Ln.xn[tid] : S(0);
+                    }
+                });
+                cur = nxt;
+            }
+            if (live) L.xn[tid] = xr;
+            if (more) {
+                L = Ln;
+                live = tid < L.rows;
+                xr = xnext;
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < t.k; i += NT) t.xc[i] = xc[i];
+        return;
+    }
     for (int li = 0; li < t.nlinks; ++li) {
         const VecChainLink<S> L = links[t.link0 + li];
         for (int p0 = 0; p0 < t.k; p0 += NB)
@@ -1883,13 +2057,6 @@ struct ChainLinkRegs {
     S tc[NB];
     S xr;
 };
-
-// Global-memory view of a pointer read from a task descriptor (lets the compiler emit global
-// rather than flat loads).
-template <typename S>
-__device__ __forceinline__ const __attribute__((address_space(1))) S* gmem(const S* p) {
-    return (const __attribute__((address_space(1))) S*)p;
-}
 
 template <typename S>
 __device__ __forceinline__ void chain_link_load(const VecChainLink<S>& L, int k, int lane, ChainLinkRegs<S>& g) {
@@ -2244,11 +2411,11 @@ __device__ __forceinline__ void wg_trsv_upper_r1(int k, const S* __restrict__ R,
     }
 }
 
-template <typename S, int NT>
-__global__ __launch_bounds__(NT) void k_vec_spars(const VecSparsTask<S>* __restrict__ tasks, int fwd) {
+template <typename S, int NT, bool FWD>
+__global__ __launch_bounds__(NT) void k_vec_spars(const VecSparsTask<S>* __restrict__ tasks) {
     __shared__ VecSmem<S, NT> sm;
     const VecSparsTask<S> t = tasks[blockIdx.x];
-    if (fwd) {
+    if constexpr (FWD) {
         wg_vec_q<S, NT>(t.ldq, t.k, t.Qs, t.ldq, t.Ts, t.x, true, sm);
         __syncthreads();
         wg_vec_q<S, NT>(t.k, t.r, t.Qsp, t.k, t.Tsp, t.x, true, sm);
@@ -2527,11 +2694,13 @@ void launch_vec_spars(const VecSparsTask<S>* d, int n, int fwd, hipStream_t s) {
     if (n <= 0) return;
     if constexpr (sizeof(S) == 4) {
         if (n <= kVecWideTasks) {
-            hipLaunchKernelGGL((k_vec_spars<S, 1024>), dim3(n), dim3(1024), 0, s, d, fwd);
+            if (fwd) hipLaunchKernelGGL((k_vec_spars<S, 1024, true>), dim3(n), dim3(1024), 0, s, d);
+            else hipLaunchKernelGGL((k_vec_spars<S, 1024, false>), dim3(n), dim3(1024), 0, s, d);
             return;
         }
     }
-    hipLaunchKernelGGL((k_vec_spars<S, WG>), dim3(n), dim3(WG), 0, s, d, fwd);
+    if (fwd) hipLaunchKernelGGL((k_vec_spars<S, WG, true>), dim3(n), dim3(WG), 0, s, d);
+    else hipLaunchKernelGGL((k_vec_spars<S, WG, false>), dim3(n), dim3(WG), 0, s, d);
 }
 template <typename S>
 void launch_vec_trsv(const VecTrsvTask<S>* d, const VecGemvLink<S>* l, int n, hipStream_t s) {

@@ -82,7 +82,7 @@ template <typename S> struct SparsApplyTask {
 template <typename S> struct VecQTask { const S* V; const S* T; S* x; int m, k, ldv, trans; };
 // [xc (k); xn_j (rows_j)] <- Q_j^T [...] along the chain of one interior.
 template <typename S> struct VecChainLink { const S* V; const S* T; S* xn; int rows, ldv; };
-template <typename S> struct VecChainTask { S* xc; int k, link0, nlinks, pad; };
+template <typename S> struct VecChainTask { S* xc; int k, link0, nlinks, rmax; };  // rmax: the longest link's rows
 // Sparsified interface forward/backward: Qs (m x k, ldq) + Ts, Qsp (k x r, ld k) + Tsp.
 template <typename S> struct VecSparsTask { const S* Qs; const S* Ts; const S* Qsp; const S* Tsp; S* x; int k, ldq, r, pad; };
 // x[0:k] -= sum_j A_j[0:k, 0:s_j] y_j, then x[0:k] <- R^{-1} x[0:k].

@@ -1641,8 +1641,9 @@ int Tree::build_solve_schedule() {
         int ncol = 0;
         for (int c : interiors[L]) ncol = std::max(ncol, cl[c].color + 1);
         // largest k sent to the one-wave chain kernel (64; SPAQR_WAVE_CHAINS=0 turns it off)
-        static const int wave_chain_k = getenv("SPAQR_WAVE_CHAINS") ? std::min(64, atoi(getenv("SPAQR_WAVE_CHAINS"))) : 64;
-        static const int pipe_chain_k = getenv("SPAQR_PIPE_CHAINS") ? std::min(512, atoi(getenv("SPAQR_PIPE_CHAINS"))) : 512;
+        // read at every plan, so a process can switch kernels between trees
+        const int wave_chain_k = getenv("SPAQR_WAVE_CHAINS") ? std::min(64, atoi(getenv("SPAQR_WAVE_CHAINS"))) : 64;
+        const int pipe_chain_k = getenv("SPAQR_PIPE_CHAINS") ? std::min(512, atoi(getenv("SPAQR_PIPE_CHAINS"))) : 512;
         std::vector<std::array<std::vector<VecChainTask<S>>, 4>> ch(ncol);
         std::vector<VecChainLink<S>> links;
         std::vector<VecTrsvTask<S>> tr;
@@ -1660,6 +1661,7 @@ int Tree::build_solve_schedule() {
                 rmax = std::max(rmax, eo.rows);
             }
             t.nlinks = (int)links.size() - t.link0;
+            t.rmax = rmax;
             int kind = 3;
             if (C.cols >= 1 && rmin >= 1) {
                 if (C.cols <= wave_chain_k && rmax <= 64) kind = 0;

@@ -78,6 +78,31 @@ def test_solve_and_gmres_parity(cfg, side):
     assert res.true_res <= 1e-12
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("cfg,side", [(2, 12), (3, 16)])
+def test_solve_on_workgroup_chain_kernel(cfg, side, prec, monkeypatch):
+    """Every elimination chain of the solve on the workgroup chain kernel (the one-wave and
+    pipelined chain kernels switched off) gives the solve of the default kernel choice: FP64
+    against the oracle at the 1e-10 bar, FP32 against the default FP32 solve."""
+    P = make_problem(cfg, side)
+    Td = build_tree(P, prec)
+    Td.factorize()
+    b = P.b[Td.cperm()]
+    xd = Td.solve(b)
+    monkeypatch.setenv("SPAQR_WAVE_CHAINS", "0")
+    monkeypatch.setenv("SPAQR_PIPE_CHAINS", "0")
+    Tp = build_tree(P, prec)
+    Tp.factorize()
+    xp = Tp.solve(b)
+    if prec == 64:
+        O = OracleTree(P.A, P.n_side, P.dim, P.nlevels, P.tol, P.skip)
+        O.factorize()
+        xo = O.solve(b)
+        assert np.linalg.norm(xp - xo) / np.linalg.norm(xo) < 1e-10
+    else:
+        assert np.linalg.norm(xp - xd) / np.linalg.norm(xd) < 1e-3
+
+
 @pytest.mark.parametrize("cfg,side", [(1, 32), (2, 12), (3, 12)])
 def test_exact_factorization_tol0(cfg, side):
     """tol = 0 (skip = nlevels - 1, pspaQR.cpp:150): no sparsification, the solve is the
