@@ -2378,7 +2378,7 @@ __device__ __forceinline__ void wg_trsv_upper_r1(int k, const S* __restrict__ R,
     }
     for (int par = 0; p0 >= 0; p0 -= NB, par ^= 1) {
         const int kb = min(NB, k - p0);  // only the first (bottom) block is partial
-        if (p0 > 0) {
+        if (p0 > 0 && tid < p0) {  // rows from p0 on are solved
             int ro = r;
             asm volatile("" : "+v"(ro));  // opaque per block: keeps 32 row offsets out of the loop
             const auto Rn = gmem(R) + (size_t)(p0 - NB) * ldr;

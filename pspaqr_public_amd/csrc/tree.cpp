@@ -1640,10 +1640,12 @@ int Tree::build_solve_schedule() {
         std::vector<VecQTask<S>> q;
         int ncol = 0;
         for (int c : interiors[L]) ncol = std::max(ncol, cl[c].color + 1);
-        // largest k sent to the one-wave chain kernel (64; SPAQR_WAVE_CHAINS=0 turns it off)
-        // read at every plan, so a process can switch kernels between trees
-        const int wave_chain_k = getenv("SPAQR_WAVE_CHAINS") ? std::min(64, atoi(getenv("SPAQR_WAVE_CHAINS"))) : 64;
-        const int pipe_chain_k = getenv("SPAQR_PIPE_CHAINS") ? std::min(512, atoi(getenv("SPAQR_PIPE_CHAINS"))) : 512;
+        // Every chain runs on the workgroup chain kernel (row-per-thread when its links fit);
+        // the one-wave (k <= SPAQR_WAVE_CHAINS <= 64) and pipelined (k <= SPAQR_PIPE_CHAINS <=
+        // 512) chain kernels are opt-in: on the 64^3 solve they are slower.  Read at every plan,
+        // so a process can switch kernels between trees.
+        const int wave_chain_k = getenv("SPAQR_WAVE_CHAINS") ? std::min(64, atoi(getenv("SPAQR_WAVE_CHAINS"))) : 0;
+        const int pipe_chain_k = getenv("SPAQR_PIPE_CHAINS") ? std::min(512, atoi(getenv("SPAQR_PIPE_CHAINS"))) : 0;
         std::vector<std::array<std::vector<VecChainTask<S>>, 4>> ch(ncol);
         std::vector<VecChainLink<S>> links;
         std::vector<VecTrsvTask<S>> tr;

@@ -80,17 +80,17 @@ def test_solve_and_gmres_parity(cfg, side):
 
 @pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("cfg,side", [(2, 12), (3, 16)])
-def test_solve_on_workgroup_chain_kernel(cfg, side, prec, monkeypatch):
-    """Every elimination chain of the solve on the workgroup chain kernel (the one-wave and
-    pipelined chain kernels switched off) gives the solve of the default kernel choice: FP64
-    against the oracle at the 1e-10 bar, FP32 against the default FP32 solve."""
+def test_solve_on_opt_in_chain_kernels(cfg, side, prec, monkeypatch):
+    """The opt-in one-wave and pipelined chain kernels of the solve (SPAQR_WAVE_CHAINS,
+    SPAQR_PIPE_CHAINS) give the solve of the default workgroup chain kernel: FP64 against the
+    oracle at the 1e-10 bar, FP32 against the default FP32 solve."""
     P = make_problem(cfg, side)
     Td = build_tree(P, prec)
     Td.factorize()
     b = P.b[Td.cperm()]
     xd = Td.solve(b)
-    monkeypatch.setenv("SPAQR_WAVE_CHAINS", "0")
-    monkeypatch.setenv("SPAQR_PIPE_CHAINS", "0")
+    monkeypatch.setenv("SPAQR_WAVE_CHAINS", "64")
+    monkeypatch.setenv("SPAQR_PIPE_CHAINS", "512")
     Tp = build_tree(P, prec)
     Tp.factorize()
     xp = Tp.solve(b)
