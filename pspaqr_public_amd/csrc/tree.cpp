@@ -32,6 +32,7 @@
 #include <stdexcept>
 #include <string>
 #include <tuple>
+#include <unordered_set>
 
 #include "kernels.hpp"
 #include "launch.hpp"
@@ -635,29 +636,150 @@ int Tree::assemble() {
 // get_sparsity + alloc_fillin (src/ptree.cpp:866-871, 140-153); returns nothing, new
 // edges are appended to ed (their device blocks are allocated by the caller).
 void Tree::fillin() {
-    for (int c : interiors[current_bottom]) {
-        std::vector<int>& d2 = cl[c].dist2;
-        for (int e : cl[c].in) d2.push_back(ed[e].n1);
-        std::sort(d2.begin(), d2.end());
-        d2.erase(std::unique(d2.begin(), d2.end()), d2.end());
-    }
-    // edge existence through a stamp array: mark[n2] == stamp when nn already has an edge to n2
-    std::vector<int> mark(cl.size(), -1);
-    int stamp = 0;
-    for (int c : interiors[current_bottom]) {
-        const std::vector<int> outs = cl[c].out;
-        for (int nn : cl[c].dist2) {
-            ++stamp;
-            for (int e : cl[nn].out) mark[ed[e].n2] = stamp;
-            for (int e : outs) {
-                const int nbr = ed[e].n2;
-                if (mark[nbr] != stamp) {
-                    new_edge(nn, nbr);
-                    mark[nbr] = stamp;
+    static const bool dbg = getenv("SPAQR_DEBUG_FILLIN") != nullptr;
+    auto tf = std::chrono::steady_clock::now();
+    double tm[6] = {0};
+    int ti = 0;
+    auto tick = [&] {
+        const auto n = std::chrono::steady_clock::now();
+        tm[ti++] = std::chrono::duration<double, std::milli>(n - tf).count();
+        tf = n;
+    };
+    const auto& ints = interiors[current_bottom];
+    const int nint = (int)ints.size();
+    parallel_ranges(nint, 64, [&](int b, int e, int) {
+        for (int i = b; i < e; ++i) {
+            Cluster& C = cl[ints[i]];
+            std::vector<int>& d2 = C.dist2;
+            for (int e2 : C.in) d2.push_back(ed[e2].n1);
+            std::sort(d2.begin(), d2.end());
+            d2.erase(std::unique(d2.begin(), d2.end()), d2.end());
+        }
+    });
+    // When no interior has another interior of this level in its distance-2 list, the
+    // interiors' out lists do not change here, so the new blocks are fixed by the edges as
+    // they stand: candidates (nn, nbr) built per interior in parallel, the first occurrence of
+    // each kept (in interior, neighbour, out-edge order), then the edges made in that order.
+    // Otherwise the interiors run one after another.
+    tick();
+    std::vector<char> is_int(cl.size(), 0);
+    for (int c : ints) is_int[c] = 1;
+    bool independent = true;
+    for (int c : ints)
+        for (int nn : cl[c].dist2)
+            if (nn != c && is_int[nn]) independent = false;
+    if (!independent) {
+        // edge existence through a stamp array: mark[n2] == stamp when nn already has an edge to n2
+        std::vector<int> mark(cl.size(), -1);
+        int stamp = 0;
+        for (int c : ints) {
+            const std::vector<int> outs = cl[c].out;
+            for (int nn : cl[c].dist2) {
+                ++stamp;
+                for (int e : cl[nn].out) mark[ed[e].n2] = stamp;
+                for (int e : outs) {
+                    const int nbr = ed[e].n2;
+                    if (mark[nbr] != stamp) {
+                        new_edge(nn, nbr);
+                        mark[nbr] = stamp;
+                    }
                 }
             }
         }
+        return;
     }
+    using Cand = std::pair<int, int>;  // (nn, nbr)
+    const int T = host_threads();
+    // per builder slot: its candidates in order, and their indices filed by nn % T (the
+    // thread that keeps the first occurrence of every pair of that nn)
+    std::vector<std::vector<Cand>> cand(T);
+    std::vector<std::vector<std::vector<int>>> byt(T, std::vector<std::vector<int>>(T));
+    const int ns = parallel_ranges(nint, 64, [&](int b, int e, int slot) {
+        std::vector<int> mark(cl.size(), -1);
+        int stamp = 0;
+        auto& out = cand[slot];
+        auto& bt = byt[slot];
+        for (int i = b; i < e; ++i) {
+            const Cluster& C = cl[ints[i]];
+            for (int nn : C.dist2) {
+                ++stamp;
+                for (int e2 : cl[nn].out) mark[ed[e2].n2] = stamp;
+                for (int e2 : C.out) {
+                    const int nbr = ed[e2].n2;
+                    if (mark[nbr] != stamp) {
+                        bt[nn % T].push_back((int)out.size());
+                        out.push_back({nn, nbr});
+                    }
+                }
+            }
+        }
+    });
+    cand.resize(ns);
+    tick();
+    std::vector<std::vector<char>> keep(ns);
+    size_t ncand = 0;
+    for (int s2 = 0; s2 < ns; ++s2) {
+        keep[s2].assign(cand[s2].size(), 0);
+        ncand += cand[s2].size();
+    }
+    parallel_ranges(T, ncand < 16384 ? T : 1, [&](int b, int e, int) {
+        for (int t = b; t < e; ++t) {
+            std::unordered_set<long long> seen;
+            for (int s2 = 0; s2 < ns; ++s2)
+                for (int j : byt[s2][t]) {
+                    const Cand& c = cand[s2][j];
+                    if (seen.insert(((long long)c.first << 32) | (unsigned)c.second).second) keep[s2][j] = 1;
+                }
+        }
+    });
+    tick();
+    // the edges in candidate order, as new_edge makes them: entries and out lists by source,
+    // in lists by target (each list appended in id order), sources and targets split over
+    // the threads by cluster number
+    const size_t base = ed.size();
+    std::vector<Cand> made;
+    std::vector<std::vector<int>> ob(T), ib(T);
+    for (int s2 = 0; s2 < ns; ++s2)
+        for (size_t j = 0; j < cand[s2].size(); ++j)
+            if (keep[s2][j]) {
+                const Cand c = cand[s2][j];
+                ob[c.first % T].push_back((int)made.size());
+                if (c.first != c.second) ib[c.second % T].push_back((int)made.size());
+                made.push_back(c);
+            }
+    const size_t nnew = made.size();
+    ed.resize(base + nnew);
+    tick();
+    parallel_ranges(T, nnew < 16384 ? T : 1, [&](int b, int e, int) {
+        for (int t = b; t < e; ++t) {
+            for (int i : ob[t]) {
+                const int c1 = made[i].first, c2 = made[i].second, id = (int)(base + i);
+                Edge& E = ed[id];
+                E.n1 = c1;
+                E.n2 = c2;
+                E.rows = cl[c2].rows;
+                E.cols = cl[c1].cols;
+                E.spars = cl[c1].lvl() > current_bottom && cl[c2].lvl() > current_bottom && c1 != c2;
+                Cluster& C1 = cl[c1];
+                C1.out.push_back(id);
+                if (E.spars) C1.sout.push_back(id);
+                if (c1 == c2) C1.self_edge = id;
+            }
+        }
+    });
+    parallel_ranges(T, nnew < 16384 ? T : 1, [&](int b, int e, int) {
+        for (int t = b; t < e; ++t)
+            for (int i : ib[t]) {
+                const int id = (int)(base + i);
+                Cluster& C2 = cl[made[i].second];
+                C2.in.push_back(id);
+                if (ed[id].spars) C2.sin.push_back(id);
+            }
+    });
+    tick();
+    if (dbg)
+        fprintf(stderr, "[fillin L%d] %zu candidates, %zu edges: dist2 %.2f cand %.2f dedup %.2f made %.2f lists %.2f ms\n",
+                current_bottom, ncand, nnew, tm[0], tm[1], tm[2], tm[3], tm[4]);
 }
 
 double Tree::factor_flops() const {
@@ -1349,12 +1471,47 @@ int Tree::factorize_t() {
                 }
             });
             hacc(14, ht);
-            for (int i = 0; i < np; ++i) {
-                const int p = bl[i];
-                cl[p].dist2 = std::move(pd2[i]);
-                for (int nn : pnb[i]) new_edge(p, nn);
-                sort_out(cl[p]);
-            }
+            // The parents' edges, in the order new_edge would make them (parent order, then
+            // neighbour order): ids from prefix sums, entries and out lists filled per parent in
+            // parallel, then every target's in lists appended in id order (targets split over
+            // the threads by cluster number).
+            std::vector<size_t> eoff(np + 1, 0);
+            for (int i = 0; i < np; ++i) eoff[i + 1] = eoff[i] + pnb[i].size();
+            const size_t base = ed.size(), nnew = eoff[np];
+            ed.resize(base + nnew);
+            parallel_ranges(np, 64, [&](int b, int e, int) {
+                for (int i = b; i < e; ++i) {
+                    const int p = bl[i];
+                    Cluster& P = cl[p];
+                    P.dist2 = std::move(pd2[i]);
+                    P.out.reserve(P.out.size() + pnb[i].size());
+                    for (size_t j = 0; j < pnb[i].size(); ++j) {
+                        const int nn = pnb[i][j], id = (int)(base + eoff[i] + j);
+                        Edge& E = ed[id];
+                        E.n1 = p;
+                        E.n2 = nn;
+                        E.rows = cl[nn].rows;
+                        E.cols = P.cols;
+                        E.spars = P.lvl() > current_bottom && cl[nn].lvl() > current_bottom && p != nn;
+                        P.out.push_back(id);
+                        if (E.spars) P.sout.push_back(id);
+                        if (p == nn) P.self_edge = id;
+                    }
+                    sort_out(P);
+                }
+            });
+            const int NT = host_threads();
+            std::vector<std::vector<int>> ib(NT);
+            for (size_t id = base; id < base + nnew; ++id)
+                if (ed[id].n1 != ed[id].n2) ib[ed[id].n2 % NT].push_back((int)id);
+            parallel_ranges(NT, nnew < 16384 ? NT : 1, [&](int b, int e, int) {  // small levels: one thread
+                for (int t = b; t < e; ++t)
+                    for (int id : ib[t]) {
+                        Cluster& C2 = cl[ed[id].n2];
+                        C2.in.push_back(id);
+                        if (ed[id].spars) C2.sin.push_back(id);
+                    }
+            });
         }
         hacc(3, ht);
         {
