@@ -1007,6 +1007,21 @@ __global__ __launch_bounds__(WG) void k_panel_apply(const PanelApplyTask<S>* __r
     wg_apply_lt<S, PENT>(t.m, t.kb, t.V, t.ldv, t.T, NB, t.Atop, t.lda, t.C, t.ldc, t.ncols, asm_);
 }
 
+// Look-ahead step of the host-driven blocked QR: a workgroup applies the previous panel to
+// its columns and, when they hold the next panel, factors that panel at once; the other
+// workgroups of the launch update the remaining trailing columns meanwhile.
+template <typename S>
+__global__ __launch_bounds__(WG) void k_panel_step(const PanelStepTask<S>* __restrict__ tasks) {
+    __shared__ ApplyPanelSmem<S> sm_;
+    const PanelStepTask<S> t = tasks[blockIdx.x];
+    if (t.apply)
+        wg_apply_lt<S, false>(t.a.m, t.a.kb, t.a.V, t.a.ldv, t.a.T, NB, nullptr, 0, t.a.C, t.a.ldc, t.a.ncols, sm_.ap);
+    if (t.qr) {
+        __syncthreads();
+        wg_panel_qr<S, false>(t.q.m, t.q.nbp, t.q.A, t.q.lda, t.q.A, t.q.lda, t.q.T, NB, sm_.pn);
+    }
+}
+
 template <typename S>
 __global__ __launch_bounds__(WG) void k_larfb(const LarfbTask<S>* __restrict__ tasks) {
     __shared__ ApplySmem<S> asm_;
@@ -2562,6 +2577,10 @@ void launch_panel_qr(const PanelTask<S>* d, int n, bool pent, hipStream_t s) {
     else hipLaunchKernelGGL((k_panel_qr<S, false>), dim3(n), dim3(WG), 0, s, d);
 }
 template <typename S>
+void launch_panel_step(const PanelStepTask<S>* d, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_panel_step<S>, dim3(n), dim3(WG), 0, s, d);
+}
+template <typename S>
 void launch_panel_apply(const PanelApplyTask<S>* d, int n, bool pent, hipStream_t s) {
     if (n <= 0) return;
     if (pent) hipLaunchKernelGGL((k_panel_apply<S, true>), dim3(n), dim3(WG), 0, s, d);
@@ -2750,6 +2769,7 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
     template void launch_larfb<S>(const LarfbTask<S>*, int, hipStream_t);                            \
     template void launch_panel_qr<S>(const PanelTask<S>*, int, bool, hipStream_t);                    \
     template void launch_panel_apply<S>(const PanelApplyTask<S>*, int, bool, hipStream_t);           \
+    template void launch_panel_step<S>(const PanelStepTask<S>*, int, hipStream_t);                    \
     template void launch_tpqrt<S>(const TpqrtTask<S>*, const TpqrtLink<S>*, int, hipStream_t);       \
     template void launch_tpqrt_wave<S>(const TpqrtTask<S>*, const TpqrtLink<S>*, int, hipStream_t);       \
     template void launch_tpqrt_coop<S>(const TpqrtCoopTask<S>*, const int*, int, const TpqrtLink<S>*, int*, hipStream_t); \

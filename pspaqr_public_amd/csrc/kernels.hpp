@@ -64,6 +64,9 @@ template <typename S> struct PanelTask { S* A; S* B; S* T; int m, nbp, lda, ldb;
 // [Atop (kb rows, lda); C (m rows, ldc)] <- H^T [...] over ncols columns (Atop unused when
 // PENT = false); V (m x kb, ldv) and T (kb x kb, ld NB) are the panel's reflectors.
 template <typename S> struct PanelApplyTask { const S* V; const S* T; S* Atop; S* C; int m, kb, ldv, lda, ldc, ncols; };
+// One step of a host-driven blocked QR: the previous panel applied to some columns, then
+// (qr != 0) the panel in those columns factored in the same workgroup.
+template <typename S> struct PanelStepTask { PanelApplyTask<S> a; PanelTask<S> q; int apply, qr; };
 
 // Gather/scatter of a sub-block: dst(i, j) = src(i, j) or src(j, i) (transpose).
 template <typename S> struct CopyTask { const S* src; S* dst; int rows, cols, lds, ldd, trans, pad; };

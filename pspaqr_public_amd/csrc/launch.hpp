@@ -29,6 +29,7 @@ void launch_spars_res(const SparsApplyTask<float>* d, int n, int mt, bool right,
 void launch_tpmqrt_res(const TpmqrtTask<float>* d, const TpmqrtLink<float>* l, int n, int mt, hipStream_t s);
 template <typename S> void launch_panel_qr(const PanelTask<S>* d, int n, bool pent, hipStream_t s);
 template <typename S> void launch_panel_apply(const PanelApplyTask<S>* d, int n, bool pent, hipStream_t s);
+template <typename S> void launch_panel_step(const PanelStepTask<S>* d, int n, hipStream_t s);
 template <typename S> void launch_tpqrt(const TpqrtTask<S>* d, const TpqrtLink<S>* l, int n, hipStream_t s);
 // Chains with k <= 64 and links of at most 64 rows: one 64-thread workgroup per chain.
 template <typename S> void launch_tpqrt_wave(const TpqrtTask<S>* d, const TpqrtLink<S>* l, int n, hipStream_t s);

@@ -1634,10 +1634,12 @@ int Tree::factorize_t() {
 
 // ---------------------------------------------------------------------------- large fronts
 // A front of k >= kBigFront columns would keep one workgroup busy for tens of milliseconds
-// (its panels are sequential).  Its panel loop runs on the host instead: per 32-column panel
-// one launch factors the panel of every large front (one workgroup each) and one launch
-// applies it to the trailing columns in 128-column chunks across many workgroups.  The
-// arithmetic per column is that of the one-workgroup kernels.
+// (its panels are sequential).  Its panel loop runs on the host instead, one launch per
+// 32-column panel with look-ahead: for every large front one workgroup applies the previous
+// panel to this panel's columns and factors them, while other workgroups apply the previous
+// panel to the remaining trailing columns in 128-column chunks.  A last launch applies the
+// final panel of the wide fronts to their columns past min(m, n).  The arithmetic per
+// column is that of the one-workgroup kernels.
 
 template <typename S>
 int Tree::geqrt_big(const std::vector<GeqrtTask<S>>& big) {
@@ -1645,33 +1647,57 @@ int Tree::geqrt_big(const std::vector<GeqrtTask<S>>& big) {
     int kmax = 0;
     for (const auto& t : big) kmax = std::max(kmax, std::min(t.m, t.n));
     // every step's descriptors are built and uploaded at once; step s launches its slice
-    std::vector<PanelTask<S>> pq;
-    std::vector<PanelApplyTask<S>> pa;
-    std::vector<int> qoff, aoff;
-    for (int p0 = 0; p0 < kmax; p0 += NB) {
-        qoff.push_back((int)pq.size());
-        aoff.push_back((int)pa.size());
+    std::vector<PanelStepTask<S>> ps;
+    std::vector<int> off;
+    const int nsteps = (kmax + NB - 1) / NB + 1;
+    for (int s = 0; s < nsteps; ++s) {
+        off.push_back((int)ps.size());
+        const int p0 = s * NB, pp = p0 - NB;  // this panel, the previous one
+        std::vector<PanelStepTask<S>> rest;
         for (const auto& t : big) {
             const int kmin = std::min(t.m, t.n);
-            if (p0 >= kmin) continue;
-            const int nbp = std::min(NB, kmin - p0);
-            S* Ap = t.A + p0 + (size_t)p0 * t.lda;
-            pq.push_back({Ap, Ap, t.T + (size_t)p0 * NB, t.m - p0, nbp, t.lda, t.lda});
-            for (int c0 = p0 + nbp; c0 < t.n; c0 += 128)
-                pa.push_back({Ap, t.T + (size_t)p0 * NB, nullptr, t.A + p0 + (size_t)c0 * t.lda, t.m - p0, nbp, t.lda, 0,
-                              t.lda, std::min(128, t.n - c0)});
+            const bool prev = s > 0 && pp < kmin, cur = p0 < kmin;
+            PanelApplyTask<S> a{};
+            if (prev) {
+                const int nbq = std::min(NB, kmin - pp);
+                a = {t.A + pp + (size_t)pp * t.lda, t.T + (size_t)pp * NB, nullptr, nullptr, t.m - pp, nbq, t.lda, 0, t.lda, 0};
+            }
+            int c0 = p0;  // first column the previous panel still has to reach
+            if (cur) {
+                const int nbp = std::min(NB, kmin - p0);
+                PanelStepTask<S> u{};
+                u.apply = prev;
+                if (prev) {
+                    u.a = a;
+                    u.a.C = t.A + pp + (size_t)p0 * t.lda;
+                    u.a.ncols = nbp;
+                }
+                u.qr = 1;
+                S* Ap = t.A + p0 + (size_t)p0 * t.lda;
+                u.q = {Ap, Ap, t.T + (size_t)p0 * NB, t.m - p0, nbp, t.lda, t.lda};
+                ps.push_back(u);
+                c0 = p0 + nbp;
+            }
+            if (prev)
+                for (; c0 < t.n; c0 += 128) {
+                    PanelStepTask<S> u{};
+                    u.apply = 1;
+                    u.a = a;
+                    u.a.C = t.A + pp + (size_t)c0 * t.lda;
+                    u.a.ncols = std::min(128, t.n - c0);
+                    rest.push_back(u);
+                }
         }
+        ps.insert(ps.end(), rest.begin(), rest.end());  // the panels first: they start the next step
     }
-    qoff.push_back((int)pq.size());
-    aoff.push_back((int)pa.size());
-    auto* dq = upload_vec(pq);
-    auto* da = upload_vec(pa);
+    off.push_back((int)ps.size());
+    auto* d = upload_vec(ps);
     int launches = 0;
-    for (size_t st = 0; st + 1 < qoff.size(); ++st) {
-        launch_panel_qr<S>(dq + qoff[st], qoff[st + 1] - qoff[st], false, stream_);
-        launch_panel_apply<S>(da + aoff[st], aoff[st + 1] - aoff[st], false, stream_);
-        launches += 1 + (aoff[st + 1] > aoff[st]);
-    }
+    for (size_t st = 0; st + 1 < off.size(); ++st)
+        if (off[st + 1] > off[st]) {
+            launch_panel_step<S>(d + off[st], off[st + 1] - off[st], stream_);
+            ++launches;
+        }
     return launches;
 }
 template int Tree::geqrt_big<float>(const std::vector<GeqrtTask<float>>&);
