@@ -32,7 +32,6 @@
 #include <stdexcept>
 #include <string>
 #include <tuple>
-#include <unordered_set>
 
 #include "kernels.hpp"
 #include "launch.hpp"
@@ -724,11 +723,13 @@ void Tree::fillin() {
     }
     parallel_ranges(T, ncand < 16384 ? T : 1, [&](int b, int e, int) {
         for (int t = b; t < e; ++t) {
-            std::unordered_set<long long> seen;
+            size_t nt = 0;
+            for (int s2 = 0; s2 < ns; ++s2) nt += byt[s2][t].size();
+            FlatKeySet seen(nt);
             for (int s2 = 0; s2 < ns; ++s2)
                 for (int j : byt[s2][t]) {
                     const Cand& c = cand[s2][j];
-                    if (seen.insert(((long long)c.first << 32) | (unsigned)c.second).second) keep[s2][j] = 1;
+                    if (seen.insert(((unsigned long long)c.first << 32) | (unsigned)c.second)) keep[s2][j] = 1;
                 }
         }
     });
@@ -737,44 +738,52 @@ void Tree::fillin() {
     // in lists by target (each list appended in id order), sources and targets split over
     // the threads by cluster number
     const size_t base = ed.size();
-    std::vector<Cand> made;
-    std::vector<std::vector<int>> ob(T), ib(T);
+    std::vector<size_t> koff(ns + 1, 0);
     for (int s2 = 0; s2 < ns; ++s2)
-        for (size_t j = 0; j < cand[s2].size(); ++j)
-            if (keep[s2][j]) {
-                const Cand c = cand[s2][j];
-                ob[c.first % T].push_back((int)made.size());
-                if (c.first != c.second) ib[c.second % T].push_back((int)made.size());
-                made.push_back(c);
-            }
-    const size_t nnew = made.size();
+        koff[s2 + 1] = koff[s2] + (size_t)std::count(keep[s2].begin(), keep[s2].end(), (char)1);
+    const size_t nnew = koff[ns];
+    std::vector<Cand> made(nnew);
+    std::vector<std::vector<std::vector<int>>> obs(ns, std::vector<std::vector<int>>(T)), ibs = obs;
+    parallel_ranges(ns, 1, [&](int b, int e, int) {
+        for (int s2 = b; s2 < e; ++s2) {
+            size_t i = koff[s2];
+            for (size_t j = 0; j < cand[s2].size(); ++j)
+                if (keep[s2][j]) {
+                    const Cand c = cand[s2][j];
+                    obs[s2][c.first % T].push_back((int)i);
+                    if (c.first != c.second) ibs[s2][c.second % T].push_back((int)i);
+                    made[i++] = c;
+                }
+        }
+    });
     ed.resize(base + nnew);
     tick();
     parallel_ranges(T, nnew < 16384 ? T : 1, [&](int b, int e, int) {
-        for (int t = b; t < e; ++t) {
-            for (int i : ob[t]) {
-                const int c1 = made[i].first, c2 = made[i].second, id = (int)(base + i);
-                Edge& E = ed[id];
-                E.n1 = c1;
-                E.n2 = c2;
-                E.rows = cl[c2].rows;
-                E.cols = cl[c1].cols;
-                E.spars = cl[c1].lvl() > current_bottom && cl[c2].lvl() > current_bottom && c1 != c2;
-                Cluster& C1 = cl[c1];
-                C1.out.push_back(id);
-                if (E.spars) C1.sout.push_back(id);
-                if (c1 == c2) C1.self_edge = id;
-            }
-        }
+        for (int t = b; t < e; ++t)
+            for (int s2 = 0; s2 < ns; ++s2)
+                for (int i : obs[s2][t]) {
+                    const int c1 = made[i].first, c2 = made[i].second, id = (int)(base + i);
+                    Edge& E = ed[id];
+                    E.n1 = c1;
+                    E.n2 = c2;
+                    E.rows = cl[c2].rows;
+                    E.cols = cl[c1].cols;
+                    E.spars = cl[c1].lvl() > current_bottom && cl[c2].lvl() > current_bottom && c1 != c2;
+                    Cluster& C1 = cl[c1];
+                    C1.out.push_back(id);
+                    if (E.spars) C1.sout.push_back(id);
+                    if (c1 == c2) C1.self_edge = id;
+                }
     });
     parallel_ranges(T, nnew < 16384 ? T : 1, [&](int b, int e, int) {
         for (int t = b; t < e; ++t)
-            for (int i : ib[t]) {
-                const int id = (int)(base + i);
-                Cluster& C2 = cl[made[i].second];
-                C2.in.push_back(id);
-                if (ed[id].spars) C2.sin.push_back(id);
-            }
+            for (int s2 = 0; s2 < ns; ++s2)
+                for (int i : ibs[s2][t]) {
+                    const int id = (int)(base + i);
+                    Cluster& C2 = cl[made[i].second];
+                    C2.in.push_back(id);
+                    if (ed[id].spars) C2.sin.push_back(id);
+                }
     });
     tick();
     if (dbg)

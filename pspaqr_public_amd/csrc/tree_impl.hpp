@@ -130,6 +130,38 @@ inline int host_threads() {
     return n;
 }
 
+// Open-addressing set of 64-bit keys (no removal; the all-ones key is reserved), sized for
+// n keys: the planner's duplicate checks without a node allocation per insert.
+class FlatKeySet {
+  public:
+    explicit FlatKeySet(size_t n) {
+        size_t c = 16;
+        bits_ = 4;
+        while (c < 2 * n + 16) {
+            c <<= 1;
+            ++bits_;
+        }
+        slots_.assign(c, kEmpty);
+    }
+    bool insert(unsigned long long k) {  // true when k was not in the set
+        const size_t mask = slots_.size() - 1;
+        size_t h = (size_t)((k * 0x9E3779B97F4A7C15ull) >> (64 - bits_));
+        while (true) {
+            if (slots_[h] == kEmpty) {
+                slots_[h] = k;
+                return true;
+            }
+            if (slots_[h] == k) return false;
+            h = (h + 1) & mask;
+        }
+    }
+
+  private:
+    static constexpr unsigned long long kEmpty = ~0ull;
+    std::vector<unsigned long long> slots_;
+    int bits_;
+};
+
 // Persistent workers for parallel_ranges: a planner level makes several short parallel passes,
 // and starting threads for each cost more than the passes.  run(T, job) runs job(t) for t in
 // [1, T) on the workers and job(0) on the caller, and returns when all have finished; callers
