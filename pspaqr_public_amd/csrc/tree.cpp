@@ -1263,6 +1263,35 @@ int Tree::factorize_t() {
             prof_add(P_SAPPLY, P.pflops, P.pbytes);
         }
         hacc(19, ht);
+        // ---- upload and launch the scaling; the RRQR descriptors are built while it runs
+        {
+            std::vector<GeqrtTask<S>> sgeq_small, sgeq_big;
+            for (const auto& t : sgeq) (std::min(t.m, t.n) >= kBigFront ? sgeq_big : sgeq_small).push_back(t);
+            const GeqrtBatch sgeq_b = prep_geqrt(sgeq_small);
+            std::vector<LarfbBuckets> sapp_c(nsp), sapp_t(nsp);
+            for (int i = 0; i < nsp; ++i) {
+                sapp_c[i] = std::move(sparts[i].c);
+                sapp_t[i] = std::move(sparts[i].t);
+            }
+            CopyTask<S>* d_scopy = upload_vec(scopy);
+            const LarfbBatch sapp_b = prep_larfb_parts(sapp_c, false), sapp_tb = prep_larfb_parts(sapp_t, true);
+            hacc(17, ht);
+            if (!sgeq.empty()) {
+                int nl = 0;
+                timed(P_SGEQRT, 0, (long)sgeq.size(), [&] {
+                    launch_copy<S>(d_scopy, (int)scopy.size(), stream_);
+                    run_geqrt(sgeq_b);
+                    nl = 1 + sgeq_b.launches() + geqrt_big<S>(sgeq_big);
+                });
+                if (profiling) prof[P_SGEQRT].launches += nl;
+            }
+            if (nsapp)
+                timed(P_SAPPLY, sapp_b.launches() + sapp_tb.launches(), nsapp, [&] {
+                    run_larfb(sapp_b);
+                    run_larfb(sapp_tb);
+                });
+            HIPCHK(hipGetLastError());
+        }
         // ---- RRQR descriptors (src/ptree.cpp:373-439)
         std::vector<CopyTask<S>> gat;
         std::vector<int> gat_begin;  // first gather copy of each RRQR task
@@ -1308,32 +1337,7 @@ int Tree::factorize_t() {
         }
         hacc(16, ht);
         {
-            std::vector<GeqrtTask<S>> sgeq_small, sgeq_big;
-            for (const auto& t : sgeq) (std::min(t.m, t.n) >= kBigFront ? sgeq_big : sgeq_small).push_back(t);
-            const GeqrtBatch sgeq_b = prep_geqrt(sgeq_small);
-            std::vector<LarfbBuckets> sapp_c(nsp), sapp_t(nsp);
-            for (int i = 0; i < nsp; ++i) {
-                sapp_c[i] = std::move(sparts[i].c);
-                sapp_t[i] = std::move(sparts[i].t);
-            }
-            CopyTask<S>* d_scopy = upload_vec(scopy);
-            const LarfbBatch sapp_b = prep_larfb_parts(sapp_c, false), sapp_tb = prep_larfb_parts(sapp_t, true);
             CopyTask<S>* d_gat = upload_vec(gat);
-            hacc(17, ht);
-            if (!sgeq.empty()) {
-                int nl = 0;
-                timed(P_SGEQRT, 0, (long)sgeq.size(), [&] {
-                    launch_copy<S>(d_scopy, (int)scopy.size(), stream_);
-                    run_geqrt(sgeq_b);
-                    nl = 1 + sgeq_b.launches() + geqrt_big<S>(sgeq_big);
-                });
-                if (profiling) prof[P_SGEQRT].launches += nl;
-            }
-            if (nsapp)
-                timed(P_SAPPLY, sapp_b.launches() + sapp_tb.launches(), nsapp, [&] {
-                    run_larfb(sapp_b);
-                    run_larfb(sapp_tb);
-                });
             if (!gat.empty()) timed(P_GATHER, 1, (long)gat.size(), [&] { launch_copy<S>(d_gat, (int)gat.size(), stream_); });
             if (!gq.empty()) {
                 RrqrAlloc mem{[&](size_t b) { return arena.alloc(b); },
@@ -1609,6 +1613,17 @@ int Tree::factorize_t() {
         if (nspa) timed(P_SPARS, sb2.launches(), nspa, [&] { run_spars(sb2); });
         HIPCHK(hipGetLastError());
         hacc(4, ht);
+        if (dbg_host && L < 3) {
+            static double prev[24];
+            if (L == 0) std::fill(prev, prev + 24, 0.0);
+            fprintf(stderr, "[host L%d] fillin %.2f S1 %.2f colour %.2f S2 %.2f launch %.2f scaling %.2f sapp %.2f rrqr %.2f "
+                            "prep %.2f spa1 %.2f parents %.2f nbrs %.2f edges %.2f blocks %.2f spars %.2f upload %.2f ms\n",
+                    L, hms[0] - prev[0], hms[1] - prev[1], hms[10] - prev[10], hms[11] - prev[11], hms[2] - prev[2],
+                    hms[18] - prev[18], hms[19] - prev[19], hms[16] - prev[16], hms[17] - prev[17], hms[12] - prev[12],
+                    hms[13] - prev[13], hms[14] - prev[14], hms[3] - prev[3], hms[7] - prev[7], hms[8] - prev[8],
+                    hms[9] - prev[9]);
+            std::copy(hms, hms + 24, prev);
+        }
     }
     auto ht2 = hclock();
     {
