@@ -1458,6 +1458,17 @@ int Tree::factorize_t() {
             const auto& bl = bottoms[Lp];
             const int np = (int)bl.size();
             std::vector<std::vector<int>> pd2(np), pnb(np);
+            // compact per-cluster lookups (the passes below touch clusters at random)
+            std::vector<int> par(cl.size()), lvl(cl.size()), rows_of(cl.size());
+            std::vector<char> elim(cl.size());
+            parallel_ranges((int)cl.size(), 4096, [&](int b, int e, int) {
+                for (int i = b; i < e; ++i) {
+                    par[i] = cl[i].parent;
+                    lvl[i] = cl[i].lvl();
+                    rows_of[i] = cl[i].rows;
+                    elim[i] = cl[i].eliminated;
+                }
+            });
             parallel_ranges(np, 64, [&](int b, int e, int) {
                 std::vector<int> seen_n(cl.size(), -1), seen_d(cl.size(), -1);
                 for (int i = b; i < e; ++i) {
@@ -1469,13 +1480,13 @@ int Tree::factorize_t() {
                     for (int ch : cl[p].children) {
                         for (int e2 : cl[ch].out) {
                             const int n2 = ed[e2].n2;
-                            if (cl[n2].eliminated) continue;
-                            const int q = cl[n2].parent;
+                            if (elim[n2]) continue;
+                            const int q = par[n2];
                             if (seen_n[q] != p) { seen_n[q] = p; nbrs.push_back(q); }
                         }
                         for (int d : cl[ch].dist2) {
-                            if (cl[d].lvl() < Lp) continue;
-                            const int q = cl[d].parent;
+                            if (lvl[d] < Lp) continue;
+                            const int q = par[d];
                             if (seen_d[q] != p) { seen_d[q] = p; d2.push_back(q); }
                         }
                     }
@@ -1503,9 +1514,9 @@ int Tree::factorize_t() {
                         Edge& E = ed[id];
                         E.n1 = p;
                         E.n2 = nn;
-                        E.rows = cl[nn].rows;
+                        E.rows = rows_of[nn];
                         E.cols = P.cols;
-                        E.spars = P.lvl() > current_bottom && cl[nn].lvl() > current_bottom && p != nn;
+                        E.spars = lvl[p] > current_bottom && lvl[nn] > current_bottom && p != nn;
                         P.out.push_back(id);
                         if (E.spars) P.sout.push_back(id);
                         if (p == nn) P.self_edge = id;
