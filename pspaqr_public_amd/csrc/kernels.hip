@@ -13,6 +13,7 @@
 // one 256-thread workgroup, so every front of a level is factored in one grouped launch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cstdint>
 #include <type_traits>
@@ -1411,14 +1412,37 @@ __global__ __launch_bounds__(WG) void k_tpmqrt(const TpmqrtTask<S>* __restrict__
     }
 }
 
+// dst <- src (trans: dst <- src^T) by 32 x 32 tiles, coalesced on both sides (a transposed
+// tile goes through LDS).  Workgroup (task, part) takes the task's tiles q = part mod parts,
+// so large blocks spread over several workgroups.
 template <typename S>
 __global__ __launch_bounds__(WG) void k_copy(const CopyTask<S>* __restrict__ tasks) {
+    __shared__ S tile[32][33];
     const CopyTask<S> t = tasks[blockIdx.x];
-    const int total = t.rows * t.cols;
-    for (int e = threadIdx.x; e < total; e += WG) {
-        int i = e % t.rows, j = e / t.rows;
-        S v = t.trans ? t.src[j + (size_t)i * t.lds] : t.src[i + (size_t)j * t.lds];
-        t.dst[i + (size_t)j * t.ldd] = v;
+    const int tr = (t.rows + 31) / 32, tc = (t.cols + 31) / 32, nt = tr * tc;
+    const int tid = threadIdx.x, lx = tid & 31, ly = tid >> 5;  // 32 x (WG / 32)
+    constexpr int LY = WG / 32;
+    for (int q = blockIdx.y; q < nt; q += gridDim.y) {
+        const int i0 = (q % tr) * 32, j0 = (q / tr) * 32;
+        if (!t.trans) {
+            const int i = i0 + lx;
+            for (int jj = ly; jj < 32; jj += LY) {
+                const int j = j0 + jj;
+                if (i < t.rows && j < t.cols) t.dst[i + (size_t)j * t.ldd] = t.src[i + (size_t)j * t.lds];
+            }
+        } else {
+            // dst(i, j) = src(j, i): src is read along j, dst written along i
+            for (int ii = ly; ii < 32; ii += LY) {
+                const int i = i0 + ii, j = j0 + lx;
+                if (i < t.rows && j < t.cols) tile[ii][lx] = t.src[j + (size_t)i * t.lds];
+            }
+            __syncthreads();
+            for (int jj = ly; jj < 32; jj += LY) {
+                const int i = i0 + lx, j = j0 + jj;
+                if (i < t.rows && j < t.cols) t.dst[i + (size_t)j * t.ldd] = tile[lx][jj];
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -2630,8 +2654,8 @@ void launch_tpmqrt(const TpmqrtTask<S>* d, const TpmqrtLink<S>* l, int n, hipStr
     if (n > 0) hipLaunchKernelGGL(k_tpmqrt<S>, dim3(n), dim3(WG), 0, s, d, l);
 }
 template <typename S>
-void launch_copy(const CopyTask<S>* d, int n, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_copy<S>, dim3(n), dim3(WG), 0, s, d);
+void launch_copy(const CopyTask<S>* d, int n, hipStream_t s, int parts) {
+    if (n > 0) hipLaunchKernelGGL(k_copy<S>, dim3(n, std::max(1, std::min(parts, 64))), dim3(WG), 0, s, d);
 }
 template <typename S>
 void launch_geqp3(const Geqp3Task<S>* d, int n, double tol, double tie_rel, hipStream_t s) {
@@ -2777,7 +2801,7 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
     template void launch_tpmqrt_wave<S>(const TpmqrtTask<S>*, const TpmqrtLink<S>*, int, int, hipStream_t); \
     template int tpqrt_coop_max_blocks<S>();                                                          \
     template void launch_tpmqrt<S>(const TpmqrtTask<S>*, const TpmqrtLink<S>*, int, hipStream_t);    \
-    template void launch_copy<S>(const CopyTask<S>*, int, hipStream_t);                              \
+    template void launch_copy<S>(const CopyTask<S>*, int, hipStream_t, int);                         \
     template void launch_geqp3<S>(const Geqp3Task<S>*, int, double, double, hipStream_t);                    \
     template void launch_spars_apply<S>(const SparsApplyTask<S>*, int, hipStream_t);                 \
     template void launch_larfb_small<S>(const LarfbTask<S>*, int, int, bool, hipStream_t);            \

@@ -44,7 +44,8 @@ template <typename S> void launch_tpmqrt(const TpmqrtTask<S>* d, const TpmqrtLin
 // class 16 / 32 / 64 >= the longest link): one 64-thread workgroup each.
 template <typename S>
 void launch_tpmqrt_wave(const TpmqrtTask<S>* d, const TpmqrtLink<S>* l, int n, int mr, hipStream_t s);
-template <typename S> void launch_copy(const CopyTask<S>* d, int n, hipStream_t s);
+// parts: workgroups per task (large blocks spread over several)
+template <typename S> void launch_copy(const CopyTask<S>* d, int n, hipStream_t s, int parts = 1);
 template <typename S> void launch_geqp3(const Geqp3Task<S>* d, int n, double tol, double tie_rel, hipStream_t s);
 template <typename S> void launch_spars_apply(const SparsApplyTask<S>* d, int n, hipStream_t s);
 // Blocks of at most 64 x 64 (mr = register length class 16 / 32 / 64 >= the reflector

@@ -1279,7 +1279,7 @@ int Tree::factorize_t() {
             if (!sgeq.empty()) {
                 int nl = 0;
                 timed(P_SGEQRT, 0, (long)sgeq.size(), [&] {
-                    launch_copy<S>(d_scopy, (int)scopy.size(), stream_);
+                    launch_copy<S>(d_scopy, (int)scopy.size(), stream_, copy_parts(scopy));
                     run_geqrt(sgeq_b);
                     nl = 1 + sgeq_b.launches() + geqrt_big<S>(sgeq_big);
                 });
@@ -1338,7 +1338,7 @@ int Tree::factorize_t() {
         hacc(16, ht);
         {
             CopyTask<S>* d_gat = upload_vec(gat);
-            if (!gat.empty()) timed(P_GATHER, 1, (long)gat.size(), [&] { launch_copy<S>(d_gat, (int)gat.size(), stream_); });
+            if (!gat.empty()) timed(P_GATHER, 1, (long)gat.size(), [&] { launch_copy<S>(d_gat, (int)gat.size(), stream_, copy_parts(gat)); });
             if (!gq.empty()) {
                 RrqrAlloc mem{[&](size_t b) { return arena.alloc(b); },
                               [&](const void* h, size_t b) { return upload(h, b); }};

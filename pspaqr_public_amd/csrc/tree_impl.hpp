@@ -162,6 +162,15 @@ class FlatKeySet {
     int bits_;
 };
 
+// Workgroups per copy task: enough that the largest block of the launch is spread at about
+// 16 tiles of 32 x 32 per workgroup.
+template <typename S>
+inline int copy_parts(const std::vector<CopyTask<S>>& v) {
+    long most = 0;
+    for (const auto& t : v) most = std::max(most, (long)((t.rows + 31) / 32) * ((t.cols + 31) / 32));
+    return (int)std::max(1L, std::min(64L, (most + 15) / 16));
+}
+
 // Persistent workers for parallel_ranges: a planner level makes several short parallel passes,
 // and starting threads for each cost more than the passes.  run(T, job) runs job(t) for t in
 // [1, T) on the workers and job(0) on the caller, and returns when all have finished; callers

@@ -155,7 +155,7 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
                 for (int q : todo)
                     for (int g = gat_begin[q]; g < gat_begin[q + 1]; ++g) regather.push_back(gat[g]);
                 auto* d_rg = upload_vec(regather);
-                timed(P_GATHER, 1, (long)regather.size(), [&] { launch_copy<S>(d_rg, (int)regather.size(), stream_); });
+                timed(P_GATHER, 1, (long)regather.size(), [&] { launch_copy<S>(d_rg, (int)regather.size(), stream_, copy_parts(regather)); });
             }
             static const int unpiv_min = getenv("SPAQR_SKETCH_UNPIVOTED") ? atoi(getenv("SPAQR_SKETCH_UNPIVOTED"))
                                                                             : kSketchUnpivoted;
@@ -289,7 +289,7 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
         }
         if (!regather.empty()) {
             auto* d_rg = upload_vec(regather);
-            timed(P_GATHER, 1, (long)regather.size(), [&] { launch_copy<S>(d_rg, (int)regather.size(), stream_); });
+            timed(P_GATHER, 1, (long)regather.size(), [&] { launch_copy<S>(d_rg, (int)regather.size(), stream_, copy_parts(regather)); });
         }
         int nd = 0;
         timed(P_GEQP3, 0, (long)gf.size(),
