@@ -2696,7 +2696,7 @@ void launch_fill(S* p, size_t n, S v, hipStream_t s) {
 }
 // Launches of at most kVecWideTasks tasks (the top levels' few, large fronts) run FP32 tasks on
 // 1024-thread workgroups: a lone task then has 16 waves of loads in flight on its CU.
-constexpr int kVecWideTasks = 64;
+constexpr int kVecWideTasks = 128;
 template <typename S>
 void launch_vec_q(const VecQTask<S>* d, int n, hipStream_t s) {
     if (n <= 0) return;
