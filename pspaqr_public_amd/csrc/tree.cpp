@@ -1010,39 +1010,75 @@ int Tree::factorize_t() {
         st.n_interiors = (int)ints.size();
         st.n_interfaces = (int)itf.size();
 
-        // ---- elimination descriptors
-        std::vector<GeqrtTask<S>> geq;
-        std::vector<LarfbTask<S>> lfb;
-        std::vector<TpqrtTask<S>> tpq;
-        std::vector<TpqrtLink<S>> tpql;
-        for (int c : ints) {
+        // ---- elimination descriptors: T blocks allocated in interior order first (a bump
+        // allocator), then the tasks built over the interiors in parallel into per-slot lists
+        // (links at per-interior offsets, larfb chunks straight into launch buckets),
+        // concatenated in slot order: the sequential order.
+        std::vector<int> lnk0(ints.size() + 1, 0);
+        for (size_t i = 0; i < ints.size(); ++i) {
+            const int c = ints[i];
             Cluster& C = cl[c];
             if (C.cols == 0) throw std::runtime_error("cluster has zero cols");
             if (C.out.empty() || ed[C.out[0]].n2 != c) throw std::runtime_error("first out-edge of an interior must be itself");
             st.max_front = std::max(st.max_front, C.cols);
             C.T = arena.alloc((size_t)NB * C.cols * sizeof(S));
-            Edge& es = E(C.self_edge);
-            geq.push_back({Ap(C.self_edge), (S*)C.T, es.rows, es.cols, es.rows, 0});
-            st.flops += flops_geqrt<S>(es.rows, es.cols);
-            prof_add(P_GEQRT, flops_geqrt<S>(es.rows, es.cols), SZ * (2.0 * es.rows * es.cols + NB * es.cols));
-            for (int e : C.in) {
-                Edge& ei = E(e);
-                lfb.push_back({Ap(C.self_edge), (S*)C.T, Ap(e), nullptr, ei.rows, ei.cols, C.cols, es.rows, ei.rows, 0, 0, 0});
-                st.flops += 4.0 * ei.rows * ei.cols * C.cols - 2.0 * ei.cols * C.cols * C.cols;
-                prof_add(P_LARFB, 4.0 * ei.rows * ei.cols * C.cols - 2.0 * ei.cols * C.cols * C.cols,
-                         SZ * ((double)ei.rows * C.cols + NB * C.cols + 2.0 * ei.rows * ei.cols));
+            for (size_t j = 1; j < C.out.size(); ++j) ed[C.out[j]].T = arena.alloc((size_t)NB * C.cols * sizeof(S));
+            lnk0[i + 1] = lnk0[i] + (int)C.out.size() - 1;
+        }
+        struct ElimPart {
+            std::vector<GeqrtTask<S>> geq;
+            std::vector<TpqrtTask<S>> tpq;
+            LarfbBuckets lc, lt;
+            long nlfb = 0;
+            double flops = 0.0;
+            double pf[3] = {0.0, 0.0, 0.0}, pb[3] = {0.0, 0.0, 0.0};  // geqrt, larfb, tpqrt
+        };
+        std::vector<ElimPart> eparts(host_threads());
+        std::vector<TpqrtLink<S>> tpql((size_t)lnk0[ints.size()]);
+        const int nep = parallel_ranges((int)ints.size(), 64, [&](int b, int e, int slot) {
+            ElimPart& P = eparts[slot];
+            for (int i = b; i < e; ++i) {
+                const Cluster& C = cl[ints[i]];
+                const Edge& es = ed[C.self_edge];
+                P.geq.push_back({Ap(C.self_edge), (S*)C.T, es.rows, es.cols, es.rows, 0});
+                const double fg = flops_geqrt<S>(es.rows, es.cols);
+                P.flops += fg;
+                P.pf[0] += fg;
+                P.pb[0] += SZ * (2.0 * es.rows * es.cols + NB * es.cols);
+                for (int e2 : C.in) {
+                    const Edge& ei = ed[e2];
+                    file_larfb({Ap(C.self_edge), (S*)C.T, Ap(e2), nullptr, ei.rows, ei.cols, C.cols, es.rows, ei.rows, 0, 0, 0},
+                               P.lc, P.lt);
+                    ++P.nlfb;
+                    const double f = 4.0 * ei.rows * ei.cols * C.cols - 2.0 * ei.cols * C.cols * C.cols;
+                    P.flops += f;
+                    P.pf[1] += f;
+                    P.pb[1] += SZ * ((double)ei.rows * C.cols + NB * C.cols + 2.0 * ei.rows * ei.cols);
+                }
+                TpqrtTask<S> tk{Ap(C.self_edge), C.cols, es.rows, lnk0[i], (int)C.out.size() - 1};
+                for (size_t j = 1; j < C.out.size(); ++j) {
+                    const Edge& eo = ed[C.out[j]];
+                    tpql[lnk0[i] + j - 1] = {Ap(C.out[j]), (S*)eo.T, eo.rows, eo.rows};
+                    const double f = 2.0 * eo.rows * C.cols * C.cols;
+                    P.flops += f;
+                    P.pf[2] += f;
+                    P.pb[2] += SZ * (2.0 * eo.rows * C.cols + (double)C.cols * C.cols + NB * C.cols);
+                }
+                P.tpq.push_back(tk);
             }
-            TpqrtTask<S> tk{Ap(C.self_edge), C.cols, es.rows, (int)tpql.size(), 0};
-            for (size_t j = 1; j < C.out.size(); ++j) {
-                Edge& eo = E(C.out[j]);
-                eo.T = arena.alloc((size_t)NB * C.cols * sizeof(S));
-                tpql.push_back({Ap(C.out[j]), (S*)eo.T, eo.rows, eo.rows});
-                st.flops += 2.0 * eo.rows * C.cols * C.cols;
-                prof_add(P_TPQRT, 2.0 * eo.rows * C.cols * C.cols,
-                         SZ * (2.0 * eo.rows * C.cols + (double)C.cols * C.cols + NB * C.cols));
-            }
-            tk.nlinks = (int)tpql.size() - tk.link0;
-            tpq.push_back(tk);
+        });
+        eparts.resize(nep);
+        std::vector<GeqrtTask<S>> geq;
+        std::vector<TpqrtTask<S>> tpq;
+        long nlfb = 0;
+        for (const auto& P : eparts) {
+            geq.insert(geq.end(), P.geq.begin(), P.geq.end());
+            tpq.insert(tpq.end(), P.tpq.begin(), P.tpq.end());
+            nlfb += P.nlfb;
+            st.flops += P.flops;
+            prof_add(P_GEQRT, P.pf[0], P.pb[0]);
+            prof_add(P_LARFB, P.pf[1], P.pb[1]);
+            prof_add(P_TPQRT, P.pf[2], P.pb[2]);
         }
         hacc(1, ht);
         // ---- upload and launch the elimination's panel work; the trailing-update and scaling
@@ -1065,8 +1101,11 @@ int Tree::factorize_t() {
                 else tpq_small.push_back(t);
             }
             const GeqrtBatch geq_b = prep_geqrt(geq_small);
-            std::vector<LarfbBuckets> lfb_c(1), lfb_t(1);
-            for (const auto& t : lfb) file_larfb(t, lfb_c[0], lfb_t[0]);
+            std::vector<LarfbBuckets> lfb_c(nep), lfb_t(nep);
+            for (int i = 0; i < nep; ++i) {
+                lfb_c[i] = std::move(eparts[i].lc);
+                lfb_t[i] = std::move(eparts[i].lt);
+            }
             const LarfbBatch lfb_b = prep_larfb_parts(lfb_c, false), lfb_tb = prep_larfb_parts(lfb_t, true);
             TpqrtTask<S>* d_tpq = upload_vec(tpq_small);
             TpqrtTask<S>* d_tpqw = upload_vec(tpq_wave);
@@ -1079,8 +1118,8 @@ int Tree::factorize_t() {
                 });
                 if (profiling) prof[P_GEQRT].launches += nl;
             }
-            if (!lfb.empty())
-                timed(P_LARFB, lfb_b.launches() + lfb_tb.launches(), (long)lfb.size(), [&] {
+            if (nlfb)
+                timed(P_LARFB, lfb_b.launches() + lfb_tb.launches(), nlfb, [&] {
                     run_larfb(lfb_b);
                     run_larfb(lfb_tb);
                 });
