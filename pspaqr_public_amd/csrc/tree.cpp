@@ -254,6 +254,8 @@ int Tree::sync() {
 
 // ============================================================================ tree
 struct Tree::SolveSchedule {
+    size_t off = 0;  // workspace entries assigned so far
+    int built = 0;   // levels whose schedule is built
     struct Level {
         void* merge_fwd = nullptr; int n_merge_fwd = 0;
         void* q = nullptr; int n_q = 0;
@@ -808,8 +810,8 @@ int Tree::factorize_t() {
     arena.reset();
     desc_arena.reset();
     stats.clear();
-    delete sched;
-    sched = nullptr;
+    factorized = false;
+    solve_sched_begin<S>();
     // level-0 blocks: copy of the pristine assembled blocks
     {
         S* base = (S*)arena.alloc(std::max<size_t>(h_blocks0.size(), 1) * sizeof(S));
@@ -1397,6 +1399,7 @@ int Tree::factorize_t() {
             HIPCHK(hipGetLastError());
         }
         if (!gq.empty()) {
+            solve_sched_levels<S>(L);  // the finished levels' solve schedule, while the device works
             std::vector<int> ranks(gq.size());
             int rrqr_err = 0;
             HIPCHK(hipMemcpyAsync(ranks.data(), d_ranks, sizeof(int) * ranks.size(), hipMemcpyDeviceToHost, stream_));
@@ -1684,7 +1687,8 @@ int Tree::factorize_t() {
     }
     hacc(5, ht2);
     factorized = true;
-    int rc = build_solve_schedule<S>();
+    solve_sched_levels<S>(nlevels);
+    const int rc = 0;
     sync_stream();
     hacc(6, ht2);
     if (dbg_host)
@@ -1862,21 +1866,39 @@ int Tree::factorize() {
 }
 
 // ---------------------------------------------------------------------------- solve
+// The solve schedule is built level by level while the factorization runs: level L once
+// its factors and ranks are final, at the next level's rank read-back, where the host would
+// otherwise wait for the device (the remaining levels at the end).  Workspace: level-0
+// clusters at their cstart, higher levels appended level by level; every level's clusters hold
+// at most n rows, so n x nlevels entries always suffice and the task pointers never move.
 template <typename S>
-int Tree::build_solve_schedule() {
+void Tree::solve_sched_begin() {
     delete sched;
     sched = new SolveSchedule();
     sched->lv.resize(nlevels);
-    // workspace: level-0 clusters at their cstart, higher levels appended
-    size_t off = (size_t)n;
     for (int o : bottoms[0]) cl[o].xoff = (size_t)cl[o].cstart;
-    for (int L = 1; L < nlevels; ++L)
-        for (int o : bottoms[L]) { cl[o].xoff = off; off += (size_t)cl[o].rows_org; }
-    if (xw) { (void)hipFree(xw); xw = nullptr; }
-    xw_len = off;
-    HIPCHK(hipMalloc(&xw, std::max<size_t>(off, 1) * sizeof(S)));
+    sched->off = (size_t)n;
+    const size_t cap = (size_t)n * (size_t)nlevels;
+    // kept across factorizations and only grown (a free would wait for the device)
+    if (!xw || xw_len < cap) {
+        if (xw) HIPCHK(hipFree(xw));
+        xw = nullptr;
+        HIPCHK(hipMalloc(&xw, std::max<size_t>(cap, 1) * sizeof(S)));
+        xw_len = cap;
+    }
+}
+
+template <typename S>
+void Tree::solve_sched_levels(int upto) {
     S* X0 = (S*)xw;
-    for (int L = 0; L < nlevels; ++L) {
+    for (; sched->built < upto; ++sched->built) {
+        const int L = sched->built;
+        if (L > 0)
+            for (int o : bottoms[L]) {
+                cl[o].xoff = sched->off;
+                sched->off += (size_t)cl[o].rows_org;
+            }
+        if (sched->off > xw_len) throw std::runtime_error("solve workspace bound exceeded");
         auto& lv = sched->lv[L];
         if (L > 0) {
             std::vector<VecCopyTask<S>> mf, mb;
@@ -1982,7 +2004,6 @@ int Tree::build_solve_schedule() {
         lv.n_trsv = (int)tr.size();
         lv.trsv_links = upload_vec(tl);
     }
-    return 0;
 }
 
 template <typename S>

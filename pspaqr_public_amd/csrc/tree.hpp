@@ -216,7 +216,8 @@ private:
     int find_out(const Cluster& c, int n2) const;
     void fillin();
     template <typename S> int factorize_t();
-    template <typename S> int build_solve_schedule();
+    template <typename S> void solve_sched_begin();
+    template <typename S> void solve_sched_levels(int upto);
     template <typename S> int solve_t(double* x);
     void* upload(const void* h, size_t bytes);
     // pinned staging for descriptor uploads: one host copy, then an asynchronous DMA; the
