@@ -31,7 +31,7 @@ constexpr double kPivotTieRel = 1e-10;
 
 // Fronts with at least this many columns run their panel loop from the host (geqrt_big,
 // tpqrt_big) so that the trailing updates spread over many workgroups.
-constexpr int kBigFront = 160;
+constexpr int kBigFront = 96;
 
 // Splits larfb work so that a large block is spread over many workgroups: Q^T C acts on each
 // column independently, so tasks are cut into column chunks; a fused trailing C R^{-1}
