@@ -82,7 +82,7 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
     for (int q : elig) wmax = std::max(wmax, gq[q].w);
     S* d_cmax2 = (S*)arena.alloc(sizeof(S) * std::max(nt, 1));
     float* d_X = nullptr;
-    constexpr int kColChunk = 256;  // columns per workgroup of the column-norm and tail kernels
+    constexpr int kColChunk = 128;  // columns per workgroup of the column-norm and tail kernels
     if (ne && (feats || sketch_mode)) {
         std::vector<ColNormTask<S>> cn;
         for (int q : elig)
