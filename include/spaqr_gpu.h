@@ -172,7 +172,11 @@ typedef struct { const void* V; const void* T; void* C; const void* R; int m, n,
 int spaqr_larfb_batched(int prec, int count, const spaqr_larfb_desc* d, void* stream);
 
 /* dtpqrt(M=rows, N=k, L=0, NB=32) chained: for each link j of a task, in order,
-   [R; B_j] <- QR, V_j in B_j, T_j (32 x k).  One task per interior front.              */
+   [R; B_j] <- QR, V_j in B_j, T_j (32 x k).  One task per interior front: the chain of
+   tsqrt_edge calls over edgesOut (src/ptree.cpp:197-214, 1165-1167).  Chains are dispatched
+   as the level planner does: k <= 64 with links of <= 64 rows one per wavefront, several
+   panels (k > 32) with more than one link on the cooperative kernel, the rest one workgroup
+   each.                                                                                   */
 typedef struct { void* B; void* T; int rows, ldb; } spaqr_tpqrt_link;
 typedef struct { void* R; int k, ldr, link0, nlinks; } spaqr_tpqrt_desc;
 int spaqr_tpqrt_batched(int prec, int count, const spaqr_tpqrt_desc* d, int nlinks, const spaqr_tpqrt_link* links,
@@ -202,6 +206,54 @@ typedef struct {
     int mc, nc, lda, kn, rn, kc, rc, ldvn, ldvc, ldd, ident, pad;
 } spaqr_spars_desc;
 int spaqr_spars_apply_batched(int prec, int count, const spaqr_spars_desc* d, void* stream);
+
+/* ------------------------------------------------------------------ rank-path kernels
+   The randomized sparsification's kernels on caller buffers (rank modes 1 and 2 above).  No
+   reference counterpart; the closest analogue is random_rrqr (src/util.cpp:643-715).  All
+   matrices are column-major with ld = rows; every call synchronizes `stream`. */
+
+/* Sketch rounds per interface in rank modes 1 and 2: retries (0..2) doublings of the sketch
+   width after a failed tail test before the geqp3 fallback (default 2). */
+int spaqr_tree_set_rank_retries(spaqr_tree* t, int retries);
+/* cluster order of each rank-log entry (the order of spaqr_tree_get_clusters' rows) */
+int spaqr_tree_rank_log_clusters(const spaqr_tree* t, int* out, int cap);
+
+/* Gaussian test matrix Omega (rows x cols, ld): entry (i, j) depends on (i, j, seed) only
+   (splitmix64 of seed ^ (j << 32 | i), then Box-Muller), so a larger matrix agrees with a
+   smaller one on their overlap.  The planner uses seed 0x5eed5ca1e. */
+int spaqr_gauss_fill(int prec, void* omega, int rows, int cols, int ld, uint64_t seed, void* stream);
+
+/* Column norms: cn[j] = ||C(:, j)||^2 (w values) and *cmax2 = max(*cmax2, max_j cn[j]); tasks
+   may share one cmax2, which must hold 0 (or a lower bound) on entry. */
+typedef struct { const void* C; void* cn; void* cmax2; int k, w; } spaqr_colnorm_desc;
+int spaqr_colnorm_batched(int prec, int count, const spaqr_colnorm_desc* d, void* stream);
+
+/* Sketch Y (k x s, ld k) = C (k x w, ld k) * Omega(0:w, 0:s) (Omega ld ldo >= every w). */
+typedef struct { const void* C; void* Y; int k, w, s, pad; } spaqr_sketch_desc;
+int spaqr_sketch_batched(int prec, int count, const spaqr_sketch_desc* d, const void* omega, int ldo, void* stream);
+
+/* The predictor's 32 input features of one interface (x, floats): from Y16, the geqrt-factored
+   16-column sketch (its R diagonal Y16[j + j k]), the column norms cn and cmax2 of C, and the
+   tree context (neighbour counts, level, tol).  x[0..15]: 1 + log10(|R_jj| / |R_00|) / 8
+   clamped to [0, 1]; x[16..26]: log2(k+1)/11, log2(w+1)/14, min(k,w)/max(k,w), level/16,
+   -log10(tol)/8, nsout/32, nsin/32, the share of columns with cn >= tol^2 cmax2 over w and
+   over min(k,w) (capped at 1), cmax2 / sum(cn), min(1, sum(cn) / cmax2 / min(k,w));
+   x[27..31] = 0. */
+typedef struct { const void* Y16; const void* cn; const void* cmax2; float* x; int k, w, nsout, nsin, level, pad; }
+    spaqr_rank_features_desc;
+int spaqr_rank_features_batched(int prec, int count, const spaqr_rank_features_desc* d, double tol, void* stream);
+
+/* Tail selection on B = Q^T C (k x w): *rank = max(*rank, the smallest r <= *ry such that
+   every column j has ||B(r:k, j)||^2 <= tol^2 * (*cmax2)), or INT_MAX when some column's tail
+   at *ry still exceeds it.  *rank must hold 0 on entry; tasks (column chunks of one interface)
+   may share it.  Sums in FP64. */
+typedef struct { const void* B; const void* cmax2; const int* ry; int* rank; int k, w; } spaqr_tail_desc;
+int spaqr_tail_select_batched(int prec, int count, const spaqr_tail_desc* d, double tol, void* stream);
+
+/* Joins the library's host worker threads.  Call it after the last tree is destroyed and
+   before the process exits (the Python binding does so at interpreter exit); a later call
+   into the library starts them again.  No reference counterpart. */
+int spaqr_shutdown(void);
 
 #ifdef __cplusplus
 }

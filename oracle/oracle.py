@@ -67,6 +67,18 @@ def lib():
         L.orc_trsv_upper.argtypes = [C.c_int, _dp, _dp]
         L.orc_ormqr_lt.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
         L.orc_ormqr_ln.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+        L.orc_set_fault.argtypes = [C.c_int]
+        # rank path
+        L.orc_gauss.argtypes = [C.c_int, C.c_int, C.c_uint64, _dp]
+        L.orc_colnorm.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        L.orc_colnorm.restype = C.c_double
+        L.orc_sketch.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp]
+        L.orc_rank_features.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
+                                        np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")]
+        L.orc_tail_select.argtypes = [C.c_int, C.c_int, _dp, C.c_int, C.c_double, C.c_double]
+        L.orc_tail_select.restype = C.c_int
+        L.orc_sketch_rank.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_int]
+        L.orc_sketch_rank.restype = C.c_int
         _lib = L
     return _lib
 
@@ -150,3 +162,19 @@ class OracleTree:
         hl = C.c_int(0)
         it = lib().orc_gmres(self._h, b, x, iters, tol, restart, true_tol, hist, hist.size, C.byref(hl))
         return x, it, hist[: hl.value]
+
+
+class planted_fault:
+    """Context manager: run the oracle with one of its planted mistakes (spaqr_oracle.cpp,
+    g_fault) so that a test can show a parity check catches it."""
+
+    def __init__(self, code: int):
+        self.code = code
+
+    def __enter__(self):
+        lib().orc_set_fault(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_fault(0)
+        return False

@@ -40,8 +40,20 @@
 #include <vector>
 
 #include "dense.hpp"
+#include "rank_path.hpp"
 
 namespace orc {
+
+// Planted mistakes, for tests that show the parity checks catch them (0: none).  Each is a
+// plausible slip in restating the reference:
+//   1 interface inference takes a neighbour's side from its ancestor one level too high
+//     (src/partition.cpp:538 divides the separator index by 2^(lvl - nbr.lvl - 1))
+//   2 the RRQR gather drops the A(c,m)_top columns of C (src/ptree.cpp:390-398)
+//   3 the truncation keeps one column fewer than choose_rank (src/ptree.cpp:420-431)
+//   4 the sparsify apply skips the left transform Q_c^T (src/ptree.cpp:1881-1889)
+//   5 the sparsified level range also takes the second level from the top
+//     (src/ptree.cpp:30, 1566, 1635 stop at nlevels - 2)
+static int g_fault = 0;
 
 // Relative band inside which geqp3 pivot candidates count as tied (lowest index wins); the
 // HIP kernel uses the same rule, so FP64 runs pick the same pivots despite rounding order.
@@ -249,7 +261,7 @@ static std::vector<ClusterID> geometric_partition(const CSC& A, int nlevels, con
             int j = At.rowind[k];
             SepID nbr = cmap[j].self;
             if (nbr.lvl < self.lvl) {
-                int pj = nbr.sep >> (self.lvl - nbr.lvl - 1);
+                int pj = nbr.sep >> (self.lvl - nbr.lvl - (g_fault == 1 ? 0 : 1));
                 if (pj % 2 == 0) left = (left.lvl == -1) ? nbr : find_lowest_common(nbr, left);
                 else right = (right.lvl == -1) ? nbr : find_lowest_common(nbr, right);
             }
@@ -427,7 +439,7 @@ struct Tree {
     }
 
     bool want_sparsify(const Cluster* c) const {  // src/ptree.cpp:29-41
-        if (c->merge_level < skip || c->merge_level >= nlevels - 2 || tol == 0) return false;
+        if (c->merge_level < skip || c->merge_level >= nlevels - (g_fault == 5 ? 1 : 2) || tol == 0) return false;
         if (c->lvl() > c->merge_level) {
             if (c->id.l.lvl <= c->merge_level && c->id.r.lvl <= c->merge_level) return true;
         }
@@ -564,7 +576,7 @@ struct Tree {
             fillin();
             eliminate(ilvl);
             scale(ilvl);
-            if (ilvl >= skip && ilvl < nlevels - 2 && tol != 0) sparsify(ilvl);
+            if (ilvl >= skip && ilvl < nlevels - (g_fault == 5 ? 1 : 2) && tol != 0) sparsify(ilvl);
             if (ilvl < nlevels - 1) merge();
         }
         t_factor = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -700,7 +712,8 @@ struct Tree {
             int k = c->cols();
             int smp = 0, spm = 0;
             for (Edge* e : c->sparsOut) smp += e->A.r;
-            for (Edge* e : c->sparsIn) spm += e->A.c;
+            if (g_fault != 2)
+                for (Edge* e : c->sparsIn) spm += e->A.c;
             Mat C(k, smp + spm);
             int is = 0;
             for (Edge* e : c->sparsOut) {
@@ -708,11 +721,12 @@ struct Tree {
                     for (int i = 0; i < k; ++i) C(i, is + j) = e->A(j, i);
                 is += e->A.r;
             }
-            for (Edge* e : c->sparsIn) {
-                for (int j = 0; j < e->A.c; ++j)
-                    for (int i = 0; i < k; ++i) C(i, is + j) = e->A(i, j);
-                is += e->A.c;
-            }
+            if (g_fault != 2)
+                for (Edge* e : c->sparsIn) {
+                    for (int j = 0; j < e->A.c; ++j)
+                        for (int i = 0; i < k; ++i) C(i, is + j) = e->A(i, j);
+                    is += e->A.c;
+                }
             int w = C.c;
             int mn = std::min(k, w);
             std::vector<int> jpvt(w);
@@ -723,6 +737,7 @@ struct Tree {
                 std::vector<double> rii(mn);
                 for (int i = 0; i < mn; ++i) rii[i] = C(i, i);
                 rank = choose_rank(rii.data(), mn, tol);
+                if (g_fault == 3 && rank > 1) rank--;
             } else {
                 // geqp3 returns without touching C (src/util.cpp:504-505); diagonal is empty
                 rank = choose_rank(nullptr, 0, tol);
@@ -751,7 +766,7 @@ struct Tree {
                     e->A = Acopy;
                 }
                 e->A.conservative_resize(c->rank, n->rank);
-                if (want_sparsify(c)) {
+                if (want_sparsify(c) && g_fault != 4) {
                     Mat A11 = Acopy.block(0, 0, c->Qsp.r, n->rank);
                     larfb_left_trans(A11.r, A11.c, c->Qsp.c, c->Qsp.data(), c->Qsp.r, c->Tsp.data(), c->Tsp.r,
                                      A11.data(), A11.r);
@@ -1099,6 +1114,28 @@ void orc_gemqrt_lt(int m, int n, int k, int nb, const double* V, const double* T
     gemqrt_lt(m, n, k, nb, V, m, T, nb, C, m);
 }
 int orc_choose_rank(const double* s, int n, double tol) { return choose_rank(s, n, tol); }
+void orc_set_fault(int f) { g_fault = f; }
+// rank path (rank_path.hpp)
+void orc_gauss(int rows, int cols, uint64_t seed, double* om) {
+    for (int j = 0; j < cols; ++j)
+        for (int i = 0; i < rows; ++i) om[i + (size_t)j * rows] = gauss_entry(i, j, seed);
+}
+double orc_colnorm(int k, int w, const double* C, double* cn) {
+    double mx = 0.0;
+    colnorm2(k, w, C, cn, &mx);
+    return mx;
+}
+void orc_sketch(int k, int w, int s, const double* C, const double* om, int ldo, double* Y) { sketch(k, w, s, C, om, ldo, Y); }
+void orc_rank_features(int k, int w, const double* rdiag, const double* cn, double cmax2, int nsout, int nsin, int level,
+                       double tol, float* x) {
+    rank_features(k, w, rdiag, cn, cmax2, nsout, nsin, level, tol, x);
+}
+int orc_tail_select(int k, int w, const double* B, int ry, double cmax2, double tol) {
+    return tail_select(k, w, B, ry, cmax2, tol);
+}
+int orc_sketch_rank(int k, int w, int s, const double* C, const double* om, int ldo, double tol, int pivoted) {
+    return sketch_rank(k, w, s, C, om, ldo, tol, pivoted != 0);
+}
 void orc_set_near_tie_band(double b) { geqp3_near_tie_band() = b; }
 long orc_larfg_tiny_alpha(int reset) {
     long v = larfg_tiny_alpha();

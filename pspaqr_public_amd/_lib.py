@@ -5,8 +5,10 @@ if the library or a GPU is missing, calls raise.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -57,6 +59,23 @@ class SparsDesc(C.Structure):
     _fields_ = [("A", _vp), ("Vn", _vp), ("Tn", _vp), ("Vc", _vp), ("Tc", _vp), ("dst", _vp), ("mc", _i), ("nc", _i),
                 ("lda", _i), ("kn", _i), ("rn", _i), ("kc", _i), ("rc", _i), ("ldvn", _i), ("ldvc", _i),
                 ("ldd", _i), ("ident", _i), ("pad", _i)]
+
+
+class ColNormDesc(C.Structure):
+    _fields_ = [("C", _vp), ("cn", _vp), ("cmax2", _vp), ("k", _i), ("w", _i)]
+
+
+class SketchDesc(C.Structure):
+    _fields_ = [("C", _vp), ("Y", _vp), ("k", _i), ("w", _i), ("s", _i), ("pad", _i)]
+
+
+class FeatDesc(C.Structure):
+    _fields_ = [("Y16", _vp), ("cn", _vp), ("cmax2", _vp), ("x", _vp), ("k", _i), ("w", _i), ("nsout", _i),
+                ("nsin", _i), ("level", _i), ("pad", _i)]
+
+
+class TailDesc(C.Structure):
+    _fields_ = [("B", _vp), ("cmax2", _vp), ("ry", _vp), ("rank", _vp), ("k", _i), ("w", _i)]
 
 
 EXPORTS = {
@@ -113,7 +132,36 @@ EXPORTS = {
     "spaqr_tpmqrt_batched": (_i, [_i, _i, _vp, _i, _vp, _vp]),
     "spaqr_geqp3_batched": (_i, [_i, _i, _vp, _d, _d, _vp]),
     "spaqr_spars_apply_batched": (_i, [_i, _i, _vp, _vp]),
+    "spaqr_tree_set_rank_retries": (_i, [_vp, _i]),
+    "spaqr_tree_rank_log_clusters": (_i, [_vp, _vp, _i]),
+    "spaqr_gauss_fill": (_i, [_i, _vp, _i, _i, _i, C.c_uint64, _vp]),
+    "spaqr_colnorm_batched": (_i, [_i, _i, _vp, _vp]),
+    "spaqr_sketch_batched": (_i, [_i, _i, _vp, _vp, _i, _vp]),
+    "spaqr_rank_features_batched": (_i, [_i, _i, _vp, _d, _vp]),
+    "spaqr_tail_select_batched": (_i, [_i, _i, _vp, _d, _vp]),
+    "spaqr_shutdown": (_i, []),
 }
+
+# Objects holding device memory or a tree (ParTree, DeviceBuffer).  At interpreter exit they are
+# released explicitly, while the HIP runtime is certainly alive, and the library's host threads
+# are joined: no HIP call is left to finalizers or static destructors.
+_live = weakref.WeakSet()
+
+
+def register(obj):
+    _live.add(obj)
+
+
+@atexit.register
+def _shutdown():
+    if _lib is None:
+        return
+    for obj in list(_live):
+        try:
+            obj.close()
+        except Exception:
+            pass
+    _lib.spaqr_shutdown()
 
 
 def lib():
@@ -139,6 +187,7 @@ class DeviceBuffer:
         self.ptr = lib().spaqr_device_alloc(max(self.nbytes, 8))
         if not self.ptr:
             raise MemoryError("spaqr_device_alloc(%d) failed" % nbytes)
+        register(self)
 
     @classmethod
     def from_array(cls, a: np.ndarray):
@@ -164,6 +213,8 @@ class DeviceBuffer:
         if self.ptr:
             lib().spaqr_device_free(self.ptr)
             self.ptr = None
+
+    close = free
 
     def __del__(self):
         try:

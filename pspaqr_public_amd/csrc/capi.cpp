@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -52,7 +53,46 @@ static int run_batched(const void* host, size_t bytes, const void* host2, size_t
 }
 
 
+// Device buffers of one batched call: descriptor copies (kept with a host copy until the
+// stream syncs) and zeroed scratch, all freed when the call returns.
+struct CallScratch {
+    hipStream_t s;
+    std::vector<void*> owned;
+    std::vector<std::vector<char>> keep;
+    explicit CallScratch(hipStream_t st) : s(st) {}
+    ~CallScratch() {
+        for (void* p : owned) (void)hipFree(p);
+    }
+    void* alloc(size_t b) {
+        void* p = nullptr;
+        if (hipMalloc(&p, b ? b : 8) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+        owned.push_back(p);
+        return p;
+    }
+    void* upload(const void* h, size_t b) {
+        void* p = alloc(b);
+        keep.emplace_back((const char*)h, (const char*)h + b);
+        if (b && hipMemcpyAsync(p, keep.back().data(), b, hipMemcpyHostToDevice, s) != hipSuccess)
+            throw std::runtime_error("descriptor upload failed");
+        return p;
+    }
+    template <typename T> T* upload_vec(const std::vector<T>& v) {
+        return v.empty() ? nullptr : (T*)upload(v.data(), v.size() * sizeof(T));
+    }
+    int finish() {
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); return -100; }
+        return 0;
+    }
+};
+
 extern "C" {
+
+int spaqr_shutdown(void) {
+    HostPool::shutdown();
+    return 0;
+}
 
 spaqr_tree* spaqr_tree_create(int nlevels, int skip) {
     if (nlevels < 1 || skip < 0) { g_err = "nlevels must be >= 1 and skip >= 0"; return nullptr; }
@@ -378,20 +418,88 @@ int spaqr_larfb_batched(int prec, int count, const spaqr_larfb_desc* d, void* st
                        [&](void* dd, void*, hipStream_t s) { launch_larfb<float>((const LarfbTask<float>*)dd, n, s); });
 }
 
+}  // extern "C"
+
+// Chains are dispatched as the level planner dispatches them: k <= 64 with links of at most
+// 64 rows one per wave; several 32-column panels (k > 32) with more than one link on the
+// cooperative kernel (workgroup p owns panel p, links pipelined through progress counters);
+// the rest one workgroup per chain.
+template <typename S>
+static int tpqrt_dispatch(int count, const spaqr_tpqrt_desc* d, int nlinks, const spaqr_tpqrt_link* links,
+                          hipStream_t s) {
+    static_assert(sizeof(TpqrtTask<S>) == sizeof(spaqr_tpqrt_desc), "tpqrt desc layout");
+    static_assert(sizeof(TpqrtLink<S>) == sizeof(spaqr_tpqrt_link), "tpqrt link layout");
+    const TpqrtTask<S>* td = (const TpqrtTask<S>*)d;
+    const TpqrtLink<S>* tl = (const TpqrtLink<S>*)links;
+    std::vector<TpqrtTask<S>> wave, plain, coop;
+    for (int i = 0; i < count; ++i) {
+        const TpqrtTask<S>& t = td[i];
+        bool small = t.k <= 64;
+        for (int j = 0; j < t.nlinks && small; ++j) small = tl[t.link0 + j].rows <= 64;
+        if (small) wave.push_back(t);
+        else if (t.k > NB && t.nlinks > 1) coop.push_back(t);
+        else plain.push_back(t);
+    }
+    CallScratch cs(s);
+    try {
+        auto* d_links = (const TpqrtLink<S>*)cs.upload(links, sizeof(*links) * std::max(nlinks, 1));
+        launch_tpqrt_wave<S>(cs.upload_vec(wave), d_links, (int)wave.size(), s);
+        launch_tpqrt<S>(cs.upload_vec(plain), d_links, (int)plain.size(), s);
+        if (!coop.empty()) {
+            const int maxb = tpqrt_coop_max_blocks<S>();
+            int panels = 0;
+            for (const auto& t : coop) panels += (t.k + NB - 1) / NB;
+            std::vector<int> zeros(panels + 1, 0);
+            int* cnt = (int*)cs.upload(zeros.data(), sizeof(int) * zeros.size());
+            int* err = cnt + panels;
+            int coff = 0;
+            for (size_t i = 0; i < coop.size();) {
+                std::vector<TpqrtCoopTask<S>> tk;
+                std::vector<int> b2t;
+                int nb = 0;
+                while (i < coop.size()) {
+                    const TpqrtTask<S>& c = coop[i];
+                    const int P = (c.k + NB - 1) / NB;
+                    if (P > maxb) throw std::runtime_error("cooperative tpqrt: chain wider than the co-resident limit");
+                    if (nb + P > maxb) break;
+                    tk.push_back({c.R, cnt + coff, c.k, c.ldr, c.link0, c.nlinks, nb, 0});
+                    for (int q = 0; q < P; ++q) b2t.push_back((int)tk.size() - 1);
+                    nb += P;
+                    coff += P;
+                    ++i;
+                }
+                launch_tpqrt_coop<S>(cs.upload_vec(tk), cs.upload_vec(b2t), nb, d_links, err, s);
+            }
+            int herr = 0;
+            if (hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                throw std::runtime_error("cooperative tpqrt: synchronization failed");
+            if (herr) throw std::runtime_error("cooperative tpqrt: a wait exceeded its spin limit");
+        }
+        return cs.finish();
+    } catch (std::exception& e) {
+        g_err = e.what();
+        (void)hipStreamSynchronize(s);
+        return -100;
+    }
+}
+
+extern "C" {
+
 int spaqr_tpqrt_batched(int prec, int count, const spaqr_tpqrt_desc* d, int nlinks, const spaqr_tpqrt_link* links,
                         void* stream) {
     if (prec != 32 && prec != 64) return -1;
     if (count < 0 || (count > 0 && !d)) return -2;
+    if (nlinks < 0 || (nlinks > 0 && !links)) return -4;
     for (int i = 0; i < count; ++i)
-        if (d[i].link0 < 0 || d[i].nlinks < 0 || d[i].link0 + d[i].nlinks > nlinks) return -3;
+        if (d[i].k < 0 || d[i].link0 < 0 || d[i].nlinks < 0 || d[i].link0 + d[i].nlinks > nlinks ||
+            d[i].ldr < std::max(1, d[i].k))
+            return -3;
+    for (int j = 0; j < nlinks; ++j)
+        if (links[j].rows < 0 || links[j].ldb < std::max(1, links[j].rows)) return -5;
     if (count == 0) return 0;
-    return run_batched(d, sizeof(*d) * count, links, sizeof(*links) * nlinks, stream,
-                       [&](void* dd, void* dl, hipStream_t s) {
-                           if (prec == 64)
-                               launch_tpqrt<double>((const TpqrtTask<double>*)dd, (const TpqrtLink<double>*)dl, count, s);
-                           else
-                               launch_tpqrt<float>((const TpqrtTask<float>*)dd, (const TpqrtLink<float>*)dl, count, s);
-                       });
+    return prec == 64 ? tpqrt_dispatch<double>(count, d, nlinks, links, (hipStream_t)stream)
+                      : tpqrt_dispatch<float>(count, d, nlinks, links, (hipStream_t)stream);
 }
 
 int spaqr_tpmqrt_batched(int prec, int count, const spaqr_tpmqrt_desc* d, int nlinks, const spaqr_tpmqrt_link* links,
@@ -512,6 +620,106 @@ int spaqr_spars_apply_batched(int prec, int count, const spaqr_spars_desc* d, vo
         if (rc) return rc;
     }
     return 0;
+}
+
+// ---------------------------------------------------------------- rank-path kernels
+int spaqr_tree_set_rank_retries(spaqr_tree* h, int retries) {
+    TREE_OR_FAIL(h);
+    if (retries < 0 || retries > 2) { h->t->last_error = "rank retries must be 0, 1 or 2"; return -2; }
+    h->t->rank_rounds = retries + 1;
+    return 0;
+}
+int spaqr_tree_rank_log_clusters(const spaqr_tree* h, int* out, int cap) {
+    if (!h || !h->t) return -1;
+    const auto& o = h->t->rlog.order;
+    const int n = (int)o.size();
+    if (out)
+        for (int i = 0; i < std::min(n, std::max(cap, 0)); ++i) out[i] = o[i];
+    return n;
+}
+
+int spaqr_gauss_fill(int prec, void* om, int rows, int cols, int ld, uint64_t seed, void* stream) {
+    if (prec != 32 && prec != 64) return -1;
+    if (!om && (size_t)rows * cols > 0) return -2;
+    if (rows < 0 || cols < 0) return -3;
+    if (ld < std::max(1, rows)) return -5;
+    if ((size_t)rows * cols == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == 64) launch_gauss<double>((double*)om, rows, cols, ld, seed, s);
+    else launch_gauss<float>((float*)om, rows, cols, ld, seed, s);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { g_err = hipGetErrorString(e); return -100; }
+    return 0;
+}
+
+int spaqr_colnorm_batched(int prec, int count, const spaqr_colnorm_desc* d, void* stream) {
+    if (prec != 32 && prec != 64) return -1;
+    if (count < 0 || (count > 0 && !d)) return -2;
+    for (int i = 0; i < count; ++i)
+        if (d[i].k < 1 || d[i].w < 1 || !d[i].C || !d[i].cn || !d[i].cmax2) return -3;
+    if (count == 0) return 0;
+    static_assert(sizeof(ColNormTask<double>) == sizeof(spaqr_colnorm_desc), "colnorm desc layout");
+    return run_batched(d, sizeof(*d) * count, nullptr, 0, stream, [&](void* dd, void*, hipStream_t s) {
+        if (prec == 64) launch_colnorm<double>((const ColNormTask<double>*)dd, count, s);
+        else launch_colnorm<float>((const ColNormTask<float>*)dd, count, s);
+    });
+}
+
+int spaqr_sketch_batched(int prec, int count, const spaqr_sketch_desc* d, const void* omega, int ldo, void* stream) {
+    if (prec != 32 && prec != 64) return -1;
+    if (count < 0 || (count > 0 && !d)) return -2;
+    if (!omega) return -4;
+    int wmax = 0;
+    for (int i = 0; i < count; ++i) {
+        if (d[i].k < 1 || d[i].w < 1 || d[i].s < 1 || !d[i].C || !d[i].Y) return -3;
+        wmax = std::max(wmax, d[i].w);
+    }
+    if (ldo < wmax) return -5;
+    if (count == 0) return 0;
+    static_assert(sizeof(SketchTask<double>) == sizeof(spaqr_sketch_desc), "sketch desc layout");
+    std::vector<SketchTile> tiles;
+    for (int i = 0; i < count; ++i)
+        for (int r0 = 0; r0 < d[i].k; r0 += 64)
+            for (int c0 = 0; c0 < d[i].s; c0 += 64) tiles.push_back({i, r0, c0, 0});
+    const int nt = (int)tiles.size();
+    return run_batched(d, sizeof(*d) * count, tiles.data(), sizeof(SketchTile) * tiles.size(), stream,
+                       [&](void* dd, void* dt, hipStream_t s) {
+                           if (prec == 64)
+                               launch_sketch<double>((const SketchTask<double>*)dd, (const SketchTile*)dt, nt,
+                                                     (const double*)omega, ldo, s);
+                           else
+                               launch_sketch<float>((const SketchTask<float>*)dd, (const SketchTile*)dt, nt,
+                                                    (const float*)omega, ldo, s);
+                       });
+}
+
+int spaqr_rank_features_batched(int prec, int count, const spaqr_rank_features_desc* d, double tol, void* stream) {
+    if (prec != 32 && prec != 64) return -1;
+    if (count < 0 || (count > 0 && !d)) return -2;
+    for (int i = 0; i < count; ++i)
+        if (d[i].k < 1 || d[i].w < 1 || !d[i].Y16 || !d[i].cn || !d[i].cmax2 || !d[i].x) return -3;
+    if (!(tol >= 0)) return -4;
+    if (count == 0) return 0;
+    static_assert(sizeof(FeatTask<double>) == sizeof(spaqr_rank_features_desc), "features desc layout");
+    return run_batched(d, sizeof(*d) * count, nullptr, 0, stream, [&](void* dd, void*, hipStream_t s) {
+        if (prec == 64) launch_rank_features<double>((const FeatTask<double>*)dd, count, tol, s);
+        else launch_rank_features<float>((const FeatTask<float>*)dd, count, tol, s);
+    });
+}
+
+int spaqr_tail_select_batched(int prec, int count, const spaqr_tail_desc* d, double tol, void* stream) {
+    if (prec != 32 && prec != 64) return -1;
+    if (count < 0 || (count > 0 && !d)) return -2;
+    for (int i = 0; i < count; ++i)
+        if (d[i].k < 1 || d[i].w < 1 || !d[i].B || !d[i].cmax2 || !d[i].ry || !d[i].rank) return -3;
+    if (!(tol >= 0)) return -4;
+    if (count == 0) return 0;
+    static_assert(sizeof(TailTask<double>) == sizeof(spaqr_tail_desc), "tail desc layout");
+    return run_batched(d, sizeof(*d) * count, nullptr, 0, stream, [&](void* dd, void*, hipStream_t s) {
+        if (prec == 64) launch_tail_select<double>((const TailTask<double>*)dd, count, tol, s);
+        else launch_tail_select<float>((const TailTask<float>*)dd, count, tol, s);
+    });
 }
 
 }  // extern "C"

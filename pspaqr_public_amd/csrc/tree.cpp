@@ -290,6 +290,7 @@ Tree::~Tree() {
     if (d_scratch) (void)hipFree(d_scratch);
     if (d_blocks0) (void)hipFree(d_blocks0);
     if (xw) (void)hipFree(xw);
+    if (d_krylov) (void)hipFree(d_krylov);
     if (d_mlp_w) (void)hipFree(d_mlp_w);
     if (d_omega) (void)hipFree(d_omega);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -607,6 +608,7 @@ int Tree::assemble() {
         const size_t nel = h_blocks0.size();
         const size_t es = (prec == 32) ? 4 : 8;
         HIPCHK(hipMalloc(&d_blocks0, std::max<size_t>(nel * es, 256)));
+        prec_asm = prec;
         if (prec == 32) {
             std::vector<float> f(h_blocks0.begin(), h_blocks0.end());
             HIPCHK(hipMemcpy(d_blocks0, f.data(), nel * 4, hipMemcpyHostToDevice));
@@ -1858,6 +1860,10 @@ int Tree::tpqrt_coop(const std::vector<TpqrtTask<S>>& chains, const TpqrtLink<S>
 int Tree::factorize() {
     try {
         if (ed0.empty()) throw std::runtime_error("assemble() first");
+        if (prec != prec_asm)
+            throw std::runtime_error("the factor precision changed after assemble(): assemble() again before factorize()");
+        factorized = false;
+        prec_fact = prec;
         return prec == 32 ? factorize_t<float>() : factorize_t<double>();
     } catch (std::exception& e) {
         last_error = e.what();
@@ -1878,13 +1884,15 @@ void Tree::solve_sched_begin() {
     sched->lv.resize(nlevels);
     for (int o : bottoms[0]) cl[o].xoff = (size_t)cl[o].cstart;
     sched->off = (size_t)n;
-    const size_t cap = (size_t)n * (size_t)nlevels;
-    // kept across factorizations and only grown (a free would wait for the device)
-    if (!xw || xw_len < cap) {
+    const size_t cap = std::max<size_t>((size_t)n * (size_t)nlevels, 1) * sizeof(S);
+    // kept across factorizations (of either precision) and only grown: a free would wait for
+    // the device
+    if (!xw || xw_bytes < cap) {
         if (xw) HIPCHK(hipFree(xw));
         xw = nullptr;
-        HIPCHK(hipMalloc(&xw, std::max<size_t>(cap, 1) * sizeof(S)));
-        xw_len = cap;
+        xw_bytes = 0;
+        HIPCHK(hipMalloc(&xw, cap));
+        xw_bytes = cap;
     }
 }
 
@@ -1898,7 +1906,7 @@ void Tree::solve_sched_levels(int upto) {
                 cl[o].xoff = sched->off;
                 sched->off += (size_t)cl[o].rows_org;
             }
-        if (sched->off > xw_len) throw std::runtime_error("solve workspace bound exceeded");
+        if (sched->off * sizeof(S) > xw_bytes) throw std::runtime_error("solve workspace bound exceeded");
         auto& lv = sched->lv[L];
         if (L > 0) {
             std::vector<VecCopyTask<S>> mf, mb;
@@ -2038,7 +2046,7 @@ int Tree::solve_dev(double* x) {
     try {
         if (!factorized || !sched) throw std::runtime_error("factorize() first");
         int rc = 0;
-        timed(P_SOLVE, 1, 1, [&] { rc = (prec == 32) ? solve_t<float>(x) : solve_t<double>(x); });
+        timed(P_SOLVE, 1, 1, [&] { rc = (prec_fact == 32) ? solve_t<float>(x) : solve_t<double>(x); });
         return rc;
     } catch (std::exception& e) {
         last_error = e.what();
@@ -2067,7 +2075,7 @@ int Tree::diag_R(int order, double* out, int maxn) {
         const int k = e.cols;
         if ((long long)k * k > maxn) return -2;
         std::vector<double> h((size_t)e.rows * e.cols);
-        if (prec == 32) {
+        if (prec_fact == 32) {
             std::vector<float> f(h.size());
             HIPCHK(hipMemcpy(f.data(), e.A, f.size() * 4, hipMemcpyDeviceToHost));
             for (size_t i = 0; i < f.size(); ++i) h[i] = f[i];
@@ -2084,6 +2092,18 @@ int Tree::diag_R(int order, double* out, int maxn) {
 }
 
 // ---------------------------------------------------------------------------- GMRES
+double* Tree::krylov_ws(size_t bytes) {
+    if (!d_krylov || krylov_bytes < bytes) {
+        sync_stream();  // queued work may still read the old basis
+        if (d_krylov) HIPCHK(hipFree(d_krylov));
+        d_krylov = nullptr;
+        krylov_bytes = 0;
+        HIPCHK(hipMalloc(&d_krylov, bytes));
+        krylov_bytes = bytes;
+    }
+    return d_krylov;
+}
+
 // dist_gmres (src/is.cpp:447-527) on device vectors: MGS Arnoldi with one dot + one axpy
 // launch per basis vector (scalars stay on the device), Givens rotations on the host from
 // one small readback per iteration.  restart > 0 restarts from the current iterate (an
@@ -2101,15 +2121,13 @@ int Tree::gmres_dev(const double* b, double* x, int iters, double tol_, int rest
     try {
         if (!factorized) throw std::runtime_error("factorize() first");
         const int m = (restart > 0) ? std::min(restart, iters) : iters;
-        double* Q = nullptr;
-        double* Z = nullptr;
-        double* w = nullptr;
-        HIPCHK(hipMalloc(&Q, sizeof(double) * (size_t)n * (m + 1)));
-        if (flexible) HIPCHK(hipMalloc(&Z, sizeof(double) * (size_t)n * m));
-        HIPCHK(hipMalloc(&w, sizeof(double) * (size_t)n));
+        if (m + 2 > 4096) throw std::runtime_error("restart length too large (at most 4094 steps per cycle)");
+        const size_t nv = (size_t)(m + 1) + (flexible ? (size_t)m : 0) + 1;  // basis, Z, work vector
+        double* Q = krylov_ws(sizeof(double) * (size_t)n * nv);
+        double* Z = flexible ? Q + (size_t)n * (m + 1) : nullptr;
+        double* w = Q + (size_t)n * (nv - 1);
         double* part = d_scratch;
         double* hdev = d_scratch + dot_blocks;  // up to 4096 scalars
-        if (m + 2 > 4096) throw std::runtime_error("restart length too large");
         auto dot = [&](const double* a, const double* bb, double* out, int sq) {
             launch_dot(n, a, bb, part, dot_blocks, out, sq, stream_);
         };
@@ -2212,9 +2230,6 @@ int Tree::gmres_dev(const double* b, double* x, int iters, double tol_, int rest
             if (true_rel <= true_tol || total >= iters) break;
         }
         sync_stream();
-        (void)hipFree(Q);
-        if (Z) (void)hipFree(Z);
-        (void)hipFree(w);
         if (res) {
             res->iters = total - 1;
             res->cycles = cycles;

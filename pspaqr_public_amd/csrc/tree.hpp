@@ -137,8 +137,11 @@ public:
     int set_rank_model(const float* w);  // kMlpParams predictor weights (host)
     struct RankLog {
         std::vector<float> x;  // kRankFeatures per entry
-        std::vector<int> rank, k, w, level, mode;
+        std::vector<int> rank, k, w, level, mode, order;  // order: the interface's cluster
     } rlog;
+    // Sketch rounds per interface in rank modes 1 and 2: the first sketch plus up to
+    // rank_rounds - 1 retries with a doubled width before the geqp3 fallback (1..3).
+    int rank_rounds = 3;
     struct RankStats {
         long tasks = 0, sketched = 0, fallback = 0, direct = 0, retried = 0;
         double sketch_cols = 0, full_cols = 0, rank_sum = 0;
@@ -207,7 +210,14 @@ private:
     struct SolveSchedule;
     SolveSchedule* sched = nullptr;
     void* xw = nullptr;  // solve workspace (precision of the factors)
-    size_t xw_len = 0;
+    size_t xw_bytes = 0;  // its capacity in bytes (grown, never shrunk)
+    int prec_asm = 0;     // factor precision d_blocks0 was assembled in
+    int prec_fact = 0;    // factor precision of the current factorization
+    // GMRES workspace (FP64): Krylov basis, Z of the flexible variant and one work vector,
+    // kept across calls and only grown
+    double* d_krylov = nullptr;
+    size_t krylov_bytes = 0;
+    double* krylov_ws(size_t bytes);
 
     bool want_sparsify(const Cluster& c) const;
     int new_edge(int c1, int c2);

@@ -175,11 +175,25 @@ inline int copy_parts(const std::vector<CopyTask<S>>& v) {
 // and starting threads for each cost more than the passes.  run(T, job) runs job(t) for t in
 // [1, T) on the workers and job(0) on the caller, and returns when all have finished; callers
 // are serialized.
+// The pool is a heap object that no static destructor touches: its workers are joined by
+// shutdown() (spaqr_shutdown), and a process that exits without it simply ends them.
 class HostPool {
   public:
     static HostPool& get() {
-        static HostPool pool(host_threads() - 1);
-        return pool;
+        std::lock_guard<std::mutex> lk(inst_mutex());
+        HostPool*& p = inst();
+        if (!p) p = new HostPool(host_threads() - 1);
+        return *p;
+    }
+    // Joins the workers; a later parallel pass starts a new pool.  No pass may be running.
+    static void shutdown() {
+        HostPool* p = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(inst_mutex());
+            p = inst();
+            inst() = nullptr;
+        }
+        delete p;
     }
     void run(int T, const std::function<void(int)>& job) {
         std::lock_guard<std::mutex> serial(call_);
@@ -207,6 +221,14 @@ class HostPool {
     }
 
   private:
+    static HostPool*& inst() {
+        static HostPool* p = nullptr;
+        return p;
+    }
+    static std::mutex& inst_mutex() {
+        static std::mutex* m = new std::mutex;  // never destroyed: usable in any exit order
+        return *m;
+    }
     explicit HostPool(int workers) {
         for (int w = 1; w <= workers; ++w) th_.emplace_back([this, w] { loop(w); });
     }

@@ -23,7 +23,6 @@ constexpr int kSketchPad = 8;
 // test measures against, so stopping it early only causes fallbacks.
 constexpr double kSketchTolFactor = 0.0;
 constexpr uint64_t kSketchSeed = 0x5eed5ca1eull;
-constexpr int kSketchRounds = 3;           // first sketch plus up to two doublings
 // Sketches at least this wide are QR'd without pivoting (blocked Householder on MFMA): any
 // prefix of a Gaussian sketch spans the dominant range, so the tail test finds a rank a few
 // columns above the pivoted one, while the pivoted QR would pay one cross-workgroup barrier
@@ -146,7 +145,8 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
         // tail test fails is gathered again and retried with a sketch twice as wide (a sketch as
         // wide as min(k, w) spans the range of C, so the last round can only fail by rounding).
         std::vector<int> todo = elig;
-        for (int round = 0; round < kSketchRounds && !todo.empty(); ++round) {
+        const int rounds = std::max(1, std::min(3, rank_rounds));
+        for (int round = 0; round < rounds && !todo.empty(); ++round) {
             int smax = 1;
             for (int q : todo) smax = std::max(smax, s_of[q]);
             const S* om = omega<S>(wmax, smax);
@@ -251,7 +251,7 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
                     if (!applied[q]) continue;
                     rk[q] = (got[q] == INT_MAX) ? -1 : got[q];
                     const int kw = std::min(gq[q].k, gq[q].w);
-                    if (rk[q] < 0 && s_of[q] < kw && round + 1 < kSketchRounds) {
+                    if (rk[q] < 0 && s_of[q] < kw && round + 1 < rounds) {
                         s_of[q] = std::min(kw, 2 * s_of[q]);
                         again.push_back(q);
                         rstats.retried++;
@@ -316,6 +316,7 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
             rlog.w.push_back(gq[q].w);
             rlog.level.push_back(L);
             rlog.mode.push_back(used_sketch[q] ? rank_mode : 0);
+            rlog.order.push_back(gq_cluster[q]);
         }
     }
     return flops;

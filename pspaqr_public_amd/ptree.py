@@ -11,7 +11,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import DeviceBuffer, GmresResult, check, lib
+from ._lib import DeviceBuffer, GmresResult, check, lib, register
 
 
 def linspace_nd(n: int, d: int) -> np.ndarray:
@@ -30,12 +30,24 @@ class ParTree:
         self.nlevels = nlevels
         self._n = 0
         self._keep = []
+        register(self)
 
-    def __del__(self):
+    def close(self):
+        """Releases the tree's device memory and stream (also done at interpreter exit)."""
+        for b in ("_bx", "_by"):
+            buf = getattr(self, b, None)
+            if buf is not None:
+                buf.free()
         h = getattr(self, "_h", None)
         if h:
             lib().spaqr_tree_destroy(h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     # ---- setters (include/tree.h:137-143, include/ptree.h:123-125)
     def set_tol(self, tol: float):
@@ -139,6 +151,17 @@ class ParTree:
         if n:
             lib().spaqr_tree_rank_log(self._h, X.ctypes.data, meta.ctypes.data, n)
         return X, meta
+
+    def rank_log_clusters(self) -> np.ndarray:
+        """Cluster order (row of clusters()) of each rank-log entry."""
+        n = lib().spaqr_tree_rank_log_clusters(self._h, None, 0)
+        out = np.zeros(max(n, 0), dtype=np.int32)
+        if n > 0:
+            lib().spaqr_tree_rank_log_clusters(self._h, out.ctypes.data, n)
+        return out
+
+    def set_rank_retries(self, retries: int):
+        check(lib().spaqr_tree_set_rank_retries(self._h, int(retries)), self._h, "set_rank_retries")
 
     def rank_log_clear(self):
         check(lib().spaqr_tree_rank_log_clear(self._h), self._h, "rank_log_clear")

@@ -48,3 +48,22 @@ def test_argument_errors_without_gpu(so):
     assert L.spaqr_geqp3_batched(64, 0, None, -1.0, 0.0, None) == -4  # negative tolerance
     assert L.spaqr_geqp3_batched(64, 0, None, 0.1, 2.0, None) == -5  # tie band out of range
     assert L.spaqr_geqrt_batched(64, 0, None, None) == 0        # empty batch is a no-op
+
+
+def test_rank_path_argument_errors_without_gpu(so):
+    from pspaqr_public_amd import _lib
+
+    L = _lib.lib()
+    assert L.spaqr_colnorm_batched(16, 1, None, None) == -1
+    assert L.spaqr_colnorm_batched(64, 1, None, None) == -2
+    d = (_lib.ColNormDesc * 1)(_lib.ColNormDesc(None, None, None, 0, 5))  # k < 1
+    assert L.spaqr_colnorm_batched(64, 1, d, None) == -3
+    s = (_lib.SketchDesc * 1)(_lib.SketchDesc(1, 1, 4, 4, 4, 0))
+    assert L.spaqr_sketch_batched(64, 1, s, None, 4, None) == -4   # no Omega
+    assert L.spaqr_sketch_batched(64, 1, s, 1, 3, None) == -5      # ldo < w
+    assert L.spaqr_tail_select_batched(64, 0, None, -1.0, None) == -4
+    assert L.spaqr_rank_features_batched(32, 0, None, 0.1, None) == 0
+    assert L.spaqr_gauss_fill(64, None, 10, 10, 10, 1, None) == -2
+    assert L.spaqr_gauss_fill(64, 1, 10, 10, 5, 1, None) == -5    # ld < rows
+    assert L.spaqr_tpqrt_batched(64, 0, None, -1, None, None) == -4  # negative link count
+    assert L.spaqr_tree_set_rank_retries(None, 1) == -1           # null tree
