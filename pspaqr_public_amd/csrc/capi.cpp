@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/spaqr_gpu.h"
+#include "errors.hpp"
 #include "kernels.hpp"
 #include "launch.hpp"
 #include "learned.hpp"
@@ -29,7 +30,6 @@ static_assert(sizeof(spaqr_tpmqrt_link) == sizeof(TpmqrtLink<double>), "tpmqrt l
 static_assert(sizeof(spaqr_geqp3_desc) == sizeof(Geqp3Task<double>), "geqp3 desc layout");
 static_assert(sizeof(spaqr_spars_desc) == sizeof(SparsApplyTask<double>), "spars desc layout");
 
-static thread_local std::string g_err;
 
 // batched-kernel call: upload descriptors, launch, synchronize, free
 template <typename F>
@@ -37,8 +37,8 @@ static int run_batched(const void* host, size_t bytes, const void* host2, size_t
     hipStream_t s = (hipStream_t)stream;
     void* d = nullptr;
     void* d2 = nullptr;
-    if (hipMalloc(&d, bytes ? bytes : 8) != hipSuccess) { g_err = "hipMalloc failed"; return -100; }
-    if (bytes2 && hipMalloc(&d2, bytes2) != hipSuccess) { (void)hipFree(d); g_err = "hipMalloc failed"; return -100; }
+    if (hipMalloc(&d, bytes ? bytes : 8) != hipSuccess) { api_error() = "hipMalloc failed"; return -100; }
+    if (bytes2 && hipMalloc(&d2, bytes2) != hipSuccess) { (void)hipFree(d); api_error() = "hipMalloc failed"; return -100; }
     hipError_t e = hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess && bytes2) e = hipMemcpyAsync(d2, host2, bytes2, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
@@ -48,7 +48,7 @@ static int run_batched(const void* host, size_t bytes, const void* host2, size_t
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(d);
     if (d2) (void)hipFree(d2);
-    if (e != hipSuccess) { g_err = hipGetErrorString(e); return -100; }
+    if (e != hipSuccess) { api_error() = hipGetErrorString(e); return -100; }
     return 0;
 }
 
@@ -82,7 +82,7 @@ struct CallScratch {
     int finish() {
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) { g_err = hipGetErrorString(e); return -100; }
+        if (e != hipSuccess) { api_error() = hipGetErrorString(e); return -100; }
         return 0;
     }
 };
@@ -95,13 +95,13 @@ int spaqr_shutdown(void) {
 }
 
 spaqr_tree* spaqr_tree_create(int nlevels, int skip) {
-    if (nlevels < 1 || skip < 0) { g_err = "nlevels must be >= 1 and skip >= 0"; return nullptr; }
+    if (nlevels < 1 || skip < 0) { api_error() = "nlevels must be >= 1 and skip >= 0"; return nullptr; }
     try {
         spaqr_tree* h = new spaqr_tree;
         h->t = new Tree(nlevels, skip);
         return h;
     } catch (std::exception& e) {
-        g_err = e.what();
+        api_error() = e.what();
         return nullptr;
     }
 }
@@ -114,7 +114,7 @@ void spaqr_tree_destroy(spaqr_tree* h) {
 
 #define TREE_OR_FAIL(h)                 \
     if (!h || !h->t) {                  \
-        g_err = "null tree handle";     \
+        api_error() = "null tree handle";     \
         return -1;                      \
     }
 
@@ -225,7 +225,7 @@ double spaqr_tree_factor_ms(const spaqr_tree* h) { return (h && h->t) ? h->t->t_
 void* spaqr_tree_stream(spaqr_tree* h) { return (h && h->t) ? (void*)h->t->stream() : nullptr; }
 const char* spaqr_last_error(const spaqr_tree* h) {
     if (h && h->t && !h->t->last_error.empty()) return h->t->last_error.c_str();
-    return g_err.c_str();
+    return api_error().c_str();
 }
 
 int spaqr_tree_sync(spaqr_tree* h) {
@@ -259,14 +259,14 @@ int spaqr_tree_profile_reset(spaqr_tree* h) {
 // ---------------------------------------------------------------- rank path
 int spaqr_tree_set_rank_mode(spaqr_tree* h, int mode) {
     TREE_OR_FAIL(h);
-    if (mode < 0 || mode > 2) { g_err = "rank mode must be 0, 1 or 2"; return -2; }
+    if (mode < 0 || mode > 2) { api_error() = "rank mode must be 0, 1 or 2"; return -2; }
     h->t->rank_mode = mode;
     return 0;
 }
 int spaqr_tree_set_rank_model(spaqr_tree* h, const float* w, int nparams) {
     TREE_OR_FAIL(h);
     if (!w) return -2;
-    if (nparams != kMlpParams) { g_err = "rank model: wrong parameter count"; return -3; }
+    if (nparams != kMlpParams) { api_error() = "rank model: wrong parameter count"; return -3; }
     try {
         return h->t->set_rank_model(w);
     } catch (std::exception& e) {
@@ -327,7 +327,7 @@ int spaqr_mlp_predict(const float* X, int n, const float* W, float* out) {
     try {
         return mlp_predict_host(X, n, W, out);
     } catch (std::exception& e) {
-        g_err = e.what();
+        api_error() = e.what();
         return -100;
     }
 }
@@ -336,7 +336,7 @@ int spaqr_mlp_grad(const float* X, const float* y, int n, const float* W, float*
     try {
         return mlp_grad_host(X, y, n, W, grad, loss);
     } catch (std::exception& e) {
-        g_err = e.what();
+        api_error() = e.what();
         return -100;
     }
 }
@@ -345,14 +345,14 @@ int spaqr_mlp_train(const float* X, const float* y, int n, float* W, int epochs,
     try {
         return mlp_train_host(X, y, n, W, epochs, lr, loss_hist);
     } catch (std::exception& e) {
-        g_err = e.what();
+        api_error() = e.what();
         return -100;
     }
 }
 
 void* spaqr_device_alloc(size_t bytes) {
     void* p = nullptr;
-    if (hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) { g_err = "hipMalloc failed"; return nullptr; }
+    if (hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) { api_error() = "hipMalloc failed"; return nullptr; }
     return p;
 }
 int spaqr_device_free(void* p) { return hipFree(p) == hipSuccess ? 0 : -100; }
@@ -478,7 +478,7 @@ static int tpqrt_dispatch(int count, const spaqr_tpqrt_desc* d, int nlinks, cons
         }
         return cs.finish();
     } catch (std::exception& e) {
-        g_err = e.what();
+        api_error() = e.what();
         (void)hipStreamSynchronize(s);
         return -100;
     }
@@ -580,10 +580,10 @@ int spaqr_geqp3_batched(int prec, int count, const spaqr_geqp3_desc* d, double t
         int herr = 0;
         hipError_t e = hipMemcpyAsync(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) { g_err = hipGetErrorString(e); rc = -100; }
-        else if (herr) { g_err = "cooperative geqp3: barrier wait exceeded its spin limit"; rc = -100; }
+        if (e != hipSuccess) { api_error() = hipGetErrorString(e); rc = -100; }
+        else if (herr) { api_error() = "cooperative geqp3: barrier wait exceeded its spin limit"; rc = -100; }
     } catch (std::exception& ex) {
-        g_err = ex.what();
+        api_error() = ex.what();
         (void)hipStreamSynchronize(s);
         rc = -100;
     }
@@ -649,7 +649,7 @@ int spaqr_gauss_fill(int prec, void* om, int rows, int cols, int ld, uint64_t se
     else launch_gauss<float>((float*)om, rows, cols, ld, seed, s);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) { g_err = hipGetErrorString(e); return -100; }
+    if (e != hipSuccess) { api_error() = hipGetErrorString(e); return -100; }
     return 0;
 }
 
