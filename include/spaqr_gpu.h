@@ -197,6 +197,18 @@ int spaqr_tpmqrt_batched(int prec, int count, const spaqr_tpmqrt_desc* d, int nl
 typedef struct { void* C; void* Vout; void* tau; void* T; void* vn1; void* vn2; int* jpvt; int* rank_out; int k, w, pad0, pad1; } spaqr_geqp3_desc;
 int spaqr_geqp3_batched(int prec, int count, const spaqr_geqp3_desc* d, double tol, double tie_rel, void* stream);
 
+/* spaqr_geqp3_batched without any device allocation: same arguments, kernels and results.
+   No counterpart in the reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its scratch
+   (cooperative exchange slots, barrier counters, error flag, descriptor tables) lives in a
+   fixed array of spaqr_geqp3_noalloc_scratch_bytes() bytes (16 MiB) that each GPU holds from
+   library load to process exit.  A call needing more returns -6 before launching anything.
+   Calls that overlap on one GPU share that array and race: issue them one at a time per GPU
+   (every call synchronizes `stream` before returning). */
+int spaqr_geqp3_batched_noalloc(int prec, int count, const spaqr_geqp3_desc* d, double tol, double tie_rel,
+                                void* stream);
+int spaqr_geqp3_noalloc_scratch_bytes(void);
+
 /* Sparsification apply + scatter (src/ptree.cpp:1867-1897 then :330): A (mc x nc) <- A Q_n
    (Vn != NULL, first rn reflectors of Vn kn x rn), A[0:kc, 0:rn] <- Q_c^T A (Vc != NULL,
    first rc reflectors), dst(i, j) = A(i, j) for i < rc, j < rn; ident != 0 writes I_rc.

@@ -140,6 +140,8 @@ EXPORTS = {
     "spaqr_rank_features_batched": (_i, [_i, _i, _vp, _d, _vp]),
     "spaqr_tail_select_batched": (_i, [_i, _i, _vp, _d, _vp]),
     "spaqr_shutdown": (_i, []),
+    "spaqr_geqp3_batched_noalloc": (_i, [_i, _i, _vp, _d, _d, _vp]),
+    "spaqr_geqp3_noalloc_scratch_bytes": (_i, []),
 }
 
 # Objects holding device memory or a tree (ParTree, DeviceBuffer).  At interpreter exit they are
