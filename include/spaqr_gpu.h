@@ -63,6 +63,26 @@ int spaqr_tree_set_precision(spaqr_tree* t, int bits);
 /* ParTree::partition(SpMat& A): A is n x n CSC (host), sorted row indices. src/ptree.cpp:495 */
 int spaqr_tree_partition(spaqr_tree* t, int nrows, int ncols, const int* colptr, const int* rowind,
                          const double* val);
+/* A batch of independent square systems factored together (no reference counterpart: the
+   reference factors one system per run).  A (n x n CSC, host) is their block-diagonal matrix,
+   system s owning unknowns [off_s, off_s + sizes[s]) in order; the coordinates (set_Xcoo) cover
+   all n unknowns.  Each system gets its own nested dissection (as spaqr_tree_partition) with
+   levels[s] levels; the tree must have been created with the largest of them.  assemble,
+   factorize, solve and spmv then act on all systems at once, every tree level's phases running
+   as one set of grouped launches over the systems' blocks of uneven sizes.  The permutation
+   keeps each system's unknowns in its own range. */
+int spaqr_tree_partition_batch(spaqr_tree* t, int nsys, const int* sizes, const int* levels, int n,
+                               const int* colptr, const int* rowind, const double* val);
+int spaqr_tree_num_systems(const spaqr_tree* t);
+/* per system: first position, size (2 ints) */
+int spaqr_tree_get_systems(const spaqr_tree* t, int* out);
+/* system of each cluster (rows of spaqr_tree_get_clusters) */
+int spaqr_tree_get_cluster_systems(const spaqr_tree* t, int* out);
+/* GMRES on every system of a batch (spaqr_gmres, or spaqr_fgmres with flexible != 0, per
+   system: its own Krylov space, stopping test and counts) with shared SpMV and preconditioner
+   launches; out gets one result per system. */
+int spaqr_gmres_batch(spaqr_tree* t, const double* b_dev, double* x_dev, int iters, double tol, int restart,
+                      double true_tol, int flexible, spaqr_gmres_result* out);
 /* ParTree::assemble(SpMat& A): builds the blocks and uploads them to HBM. src/ptree.cpp:696 */
 int spaqr_tree_assemble(spaqr_tree* t);
 /* ParTree::factorize()                                                 src/ptree.cpp:873 */

@@ -142,6 +142,11 @@ EXPORTS = {
     "spaqr_shutdown": (_i, []),
     "spaqr_geqp3_batched_noalloc": (_i, [_i, _i, _vp, _d, _d, _vp]),
     "spaqr_geqp3_noalloc_scratch_bytes": (_i, []),
+    "spaqr_tree_partition_batch": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp, _vp]),
+    "spaqr_tree_num_systems": (_i, [_vp]),
+    "spaqr_tree_get_systems": (_i, [_vp, _vp]),
+    "spaqr_tree_get_cluster_systems": (_i, [_vp, _vp]),
+    "spaqr_gmres_batch": (_i, [_vp, _vp, _vp, _i, _d, _i, _d, _i, _vp]),
 }
 
 # Objects holding device memory or a tree (ParTree, DeviceBuffer).  At interpreter exit they are

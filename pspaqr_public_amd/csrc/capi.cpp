@@ -163,6 +163,49 @@ int spaqr_tree_partition(spaqr_tree* h, int nrows, int ncols, const int* colptr,
     A.val.assign(val, val + colptr[ncols]);
     return h->t->partition(A);
 }
+int spaqr_tree_partition_batch(spaqr_tree* h, int nsys, const int* sizes, const int* levels, int n, const int* colptr,
+                               const int* rowind, const double* val) {
+    TREE_OR_FAIL(h);
+    if (nsys < 1 || !sizes || !levels) return -2;
+    if (n <= 0 || !colptr || !rowind || !val) return -5;
+    HostCSC A;
+    A.nr = A.nc = n;
+    A.colptr.assign(colptr, colptr + n + 1);
+    A.rowind.assign(rowind, rowind + colptr[n]);
+    A.val.assign(val, val + colptr[n]);
+    return h->t->partition_batch(A, std::vector<int>(sizes, sizes + nsys), std::vector<int>(levels, levels + nsys));
+}
+int spaqr_tree_num_systems(const spaqr_tree* h) { return (h && h->t) ? h->t->n_systems() : -1; }
+int spaqr_tree_get_systems(const spaqr_tree* h, int* out) {
+    TREE_OR_FAIL(h);
+    if (!out) return -2;
+    for (int s = 0; s < h->t->n_systems(); ++s) {
+        out[2 * s] = h->t->system_offset(s);
+        out[2 * s + 1] = h->t->system_size(s);
+    }
+    return 0;
+}
+int spaqr_tree_get_cluster_systems(const spaqr_tree* h, int* out) {
+    TREE_OR_FAIL(h);
+    if (!out) return -2;
+    for (int o = 0; o < h->t->num_clusters(); ++o) out[o] = h->t->cluster(o).inst;
+    return 0;
+}
+int spaqr_gmres_batch(spaqr_tree* h, const double* b, double* x, int iters, double tol, int restart, double true_tol,
+                      int flexible, spaqr_gmres_result* out) {
+    TREE_OR_FAIL(h);
+    if (!b || !x || iters <= 0) return -2;
+    std::vector<GmresResult> r;
+    const int rc = h->t->gmres_batch_dev(b, x, iters, tol, restart, true_tol, flexible != 0, r);
+    if (rc == 0 && out)
+        for (size_t s = 0; s < r.size(); ++s) {
+            out[s].iters = r[s].iters;
+            out[s].cycles = r[s].cycles;
+            out[s].prec_res = r[s].prec_res;
+            out[s].true_res = r[s].true_res;
+        }
+    return rc;
+}
 int spaqr_tree_assemble(spaqr_tree* h) { TREE_OR_FAIL(h); return h->t->assemble(); }
 int spaqr_tree_factorize(spaqr_tree* h) { TREE_OR_FAIL(h); return h->t->factorize(); }
 int spaqr_tree_solve(spaqr_tree* h, double* x) { TREE_OR_FAIL(h); if (!x) return -2; return h->t->solve_dev(x); }

@@ -85,4 +85,19 @@ void launch_axpy_dev(int n, const double* h, const double* x, double* y, hipStre
 void launch_scal_dev(int n, const double* h, const double* x, double* y, hipStream_t s);
 void launch_axpby(int n, double a, const double* x, double b, double* y, hipStream_t s);
 
+// Batches of systems (krylov.hip): system s owns positions [off[s], off[s + 1]); off is a device
+// array of nseg + 1 ints.  partial needs seg_partials(nseg) doubles.
+int seg_partials(int nseg);
+// out[s * ostride] = <a, b> on system s (its square root if sqrt_mode)
+void launch_seg_dot(int nseg, const int* off, const double* a, const double* b, double* partial, double* out,
+                    int ostride, int sqrt_mode, hipStream_t s);
+// y <- y - h[s * hstride] x
+void launch_seg_axpy(int nseg, const int* off, const double* h, int hstride, const double* x, double* y, hipStream_t s);
+// y <- x / h[s * hstride] where keep[s] and h > 0, else y <- 0
+void launch_seg_scal(int nseg, const int* off, const double* h, int hstride, const int* keep, const double* x,
+                     double* y, hipStream_t s);
+// x <- x + coef[s * cstride] v
+void launch_seg_axpy_coef(int nseg, const int* off, const double* coef, int cstride, const double* v, double* x,
+                          hipStream_t s);
+
 }  // namespace spaqr

@@ -291,13 +291,14 @@ Tree::~Tree() {
     if (d_blocks0) (void)hipFree(d_blocks0);
     if (xw) (void)hipFree(xw);
     if (d_krylov) (void)hipFree(d_krylov);
+    if (d_seg) (void)hipFree(d_seg);
     if (d_mlp_w) (void)hipFree(d_mlp_w);
     if (d_omega) (void)hipFree(d_omega);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-bool Tree::want_sparsify(const Cluster& c) const {  // src/ptree.cpp:29-41
-    if (c.merge_level < skip || c.merge_level >= nlevels - 2 || tol == 0) return false;
+bool Tree::want_sparsify(const Cluster& c) const {  // src/ptree.cpp:29-41, with the levels of c's system
+    if (c.merge_level < skip || c.merge_level >= inst_lv[c.inst] - 2 || tol == 0) return false;
     if (c.lvl() > c.merge_level && c.id.l.lvl <= c.merge_level && c.id.r.lvl <= c.merge_level) return true;
     return false;
 }
@@ -386,125 +387,132 @@ int Tree::new_edge(int c1, int c2) {  // new_edgeOut, src/ptree.cpp:101-120
 }
 
 // ---------------------------------------------------------------------------- partition
-int Tree::partition(const HostCSC& A) {
-    try {
-        if (A.nr != A.nc) throw std::runtime_error("only square systems are supported");
-        if (X.empty() || xdim <= 0) throw std::runtime_error("geometric partition needs coordinates (set_Xcoo)");
-        A0 = A;
-        n = A.nc;
-        if ((long long)X.size() != (long long)xdim * n) throw std::runtime_error("coordinate array size mismatch");
-        // recursive geometric bisection (src/partition.cpp:241-304)
-        HostCSC G = ata_pattern(A);
-        std::vector<int> parts(n, 0);
-        std::vector<double> vals;
-        for (int depth = 0; depth < nlevels - 1; ++depth) {
-            const int np = 1 << depth;
-            std::vector<std::vector<int>> dofs(np);
-            for (int i = 0; i < n; ++i) dofs[parts[i]].push_back(i);
-            for (int k = 0; k < np; ++k) {
-                const auto& d = dofs[k];
-                const int N = (int)d.size();
-                if (N == 0) continue;
-                int best = -1;
-                double range = -1.0;
-                for (int dim = 0; dim < xdim; ++dim) {
-                    double lo = INFINITY, hi = -INFINITY;
-                    for (int i : d) { double v = X[dim + (size_t)i * xdim]; lo = std::min(lo, v); hi = std::max(hi, v); }
-                    if (hi - lo > range) { range = hi - lo; best = dim; }
-                }
-                vals.resize(N);
-                for (int i = 0; i < N; ++i) vals[i] = X[best + (size_t)d[i] * xdim];
-                std::nth_element(vals.begin(), vals.begin() + N / 2, vals.end());
-                const double mid = vals[N / 2];
-                for (int i : d) parts[i] = (X[best + (size_t)i * xdim] <= mid) ? 2 * k : 2 * k + 1;
+// Ordering of one square system (A n x n, coordinates X d x n): its column permutation and the
+// cluster ID of every permuted position.
+static void order_system(const HostCSC& A, const double* Xc, int xdim, int nlevels, std::vector<int>& cperm_,
+                         std::vector<ClusterID>& cpermed) {
+    const int n = A.nc;
+    auto X = [&](size_t idx) { return Xc[idx]; };
+    // recursive geometric bisection (src/partition.cpp:241-304)
+    HostCSC G = ata_pattern(A);
+    std::vector<int> parts(n, 0);
+    std::vector<double> vals;
+    for (int depth = 0; depth < nlevels - 1; ++depth) {
+        const int np = 1 << depth;
+        std::vector<std::vector<int>> dofs(np);
+        for (int i = 0; i < n; ++i) dofs[parts[i]].push_back(i);
+        for (int k = 0; k < np; ++k) {
+            const auto& d = dofs[k];
+            const int N = (int)d.size();
+            if (N == 0) continue;
+            int best = -1;
+            double range = -1.0;
+            for (int dim = 0; dim < xdim; ++dim) {
+                double lo = INFINITY, hi = -INFINITY;
+                for (int i : d) { double v = X(dim + (size_t)i * xdim); lo = std::min(lo, v); hi = std::max(hi, v); }
+                if (hi - lo > range) { range = hi - lo; best = dim; }
             }
+            vals.resize(N);
+            for (int i = 0; i < N; ++i) vals[i] = X(best + (size_t)d[i] * xdim);
+            std::nth_element(vals.begin(), vals.begin() + N / 2, vals.end());
+            const double mid = vals[N / 2];
+            for (int i : d) parts[i] = (X(best + (size_t)i * xdim) <= mid) ? 2 * k : 2 * k + 1;
         }
-        // separators: left-part vertices with an AtA neighbour in the right part (:306-333)
-        std::vector<ClusterID> cmap(n);
-        for (int depth = 0; depth < nlevels - 1; ++depth) {
-            const int l = nlevels - depth - 1;
-            const int sh = l - 1;
-            for (int i = 0; i < n; ++i) {
-                if (cmap[i].self.lvl != -1) continue;
-                const int pi = parts[i] >> sh;
-                if (pi & 1) continue;
-                bool sep = false;
-                for (int k = G.colptr[i]; k < G.colptr[i + 1] && !sep; ++k) {
-                    int j = G.rowind[k];
-                    if (cmap[j].self.lvl == -1 && (parts[j] >> sh) == pi + 1) sep = true;
-                }
-                if (sep) { cmap[i].self = SepID(l, pi / 2); cmap[i].section = parts[i]; }
-            }
-        }
-        for (int i = 0; i < n; ++i)
-            if (cmap[i].self.lvl == -1) {
-                cmap[i].self = cmap[i].l = cmap[i].r = SepID(0, parts[i]);
-                cmap[i].section = parts[i];
-            }
-        // interfaces from the rows of A (src/partition.cpp:514-587), rmap == cmap (square)
-        HostCSC At = csc_transpose(A);
-        std::vector<ClusterID> rmap = cmap;
+    }
+    // separators: left-part vertices with an AtA neighbour in the right part (:306-333)
+    std::vector<ClusterID> cmap(n);
+    for (int depth = 0; depth < nlevels - 1; ++depth) {
+        const int l = nlevels - depth - 1;
+        const int sh = l - 1;
         for (int i = 0; i < n; ++i) {
-            const SepID self = rmap[i].self;
-            if (self.lvl <= 0) continue;
-            SepID left(-1, 0), right(-1, 0);
+            if (cmap[i].self.lvl != -1) continue;
+            const int pi = parts[i] >> sh;
+            if (pi & 1) continue;
+            bool sep = false;
+            for (int k = G.colptr[i]; k < G.colptr[i + 1] && !sep; ++k) {
+                int j = G.rowind[k];
+                if (cmap[j].self.lvl == -1 && (parts[j] >> sh) == pi + 1) sep = true;
+            }
+            if (sep) { cmap[i].self = SepID(l, pi / 2); cmap[i].section = parts[i]; }
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        if (cmap[i].self.lvl == -1) {
+            cmap[i].self = cmap[i].l = cmap[i].r = SepID(0, parts[i]);
+            cmap[i].section = parts[i];
+        }
+    // interfaces from the rows of A (src/partition.cpp:514-587), rmap == cmap (square)
+    HostCSC At = csc_transpose(A);
+    std::vector<ClusterID> rmap = cmap;
+    for (int i = 0; i < n; ++i) {
+        const SepID self = rmap[i].self;
+        if (self.lvl <= 0) continue;
+        SepID left(-1, 0), right(-1, 0);
+        for (int k = At.colptr[i]; k < At.colptr[i + 1]; ++k) {
+            const SepID nb = cmap[At.rowind[k]].self;
+            if (nb.lvl < self.lvl) {
+                int pj = nb.sep >> (self.lvl - nb.lvl - 1);
+                if ((pj & 1) == 0) left = (left.lvl == -1) ? nb : lowest_common(nb, left);
+                else right = (right.lvl == -1) ? nb : lowest_common(nb, right);
+            }
+        }
+        rmap[i].l = left;
+        rmap[i].r = right;
+    }
+    cmap = rmap;
+    for (int i = 0; i < n; ++i) {
+        const SepID self = rmap[i].self;
+        SepID left = rmap[i].l, right = rmap[i].r;
+        if (right == SepID()) {
             for (int k = At.colptr[i]; k < At.colptr[i + 1]; ++k) {
-                const SepID nb = cmap[At.rowind[k]].self;
-                if (nb.lvl < self.lvl) {
-                    int pj = nb.sep >> (self.lvl - nb.lvl - 1);
-                    if ((pj & 1) == 0) left = (left.lvl == -1) ? nb : lowest_common(nb, left);
-                    else right = (right.lvl == -1) ? nb : lowest_common(nb, right);
-                }
+                const ClusterID& cj = cmap[At.rowind[k]];
+                if (self == cj.self && cj.r != SepID()) right = (right == SepID()) ? cj.r : highest_common(right, cj.r);
             }
-            rmap[i].l = left;
-            rmap[i].r = right;
-        }
-        cmap = rmap;
-        for (int i = 0; i < n; ++i) {
-            const SepID self = rmap[i].self;
-            SepID left = rmap[i].l, right = rmap[i].r;
-            if (right == SepID()) {
-                for (int k = At.colptr[i]; k < At.colptr[i + 1]; ++k) {
-                    const ClusterID& cj = cmap[At.rowind[k]];
-                    if (self == cj.self && cj.r != SepID()) right = (right == SepID()) ? cj.r : highest_common(right, cj.r);
-                }
-            } else if (left == SepID()) {
-                for (int k = At.colptr[i]; k < At.colptr[i + 1]; ++k) {
-                    const ClusterID& cj = cmap[At.rowind[k]];
-                    if (self == cj.self && cj.l != SepID()) left = (left == SepID()) ? cj.l : highest_common(left, cj.l);
-                }
+        } else if (left == SepID()) {
+            for (int k = At.colptr[i]; k < At.colptr[i + 1]; ++k) {
+                const ClusterID& cj = cmap[At.rowind[k]];
+                if (self == cj.self && cj.l != SepID()) left = (left == SepID()) ? cj.l : highest_common(left, cj.l);
             }
-            rmap[i].l = left;
-            rmap[i].r = right;
         }
-        for (int i = 0; i < n; ++i) {
-            if (rmap[i].l == SepID()) rmap[i].l = rmap[i].self;
-            if (rmap[i].r == SepID()) rmap[i].r = rmap[i].self;
-        }
-        cmap = rmap;
-        // initPermutation: stable sorts by the successively merged IDs
-        cperm_.resize(n);
-        for (int i = 0; i < n; ++i) cperm_[i] = i;
-        std::vector<ClusterID> pm = cmap;
-        auto cmp = [&pm](int a, int b) { return pm[a] < pm[b]; };
+        rmap[i].l = left;
+        rmap[i].r = right;
+    }
+    for (int i = 0; i < n; ++i) {
+        if (rmap[i].l == SepID()) rmap[i].l = rmap[i].self;
+        if (rmap[i].r == SepID()) rmap[i].r = rmap[i].self;
+    }
+    cmap = rmap;
+    // initPermutation: stable sorts by the successively merged IDs
+    cperm_.resize(n);
+    for (int i = 0; i < n; ++i) cperm_[i] = i;
+    std::vector<ClusterID> pm = cmap;
+    auto cmp = [&pm](int a, int b) { return pm[a] < pm[b]; };
+    std::stable_sort(cperm_.begin(), cperm_.end(), cmp);
+    for (int lvl = 1; lvl < nlevels; ++lvl) {
+        for (auto& s : pm) s = merge_if(s, lvl);
         std::stable_sort(cperm_.begin(), cperm_.end(), cmp);
-        for (int lvl = 1; lvl < nlevels; ++lvl) {
-            for (auto& s : pm) s = merge_if(s, lvl);
-            std::stable_sort(cperm_.begin(), cperm_.end(), cmp);
-        }
-        cpermed.resize(n);
-        for (int k = 0; k < n; ++k) cpermed[k] = cmap[cperm_[k]];
-        // clusters: runs of equal IDs at level 0, then grouped parents per level
-        cl.clear();
-        for (auto& v : bottoms) v.clear();
-        for (auto& v : interiors) v.clear();
-        for (auto& v : interfaces) v.clear();
-        for (int k = 0; k < n;) {
+    }
+    cpermed.resize(n);
+    for (int k = 0; k < n; ++k) cpermed[k] = cmap[cperm_[k]];
+}
+
+// The cluster hierarchy from the ordering (src/ptree.cpp:587-691): level-0 clusters are the runs
+// of equal IDs, parents are grouped per level by merge_if.  Systems of a batch are kept apart:
+// each owns a contiguous range of positions, and its clusters exist up to its own top level.
+void Tree::make_hierarchy() {
+    cl.clear();
+    for (auto& v : bottoms) v.clear();
+    for (auto& v : interiors) v.clear();
+    for (auto& v : interfaces) v.clear();
+    for (size_t s = 0; s < inst_n.size(); ++s) {
+        const int k1 = inst_off[s] + inst_n[s];
+        for (int k = inst_off[s]; k < k1;) {
             int kn = k + 1;
-            while (kn < n && cpermed[kn] == cpermed[k]) kn++;
+            while (kn < k1 && cpermed[kn] == cpermed[k]) kn++;
             Cluster c;
             c.order = (int)cl.size();
             c.merge_level = 0;
+            c.inst = (int)s;
             c.id = cpermed[k];
             c.cstart = k;
             c.rows = c.cols = c.rows_org = c.cols_org = c.rank = kn - k;
@@ -513,34 +521,116 @@ int Tree::partition(const HostCSC& A) {
             (c.lvl() == 0 ? interiors[0] : interfaces[0]).push_back(c.order);
             k = kn;
         }
-        for (int lvl = 1; lvl < nlevels; ++lvl) {
-            const auto& prev = bottoms[lvl - 1];
-            size_t b = 0;
-            while (b < prev.size() && cl[prev[b]].lvl() < lvl) b++;
-            for (size_t k = b; k < prev.size();) {
-                const ClusterID pid = merge_if(cl[prev[k]].id, lvl);
-                std::vector<int> ch;
-                int sz = 0;
-                const int cst = cl[prev[k]].cstart;
-                while (k < prev.size() && merge_if(cl[prev[k]].id, lvl) == pid) {
-                    ch.push_back(prev[k]);
-                    sz += cl[prev[k]].cols;
-                    k++;
+    }
+    for (int lvl = 1; lvl < nlevels; ++lvl) {
+        const auto prev = bottoms[lvl - 1];
+        for (size_t b0 = 0; b0 < prev.size();) {
+            const int s = cl[prev[b0]].inst;
+            size_t e0 = b0;
+            while (e0 < prev.size() && cl[prev[e0]].inst == s) e0++;
+            if (lvl < inst_lv[s]) {
+                size_t b = b0;
+                while (b < e0 && cl[prev[b]].lvl() < lvl) b++;
+                for (size_t k = b; k < e0;) {
+                    const ClusterID pid = merge_if(cl[prev[k]].id, lvl);
+                    std::vector<int> ch;
+                    int sz = 0;
+                    const int cst = cl[prev[k]].cstart;
+                    while (k < e0 && merge_if(cl[prev[k]].id, lvl) == pid) {
+                        ch.push_back(prev[k]);
+                        sz += cl[prev[k]].cols;
+                        k++;
+                    }
+                    Cluster p;
+                    p.order = (int)cl.size();
+                    p.merge_level = lvl;
+                    p.inst = s;
+                    p.id = pid;
+                    p.cstart = cst;
+                    p.rows = p.cols = p.rows_org = p.cols_org = p.rank = sz;
+                    std::sort(ch.begin(), ch.end(), [this](int a, int b2) { return cl[a].id < cl[b2].id; });
+                    p.children = ch;
+                    for (int c : ch) cl[c].parent = p.order;
+                    cl.push_back(p);
+                    bottoms[lvl].push_back(p.order);
+                    (pid.self.lvl == lvl ? interiors[lvl] : interfaces[lvl]).push_back(p.order);
                 }
-                Cluster p;
-                p.order = (int)cl.size();
-                p.merge_level = lvl;
-                p.id = pid;
-                p.cstart = cst;
-                p.rows = p.cols = p.rows_org = p.cols_org = p.rank = sz;
-                std::sort(ch.begin(), ch.end(), [this](int a, int b2) { return cl[a].id < cl[b2].id; });
-                p.children = ch;
-                for (int c : ch) cl[c].parent = p.order;
-                cl.push_back(p);
-                bottoms[lvl].push_back(p.order);
-                (pid.self.lvl == lvl ? interiors[lvl] : interfaces[lvl]).push_back(p.order);
             }
+            b0 = e0;
         }
+    }
+}
+
+int Tree::partition(const HostCSC& A) {
+    try {
+        if (A.nr != A.nc) throw std::runtime_error("only square systems are supported");
+        if (X.empty() || xdim <= 0) throw std::runtime_error("geometric partition needs coordinates (set_Xcoo)");
+        A0 = A;
+        n = A.nc;
+        if ((long long)X.size() != (long long)xdim * n) throw std::runtime_error("coordinate array size mismatch");
+        inst_n.assign(1, n);
+        inst_lv.assign(1, nlevels);
+        inst_off.assign(1, 0);
+        order_system(A, X.data(), xdim, nlevels, cperm_, cpermed);
+        make_hierarchy();
+        return 0;
+    } catch (std::exception& e) {
+        last_error = e.what();
+        return -1;
+    }
+}
+
+int Tree::partition_batch(const HostCSC& A, const std::vector<int>& sizes, const std::vector<int>& levels) {
+    try {
+        if (A.nr != A.nc) throw std::runtime_error("only square systems are supported");
+        if (sizes.empty() || sizes.size() != levels.size()) throw std::runtime_error("one size and one level count per system");
+        int total = 0, lmax = 0;
+        for (size_t s = 0; s < sizes.size(); ++s) {
+            if (sizes[s] < 1 || levels[s] < 1) throw std::runtime_error("system sizes and level counts must be positive");
+            total += sizes[s];
+            lmax = std::max(lmax, levels[s]);
+        }
+        if (total != A.nc) throw std::runtime_error("the system sizes must add up to the matrix size");
+        if (lmax != nlevels) throw std::runtime_error("the tree's level count must equal the largest system's");
+        if (X.empty() || xdim <= 0) throw std::runtime_error("geometric partition needs coordinates (set_Xcoo)");
+        n = A.nc;
+        if ((long long)X.size() != (long long)xdim * n) throw std::runtime_error("coordinate array size mismatch");
+        inst_n = sizes;
+        inst_lv = levels;
+        inst_off.assign(sizes.size() + 1, 0);
+        for (size_t s = 0; s < sizes.size(); ++s) inst_off[s + 1] = inst_off[s] + sizes[s];
+        for (size_t s = 0; s < sizes.size(); ++s)  // no coupling between systems
+            for (int j = inst_off[s]; j < inst_off[s + 1]; ++j)
+                for (int k = A.colptr[j]; k < A.colptr[j + 1]; ++k)
+                    if (A.rowind[k] < inst_off[s] || A.rowind[k] >= inst_off[s + 1])
+                        throw std::runtime_error("the batch matrix must be block diagonal over the systems");
+        A0 = A;
+        cperm_.assign(n, 0);
+        cpermed.assign(n, ClusterID());
+        parallel_ranges((int)sizes.size(), 1, [&](int b0, int e0, int) {
+            for (int s = b0; s < e0; ++s) {
+                const int off = inst_off[s], ns = sizes[s];
+                HostCSC As;
+                As.nr = As.nc = ns;
+                As.colptr.assign(ns + 1, 0);
+                for (int j = 0; j < ns; ++j) {
+                    for (int k = A.colptr[off + j]; k < A.colptr[off + j + 1]; ++k) {
+                        As.rowind.push_back(A.rowind[k] - off);
+                        As.val.push_back(A.val[k]);
+                    }
+                    As.colptr[j + 1] = (int)As.rowind.size();
+                }
+                std::vector<int> cp;
+                std::vector<ClusterID> ids;
+                order_system(As, X.data() + (size_t)off * xdim, xdim, levels[s], cp, ids);
+                for (int k = 0; k < ns; ++k) {
+                    cperm_[off + k] = off + cp[k];
+                    cpermed[off + k] = ids[k];
+                }
+            }
+        });
+        inst_off.pop_back();
+        make_hierarchy();
         return 0;
     } catch (std::exception& e) {
         last_error = e.what();
@@ -1343,7 +1433,9 @@ int Tree::factorize_t() {
         std::vector<char> used_sketch;
         int* d_ranks = nullptr;
         int* d_rrqr_err = nullptr;
-        const bool spars_level = (L >= skip && L < nlevels - 2 && tol != 0);
+        // the level guard of src/ptree.cpp:1566,1635; want_sparsify applies each system's own
+        // top-two-levels rule
+        const bool spars_level = (L >= skip && tol != 0);
         if (spars_level) {
             std::vector<int> who;
             for (int c : itf)

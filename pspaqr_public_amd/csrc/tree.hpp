@@ -42,6 +42,7 @@ struct HostCSC {
 
 struct Cluster {
     int order = -1, merge_level = 0;
+    int inst = 0;                         // system of a batch this cluster belongs to
     ClusterID id;
     int cstart = 0;
     int rows = 0, cols = 0, rows_org = 0, cols_org = 0, rank = 0;
@@ -115,6 +116,19 @@ public:
     int xdim = 0;
 
     int partition(const HostCSC& A);
+    // A batch of independent square systems factored together: A is their block-diagonal
+    // matrix (system s owns unknowns [off_s, off_s + n_s), in order), X the coordinates of all
+    // unknowns; system s gets its own nested dissection with levels[s] levels.  Every tree
+    // level's phases then run as one set of grouped launches over all systems.
+    int partition_batch(const HostCSC& A, const std::vector<int>& sizes, const std::vector<int>& levels);
+    int n_systems() const { return (int)inst_n.size(); }
+    int system_offset(int s) const { return inst_off[s]; }
+    int system_size(int s) const { return inst_n[s]; }
+    // GMRES on every system of the batch at once (one Krylov space per system, shared SpMV and
+    // preconditioner launches, per-system reductions), each with the stopping rules of
+    // gmres_dev; res gets one entry per system.
+    int gmres_batch_dev(const double* b, double* x, int iters, double tol, int restart, double true_tol, bool flexible,
+                        std::vector<GmresResult>& res);
     int assemble();
     int factorize();
     int solve_dev(double* x);                       // preconditioner apply, in place, device vector
@@ -157,6 +171,9 @@ private:
     int n = 0;
     HostCSC A0, App;
     std::vector<int> cperm_;
+    std::vector<int> inst_n, inst_lv, inst_off;  // per system: unknowns, levels, first position
+    int* d_seg = nullptr;                        // device copy of inst_off plus n (batch GMRES)
+    void make_hierarchy();
     std::vector<Cluster> cl0;                 // post-assembly structure, restored by factorize()
     std::vector<Edge> ed0;
     void* d_blocks0 = nullptr;                // pristine assembled blocks (factor precision)
