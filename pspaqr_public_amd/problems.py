@@ -184,6 +184,68 @@ class Problem:
     meta: dict = field(default_factory=dict)
 
 
+@dataclass
+class Batch:
+    """Independent systems factored together: A is their block-diagonal matrix (system i owns
+    unknowns offsets[i] .. offsets[i] + sizes[i] - 1, each in its natural grid ordering), X the
+    coordinates of all unknowns (d x n), b the right-hand sides stacked the same way."""
+    name: str
+    systems: list
+    A: sp.csc_matrix
+    b: np.ndarray
+    X: np.ndarray
+    sizes: list
+    levels: list
+    tol: float
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def offsets(self):
+        return np.concatenate([[0], np.cumsum(self.sizes)[:-1]]).astype(int)
+
+
+def _coords(n: int, d: int) -> np.ndarray:
+    grids = np.meshgrid(*[np.arange(n, dtype=np.float64)] * d, indexing="ij")
+    return np.stack([g.ravel() for g in grids], axis=0)
+
+
+BATCH_SIDES = (16, 24, 32, 40, 48)
+
+
+def make_batch(count: int = 48, sides=BATCH_SIDES, seed0: int = 3, tol: float = 1e-2) -> Batch:
+    """The batch workload: ``count`` 3D 7-point diffusion systems, system i with seed seed0 + i.
+    From std::mt19937(seed) the first canonical draw picks the grid side uniformly from ``sides``,
+    the second s ~ U[0.5, 2]; the following draws give the cell coefficients exp(N(0, s^2))
+    (Box-Muller, harmonic face averages as in config 2); the right-hand side is U[-1, 1] from
+    seed + 1000.  Levels by the driver's default rule (pspaQR.cpp:108-113)."""
+    systems, blocks, bs, Xs, sizes, levels = [], [], [], [], [], []
+    for i in range(count):
+        seed = seed0 + i
+        n = sides[min(len(sides) - 1, int(mt_canonical(seed, 1)[0] * len(sides)))]
+        sig = 0.5 + 1.5 * mt_canonical(seed, 2)[1]
+        N = n ** 3
+        u = mt_canonical(seed, 2 + 2 * ((N + 1) // 2))[2:]
+        u1, u2 = u[0::2], u[1::2]
+        rad = np.sqrt(-2.0 * np.log1p(-u1))
+        z = np.empty(2 * len(u1))
+        z[0::2] = rad * np.cos(2.0 * math.pi * u2)
+        z[1::2] = rad * np.sin(2.0 * math.pi * u2)
+        kcell = np.exp(sig * z[:N]).reshape((n,) * 3)
+        A = column_scale(_assemble(n, 3, _harmonic_faces(kcell, 3)))
+        b = mt_uniform(seed + 1000, N)
+        lv = default_levels(N)
+        systems.append(Problem("diffusion3d_%d_s%d" % (n, seed), A, b, n, 3, lv, tol, meta={"sigma": sig}))
+        blocks.append(A)
+        bs.append(b)
+        Xs.append(_coords(n, 3))
+        sizes.append(N)
+        levels.append(lv)
+    A = sp.block_diag(blocks, format="csc")
+    A.sort_indices()
+    return Batch("batch%d_diffusion3d" % count, systems, A, np.concatenate(bs), np.asfortranarray(np.concatenate(Xs, axis=1)),
+                 sizes, levels, tol)
+
+
 def default_levels(ncols: int) -> int:
     """nlevels = ceil(log2(ncols/64)) with integer division (pspaQR.cpp:108-113)."""
     return int(math.ceil(math.log2(ncols // 64)))
@@ -215,4 +277,6 @@ def make_problem(cfg: int | str, scale_side: int | None = None) -> Problem:
         b = mt_normal(7 + 1000, n ** 3)
         lv = default_levels(n ** 3)
         return Problem("anisotropic3d_%d" % n, column_scale(A), b, n, 3, lv, 1e-2)
+    if cfg in (4, "batch"):
+        raise ValueError("config 4 is a batch of systems: use make_batch()")
     raise ValueError("unknown config %r" % (cfg,))

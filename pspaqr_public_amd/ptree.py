@@ -88,6 +88,50 @@ class ParTree:
         check(lib().spaqr_tree_partition(self._h, A.shape[0], A.shape[1], cp.ctypes.data, ri.ctypes.data,
                                          va.ctypes.data), self._h, "partition")
 
+    def partition_batch(self, A, sizes, levels):
+        """Independent systems on the diagonal blocks of A (spaqr_tree_partition_batch)."""
+        A = A.tocsc()
+        A.sort_indices()
+        self._n = A.shape[1]
+        cp = np.ascontiguousarray(A.indptr, dtype=np.int32)
+        ri = np.ascontiguousarray(A.indices, dtype=np.int32)
+        va = np.ascontiguousarray(A.data, dtype=np.float64)
+        sz = np.ascontiguousarray(sizes, dtype=np.int32)
+        lv = np.ascontiguousarray(levels, dtype=np.int32)
+        self._keep = [cp, ri, va, sz, lv]
+        check(lib().spaqr_tree_partition_batch(self._h, len(sz), sz.ctypes.data, lv.ctypes.data, A.shape[1],
+                                               cp.ctypes.data, ri.ctypes.data, va.ctypes.data), self._h,
+              "partition_batch")
+
+    def systems(self) -> np.ndarray:
+        """(offset, size) per system."""
+        ns = lib().spaqr_tree_num_systems(self._h)
+        out = np.zeros(2 * ns, dtype=np.int32)
+        check(lib().spaqr_tree_get_systems(self._h, out.ctypes.data), self._h, "get_systems")
+        return out.reshape(ns, 2)
+
+    def cluster_systems(self) -> np.ndarray:
+        out = np.zeros(lib().spaqr_tree_num_clusters(self._h), dtype=np.int32)
+        check(lib().spaqr_tree_get_cluster_systems(self._h, out.ctypes.data), self._h, "get_cluster_systems")
+        return out
+
+    def gmres_batch(self, b: np.ndarray, iters: int = 300, tol: float = 1e-12, restart: int = 0,
+                    true_tol: float = 1e-12, flexible: bool = False):
+        """GMRES on every system of a batch; returns (x, [GmresResult per system])."""
+        ns = lib().spaqr_tree_num_systems(self._h)
+        self._bx.upload(np.asarray(b, dtype=np.float64))
+        res = (GmresResult * ns)()
+        check(lib().spaqr_gmres_batch(self._h, self._bx.ptr, self._by.ptr, iters, tol, restart, true_tol,
+                                      int(bool(flexible)), res), self._h, "gmres_batch")
+        return self._by.download(np.float64, self._n), list(res)
+
+    def gmres_batch_device(self, b_ptr, x_ptr, iters=300, tol=1e-12, restart=0, true_tol=1e-12, flexible=False):
+        ns = lib().spaqr_tree_num_systems(self._h)
+        res = (GmresResult * ns)()
+        check(lib().spaqr_gmres_batch(self._h, b_ptr, x_ptr, iters, tol, restart, true_tol, int(bool(flexible)), res),
+              self._h, "gmres_batch")
+        return list(res)
+
     def assemble(self, A=None):
         check(lib().spaqr_tree_assemble(self._h), self._h, "assemble")
         self._bx = DeviceBuffer(8 * self._n)
@@ -223,4 +267,18 @@ def build_tree(problem, precision: int = 64, nlevels=None, tol=None, skip=None) 
     t.set_Xcoo(linspace_nd(problem.n_side, problem.dim))
     t.partition(problem.A)
     t.assemble(problem.A)
+    return t
+
+
+def build_batch_tree(batch, precision: int = 64, tol=None) -> ParTree:
+    """One tree over every system of a Batch (problems.make_batch)."""
+    t = ParTree(max(batch.levels), 0)
+    t.set_scale(1)
+    t.set_tol(batch.tol if tol is None else tol)
+    t.set_order(1)
+    t.set_square(1)
+    t.set_precision(precision)
+    t.set_Xcoo(batch.X)
+    t.partition_batch(batch.A, batch.sizes, batch.levels)
+    t.assemble()
     return t
