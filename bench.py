@@ -48,7 +48,8 @@ def profile(tree):
                              bytes=rows[i, 4]) for i in range(n)}
 
 
-def run_steps(tree, b_dev, x_dev, n, steps, restart, iters, tag, flexible=False):
+def run_steps(tree, b_dev, x_dev, n, steps, restart, iters, tag, flexible=False, batch=False):
+    """steps x (factorize + GMRES to 1e-12); returns (s/step, factor s/step, [GmresResult per system])."""
     from pspaqr_public_amd._lib import lib
 
     res = None
@@ -58,12 +59,18 @@ def run_steps(tree, b_dev, x_dev, n, steps, restart, iters, tag, flexible=False)
         f0 = time.perf_counter()
         tree.factorize()
         fact_s += time.perf_counter() - f0
-        res = tree.gmres_device(b_dev.ptr, x_dev.ptr, iters=iters, tol=1e-12, restart=restart, true_tol=1e-12,
-                                flexible=flexible)
+        if batch:
+            res = tree.gmres_batch_device(b_dev.ptr, x_dev.ptr, iters=iters, tol=1e-12, restart=restart,
+                                          true_tol=1e-12, flexible=flexible)
+        else:
+            res = [tree.gmres_device(b_dev.ptr, x_dev.ptr, iters=iters, tol=1e-12, restart=restart, true_tol=1e-12,
+                                     flexible=flexible)]
     lib().spaqr_tree_sync(tree.handle)
     t1 = time.perf_counter()
-    log("[%s] %d step(s): %.4f s/step (factor %.4f s), gmres iters %d cycles %d true relres %.3e" % (
-        tag, steps, (t1 - t0) / max(steps, 1), fact_s / max(steps, 1), res.iters, res.cycles, res.true_res))
+    log("[%s] %d step(s): %.4f s/step (factor %.4f s), gmres iters %s, worst true relres %.3e" % (
+        tag, steps, (t1 - t0) / max(steps, 1), fact_s / max(steps, 1),
+        res[0].iters if len(res) == 1 else "%d..%d" % (min(r.iters for r in res), max(r.iters for r in res)),
+        max(r.true_res for r in res)))
     return (t1 - t0) / max(steps, 1), fact_s / max(steps, 1), res
 
 
@@ -76,8 +83,9 @@ def set_rank_mode(tree, mode):
 
 
 def cpu_baseline(cfg, side):
-    """Oracle (CPU FP64 restatement of the reference path, one thread) on a bounded sample."""
-    from oracle.oracle import OracleTree
+    """Oracle (CPU FP64 restatement of the reference path) on a bounded sample; returns
+    (problem, seconds, GMRES iterations, true relres, threads used)."""
+    from oracle.oracle import OracleTree, threads
     from pspaqr_public_amd.problems import make_problem
 
     P = make_problem(cfg, side)
@@ -89,9 +97,9 @@ def cpu_baseline(cfg, side):
     x, it, _ = O.gmres(b, iters=300, tol=1e-12, restart=P.meta.get("restart", 0), true_tol=1e-12)
     t2 = time.perf_counter()
     r = np.linalg.norm(b - O.spmv(x)) / np.linalg.norm(b)
-    log("[cpu oracle] %s: factor+gmres %.2f s (partition+assemble %.2f s), iters %d, relres %.2e" % (
-        P.name, t2 - t1, t1 - t0, it, r))
-    return P, t2 - t1, it, r
+    log("[cpu oracle] %s: factor+gmres %.2f s (partition+assemble %.2f s), iters %d, relres %.2e, %d thread(s)" % (
+        P.name, t2 - t1, t1 - t0, it, r, threads()))
+    return P, t2 - t1, it, r, threads()
 
 
 def main():
@@ -105,6 +113,7 @@ def main():
     ap.add_argument("--cpu-side", type=int, default=32, help="grid side of the CPU-baseline sample")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-mode2", action="store_true", help="skip the no-predictor leg")
     ap.add_argument("--rank-mode", type=int, default=1,
                     help="FP32 path's sparsification ranks: 1 learned sketch (default), 2 sketch without the "
                          "predictor, 0 geqp3 as the reference")
@@ -128,93 +137,136 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    from pspaqr_public_amd import autotune
     from pspaqr_public_amd._lib import DeviceBuffer, lib
-    from pspaqr_public_amd.problems import make_problem
-    from pspaqr_public_amd.ptree import build_tree
+    from pspaqr_public_amd.problems import make_batch, make_problem
+    from pspaqr_public_amd.ptree import build_batch_tree, build_tree
 
-    P = make_problem(args.config, args.side)
-    restart = P.meta.get("restart", 50) or 50
-    log("[bench] %s n=%d nnz=%d levels=%d tol=%g" % (P.name, P.A.shape[0], P.A.nnz, P.nlevels, P.tol))
+    batch = args.config == 4
+    if batch:
+        W = make_batch()
+        restart, name, wkey = 50, W.name, "batch48"
+        n_total, levels, tol = int(W.A.shape[0]), max(W.levels), W.tol
+        log("[bench] %s: %d systems, n=%d nnz=%d, levels %s, tol %g" % (W.name, len(W.systems), n_total, W.A.nnz,
+                                                                     sorted(set(W.levels)), W.tol))
+    else:
+        W = make_problem(args.config, args.side)
+        restart, name, n_total, levels, tol = W.meta.get("restart", 50) or 50, W.name, int(W.A.shape[0]), W.nlevels, W.tol
+        wkey = {1: "poisson2d_64", 2: "diffusion3d_32", 3: "convdiff3d_64", 5: "anisotropic3d_96"}.get(args.config)
+        if args.side is not None:
+            wkey = None
+        log("[bench] %s n=%d nnz=%d levels=%d tol=%g" % (W.name, n_total, W.A.nnz, W.nlevels, W.tol))
 
-    def prepare(prec, rank_mode=0):
-        t = build_tree(P, prec)
+    def prepare(prec, rank_mode=0, tuned=True):
+        t = build_batch_tree(W, prec) if batch else build_tree(W, prec)
         set_rank_mode(t, rank_mode)
-        b = P.b[t.cperm()]
-        bd = DeviceBuffer.from_array(b)
-        xd = DeviceBuffer(8 * t.n)
-        return t, b, bd, xd
+        used = autotune.apply(t, wkey, prec) if (tuned and wkey) else {}
+        b = W.b[t.cperm()]
+        return t, b, DeviceBuffer.from_array(b), DeviceBuffer(8 * t.n), used
 
-    # ---- FP32 fronts (the timed path)
+    # ---- FP32 fronts with learned ranks (the timed path)
     t0 = time.perf_counter()
-    T32, b, b32, x32 = prepare(32, args.rank_mode)
-    log("[bench] partition+assemble %.2f s" % (time.perf_counter() - t0))
+    T32, b, b32, x32, tuning = prepare(32, args.rank_mode)
+    log("[bench] partition+assemble %.2f s; tuned widths %s" % (time.perf_counter() - t0, tuning or "built-in"))
     lib().spaqr_tree_set_profiling(T32.handle, 1)
     if args.warmup:
-        run_steps(T32, b32, x32, T32.n, args.warmup, restart, args.iters, "fp32 warmup", True)
+        run_steps(T32, b32, x32, T32.n, args.warmup, restart, args.iters, "fp32 warmup", True, batch)
     lib().spaqr_tree_profile_reset(T32.handle)
     T32.rank_stats_reset()
-    step_s, fact_s, res32 = run_steps(T32, b32, x32, T32.n, args.steps, restart, args.iters, "fp32", True)
+    step_s, fact_s, res32 = run_steps(T32, b32, x32, T32.n, args.steps, restart, args.iters, "fp32", True, batch)
     rstats = T32.rank_stats()
     prof = profile(T32)
     stats = T32.level_stats()
     x = x32.download(np.float64, T32.n)
-    relres32 = float(np.linalg.norm(b - T32.spmv(x)) / np.linalg.norm(b))
+    r = b - T32.spmv(x)
+    if batch:
+        relres32 = max(float(np.linalg.norm(r[o:o + m]) / np.linalg.norm(b[o:o + m])) for o, m in T32.systems())
+    else:
+        relres32 = float(np.linalg.norm(r) / np.linalg.norm(b))
+    lib().spaqr_tree_set_profiling(T32.handle, 0)
+    T32.close()
+
+    # ---- the same path without the predictor (sketches as wide as min(k, w)): the predictor's gain
+    mode2 = None
+    if args.rank_mode == 1 and not args.skip_mode2:
+        T2, _, b2, x2, _ = prepare(32, 2)
+        run_steps(T2, b2, x2, T2.n, 1, restart, args.iters, "fp32 no-predictor warmup", True, batch)
+        T2.rank_stats_reset()
+        s2, f2, r2 = run_steps(T2, b2, x2, T2.n, max(1, min(args.steps, 3)), restart, args.iters, "fp32 no-predictor",
+                               True, batch)
+        st2 = T2.rank_stats()
+        mode2 = dict(seconds=s2, factor_s=f2, gmres_iters=max(q.iters for q in r2),
+                     worst_true_relres=max(q.true_res for q in r2),
+                     sketch_cols_per_step=st2["sketch_cols"] / max(1, min(args.steps, 3)))
+        T2.close()
 
     # ---- plain batched FP64 on the same GPU
     fp64 = None
     if not args.skip_fp64:
-        T64, b64h, b64, x64 = prepare(64)
-        run_steps(T64, b64, x64, T64.n, 1, 0, args.iters, "fp64 warmup")
-        s64, f64, res64 = run_steps(T64, b64, x64, T64.n, 1, 0, args.iters, "fp64")
-        fp64 = dict(seconds=s64, factor_s=f64, iters=res64.iters, true_relres=res64.true_res)
-        del T64
+        T64, _, b64, x64, _ = prepare(64)
+        run_steps(T64, b64, x64, T64.n, 1, 0, args.iters, "fp64 warmup", False, batch)
+        s64, f64, res64 = run_steps(T64, b64, x64, T64.n, 1, 0, args.iters, "fp64", False, batch)
+        fp64 = dict(seconds=s64, factor_s=f64, gmres_iters=max(q.iters for q in res64),
+                    worst_true_relres=max(q.true_res for q in res64))
+        T64.close()
 
     # HBM bytes per step of each launch class, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
     # (calibrated on a device copy) kept beside the kernel-trace summaries; per launch below
     bytes_per_step = {}
     tp = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-    if os.path.exists(tp):
+    if os.path.exists(tp) and not batch:
         with open(tp) as f:
             bytes_per_step = json.load(f).get("bytes_per_step", {})
-    traffic_by_class = {c: int(round(v * max(args.steps, 1) / max(prof[c]["launches"], 1)))
+    traffic_by_class = {c: int(round(v / max(prof[c]["launches"] / max(args.steps, 1), 1e-9)))
                         for c, v in bytes_per_step.items() if c in prof and prof[c]["launches"]}
 
-    # ---- roofline of the dominant kernel class (by time, over the timed steps)
-    cls = max((c for c in prof if c not in ("precond_solve", "gmres_orth")), key=lambda c: prof[c]["ms"])
+    # ---- roofline of the dominant launch class (by time, over the timed steps)
+    cls = max(prof, key=lambda c: prof[c]["ms"])
     p = prof[cls]
     per_launch_ms = p["ms"] / max(p["launches"], 1)
+    common = dict(traffic=traffic_by_class.get(cls), kernel=cls, launches=p["launches"], tasks=p["tasks"],
+                  avg_launch_ms=per_launch_ms)
     if cls in MFMA_CLASSES:
         achieved = p["flops"] / (p["ms"] * 1e-3) / 1e12 if p["ms"] > 0 else 0.0
         roof = dict(bound="mfma", achieved=achieved, peak=PEAK["fp32"], unit="TFLOP/s", frac=achieved / PEAK["fp32"],
-                    traffic=traffic_by_class.get(cls), kernel=cls, launches=p["launches"], tasks=p["tasks"],
-                    avg_launch_ms=per_launch_ms, algorithmic_flops_per_launch=p["flops"] / max(p["launches"], 1))
+                    algorithmic_flops_per_launch=p["flops"] / max(p["launches"], 1), **common)
     else:
         achieved = p["bytes"] / (p["ms"] * 1e-3) / 1e9 if p["ms"] > 0 else 0.0
         roof = dict(bound="hbm", achieved=achieved, peak=PEAK["hbm"], unit="GB/s", frac=achieved / PEAK["hbm"],
-                    traffic=traffic_by_class.get(cls), kernel=cls, launches=p["launches"], tasks=p["tasks"],
-                    avg_launch_ms=per_launch_ms, algorithmic_bytes_per_launch=p["bytes"] / max(p["launches"], 1))
+                    algorithmic_bytes_per_launch=p["bytes"] / max(p["launches"], 1), **common)
 
     # ---- CPU baseline (oracle, bounded sample) and the GPU on the same sample
     cpu = None
-    sample_gpu = None
     if not args.skip_cpu:
-        Ps, cpu_s, cit, cres = cpu_baseline(args.config, args.cpu_side)
+        cfg_cpu = 2 if batch else args.config
+        Ps, cpu_s, cit, cres, threads = cpu_baseline(cfg_cpu, args.cpu_side)
         Ts = build_tree(Ps, 32)
         set_rank_mode(Ts, args.rank_mode)
         bs = Ps.b[Ts.cperm()]
         bsd, xsd = DeviceBuffer.from_array(bs), DeviceBuffer(8 * Ts.n)
         run_steps(Ts, bsd, xsd, Ts.n, 1, restart, args.iters, "fp32 sample warmup", True)
         sg, _, _ = run_steps(Ts, bsd, xsd, Ts.n, 1, restart, args.iters, "fp32 sample", True)
-        sample_gpu = sg
         cpu = dict(value=cpu_s, unit="s (factor+GMRES to 1e-12 on the %s sample, n=%d)" % (Ps.name, Ps.A.shape[0]),
-                   cores=1, kind="port",
-                   sample="%s: same generator and tree parameters at side %d (oracle: FP64, one thread, "
-                          "GMRES iters %d, relres %.2e); GPU FP32 path on the same sample: %.4f s"
-                          % (Ps.name, args.cpu_side, cit, cres, sg))
+                   cores=threads, kind="port",
+                   sample="%s: same generator and tree parameters at side %d (oracle: FP64, %d thread(s), GMRES "
+                          "iters %d, relres %.2e); GPU FP32 path on the same sample: %.4f s"
+                          % (Ps.name, args.cpu_side, threads, cit, cres, sg))
+        Ts.close()
 
     per_class = {c: dict(ms_per_step=round(v["ms"] / max(args.steps, 1), 4), launches=v["launches"],
                          gflop=round(v["flops"] / 1e9, 4), gbytes=round(v["bytes"] / 1e9, 4))
                  for c, v in prof.items() if v["launches"]}
+    if batch:
+        workload = ("%s: %d independent 3D 7-point diffusion systems (sides %s, n=%d in total), per-system tensor-grid "
+                    "ND, tol %g; FP32 fronts of all systems in shared per-level grouped launches, learned sketch "
+                    "ranks, batched FP64 flexible GMRES(%d) to true relres 1e-12 on every system"
+                    % (name, len(W.systems), sorted(set(q.n_side for q in W.systems)), n_total, tol, restart))
+    elif args.config == 3:
+        workload = ("%s: 3D 7-point upwind convection-diffusion %d^3 (n=%d), tensor-grid ND -n %d -d 3, lvl %d, "
+                    "tol %g; FP32 fronts in grouped batched launches, learned sketch ranks, FP64 flexible GMRES(%d) "
+                    "to true relres 1e-12" % (name, W.n_side, n_total, W.n_side, levels, tol, restart))
+    else:
+        workload = "%s (n=%d, lvl %d, tol %g)" % (name, n_total, levels, tol)
     out = {
         "metric": METRIC,
         "value": round(step_s, 6),
@@ -228,23 +280,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": "%s: 3D 7-point upwind convection-diffusion %d^3 (n=%d), tensor-grid ND -n %d -d 3, lvl %d, "
-                        "tol %g; FP32 fronts in grouped batched launches, learned sketch ranks, FP64 flexible GMRES(%d) to "
-                        "true relres 1e-12"
-                        % (P.name, P.n_side, P.A.shape[0], P.n_side, P.nlevels, P.tol, restart)
-                        if args.config == 3 else "%s (n=%d, lvl %d, tol %g)" % (P.name, P.A.shape[0], P.nlevels, P.tol),
-            "n": int(P.A.shape[0]),
-            "levels": int(P.nlevels),
-            "tol": P.tol,
-            "restart": restart,
-        },
+        "config": {"workload": workload, "n": n_total, "levels": int(levels), "tol": tol, "restart": restart},
         "factor_s": round(fact_s, 6),
-        "gmres_iters": res32.iters,
-        "gmres_cycles": res32.cycles,
+        "gmres_iters": res32[0].iters if len(res32) == 1 else [q.iters for q in res32],
+        "gmres_cycles": max(q.cycles for q in res32),
         "true_relres": relres32,
         "fp64_plain": fp64,
         "speedup_vs_fp64": (fp64["seconds"] / step_s) if fp64 else None,
+        "no_predictor": mode2,
+        "predictor_gain": (mode2["seconds"] / step_s) if mode2 else None,
+        "tuning": tuning or "built-in",
         "factor_gflop": round(float(stats[:, 7].sum()) / 1e9, 3),
         "rank_mode": args.rank_mode,
         "rank_stats_per_step": {k: (v / max(args.steps, 1)) for k, v in rstats.items()},

@@ -74,6 +74,22 @@ int spaqr_tree_partition(spaqr_tree* t, int nrows, int ncols, const int* colptr,
 int spaqr_tree_partition_batch(spaqr_tree* t, int nsys, const int* sizes, const int* levels, int n,
                                const int* colptr, const int* rowind, const double* val);
 int spaqr_tree_num_systems(const spaqr_tree* t);
+/* Planner tunables (no reference counterpart; the reference fixes nb = min(32, k) at
+   src/ptree.cpp:203-206 and its laqps block at :407).  value 0 restores the built-in choice.
+     0 big-front threshold: min(m, n) from which a front's panels run one launch per panel
+       (built in 96; 16 .. 2^30)
+     1 trailing-update chain column tile (64; 16, 32, 64)
+     2 Q^T C column chunk (128; 16, 32, 64, 128 -- FP32 blocks over 512 rows use at most 64)
+     3 C R^{-1} row chunk (128 FP32 / 256 FP64; 32, 64, 128, 256)
+     4 sparsify right-transform row chunk (128 FP32 / 256 FP64; 32, 64, 128, 256)
+     5 sparsify left-transform column chunk (128; 32, 64, 128)
+     6 rank-path column-norm / tail-test column chunk (128; 32, 64, 128, 256)
+     7 sketch width from which the sketch QR runs unpivoted (96; 16 .. 2^30)
+     8 multi-panel coupling chains on the cooperative kernel (1 on, 2 off)
+     9 blocks of at most 64 x 64 on the one-wave kernels (1 on, 2 off)
+   The committed autotuner (python -m pspaqr_public_amd.autotune) picks them per workload. */
+int spaqr_tree_set_tuning(spaqr_tree* t, int param, int value);
+int spaqr_tree_get_tuning(const spaqr_tree* t, int param);
 /* per system: first position, size (2 ints) */
 int spaqr_tree_get_systems(const spaqr_tree* t, int* out);
 /* system of each cluster (rows of spaqr_tree_get_clusters) */

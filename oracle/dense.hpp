@@ -9,6 +9,7 @@
 // Q*R == A, brute-force pivoting on tiny inputs).
 #pragma once
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstddef>
 #include <vector>
@@ -53,8 +54,8 @@ struct Mat {
 // path and is omitted, both here and in the HIP kernels.)
 constexpr double kAlphaSignBand = 1e-10;
 
-inline long& larfg_tiny_alpha() {
-    static long count = 0;
+inline std::atomic<long>& larfg_tiny_alpha() {  // diagnostic counter (updated from several threads)
+    static std::atomic<long> count{0};
     return count;
 }
 
@@ -340,8 +341,8 @@ inline void ormqr_ln_vec(int m, int k, const double* V, int ldv, const double* t
     }
 }
 
-inline long& geqp3_near_ties() {
-    static long count = 0;
+inline std::atomic<long>& geqp3_near_ties() {
+    static std::atomic<long> count{0};
     return count;
 }
 inline double& geqp3_near_tie_band() {

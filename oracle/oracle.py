@@ -178,3 +178,9 @@ class planted_fault:
     def __exit__(self, *exc):
         lib().orc_set_fault(0)
         return False
+
+
+def threads() -> int:
+    """Threads the oracle's factorization uses (1: it runs sequentially)."""
+    L = lib()
+    return int(L.orc_threads()) if hasattr(L, "orc_threads") else 1

@@ -42,6 +42,10 @@
 #include "dense.hpp"
 #include "rank_path.hpp"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 namespace orc {
 
 // Planted mistakes, for tests that show the parity checks catch them (0: none).  Each is a
@@ -54,6 +58,23 @@ namespace orc {
 //   5 the sparsified level range also takes the second level from the top
 //     (src/ptree.cpp:30, 1566, 1635 stop at nlevels - 2)
 static int g_fault = 0;
+
+// f(i) for i in [0, n) on the OpenMP threads.  Tasks are independent (each writes only its own
+// blocks), so the results do not depend on the thread count; the first exception is rethrown.
+template <class F>
+static void par_for(int n, F&& f) {
+    std::exception_ptr err;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < n; ++i) {
+        try {
+            f(i);
+        } catch (...) {
+#pragma omp critical
+            if (!err) err = std::current_exception();
+        }
+    }
+    if (err) std::rethrow_exception(err);
+}
 
 // Relative band inside which geqp3 pivot candidates count as tied (lowest index wins); the
 // HIP kernel uses the same rule, so FP64 runs pick the same pivots despite rounding order.
@@ -417,6 +438,7 @@ struct Tree {
     std::vector<ClusterID> cpermed;
     std::vector<std::vector<Cluster*>> bottoms, interiors, interfaces;
     std::vector<std::vector<Cluster*>> update_order;  // per level: interiors by (colour, cluster order)
+    std::vector<std::vector<int>> update_colour;      // their colours
     std::vector<std::unique_ptr<Cluster>> owned_c;
     std::vector<std::unique_ptr<Edge>> owned_e;
     int max_order = 0;
@@ -602,26 +624,31 @@ struct Tree {
 
     void eliminate(int ilvl) {
         auto& ints = interiors[current_bottom];
-        // geqrt_cluster (src/ptree.cpp:170-195)
-        for (Cluster* c : ints) {
+        const int ni = (int)ints.size();
+        std::vector<std::vector<OpTrace>> tl(ni);
+        // per interior (each touches only its own row and column blocks): geqrt_cluster
+        // (src/ptree.cpp:170-195), larfb_edge on every edgesIn (:216-221), then the tsqrt chain
+        // over edgesOut after self (:197-214, 1134-1141, 1453-1461)
+        par_for(ni, [&](int i) {
+            Cluster* c = ints[i];
             if (c->cols() == 0) throw std::runtime_error("cluster has zero cols");
             int k = c->cols(), m = c->rows();
             c->T = Mat(k, k);
             geqrt(m, k, k, c->eself->A.data(), m, c->T.data(), k);
             c->eliminated = true;
-            trace.push_back({ilvl, 0, m, k, k});
-        }
-        // larfb_edge on every edgesIn (src/ptree.cpp:216-221)
-        for (Cluster* c : ints) {
+            tl[i].push_back({ilvl, 0, m, k, k});
+        });
+        par_for(ni, [&](int i) {
+            Cluster* c = ints[i];
             int k = c->cols();
             for (Edge* e : c->edgesIn) {
                 larfb_left_trans(c->rows(), e->A.c, k, c->eself->A.data(), c->rows(), c->T.data(), k, e->A.data(),
                                  e->A.r);
-                trace.push_back({ilvl, 1, c->rows(), e->A.c, k});
+                tl[i].push_back({ilvl, 1, c->rows(), e->A.c, k});
             }
-        }
-        // tsqrt chain over edgesOut after self (src/ptree.cpp:197-214, 1134-1141, 1453-1461)
-        for (Cluster* c : ints) {
+        });
+        par_for(ni, [&](int i) {
+            Cluster* c = ints[i];
             int k = c->cols();
             int nb = std::min(32, k);
             if (c->edgesOut.empty() || c->edgesOut[0]->n2 != c) throw std::runtime_error("first edgeOut must be self");
@@ -632,9 +659,14 @@ struct Tree {
                 tpqrt(e->A.r, k, nb, R.data(), k, e->A.data(), e->A.r, T.data(), nb);
                 c->eself->A.set_block(0, 0, R);
                 c->Tmap[e->n2->order] = std::move(T);
-                trace.push_back({ilvl, 2, e->A.r, k, k});
+                tl[i].push_back({ilvl, 2, e->A.r, k, k});
             }
-        }
+        });
+        // trace records in the sequential order: geqrt of every interior, then larfb, then tpqrt
+        for (int kind = 0; kind < 3; ++kind)
+            for (int i = 0; i < ni; ++i)
+                for (const auto& r : tl[i])
+                    if (r.op == kind) trace.push_back(r);
         // ssrfb_edges: for every m in edgesIn(c), the chain over n in edgesOut(c)
         // (src/ptree.cpp:223-266, 1366-1406); interiors colour by colour.
         {
@@ -647,14 +679,28 @@ struct Tree {
             std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return col[a] < col[b]; });
             if ((int)update_order.size() <= current_bottom) update_order.resize(current_bottom + 1);
             update_order[current_bottom].clear();
-            for (int i : idx) update_order[current_bottom].push_back(ints[i]);
+            if ((int)update_colour.size() <= current_bottom) update_colour.resize(current_bottom + 1);
+            update_colour[current_bottom].clear();
+            for (int i : idx) {
+                update_order[current_bottom].push_back(ints[i]);
+                update_colour[current_bottom].push_back(col[i]);
+            }
             if (getenv("SPAQR_ORACLE_COLOURS")) {
                 int nc = 0;
                 for (int c : col) nc = std::max(nc, c + 1);
                 fprintf(stderr, "[oracle] level %d: %zu interiors, %d colours\n", current_bottom, ints.size(), nc);
             }
         }
-        for (Cluster* c : update_order[current_bottom]) {
+        // colour by colour; the interiors of one colour share no out-neighbour, so they update
+        // disjoint blocks and run in parallel
+        const auto& uo = update_order[current_bottom];
+        std::vector<std::vector<OpTrace>> ut(uo.size());
+        for (size_t g0 = 0; g0 < uo.size();) {
+            size_t g1 = g0 + 1;
+            while (g1 < uo.size() && update_colour[current_bottom][g1] == update_colour[current_bottom][g0]) ++g1;
+            par_for((int)(g1 - g0), [&](int ii) {
+            Cluster* c = uo[g0 + ii];
+            auto& trc = ut[g0 + ii];
             int k = c->cols();
             int nb = std::min(32, k);
             for (Edge* cm : c->edgesIn) {
@@ -667,48 +713,59 @@ struct Tree {
                     const Mat& T = c->Tmap.at(cn->n2->order);
                     tpmqrt_lt(mn->A.r, mn->A.c, k, nb, cn->A.data(), cn->A.r, T.data(), nb, top.data(), k,
                               mn->A.data(), mn->A.r);
-                    trace.push_back({ilvl, 3, mn->A.r, mn->A.c, k});
+                    trc.push_back({ilvl, 3, mn->A.r, mn->A.c, k});
                 }
                 cm->A.set_block(0, 0, top);
             }
+            });
+            g0 = g1;
         }
+        for (const auto& v : ut) trace.insert(trace.end(), v.begin(), v.end());
     }
 
     void scale(int ilvl) {
         auto& itf = interfaces[current_bottom];
-        // scale_cluster (src/ptree.cpp:269-286)
-        for (Cluster* c : itf) {
-            if (!want_sparsify(c)) continue;
+        const int nf = (int)itf.size();
+        std::vector<std::vector<OpTrace>> tl(nf), ta(nf);
+        // scale_cluster (src/ptree.cpp:269-286), one interface per task
+        par_for(nf, [&](int i) {
+            Cluster* c = itf[i];
+            if (!want_sparsify(c)) return;
             Edge* e = c->eself;
             int k = c->cols(), m = c->rows();
             c->Qs = e->A;
             c->Ts = Mat(k, k);
             geqrt(m, k, k, c->Qs.data(), m, c->Ts.data(), k);
             for (int j = 0; j < k; ++j)
-                for (int i = 0; i < m; ++i) e->A(i, j) = (i == j) ? 1.0 : 0.0;
-            trace.push_back({ilvl, 4, m, k, k});
-        }
-        // scale apply on every interface-interface edge (src/ptree.cpp:1621-1633)
-        for (Cluster* n1 : itf)
+                for (int i2 = 0; i2 < m; ++i2) e->A(i2, j) = (i2 == j) ? 1.0 : 0.0;
+            tl[i].push_back({ilvl, 4, m, k, k});
+        });
+        // scale apply on every interface-interface edge (src/ptree.cpp:1621-1633); each edge belongs
+        // to the sparsOut list of one interface, the task that updates it
+        par_for(nf, [&](int i) {
+            Cluster* n1 = itf[i];
             for (Edge* e : n1->sparsOut) {
                 Cluster* c = e->n2;
                 if (want_sparsify(c))
                     larfb_left_trans(e->A.r, e->A.c, c->cols(), c->Qs.data(), c->Qs.r, c->Ts.data(), c->Ts.r,
                                      e->A.data(), e->A.r);
-                if (want_sparsify(n1)) {
-                    int k = n1->cols();
-                    trsm_right_upper(e->A.r, e->A.c, n1->Qs.data(), n1->Qs.r, e->A.data(), e->A.r);
-                    (void)k;
-                }
-                if (want_sparsify(c) || want_sparsify(n1)) trace.push_back({ilvl, 5, e->A.r, e->A.c, 0});
+                if (want_sparsify(n1)) trsm_right_upper(e->A.r, e->A.c, n1->Qs.data(), n1->Qs.r, e->A.data(), e->A.r);
+                if (want_sparsify(c) || want_sparsify(n1)) ta[i].push_back({ilvl, 5, e->A.r, e->A.c, 0});
             }
+        });
+        for (const auto& v : tl) trace.insert(trace.end(), v.begin(), v.end());
+        for (const auto& v : ta) trace.insert(trace.end(), v.begin(), v.end());
     }
 
     void sparsify(int ilvl) {
         auto& itf = interfaces[current_bottom];
-        // sparsify_rrqr_only (src/ptree.cpp:373-439)
-        for (Cluster* c : itf) {
-            if (!want_sparsify(c)) continue;
+        const int nf = (int)itf.size();
+        std::vector<std::vector<OpTrace>> tr(nf), ta(nf);
+        // sparsify_rrqr_only (src/ptree.cpp:373-439), one interface per task: each reads its
+        // coupling blocks and rewrites only its own factors and diagonal block
+        par_for(nf, [&](int ii) {
+            Cluster* c = itf[ii];
+            if (!want_sparsify(c)) return;
             int k = c->cols();
             int smp = 0, spm = 0;
             for (Edge* e : c->sparsOut) smp += e->A.r;
@@ -742,21 +799,27 @@ struct Tree {
                 // geqp3 returns without touching C (src/util.cpp:504-505); diagonal is empty
                 rank = choose_rank(nullptr, 0, tol);
             }
-            trace.push_back({ilvl, 6, k, w, rank});
+            tr[ii].push_back({ilvl, 6, k, w, rank});
             c->Qsp = C.block(0, 0, k, rank);
             c->tausp.assign(tau.begin(), tau.begin() + rank);
             c->Tsp = Mat(rank, rank);
             larft(k, rank, c->Qsp.data(), k, c->tausp.data(), c->Tsp.data(), rank);
             c->sparsified = true;
-            ranks_before[ilvl].push_back(k);
-            ranks_after[ilvl].push_back(rank);
             c->rank = rank;
             c->eself->A = Mat(rank, rank);
             for (int i = 0; i < rank; ++i) c->eself->A(i, i) = 1.0;
             c->reset_size(rank, rank);
-        }
-        // sparsify apply (src/ptree.cpp:1867-1897) on every interface-interface edge
-        for (Cluster* n : itf)
+        });
+        for (int ii = 0; ii < nf; ++ii)
+            for (const auto& r : tr[ii]) {
+                trace.push_back(r);
+                ranks_before[ilvl].push_back(r.m);
+                ranks_after[ilvl].push_back(r.k);
+            }
+        // sparsify apply (src/ptree.cpp:1867-1897) on every interface-interface edge; each edge
+        // belongs to the sparsOut list of one interface, the task that rewrites it
+        par_for(nf, [&](int ii) {
+            Cluster* n = itf[ii];
             for (Edge* e : n->sparsOut) {
                 Cluster* c = e->n2;
                 Mat Acopy = e->A;
@@ -773,8 +836,10 @@ struct Tree {
                     for (int j = 0; j < n->rank; ++j)
                         for (int i = 0; i < c->rank; ++i) e->A(i, j) = A11(i, j);
                 }
-                if (want_sparsify(n) || want_sparsify(c)) trace.push_back({ilvl, 7, Acopy.r, Acopy.c, 0});
+                if (want_sparsify(n) || want_sparsify(c)) ta[ii].push_back({ilvl, 7, Acopy.r, Acopy.c, 0});
             }
+        });
+        for (const auto& v : ta) trace.insert(trace.end(), v.begin(), v.end());
     }
 
     void merge() {  // src/ptree.cpp:1940-2123 (one rank)
@@ -1115,6 +1180,13 @@ void orc_gemqrt_lt(int m, int n, int k, int nb, const double* V, const double* T
 }
 int orc_choose_rank(const double* s, int n, double tol) { return choose_rank(s, n, tol); }
 void orc_set_fault(int f) { g_fault = f; }
+int orc_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 // rank path (rank_path.hpp)
 void orc_gauss(int rows, int cols, uint64_t seed, double* om) {
     for (int j = 0; j < cols; ++j)

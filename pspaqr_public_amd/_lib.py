@@ -144,6 +144,8 @@ EXPORTS = {
     "spaqr_geqp3_noalloc_scratch_bytes": (_i, []),
     "spaqr_tree_partition_batch": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp, _vp]),
     "spaqr_tree_num_systems": (_i, [_vp]),
+    "spaqr_tree_set_tuning": (_i, [_vp, _i, _i]),
+    "spaqr_tree_get_tuning": (_i, [_vp, _i]),
     "spaqr_tree_get_systems": (_i, [_vp, _vp]),
     "spaqr_tree_get_cluster_systems": (_i, [_vp, _vp]),
     "spaqr_gmres_batch": (_i, [_vp, _vp, _vp, _i, _d, _i, _d, _i, _vp]),

@@ -1,0 +1,147 @@
+"""Autotuner of the level planner's tile, chunk and panel-path widths (spaqr_tree_set_tuning).
+
+For one workload it times the factorization (the part these widths change) on the GPU and
+runs a coordinate descent over the candidate values of every tunable: each parameter in turn
+takes the value with the lowest median time while the others stay at their current best; two
+passes.  The result is stored in pspaqr_public_amd/tuning.json under the workload's key,
+beside the built-in choice's time, and ``apply`` sets it on a tree.
+
+    python -m pspaqr_public_amd.autotune --workload convdiff3d_64 --prec 32
+    python -m pspaqr_public_amd.autotune --workload batch48 --prec 32
+    python -m pspaqr_public_amd.autotune --workload diffusion3d_32 --prec 64
+
+Workloads are tuned as the benchmark runs them: FP32 with learned sketch ranks, FP64 with
+geqp3 ranks.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, "tuning.json")
+
+# candidate values (every one within the range spaqr_tree_set_tuning accepts)
+CANDIDATES = {
+    "big_front": [64, 96, 128, 192, 256],
+    "tpmqrt_tile": [32, 64],
+    "larfb_cols": [64, 128],
+    "trsm_rows": [64, 128, 256],
+    "spars_rows": [64, 128, 256],
+    "spars_cols": [64, 128],
+    "rank_cols": [64, 128, 256],
+    "sketch_unpivoted": [64, 96, 128, 192],
+    "tpqrt_coop": [1, 2],
+    "small_tasks": [1, 2],
+}
+BUILT_IN = {"big_front": 96, "tpmqrt_tile": 64, "larfb_cols": 128, "spars_cols": 128, "rank_cols": 128,
+            "sketch_unpivoted": 96, "tpqrt_coop": 1, "small_tasks": 1}
+
+
+def key(workload: str, prec: int) -> str:
+    return "%s/fp%d" % (workload, prec)
+
+
+def load_table(path: str = TABLE) -> dict:
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def apply(tree, workload: str, prec: int, path: str = TABLE) -> dict:
+    """Sets the tuned widths of (workload, prec) on a tree; returns them ({} if none)."""
+    entry = load_table(path).get(key(workload, prec))
+    if not entry:
+        return {}
+    tree.set_tuning(entry["params"])
+    return dict(entry["params"])
+
+
+def make_workload(workload: str, prec: int):
+    """(tree, label) with the matrix assembled and the benchmark's rank mode set."""
+    from .problems import make_batch, make_problem
+    from .ptree import build_batch_tree, build_tree
+    from .rank_model import load_weights
+
+    if workload == "batch48":
+        B = make_batch()
+        t = build_batch_tree(B, prec)
+    else:
+        cfg = {"convdiff3d_64": 3, "diffusion3d_32": 2, "poisson2d_64": 1, "anisotropic3d_96": 5}[workload]
+        t = build_tree(make_problem(cfg), prec)
+    if prec == 32:
+        t.set_rank_mode(1)
+        t.set_rank_model(load_weights())
+    return t
+
+
+def _time(tree, reps: int) -> float:
+    from ._lib import lib
+
+    tree.factorize()  # warm-up with this setting
+    lib().spaqr_tree_sync(tree.handle)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        tree.factorize()
+        lib().spaqr_tree_sync(tree.handle)
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[len(ts) // 2] * 1e3
+
+
+def tune(workload: str, prec: int, reps: int = 3, passes: int = 2, log=print) -> dict:
+    t = make_workload(workload, prec)
+    params = {k: 0 for k in CANDIDATES}  # 0: built-in value
+    t.set_tuning(params)
+    base = _time(t, reps)
+    log("[autotune] %s built-in: %.2f ms" % (key(workload, prec), base))
+    best = base
+    sweep = []
+    for p in range(passes):
+        for name, values in CANDIDATES.items():
+            cur = params[name]
+            for v in values:
+                if v == (cur if cur else BUILT_IN.get(name)):
+                    continue
+                trial = dict(params)
+                trial[name] = v
+                t.set_tuning(trial)
+                ms = _time(t, reps)
+                sweep.append({"pass": p, "param": name, "value": v, "ms": round(ms, 3)})
+                log("[autotune] pass %d %-16s = %-5d %.2f ms (best %.2f)" % (p, name, v, ms, best))
+                if ms < best * 0.995:  # a real gain, not noise
+                    best = ms
+                    params[name] = v
+            t.set_tuning(params)
+    t.set_tuning(params)
+    final = _time(t, reps)
+    chosen = {k: v for k, v in params.items() if v}
+    log("[autotune] %s chosen %s: %.2f ms (built-in %.2f ms)" % (key(workload, prec), chosen, final, base))
+    return {"params": chosen, "ms": round(final, 3), "built_in_ms": round(base, 3), "reps": reps, "sweep": sweep}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="convdiff3d_64",
+                    choices=["convdiff3d_64", "batch48", "diffusion3d_32", "poisson2d_64", "anisotropic3d_96"])
+    ap.add_argument("--prec", type=int, default=32, choices=[32, 64])
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--passes", type=int, default=2)
+    ap.add_argument("--out", default=TABLE)
+    args = ap.parse_args()
+    res = tune(args.workload, args.prec, args.reps, args.passes, log=lambda s: print(s, flush=True))
+    table = load_table(args.out)
+    table[key(args.workload, args.prec)] = res
+    with open(args.out, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    print("[autotune] wrote %s" % args.out, flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

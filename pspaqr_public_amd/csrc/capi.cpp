@@ -175,6 +175,32 @@ int spaqr_tree_partition_batch(spaqr_tree* h, int nsys, const int* sizes, const 
     A.val.assign(val, val + colptr[n]);
     return h->t->partition_batch(A, std::vector<int>(sizes, sizes + nsys), std::vector<int>(levels, levels + nsys));
 }
+int spaqr_tree_set_tuning(spaqr_tree* h, int param, int value) {
+    TREE_OR_FAIL(h);
+    if (param < 0 || param >= T_NPARAM) { h->t->last_error = "unknown tuning parameter"; return -2; }
+    auto in = [&](std::initializer_list<int> ok) {
+        for (int v : ok)
+            if (v == value) return true;
+        return false;
+    };
+    bool valid = value == 0;
+    switch (param) {
+        case T_BIG_FRONT: case T_SKETCH_UNPIVOTED: valid = valid || (value >= 16 && value <= (1 << 30)); break;
+        case T_TPMQRT_TILE: valid = valid || in({16, 32, 64}); break;
+        case T_LARFB_COLS: valid = valid || in({16, 32, 64, 128}); break;
+        case T_TRSM_ROWS: case T_SPARS_ROWS: case T_RANK_COLS: valid = valid || in({32, 64, 128, 256}); break;
+        case T_SPARS_COLS: valid = valid || in({32, 64, 128}); break;
+        case T_TPQRT_COOP: case T_SMALL_TASKS: valid = valid || in({1, 2}); break;
+        default: break;
+    }
+    if (!valid) { h->t->last_error = "tuning value out of range for this parameter"; return -3; }
+    h->t->tune[param] = value;
+    return 0;
+}
+int spaqr_tree_get_tuning(const spaqr_tree* h, int param) {
+    if (!h || !h->t || param < 0 || param >= T_NPARAM) return -1;
+    return h->t->tune[param];
+}
 int spaqr_tree_num_systems(const spaqr_tree* h) { return (h && h->t) ? h->t->n_systems() : -1; }
 int spaqr_tree_get_systems(const spaqr_tree* h, int* out) {
     TREE_OR_FAIL(h);

@@ -256,6 +256,9 @@ int Tree::sync() {
 struct Tree::SolveSchedule {
     size_t off = 0;  // workspace entries assigned so far
     int built = 0;   // levels whose schedule is built
+    // algorithmic work of one application: factor entries read (V, T, R, Qs, Ts, Qsp, Tsp and
+    // the off-diagonal blocks of the back substitution) and vector entries read and written
+    double flops = 0.0, bytes = 0.0;
     struct Level {
         void* merge_fwd = nullptr; int n_merge_fwd = 0;
         void* q = nullptr; int n_q = 0;
@@ -932,7 +935,11 @@ int Tree::factorize_t() {
     auto Ap = [&](int id) -> S* { return (S*)ed[id].A; };
     // Blocks of at most 64 x 64 go to the one-wave kernels (SPAQR_SMALL_TASKS=0 keeps every
     // task on the workgroup kernels); the rest stay on the workgroup kernels.
-    static const bool small_tasks = getenv("SPAQR_SMALL_TASKS") == nullptr || atoi(getenv("SPAQR_SMALL_TASKS")) != 0;
+    static const bool small_env = getenv("SPAQR_SMALL_TASKS") == nullptr || atoi(getenv("SPAQR_SMALL_TASKS")) != 0;
+    const bool small_tasks = small_env && tuned(T_SMALL_TASKS, 1) == 1;
+    const int big_front = tuned(T_BIG_FRONT, kBigFront);
+    const int larfb_cols = tuned(T_LARFB_COLS, 128);
+    const int trsm_rows = tuned(T_TRSM_ROWS, sizeof(S) == 4 ? 128 : 256);
     // SPAQR_DEBUG_SHAPES=1: flop-weighted shapes of the workgroup-kernel apply tasks, printed at the end
     static const bool dbg_shapes = getenv("SPAQR_DEBUG_SHAPES") != nullptr;
     std::map<std::string, std::array<double, 2>> shapes;  // key -> {tasks, flops}
@@ -968,7 +975,7 @@ int Tree::factorize_t() {
     // the chunks of one task, as split_larfb cuts them, filed by bucket
     auto file_larfb = [&](const LarfbTask<S>& t, LarfbBuckets& cols, LarfbBuckets& tr) {
         if (t.V) {
-            const int cw = (sizeof(S) == 4 && t.m > 512) ? 64 : 128;
+            const int cw = (sizeof(S) == 4 && t.m > 512) ? std::min(64, larfb_cols) : larfb_cols;
             for (int c0 = 0; c0 < t.n; c0 += cw) {
                 LarfbTask<S> u = t;
                 u.C = t.C + (size_t)c0 * t.ldc;
@@ -979,7 +986,7 @@ int Tree::factorize_t() {
             }
         }
         if (t.R) {
-            const int rc = sizeof(S) == 4 ? 128 : 256;
+            const int rc = trsm_rows;
             for (int r0 = 0; r0 < t.m; r0 += rc) {
                 LarfbTask<S> u = t;
                 u.V = nullptr;
@@ -1180,9 +1187,10 @@ int Tree::factorize_t() {
         TpqrtLink<S>* d_tpql = nullptr;
         {
             std::vector<GeqrtTask<S>> geq_small, geq_big;
-            for (const auto& t : geq) (std::min(t.m, t.n) >= kBigFront ? geq_big : geq_small).push_back(t);
+            for (const auto& t : geq) (std::min(t.m, t.n) >= big_front ? geq_big : geq_small).push_back(t);
             std::vector<TpqrtTask<S>> tpq_wave, tpq_small, tpq_big, tpq_coop;
-            static const bool tpqrt_coop_on = getenv("SPAQR_TPQRT_COOP") == nullptr || atoi(getenv("SPAQR_TPQRT_COOP")) != 0;
+            static const bool coop_env = getenv("SPAQR_TPQRT_COOP") == nullptr || atoi(getenv("SPAQR_TPQRT_COOP")) != 0;
+            const bool tpqrt_coop_on = coop_env && tuned(T_TPQRT_COOP, 1) == 1;
             // small chains (k <= 64, links of <= 64 rows) go one per wave; the rest stay on the
             // one-workgroup kernel: with the register-resident panel QR the host-driven panel
             // loop (tpqrt_big) costs more in launch gaps than it gains
@@ -1238,7 +1246,7 @@ int Tree::factorize_t() {
         // Trailing-update chains, built over the interiors in parallel: links into one array at
         // per-slot offsets (counted first), tasks straight into their colour's launch buckets
         // (0-2 one-wave by link rows, 3-6 C-resident, 7 workgroup), all in interior order.
-        const int TILE = 64;
+        const int TILE = tuned(T_TPMQRT_TILE, 64);
         using TpmBuckets = std::array<std::vector<TpmqrtTask<S>>, 8>;
         const int nint = (int)ints.size();
         const int NS = std::min(host_threads(), std::max(1, nint / 64));
@@ -1399,7 +1407,7 @@ int Tree::factorize_t() {
         // ---- upload and launch the scaling; the RRQR descriptors are built while it runs
         {
             std::vector<GeqrtTask<S>> sgeq_small, sgeq_big;
-            for (const auto& t : sgeq) (std::min(t.m, t.n) >= kBigFront ? sgeq_big : sgeq_small).push_back(t);
+            for (const auto& t : sgeq) (std::min(t.m, t.n) >= big_front ? sgeq_big : sgeq_small).push_back(t);
             const GeqrtBatch sgeq_b = prep_geqrt(sgeq_small);
             std::vector<LarfbBuckets> sapp_c(nsp), sapp_t(nsp);
             for (int i = 0; i < nsp; ++i) {
@@ -1530,7 +1538,7 @@ int Tree::factorize_t() {
         // The sparsify apply's right transform (A <- A Q_n, in place on the interface blocks, row
         // chunks) needs nothing from the merge: it is queued first and runs while the host merges.
         // (FP32 row chunks of 128: one chunk per workgroup of the C-resident kernel)
-        constexpr int kSparsRows = sizeof(S) == 4 ? 128 : 256, kSparsCols = 128;
+        const int kSparsRows = tuned(T_SPARS_ROWS, sizeof(S) == 4 ? 128 : 256), kSparsCols = tuned(T_SPARS_COLS, 128);
         {
             std::vector<SparsBuckets> spa1(host_threads());
             const int np1 = parallel_ranges((int)itf.size(), 256, [&](int b, int e, int slot) {
@@ -2083,6 +2091,35 @@ void Tree::solve_sched_levels(int upto) {
             }
             fprintf(stderr, "\n");
         }
+        {
+            // this level's share of one application: Q^T on the interiors, the chains (V, T of
+            // every link and the neighbour's vector segment), the sparsified interfaces forward
+            // and backward (Qs, Ts, Qsp, Tsp, and the R part of Qs), and the back substitution
+            // (R and the off-diagonal blocks)
+            const double SZ = sizeof(S);
+            double fl = 0.0, by = 0.0;
+            for (const auto& t : q) { fl += 4.0 * t.m * t.k; by += SZ * ((double)t.m * t.k + NB * t.k + 2.0 * t.m); }
+            for (const auto& l : links) by += SZ * 2.0 * l.rows;
+            for (int c : interiors[L]) {
+                const Cluster& C = cl[c];
+                for (size_t j = 1; j < C.out.size(); ++j) {
+                    const Edge& eo = ed[C.out[j]];
+                    fl += 4.0 * eo.rows * C.cols;
+                    by += SZ * ((double)eo.rows * C.cols + NB * C.cols);
+                }
+                for (int e : C.in) {
+                    fl += 2.0 * C.cols * ed[e].cols;
+                    by += SZ * ((double)C.cols * ed[e].cols + ed[e].cols);
+                }
+            }
+            for (const auto& t : tr) { fl += (double)t.k * t.k; by += SZ * (0.5 * t.k * t.k + 2.0 * t.k); }
+            for (const auto& t : sp) {
+                fl += 2.0 * (4.0 * t.ldq * t.k + 4.0 * t.k * t.r) + (double)t.k * t.k;
+                by += SZ * (2.0 * ((double)t.ldq * t.k + NB * t.k + (double)t.k * t.r + NB * t.r) + 4.0 * t.k);
+            }
+            sched->flops += fl;
+            sched->bytes += by;
+        }
         lv.q = upload_vec(q);
         lv.n_q = (int)q.size();
         if (getenv("SPAQR_DEBUG_HOST")) {
@@ -2139,6 +2176,8 @@ int Tree::solve_dev(double* x) {
         if (!factorized || !sched) throw std::runtime_error("factorize() first");
         int rc = 0;
         timed(P_SOLVE, 1, 1, [&] { rc = (prec_fact == 32) ? solve_t<float>(x) : solve_t<double>(x); });
+        // plus the FP64 vector in and out
+        prof_add(P_SOLVE, sched->flops, sched->bytes + 16.0 * n);
         return rc;
     } catch (std::exception& e) {
         last_error = e.what();

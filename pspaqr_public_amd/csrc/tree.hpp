@@ -94,6 +94,21 @@ struct ProfEntry {
     long launches = 0, tasks = 0;
 };
 
+// Planner tunables: tile, chunk and panel-path widths picked per workload by the autotuner.
+enum TuneParam {
+    T_BIG_FRONT = 0,    // min(m, n) from which a front's panels run as one launch per panel (96)
+    T_TPMQRT_TILE,      // column tile of a trailing-update chain task (64)
+    T_LARFB_COLS,       // column chunk of a Q^T C task (128; at most 64 for FP32 blocks over 512 rows)
+    T_TRSM_ROWS,        // row chunk of a C R^{-1} task (128 FP32, 256 FP64)
+    T_SPARS_ROWS,       // row chunk of the sparsify right transform (128 FP32, 256 FP64)
+    T_SPARS_COLS,       // column chunk of the sparsify left transform and scatter (128)
+    T_RANK_COLS,        // column chunk of the rank path's column-norm and tail kernels (128)
+    T_SKETCH_UNPIVOTED, // sketch width from which the sketch QR runs unpivoted (96)
+    T_TPQRT_COOP,       // 1: multi-panel coupling chains on the cooperative kernel, 2: off (1)
+    T_SMALL_TASKS,      // 1: blocks of <= 64 x 64 on the one-wave kernels, 2: off (1)
+    T_NPARAM
+};
+
 struct GmresResult {
     int iters = 0;        // 0-based k of the last cycle plus earlier cycles' steps (as dist_gmres counts)
     int cycles = 0;
@@ -156,6 +171,9 @@ public:
     // Sketch rounds per interface in rank modes 1 and 2: the first sketch plus up to
     // rank_rounds - 1 retries with a doubled width before the geqp3 fallback (1..3).
     int rank_rounds = 3;
+    // Planner tunables (TuneParam, spaqr_tree_set_tuning); 0 keeps the built-in value.
+    int tune[16] = {0};
+    int tuned(int p, int dflt) const { return tune[p] > 0 ? tune[p] : dflt; }
     struct RankStats {
         long tasks = 0, sketched = 0, fallback = 0, direct = 0, retried = 0;
         double sketch_cols = 0, full_cols = 0, rank_sum = 0;

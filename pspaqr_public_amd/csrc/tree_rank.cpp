@@ -81,7 +81,7 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
     for (int q : elig) wmax = std::max(wmax, gq[q].w);
     S* d_cmax2 = (S*)arena.alloc(sizeof(S) * std::max(nt, 1));
     float* d_X = nullptr;
-    constexpr int kColChunk = 128;  // columns per workgroup of the column-norm and tail kernels
+    const int kColChunk = tuned(T_RANK_COLS, 128);  // columns per workgroup of the column-norm and tail kernels
     if (ne && (feats || sketch_mode)) {
         std::vector<ColNormTask<S>> cn;
         for (int q : elig)
@@ -157,8 +157,9 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
                 auto* d_rg = upload_vec(regather);
                 timed(P_GATHER, 1, (long)regather.size(), [&] { launch_copy<S>(d_rg, (int)regather.size(), stream_, copy_parts(regather)); });
             }
-            static const int unpiv_min = getenv("SPAQR_SKETCH_UNPIVOTED") ? atoi(getenv("SPAQR_SKETCH_UNPIVOTED"))
+            static const int unpiv_env = getenv("SPAQR_SKETCH_UNPIVOTED") ? atoi(getenv("SPAQR_SKETCH_UNPIVOTED"))
                                                                             : kSketchUnpivoted;
+            const int unpiv_min = tuned(T_SKETCH_UNPIVOTED, unpiv_env);
             std::vector<SketchTask<S>> sk;
             std::vector<SketchTile> tiles;
             std::vector<Geqp3Task<S>> gy;
@@ -172,7 +173,7 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
                     for (int c0 = 0; c0 < s; c0 += 64) tiles.push_back({(int)sk.size(), r0, c0, 0});
                 sk.push_back({gq[q].C, Y, k, w, s, 0});
                 if (unpiv) {
-                    (std::min(k, s) >= kBigFront ? uy_big : uy_small).push_back({Y, gq[q].T, k, s, k, 0});
+                    (std::min(k, s) >= tuned(T_BIG_FRONT, kBigFront) ? uy_big : uy_small).push_back({Y, gq[q].T, k, s, k, 0});
                     ry0[q] = std::min(k, s);
                     prof_add(P_GEQP3, flops_geqrt<S>(k, s), SZ * (2.0 * k * s + NB * s));
                 } else {

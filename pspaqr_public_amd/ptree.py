@@ -88,6 +88,17 @@ class ParTree:
         check(lib().spaqr_tree_partition(self._h, A.shape[0], A.shape[1], cp.ctypes.data, ri.ctypes.data,
                                          va.ctypes.data), self._h, "partition")
 
+    TUNABLES = ("big_front", "tpmqrt_tile", "larfb_cols", "trsm_rows", "spars_rows", "spars_cols", "rank_cols",
+                "sketch_unpivoted", "tpqrt_coop", "small_tasks")
+
+    def set_tuning(self, params: dict):
+        """Planner tunables by name (spaqr_tree_set_tuning); 0 restores the built-in value."""
+        for k, v in params.items():
+            check(lib().spaqr_tree_set_tuning(self._h, self.TUNABLES.index(k), int(v)), self._h, "set_tuning " + k)
+
+    def tuning(self) -> dict:
+        return {k: lib().spaqr_tree_get_tuning(self._h, i) for i, k in enumerate(self.TUNABLES)}
+
     def partition_batch(self, A, sizes, levels):
         """Independent systems on the diagonal blocks of A (spaqr_tree_partition_batch)."""
         A = A.tocsc()
