@@ -90,6 +90,16 @@ int spaqr_tree_num_systems(const spaqr_tree* t);
    The committed autotuner (python -m pspaqr_public_amd.autotune) picks them per workload. */
 int spaqr_tree_set_tuning(spaqr_tree* t, int param, int value);
 int spaqr_tree_get_tuning(const spaqr_tree* t, int param);
+/* hipGraph mode (no reference counterpart; it replaces the per-level task flows of
+   src/ptree.cpp:885-2123 being re-issued on every factorization).  With on != 0, a
+   factorization records every stretch of launches between two host synchronisations (the
+   per-level rank read-backs) as a graph -- after one ordinary factorization that sizes the
+   device arenas -- and launches it; later factorizations of the same assembled matrix with the
+   same settings replay the graphs without host planning, read the ranks back at the same
+   points and plan afresh if any differs.  assemble() and any setting change drop the plan.
+   Needs rank mode 0.  info (3 long longs): graph mode on, recorded segments, replays so far. */
+int spaqr_tree_set_graphs(spaqr_tree* t, int on);
+int spaqr_tree_graph_info(const spaqr_tree* t, long long* info);
 /* per system: first position, size (2 ints) */
 int spaqr_tree_get_systems(const spaqr_tree* t, int* out);
 /* system of each cluster (rows of spaqr_tree_get_clusters) */

@@ -145,6 +145,8 @@ EXPORTS = {
     "spaqr_tree_partition_batch": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp, _vp]),
     "spaqr_tree_num_systems": (_i, [_vp]),
     "spaqr_tree_set_tuning": (_i, [_vp, _i, _i]),
+    "spaqr_tree_set_graphs": (_i, [_vp, _i]),
+    "spaqr_tree_graph_info": (_i, [_vp, _vp]),
     "spaqr_tree_get_tuning": (_i, [_vp, _i]),
     "spaqr_tree_get_systems": (_i, [_vp, _vp]),
     "spaqr_tree_get_cluster_systems": (_i, [_vp, _vp]),

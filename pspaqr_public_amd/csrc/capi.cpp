@@ -201,6 +201,18 @@ int spaqr_tree_get_tuning(const spaqr_tree* h, int param) {
     if (!h || !h->t || param < 0 || param >= T_NPARAM) return -1;
     return h->t->tune[param];
 }
+int spaqr_tree_set_graphs(spaqr_tree* h, int on) {
+    TREE_OR_FAIL(h);
+    return h->t->set_graphs(on != 0);
+}
+int spaqr_tree_graph_info(const spaqr_tree* h, long long* out) {
+    TREE_OR_FAIL(h);
+    if (!out) return -2;
+    out[0] = h->t->graphs_on();
+    out[1] = h->t->graph_segments();
+    out[2] = h->t->graph_replays;
+    return 0;
+}
 int spaqr_tree_num_systems(const spaqr_tree* h) { return (h && h->t) ? h->t->n_systems() : -1; }
 int spaqr_tree_get_systems(const spaqr_tree* h, int* out) {
     TREE_OR_FAIL(h);

@@ -3207,10 +3207,11 @@ int geqp3_coop_max_blocks(size_t lds_bytes, bool hbm) {
 
 template <typename S>
 hipError_t launch_geqp3_coop(const Geqp3CoopTask<S>* d_tasks, const int* d_blk2task, int* d_err, int nblocks,
-                             double tol, double tie_rel, size_t lds_bytes, bool hbm, hipStream_t s) {
+                             double tol, double tie_rel, size_t lds_bytes, bool hbm, hipStream_t s, bool cooperative) {
     if (nblocks <= 0) return hipSuccess;
     void* args[] = {(void*)&d_tasks, (void*)&d_blk2task, (void*)&d_err, (void*)&tol, (void*)&tie_rel};
     const void* fn = hbm ? (const void*)k_geqp3_coop<S, true> : (const void*)k_geqp3_coop<S, false>;
+    if (!cooperative) return hipLaunchKernel(fn, dim3(nblocks), dim3(WG), args, (size_t)lds_bytes, s);
     return hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(WG), args, (unsigned)lds_bytes, s);
 }
 
@@ -3232,8 +3233,8 @@ template size_t geqp3_coop_lds_budget<float>(int);
 template int geqp3_coop_max_blocks<double>(size_t, bool);
 template int geqp3_coop_max_blocks<float>(size_t, bool);
 template hipError_t launch_geqp3_coop<double>(const Geqp3CoopTask<double>*, const int*, int*, int, double, double,
-                                              size_t, bool, hipStream_t);
+                                              size_t, bool, hipStream_t, bool);
 template hipError_t launch_geqp3_coop<float>(const Geqp3CoopTask<float>*, const int*, int*, int, double, double,
-                                             size_t, bool, hipStream_t);
+                                             size_t, bool, hipStream_t, bool);
 
 }  // namespace spaqr

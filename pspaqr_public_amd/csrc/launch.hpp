@@ -75,7 +75,8 @@ template <typename S> int geqp3_coop_max_blocks(size_t lds_bytes, bool hbm);
 // hbm: column slices stay in the gathered C in HBM (for interfaces too large for the LDS layout)
 template <typename S>
 hipError_t launch_geqp3_coop(const Geqp3CoopTask<S>* d_tasks, const int* d_blk2task, int* d_err, int nblocks,
-                             double tol, double tie_rel, size_t lds_bytes, bool hbm, hipStream_t s);
+                             double tol, double tie_rel, size_t lds_bytes, bool hbm, hipStream_t s,
+                             bool cooperative = true);
 
 void launch_spmv(int n, const int* rowptr, const int* col, const double* val, const double* x, double* y,
                  hipStream_t s);

@@ -105,6 +105,7 @@ int geqp3_grouped(const std::vector<Geqp3Task<S>>& tasks, double tol, double tie
         for (auto& L : launches[mode]) ntask += L.t.size();
     zeros.assign(std::max<size_t>(ntask, 1), 0);
     int* d_bars = (int*)mem.upload(zeros.data(), zeros.size() * sizeof(int));
+    if (mem.graph) launch_fill<int>(d_bars, zeros.size(), 0, stream);
     struct Ready { Geqp3CoopTask<S>* t; int* b2t; int blocks; bool hbm; };
     std::vector<Ready> ready;
     size_t bar_off = 0;
@@ -121,7 +122,7 @@ int geqp3_grouped(const std::vector<Geqp3Task<S>>& tasks, double tol, double tie
                              ? nullptr
                              : (Geqp3Task<S>*)mem.upload(fallback.data(), fallback.size() * sizeof(Geqp3Task<S>));
     for (const auto& r : ready) {
-        hipError_t e = launch_geqp3_coop<S>(r.t, r.b2t, d_err, r.blocks, tol, tie_rel, lds, r.hbm, stream);
+        hipError_t e = launch_geqp3_coop<S>(r.t, r.b2t, d_err, r.blocks, tol, tie_rel, lds, r.hbm, stream, !mem.graph);
         if (e != hipSuccess) throw std::runtime_error(std::string("cooperative geqp3 launch: ") + hipGetErrorString(e));
     }
     if (d_fb) launch_geqp3<S>(d_fb, (int)fallback.size(), tol, tie_rel, stream);

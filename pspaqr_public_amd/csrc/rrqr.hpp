@@ -14,6 +14,11 @@ namespace spaqr {
 struct RrqrAlloc {
     std::function<void*(size_t)> alloc;                  // device scratch, alive until the stream syncs
     std::function<void*(const void*, size_t)> upload;    // async H2D copy of a host buffer, returns device ptr
+    // the launches are being recorded into a hip graph: counters that must start at zero are
+    // cleared by a kernel of the graph (an upload is not replayed), and the cooperative geqp3 is
+    // an ordinary launch (its grid never exceeds the co-resident limit; a wait that exceeds its
+    // spin limit raises the error flag instead of hanging)
+    bool graph = false;
 };
 
 // Cooperative geqp3 workgroups resident per CU: each gets an equal share of the CU's LDS for

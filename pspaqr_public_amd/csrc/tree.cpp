@@ -221,8 +221,97 @@ void* Tree::upload(const void* h, size_t bytes) {
     } else {
         memcpy(p, h, bytes);
     }
-    HIPCHK(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, stream_));
+    h2d(d, p, bytes);
     return d;
+}
+
+// Host-to-device descriptor copy: queued on the tree's stream, or, while that stream is being
+// captured into a graph, done at once on a separate stream, so that the graph holds only the
+// kernels and device copies and its replays read the descriptors where they already are.
+void Tree::h2d(void* d, const void* h, size_t bytes) {
+    if (!capturing) {
+        HIPCHK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream_));
+        return;
+    }
+    if (!up_stream_) HIPCHK(hipStreamCreateWithFlags(&up_stream_, hipStreamNonBlocking));
+    HIPCHK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, up_stream_));
+    HIPCHK(hipStreamSynchronize(up_stream_));
+}
+
+// ---------------------------------------------------------------------------- graphs
+std::vector<long long> Tree::plan_signature() const {
+    long long tb = 0;
+    static_assert(sizeof(tb) == sizeof(tol), "tolerance bits");
+    std::memcpy(&tb, &tol, sizeof(tb));
+    std::vector<long long> s = {prec, asm_gen, tb, skip, rank_mode};
+    for (int p = 0; p < T_NPARAM; ++p) s.push_back(tune[p]);
+    return s;
+}
+
+void Tree::seg_begin() {
+    if (!graph_rec) return;
+    HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
+    capturing = true;
+}
+
+void Tree::seg_end(int* ranks_dev, int nranks, int* err_dev) {
+    if (!graph_rec || !capturing) return;
+    hipGraph_t g = nullptr;
+    capturing = false;
+    HIPCHK(hipStreamEndCapture(stream_, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(e);
+    gplan.exec.push_back(ex);
+    gplan.rb_ranks_dev.push_back(ranks_dev);
+    gplan.rb_err_dev.push_back(err_dev);
+    gplan.rb_ranks.emplace_back((size_t)std::max(nranks, 0), 0);
+    HIPCHK(hipGraphLaunch(ex, stream_));
+}
+
+void Tree::drop_graphs() {
+    if (capturing) {  // an error while recording: end the capture and discard it
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        capturing = false;
+    }
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto ex : gplan.exec) (void)hipGraphExecDestroy(ex);
+    gplan = GraphPlan();
+}
+
+int Tree::set_graphs(bool on) {
+    if (!on) drop_graphs();
+    graphs = on;
+    return 0;
+}
+
+template <typename S>
+int Tree::replay_t() {
+    auto t0 = std::chrono::steady_clock::now();
+    sync_stream();
+    factorized = false;
+    for (size_t i = 0; i < gplan.exec.size(); ++i) {
+        HIPCHK(hipGraphLaunch(gplan.exec[i], stream_));
+        if (!gplan.rb_ranks_dev[i]) continue;
+        std::vector<int> r(gplan.rb_ranks[i].size());
+        int err = 0;
+        HIPCHK(hipMemcpyAsync(r.data(), gplan.rb_ranks_dev[i], sizeof(int) * r.size(), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipMemcpyAsync(&err, gplan.rb_err_dev[i], sizeof(int), hipMemcpyDeviceToHost, stream_));
+        sync_stream();
+        if (err) throw std::runtime_error("cooperative geqp3: a barrier wait exceeded its spin limit");
+        if (r != gplan.rb_ranks[i]) return 1;  // the plan no longer fits this matrix
+    }
+    int chain_err = 0;
+    HIPCHK(hipMemcpyAsync(&chain_err, gplan.chain_err, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    sync_stream();
+    if (chain_err) throw std::runtime_error("cooperative tpqrt: a wait exceeded its spin limit");
+    factorized = true;
+    ++graph_replays;
+    t_factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
 }
 
 void Tree::sync_stream() {
@@ -282,6 +371,8 @@ Tree::Tree(int nl, int sk) : nlevels(nl), skip(sk) {
 }
 
 Tree::~Tree() {
+    drop_graphs();
+    if (up_stream_) (void)hipStreamDestroy(up_stream_);
     if (stream_) (void)hipStreamSynchronize(stream_);
     delete sched;
     arena.release();
@@ -702,6 +793,8 @@ int Tree::assemble() {
         const size_t es = (prec == 32) ? 4 : 8;
         HIPCHK(hipMalloc(&d_blocks0, std::max<size_t>(nel * es, 256)));
         prec_asm = prec;
+        ++asm_gen;
+        drop_graphs();
         if (prec == 32) {
             std::vector<float> f(h_blocks0.begin(), h_blocks0.end());
             HIPCHK(hipMemcpy(d_blocks0, f.data(), nel * 4, hipMemcpyHostToDevice));
@@ -907,6 +1000,7 @@ int Tree::factorize_t() {
     stats.clear();
     factorized = false;
     solve_sched_begin<S>();
+    seg_begin();
     // level-0 blocks: copy of the pristine assembled blocks
     {
         S* base = (S*)arena.alloc(std::max<size_t>(h_blocks0.size(), 1) * sizeof(S));
@@ -1485,8 +1579,10 @@ int Tree::factorize_t() {
             if (!gq.empty()) {
                 RrqrAlloc mem{[&](size_t b) { return arena.alloc(b); },
                               [&](const void* h, size_t b) { return upload(h, b); }};
+                mem.graph = capturing;
                 const int zero = 0;
                 d_rrqr_err = (int*)upload(&zero, sizeof(int));
+                launch_fill<int>(d_rrqr_err, 1, 0, stream_);  // also on every graph replay
                 gat_begin.push_back((int)gat.size());
                 used_sketch.assign(gq.size(), 0);
                 if (rank_mode == 0 && !rank_log) {
@@ -1501,6 +1597,7 @@ int Tree::factorize_t() {
             HIPCHK(hipGetLastError());
         }
         if (!gq.empty()) {
+            seg_end(d_ranks, (int)gq.size(), d_rrqr_err);
             solve_sched_levels<S>(L);  // the finished levels' solve schedule, while the device works
             std::vector<int> ranks(gq.size());
             int rrqr_err = 0;
@@ -1525,6 +1622,10 @@ int Tree::factorize_t() {
                 C.rows = C.cols = r;  // reset_size (src/ptree.cpp:435-438)
             }
             st.n_sparsified = (int)gq.size();
+            if (graph_rec) {
+                gplan.rb_ranks.back() = ranks;
+                seg_begin();
+            }
         }
         hacc(2, ht);
         if (dbg_host)
@@ -1781,6 +1882,8 @@ int Tree::factorize_t() {
         }
     }
     auto ht2 = hclock();
+    seg_end(nullptr, 0, nullptr);
+    gplan.chain_err = d_chain_err;
     {
         int chain_err = 0;
         HIPCHK(hipMemcpyAsync(&chain_err, d_chain_err, sizeof(int), hipMemcpyDeviceToHost, stream_));
@@ -1962,11 +2065,38 @@ int Tree::factorize() {
         if (ed0.empty()) throw std::runtime_error("assemble() first");
         if (prec != prec_asm)
             throw std::runtime_error("the factor precision changed after assemble(): assemble() again before factorize()");
+        if (graphs) {
+            if (rank_mode != 0 || rank_log)
+                throw std::runtime_error("graph mode needs rank mode 0 (geqp3 ranks) and no rank log");
+            if (gplan.valid && gplan.sig == plan_signature()) {
+                const int r = prec_fact == 32 ? replay_t<float>() : replay_t<double>();
+                if (r == 0) return 0;
+            }
+            drop_graphs();
+            if (graph_warm_sig != plan_signature()) {
+                // one factorization outside any capture grows the arenas; the recorded one then
+                // makes the same allocations from the retained chunks and calls no hipMalloc
+                factorized = false;
+                prec_fact = prec;
+                const int rc0 = prec == 32 ? factorize_t<float>() : factorize_t<double>();
+                if (rc0) return rc0;
+                graph_warm_sig = plan_signature();
+            }
+            graph_rec = true;
+        }
         factorized = false;
         prec_fact = prec;
-        return prec == 32 ? factorize_t<float>() : factorize_t<double>();
+        const int rc = prec == 32 ? factorize_t<float>() : factorize_t<double>();
+        if (graph_rec) {
+            graph_rec = false;
+            gplan.valid = rc == 0;
+            gplan.sig = plan_signature();
+        }
+        return rc;
     } catch (std::exception& e) {
         last_error = e.what();
+        graph_rec = false;
+        drop_graphs();
         return -1;
     }
 }
@@ -2113,9 +2243,10 @@ void Tree::solve_sched_levels(int upto) {
                 }
             }
             for (const auto& t : tr) { fl += (double)t.k * t.k; by += SZ * (0.5 * t.k * t.k + 2.0 * t.k); }
-            for (const auto& t : sp) {
-                fl += 2.0 * (4.0 * t.ldq * t.k + 4.0 * t.k * t.r) + (double)t.k * t.k;
-                by += SZ * (2.0 * ((double)t.ldq * t.k + NB * t.k + (double)t.k * t.r + NB * t.r) + 4.0 * t.k);
+            for (const auto& t : sp) {  // forward: Qs^T, Qsp^T; backward: Qsp, R_s^{-1} (upper part of Qs)
+                fl += 4.0 * t.ldq * t.k + 8.0 * t.k * t.r + (double)t.k * t.k;
+                by += SZ * ((double)t.ldq * t.k + NB * t.k + 2.0 * ((double)t.k * t.r + NB * t.r) + 0.5 * t.k * t.k +
+                            4.0 * t.k);
             }
             sched->flops += fl;
             sched->bytes += by;

@@ -173,6 +173,15 @@ public:
     int rank_rounds = 3;
     // Planner tunables (TuneParam, spaqr_tree_set_tuning); 0 keeps the built-in value.
     int tune[16] = {0};
+    // hipGraph mode: the first factorization records every stretch of launches between two host
+    // synchronisations (the rank read-backs) as a graph and launches it; a later factorization
+    // of the same assembled matrix with the same settings replays the graphs without planning,
+    // reading the ranks back at the same points, and plans afresh if any rank differs.  Needs
+    // rank mode 0 (geqp3 ranks).
+    int set_graphs(bool on);
+    bool graphs_on() const { return graphs; }
+    int graph_segments() const { return (int)gplan.exec.size(); }
+    long graph_replays = 0;
     int tuned(int p, int dflt) const { return tune[p] > 0 ? tune[p] : dflt; }
     struct RankStats {
         long tasks = 0, sketched = 0, fallback = 0, direct = 0, retried = 0;
@@ -203,6 +212,28 @@ private:
     bool factorized = false;
     std::vector<LevelStats> stats;
     hipStream_t stream_ = nullptr;
+    // graph mode
+    bool graphs = false;
+    bool capturing = false;
+    hipStream_t up_stream_ = nullptr;  // descriptor uploads while the main stream is captured
+    long asm_gen = 0;                  // bumped by assemble(): a new matrix invalidates the plan
+    struct GraphPlan {
+        bool valid = false;
+        std::vector<hipGraphExec_t> exec;  // segments in order
+        std::vector<int*> rb_ranks_dev;    // rank read-back after the segment (nullptr: none)
+        std::vector<int*> rb_err_dev;
+        std::vector<std::vector<int>> rb_ranks;
+        int* chain_err = nullptr;
+        std::vector<long long> sig;
+    } gplan;
+    std::vector<long long> plan_signature() const;
+    void seg_begin();
+    void seg_end(int* ranks_dev, int nranks, int* err_dev);
+    void drop_graphs();
+    template <typename S> int replay_t();
+    void h2d(void* d, const void* h, size_t bytes);
+    bool graph_rec = false;  // this factorization records the plan
+    std::vector<long long> graph_warm_sig;  // settings whose arenas are already grown
 
     // device memory
     // Bump allocator over retained 256 MiB (or larger) hipMalloc chunks: reset() rewinds it

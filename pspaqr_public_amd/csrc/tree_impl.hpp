@@ -308,7 +308,7 @@ std::array<std::pair<T*, int>, NBK> Tree::upload_buckets(const std::vector<std::
     });
     std::array<std::pair<T*, int>, NBK> out;
     for (size_t q = 0; q < NBK; ++q) {
-        if (nq[q]) HIPCHK(hipMemcpyAsync(db[q], hb[q], nq[q] * sizeof(T), hipMemcpyHostToDevice, stream_));
+        if (nq[q]) h2d(db[q], hb[q], nq[q] * sizeof(T));
         out[q] = {db[q], (int)nq[q]};
     }
     return out;
@@ -328,7 +328,7 @@ void split_small(const std::vector<T>& in, D dim, std::vector<T> (&small)[3], st
 
 template <class F>
 void Tree::timed(int cat, long launches, long tasks, F&& f) {
-    if (!profiling) {
+    if (!profiling || capturing) {  // events are not recorded into graphs
         f();
         return;
     }

@@ -96,6 +96,15 @@ class ParTree:
         for k, v in params.items():
             check(lib().spaqr_tree_set_tuning(self._h, self.TUNABLES.index(k), int(v)), self._h, "set_tuning " + k)
 
+    def set_graphs(self, on: bool = True):
+        """hipGraph mode: record the factorization's launch segments once, replay them after."""
+        check(lib().spaqr_tree_set_graphs(self._h, int(bool(on))), self._h, "set_graphs")
+
+    def graph_info(self) -> dict:
+        out = np.zeros(3, dtype=np.int64)
+        check(lib().spaqr_tree_graph_info(self._h, out.ctypes.data), self._h, "graph_info")
+        return dict(on=bool(out[0]), segments=int(out[1]), replays=int(out[2]))
+
     def tuning(self) -> dict:
         return {k: lib().spaqr_tree_get_tuning(self._h, i) for i, k in enumerate(self.TUNABLES)}
 
