@@ -653,6 +653,7 @@ void Tree::make_hierarchy() {
             b0 = e0;
         }
     }
+    cl_part = cl;
 }
 
 int Tree::partition(const HostCSC& A) {
@@ -735,7 +736,8 @@ int Tree::partition_batch(const HostCSC& A, const std::vector<int>& sizes, const
 // ---------------------------------------------------------------------------- assemble
 int Tree::assemble() {
     try {
-        if (cl.empty()) throw std::runtime_error("partition() first");
+        if (cl_part.empty()) throw std::runtime_error("partition() first");
+        cl = cl_part;  // a factorization since partition() has resized and flagged clusters
         App = permute_rc(A0, cperm_);
         ed.clear();
         for (auto& c : cl) { c.out.clear(); c.in.clear(); c.sout.clear(); c.sin.clear(); c.self_edge = -1; c.dist2.clear(); }

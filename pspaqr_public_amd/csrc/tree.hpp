@@ -201,6 +201,7 @@ private:
     std::vector<int> inst_n, inst_lv, inst_off;  // per system: unknowns, levels, first position
     int* d_seg = nullptr;                        // device copy of inst_off plus n (batch GMRES)
     void make_hierarchy();
+    std::vector<Cluster> cl_part;             // the hierarchy as partition() left it (assemble() starts there)
     std::vector<Cluster> cl0;                 // post-assembly structure, restored by factorize()
     std::vector<Edge> ed0;
     void* d_blocks0 = nullptr;                // pristine assembled blocks (factor precision)
