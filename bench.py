@@ -110,10 +110,13 @@ def main():
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--side", type=int, default=None)
     ap.add_argument("--iters", type=int, default=300)
-    ap.add_argument("--cpu-side", type=int, default=32, help="grid side of the CPU-baseline sample")
+    ap.add_argument("--cpu-side", type=int, default=None,
+                    help="grid side of a smaller CPU-baseline sample (default: the benchmarked workload itself; "
+                         "the batch uses the 32^3 config-2 system)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-mode2", action="store_true", help="skip the no-predictor leg")
+    ap.add_argument("--graphs", action="store_true", help="also time the FP64 path replayed from hip graphs")
     ap.add_argument("--rank-mode", type=int, default=1,
                     help="FP32 path's sparsification ranks: 1 learned sketch (default), 2 sketch without the "
                          "predictor, 0 geqp3 as the reference")
@@ -209,6 +212,16 @@ def main():
         fp64 = dict(seconds=s64, factor_s=f64, gmres_iters=max(q.iters for q in res64),
                     worst_true_relres=max(q.true_res for q in res64))
         T64.close()
+        if args.graphs:
+            # the same FP64 path with every level's launches replayed from recorded hip graphs
+            TG, _, bg, xg, _ = prepare(64)
+            TG.set_graphs(True)
+            run_steps(TG, bg, xg, TG.n, 2, 0, args.iters, "fp64 graphs record", False, batch)
+            sg, fg, rg = run_steps(TG, bg, xg, TG.n, max(1, args.steps), 0, args.iters, "fp64 graphs replay", False,
+                                   batch)
+            fp64["graph_replay"] = dict(seconds=sg, factor_s=fg, gmres_iters=max(q.iters for q in rg),
+                                        worst_true_relres=max(q.true_res for q in rg), **TG.graph_info())
+            TG.close()
 
     # HBM bytes per step of each launch class, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
     # (calibrated on a device copy) kept beside the kernel-trace summaries; per launch below
@@ -239,19 +252,25 @@ def main():
     cpu = None
     if not args.skip_cpu:
         cfg_cpu = 2 if batch else args.config
-        Ps, cpu_s, cit, cres, threads = cpu_baseline(cfg_cpu, args.cpu_side)
-        Ts = build_tree(Ps, 32)
-        set_rank_mode(Ts, args.rank_mode)
-        bs = Ps.b[Ts.cperm()]
-        bsd, xsd = DeviceBuffer.from_array(bs), DeviceBuffer(8 * Ts.n)
-        run_steps(Ts, bsd, xsd, Ts.n, 1, restart, args.iters, "fp32 sample warmup", True)
-        sg, _, _ = run_steps(Ts, bsd, xsd, Ts.n, 1, restart, args.iters, "fp32 sample", True)
-        cpu = dict(value=cpu_s, unit="s (factor+GMRES to 1e-12 on the %s sample, n=%d)" % (Ps.name, Ps.A.shape[0]),
+        side_cpu = args.cpu_side if (args.cpu_side or batch) else args.side
+        Ps, cpu_s, cit, cres, threads = cpu_baseline(cfg_cpu, side_cpu if not batch else (args.cpu_side or None))
+        if not batch and side_cpu == args.side:
+            sg = step_s  # the sample is the benchmarked workload: its GPU time is the timed step
+            what = "the benchmarked workload itself"
+        else:
+            Ts = build_tree(Ps, 32)
+            set_rank_mode(Ts, args.rank_mode)
+            bs = Ps.b[Ts.cperm()]
+            bsd, xsd = DeviceBuffer.from_array(bs), DeviceBuffer(8 * Ts.n)
+            run_steps(Ts, bsd, xsd, Ts.n, 1, restart, args.iters, "fp32 sample warmup", True)
+            sg, _, _ = run_steps(Ts, bsd, xsd, Ts.n, 1, restart, args.iters, "fp32 sample", True)
+            Ts.close()
+            what = "a smaller system of the same generator"
+        cpu = dict(value=cpu_s, unit="s (factor+GMRES to 1e-12 on %s, n=%d)" % (Ps.name, Ps.A.shape[0]),
                    cores=threads, kind="port",
-                   sample="%s: same generator and tree parameters at side %d (oracle: FP64, %d thread(s), GMRES "
-                          "iters %d, relres %.2e); GPU FP32 path on the same sample: %.4f s"
-                          % (Ps.name, args.cpu_side, threads, cit, cres, sg))
-        Ts.close()
+                   sample="%s (%s): the oracle -- the CPU FP64 restatement of the reference path, its per-level "
+                          "loops on %d OpenMP threads -- GMRES iters %d, relres %.2e; the GPU FP32 path on the same "
+                          "system: %.4f s" % (Ps.name, what, threads, cit, cres, sg))
 
     per_class = {c: dict(ms_per_step=round(v["ms"] / max(args.steps, 1), 4), launches=v["launches"],
                          gflop=round(v["flops"] / 1e9, 4), gbytes=round(v["bytes"] / 1e9, 4))

@@ -115,6 +115,22 @@ int spaqr_tree_assemble(spaqr_tree* t);
 int spaqr_tree_factorize(spaqr_tree* t);
 /* ParTree::solve(VectorXd& x): x <- M^{-1} x, in place (device)       src/ptree.cpp:2788 */
 int spaqr_tree_solve(spaqr_tree* t, double* x_dev);
+/* The preconditioner apply in phases, for callers that replay it as a list of operations (the
+   fwd()/bwd() objects of include/operations.h:13-19 and include/toperations.h:14-20):
+   solve_load copies x (device, FP64, tree ordering) into the solve workspace; solve_phase runs
+   phase `phase` of level `level` forward (dir 1) or backward (dir 0) -- 0 merge
+   (src/ptree.cpp:2244-2252 / 3105-3123), 1 interior QR (Q^T and the coupling chains
+   :2846-2851, 2160-2186 / triangular solve with the off-diagonal reduction :2188-2192),
+   2 sparsification (:2957-2963 / 3146-3174); solve_store copies the workspace back to x.
+   Forward over levels 0.. and phases 0, 1, 2, then backward over levels ..0 and phases 2, 1, 0
+   is spaqr_tree_solve, kernel for kernel. */
+int spaqr_tree_solve_load(spaqr_tree* t, const double* x_dev);
+int spaqr_tree_solve_phase(spaqr_tree* t, int level, int phase, int dir);
+int spaqr_tree_solve_store(spaqr_tree* t, double* x_dev);
+/* Host vectors between the natural and the tree ordering: to_natural != 0: out[cperm[k]] = in[k]
+   (the cperm.asPermutation() * x of src/ptree.cpp:2783); else out[k] = in[cperm[k]]
+   (its transpose, :2272).                                                                   */
+int spaqr_tree_permute(const spaqr_tree* t, int to_natural, const double* in, double* out);
 /* ParTree::spmv(VectorXd x): y = A x (device)                         src/ptree.cpp:3212 */
 int spaqr_tree_spmv(spaqr_tree* t, const double* x_dev, double* y_dev);
 /* dist_gmres(A, b, M, iter, tol, verb=true): x solves A x = b (device vectors, permuted

@@ -146,6 +146,10 @@ EXPORTS = {
     "spaqr_tree_num_systems": (_i, [_vp]),
     "spaqr_tree_set_tuning": (_i, [_vp, _i, _i]),
     "spaqr_tree_set_graphs": (_i, [_vp, _i]),
+    "spaqr_tree_solve_load": (_i, [_vp, _vp]),
+    "spaqr_tree_solve_phase": (_i, [_vp, _i, _i, _i]),
+    "spaqr_tree_solve_store": (_i, [_vp, _vp]),
+    "spaqr_tree_permute": (_i, [_vp, _i, _vp, _vp]),
     "spaqr_tree_graph_info": (_i, [_vp, _vp]),
     "spaqr_tree_get_tuning": (_i, [_vp, _i]),
     "spaqr_tree_get_systems": (_i, [_vp, _vp]),

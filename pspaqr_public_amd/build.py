@@ -55,6 +55,24 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+CONSUMER_SRC = os.path.join(os.path.dirname(HERE), "tests", "cxx", "ptree_consumer.cpp")
+CONSUMER = os.path.join(os.path.dirname(HERE), "tests", "cxx", "ptree_consumer")
+
+
+def build_consumer(verbose: bool = False) -> str:
+    """The C++ test program that drives the library through include/spaqr_ptree.hpp (plain g++,
+    linked against libspaqr_gpu.so, found next to it at run time through its rpath)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(os.path.dirname(HERE), "include"), CONSUMER_SRC,
+           "-L", HERE, "-lspaqr_gpu", "-Wl,-rpath,$ORIGIN/../../pspaqr_public_amd", "-o", CONSUMER]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("consumer build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    if verbose:
+        print("built", CONSUMER)
+    return CONSUMER
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_consumer(verbose=True)
     sys.exit(0)

@@ -247,6 +247,30 @@ int spaqr_gmres_batch(spaqr_tree* h, const double* b, double* x, int iters, doub
 int spaqr_tree_assemble(spaqr_tree* h) { TREE_OR_FAIL(h); return h->t->assemble(); }
 int spaqr_tree_factorize(spaqr_tree* h) { TREE_OR_FAIL(h); return h->t->factorize(); }
 int spaqr_tree_solve(spaqr_tree* h, double* x) { TREE_OR_FAIL(h); if (!x) return -2; return h->t->solve_dev(x); }
+int spaqr_tree_solve_load(spaqr_tree* h, const double* x) {
+    TREE_OR_FAIL(h);
+    if (!x) return -2;
+    return h->t->solve_load(x);
+}
+int spaqr_tree_solve_phase(spaqr_tree* h, int level, int phase, int dir) {
+    TREE_OR_FAIL(h);
+    return h->t->solve_phase(level, phase, dir);
+}
+int spaqr_tree_solve_store(spaqr_tree* h, double* x) {
+    TREE_OR_FAIL(h);
+    if (!x) return -2;
+    return h->t->solve_store(x);
+}
+int spaqr_tree_permute(const spaqr_tree* h, int to_natural, const double* in, double* out) {
+    TREE_OR_FAIL(h);
+    if (!in || !out || in == out) return -3;
+    const auto& p = h->t->cperm();
+    if (to_natural)
+        for (size_t k = 0; k < p.size(); ++k) out[p[k]] = in[k];
+    else
+        for (size_t k = 0; k < p.size(); ++k) out[k] = in[p[k]];
+    return 0;
+}
 int spaqr_tree_spmv(spaqr_tree* h, const double* x, double* y) {
     TREE_OR_FAIL(h);
     if (!x || !y) return -2;

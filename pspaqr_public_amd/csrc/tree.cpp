@@ -2276,32 +2276,85 @@ void Tree::solve_sched_levels(int upto) {
     }
 }
 
+// One phase of one level of the preconditioner apply on the solve workspace (ParTree::solve,
+// src/ptree.cpp:2798-3203): 0 merge (merge_fwd :2244-2252 / merge_bwd :3105-3123), 1 interior
+// QR (fwd_geqrt :2846-2851 and fwd_tsqrt :2160-2186 / bwd_trsv :2188-2192 with the reduction of
+// the off-diagonal blocks), 2 sparsification (fwd_spars :2957-2963 / bwd_spars :3146-3174).
+template <typename S>
+void Tree::solve_phase_t(int L, int phase, int dir) {
+    auto& lv = sched->lv[L];
+    if (phase == 0) {
+        if (L > 0 && dir) launch_vec_copy<S>((const VecCopyTask<S>*)lv.merge_fwd, lv.n_merge_fwd, stream_);
+        if (L > 0 && !dir) launch_vec_copy<S>((const VecCopyTask<S>*)lv.merge_bwd, lv.n_merge_bwd, stream_);
+    } else if (phase == 1) {
+        if (dir) {
+            launch_vec_q<S>((const VecQTask<S>*)lv.q, lv.n_q, stream_);
+            const auto links = (const VecChainLink<S>*)lv.chain_links;
+            for (auto& c : lv.chain) {
+                launch_vec_chain_wave<S>((const VecChainTask<S>*)c[0].first, links, c[0].second, stream_);
+                launch_vec_chain_pipe<S>((const VecChainTask<S>*)c[1].first, links, c[1].second, 1, stream_);
+                launch_vec_chain_pipe<S>((const VecChainTask<S>*)c[2].first, links, c[2].second, 2, stream_);
+                launch_vec_chain<S>((const VecChainTask<S>*)c[3].first, links, c[3].second, stream_);
+            }
+        } else {
+            launch_vec_trsv<S>((const VecTrsvTask<S>*)lv.trsv, (const VecGemvLink<S>*)lv.trsv_links, lv.n_trsv, stream_);
+        }
+    } else {
+        launch_vec_spars<S>((const VecSparsTask<S>*)lv.spars, lv.n_spars, dir ? 1 : 0, stream_);
+    }
+}
+
 template <typename S>
 int Tree::solve_t(double* x) {
     S* X0 = (S*)xw;
     launch_convert_d2s<S>(n, x, X0, stream_);
-    for (int L = 0; L < nlevels; ++L) {
-        auto& lv = sched->lv[L];
-        if (L > 0) launch_vec_copy<S>((const VecCopyTask<S>*)lv.merge_fwd, lv.n_merge_fwd, stream_);
-        launch_vec_q<S>((const VecQTask<S>*)lv.q, lv.n_q, stream_);
-        const auto links = (const VecChainLink<S>*)lv.chain_links;
-        for (auto& c : lv.chain) {
-            launch_vec_chain_wave<S>((const VecChainTask<S>*)c[0].first, links, c[0].second, stream_);
-            launch_vec_chain_pipe<S>((const VecChainTask<S>*)c[1].first, links, c[1].second, 1, stream_);
-            launch_vec_chain_pipe<S>((const VecChainTask<S>*)c[2].first, links, c[2].second, 2, stream_);
-            launch_vec_chain<S>((const VecChainTask<S>*)c[3].first, links, c[3].second, stream_);
-        }
-        launch_vec_spars<S>((const VecSparsTask<S>*)lv.spars, lv.n_spars, 1, stream_);
-    }
-    for (int L = nlevels - 1; L >= 0; --L) {
-        auto& lv = sched->lv[L];
-        launch_vec_spars<S>((const VecSparsTask<S>*)lv.spars, lv.n_spars, 0, stream_);
-        launch_vec_trsv<S>((const VecTrsvTask<S>*)lv.trsv, (const VecGemvLink<S>*)lv.trsv_links, lv.n_trsv, stream_);
-        if (L > 0) launch_vec_copy<S>((const VecCopyTask<S>*)lv.merge_bwd, lv.n_merge_bwd, stream_);
-    }
+    for (int L = 0; L < nlevels; ++L)
+        for (int ph = 0; ph < 3; ++ph) solve_phase_t<S>(L, ph, 1);
+    for (int L = nlevels - 1; L >= 0; --L)
+        for (int ph = 2; ph >= 0; --ph) solve_phase_t<S>(L, ph, 0);
     launch_convert_s2d<S>(n, X0, x, stream_);
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+int Tree::solve_load(const double* x) {
+    try {
+        if (!factorized || !sched) throw std::runtime_error("factorize() first");
+        if (prec_fact == 32) launch_convert_d2s<float>(n, x, (float*)xw, stream_);
+        else launch_convert_d2s<double>(n, x, (double*)xw, stream_);
+        HIPCHK(hipGetLastError());
+        return 0;
+    } catch (std::exception& e) {
+        last_error = e.what();
+        return -1;
+    }
+}
+
+int Tree::solve_phase(int level, int phase, int dir) {
+    try {
+        if (!factorized || !sched) throw std::runtime_error("factorize() first");
+        if (level < 0 || level >= nlevels || phase < 0 || phase > 2) throw std::runtime_error("no such solve phase");
+        if (prec_fact == 32) solve_phase_t<float>(level, phase, dir);
+        else solve_phase_t<double>(level, phase, dir);
+        HIPCHK(hipGetLastError());
+        return 0;
+    } catch (std::exception& e) {
+        last_error = e.what();
+        return -1;
+    }
+}
+
+int Tree::solve_store(double* x) {
+    try {
+        if (!factorized || !sched) throw std::runtime_error("factorize() first");
+        if (prec_fact == 32) launch_convert_s2d<float>(n, (const float*)xw, x, stream_);
+        else launch_convert_s2d<double>(n, (const double*)xw, x, stream_);
+        HIPCHK(hipGetLastError());
+        return 0;
+    } catch (std::exception& e) {
+        last_error = e.what();
+        return -1;
+    }
 }
 
 int Tree::solve_dev(double* x) {

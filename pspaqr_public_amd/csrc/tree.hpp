@@ -147,6 +147,14 @@ public:
     int assemble();
     int factorize();
     int solve_dev(double* x);                       // preconditioner apply, in place, device vector
+    // The preconditioner apply in pieces: load x into the solve workspace, run one phase of one
+    // level forward (dir 1) or backward (dir 0) -- phase 0 merge, 1 interior QR (forward: Q^T
+    // and the coupling chains; backward: the triangular solve), 2 sparsification -- and store
+    // the workspace back.  Forward over the levels and phases in order, then backward in
+    // reverse, is solve_dev.
+    int solve_load(const double* x);
+    int solve_phase(int level, int phase, int dir);
+    int solve_store(double* x);
     int spmv_dev(const double* x, double* y);       // y = A_perm x, device vectors
     int gmres_dev(const double* b, double* x, int iters, double tol, int restart, double true_tol,
                   GmresResult* res, double* hist, int hist_cap, int* hist_len, bool flexible = false);
@@ -296,6 +304,7 @@ private:
     template <typename S> void solve_sched_begin();
     template <typename S> void solve_sched_levels(int upto);
     template <typename S> int solve_t(double* x);
+    template <typename S> void solve_phase_t(int level, int phase, int dir);
     void* upload(const void* h, size_t bytes);
     // pinned staging for descriptor uploads: one host copy, then an asynchronous DMA; the
     // chunks are reused after every stream synchronisation

@@ -1,0 +1,136 @@
+// A C++ caller of the MI355X path written against the reference's API shape: ParTree,
+// dist_gmres and the Operation list (include/spaqr_ptree.hpp over include/spaqr_gpu.h), with
+// no Python in the process.  It builds the 2D 5-point Poisson problem on a 64 x 64 grid with the
+// driver's preprocessing (A <- A D 10, pspaQR.cpp:132-147; RHS std::mt19937(0) +
+// uniform_real_distribution(-1, 1)), then partition -> assemble -> factorize -> dist_gmres, and
+// checks: true relative residual <= 1e-12; the Operation replay equals solve(x) bit for bit;
+// solve(b, x) in the natural ordering equals the permuted in-place solve.  Prints one line of
+// results and exits non-zero on any failed check.
+#include <cmath>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "spaqr_ptree.hpp"
+
+struct Csc {  // compressed column storage with the accessors of Eigen::SparseMatrix
+    int n = 0;
+    std::vector<int> colptr, rowind;
+    std::vector<double> val;
+    int rows() const { return n; }
+    int cols() const { return n; }
+    const int* outerIndexPtr() const { return colptr.data(); }
+    const int* innerIndexPtr() const { return rowind.data(); }
+    const double* valuePtr() const { return val.data(); }
+};
+
+struct Coords {  // d x n column-major, as Eigen::MatrixXd
+    int d = 0, n = 0;
+    std::vector<double> x;
+    int rows() const { return d; }
+    int cols() const { return n; }
+    const double* data() const { return x.data(); }
+};
+
+static Csc poisson2d(int m) {
+    Csc A;
+    A.n = m * m;
+    A.colptr.push_back(0);
+    for (int j = 0; j < A.n; ++j) {  // column j, rows in increasing order
+        const int i0 = j / m, i1 = j % m;
+        const int nb[5] = {j - m, j - 1, j, j + 1, j + m};
+        const bool ok[5] = {i0 > 0, i1 > 0, true, i1 < m - 1, i0 < m - 1};
+        for (int q = 0; q < 5; ++q)
+            if (ok[q]) {
+                A.rowind.push_back(nb[q]);
+                A.val.push_back(q == 2 ? 4.0 : -1.0);
+            }
+        A.colptr.push_back((int)A.rowind.size());
+    }
+    for (int j = 0; j < A.n; ++j) {  // A <- A D 10, D_jj = 1 / ||A(:, j)||
+        double s = 0;
+        for (int k = A.colptr[j]; k < A.colptr[j + 1]; ++k) s += A.val[k] * A.val[k];
+        const double d = 1.0 / std::sqrt(s);
+        for (int k = A.colptr[j]; k < A.colptr[j + 1]; ++k) A.val[k] = A.val[k] * d * 10.0;
+    }
+    return A;
+}
+
+int main() {
+    const int m = 64, n = m * m;
+    Csc A = poisson2d(m);
+    Coords X;
+    X.d = 2;
+    X.n = n;
+    for (int i0 = 0; i0 < m; ++i0)
+        for (int i1 = 0; i1 < m; ++i1) {
+            X.x.push_back(i0);
+            X.x.push_back(i1);
+        }
+    std::mt19937 gen(0);
+    std::uniform_real_distribution<double> dis(-1.0, 1.0);
+    std::vector<double> b(n);
+    for (auto& v : b) v = dis(gen);
+
+    int failed = 0;
+    try {
+        spaqr::ParTree t(6, 0);
+        t.set_tol(1e-2);
+        t.set_scale(1);
+        t.set_order(1.0f);
+        t.set_square(1);
+        t.set_nthreads(16);
+        t.set_Xcoo(&X);
+        t.partition(A);
+        t.assemble(A);
+        t.factorize();
+
+        // the right-hand side in the tree ordering
+        std::vector<double> bt(n);
+        spaqr::detail::check(spaqr_tree_permute(t.handle(), 0, b.data(), bt.data()), t.handle(), "permute");
+
+        // dist_gmres: b is overwritten with the solution
+        std::vector<double> x = bt;
+        const unsigned long it = spaqr::dist_gmres(A, x, t, 300, 1e-12, true);
+        std::vector<double> Ax = t.spmv(x);
+        double rn = 0, bn = 0;
+        for (int i = 0; i < n; ++i) {
+            rn += (bt[i] - Ax[i]) * (bt[i] - Ax[i]);
+            bn += bt[i] * bt[i];
+        }
+        const double relres = std::sqrt(rn / bn);
+        if (!(relres <= 1e-12)) failed |= 1;
+
+        // the Operation replay is solve(x), kernel for kernel
+        std::vector<double> x1 = bt, x2 = bt;
+        t.solve(x1);
+        t.apply_operations(x2);
+        double dmax = 0;
+        for (int i = 0; i < n; ++i) dmax = std::fmax(dmax, std::fabs(x1[i] - x2[i]));
+        if (dmax != 0.0) failed |= 2;
+
+        // solve(b, x) in the natural ordering against the in-place solve in the tree ordering
+        std::vector<double> xn, x1n(n);
+        t.solve(b, xn);
+        spaqr::detail::check(spaqr_tree_permute(t.handle(), 1, x1.data(), x1n.data()), t.handle(), "permute");
+        double dn = 0;
+        for (int i = 0; i < n; ++i) dn = std::fmax(dn, std::fabs(xn[i] - x1n[i]));
+        if (dn != 0.0) failed |= 4;
+
+        // extract_x returns the last preconditioner result in the natural ordering
+        std::vector<double> xe;
+        t.extract_x(xe);
+        double de = 0;
+        for (int i = 0; i < n; ++i) de = std::fmax(de, std::fabs(xe[i] - xn[i]));
+        if (de != 0.0) failed |= 8;
+
+        std::printf("consumer: n=%d levels=6 gmres_iters=%lu relres=%.3e ops=%zu replay_diff=%g natural_diff=%g "
+                    "extract_diff=%g failed=%d\n",
+                    n, it, relres, t.operations().size(), dmax, dn, de, failed);
+    } catch (std::exception& e) {
+        std::printf("consumer: error: %s\n", e.what());
+        failed |= 16;
+    }
+    spaqr_shutdown();
+    return failed ? 1 : 0;
+}
