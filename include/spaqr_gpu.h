@@ -90,6 +90,18 @@ int spaqr_tree_num_systems(const spaqr_tree* t);
    The committed autotuner (python -m pspaqr_public_amd.autotune) picks them per workload. */
 int spaqr_tree_set_tuning(spaqr_tree* t, int param, int value);
 int spaqr_tree_get_tuning(const spaqr_tree* t, int param);
+/* Numerical status (the per-block info[] of the factorization).  After each level's
+   elimination a check reads the diagonal of every interior's R: a non-finite entry (flag 1) or a
+   pivot at or below rel * max |R_ii| (flag 2; rel 0, the default, flags exact zeros only) is
+   read back at the level's next host synchronisation and spaqr_tree_factorize returns 1 (a
+   numerical failure, in the LAPACK info convention) naming the level and cluster.  The
+   reference asserts on a NaN residual (src/is.cpp:460) and exits on an empty interior
+   (src/ptree.cpp:171-175).  get_info copies up to cap flags, one per cluster (rows of
+   spaqr_tree_get_clusters), also flag 4 for an interface whose sketch failed the tail test and
+   took geqp3's rank; returns the number of clusters.  spaqr_tree_partition rejects non-finite
+   entries and zero columns before any GPU work. */
+int spaqr_tree_set_pivot_check(spaqr_tree* t, double rel);
+int spaqr_tree_get_info(spaqr_tree* t, int* out, int cap);
 /* hipGraph mode (no reference counterpart; it replaces the per-level task flows of
    src/ptree.cpp:885-2123 being re-issued on every factorization).  With on != 0, a
    factorization records every stretch of launches between two host synchronisations (the

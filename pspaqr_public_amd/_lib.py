@@ -146,6 +146,8 @@ EXPORTS = {
     "spaqr_tree_num_systems": (_i, [_vp]),
     "spaqr_tree_set_tuning": (_i, [_vp, _i, _i]),
     "spaqr_tree_set_graphs": (_i, [_vp, _i]),
+    "spaqr_tree_set_pivot_check": (_i, [_vp, _d]),
+    "spaqr_tree_get_info": (_i, [_vp, _vp, _i]),
     "spaqr_tree_solve_load": (_i, [_vp, _vp]),
     "spaqr_tree_solve_phase": (_i, [_vp, _i, _i, _i]),
     "spaqr_tree_solve_store": (_i, [_vp, _vp]),

@@ -201,6 +201,25 @@ int spaqr_tree_get_tuning(const spaqr_tree* h, int param) {
     if (!h || !h->t || param < 0 || param >= T_NPARAM) return -1;
     return h->t->tune[param];
 }
+int spaqr_tree_set_pivot_check(spaqr_tree* h, double rel) {
+    TREE_OR_FAIL(h);
+    if (!(rel >= 0.0 && rel < 1.0)) { h->t->last_error = "the pivot threshold must be in [0, 1)"; return -2; }
+    h->t->pivot_rel = rel;
+    return 0;
+}
+int spaqr_tree_get_info(spaqr_tree* h, int* out, int cap) {
+    TREE_OR_FAIL(h);
+    try {
+        std::vector<int> v;
+        const int n = h->t->get_info(v);
+        if (out)
+            for (int i = 0; i < std::min(n, std::max(cap, 0)); ++i) out[i] = v[i];
+        return n;
+    } catch (std::exception& e) {
+        h->t->last_error = e.what();
+        return -100;
+    }
+}
 int spaqr_tree_set_graphs(spaqr_tree* h, int on) {
     TREE_OR_FAIL(h);
     return h->t->set_graphs(on != 0);

@@ -37,9 +37,40 @@
 #include "launch.hpp"
 #include "rrqr.hpp"
 
+#include "status.hpp"
 #include "tree_impl.hpp"
 
 namespace spaqr {
+
+namespace {
+struct NumericalFailure : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+}  // namespace
+
+// Reads the per-level status flags written by the pivot checks so far (the stream has been
+// synchronised) and ends the factorization on the first level that raised one.
+void Tree::check_levels(int upto) {
+    if (!d_lflags) return;
+    std::vector<int> f(nlevels), first(nlevels);
+    HIPCHK(hipMemcpy(f.data(), d_lflags, sizeof(int) * nlevels, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(first.data(), d_lfirst, sizeof(int) * nlevels, hipMemcpyDeviceToHost));
+    for (int L = 0; L <= std::min(upto, nlevels - 1); ++L)
+        if (f[L]) {
+            std::string what = (f[L] & kInfoNonFinite) ? "a non-finite entry" : "a zero pivot";
+            if ((f[L] & kInfoZeroPivot) && pivot_rel > 0) what = "a pivot at or below the set threshold";
+            throw NumericalFailure("numerical failure: " + what + " in the R factor of interior cluster " +
+                                   std::to_string(first[L]) + " at level " + std::to_string(L) +
+                                   " (spaqr_tree_get_info lists every flagged cluster)");
+        }
+}
+
+int Tree::get_info(std::vector<int>& out) {
+    out.assign(cl.size(), 0);
+    if (d_cinfo) HIPCHK(hipMemcpy(out.data(), d_cinfo, sizeof(int) * cl.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < out.size() && i < h_info.size(); ++i) out[i] |= h_info[i];
+    return (int)out.size();
+}
 
 bool ClusterID::operator<(const ClusterID& o) const {  // include/cluster.h:128-134
     if (self != o.self) return self < o.self;
@@ -308,6 +339,7 @@ int Tree::replay_t() {
     HIPCHK(hipMemcpyAsync(&chain_err, gplan.chain_err, sizeof(int), hipMemcpyDeviceToHost, stream_));
     sync_stream();
     if (chain_err) throw std::runtime_error("cooperative tpqrt: a wait exceeded its spin limit");
+    check_levels(nlevels - 1);
     factorized = true;
     ++graph_replays;
     t_factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -315,6 +347,10 @@ int Tree::replay_t() {
 }
 
 void Tree::sync_stream() {
+    if (!stream_) {  // nothing was ever issued
+        staging.reset();
+        return;
+    }
     HIPCHK(hipStreamSynchronize(stream_));
     staging.reset();  // every upload queued so far has completed
     for (auto& p : prof_pending) {
@@ -367,7 +403,27 @@ Tree::Tree(int nl, int sk) : nlevels(nl), skip(sk) {
     bottoms.resize(nl);
     interiors.resize(nl);
     interfaces.resize(nl);
-    HIPCHK(hipStreamCreate(&stream_));
+}
+
+// The stream is made on the first call that uses the GPU (assemble): ordering and the cluster
+// hierarchy are host work and run on a machine without one.
+void Tree::ensure_stream() {
+    if (!stream_) HIPCHK(hipStreamCreate(&stream_));
+}
+
+// The matrix a tree accepts (the reference asserts these: src/is.cpp:460 on a NaN residual,
+// src/ptree.cpp:171-175 exits on an empty interior): finite values, no empty column.
+static void check_matrix(const HostCSC& A) {
+    for (int j = 0; j < A.nc; ++j) {
+        bool nonzero = false;
+        for (int k = A.colptr[j]; k < A.colptr[j + 1]; ++k) {
+            if (!std::isfinite(A.val[k]))
+                throw std::runtime_error("A(" + std::to_string(A.rowind[k]) + ", " + std::to_string(j) + ") is not finite");
+            if (A.rowind[k] < 0 || A.rowind[k] >= A.nr) throw std::runtime_error("row index out of range in column " + std::to_string(j));
+            nonzero = nonzero || A.val[k] != 0.0;
+        }
+        if (!nonzero) throw std::runtime_error("column " + std::to_string(j) + " of A is zero: A is singular");
+    }
 }
 
 Tree::~Tree() {
@@ -660,9 +716,12 @@ int Tree::partition(const HostCSC& A) {
     try {
         if (A.nr != A.nc) throw std::runtime_error("only square systems are supported");
         if (X.empty() || xdim <= 0) throw std::runtime_error("geometric partition needs coordinates (set_Xcoo)");
+        if ((long long)X.size() != (long long)xdim * A.nc) throw std::runtime_error("coordinate array size mismatch");
+        for (double v : X)
+            if (!std::isfinite(v)) throw std::runtime_error("coordinates must be finite");
+        check_matrix(A);
         A0 = A;
         n = A.nc;
-        if ((long long)X.size() != (long long)xdim * n) throw std::runtime_error("coordinate array size mismatch");
         inst_n.assign(1, n);
         inst_lv.assign(1, nlevels);
         inst_off.assign(1, 0);
@@ -686,6 +745,7 @@ int Tree::partition_batch(const HostCSC& A, const std::vector<int>& sizes, const
             lmax = std::max(lmax, levels[s]);
         }
         if (total != A.nc) throw std::runtime_error("the system sizes must add up to the matrix size");
+        check_matrix(A);
         if (lmax != nlevels) throw std::runtime_error("the tree's level count must equal the largest system's");
         if (X.empty() || xdim <= 0) throw std::runtime_error("geometric partition needs coordinates (set_Xcoo)");
         n = A.nc;
@@ -737,6 +797,7 @@ int Tree::partition_batch(const HostCSC& A, const std::vector<int>& sizes, const
 int Tree::assemble() {
     try {
         if (cl_part.empty()) throw std::runtime_error("partition() first");
+        ensure_stream();
         cl = cl_part;  // a factorization since partition() has resized and flagged clusters
         App = permute_rc(A0, cperm_);
         ed.clear();
@@ -1011,6 +1072,14 @@ int Tree::factorize_t() {
     }
     d_chain_err = (int*)arena.alloc(sizeof(int));
     launch_fill<int>(d_chain_err, 1, 0, stream_);
+    // status: per cluster, and per level (flags, first cluster)
+    d_cinfo = (int*)arena.alloc(sizeof(int) * std::max<size_t>(cl.size(), 1));
+    d_lflags = (int*)arena.alloc(sizeof(int) * nlevels);
+    d_lfirst = (int*)arena.alloc(sizeof(int) * nlevels);
+    launch_fill<int>(d_cinfo, cl.size(), 0, stream_);
+    launch_fill<int>(d_lflags, (size_t)nlevels, 0, stream_);
+    launch_fill<int>(d_lfirst, (size_t)nlevels, INT_MAX, stream_);
+    h_info.assign(cl.size(), 0);
     // host-phase timers (SPAQR_DEBUG_HOST=1 prints them per factorization)
     static const bool dbg_host = getenv("SPAQR_DEBUG_HOST") != nullptr;
     double hms[24] = {0};
@@ -1331,6 +1400,14 @@ int Tree::factorize_t() {
                 });
                 if (profiling) prof[P_TPQRT].launches += nl;
             }
+            // the pivots of every interior's R, now final
+            std::vector<PivotCheck<S>> pc;
+            pc.reserve(ints.size());
+            for (int c : ints) {
+                const Edge& es = ed[cl[c].self_edge];
+                pc.push_back({(const S*)es.A, cl[c].cols, es.rows, c, L});
+            }
+            launch_check_pivots<S>(upload_vec(pc), (int)pc.size(), pivot_rel, d_cinfo, d_lflags, d_lfirst, stream_);
             HIPCHK(hipGetLastError());
         }
         hacc(2, ht);
@@ -1607,6 +1684,7 @@ int Tree::factorize_t() {
             HIPCHK(hipMemcpyAsync(&rrqr_err, d_rrqr_err, sizeof(int), hipMemcpyDeviceToHost, stream_));
             sync_stream();
             if (rrqr_err) throw std::runtime_error("cooperative geqp3: a barrier wait exceeded its spin limit");
+            check_levels(L);
             for (size_t q = 0; q < gq.size(); ++q) {
                 Cluster& C = cl[gq_cluster[q]];
                 const int k = C.cols;
@@ -1891,6 +1969,7 @@ int Tree::factorize_t() {
         HIPCHK(hipMemcpyAsync(&chain_err, d_chain_err, sizeof(int), hipMemcpyDeviceToHost, stream_));
         sync_stream();
         if (chain_err) throw std::runtime_error("cooperative tpqrt: a wait exceeded its spin limit");
+        check_levels(nlevels - 1);
     }
     hacc(5, ht2);
     factorized = true;
@@ -2095,6 +2174,12 @@ int Tree::factorize() {
             gplan.sig = plan_signature();
         }
         return rc;
+    } catch (NumericalFailure& e) {
+        last_error = e.what();
+        graph_rec = false;
+        drop_graphs();
+        factorized = false;
+        return 1;
     } catch (std::exception& e) {
         last_error = e.what();
         graph_rec = false;

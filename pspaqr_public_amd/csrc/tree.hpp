@@ -179,6 +179,11 @@ public:
     // Sketch rounds per interface in rank modes 1 and 2: the first sketch plus up to
     // rank_rounds - 1 retries with a doubled width before the geqp3 fallback (1..3).
     int rank_rounds = 3;
+    // Pivot check of every interior's R after its elimination: non-finite diagonal entries and
+    // pivots at or below pivot_rel * max |R_ii| (0: exactly zero) end the factorization with a
+    // numerical failure (factorize returns 1).  get_info: flags per cluster (status.hpp).
+    double pivot_rel = 0.0;
+    int get_info(std::vector<int>& out);
     // Planner tunables (TuneParam, spaqr_tree_set_tuning); 0 keeps the built-in value.
     int tune[16] = {0};
     // hipGraph mode: the first factorization records every stretch of launches between two host
@@ -236,6 +241,12 @@ private:
         std::vector<long long> sig;
     } gplan;
     std::vector<long long> plan_signature() const;
+    void ensure_stream();
+    int* d_cinfo = nullptr;   // per cluster (arena)
+    int* d_lflags = nullptr;  // per level: OR of the flags, and the first cluster that raised them
+    int* d_lfirst = nullptr;
+    std::vector<int> h_info;  // host-side flags (predictor misses)
+    void check_levels(int upto);
     void seg_begin();
     void seg_end(int* ranks_dev, int nranks, int* err_dev);
     void drop_graphs();

@@ -11,6 +11,7 @@
 #include "launch.hpp"
 #include "learned.hpp"
 #include "rrqr.hpp"
+#include "status.hpp"
 #include "tree_impl.hpp"
 
 namespace spaqr {
@@ -300,7 +301,10 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
     rstats.tasks += nt;
     for (int q = 0; q < nt; ++q) {
         if (used_sketch[q]) rstats.sketched++;
-        else if (sketch_mode && is_elig[q]) rstats.fallback++;
+        else if (sketch_mode && is_elig[q]) {
+            rstats.fallback++;
+            if ((size_t)gq_cluster[q] < h_info.size()) h_info[gq_cluster[q]] |= kInfoPredictorMiss;
+        }
         else rstats.direct++;
     }
     if (rank_log && ne) {

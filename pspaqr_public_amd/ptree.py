@@ -96,6 +96,17 @@ class ParTree:
         for k, v in params.items():
             check(lib().spaqr_tree_set_tuning(self._h, self.TUNABLES.index(k), int(v)), self._h, "set_tuning " + k)
 
+    def set_pivot_check(self, rel: float):
+        check(lib().spaqr_tree_set_pivot_check(self._h, float(rel)), self._h, "set_pivot_check")
+
+    def info(self) -> np.ndarray:
+        """Status flags per cluster: 1 non-finite R entry, 2 zero pivot, 4 predictor miss."""
+        n = lib().spaqr_tree_get_info(self._h, None, 0)
+        out = np.zeros(max(n, 0), dtype=np.int32)
+        if n > 0:
+            lib().spaqr_tree_get_info(self._h, out.ctypes.data, n)
+        return out
+
     def set_graphs(self, on: bool = True):
         """hipGraph mode: record the factorization's launch segments once, replay them after."""
         check(lib().spaqr_tree_set_graphs(self._h, int(bool(on))), self._h, "set_graphs")
