@@ -3212,7 +3212,17 @@ hipError_t launch_geqp3_coop(const Geqp3CoopTask<S>* d_tasks, const int* d_blk2t
     void* args[] = {(void*)&d_tasks, (void*)&d_blk2task, (void*)&d_err, (void*)&tol, (void*)&tie_rel};
     const void* fn = hbm ? (const void*)k_geqp3_coop<S, true> : (const void*)k_geqp3_coop<S, false>;
     if (!cooperative) return hipLaunchKernel(fn, dim3(nblocks), dim3(WG), args, (size_t)lds_bytes, s);
-    return hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(WG), args, (unsigned)lds_bytes, s);
+    hipError_t e = hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(WG), args, (unsigned)lds_bytes, s);
+    if (e == hipErrorIllegalState) {
+        // The device's cooperative queue belongs to one process at a time: while another
+        // process holds it, every cooperative launch here fails with this code.  The grid is
+        // within the co-resident limit and the blocks meet at their own barrier, whose wait has
+        // a spin limit, so an ordinary launch computes the same result (and a block that is
+        // never scheduled ends in an error, not a hang).
+        (void)hipGetLastError();
+        e = hipLaunchKernel(fn, dim3(nblocks), dim3(WG), args, (size_t)lds_bytes, s);
+    }
+    return e;
 }
 
 template <typename S>

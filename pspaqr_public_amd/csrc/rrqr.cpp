@@ -123,7 +123,11 @@ int geqp3_grouped(const std::vector<Geqp3Task<S>>& tasks, double tol, double tie
                              : (Geqp3Task<S>*)mem.upload(fallback.data(), fallback.size() * sizeof(Geqp3Task<S>));
     for (const auto& r : ready) {
         hipError_t e = launch_geqp3_coop<S>(r.t, r.b2t, d_err, r.blocks, tol, tie_rel, lds, r.hbm, stream, !mem.graph);
-        if (e != hipSuccess) throw std::runtime_error(std::string("cooperative geqp3 launch: ") + hipGetErrorString(e));
+        if (e != hipSuccess)
+            throw std::runtime_error(std::string("cooperative geqp3 launch: ") + hipGetErrorString(e) + " (blocks " +
+                                     std::to_string(r.blocks) + ", dynamic LDS " + std::to_string(lds) + " B, " +
+                                     (r.hbm ? "HBM" : "LDS") + " slices, co-resident limit " +
+                                     std::to_string(max_blocks_cache[pi][r.hbm ? 1 : 0]) + ")");
     }
     if (d_fb) launch_geqp3<S>(d_fb, (int)fallback.size(), tol, tie_rel, stream);
     if (dispatches) *dispatches = (int)ready.size() + (d_fb ? 1 : 0);

@@ -2,7 +2,12 @@
 reference-shaped C++ API (include/spaqr_ptree.hpp over include/spaqr_gpu.h) with no Python in
 its process: partition -> assemble -> factorize -> dist_gmres to 1e-12 on the 2D Poisson
 problem, the Operation replay equal to solve(x), solve(b, x) and extract_x in the natural
-ordering."""
+ordering.
+
+The program runs as a child of this test process, which has made cooperative launches of its
+own and so holds the device's cooperative queue: the child's cooperative geqp3 launches are
+refused with hipErrorIllegalState and must go through the ordinary launch within the
+co-resident limit."""
 import os
 import subprocess
 
