@@ -120,6 +120,8 @@ def main():
     ap.add_argument("--rank-mode", type=int, default=1,
                     help="FP32 path's sparsification ranks: 1 learned sketch (default), 2 sketch without the "
                          "predictor, 0 geqp3 as the reference")
+    ap.add_argument("--tune", default="", metavar="NAME=V,...",
+                    help="planner tunables applied after the autotuned table (FP32 leg), e.g. sketch_bf16=2")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times and exit: the reference dispatch "
                          "that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -164,6 +166,10 @@ def main():
         t = build_batch_tree(W, prec) if batch else build_tree(W, prec)
         set_rank_mode(t, rank_mode)
         used = autotune.apply(t, wkey, prec) if (tuned and wkey) else {}
+        if prec == 32 and args.tune:
+            extra = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.tune.split(",")}
+            t.set_tuning(extra)
+            used = dict(used, **extra)
         b = W.b[t.cperm()]
         return t, b, DeviceBuffer.from_array(b), DeviceBuffer(8 * t.n), used
 

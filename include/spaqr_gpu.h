@@ -87,6 +87,9 @@ int spaqr_tree_num_systems(const spaqr_tree* t);
      7 sketch width from which the sketch QR runs unpivoted (96; 16 .. 2^30)
      8 multi-panel coupling chains on the cooperative kernel (1 on, 2 off)
      9 blocks of at most 64 x 64 on the one-wave kernels (1 on, 2 off)
+    10 FP32 fronts: rank-path sketches C Omega in bf16 with FP32 accumulation (1, built in) or
+       in FP32 (2)
+    11 output-tile columns of the bf16 sketch (128; 64, 128, 256)
    The committed autotuner (python -m pspaqr_public_amd.autotune) picks them per workload. */
 int spaqr_tree_set_tuning(spaqr_tree* t, int param, int value);
 int spaqr_tree_get_tuning(const spaqr_tree* t, int param);
@@ -306,7 +309,8 @@ int spaqr_tree_rank_log_clusters(const spaqr_tree* t, int* out, int cap);
 
 /* Gaussian test matrix Omega (rows x cols, ld): entry (i, j) depends on (i, j, seed) only
    (splitmix64 of seed ^ (j << 32 | i), then Box-Muller), so a larger matrix agrees with a
-   smaller one on their overlap.  The planner uses seed 0x5eed5ca1e. */
+   smaller one on their overlap.  The planner uses seed 0x5eed5ca1e.  prec 16 writes bf16: the
+   FP32 matrix rounded to nearest even. */
 int spaqr_gauss_fill(int prec, void* omega, int rows, int cols, int ld, uint64_t seed, void* stream);
 
 /* Column norms: cn[j] = ||C(:, j)||^2 (w values) and *cmax2 = max(*cmax2, max_j cn[j]); tasks
@@ -317,6 +321,15 @@ int spaqr_colnorm_batched(int prec, int count, const spaqr_colnorm_desc* d, void
 /* Sketch Y (k x s, ld k) = C (k x w, ld k) * Omega(0:w, 0:s) (Omega ld ldo >= every w). */
 typedef struct { const void* C; void* Y; int k, w, s, pad; } spaqr_sketch_desc;
 int spaqr_sketch_batched(int prec, int count, const spaqr_sketch_desc* d, const void* omega, int ldo, void* stream);
+/* The BF16 sketch of FP32 C (no reference counterpart; the reference sketches in FP64,
+   random_rrqr, src/util.cpp:643-715): C and Omega rounded to bf16 (nearest even), products
+   accumulated in FP32 on the bf16 matrix cores, so |Y - C Omega| <= (2 u + u^2 + c w 2^-24)
+   |C| |Omega| elementwise with u = 2^-8.  Omega is bf16 (spaqr_gauss_fill prec 16) with ldo a
+   multiple of 8 and >= every w rounded up to 32 (rows w .. of each 32-row step are read and
+   discarded).  tile_cols (64, 128, 256): output-tile columns per workgroup; C is read once per
+   tile_cols columns of Y.  -4 Omega null or not 16-byte aligned, -5 bad ldo, -6 bad tile_cols. */
+int spaqr_sketch_bf16_batched(int count, const spaqr_sketch_desc* d, const void* omega_bf16, int ldo, int tile_cols,
+                              void* stream);
 
 /* The predictor's 32 input features of one interface (x, floats): from Y16, the geqrt-factored
    16-column sketch (its R diagonal Y16[j + j k]), the column norms cn and cmax2 of C, and the

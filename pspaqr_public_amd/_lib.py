@@ -137,6 +137,7 @@ EXPORTS = {
     "spaqr_gauss_fill": (_i, [_i, _vp, _i, _i, _i, C.c_uint64, _vp]),
     "spaqr_colnorm_batched": (_i, [_i, _i, _vp, _vp]),
     "spaqr_sketch_batched": (_i, [_i, _i, _vp, _vp, _i, _vp]),
+    "spaqr_sketch_bf16_batched": (_i, [_i, _vp, _vp, _i, _i, _vp]),
     "spaqr_rank_features_batched": (_i, [_i, _i, _vp, _d, _vp]),
     "spaqr_tail_select_batched": (_i, [_i, _i, _vp, _d, _vp]),
     "spaqr_shutdown": (_i, []),

@@ -36,9 +36,10 @@ CANDIDATES = {
     "sketch_unpivoted": [64, 96, 128, 192],
     "tpqrt_coop": [1, 2],
     "small_tasks": [1, 2],
+    "sketch_tile": [64, 128, 256],
 }
 BUILT_IN = {"big_front": 96, "tpmqrt_tile": 64, "larfb_cols": 128, "spars_cols": 128, "rank_cols": 128,
-            "sketch_unpivoted": 96, "tpqrt_coop": 1, "small_tasks": 1}
+            "sketch_unpivoted": 96, "tpqrt_coop": 1, "small_tasks": 1, "sketch_tile": 128}
 
 
 def key(workload: str, prec: int) -> str:

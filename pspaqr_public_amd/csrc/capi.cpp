@@ -190,7 +190,8 @@ int spaqr_tree_set_tuning(spaqr_tree* h, int param, int value) {
         case T_LARFB_COLS: valid = valid || in({16, 32, 64, 128}); break;
         case T_TRSM_ROWS: case T_SPARS_ROWS: case T_RANK_COLS: valid = valid || in({32, 64, 128, 256}); break;
         case T_SPARS_COLS: valid = valid || in({32, 64, 128}); break;
-        case T_TPQRT_COOP: case T_SMALL_TASKS: valid = valid || in({1, 2}); break;
+        case T_TPQRT_COOP: case T_SMALL_TASKS: case T_SKETCH_BF16: valid = valid || in({1, 2}); break;
+        case T_SKETCH_TILE: valid = valid || in({64, 128, 256}); break;
         default: break;
     }
     if (!valid) { h->t->last_error = "tuning value out of range for this parameter"; return -3; }
@@ -763,14 +764,15 @@ int spaqr_tree_rank_log_clusters(const spaqr_tree* h, int* out, int cap) {
 }
 
 int spaqr_gauss_fill(int prec, void* om, int rows, int cols, int ld, uint64_t seed, void* stream) {
-    if (prec != 32 && prec != 64) return -1;
+    if (prec != 16 && prec != 32 && prec != 64) return -1;
     if (!om && (size_t)rows * cols > 0) return -2;
     if (rows < 0 || cols < 0) return -3;
     if (ld < std::max(1, rows)) return -5;
     if ((size_t)rows * cols == 0) return 0;
     hipStream_t s = (hipStream_t)stream;
     if (prec == 64) launch_gauss<double>((double*)om, rows, cols, ld, seed, s);
-    else launch_gauss<float>((float*)om, rows, cols, ld, seed, s);
+    else if (prec == 32) launch_gauss<float>((float*)om, rows, cols, ld, seed, s);
+    else launch_gauss_bf16(om, rows, cols, ld, seed, s);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) { api_error() = hipGetErrorString(e); return -100; }
@@ -815,6 +817,30 @@ int spaqr_sketch_batched(int prec, int count, const spaqr_sketch_desc* d, const 
                            else
                                launch_sketch<float>((const SketchTask<float>*)dd, (const SketchTile*)dt, nt,
                                                     (const float*)omega, ldo, s);
+                       });
+}
+
+int spaqr_sketch_bf16_batched(int count, const spaqr_sketch_desc* d, const void* omega_bf16, int ldo, int tile_cols,
+                              void* stream) {
+    if (count < 0 || (count > 0 && !d)) return -2;
+    if (!omega_bf16 || ((uintptr_t)omega_bf16 & 15) != 0) return -4;
+    int wmax = 0;
+    for (int i = 0; i < count; ++i) {
+        if (d[i].k < 1 || d[i].w < 1 || d[i].s < 1 || !d[i].C || !d[i].Y) return -3;
+        wmax = std::max(wmax, d[i].w);
+    }
+    if (ldo % 8 != 0 || ldo < (wmax + 31) / 32 * 32) return -5;
+    if (tile_cols != 64 && tile_cols != 128 && tile_cols != 256) return -6;
+    if (count == 0) return 0;
+    std::vector<SketchTile> tiles;
+    for (int i = 0; i < count; ++i)
+        for (int r0 = 0; r0 < d[i].k; r0 += 64)
+            for (int c0 = 0; c0 < d[i].s; c0 += tile_cols) tiles.push_back({i, r0, c0, 0});
+    const int nt = (int)tiles.size();
+    return run_batched(d, sizeof(*d) * count, tiles.data(), sizeof(SketchTile) * tiles.size(), stream,
+                       [&](void* dd, void* dt, hipStream_t s) {
+                           launch_sketch_bf16((const SketchTask<float>*)dd, (const SketchTile*)dt, nt, omega_bf16,
+                                              ldo, tile_cols, s);
                        });
 }
 

@@ -26,16 +26,37 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
+__device__ __forceinline__ double gauss_entry(int i, int j, uint64_t seed) {
+    const uint64_t h1 = splitmix64(seed ^ (((uint64_t)j << 32) | (uint32_t)i));
+    const uint64_t h2 = splitmix64(h1);
+    const double u1 = 1.0 - (double)(h1 >> 11) * 0x1.0p-53;  // (0, 1]
+    const double u2 = (double)(h2 >> 11) * 0x1.0p-53;        // [0, 1)
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
 template <typename S>
 __global__ __launch_bounds__(WG) void k_gauss(S* om, int rows, int cols, int ld, uint64_t seed) {
     const size_t total = (size_t)rows * cols;
     for (size_t e = (size_t)blockIdx.x * WG + threadIdx.x; e < total; e += (size_t)gridDim.x * WG) {
         const int i = (int)(e % rows), j = (int)(e / rows);
-        const uint64_t h1 = splitmix64(seed ^ (((uint64_t)j << 32) | (uint32_t)i));
-        const uint64_t h2 = splitmix64(h1);
-        const double u1 = 1.0 - (double)(h1 >> 11) * 0x1.0p-53;  // (0, 1]
-        const double u2 = (double)(h2 >> 11) * 0x1.0p-53;        // [0, 1)
-        om[i + (size_t)j * ld] = (S)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+        om[i + (size_t)j * ld] = (S)gauss_entry(i, j, seed);
+    }
+}
+
+// The same matrix rounded to bf16 (round to nearest even) from its FP32 value, so that it is
+// exactly the FP32 Omega rounded.  The rounding is done on the float's bits: written as
+// (__bf16)(float)x the compiler fuses the two conversions into one from double, which rounds
+// the float's ties by the discarded double bits instead of to even.
+__device__ __forceinline__ uint16_t bf16_bits_rne(float f) {  // finite f
+    const uint32_t b = __float_as_uint(f);
+    return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+
+__global__ __launch_bounds__(WG) void k_gauss_bf16(uint16_t* om, int rows, int cols, int ld, uint64_t seed) {
+    const size_t total = (size_t)rows * cols;
+    for (size_t e = (size_t)blockIdx.x * WG + threadIdx.x; e < total; e += (size_t)gridDim.x * WG) {
+        const int i = (int)(e % rows), j = (int)(e / rows);
+        om[i + (size_t)j * ld] = bf16_bits_rne((float)gauss_entry(i, j, seed));
     }
 }
 
@@ -163,6 +184,96 @@ __global__ __launch_bounds__(WG) void k_sketch(const SketchTask<S>* __restrict__
                 const int c = tl.c0 + wv * 16 + j;
                 if (c < t.s) t.Y[r + (size_t)c * t.k] = acc[j];
             }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// BF16 sketch Y = C Omega for FP32 C: C and Omega rounded to bf16 (round to nearest even) as
+// they are staged in LDS, products accumulated in FP32 on v_mfma_f32_32x32x16_bf16.  One
+// workgroup computes a 64 x TN tile (TN = 64 NC): wave (wm, wn) owns rows 32 wm .. and the NC
+// 32-column blocks from TN/2 wn.  Operand layout of the instruction: lane l holds row
+// (A) / column (B) l % 32 at the 8 consecutive k from 8 (l / 32); accumulator entry q of lane l
+// is row 8 (q / 4) + 4 (l / 32) + q % 4, column l % 32.  Both LDS tiles keep k contiguous
+// (rows of 40 bf16 = 80 bytes: 16-byte aligned, no bank conflicts for 16-byte accesses), so
+// each operand is one 16-byte LDS read.  Omega is bf16 in HBM, column-major with ldo a
+// multiple of 8 and at least w rounded up to 32 (one 16-byte load per thread and group; rows
+// from w on are read and discarded, so their values do not matter); C is read once per TN
+// columns of Y.  Tiles that share rows of C are consecutive in the tile list and run on one
+// XCD (workgroups go to the 8 XCDs round-robin in blockIdx order), where the rows stay in L2.
+// ---------------------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int full = n & ~7;
+    return b >= full ? b : (b & 7) * (full >> 3) + (b >> 3);
+}
+
+template <int NC>
+__global__ __launch_bounds__(WG) void k_sketch_bf16(const SketchTask<float>* __restrict__ tasks,
+                                                    const SketchTile* __restrict__ tiles,
+                                                    const __bf16* __restrict__ om, int ldo) {
+    constexpr int TN = 64 * NC, KP = 40;
+    __shared__ __attribute__((aligned(16))) __bf16 Cs[64 * KP];  // Cs[row][kk]
+    __shared__ __attribute__((aligned(16))) __bf16 Os[TN * KP];  // Os[column][kk]
+    const SketchTile tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const SketchTask<float> t = tasks[tl.task];
+    const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6, wm = wv & 1, wn = wv >> 1;
+    // C staging: thread owns row cr of the tile and the 8 columns 8 cg .. of each 32-column step
+    const int cr = tid & 63, cg = tid >> 6;
+    const float* Crow = t.C + min(tl.r0 + cr, t.k - 1);
+    float cv[8];
+    bf16x8 ov[NC];
+    auto load_step = [&](int w0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) cv[q] = Crow[(size_t)min(w0 + 8 * cg + q, t.w - 1) * t.k];
+#pragma unroll
+        for (int u = 0; u < NC; ++u) {  // Omega group (column oc, rows 8 og ..) of the step
+            const int p = tid + WG * u, og = p & 3, oc = p >> 2;
+            ov[u] = *reinterpret_cast<const bf16x8*>(om + (size_t)min(tl.c0 + oc, t.s - 1) * ldo + w0 + 8 * og);
+        }
+    };
+    floatx16 acc[NC];
+#pragma unroll
+    for (int nb = 0; nb < NC; ++nb)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[nb][q] = 0.f;
+    load_step(0);
+    for (int w0 = 0; w0 < t.w; w0 += 32) {
+        bf16x8 cb;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) cb[q] = (w0 + 8 * cg + q < t.w) ? (__bf16)cv[q] : (__bf16)0.f;
+        *reinterpret_cast<bf16x8*>(&Cs[cr * KP + 8 * cg]) = cb;
+#pragma unroll
+        for (int u = 0; u < NC; ++u) {
+            const int p = tid + WG * u, og = p & 3, oc = p >> 2;
+            bf16x8 v = ov[u];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (w0 + 8 * og + q >= t.w) v[q] = (__bf16)0.f;
+            *reinterpret_cast<bf16x8*>(&Os[oc * KP + 8 * og]) = v;
+        }
+        __syncthreads();
+        if (w0 + 32 < t.w) load_step(w0 + 32);
+#pragma unroll
+        for (int sub = 0; sub < 2; ++sub) {
+            const int kk = 16 * sub + 8 * (l >> 5);
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&Cs[(32 * wm + (l & 31)) * KP + kk]);
+#pragma unroll
+            for (int nb = 0; nb < NC; ++nb) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Os[(wn * (TN / 2) + 32 * nb + (l & 31)) * KP + kk]);
+                acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[nb], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int nb = 0; nb < NC; ++nb) {
+        const int c = tl.c0 + wn * (TN / 2) + 32 * nb + (l & 31);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int r = tl.r0 + 32 * wm + (q >> 2) * 8 + (l >> 5) * 4 + (q & 3);
+            if (r < t.k && c < t.s) t.Y[r + (size_t)c * t.k] = acc[nb][q];
         }
     }
 }
@@ -423,6 +534,20 @@ void launch_colnorm(const ColNormTask<S>* d, int n, hipStream_t s) {
 template <typename S>
 void launch_sketch(const SketchTask<S>* d, const SketchTile* tiles, int ntiles, const S* om, int ldo, hipStream_t s) {
     if (ntiles > 0) hipLaunchKernelGGL(k_sketch<S>, dim3(ntiles), dim3(WG), 0, s, d, tiles, om, ldo);
+}
+void launch_gauss_bf16(void* om, int rows, int cols, int ld, uint64_t seed, hipStream_t s) {
+    const size_t total = (size_t)rows * cols;
+    if (total == 0) return;
+    const int blocks = (int)std::min<size_t>((total + WG - 1) / WG, 4096);
+    hipLaunchKernelGGL(k_gauss_bf16, dim3(blocks), dim3(WG), 0, s, (uint16_t*)om, rows, cols, ld, seed);
+}
+void launch_sketch_bf16(const SketchTask<float>* d, const SketchTile* tiles, int ntiles, const void* om, int ldo,
+                        int tile_cols, hipStream_t s) {
+    if (ntiles <= 0) return;
+    const __bf16* o = (const __bf16*)om;
+    if (tile_cols == 256) hipLaunchKernelGGL(k_sketch_bf16<4>, dim3(ntiles), dim3(WG), 0, s, d, tiles, o, ldo);
+    else if (tile_cols == 128) hipLaunchKernelGGL(k_sketch_bf16<2>, dim3(ntiles), dim3(WG), 0, s, d, tiles, o, ldo);
+    else hipLaunchKernelGGL(k_sketch_bf16<1>, dim3(ntiles), dim3(WG), 0, s, d, tiles, o, ldo);
 }
 template <typename S>
 void launch_rank_features(const FeatTask<S>* d, int n, double tol, hipStream_t s) {

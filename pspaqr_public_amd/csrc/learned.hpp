@@ -53,6 +53,13 @@ template <typename S> void launch_gauss(S* om, int rows, int cols, int ld, uint6
 template <typename S> void launch_colnorm(const ColNormTask<S>* d, int n, hipStream_t s);
 template <typename S>
 void launch_sketch(const SketchTask<S>* d, const SketchTile* tiles, int ntiles, const S* om, int ldo, hipStream_t s);
+// BF16 sketch (FP32 C and Y): Omega as bf16 in HBM (ldo a multiple of 8, at least the largest
+// w rounded up to 32), one 64 x tile_cols tile per workgroup (tile_cols 64, 128 or 256; the
+// tiles' c0 are multiples of tile_cols).  launch_gauss_bf16 writes the FP32 Omega of
+// launch_gauss rounded to bf16.
+void launch_gauss_bf16(void* om, int rows, int cols, int ld, uint64_t seed, hipStream_t s);
+void launch_sketch_bf16(const SketchTask<float>* d, const SketchTile* tiles, int ntiles, const void* om, int ldo,
+                        int tile_cols, hipStream_t s);
 template <typename S> void launch_rank_features(const FeatTask<S>* d, int n, double tol, hipStream_t s);
 template <typename S> void launch_tail_select(const TailTask<S>* d, int n, double tol, hipStream_t s);
 

@@ -444,6 +444,7 @@ Tree::~Tree() {
     if (d_seg) (void)hipFree(d_seg);
     if (d_mlp_w) (void)hipFree(d_mlp_w);
     if (d_omega) (void)hipFree(d_omega);
+    if (d_omega_bf) (void)hipFree(d_omega_bf);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 

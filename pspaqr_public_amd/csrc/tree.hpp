@@ -106,6 +106,8 @@ enum TuneParam {
     T_SKETCH_UNPIVOTED, // sketch width from which the sketch QR runs unpivoted (96)
     T_TPQRT_COOP,       // 1: multi-panel coupling chains on the cooperative kernel, 2: off (1)
     T_SMALL_TASKS,      // 1: blocks of <= 64 x 64 on the one-wave kernels, 2: off (1)
+    T_SKETCH_BF16,      // FP32 fronts: 1 sketches C Omega in bf16 with FP32 accumulation, 2 in FP32 (1)
+    T_SKETCH_TILE,      // output-tile columns of the bf16 sketch: 64, 128, 256 (128)
     T_NPARAM
 };
 
@@ -281,6 +283,13 @@ private:
     bool has_model = false;
     void* d_omega = nullptr;
     int om_rows = 0, om_cols = 0, om_elem = 0;
+    void* d_omega_bf = nullptr;  // the same Gaussian rounded to bf16, ld ob_rows (a multiple of 32)
+    int ob_rows = 0, ob_cols = 0;
+    const void* omega_bf16(int rows, int cols);
+    template <typename S> bool sketch_bf16() const { return sizeof(S) == 4 && tuned(T_SKETCH_BF16, 1) == 1; }
+    // the sketch launch: bf16 when omb is set (FP32 fronts), else the FP32 / FP64 kernel on om
+    template <typename S, typename Task, typename Tile>
+    void run_sketch(const Task* d, const Tile* tiles, int nt, const S* om, const void* omb, int tcols);
     template <typename S>
     double rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTask<S>>& gat,
                     const std::vector<int>& gat_begin, const std::vector<int>& gq_cluster, int L, int* d_ranks,

@@ -57,6 +57,33 @@ const S* Tree::omega(int rows, int cols) {
     return (const S*)d_omega;
 }
 
+const void* Tree::omega_bf16(int rows, int cols) {
+    rows = (std::max(rows, 1) + 31) / 32 * 32;
+    cols = std::max(cols, 1);
+    if (!d_omega_bf || rows > ob_rows || cols > ob_cols) {
+        sync_stream();  // the previous matrix may still be read by queued work
+        if (d_omega_bf) (void)hipFree(d_omega_bf);
+        d_omega_bf = nullptr;
+        const int r2 = std::max(rows, ob_rows), c2 = std::max(cols, ob_cols);
+        HIPCHK(hipMalloc(&d_omega_bf, 2 * (size_t)r2 * c2));
+        launch_gauss_bf16(d_omega_bf, r2, c2, r2, kSketchSeed, stream_);
+        ob_rows = r2;
+        ob_cols = c2;
+    }
+    return d_omega_bf;
+}
+
+template <typename S, typename Task, typename Tile>
+void Tree::run_sketch(const Task* d, const Tile* tiles, int nt, const S* om, const void* omb, int tcols) {
+    if constexpr (sizeof(S) == 4) {
+        if (omb) {
+            launch_sketch_bf16(d, tiles, nt, omb, ob_rows, tcols, stream_);
+            return;
+        }
+    }
+    launch_sketch<S>(d, tiles, nt, om, om_rows, stream_);
+}
+
 template <typename S>
 double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTask<S>>& gat,
                         const std::vector<int>& gat_begin, const std::vector<int>& gq_cluster, int L, int* d_ranks,
@@ -97,7 +124,9 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
         for (int q : elig) prof_add(P_RSKETCH, 2.0 * gq[q].k * gq[q].w, SZ * gq[q].k * gq[q].w);
     }
     if (ne && feats) {
-        const S* om = omega<S>(wmax, kSketchFeatureCols);
+        const bool bf = sketch_bf16<S>();
+        const S* om = bf ? nullptr : omega<S>(wmax, kSketchFeatureCols);
+        const void* omb = bf ? omega_bf16(wmax, kSketchFeatureCols) : nullptr;
         std::vector<SketchTask<S>> sk;
         std::vector<SketchTile> tiles;
         std::vector<GeqrtTask<S>> qr;
@@ -120,7 +149,7 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
         auto* d_qr = upload_vec(qr);
         auto* d_ft = upload_vec(ft);
         timed(P_RSKETCH, 3, ne, [&] {
-            launch_sketch<S>(d_sk, d_tl, (int)tiles.size(), om, om_rows, stream_);
+            run_sketch(d_sk, d_tl, (int)tiles.size(), om, omb, 64);
             launch_geqrt<S>(d_qr, ne, stream_);
             launch_rank_features<S>(d_ft, ne, tol, stream_);
         });
@@ -150,7 +179,10 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
         for (int round = 0; round < rounds && !todo.empty(); ++round) {
             int smax = 1;
             for (int q : todo) smax = std::max(smax, s_of[q]);
-            const S* om = omega<S>(wmax, smax);
+            const bool bf = sketch_bf16<S>();
+            const int tcols = bf ? tuned(T_SKETCH_TILE, 128) : 64;
+            const S* om = bf ? nullptr : omega<S>(wmax, smax);
+            const void* omb = bf ? omega_bf16(wmax, smax) : nullptr;
             if (round > 0) {  // C was overwritten by the previous round's Q^T C
                 std::vector<CopyTask<S>> regather;
                 for (int q : todo)
@@ -171,7 +203,7 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
                 const bool unpiv = unpiv_min > 0 && s >= unpiv_min;
                 S* Y = unpiv ? gq[q].Vout : (S*)arena.alloc(sizeof(S) * (size_t)k * s);
                 for (int r0 = 0; r0 < k; r0 += 64)
-                    for (int c0 = 0; c0 < s; c0 += 64) tiles.push_back({(int)sk.size(), r0, c0, 0});
+                    for (int c0 = 0; c0 < s; c0 += tcols) tiles.push_back({(int)sk.size(), r0, c0, 0});
                 sk.push_back({gq[q].C, Y, k, w, s, 0});
                 if (unpiv) {
                     (std::min(k, s) >= tuned(T_BIG_FRONT, kBigFront) ? uy_big : uy_small).push_back({Y, gq[q].T, k, s, k, 0});
@@ -192,7 +224,7 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
             auto* d_sk = upload_vec(sk);
             auto* d_tl = upload_vec(tiles);
             timed(P_RSKETCH, 1, (long)todo.size(),
-                  [&] { launch_sketch<S>(d_sk, d_tl, (int)tiles.size(), om, om_rows, stream_); });
+                  [&] { run_sketch(d_sk, d_tl, (int)tiles.size(), om, omb, tcols); });
             // rank bounds of the unpivoted QRs (full width); the pivoted QRs write their own
             if (!uy_small.empty() || !uy_big.empty())
                 HIPCHK(hipMemcpyAsync(d_ranks, upload(ry0.data(), sizeof(int) * nt), sizeof(int) * nt,

@@ -61,6 +61,17 @@ def test_rank_path_argument_errors_without_gpu(so):
     s = (_lib.SketchDesc * 1)(_lib.SketchDesc(1, 1, 4, 4, 4, 0))
     assert L.spaqr_sketch_batched(64, 1, s, None, 4, None) == -4   # no Omega
     assert L.spaqr_sketch_batched(64, 1, s, 1, 3, None) == -5      # ldo < w
+    # the bf16 sketch: Omega null or misaligned, ldo not a multiple of 8 or below w rounded up to
+    # 32, tile width not 64 / 128 / 256
+    assert L.spaqr_sketch_bf16_batched(1, s, None, 32, 128, None) == -4
+    assert L.spaqr_sketch_bf16_batched(1, s, 4104, 32, 128, None) == -4
+    assert L.spaqr_sketch_bf16_batched(1, s, 4096, 36, 128, None) == -5
+    assert L.spaqr_sketch_bf16_batched(1, s, 4096, 24, 128, None) == -5
+    w40 = (_lib.SketchDesc * 1)(_lib.SketchDesc(1, 1, 4, 40, 4, 0))
+    assert L.spaqr_sketch_bf16_batched(1, w40, 4096, 56, 128, None) == -5   # 40 rounds up to 64
+    assert L.spaqr_sketch_bf16_batched(1, s, 4096, 32, 96, None) == -6
+    assert L.spaqr_sketch_bf16_batched(0, None, 4096, 32, 128, None) == 0
+    assert L.spaqr_gauss_fill(8, 1, 10, 10, 10, 1, None) == -1     # precision 16, 32 or 64
     assert L.spaqr_tail_select_batched(64, 0, None, -1.0, None) == -4
     assert L.spaqr_rank_features_batched(32, 0, None, 0.1, None) == 0
     assert L.spaqr_gauss_fill(64, None, 10, 10, 10, 1, None) == -2
