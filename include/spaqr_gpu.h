@@ -105,6 +105,13 @@ int spaqr_tree_get_tuning(const spaqr_tree* t, int param);
    entries and zero columns before any GPU work. */
 int spaqr_tree_set_pivot_check(spaqr_tree* t, double rel);
 int spaqr_tree_get_info(spaqr_tree* t, int* out, int cap);
+/* Orthogonalisation of spaqr_gmres (src/is.cpp:447-527 uses modified Gram-Schmidt with the
+   Givens rotations on the host).  0: that scheme, one dot and one axpy launch per basis
+   vector.  1 (default): classical Gram-Schmidt applied twice in fused passes (eight launches
+   per iteration at any basis size) with the Givens rotations, the least-squares solve and the
+   update of x on the device; one residual estimate is read back per iteration.  Both give the
+   same Hessenberg matrix in exact arithmetic and the same stopping test. */
+int spaqr_tree_set_krylov_mode(spaqr_tree* t, int mode);
 /* hipGraph mode (no reference counterpart; it replaces the per-level task flows of
    src/ptree.cpp:885-2123 being re-issued on every factorization).  With on != 0, a
    factorization records every stretch of launches between two host synchronisations (the

@@ -138,6 +138,7 @@ EXPORTS = {
     "spaqr_colnorm_batched": (_i, [_i, _i, _vp, _vp]),
     "spaqr_sketch_batched": (_i, [_i, _i, _vp, _vp, _i, _vp]),
     "spaqr_sketch_bf16_batched": (_i, [_i, _vp, _vp, _i, _i, _vp]),
+    "spaqr_tree_set_krylov_mode": (_i, [_vp, _i]),
     "spaqr_rank_features_batched": (_i, [_i, _i, _vp, _d, _vp]),
     "spaqr_tail_select_batched": (_i, [_i, _i, _vp, _d, _vp]),
     "spaqr_shutdown": (_i, []),

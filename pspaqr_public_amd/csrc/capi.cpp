@@ -208,6 +208,12 @@ int spaqr_tree_set_pivot_check(spaqr_tree* h, double rel) {
     h->t->pivot_rel = rel;
     return 0;
 }
+int spaqr_tree_set_krylov_mode(spaqr_tree* h, int mode) {
+    TREE_OR_FAIL(h);
+    if (mode != 0 && mode != 1) { h->t->last_error = "Krylov mode must be 0 or 1"; return -2; }
+    h->t->krylov_mode = mode;
+    return 0;
+}
 int spaqr_tree_get_info(spaqr_tree* h, int* out, int cap) {
     TREE_OR_FAIL(h);
     try {

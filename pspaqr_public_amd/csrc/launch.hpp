@@ -90,6 +90,19 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
 // array of nseg + 1 ints.  partial needs seg_partials(nseg) doubles.
 int seg_partials(int nseg);
 // out[s * ostride] = <a, b> on system s (its square root if sqrt_mode)
+// One system, fused CGS2 (krylov.hip): v <- v - Q Q^T v twice over Q(:, 0:nvec) (columns of
+// length n); hcol(0:nvec) receives the sum of both passes' coefficients, normpart (mdot_blocks(n)
+// values) the chunk sums of the final v's squares; part holds nvec * mdot_blocks(n) partials,
+// c nvec coefficients.  Then the device Givens step (column k of H, ld ldh, rotations cs / sn,
+// right-hand side beta, est[0] = |beta[k + 1]| / ref_norm), the triangular solve for y and
+// x <- x + V(:, 0:kk) y.
+int mdot_blocks(int n);
+void launch_cgs2(int n, const double* Q, int nvec, double* v, double* part, double* c, double* hcol, double* normpart,
+                 hipStream_t s);
+void launch_givens_step(int k, int nb, const double* normpart, double* hcol, double* H, int ldh, double* cs, double* sn,
+                        double* beta, double ref_norm, double* est, hipStream_t s);
+void launch_lsq_update(int n, int kk, const double* H, int ldh, const double* beta, double* y, const double* V,
+                       double* x, hipStream_t s);
 void launch_seg_dot(int nseg, const int* off, const double* a, const double* b, double* partial, double* out,
                     int ostride, int sqrt_mode, hipStream_t s);
 // y <- y - h[s * hstride] x

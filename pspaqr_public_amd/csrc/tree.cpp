@@ -2507,9 +2507,12 @@ double* Tree::krylov_ws(size_t bytes) {
     return d_krylov;
 }
 
-// dist_gmres (src/is.cpp:447-527) on device vectors: MGS Arnoldi with one dot + one axpy
-// launch per basis vector (scalars stay on the device), Givens rotations on the host from
-// one small readback per iteration.  restart > 0 restarts from the current iterate (an
+// dist_gmres (src/is.cpp:447-527) on device vectors.  Krylov mode 0 is the reference's
+// scheme: MGS Arnoldi with one dot + one axpy launch per basis vector (scalars stay on the
+// device), Givens rotations on the host from one small readback per iteration.  Mode 1 (the
+// default): classical Gram-Schmidt applied twice in fused passes (eight launches per
+// iteration whatever the basis size), Givens rotations, the triangular solve and the update
+// of x on the device; the host reads back one residual estimate per iteration.  restart > 0 restarts from the current iterate (an
 // extension; the reference runs one cycle) until ||b - A x|| / ||b|| <= true_tol.
 int Tree::gmres_dev(const double* b, double* x, int iters, double tol_, int restart, double true_tol,
                     GmresResult* res, double* hist, int hist_cap, int* hist_len, bool flexible) {
@@ -2526,7 +2529,24 @@ int Tree::gmres_dev(const double* b, double* x, int iters, double tol_, int rest
         const int m = (restart > 0) ? std::min(restart, iters) : iters;
         if (m + 2 > 4096) throw std::runtime_error("restart length too large (at most 4094 steps per cycle)");
         const size_t nv = (size_t)(m + 1) + (flexible ? (size_t)m : 0) + 1;  // basis, Z, work vector
-        double* Q = krylov_ws(sizeof(double) * (size_t)n * nv);
+        // fused mode: H, cs, sn, beta, hcol, c, y, est, the multi-dot partials and norm chunks
+        const bool fused = krylov_mode == 1;
+        const int nb = mdot_blocks(n);
+        const size_t ldh = (size_t)m + 1;
+        // H (ldh x m), cs, sn (m), beta (ldh), hcol, c (ldh + 1 each), y (ldh), est (1), partials
+        // (ldh x nb), norm chunks (nb)
+        const size_t small = fused ? ldh * m + 2 * (size_t)m + 4 * ldh + 3 + ldh * nb + nb : 0;
+        double* Q = krylov_ws(sizeof(double) * ((size_t)n * nv + small));
+        double* d_H = Q + (size_t)n * nv;
+        double* d_cs = d_H + ldh * m;
+        double* d_sn = d_cs + m;
+        double* d_beta = d_sn + m;
+        double* d_hcol = d_beta + ldh;
+        double* d_c = d_hcol + ldh + 1;
+        double* d_y = d_c + ldh + 1;
+        double* d_est = d_y + ldh;
+        double* d_part = d_est + 1;
+        double* d_normpart = d_part + ldh * nb;
         double* Z = flexible ? Q + (size_t)n * (m + 1) : nullptr;
         double* w = Q + (size_t)n * (nv - 1);
         double* part = d_scratch;
@@ -2562,6 +2582,7 @@ int Tree::gmres_dev(const double* b, double* x, int iters, double tol_, int rest
             if (r_norm == 0.0) { true_rel = 0.0; break; }
             if (ref_norm < 0) ref_norm = flexible ? bnorm : r_norm;
             launch_scal_dev(n, hdev, Q, Q, stream_);
+            if (fused) HIPCHK(hipMemcpyAsync(d_beta, hdev, sizeof(double), hipMemcpyDeviceToDevice, stream_));
             std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), beta(m + 1, 0.0);
             auto h = [&](int i, int j) -> double& { return H[i + (size_t)j * (m + 1)]; };
             beta[0] = r_norm;
@@ -2578,6 +2599,22 @@ int Tree::gmres_dev(const double* b, double* x, int iters, double tol_, int rest
                 } else {
                     spmv_dev(qk, qn);
                     if (solve_dev(qn) != 0) throw std::runtime_error(last_error);
+                }
+                if (fused) {
+                    // CGS2 in fused passes, the Givens step on the device; one scalar comes back
+                    timed(P_KRYLOV, 8, 1, [&] {
+                        launch_cgs2(n, Q, k + 1, qn, d_part, d_c, d_hcol, d_normpart, stream_);
+                        launch_givens_step(k, nb, d_normpart, d_hcol, d_H, (int)ldh, d_cs, d_sn, d_beta, ref_norm, d_est,
+                                           stream_);
+                        launch_scal_dev(n, d_hcol + k + 1, qn, qn, stream_);
+                    });
+                    prof_add(P_KRYLOV, 8.0 * n * (k + 1) + 3.0 * n, 8.0 * n * (4.0 * (k + 1) + 6.0));
+                    HIPCHK(hipMemcpyAsync(&last_err, d_est, sizeof(double), hipMemcpyDeviceToHost, stream_));
+                    sync_stream();
+                    if (!(last_err == last_err)) throw std::runtime_error("GMRES: NaN residual");
+                    if (hist && nh < hist_cap) hist[nh++] = last_err;
+                    if (last_err <= tol_) break;
+                    continue;
                 }
                 timed(P_KRYLOV, 2L * (k + 1) * 2 + 3, 1, [&] {
                     for (int i = 0; i <= k; ++i) {
@@ -2611,14 +2648,18 @@ int Tree::gmres_dev(const double* b, double* x, int iters, double tol_, int rest
                 if (last_err <= tol_) break;
             }
             k = std::min(k, m - 1);
-            std::vector<double> y(k + 1, 0.0);
-            for (int i = k; i >= 0; --i) {
-                y[i] = beta[i];
-                for (int j = i + 1; j <= k; ++j) y[i] -= h(i, j) * y[j];
-                y[i] /= h(i, i);
-            }
             const double* basis = flexible ? Z : Q;
-            for (int j = 0; j <= k; ++j) launch_axpby(n, y[j], basis + (size_t)j * n, 1.0, x, stream_);
+            if (fused) {
+                launch_lsq_update(n, k + 1, d_H, (int)ldh, d_beta, d_y, basis, x, stream_);
+            } else {
+                std::vector<double> y(k + 1, 0.0);
+                for (int i = k; i >= 0; --i) {
+                    y[i] = beta[i];
+                    for (int j = i + 1; j <= k; ++j) y[i] -= h(i, j) * y[j];
+                    y[i] /= h(i, i);
+                }
+                for (int j = 0; j <= k; ++j) launch_axpby(n, y[j], basis + (size_t)j * n, 1.0, x, stream_);
+            }
             total += k + 1;
             first = false;
             // true residual

@@ -188,6 +188,7 @@ public:
     int get_info(std::vector<int>& out);
     // Planner tunables (TuneParam, spaqr_tree_set_tuning); 0 keeps the built-in value.
     int tune[16] = {0};
+    int krylov_mode = 1;  // 0: MGS + host Givens (dist_gmres's scheme), 1: fused CGS2 + device Givens
     // hipGraph mode: the first factorization records every stretch of launches between two host
     // synchronisations (the rank read-backs) as a graph and launches it; a later factorization
     // of the same assembled matrix with the same settings replays the graphs without planning,

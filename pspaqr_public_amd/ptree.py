@@ -96,6 +96,11 @@ class ParTree:
         for k, v in params.items():
             check(lib().spaqr_tree_set_tuning(self._h, self.TUNABLES.index(k), int(v)), self._h, "set_tuning " + k)
 
+    def set_krylov_mode(self, mode: int):
+        """GMRES orthogonalisation: 0 MGS with host Givens (the reference's), 1 fused CGS2 with
+        device Givens (default)."""
+        check(lib().spaqr_tree_set_krylov_mode(self._h, int(mode)), self._h, "set_krylov_mode")
+
     def set_pivot_check(self, rel: float):
         check(lib().spaqr_tree_set_pivot_check(self._h, float(rel)), self._h, "set_pivot_check")
 
