@@ -214,9 +214,15 @@ def main():
     if not args.skip_fp64:
         T64, _, b64, x64, _ = prepare(64)
         run_steps(T64, b64, x64, T64.n, 1, 0, args.iters, "fp64 warmup", False, batch)
+        lib().spaqr_tree_set_profiling(T64.handle, 1)
+        lib().spaqr_tree_profile_reset(T64.handle)
         s64, f64, res64 = run_steps(T64, b64, x64, T64.n, 1, 0, args.iters, "fp64", False, batch)
+        p64 = profile(T64)
         fp64 = dict(seconds=s64, factor_s=f64, gmres_iters=max(q.iters for q in res64),
-                    worst_true_relres=max(q.true_res for q in res64))
+                    worst_true_relres=max(q.true_res for q in res64),
+                    per_class={c: dict(ms_per_step=round(v["ms"], 4), launches=v["launches"],
+                                       gflop=round(v["flops"] * 1e-9, 4), gbytes=round(v["bytes"] * 1e-9, 4))
+                               for c, v in p64.items() if v["launches"]})
         T64.close()
         if args.graphs:
             # the same FP64 path with every level's launches replayed from recorded hip graphs

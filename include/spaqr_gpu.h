@@ -57,6 +57,19 @@ int spaqr_tree_set_order(spaqr_tree* t, float order);
 int spaqr_tree_set_verbose(spaqr_tree* t, int verbose);
 /* Tree::set_Xcoo: coordinates X (d x n, column-major)                    src/tree.cpp:20 */
 int spaqr_tree_set_Xcoo(spaqr_tree* t, const double* X, int d, int n);
+/* Orderings without coordinates (the reference's METIS path, src/ptree.cpp:503-518 ->
+   MetisPartition, src/partition.cpp:354-424).  With no coordinates, partition() orders by
+   leaf parts: one per column in [0, 2^(levels-1)), numbered as a recursive bisection numbers
+   them (part k of depth d splits into 2k and 2k + 1), e.g. the output of
+   METIS_PartGraphRecursive on A^T A with 2^(levels-1) parts.  set_parts gives them (n = 0
+   clears); without them the library bisects the graph of A^T A itself (level structures from
+   a pseudo-peripheral vertex, component by component; not METIS's partition, so not its
+   ordering).  Separators and interfaces then follow the METIS path's rules (src/partition.cpp:
+   380-424, getInterfacesUsingATA :637-682).  Coordinates, when set, take precedence (as in the
+   reference).  get_parts copies up to cap leaf parts of the last partition (natural order, every
+   path) and returns their number. */
+int spaqr_tree_set_parts(spaqr_tree* t, const int* parts, int n);
+int spaqr_tree_get_parts(const spaqr_tree* t, int* out, int cap);
 /* Factor precision: 64 (reference FP64 path) or 32 (FP32 fronts; GMRES stays FP64).
    No counterpart in the reference.  Set before spaqr_tree_assemble. */
 int spaqr_tree_set_precision(spaqr_tree* t, int bits);
@@ -355,6 +368,26 @@ int spaqr_rank_features_batched(int prec, int count, const spaqr_rank_features_d
    may share it.  Sums in FP64. */
 typedef struct { const void* B; const void* cmax2; const int* ry; int* rank; int k, w; } spaqr_tail_desc;
 int spaqr_tail_select_batched(int prec, int count, const spaqr_tail_desc* d, double tol, void* stream);
+
+/* Matrix Market files (include/mmio.hpp:174-377 of the reference: sp_mmread, dense_mmread /
+   vector_mmread, sp_mmwrite, dense_mmwrite; the driver reads A and the coordinates with them,
+   pspaQR.cpp:85,161-167).  Host only, no GPU.
+   read_sparse: a coordinate file of type real, integer or pattern (entries 1), property general,
+   symmetric / hermitian (off-diagonal entries mirrored) or skew-symmetric (mirrored, negated),
+   into CSC with rows sorted in each column and duplicates summed.  Call it first with colptr,
+   rowind and val NULL for the sizes (nnz counts the mirrored entries), then with arrays of
+   ncols + 1, nnz and nnz entries.  read_dense: an array file (general) into column-major data
+   (a vector is an n x 1 array); data NULL for the sizes.  write_sparse: a real coordinate file
+   (symmetric != 0 writes the lower triangle with property symmetric); write_dense: a real array
+   file.  Values are written with 17 significant digits.  Return 0, -1 the file cannot be opened,
+   -2 unsupported or malformed header, -3 malformed line or index out of range, -4 entry count
+   differs from the header. */
+int spaqr_mm_read_sparse(const char* path, int* nrows, int* ncols, long long* nnz, int* colptr, int* rowind,
+                         double* val);
+int spaqr_mm_read_dense(const char* path, int* nrows, int* ncols, double* data);
+int spaqr_mm_write_sparse(const char* path, int nrows, int ncols, const int* colptr, const int* rowind,
+                          const double* val, int symmetric);
+int spaqr_mm_write_dense(const char* path, int nrows, int ncols, const double* data);
 
 /* Joins the library's host worker threads.  Call it after the last tree is destroyed and
    before the process exits (the Python binding does so at interpreter exit); a later call

@@ -41,6 +41,67 @@ inline void check(int rc, const spaqr_tree* t, const char* what) {
         throw std::runtime_error(std::string(what) + " failed: " + (m ? m : ""));
     }
 }
+}  // namespace detail
+
+/* Compressed-column and column-major dense matrices with the accessors the ParTree templates
+   read (the subset of Eigen::SparseMatrix / Eigen::MatrixXd they use). */
+struct SparseMatrix {
+    int nr = 0, nc = 0;
+    std::vector<int> colptr, rowind;
+    std::vector<double> val;
+    int rows() const { return nr; }
+    int cols() const { return nc; }
+    const int* outerIndexPtr() const { return colptr.data(); }
+    const int* innerIndexPtr() const { return rowind.data(); }
+    const double* valuePtr() const { return val.data(); }
+};
+struct DenseMatrix {
+    int nr = 0, nc = 0;
+    std::vector<double> x;
+    int rows() const { return nr; }
+    int cols() const { return nc; }
+    const double* data() const { return x.data(); }
+};
+
+/* include/mmio.hpp:174-377: Matrix Market files through spaqr_mm_* (the driver reads A with
+   sp_mmread and the coordinates with dense_mmread, pspaQR.cpp:85,162) */
+namespace mmio {
+inline SparseMatrix sp_mmread(const std::string& path) {
+    SparseMatrix A;
+    long long nnz = 0;
+    detail::check(spaqr_mm_read_sparse(path.c_str(), &A.nr, &A.nc, &nnz, nullptr, nullptr, nullptr), nullptr,
+                  "sp_mmread");
+    A.colptr.resize((size_t)A.nc + 1);
+    A.rowind.resize((size_t)nnz);
+    A.val.resize((size_t)nnz);
+    detail::check(spaqr_mm_read_sparse(path.c_str(), &A.nr, &A.nc, &nnz, A.colptr.data(), A.rowind.data(),
+                                       A.val.data()),
+                  nullptr, "sp_mmread");
+    return A;
+}
+inline DenseMatrix dense_mmread(const std::string& path) {
+    DenseMatrix X;
+    detail::check(spaqr_mm_read_dense(path.c_str(), &X.nr, &X.nc, nullptr), nullptr, "dense_mmread");
+    X.x.resize((size_t)X.nr * X.nc);
+    detail::check(spaqr_mm_read_dense(path.c_str(), &X.nr, &X.nc, X.x.data()), nullptr, "dense_mmread");
+    return X;
+}
+inline std::vector<double> vector_mmread(const std::string& path) {
+    DenseMatrix X = dense_mmread(path);
+    if (X.nc != 1) throw std::runtime_error("vector_mmread: " + path + " is not an n x 1 array");
+    return X.x;
+}
+inline void sp_mmwrite(const std::string& path, const SparseMatrix& A, bool symmetric = false) {
+    detail::check(spaqr_mm_write_sparse(path.c_str(), A.nr, A.nc, A.colptr.data(), A.rowind.data(), A.val.data(),
+                                        symmetric ? 1 : 0),
+                  nullptr, "sp_mmwrite");
+}
+inline void dense_mmwrite(const std::string& path, const DenseMatrix& X) {
+    detail::check(spaqr_mm_write_dense(path.c_str(), X.nr, X.nc, X.x.data()), nullptr, "dense_mmwrite");
+}
+}  // namespace mmio
+
+namespace detail {
 
 /* a device vector of FP64 owned by the caller (spaqr_device_alloc / free) */
 class DeviceVec {
@@ -115,6 +176,12 @@ class ParTree {
     void set_ttor_log(int) {}
     template <class Mat> void set_Xcoo(Mat* X) {
         detail::check(spaqr_tree_set_Xcoo(t_, X->data(), (int)X->rows(), (int)X->cols()), t_, "set_Xcoo");
+    }
+    /* leaf parts in place of coordinates (the METIS path, src/ptree.cpp:503-518): e.g. the
+       output of METIS_PartGraphRecursive on A^T A with 2^(nlevels-1) parts; with neither
+       coordinates nor parts partition() bisects the graph itself */
+    void set_parts(const std::vector<int>& parts) {
+        detail::check(spaqr_tree_set_parts(t_, parts.data(), (int)parts.size()), t_, "set_parts");
     }
     /* FP32 fronts (no counterpart in the reference): call before assemble */
     void set_precision(int bits) { detail::check(spaqr_tree_set_precision(t_, bits), t_, "set_precision"); }

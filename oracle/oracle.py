@@ -33,6 +33,8 @@ def lib():
         L = C.CDLL(_LIB)
         L.orc_tree_create.restype = C.c_void_p
         L.orc_tree_create.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]
+        L.orc_tree_create_parts.restype = C.c_void_p
+        L.orc_tree_create_parts.argtypes = [C.c_int, _ip, _ip, _dp, _ip, C.c_int, C.c_double, C.c_int]
         for f in ("orc_partition_assemble", "orc_factorize", "orc_num_clusters", "orc_trace_len"):
             getattr(L, f).argtypes = [C.c_void_p]
             getattr(L, f).restype = C.c_int
@@ -96,7 +98,10 @@ class OracleTree:
     """The reference's ParTree driver sequence on the CPU: partition, assemble, factorize,
     solve, spmv and dist_gmres (pspaQR.cpp:183-268)."""
 
-    def __init__(self, A, n_side: int, dim: int, nlevels: int, tol: float, skip: int = 0):
+    def __init__(self, A, n_side: int, dim: int, nlevels: int, tol: float, skip: int = 0, parts=None):
+        """parts: leaf parts (one per column) ordering the tree as the reference's METIS path
+        does (MetisPartition with these parts, src/partition.cpp:372-424); n_side and dim are
+        then ignored."""
         A = A.tocsc()
         A.sort_indices()
         self.n = A.shape[0]
@@ -104,7 +109,12 @@ class OracleTree:
         self._rowind = np.ascontiguousarray(A.indices, dtype=np.int32)
         self._val = np.ascontiguousarray(A.data, dtype=np.float64)
         L = lib()
-        self._h = L.orc_tree_create(self.n, self._colptr, self._rowind, self._val, n_side, dim, nlevels, tol, skip)
+        if parts is None:
+            self._h = L.orc_tree_create(self.n, self._colptr, self._rowind, self._val, n_side, dim, nlevels, tol, skip)
+        else:
+            self._parts = np.ascontiguousarray(parts, dtype=np.int32)
+            self._h = L.orc_tree_create_parts(self.n, self._colptr, self._rowind, self._val, self._parts, nlevels, tol,
+                                              skip)
         self.nlevels = nlevels
         if L.orc_partition_assemble(self._h) != 0:
             raise RuntimeError("oracle partition/assemble failed")

@@ -316,6 +316,56 @@ static std::vector<ClusterID> geometric_partition(const CSC& A, int nlevels, con
     return rmap;
 }
 
+// MetisPartition (src/partition.cpp:372-424) with the leaf parts given in place of
+// partition_metis_RB's: separators on A^T A by the same rule as the geometric path, then
+// getInterfacesUsingATA (src/partition.cpp:637-682) -- left starts at the vertex's own leaf,
+// right at the first lower-level neighbour in the right half, both widened by
+// find_highest_common.
+static std::vector<ClusterID> parts_partition(const CSC& A, int nlevels, const std::vector<int>& parts) {
+    const int c = A.nc;
+    CSC G = ata_structure(A);
+    std::vector<ClusterID> cmap(c);
+    for (int depth = 0; depth < nlevels - 1; ++depth) {
+        const int l = nlevels - depth - 1;
+        for (int i = 0; i < c; ++i) {
+            if (cmap[i].self.lvl != -1) continue;
+            const int pi = part_at_lvl(parts[i], l);
+            if (pi % 2 != 0) continue;
+            bool sep = false;
+            for (int k = G.colptr[i]; k < G.colptr[i + 1]; ++k) {
+                const int j = G.rowind[k];
+                if (cmap[j].self.lvl == -1 && part_at_lvl(parts[j], l) == pi + 1) { sep = true; break; }
+            }
+            if (sep) { cmap[i].self = SepID(l, pi / 2); cmap[i].section = parts[i]; }
+        }
+    }
+    for (int i = 0; i < c; ++i)
+        if (cmap[i].self.lvl == -1) {
+            cmap[i].self = cmap[i].l = cmap[i].r = SepID(0, parts[i]);
+            cmap[i].section = parts[i];
+        }
+    for (int i = 0; i < c; ++i) {
+        const SepID self = cmap[i].self;
+        if (self.lvl <= 0) continue;
+        SepID left(0, parts[i]), right(-1, 0);
+        const int pi = part_at_lvl(parts[i], self.lvl);
+        for (int k = G.colptr[i]; k < G.colptr[i + 1]; ++k) {
+            const int j = G.rowind[k];
+            const SepID nbr = cmap[j].self;
+            if (nbr.lvl < self.lvl) {
+                const int pj = part_at_lvl(parts[j], self.lvl);
+                if (pj <= pi) left = find_highest_common(nbr, left);
+                else right = (right.lvl == -1) ? nbr : find_highest_common(nbr, right);
+            }
+        }
+        if (right.lvl == -1) throw std::runtime_error("separator without a right neighbour");
+        cmap[i].l = left;
+        cmap[i].r = right;
+        cmap[i].section = parts[i];
+    }
+    return cmap;
+}
+
 // initPermutation (src/tree.cpp:48-57).
 static std::vector<int> init_permutation(int nlevels, const std::vector<ClusterID>& cmap) {
     std::vector<int> cperm(cmap.size());
@@ -434,6 +484,7 @@ struct Tree {
     CSC A, App;
     std::vector<double> X;
     int d = 0;
+    std::vector<int> given_parts;  // leaf parts in place of coordinates (the METIS path)
     std::vector<int> cperm;
     std::vector<ClusterID> cpermed;
     std::vector<std::vector<Cluster*>> bottoms, interiors, interfaces;
@@ -480,7 +531,8 @@ struct Tree {
 
     // ---------------------------------------------------------- partition
     void partition() {  // src/ptree.cpp:495-694
-        std::vector<ClusterID> cmap = geometric_partition(A, nlevels, X, d);
+        std::vector<ClusterID> cmap =
+            given_parts.empty() ? geometric_partition(A, nlevels, X, d) : parts_partition(A, nlevels, given_parts);
         cperm = init_permutation(nlevels, cmap);
         cpermed.resize(n);
         for (int k = 0; k < n; ++k) cpermed[k] = cmap[cperm[k]];
@@ -1049,6 +1101,16 @@ void* orc_tree_create(int n, const int* colptr, const int* rowind, const double*
     t->A.val.assign(val, val + colptr[n]);
     t->X = linspace_nd(cn, cd);
     t->d = cd;
+    return t;
+}
+
+// The same tree ordered by given leaf parts (the reference's METIS path) instead of coordinates.
+void* orc_tree_create_parts(int n, const int* colptr, const int* rowind, const double* val, const int* parts,
+                            int nlevels, double tol, int skip) {
+    Tree* t = (Tree*)orc_tree_create(n, colptr, rowind, val, 1, 1, nlevels, tol, skip);
+    t->X.clear();
+    t->d = 0;
+    t->given_parts.assign(parts, parts + n);
     return t;
 }
 

@@ -139,6 +139,12 @@ EXPORTS = {
     "spaqr_sketch_batched": (_i, [_i, _i, _vp, _vp, _i, _vp]),
     "spaqr_sketch_bf16_batched": (_i, [_i, _vp, _vp, _i, _i, _vp]),
     "spaqr_tree_set_krylov_mode": (_i, [_vp, _i]),
+    "spaqr_tree_set_parts": (_i, [_vp, _vp, _i]),
+    "spaqr_tree_get_parts": (_i, [_vp, _vp, _i]),
+    "spaqr_mm_read_sparse": (_i, [C.c_char_p, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "spaqr_mm_read_dense": (_i, [C.c_char_p, _vp, _vp, _vp]),
+    "spaqr_mm_write_sparse": (_i, [C.c_char_p, _i, _i, _vp, _vp, _vp, _i]),
+    "spaqr_mm_write_dense": (_i, [C.c_char_p, _i, _i, _vp]),
     "spaqr_rank_features_batched": (_i, [_i, _i, _vp, _d, _vp]),
     "spaqr_tail_select_batched": (_i, [_i, _i, _vp, _d, _vp]),
     "spaqr_shutdown": (_i, []),

@@ -152,6 +152,18 @@ int spaqr_tree_set_Xcoo(spaqr_tree* h, const double* X, int d, int n) {
     h->t->xdim = d;
     return 0;
 }
+int spaqr_tree_set_parts(spaqr_tree* h, const int* parts, int n) {
+    TREE_OR_FAIL(h);
+    if (n < 0 || (n > 0 && !parts)) return -2;
+    h->t->user_parts.assign(parts, parts + n);
+    return 0;
+}
+int spaqr_tree_get_parts(const spaqr_tree* h, int* out, int cap) {
+    if (!h || !h->t) return -1;
+    const auto& p = h->t->leaf_parts;
+    if (out) std::copy(p.begin(), p.begin() + std::min<size_t>(p.size(), (size_t)std::max(cap, 0)), out);
+    return (int)p.size();
+}
 int spaqr_tree_partition(spaqr_tree* h, int nrows, int ncols, const int* colptr, const int* rowind, const double* val) {
     TREE_OR_FAIL(h);
     if (nrows <= 0 || ncols <= 0 || !colptr || !rowind || !val) return -2;

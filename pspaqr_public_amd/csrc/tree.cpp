@@ -411,6 +411,18 @@ void Tree::ensure_stream() {
     if (!stream_) HIPCHK(hipStreamCreate(&stream_));
 }
 
+// Caller-given leaf parts of one system (positions off .. off + n): one per column, each in
+// [0, 2^(nlevels-1)), the numbering of a recursive bisection (part k of depth d splits into 2k
+// and 2k + 1; METIS_PartGraphRecursive with 2^(nlevels-1) parts numbers them so).
+static void check_parts(const std::vector<int>& parts, int off, int n, int nlevels) {
+    if ((long long)parts.size() < (long long)off + n) throw std::runtime_error("one leaf part per column");
+    const int np = 1 << std::max(0, nlevels - 1);
+    for (int i = off; i < off + n; ++i)
+        if (parts[i] < 0 || parts[i] >= np)
+            throw std::runtime_error("leaf part " + std::to_string(parts[i]) + " of column " + std::to_string(i - off) +
+                                     " is outside [0, 2^(levels-1))");
+}
+
 // The matrix a tree accepts (the reference asserts these: src/is.cpp:460 on a NaN residual,
 // src/ptree.cpp:171-175 exits on an empty interior): finite values, no empty column.
 static void check_matrix(const HostCSC& A) {
@@ -538,17 +550,74 @@ int Tree::new_edge(int c1, int c2) {  // new_edgeOut, src/ptree.cpp:101-120
 }
 
 // ---------------------------------------------------------------------------- partition
-// Ordering of one square system (A n x n, coordinates X d x n): its column permutation and the
-// cluster ID of every permuted position.
-static void order_system(const HostCSC& A, const double* Xc, int xdim, int nlevels, std::vector<int>& cperm_,
-                         std::vector<ClusterID>& cpermed) {
+// Leaf parts of a graph without coordinates, in place of METIS_PartGraphRecursive
+// (src/partition.cpp:354-370): recursive bisection of the graph G (A^T A) into 2^(nlevels-1)
+// parts, part k of depth d splitting into 2k and 2k + 1 (the numbering the separator rules read).
+// Each part's induced subgraph is ordered by level structures: component by component (in the
+// order of their smallest vertex), a breadth-first search from the vertex the first search
+// reaches last (a pseudo-peripheral vertex); the first ceil(N/2) vertices of that order form
+// part 2k.  Neighbours are visited in increasing order, so the result is deterministic.
+static void graph_parts(const HostCSC& G, int nlevels, std::vector<int>& parts) {
+    const int n = G.nc;
+    parts.assign(n, 0);
+    std::vector<int> mark(n, 0), done(n, 0), order, comp, bfs, next(n, 0);
+    int stamp = 0;
+    auto search = [&](int src, int k, std::vector<int>& out) {
+        out.clear();
+        ++stamp;
+        mark[src] = stamp;
+        out.push_back(src);
+        for (size_t h = 0; h < out.size(); ++h) {
+            const int v = out[h];
+            for (int q = G.colptr[v]; q < G.colptr[v + 1]; ++q) {
+                const int u = G.rowind[q];
+                if (parts[u] == k && mark[u] != stamp) {
+                    mark[u] = stamp;
+                    out.push_back(u);
+                }
+            }
+        }
+    };
+    for (int depth = 0; depth < nlevels - 1; ++depth) {
+        const int np = 1 << depth;
+        std::vector<std::vector<int>> dofs(np);
+        for (int i = 0; i < n; ++i) dofs[parts[i]].push_back(i);
+        for (int k = 0; k < np; ++k) {
+            const auto& d = dofs[k];
+            const int N = (int)d.size();
+            if (N == 0) continue;
+            order.clear();
+            for (int s : d) {
+                if (done[s]) continue;
+                search(s, k, comp);           // the component of s
+                search(comp.back(), k, bfs);  // level structure from its far end
+                for (int v : bfs) done[v] = 1;
+                order.insert(order.end(), bfs.begin(), bfs.end());
+            }
+            for (int v : d) done[v] = 0;
+            const int left = N - N / 2;
+            for (int q = 0; q < N; ++q) next[order[q]] = (q < left) ? 2 * k : 2 * k + 1;
+        }
+        parts.swap(next);  // the searches above read this depth's parts only
+    }
+}
+
+// Ordering of one square system (A n x n): its column permutation and the cluster ID of every
+// permuted position.  Leaf parts come from the coordinates X (d x n, geometric bisection), else
+// from the caller (gparts), else from graph_parts; separators follow the same rule on every path;
+// interfaces are inferred from the rows of A on the geometric path (getInterfacesUsingA) and
+// from A^T A on the others (the METIS path: getInterfacesUsingATA, src/partition.cpp:637-682).
+static void order_system(const HostCSC& A, const double* Xc, int xdim, const int* gparts, int nlevels,
+                         std::vector<int>& cperm_, std::vector<ClusterID>& cpermed, std::vector<int>* parts_out) {
     const int n = A.nc;
     auto X = [&](size_t idx) { return Xc[idx]; };
-    // recursive geometric bisection (src/partition.cpp:241-304)
     HostCSC G = ata_pattern(A);
     std::vector<int> parts(n, 0);
     std::vector<double> vals;
-    for (int depth = 0; depth < nlevels - 1; ++depth) {
+    if (!Xc && gparts) parts.assign(gparts, gparts + n);
+    else if (!Xc) graph_parts(G, nlevels, parts);
+    // recursive geometric bisection (src/partition.cpp:241-304)
+    for (int depth = 0; Xc && depth < nlevels - 1; ++depth) {
         const int np = 1 << depth;
         std::vector<std::vector<int>> dofs(np);
         for (int i = 0; i < n; ++i) dofs[parts[i]].push_back(i);
@@ -592,6 +661,29 @@ static void order_system(const HostCSC& A, const double* Xc, int xdim, int nleve
             cmap[i].self = cmap[i].l = cmap[i].r = SepID(0, parts[i]);
             cmap[i].section = parts[i];
         }
+    if (parts_out) *parts_out = parts;
+    if (!Xc) {
+        // interfaces from A^T A (src/partition.cpp:637-682): left starts at the vertex's own leaf
+        // part, right at the first lower-level neighbour in the right half
+        std::vector<ClusterID> out = cmap;
+        for (int i = 0; i < n; ++i) {
+            const SepID self = cmap[i].self;
+            if (self.lvl <= 0) continue;
+            SepID left(0, parts[i]), right(-1, 0);
+            const int pi = parts[i] >> (self.lvl - 1);
+            for (int k = G.colptr[i]; k < G.colptr[i + 1]; ++k) {
+                const int j = G.rowind[k];
+                const SepID nb = cmap[j].self;
+                if (nb.lvl >= self.lvl) continue;
+                if ((parts[j] >> (self.lvl - 1)) <= pi) left = highest_common(nb, left);
+                else right = (right.lvl == -1) ? nb : highest_common(nb, right);
+            }
+            if (right.lvl == -1) throw std::runtime_error("a separator vertex has no neighbour on its right");
+            out[i].l = left;
+            out[i].r = right;
+        }
+        cmap = out;
+    } else {
     // interfaces from the rows of A (src/partition.cpp:514-587), rmap == cmap (square)
     HostCSC At = csc_transpose(A);
     std::vector<ClusterID> rmap = cmap;
@@ -633,6 +725,7 @@ static void order_system(const HostCSC& A, const double* Xc, int xdim, int nleve
         if (rmap[i].r == SepID()) rmap[i].r = rmap[i].self;
     }
     cmap = rmap;
+    }
     // initPermutation: stable sorts by the successively merged IDs
     cperm_.resize(n);
     for (int i = 0; i < n; ++i) cperm_[i] = i;
@@ -716,17 +809,19 @@ void Tree::make_hierarchy() {
 int Tree::partition(const HostCSC& A) {
     try {
         if (A.nr != A.nc) throw std::runtime_error("only square systems are supported");
-        if (X.empty() || xdim <= 0) throw std::runtime_error("geometric partition needs coordinates (set_Xcoo)");
-        if ((long long)X.size() != (long long)xdim * A.nc) throw std::runtime_error("coordinate array size mismatch");
+        const bool geo = !X.empty() && xdim > 0;
+        if (geo && (long long)X.size() != (long long)xdim * A.nc) throw std::runtime_error("coordinate array size mismatch");
         for (double v : X)
             if (!std::isfinite(v)) throw std::runtime_error("coordinates must be finite");
+        if (!geo && !user_parts.empty()) check_parts(user_parts, 0, A.nc, nlevels);
         check_matrix(A);
         A0 = A;
         n = A.nc;
         inst_n.assign(1, n);
         inst_lv.assign(1, nlevels);
         inst_off.assign(1, 0);
-        order_system(A, X.data(), xdim, nlevels, cperm_, cpermed);
+        order_system(A, geo ? X.data() : nullptr, xdim, (!geo && !user_parts.empty()) ? user_parts.data() : nullptr,
+                     nlevels, cperm_, cpermed, &leaf_parts);
         make_hierarchy();
         return 0;
     } catch (std::exception& e) {
@@ -748,9 +843,14 @@ int Tree::partition_batch(const HostCSC& A, const std::vector<int>& sizes, const
         if (total != A.nc) throw std::runtime_error("the system sizes must add up to the matrix size");
         check_matrix(A);
         if (lmax != nlevels) throw std::runtime_error("the tree's level count must equal the largest system's");
-        if (X.empty() || xdim <= 0) throw std::runtime_error("geometric partition needs coordinates (set_Xcoo)");
+        const bool geo = !X.empty() && xdim > 0;
         n = A.nc;
-        if ((long long)X.size() != (long long)xdim * n) throw std::runtime_error("coordinate array size mismatch");
+        if (geo && (long long)X.size() != (long long)xdim * n) throw std::runtime_error("coordinate array size mismatch");
+        const bool given = !geo && !user_parts.empty();
+        if (given) {
+            if ((int)user_parts.size() != n) throw std::runtime_error("one leaf part per column of the batch");
+            for (int s = 0, off = 0; s < (int)sizes.size(); off += sizes[s], ++s) check_parts(user_parts, off, sizes[s], levels[s]);
+        }
         inst_n = sizes;
         inst_lv = levels;
         inst_off.assign(sizes.size() + 1, 0);
@@ -763,6 +863,7 @@ int Tree::partition_batch(const HostCSC& A, const std::vector<int>& sizes, const
         A0 = A;
         cperm_.assign(n, 0);
         cpermed.assign(n, ClusterID());
+        leaf_parts.assign(n, 0);
         parallel_ranges((int)sizes.size(), 1, [&](int b0, int e0, int) {
             for (int s = b0; s < e0; ++s) {
                 const int off = inst_off[s], ns = sizes[s];
@@ -776,12 +877,14 @@ int Tree::partition_batch(const HostCSC& A, const std::vector<int>& sizes, const
                     }
                     As.colptr[j + 1] = (int)As.rowind.size();
                 }
-                std::vector<int> cp;
+                std::vector<int> cp, pt;
                 std::vector<ClusterID> ids;
-                order_system(As, X.data() + (size_t)off * xdim, xdim, levels[s], cp, ids);
+                order_system(As, geo ? X.data() + (size_t)off * xdim : nullptr, xdim,
+                             given ? user_parts.data() + off : nullptr, levels[s], cp, ids, &pt);
                 for (int k = 0; k < ns; ++k) {
                     cperm_[off + k] = off + cp[k];
                     cpermed[off + k] = ids[k];
+                    leaf_parts[off + k] = pt[k];
                 }
             }
         });

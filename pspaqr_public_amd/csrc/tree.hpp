@@ -131,6 +131,8 @@ public:
     int prec = 64;  // 64: FP64 factors; 32: FP32 factors (GMRES stays FP64)
     std::vector<double> X;  // coordinates, d x n
     int xdim = 0;
+    std::vector<int> user_parts;  // leaf parts given in place of coordinates (the METIS path)
+    std::vector<int> leaf_parts;  // the leaf parts of the last partition, natural order
 
     int partition(const HostCSC& A);
     // A batch of independent square systems factored together: A is their block-diagonal

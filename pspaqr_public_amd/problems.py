@@ -16,6 +16,8 @@ from __future__ import annotations
 import math
 from dataclasses import dataclass, field
 
+import os
+
 import numpy as np
 import scipy.sparse as sp
 
@@ -182,6 +184,35 @@ class Problem:
     tol: float
     skip: int = 0
     meta: dict = field(default_factory=dict)
+    X: np.ndarray | None = None      # coordinates (d x n) read from a file, in place of the n^d grid
+    parts: np.ndarray | None = None  # leaf parts (METIS-style), when there are no coordinates
+
+
+def load_problem(matrix: str, nlevels: int, tol: float, coordinates: str | None = None, n: int = -1, d: int = -1,
+                 skip: int = 0, rhs: str | None = None, seed: int = 0) -> Problem:
+    """The reference driver's input path (pspaQR.cpp:85-167): A from a Matrix Market file, scaled
+    A <- A D 10; coordinates from an array file (d x N), or the n^d tensor grid when n and d are
+    given (they take precedence, as -n/-d overwrite --coordinates), or none -- the tree then orders
+    by graph bisection.  b from an array file, else uniform(-1, 1) from std::mt19937(seed)."""
+    from . import mmio
+
+    A = mmio.read_sparse(matrix)
+    if A.shape[0] != A.shape[1]:
+        raise ValueError("only square systems are supported (A is %d x %d)" % A.shape)
+    N = A.shape[1]
+    X = None
+    if n > 0 and d > 0:
+        if n ** d != N:
+            raise ValueError("n^d = %d but A is %d x %d" % (n ** d, N, N))
+    else:
+        n, d = 0, 0
+        if coordinates:
+            X = mmio.read_dense(coordinates)
+            if X.shape[1] != N:
+                raise ValueError("the coordinate file must hold a d x N matrix (it is %d x %d)" % X.shape)
+    b = mmio.read_vector(rhs) if rhs else mt_uniform(seed, N)
+    name = os.path.splitext(os.path.basename(str(matrix)))[0]
+    return Problem(name, column_scale(A), b, n, d, nlevels, tol, skip, X=X)
 
 
 @dataclass

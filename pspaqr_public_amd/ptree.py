@@ -76,6 +76,23 @@ class ParTree:
         self._X = X
         check(lib().spaqr_tree_set_Xcoo(self._h, X.ctypes.data, X.shape[0], X.shape[1]), self._h, "set_Xcoo")
 
+    def set_parts(self, parts):
+        """Leaf parts in place of coordinates (spaqr_tree_set_parts); None clears them."""
+        if parts is None:
+            check(lib().spaqr_tree_set_parts(self._h, None, 0), self._h, "set_parts")
+            return
+        p = np.ascontiguousarray(parts, dtype=np.int32)
+        self._parts = p
+        check(lib().spaqr_tree_set_parts(self._h, p.ctypes.data, int(p.size)), self._h, "set_parts")
+
+    def parts(self) -> np.ndarray:
+        """Leaf part of every column (natural order) used by the last partition."""
+        n = lib().spaqr_tree_get_parts(self._h, None, 0)
+        out = np.zeros(max(n, 0), dtype=np.int32)
+        if n > 0:
+            lib().spaqr_tree_get_parts(self._h, out.ctypes.data, n)
+        return out
+
     # ---- partition / assemble / factorize
     def partition(self, A):
         A = A.tocsc()
@@ -300,7 +317,12 @@ def build_tree(problem, precision: int = 64, nlevels=None, tol=None, skip=None) 
     t.set_order(1)
     t.set_square(1)
     t.set_precision(precision)
-    t.set_Xcoo(linspace_nd(problem.n_side, problem.dim))
+    if getattr(problem, "X", None) is not None:
+        t.set_Xcoo(problem.X)
+    elif problem.n_side > 0 and problem.dim > 0:
+        t.set_Xcoo(linspace_nd(problem.n_side, problem.dim))
+    elif getattr(problem, "parts", None) is not None:
+        t.set_parts(problem.parts)
     t.partition(problem.A)
     t.assemble(problem.A)
     return t

@@ -1,14 +1,20 @@
 // A C++ caller of the MI355X path written against the reference's API shape: ParTree,
-// dist_gmres and the Operation list (include/spaqr_ptree.hpp over include/spaqr_gpu.h), with
-// no Python in the process.  It builds the 2D 5-point Poisson problem on a 64 x 64 grid with the
-// driver's preprocessing (A <- A D 10, pspaQR.cpp:132-147; RHS std::mt19937(0) +
-// uniform_real_distribution(-1, 1)), then partition -> assemble -> factorize -> dist_gmres, and
-// checks: true relative residual <= 1e-12; the Operation replay equals solve(x) bit for bit;
-// solve(b, x) in the natural ordering equals the permuted in-place solve.  Prints one line of
-// results and exits non-zero on any failed check.
+// dist_gmres, the Operation list and mmio (include/spaqr_ptree.hpp over include/spaqr_gpu.h),
+// with no Python in the process.  It builds the 2D 5-point Poisson problem on a 64 x 64 grid with
+// the driver's preprocessing (A <- A D 10, pspaQR.cpp:132-147; RHS std::mt19937(0) +
+// uniform_real_distribution(-1, 1)), passes A and the coordinates through Matrix Market files
+// (sp_mmwrite / sp_mmread, dense_mmwrite / dense_mmread: the driver's input path), then
+// partition -> assemble -> factorize -> dist_gmres, and checks: the files give back the same
+// matrix and coordinates; true relative residual <= 1e-12; the Operation replay equals solve(x)
+// bit for bit; solve(b, x) in the natural ordering equals the permuted in-place solve.  Prints
+// one line of results and exits non-zero on any failed check.
+#include <unistd.h>
+
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "spaqr_ptree.hpp"
@@ -74,6 +80,30 @@ int main() {
 
     int failed = 0;
     try {
+        // through Matrix Market files, as the driver reads its inputs
+        char dir[] = "/tmp/spaqr_consumer_XXXXXX";
+        if (!mkdtemp(dir)) throw std::runtime_error("mkdtemp failed");
+        const std::string fa = std::string(dir) + "/A.mtx", fx = std::string(dir) + "/X.mtx";
+        {
+            spaqr::SparseMatrix S;
+            S.nr = S.nc = A.n;
+            S.colptr = A.colptr;
+            S.rowind = A.rowind;
+            S.val = A.val;
+            spaqr::DenseMatrix D;
+            D.nr = X.d;
+            D.nc = X.n;
+            D.x = X.x;
+            spaqr::mmio::sp_mmwrite(fa, S);
+            spaqr::mmio::dense_mmwrite(fx, D);
+            const spaqr::SparseMatrix S2 = spaqr::mmio::sp_mmread(fa);
+            const spaqr::DenseMatrix D2 = spaqr::mmio::dense_mmread(fx);
+            if (S2.nr != S.nr || S2.colptr != S.colptr || S2.rowind != S.rowind || S2.val != S.val || D2.x != D.x)
+                failed |= 32;
+            std::remove(fa.c_str());
+            std::remove(fx.c_str());
+            rmdir(dir);
+        }
         spaqr::ParTree t(6, 0);
         t.set_tol(1e-2);
         t.set_scale(1);
