@@ -153,12 +153,23 @@ def test_flexible_gmres(cfg, side):
     assert np.linalg.norm(b - Te.spmv(x)) / np.linalg.norm(b) < 1e-13
 
 
-def test_refactorize_is_repeatable():
-    P = make_problem(2, 12)
-    T = build_tree(P, 64)
+@pytest.mark.parametrize("cfg,side,prec,mode", [(2, 12, 64, 0), (3, 16, 32, 1), (5, 16, 32, 1)])
+def test_refactorize_is_repeatable(cfg, side, prec, mode):
+    """Later factorizations restore level 0's fill-in recorded by the first one (and a new
+    assemble() records it again): every result bit for bit the first's."""
+    P = make_problem(cfg, side)
+    T = build_tree(P, prec)
+    if mode:
+        from pspaqr_public_amd import rank_model
+        T.set_rank_mode(mode)
+        T.set_rank_model(rank_model.load_weights())
     T.factorize()
     b = P.b[T.cperm()]
     x1 = T.solve(b)
+    c1 = T.clusters()
+    for rep in range(2):
+        T.factorize()
+        assert np.array_equal(T.solve(b), x1) and np.array_equal(T.clusters(), c1)
+    T.assemble()
     T.factorize()
-    x2 = T.solve(b)
-    assert np.array_equal(x1, x2)
+    assert np.array_equal(T.solve(b), x1)
