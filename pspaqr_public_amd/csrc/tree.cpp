@@ -1158,9 +1158,8 @@ double Tree::factor_flops() const {
 template <typename S>
 int Tree::factorize_t() {
     auto t0 = std::chrono::steady_clock::now();
-    const bool have_fill0 = fill0_gen == asm_gen;
-    cl = have_fill0 ? cl_fill0 : cl0;
-    ed = have_fill0 ? ed_fill0 : ed0;
+    cl = cl0;
+    ed = ed0;
     current_bottom = 0;
     sync_stream();  // earlier work on the retained chunks has finished before they are reused
     arena.reset();
@@ -1173,7 +1172,7 @@ int Tree::factorize_t() {
     {
         S* base = (S*)arena.alloc(std::max<size_t>(h_blocks0.size(), 1) * sizeof(S));
         HIPCHK(hipMemcpyAsync(base, d_blocks0, h_blocks0.size() * sizeof(S), hipMemcpyDeviceToDevice, stream_));
-        for (size_t i = 0; i < ed0.size(); ++i) ed[i].A = base + h_block_off[i];
+        for (size_t i = 0; i < ed.size(); ++i) ed[i].A = base + h_block_off[i];
     }
     d_chain_err = (int*)arena.alloc(sizeof(int));
     launch_fill<int>(d_chain_err, 1, 0, stream_);
@@ -1363,14 +1362,8 @@ int Tree::factorize_t() {
         current_bottom = L;
         LevelStats st;
         st.level = L;
-        const bool restored = L == 0 && have_fill0;
-        const size_t ne_before = restored ? ed0.size() : ed.size();
-        if (!restored) fillin();
-        if (L == 0 && !have_fill0) {
-            cl_fill0 = cl;
-            ed_fill0 = ed;
-            fill0_gen = asm_gen;
-        }
+        const size_t ne_before = ed.size();
+        fillin();
         hacc(0, ht);
         {
             size_t tot = 0;

@@ -222,11 +222,6 @@ private:
     std::vector<Cluster> cl_part;             // the hierarchy as partition() left it (assemble() starts there)
     std::vector<Cluster> cl0;                 // post-assembly structure, restored by factorize()
     std::vector<Edge> ed0;
-    // the same after level 0's fill-in, which depends on the structure only: recorded by the
-    // first factorization after an assemble() and restored by the later ones
-    std::vector<Cluster> cl_fill0;
-    std::vector<Edge> ed_fill0;
-    long fill0_gen = -1;
     void* d_blocks0 = nullptr;                // pristine assembled blocks (factor precision)
     std::vector<ClusterID> cpermed;
     std::vector<Cluster> cl;                  // indexed by order

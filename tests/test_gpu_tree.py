@@ -155,8 +155,8 @@ def test_flexible_gmres(cfg, side):
 
 @pytest.mark.parametrize("cfg,side,prec,mode", [(2, 12, 64, 0), (3, 16, 32, 1), (5, 16, 32, 1)])
 def test_refactorize_is_repeatable(cfg, side, prec, mode):
-    """Later factorizations restore level 0's fill-in recorded by the first one (and a new
-    assemble() records it again): every result bit for bit the first's."""
+    """Factorizing again, and again after a new assemble(), gives the first factorization's
+    results bit for bit (host planning, level-0 blocks and arenas all start afresh)."""
     P = make_problem(cfg, side)
     T = build_tree(P, prec)
     if mode:
