@@ -1,6 +1,7 @@
 """Autotuner of the level planner's tile, chunk and panel-path widths (spaqr_tree_set_tuning).
 
-For one workload it times the factorization (the part these widths change) on the GPU and
+For one workload it times the benchmark's step on the GPU -- factorize plus GMRES to a true
+relative residual 1e-12, since the sketch widths can change ranks and so iteration counts -- and
 runs a coordinate descent over the candidate values of every tunable: each parameter in turn
 takes the value with the lowest median time while the others stay at their current best; two
 passes.  The result is stored in pspaqr_public_amd/tuning.json under the workload's key,
@@ -21,6 +22,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
@@ -37,9 +40,12 @@ CANDIDATES = {
     "tpqrt_coop": [1, 2],
     "small_tasks": [1, 2],
     "sketch_tile": [64, 128, 256],
+    "sketch_grow": [100, 105, 110, 115, 125, 150],
+    "sketch_pad": [1, 5, 9, 17],
 }
 BUILT_IN = {"big_front": 96, "tpmqrt_tile": 64, "larfb_cols": 128, "spars_cols": 128, "rank_cols": 128,
-            "sketch_unpivoted": 96, "tpqrt_coop": 1, "small_tasks": 1, "sketch_tile": 128}
+            "sketch_unpivoted": 96, "tpqrt_coop": 1, "small_tasks": 1, "sketch_tile": 128, "sketch_grow": 125,
+            "sketch_pad": 9}
 
 
 def key(workload: str, prec: int) -> str:
@@ -63,32 +69,49 @@ def apply(tree, workload: str, prec: int, path: str = TABLE) -> dict:
 
 
 def make_workload(workload: str, prec: int):
-    """(tree, label) with the matrix assembled and the benchmark's rank mode set."""
+    """(tree, step) with the matrix assembled and the benchmark's rank mode set; step() runs the
+    benchmark's step on it: factorize + GMRES to a true relative residual 1e-12 (flexible for
+    FP32 fronts, restarts of 50), b and x resident on the device."""
+    from ._lib import DeviceBuffer
     from .problems import make_batch, make_problem
     from .ptree import build_batch_tree, build_tree
     from .rank_model import load_weights
 
-    if workload == "batch48":
-        B = make_batch()
-        t = build_batch_tree(B, prec)
+    batch = workload == "batch48"
+    if batch:
+        W = make_batch()
+        t = build_batch_tree(W, prec)
     else:
         cfg = {"convdiff3d_64": 3, "diffusion3d_32": 2, "poisson2d_64": 1, "anisotropic3d_96": 5}[workload]
-        t = build_tree(make_problem(cfg), prec)
+        W = make_problem(cfg)
+        t = build_tree(W, prec)
     if prec == 32:
         t.set_rank_mode(1)
         t.set_rank_model(load_weights())
-    return t
+    bd = DeviceBuffer.from_array(np.ascontiguousarray(W.b[t.cperm()], dtype=np.float64))
+    xd = DeviceBuffer(8 * t.n)
+    flexible = prec == 32
+
+    def step():
+        t.factorize()
+        if batch:
+            t.gmres_batch_device(bd.ptr, xd.ptr, iters=300, tol=1e-12, restart=50, true_tol=1e-12, flexible=flexible)
+        else:
+            t.gmres_device(bd.ptr, xd.ptr, iters=300, tol=1e-12, restart=50, true_tol=1e-12, flexible=flexible)
+
+    t._tune_buffers = (bd, xd)
+    return t, step
 
 
-def _time(tree, reps: int) -> float:
+def _time(tree, step, reps: int) -> float:
     from ._lib import lib
 
-    tree.factorize()  # warm-up with this setting
+    step()  # warm-up with this setting
     lib().spaqr_tree_sync(tree.handle)
     ts = []
     for _ in range(reps):
         t0 = time.perf_counter()
-        tree.factorize()
+        step()
         lib().spaqr_tree_sync(tree.handle)
         ts.append(time.perf_counter() - t0)
     ts.sort()
@@ -96,10 +119,10 @@ def _time(tree, reps: int) -> float:
 
 
 def tune(workload: str, prec: int, reps: int = 3, passes: int = 2, log=print) -> dict:
-    t = make_workload(workload, prec)
+    t, step = make_workload(workload, prec)
     params = {k: 0 for k in CANDIDATES}  # 0: built-in value
     t.set_tuning(params)
-    base = _time(t, reps)
+    base = _time(t, step, reps)
     log("[autotune] %s built-in: %.2f ms" % (key(workload, prec), base))
     best = base
     sweep = []
@@ -112,7 +135,7 @@ def tune(workload: str, prec: int, reps: int = 3, passes: int = 2, log=print) ->
                 trial = dict(params)
                 trial[name] = v
                 t.set_tuning(trial)
-                ms = _time(t, reps)
+                ms = _time(t, step, reps)
                 sweep.append({"pass": p, "param": name, "value": v, "ms": round(ms, 3)})
                 log("[autotune] pass %d %-16s = %-5d %.2f ms (best %.2f)" % (p, name, v, ms, best))
                 if ms < best * 0.995:  # a real gain, not noise
@@ -120,7 +143,7 @@ def tune(workload: str, prec: int, reps: int = 3, passes: int = 2, log=print) ->
                     params[name] = v
             t.set_tuning(params)
     t.set_tuning(params)
-    final = _time(t, reps)
+    final = _time(t, step, reps)
     chosen = {k: v for k, v in params.items() if v}
     log("[autotune] %s chosen %s: %.2f ms (built-in %.2f ms)" % (key(workload, prec), chosen, final, base))
     return {"params": chosen, "ms": round(final, 3), "built_in_ms": round(base, 3), "reps": reps, "sweep": sweep}

@@ -108,6 +108,8 @@ enum TuneParam {
     T_SMALL_TASKS,      // 1: blocks of <= 64 x 64 on the one-wave kernels, 2: off (1)
     T_SKETCH_BF16,      // FP32 fronts: 1 sketches C Omega in bf16 with FP32 accumulation, 2 in FP32 (1)
     T_SKETCH_TILE,      // output-tile columns of the bf16 sketch: 64, 128, 256 (128)
+    T_SKETCH_GROW,      // learned sketch width = grow% x predicted rank + pad: 100 .. 200 (125)
+    T_SKETCH_PAD,       // the pad + 1 (0 restores the built-in 8): 1 .. 65
     T_NPARAM
 };
 

@@ -161,12 +161,14 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
             float* d_frac = (float*)arena.alloc(sizeof(float) * ne);
             timed(P_RSKETCH, 1, ne, [&] { launch_mlp_predict(d_X, ne, d_mlp_w, d_frac, stream_); });
             std::vector<float> frac(ne);
+            const double grow = tuned(T_SKETCH_GROW, (int)std::lround(100 * kSketchGrow)) / 100.0;
+            const int pad = tuned(T_SKETCH_PAD, kSketchPad + 1) - 1;  // stored + 1: 0 means built-in
             HIPCHK(hipMemcpyAsync(frac.data(), d_frac, sizeof(float) * ne, hipMemcpyDeviceToHost, stream_));
             sync_stream();
             for (int i = 0; i < ne; ++i) {
                 const int q = elig[i], kw = std::min(gq[q].k, gq[q].w);
                 const double f = std::isfinite(frac[i]) ? std::min(1.0f, std::max(0.0f, frac[i])) : 1.0;
-                s_of[q] = std::max(1, std::min(kw, (int)std::ceil(kSketchGrow * f * kw) + kSketchPad));
+                s_of[q] = std::max(1, std::min(kw, (int)std::ceil(grow * f * kw) + pad));
             }
         } else {
             for (int q : elig) s_of[q] = std::min(gq[q].k, gq[q].w);

@@ -106,7 +106,8 @@ class ParTree:
                                          va.ctypes.data), self._h, "partition")
 
     TUNABLES = ("big_front", "tpmqrt_tile", "larfb_cols", "trsm_rows", "spars_rows", "spars_cols", "rank_cols",
-                "sketch_unpivoted", "tpqrt_coop", "small_tasks", "sketch_bf16", "sketch_tile")
+                "sketch_unpivoted", "tpqrt_coop", "small_tasks", "sketch_bf16", "sketch_tile", "sketch_grow",
+                "sketch_pad")
 
     def set_tuning(self, params: dict):
         """Planner tunables by name (spaqr_tree_set_tuning); 0 restores the built-in value."""
