@@ -2451,6 +2451,14 @@ void Tree::solve_sched_levels(int upto) {
                     for (auto& t : ch[q][c]) { ++cnt[c]; kmax[c] = std::max(kmax[c], t.k); }
             fprintf(stderr, "[solve L%d] %d colours; chains (count/max k): wave %d/%d pipe1 %d/%d pipe2 %d/%d plain %d/%d\n",
                     L, ncol, cnt[0], kmax[0], cnt[1], kmax[1], cnt[2], kmax[2], cnt[3], kmax[3]);
+            for (int q = 0; q < ncol; ++q) {  // per colour: the longest chain in items (links x panels)
+                int items = 0, nl = 0, np = 0, rm = 0;
+                for (auto& t : ch[q][3]) {
+                    const int p = (t.k + NB - 1) / NB;
+                    if (t.nlinks * p > items) { items = t.nlinks * p; nl = t.nlinks; np = p; rm = t.rmax; }
+                }
+                fprintf(stderr, "[solve L%d c%d] longest chain: %d links x %d panels, rmax %d\n", L, q, nl, np, rm);
+            }
         }
         lv.chain_links = upload_vec(links);
         for (int k = 0; k < ncol; ++k) {

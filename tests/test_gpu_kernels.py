@@ -322,3 +322,41 @@ def test_geqp3_hbm_slices_match_oracle():
     assert np.array_equal(b["jp"].download(np.int32, w)[:r], jp[:r])
     assert rel(unF(b["V"].download(np.float64, k * r), k, r), Cf[:, :r]) < 1e-11
     assert rel(b["tau"].download(np.float64, r), tau[:r]) < 1e-11
+
+
+def test_full_T_from_the_blocked_geqrt():
+    """The reference's geqrt_cluster asks dgeqrt for nb = k, one k x k T (src/ptree.cpp:187);
+    spaqr_geqrt_batched returns the NB = 32 blocked T.  A caller that needs the k x k T builds it
+    with larft's recurrence over all k reflectors (tau_j = T's diagonal), and that T satisfies
+    I - V T V^T = H_0 H_1 ... H_{k-1}, the product of the reflectors -- checked here on the GPU's
+    V and blocked T for a square block and a tall one."""
+    rng = np.random.default_rng(5)
+    for (m, n) in [(96, 96), (150, 70)]:
+        A = rng.standard_normal((m, n))
+        Ab = dbuf(F(A))
+        Tb = G.DeviceBuffer(NB * n * 8)
+        d = (G.GeqrtDesc * 1)(G.GeqrtDesc(Ab.ptr, Tb.ptr, m, n, m, 0))
+        assert G.lib().spaqr_geqrt_batched(64, 1, d, None) == 0
+        VR = unF(Ab.download(np.float64, m * n), m, n)
+        Tblk = unF(Tb.download(np.float64, NB * n), NB, n)
+        k = min(m, n)
+        V = np.tril(VR[:, :k], -1) + np.eye(m, k)
+        tau = np.array([Tblk[j % NB, j] for j in range(k)])
+        # larft (forward, columnwise): T(0:j, j) = -tau_j T(0:j, 0:j) V(:, 0:j)^T v_j
+        Tf = np.zeros((k, k))
+        for j in range(k):
+            Tf[j, j] = tau[j]
+            if j:
+                Tf[:j, j] = -tau[j] * (Tf[:j, :j] @ (V[:, :j].T @ V[:, j]))
+        Q = np.eye(m)
+        for j in range(k):
+            v = V[:, j:j + 1]
+            Q = Q @ (np.eye(m) - tau[j] * (v @ v.T))
+        assert np.linalg.norm(np.eye(m) - V @ Tf @ V.T - Q) < 1e-12 * m
+        # and each 32 x 32 diagonal block of the full T is the blocked T's panel block
+        for p0 in range(0, k, NB):
+            nb = min(NB, k - p0)
+            assert rel(np.triu(Tf[p0:p0 + nb, p0:p0 + nb]), np.triu(Tblk[:nb, p0:p0 + nb])) < 1e-12
+        # Q^T A = R
+        R = np.triu(VR[:k, :n])
+        assert np.linalg.norm(Q.T @ A - np.vstack([R, np.zeros((m - k, n))])) < 1e-11 * np.linalg.norm(A)
