@@ -57,6 +57,8 @@ namespace orc {
 //   4 the sparsify apply skips the left transform Q_c^T (src/ptree.cpp:1881-1889)
 //   5 the sparsified level range also takes the second level from the top
 //     (src/ptree.cpp:30, 1566, 1635 stop at nlevels - 2)
+//   6 getInterfacesUsingATA counts a neighbour in the vertex's own half as right of it
+//     (src/partition.cpp:659 tests pj <= pi)
 static int g_fault = 0;
 
 // f(i) for i in [0, n) on the OpenMP threads.  Tasks are independent (each writes only its own
@@ -354,7 +356,7 @@ static std::vector<ClusterID> parts_partition(const CSC& A, int nlevels, const s
             const SepID nbr = cmap[j].self;
             if (nbr.lvl < self.lvl) {
                 const int pj = part_at_lvl(parts[j], self.lvl);
-                if (pj <= pi) left = find_highest_common(nbr, left);
+                if (g_fault == 6 ? pj < pi : pj <= pi) left = find_highest_common(nbr, left);
                 else right = (right.lvl == -1) ? nbr : find_highest_common(nbr, right);
             }
         }

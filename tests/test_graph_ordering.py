@@ -137,3 +137,36 @@ def test_bad_parts_and_precedence():
     t.partition(P.A)
     O = OracleTree(P.A, P.n_side, P.dim, P.nlevels, P.tol, 0)
     assert np.array_equal(t.cperm(), O.cperm())
+
+
+def test_metis_path_closed_form_on_a_path_graph():
+    """The METIS path's rules (src/partition.cpp:380-424, 637-682) by hand on a path graph:
+    A tridiagonal (32 vertices), so A^T A joins vertices up to 2 apart; 3 levels, leaf parts
+    0..3 of 8 consecutive vertices each.  Level-2 separator: the vertices of parts {0, 1}
+    within distance 2 of parts {2, 3}: 14, 15, with left (0, 1) (their own leaf) and right
+    (0, 2).  Level-1 separators: 6, 7 of part 0 next to part 1 (left (0, 0), right (0, 1)) and
+    22, 23 of part 2 next to part 3 (left (0, 2), right (0, 3)).  Every other vertex is its
+    leaf, with l = r = self.  Both the library and the oracle must give exactly this."""
+    n = 32
+    A = sp.diags([-np.ones(n - 1), 4 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1], format="csc")
+    parts = np.repeat(np.arange(4), 8).astype(np.int32)
+    want = np.zeros((n, 6), dtype=np.int64)
+    for i in range(n):
+        want[i] = (0, parts[i], 0, parts[i], 0, parts[i])
+    for i in (14, 15):
+        want[i] = (2, 0, 0, 1, 0, 2)
+    for i in (6, 7):
+        want[i] = (1, 0, 0, 0, 0, 1)
+    for i in (22, 23):
+        want[i] = (1, 1, 0, 2, 0, 3)
+    t = _tree(3, parts=parts)
+    t.partition(A)
+    assert np.array_equal(_selves(t), want)
+    O = OracleTree(A, 0, 0, 3, 1e-2, 0, parts=parts)
+    assert np.array_equal(O.cperm(), t.cperm())
+    assert np.array_equal(O.clusters()[:, :12], t.clusters()[:, :12])
+    # and the pin catches a planted mistake in the oracle's interface rule
+    from oracle.oracle import planted_fault
+    with planted_fault(6):
+        Of = OracleTree(A, 0, 0, 3, 1e-2, 0, parts=parts)
+        assert not np.array_equal(Of.clusters()[:, :12], t.clusters()[:, :12])
