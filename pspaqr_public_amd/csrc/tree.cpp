@@ -611,7 +611,16 @@ static void order_system(const HostCSC& A, const double* Xc, int xdim, const int
                          std::vector<int>& cperm_, std::vector<ClusterID>& cpermed, std::vector<int>* parts_out) {
     const int n = A.nc;
     auto X = [&](size_t idx) { return Xc[idx]; };
+    static const bool dbg = getenv("SPAQR_DEBUG_HOST") != nullptr;
+    auto tp = std::chrono::steady_clock::now();
+    auto lap = [&](const char* what) {
+        if (!dbg) return;
+        const auto nw = std::chrono::steady_clock::now();
+        fprintf(stderr, "[partition] %s %.1f ms\n", what, std::chrono::duration<double, std::milli>(nw - tp).count());
+        tp = nw;
+    };
     HostCSC G = ata_pattern(A);
+    lap("A^T A pattern");
     std::vector<int> parts(n, 0);
     std::vector<double> vals;
     if (!Xc && gparts) parts.assign(gparts, gparts + n);
@@ -639,6 +648,7 @@ static void order_system(const HostCSC& A, const double* Xc, int xdim, const int
             for (int i : d) parts[i] = (X(best + (size_t)i * xdim) <= mid) ? 2 * k : 2 * k + 1;
         }
     }
+    lap("bisection");
     // separators: left-part vertices with an AtA neighbour in the right part (:306-333)
     std::vector<ClusterID> cmap(n);
     for (int depth = 0; depth < nlevels - 1; ++depth) {
@@ -661,6 +671,7 @@ static void order_system(const HostCSC& A, const double* Xc, int xdim, const int
             cmap[i].self = cmap[i].l = cmap[i].r = SepID(0, parts[i]);
             cmap[i].section = parts[i];
         }
+    lap("separators");
     if (parts_out) *parts_out = parts;
     if (!Xc) {
         // interfaces from A^T A (src/partition.cpp:637-682): left starts at the vertex's own leaf
@@ -726,18 +737,42 @@ static void order_system(const HostCSC& A, const double* Xc, int xdim, const int
     }
     cmap = rmap;
     }
-    // initPermutation: stable sorts by the successively merged IDs
+    lap("interfaces");
+    // initPermutation (src/tree.cpp:48-57): stable sorts by the successively merged IDs.  The
+    // first is a comparison sort; after it, a vertex's merged ID at every level is a function of
+    // its level-0 ID (merge_if reads the ID only), so each later pass is a stable counting sort
+    // by the rank of that value among the few distinct level-0 IDs' merged values: the same
+    // permutation, O(n) per level.
     cperm_.resize(n);
     for (int i = 0; i < n; ++i) cperm_[i] = i;
-    std::vector<ClusterID> pm = cmap;
-    auto cmp = [&pm](int a, int b) { return pm[a] < pm[b]; };
-    std::stable_sort(cperm_.begin(), cperm_.end(), cmp);
+    std::stable_sort(cperm_.begin(), cperm_.end(), [&](int a, int b) { return cmap[a] < cmap[b]; });
+    std::vector<int> d0(n);
+    std::vector<ClusterID> u;  // distinct level-0 IDs, increasing
+    for (int k = 0; k < n; ++k) {
+        const ClusterID& c = cmap[cperm_[k]];
+        if (u.empty() || u.back() < c) u.push_back(c);
+        d0[cperm_[k]] = (int)u.size() - 1;
+    }
+    const int nq = (int)u.size();
+    std::vector<int> qord(nq), qrank(nq), cnt, nxt(n);
     for (int lvl = 1; lvl < nlevels; ++lvl) {
-        for (auto& s : pm) s = merge_if(s, lvl);
-        std::stable_sort(cperm_.begin(), cperm_.end(), cmp);
+        for (auto& c : u) c = merge_if(c, lvl);
+        for (int q = 0; q < nq; ++q) qord[q] = q;
+        std::stable_sort(qord.begin(), qord.end(), [&](int a, int b) { return u[a] < u[b]; });
+        int r = 0;
+        for (int j = 0; j < nq; ++j) {
+            if (j > 0 && u[qord[j - 1]] < u[qord[j]]) ++r;
+            qrank[qord[j]] = r;
+        }
+        cnt.assign((size_t)r + 2, 0);
+        for (int k = 0; k < n; ++k) cnt[qrank[d0[cperm_[k]]] + 1]++;
+        for (int j = 0; j <= r; ++j) cnt[j + 1] += cnt[j];
+        for (int k = 0; k < n; ++k) nxt[cnt[qrank[d0[cperm_[k]]]]++] = cperm_[k];
+        cperm_.swap(nxt);
     }
     cpermed.resize(n);
     for (int k = 0; k < n; ++k) cpermed[k] = cmap[cperm_[k]];
+    lap("permutation");
 }
 
 // The cluster hierarchy from the ordering (src/ptree.cpp:587-691): level-0 clusters are the runs
