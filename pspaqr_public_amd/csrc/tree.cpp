@@ -1026,7 +1026,35 @@ int Tree::assemble() {
 // ---------------------------------------------------------------------------- fill-in
 // get_sparsity + alloc_fillin (src/ptree.cpp:866-871, 140-153); returns nothing, new
 // edges are appended to ed (their device blocks are allocated by the caller).
-void Tree::fillin() {
+// Fill-in, first half: each interior's distance-2 list gains the sources of its in-edges.
+// Returns whether the level's interiors are independent: no interior has another interior of
+// this level in its distance-2 list.  Then fillin_edges changes no interior's out list (it may
+// add in-edges to interiors, blocks that are zero until the trailing update), so the
+// elimination's panel work can be queued before the new edges are made.
+bool Tree::fillin_dist2() {
+    const auto& ints = interiors[current_bottom];
+    parallel_ranges((int)ints.size(), 64, [&](int b, int e, int) {
+        for (int i = b; i < e; ++i) {
+            Cluster& C = cl[ints[i]];
+            std::vector<int>& d2 = C.dist2;
+            for (int e2 : C.in) d2.push_back(ed[e2].n1);
+            std::sort(d2.begin(), d2.end());
+            d2.erase(std::unique(d2.begin(), d2.end()), d2.end());
+        }
+    });
+    std::vector<char> is_int(cl.size(), 0);
+    for (int c : ints) is_int[c] = 1;
+    for (int c : ints)
+        for (int nn : cl[c].dist2)
+            if (nn != c && is_int[nn]) return false;
+    return true;
+}
+
+// Fill-in, second half: the new blocks.  Independent interiors: the new blocks are fixed by the
+// edges as they stand: candidates (nn, nbr) built per interior in parallel, the first occurrence
+// of each kept (in interior, neighbour, out-edge order), then the edges made in that order.
+// Otherwise the interiors run one after another.
+void Tree::fillin_edges(bool independent) {
     static const bool dbg = getenv("SPAQR_DEBUG_FILLIN") != nullptr;
     auto tf = std::chrono::steady_clock::now();
     double tm[6] = {0};
@@ -1038,27 +1066,7 @@ void Tree::fillin() {
     };
     const auto& ints = interiors[current_bottom];
     const int nint = (int)ints.size();
-    parallel_ranges(nint, 64, [&](int b, int e, int) {
-        for (int i = b; i < e; ++i) {
-            Cluster& C = cl[ints[i]];
-            std::vector<int>& d2 = C.dist2;
-            for (int e2 : C.in) d2.push_back(ed[e2].n1);
-            std::sort(d2.begin(), d2.end());
-            d2.erase(std::unique(d2.begin(), d2.end()), d2.end());
-        }
-    });
-    // When no interior has another interior of this level in its distance-2 list, the
-    // interiors' out lists do not change here, so the new blocks are fixed by the edges as
-    // they stand: candidates (nn, nbr) built per interior in parallel, the first occurrence of
-    // each kept (in interior, neighbour, out-edge order), then the edges made in that order.
-    // Otherwise the interiors run one after another.
     tick();
-    std::vector<char> is_int(cl.size(), 0);
-    for (int c : ints) is_int[c] = 1;
-    bool independent = true;
-    for (int c : ints)
-        for (int nn : cl[c].dist2)
-            if (nn != c && is_int[nn]) independent = false;
     if (!independent) {
         // edge existence through a stamp array: mark[n2] == stamp when nn already has an edge to n2
         std::vector<int> mark(cl.size(), -1);
@@ -1179,7 +1187,7 @@ void Tree::fillin() {
     });
     tick();
     if (dbg)
-        fprintf(stderr, "[fillin L%d] %zu candidates, %zu edges: dist2 %.2f cand %.2f dedup %.2f made %.2f lists %.2f ms\n",
+        fprintf(stderr, "[fillin L%d] %zu candidates, %zu edges: setup %.2f cand %.2f dedup %.2f made %.2f lists %.2f ms\n",
                 current_bottom, ncand, nnew, tm[0], tm[1], tm[2], tm[3], tm[4]);
 }
 
@@ -1193,8 +1201,15 @@ double Tree::factor_flops() const {
 template <typename S>
 int Tree::factorize_t() {
     auto t0 = std::chrono::steady_clock::now();
-    cl = cl0;
-    ed = ed0;
+    // the pristine structure; element-wise on the host threads when the cluster count is unchanged
+    // (every cluster's lists keep their buffers)
+    if (cl.size() == cl0.size())
+        parallel_ranges((int)cl.size(), 512, [&](int b, int e, int) {
+            for (int i = b; i < e; ++i) cl[i] = cl0[i];
+        });
+    else
+        cl = cl0;
+    ed.assign(ed0.begin(), ed0.end());
     current_bottom = 0;
     sync_stream();  // earlier work on the retained chunks has finished before they are reused
     arena.reset();
@@ -1397,10 +1412,11 @@ int Tree::factorize_t() {
         current_bottom = L;
         LevelStats st;
         st.level = L;
-        const size_t ne_before = ed.size();
-        fillin();
-        hacc(0, ht);
-        {
+        // fill-in blocks (src/ptree.cpp:885-1010), zeroed on the stream.  With independent
+        // interiors they are made after the elimination's panel work is queued, while it runs.
+        auto fill_in = [&](bool independent) {
+            const size_t ne_before = ed.size();
+            fillin_edges(independent);
             size_t tot = 0;
             for (size_t i = ne_before; i < ed.size(); ++i) tot += (size_t)ed[i].rows * ed[i].cols;
             S* blk = zalloc(tot);
@@ -1409,7 +1425,10 @@ int Tree::factorize_t() {
                 ed[i].A = blk + off;
                 off += (size_t)ed[i].rows * ed[i].cols;
             }
-        }
+        };
+        const bool independent = fillin_dist2();
+        if (!independent) fill_in(false);
+        hacc(0, ht);
         const auto& ints = interiors[L];
         const auto& itf = interfaces[L];
         st.n_interiors = (int)ints.size();
@@ -1551,10 +1570,16 @@ int Tree::factorize_t() {
         }
         hacc(2, ht);
         // colour layers for the shared-block updates: interiors that share an out-neighbour get
-        // different colours, and the colours run in order (the update order of the level)
+        // different colours, and the colours run in order (the update order of the level).
+        // The interiors' out lists are final here, so the colouring and then the fill-in of
+        // independent interiors run while the panel work does.
         int ncolors = colour_interiors(ints);
         st.colors = ncolors;
         hacc(10, ht);
+        if (independent) {
+            fill_in(true);
+            hacc(0, ht);
+        }
         // Trailing-update chains, built over the interiors in parallel: links into one array at
         // per-slot offsets (counted first), tasks straight into their colour's launch buckets
         // (0-2 one-wave by link rows, 3-6 C-resident, 7 workgroup), all in interior order.
@@ -1745,6 +1770,96 @@ int Tree::factorize_t() {
                 });
             HIPCHK(hipGetLastError());
         }
+        // ---- the merge's structure (src/ptree.cpp:1940-2123): which parents become neighbours,
+        // their distance-2 lists and new edges, none of which depends on a rank.
+        // Per parent: the merged neighbour and distance-2 lists (reads of the children only, so
+        // the parents run in parallel; duplicates dropped through per-thread stamp arrays, stamp
+        // = the parent), then the new edges in parent order with ids from prefix sums, entries
+        // and out lists filled per parent in parallel, then every target's in lists appended in
+        // id order (targets split over the threads by cluster number).  Sizes come later.
+        const int Lp = L + 1;
+        bool merged = false;
+        size_t ne_merge = 0;
+        auto merge_structure = [&]() {
+            merged = true;
+            for (int c : ints) cl[c].eliminated = true;
+            ne_merge = ed.size();
+            const auto& bl = bottoms[Lp];
+            const int np = (int)bl.size();
+            std::vector<std::vector<int>> pd2(np), pnb(np);
+            // compact per-cluster lookups (the passes below touch clusters at random)
+            std::vector<int> par(cl.size()), lvl(cl.size());
+            std::vector<char> elim(cl.size());
+            parallel_ranges((int)cl.size(), 4096, [&](int b, int e, int) {
+                for (int i = b; i < e; ++i) {
+                    par[i] = cl[i].parent;
+                    lvl[i] = cl[i].lvl();
+                    elim[i] = cl[i].eliminated;
+                }
+            });
+            parallel_ranges(np, 64, [&](int b, int e, int) {
+                std::vector<int> seen_n(cl.size(), -1), seen_d(cl.size(), -1);
+                for (int i = b; i < e; ++i) {
+                    const int p = bl[i];
+                    std::vector<int>& d2 = pd2[i];
+                    std::vector<int>& nbrs = pnb[i];
+                    for (int d : cl[p].dist2)
+                        if (seen_d[d] != p) { seen_d[d] = p; d2.push_back(d); }
+                    for (int ch : cl[p].children) {
+                        for (int e2 : cl[ch].out) {
+                            const int n2 = ed[e2].n2;
+                            if (elim[n2]) continue;
+                            const int q = par[n2];
+                            if (seen_n[q] != p) { seen_n[q] = p; nbrs.push_back(q); }
+                        }
+                        for (int d : cl[ch].dist2) {
+                            if (lvl[d] < Lp) continue;
+                            const int q = par[d];
+                            if (seen_d[q] != p) { seen_d[q] = p; d2.push_back(q); }
+                        }
+                    }
+                    std::sort(d2.begin(), d2.end());
+                    std::sort(nbrs.begin(), nbrs.end());
+                }
+            });
+            hacc(14, ht);
+            std::vector<size_t> eoff(np + 1, 0);
+            for (int i = 0; i < np; ++i) eoff[i + 1] = eoff[i] + pnb[i].size();
+            const size_t base = ed.size(), nnew = eoff[np];
+            ed.resize(base + nnew);
+            parallel_ranges(np, 64, [&](int b, int e, int) {
+                for (int i = b; i < e; ++i) {
+                    const int p = bl[i];
+                    Cluster& P = cl[p];
+                    P.dist2 = std::move(pd2[i]);
+                    P.out.reserve(P.out.size() + pnb[i].size());
+                    for (size_t j = 0; j < pnb[i].size(); ++j) {
+                        const int nn = pnb[i][j], id = (int)(base + eoff[i] + j);
+                        Edge& E = ed[id];
+                        E.n1 = p;
+                        E.n2 = nn;
+                        E.spars = lvl[p] > Lp && lvl[nn] > Lp && p != nn;
+                        P.out.push_back(id);
+                        if (E.spars) P.sout.push_back(id);
+                        if (p == nn) P.self_edge = id;
+                    }
+                    sort_out(P);
+                }
+            });
+            const int NT = host_threads();
+            std::vector<std::vector<int>> ib(NT);
+            for (size_t id = base; id < base + nnew; ++id)
+                if (ed[id].n1 != ed[id].n2) ib[ed[id].n2 % NT].push_back((int)id);
+            parallel_ranges(NT, nnew < 16384 ? NT : 1, [&](int b, int e, int) {  // small levels: one thread
+                for (int t = b; t < e; ++t)
+                    for (int id : ib[t]) {
+                        Cluster& C2 = cl[ed[id].n2];
+                        C2.in.push_back(id);
+                        if (ed[id].spars) C2.sin.push_back(id);
+                    }
+            });
+            hacc(3, ht);
+        };
         // ---- RRQR descriptors (src/ptree.cpp:373-439)
         std::vector<CopyTask<S>> gat;
         std::vector<int> gat_begin;  // first gather copy of each RRQR task
@@ -1817,6 +1932,10 @@ int Tree::factorize_t() {
         if (!gq.empty()) {
             seg_end(d_ranks, (int)gq.size(), d_rrqr_err);
             solve_sched_levels<S>(L);  // the finished levels' solve schedule, while the device works
+            if (L < nlevels - 1) {
+                hacc(2, ht);
+                merge_structure();  // needs no rank: runs while the device works too
+            }
             std::vector<int> ranks(gq.size());
             int rrqr_err = 0;
             HIPCHK(hipMemcpyAsync(ranks.data(), d_ranks, sizeof(int) * ranks.size(), hipMemcpyDeviceToHost, stream_));
@@ -1853,8 +1972,9 @@ int Tree::factorize_t() {
         stats.push_back(st);
         if (L == nlevels - 1) break;
 
-        // ---- merge (src/ptree.cpp:1940-2123)
-        for (int c : ints) cl[c].eliminated = true;
+        // ---- merge (src/ptree.cpp:1940-2123); its structure was built above when this level
+        // read ranks back
+        if (!merged) merge_structure();
         // The sparsify apply's right transform (A <- A Q_n, in place on the interface blocks, row
         // chunks) needs nothing from the merge: it is queued first and runs while the host merges.
         // (FP32 row chunks of 128: one chunk per workgroup of the C-resident kernel)
@@ -1899,7 +2019,7 @@ int Tree::factorize_t() {
             }
         }
         hacc(12, ht);
-        const int Lp = L + 1;
+        // the parents' sizes from their children's (ranks now known), then the new edges' sizes
         current_bottom = Lp;
         for (int p : bottoms[Lp]) {
             int rs = 0, cs = 0;
@@ -1913,94 +2033,14 @@ int Tree::factorize_t() {
             cl[p].cols = cl[p].cols_org = cs;
             cl[p].rank = cs;
         }
-        const size_t ne_merge = ed.size();
         hacc(13, ht);
-        {
-            // Per parent: the merged neighbour and distance-2 lists (reads of the children only,
-            // so the parents run in parallel; duplicates dropped through per-thread stamp arrays,
-            // stamp = the parent), then the new edges in parent order.
-            const auto& bl = bottoms[Lp];
-            const int np = (int)bl.size();
-            std::vector<std::vector<int>> pd2(np), pnb(np);
-            // compact per-cluster lookups (the passes below touch clusters at random)
-            std::vector<int> par(cl.size()), lvl(cl.size()), rows_of(cl.size());
-            std::vector<char> elim(cl.size());
-            parallel_ranges((int)cl.size(), 4096, [&](int b, int e, int) {
-                for (int i = b; i < e; ++i) {
-                    par[i] = cl[i].parent;
-                    lvl[i] = cl[i].lvl();
-                    rows_of[i] = cl[i].rows;
-                    elim[i] = cl[i].eliminated;
-                }
-            });
-            parallel_ranges(np, 64, [&](int b, int e, int) {
-                std::vector<int> seen_n(cl.size(), -1), seen_d(cl.size(), -1);
-                for (int i = b; i < e; ++i) {
-                    const int p = bl[i];
-                    std::vector<int>& d2 = pd2[i];
-                    std::vector<int>& nbrs = pnb[i];
-                    for (int d : cl[p].dist2)
-                        if (seen_d[d] != p) { seen_d[d] = p; d2.push_back(d); }
-                    for (int ch : cl[p].children) {
-                        for (int e2 : cl[ch].out) {
-                            const int n2 = ed[e2].n2;
-                            if (elim[n2]) continue;
-                            const int q = par[n2];
-                            if (seen_n[q] != p) { seen_n[q] = p; nbrs.push_back(q); }
-                        }
-                        for (int d : cl[ch].dist2) {
-                            if (lvl[d] < Lp) continue;
-                            const int q = par[d];
-                            if (seen_d[q] != p) { seen_d[q] = p; d2.push_back(q); }
-                        }
-                    }
-                    std::sort(d2.begin(), d2.end());
-                    std::sort(nbrs.begin(), nbrs.end());
-                }
-            });
-            hacc(14, ht);
-            // The parents' edges, in the order new_edge would make them (parent order, then
-            // neighbour order): ids from prefix sums, entries and out lists filled per parent in
-            // parallel, then every target's in lists appended in id order (targets split over
-            // the threads by cluster number).
-            std::vector<size_t> eoff(np + 1, 0);
-            for (int i = 0; i < np; ++i) eoff[i + 1] = eoff[i] + pnb[i].size();
-            const size_t base = ed.size(), nnew = eoff[np];
-            ed.resize(base + nnew);
-            parallel_ranges(np, 64, [&](int b, int e, int) {
-                for (int i = b; i < e; ++i) {
-                    const int p = bl[i];
-                    Cluster& P = cl[p];
-                    P.dist2 = std::move(pd2[i]);
-                    P.out.reserve(P.out.size() + pnb[i].size());
-                    for (size_t j = 0; j < pnb[i].size(); ++j) {
-                        const int nn = pnb[i][j], id = (int)(base + eoff[i] + j);
-                        Edge& E = ed[id];
-                        E.n1 = p;
-                        E.n2 = nn;
-                        E.rows = rows_of[nn];
-                        E.cols = P.cols;
-                        E.spars = lvl[p] > current_bottom && lvl[nn] > current_bottom && p != nn;
-                        P.out.push_back(id);
-                        if (E.spars) P.sout.push_back(id);
-                        if (p == nn) P.self_edge = id;
-                    }
-                    sort_out(P);
-                }
-            });
-            const int NT = host_threads();
-            std::vector<std::vector<int>> ib(NT);
-            for (size_t id = base; id < base + nnew; ++id)
-                if (ed[id].n1 != ed[id].n2) ib[ed[id].n2 % NT].push_back((int)id);
-            parallel_ranges(NT, nnew < 16384 ? NT : 1, [&](int b, int e, int) {  // small levels: one thread
-                for (int t = b; t < e; ++t)
-                    for (int id : ib[t]) {
-                        Cluster& C2 = cl[ed[id].n2];
-                        C2.in.push_back(id);
-                        if (ed[id].spars) C2.sin.push_back(id);
-                    }
-            });
-        }
+        parallel_ranges((int)(ed.size() - ne_merge), 4096, [&](int b, int e, int) {
+            for (int i = b; i < e; ++i) {
+                Edge& E = ed[ne_merge + i];
+                E.rows = cl[E.n2].rows;
+                E.cols = cl[E.n1].cols;
+            }
+        });
         hacc(3, ht);
         {
             size_t tot = 0;

@@ -324,7 +324,8 @@ private:
     int colour_interiors(const std::vector<int>& ints);
     void sort_out(Cluster& c);
     int find_out(const Cluster& c, int n2) const;
-    void fillin();
+    bool fillin_dist2();
+    void fillin_edges(bool independent);
     template <typename S> int factorize_t();
     template <typename S> void solve_sched_begin();
     template <typename S> void solve_sched_levels(int upto);
