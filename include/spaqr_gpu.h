@@ -106,6 +106,7 @@ int spaqr_tree_num_systems(const spaqr_tree* t);
     12 rank mode 1's sketch width s = ceil(grow/100 x predicted rank) + pad: grow (125; 100 .. 200)
     13 ... and pad + 1 (built in 8, i.e. 9; 1 .. 65).  Any width is safe: an interface whose
        sketch fails the tail test is retried wider, then falls back to geqp3
+    14 column chunk of the trailing update in a big front's panel steps (128; 32, 64, 128, 256)
    The committed autotuner (python -m pspaqr_public_amd.autotune) picks them per workload. */
 int spaqr_tree_set_tuning(spaqr_tree* t, int param, int value);
 int spaqr_tree_get_tuning(const spaqr_tree* t, int param);

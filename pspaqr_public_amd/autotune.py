@@ -42,10 +42,11 @@ CANDIDATES = {
     "sketch_tile": [64, 128, 256],
     "sketch_grow": [100, 105, 110, 115, 125, 150],
     "sketch_pad": [1, 5, 9, 17],
+    "panel_cols": [32, 64, 128, 256],
 }
 BUILT_IN = {"big_front": 96, "tpmqrt_tile": 64, "larfb_cols": 128, "spars_cols": 128, "rank_cols": 128,
             "sketch_unpivoted": 96, "tpqrt_coop": 1, "small_tasks": 1, "sketch_tile": 128, "sketch_grow": 125,
-            "sketch_pad": 9}
+            "sketch_pad": 9, "panel_cols": 128}
 
 
 def key(workload: str, prec: int) -> str:

@@ -206,6 +206,7 @@ int spaqr_tree_set_tuning(spaqr_tree* h, int param, int value) {
         case T_SKETCH_TILE: valid = valid || in({64, 128, 256}); break;
         case T_SKETCH_GROW: valid = valid || (value >= 100 && value <= 200); break;
         case T_SKETCH_PAD: valid = valid || (value >= 1 && value <= 65); break;
+        case T_PANEL_COLS: valid = valid || in({32, 64, 128, 256}); break;
         default: break;
     }
     if (!valid) { h->t->last_error = "tuning value out of range for this parameter"; return -3; }

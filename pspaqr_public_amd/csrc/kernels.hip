@@ -415,11 +415,21 @@ __device__ __forceinline__ void wg_panel_qr_col(int m, int nbp, S* Rt, int ldr, 
         if (tid == 0) { sm.tau[j] = tau; sm.scl[j] = scal; }
         col_chunks<S, SR>(xs, [&](int i, S x) { a[i] -= x * sw; });
         if (!PENT) {
+            // the segment's four top rows: every read issued, then every write (unchanged
+            // entries written back as read), no branch between them.  Lanes of one segment
+            // are one half-wave, so the column-j reads precede lane j's write of its diagonal.
+            S tr[4], tj[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int r = g + 8 * q;
-                if (r > j) sm.top[r * 33 + c] -= sm.top[r * 33 + j] * sw;
-                else if (r == j) sm.top[r * 33 + c] = (c == j) ? beta : sm.top[r * 33 + c] - w;
+                tr[q] = sm.top[r * 33 + c];
+                tj[q] = sm.top[r * 33 + j];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = g + 8 * q;
+                const S upd = (r > j) ? tr[q] - tj[q] * sw : tr[q] - w;
+                sm.top[r * 33 + c] = (r < j) ? tr[q] : ((r == j && c == j) ? beta : upd);
             }
         } else if ((j & 7) == g) {
             sm.top[j * 33 + c] = (c == j) ? beta : sm.top[j * 33 + c] - w;
@@ -539,6 +549,110 @@ __device__ __forceinline__ void wg_apply_lt_mfma(int m, int kb, const float* __r
             }
         }
     };
+    if (!TRC && ncols >= 1 && ncols <= 32) {
+        // At most 32 columns (a panel's own columns): the waves split each tile's rows for
+        // W = Atop + V^T C (wave w: rows 16w .. 16w + 15; partial W summed through LDS), every
+        // wave forms T^T W, and waves 0 and 1 update the tile's two 32-row halves.
+        const int nc = ncols;
+        struct NarrowRegs {
+            float v[8];
+            float c[8];
+        };
+        auto load_n = [&](int r0, int rb, NarrowRegs& g) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int e = tid + q * WG, r = e & 63, i = e >> 6;
+                g.v[q] = V[(r0 + min(r, rb - 1)) + (size_t)min(i, kb - 1) * ldv];
+                g.c[q] = C[(r0 + min(r, rb - 1)) + (size_t)min(i, nc - 1) * ldc];
+            }
+        };
+        auto store_n = [&](int r0, int rb, const NarrowRegs& g) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int e = tid + q * WG, r = e & 63, i = e >> 6, R = r0 + r;
+                float v = 0.f;
+                if (r < rb && i < kb) v = PENT ? g.v[q] : ((R > i) ? g.v[q] : (R == i ? 1.f : 0.f));
+                sm.Vs[i * 65 + r] = v;
+                sm.Cs[i * 65 + r] = (r < rb && i < nc) ? g.c[q] : 0.f;
+            }
+        };
+        floatx16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            float a = 0.f;
+            if (PENT && w == 0) {
+                const int i = mfma_row(l, r);
+                const float av = Atop[min(i, kb - 1) + (size_t)min(li, nc - 1) * lda];
+                a = (i < kb && li < nc) ? av : 0.f;
+            }
+            acc[r] = a;
+        }
+        NarrowRegs g;
+        load_n(0, min(64, m), g);
+        for (int r0 = 0; r0 < m; r0 += 64) {
+            const int rb = min(64, m - r0);
+            __syncthreads();
+            store_n(r0, rb, g);
+            __syncthreads();
+            if (r0 + 64 < m) load_n(r0 + 64, min(64, m - r0 - 64), g);
+#pragma unroll
+            for (int kk = 16 * w; kk < 16 * w + 16; kk += 2) {
+                const float a = sm.Vs[li * 65 + kk + lk];
+                const float b = sm.Cs[li * 65 + kk + lk];
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Ws[li * 33 + mfma_row(l, r)] = acc[r];
+        load_n(0, min(64, m), g);
+        __syncthreads();
+        floatx16 w2;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) w2[r] = 0.f;
+#pragma unroll
+        for (int kk = 0; kk < 32; kk += 2) {
+            const float a = sm.Ts[li * 33 + kk + lk];
+            const int o = li * 33 + kk + lk;
+            const float b = (sm.Ws[o] + sm.Ws[32 * 33 + o]) + (sm.Ws[2 * 32 * 33 + o] + sm.Ws[3 * 32 * 33 + o]);
+            w2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, w2, 0, 0, 0);
+        }
+        __syncthreads();  // every wave has read the partial blocks
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = mfma_row(l, r);
+            const float v = (i < kb) ? w2[r] : 0.f;
+            Ws[li * 33 + i] = v;
+            if (PENT && w == 0 && i < kb && li < nc) Atop[i + (size_t)li * lda] -= v;
+        }
+        for (int r0 = 0; r0 < m; r0 += 64) {
+            const int rb = min(64, m - r0);
+            __syncthreads();
+            store_n(r0, rb, g);
+            __syncthreads();
+            if (r0 + 64 < m) load_n(r0 + 64, min(64, m - r0 - 64), g);
+            if (w < 2) {
+                floatx16 d;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) d[r] = 0.f;
+#pragma unroll
+                for (int kk = 0; kk < 32; kk += 2) {
+                    const float a = sm.Vs[(kk + lk) * 65 + w * 32 + li];
+                    const float b = Ws[li * 33 + kk + lk];
+                    d = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, d, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 16; ++r) sm.Cs[li * 65 + w * 32 + mfma_row(l, r)] -= d[r];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int e = tid + q * WG, r = e & 63, j = e >> 6;
+                if (r < rb && j < nc) C[(r0 + r) + (size_t)j * ldc] = sm.Cs[j * 65 + r];
+            }
+        }
+        __syncthreads();
+        return;
+    }
     for (int c0 = 0; c0 < ncols; c0 += 128) {
         const int nc = min(128, ncols - c0);
         const int jc = 32 * w + li;  // this lane's column inside the chunk

@@ -2193,6 +2193,7 @@ int Tree::geqrt_big(const std::vector<GeqrtTask<S>>& big) {
     std::vector<PanelStepTask<S>> ps;
     std::vector<int> off;
     const int nsteps = (kmax + NB - 1) / NB + 1;
+    const int chunk = tuned(T_PANEL_COLS, 128);
     for (int s = 0; s < nsteps; ++s) {
         off.push_back((int)ps.size());
         const int p0 = s * NB, pp = p0 - NB;  // this panel, the previous one
@@ -2222,12 +2223,12 @@ int Tree::geqrt_big(const std::vector<GeqrtTask<S>>& big) {
                 c0 = p0 + nbp;
             }
             if (prev)
-                for (; c0 < t.n; c0 += 128) {
+                for (; c0 < t.n; c0 += chunk) {
                     PanelStepTask<S> u{};
                     u.apply = 1;
                     u.a = a;
                     u.a.C = t.A + pp + (size_t)c0 * t.lda;
-                    u.a.ncols = std::min(128, t.n - c0);
+                    u.a.ncols = std::min(chunk, t.n - c0);
                     rest.push_back(u);
                 }
         }

@@ -110,6 +110,7 @@ enum TuneParam {
     T_SKETCH_TILE,      // output-tile columns of the bf16 sketch: 64, 128, 256 (128)
     T_SKETCH_GROW,      // learned sketch width = grow% x predicted rank + pad: 100 .. 200 (125)
     T_SKETCH_PAD,       // the pad + 1 (0 restores the built-in 8): 1 .. 65
+    T_PANEL_COLS,       // column chunk of the trailing update in a big front's panel steps (128)
     T_NPARAM
 };
 

@@ -107,7 +107,7 @@ class ParTree:
 
     TUNABLES = ("big_front", "tpmqrt_tile", "larfb_cols", "trsm_rows", "spars_rows", "spars_cols", "rank_cols",
                 "sketch_unpivoted", "tpqrt_coop", "small_tasks", "sketch_bf16", "sketch_tile", "sketch_grow",
-                "sketch_pad")
+                "sketch_pad", "panel_cols")
 
     def set_tuning(self, params: dict):
         """Planner tunables by name (spaqr_tree_set_tuning); 0 restores the built-in value."""
