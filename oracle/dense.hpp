@@ -319,6 +319,16 @@ inline void trsv_upper(int n, const double* R, int ldr, double* x) {
     }
 }
 
+// cblas_dtrsv(Upper, Trans, NonUnit): x <- R^{-T} x (forward substitution with R^T), as
+// tQR::fwd / tScaleD::fwd call it (src/toperations.cpp:19,72).
+inline void trsv_upper_trans(int n, const double* R, int ldr, double* x) {
+    for (int j = 0; j < n; ++j) {
+        double s = x[j];
+        for (int i = 0; i < j; ++i) s -= R[i + (size_t)j * ldr] * x[i];
+        x[j] = s / R[j + (size_t)j * ldr];
+    }
+}
+
 // dormqr('L','T', n=1): x <- Q^T x = H(k)...H(1) x.
 inline void ormqr_lt_vec(int m, int k, const double* V, int ldv, const double* tau, double* x) {
     for (int i = 0; i < k; ++i) {

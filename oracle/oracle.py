@@ -51,6 +51,19 @@ def lib():
         L.orc_gmres.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_double, _dp, C.c_int,
                                 C.POINTER(C.c_int)]
         L.orc_gmres.restype = C.c_int
+        L.orc_tree_create_rect.restype = C.c_void_p
+        L.orc_tree_create_rect.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _ip, C.c_int, C.c_double, C.c_int]
+        L.orc_nrows.argtypes = [C.c_void_p]
+        L.orc_nrows.restype = C.c_int
+        L.orc_get_rperm.argtypes = [C.c_void_p, _ip]
+        L.orc_get_rows.argtypes = [C.c_void_p, _ip]
+        L.orc_get_r2c.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int]
+        L.orc_get_r2c.restype = C.c_int
+        L.orc_solve_ls.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_solve_nrml.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_spmv_t.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_cgls.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.POINTER(C.c_double)]
+        L.orc_cgls.restype = C.c_int
         # dense kernels
         L.orc_geqrt.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.orc_tpqrt.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
@@ -104,12 +117,18 @@ class OracleTree:
         then ignored."""
         A = A.tocsc()
         A.sort_indices()
-        self.n = A.shape[0]
+        self.m, self.n = A.shape
         self._colptr = np.ascontiguousarray(A.indptr, dtype=np.int32)
         self._rowind = np.ascontiguousarray(A.indices, dtype=np.int32)
         self._val = np.ascontiguousarray(A.data, dtype=np.float64)
         L = lib()
-        if parts is None:
+        if self.m != self.n:
+            if parts is None:
+                raise ValueError("a rectangular system is ordered by given leaf parts")
+            self._parts = np.ascontiguousarray(parts, dtype=np.int32)
+            self._h = L.orc_tree_create_rect(self.m, self.n, self._colptr, self._rowind, self._val, self._parts, nlevels,
+                                             tol, skip)
+        elif parts is None:
             self._h = L.orc_tree_create(self.n, self._colptr, self._rowind, self._val, n_side, dim, nlevels, tol, skip)
         else:
             self._parts = np.ascontiguousarray(parts, dtype=np.int32)
@@ -161,9 +180,54 @@ class OracleTree:
 
     def spmv(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
-        y = np.zeros(self.n)
+        y = np.zeros(self.m)
         lib().orc_spmv(self._h, x, y)
         return y
+
+    def spmv_t(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        y = np.zeros(self.n)
+        lib().orc_spmv_t(self._h, r, y)
+        return y
+
+    def rperm(self):
+        p = np.zeros(self.m, dtype=np.int32)
+        lib().orc_get_rperm(self._h, p)
+        return p
+
+    def rows(self):
+        """Per cluster: rstart, rows at creation, extra rows received, row_rank, extra rows moved."""
+        nc = lib().orc_num_clusters(self._h)
+        out = np.zeros(5 * nc, dtype=np.int32)
+        lib().orc_get_rows(self._h, out)
+        return out.reshape(nc, 5)
+
+    def r2c(self, order: int):
+        buf = np.zeros(1 << 20, dtype=np.int32)
+        k = lib().orc_get_r2c(self._h, order, buf, buf.size)
+        return buf[:k].copy()
+
+    def solve_ls(self, b):
+        """ParTree::solve(b, x): b (m, permuted rows) -> x (n, permuted columns)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n)
+        lib().orc_solve_ls(self._h, b, x)
+        return x
+
+    def solve_nrml(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        x = np.zeros(self.n)
+        lib().orc_solve_nrml(self._h, y, x)
+        return x
+
+    def cgls(self, b, iters: int = 300, tol: float = 1e-12):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n)
+        rel = C.c_double(0.0)
+        it = lib().orc_cgls(self._h, b, x, iters, tol, C.byref(rel))
+        if it < 0:
+            raise RuntimeError("oracle cgls failed")
+        return x, it, rel.value
 
     def gmres(self, b, iters: int = 300, tol: float = 1e-12, restart: int = 0, true_tol: float = 1e-12):
         b = np.ascontiguousarray(b, dtype=np.float64)

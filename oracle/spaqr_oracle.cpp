@@ -26,6 +26,7 @@
 // The RRQR is full geqp3 + choose_rank (the reference's non-MKL branch,
 // src/ptree.cpp:411-416).  Only square systems (rperm == cperm, c2r_count == 1).
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -59,6 +60,8 @@ namespace orc {
 //     (src/ptree.cpp:30, 1566, 1635 stop at nlevels - 2)
 //   6 getInterfacesUsingATA counts a neighbour in the vertex's own half as right of it
 //     (src/partition.cpp:659 tests pj <= pi)
+//   7 sparsify_extra keeps one row fewer than choose_rank (src/ptree.cpp:480-492)
+//   8 the solve skips orthogonalD_extra_fwd, Q_ex^T on the extra rows (src/ptree.cpp:2531)
 static int g_fault = 0;
 
 // f(i) for i in [0, n) on the OpenMP threads.  Tasks are independent (each writes only its own
@@ -439,7 +442,18 @@ struct Cluster {  // include/cluster.h:142-389
     int order, merge_level;
     bool eliminated = false;
     int rank;
+    int row_rank = 0;  // extra rows kept (rows - cols; after sparsify_extra, its rank)
     int csize_org, rsize_org;
+    // rectangular systems: an interior's extra-row targets (cluster order per extra row, in
+    // row order) and row_perm; an interface's received extra rows (interior order, start in
+    // that interior's row_perm, count), sorted; the extra sparsification's reflectors
+    std::vector<int> r2c;
+    std::vector<int> row_perm;
+    std::vector<std::array<int, 3>> extra_from;
+    int tot_extra = 0;
+    Mat Qex, Tex;
+    std::vector<double> tauex;
+    int nex = 0;  // rows of Qex: the extra rows at sparsify_extra
     Cluster* parent = nullptr;
     std::vector<Cluster*> children;
     int rposparent = -1, cposparent = -1;
@@ -459,6 +473,7 @@ struct Cluster {  // include/cluster.h:142-389
     Cluster(int cs, int cz, int rs, int rz, ClusterID id_, int order_, int ml)
         : cstart(cs), csize(cz), rstart(rs), rsize(rz), id(id_), order(order_), merge_level(ml) {
         rank = cz;
+        row_rank = rz - cz;
         csize_org = cz;
         rsize_org = rz;
         x.assign(rz, 0.0);
@@ -466,7 +481,7 @@ struct Cluster {  // include/cluster.h:142-389
     int lvl() const { return id.self.lvl; }
     int cols() const { return csize; }
     int rows() const { return rsize; }
-    void reset_size(int r, int c) { rsize = r; csize = c; rank = c; }
+    void reset_size(int r, int c) { rsize = r; csize = c; rank = c; row_rank = r - c; }  // src/cluster.cpp:214-218
     void set_size(int r, int c) { reset_size(r, c); x.assign(r, 0.0); }
     Edge* find_out(const Cluster* n) const {
         for (Edge* e : edgesOut) if (e->n2 == n) return e;
@@ -483,7 +498,9 @@ struct Tree {
     int nlevels, skip;
     double tol = 0.1;
     int n = 0;
+    int m = 0;                      // rows of A (m > n: a least-squares system)
     CSC A, App;
+    std::vector<int> rperm;         // row permutation (== cperm for square systems)
     std::vector<double> X;
     int d = 0;
     std::vector<int> given_parts;  // leaf parts in place of coordinates (the METIS path)
@@ -532,20 +549,65 @@ struct Tree {
     }
 
     // ---------------------------------------------------------- partition
+    bool square() const { return m == n; }
+
+    // form_rmap without a matching (src/tree.cpp:102-165): every row of Ap = A(:, cperm) goes
+    // to the cluster whose columns carry the largest sum of squares of that row (the first
+    // such cluster in column order wins ties; a zero row goes to the last column), filed
+    // under that cluster's first column the row touches; rows are then ordered by that column,
+    // and in row order within it.  c2r_count[j]: the rows filed under permuted column j.
+    void form_rmap(const CSC& Ap, std::vector<int>& c2r_count) {
+        CSC At = transpose(Ap);  // column i of At: the entries of row i, by increasing column
+        std::vector<std::vector<int>> c2r(n);
+        c2r_count.assign(n, 0);
+        for (int i = 0; i < m; ++i) {
+            double max_val = -1.0;
+            int cidx = -1;
+            for (int k = At.colptr[i]; k < At.colptr[i + 1];) {
+                const ClusterID& cid = cpermed[At.rowind[k]];
+                double val = At.val[k] * At.val[k];
+                int kn = k + 1;
+                while (kn < At.colptr[i + 1] && cpermed[At.rowind[kn]] == cid) {
+                    val += At.val[kn] * At.val[kn];
+                    ++kn;
+                }
+                if (val > max_val) { max_val = val; cidx = At.rowind[k]; }
+                k = kn;
+            }
+            if (cidx == -1) cidx = n - 1;
+            c2r_count[cidx]++;
+            c2r[cidx].push_back(i);
+        }
+        rperm.clear();
+        for (int j = 0; j < n; ++j) rperm.insert(rperm.end(), c2r[j].begin(), c2r[j].end());
+    }
+
     void partition() {  // src/ptree.cpp:495-694
+        if (!square() && given_parts.empty())
+            throw std::runtime_error("a rectangular system needs leaf parts (the METIS path): the geometric path "
+                                     "assigns rows to clusters by a bipartite matching (HSL MC64) for non-square A");
         std::vector<ClusterID> cmap =
             given_parts.empty() ? geometric_partition(A, nlevels, X, d) : parts_partition(A, nlevels, given_parts);
         cperm = init_permutation(nlevels, cmap);
         cpermed.resize(n);
         for (int k = 0; k < n; ++k) cpermed[k] = cmap[cperm[k]];
-        // leaf clusters: runs of equal ClusterID
+        std::vector<int> c2r_count(n, 1);
+        if (square()) {
+            rperm = cperm;
+        } else {
+            std::vector<int> ident(m);
+            for (int i = 0; i < m; ++i) ident[i] = i;
+            form_rmap(permute(A, ident, cperm), c2r_count);
+        }
+        // leaf clusters: runs of equal ClusterID, with the rows filed under their columns
         for (int k = 0, l = 0; k < n;) {
-            int kn = k + 1;
-            while (kn < n && cpermed[kn] == cpermed[k]) kn++;
-            Cluster* s = new_cluster(k, kn - k, l, kn - k, cpermed[k], 0);
+            int kn = k + 1, nr = c2r_count[k];
+            while (kn < n && cpermed[kn] == cpermed[k]) nr += c2r_count[kn++];
+            if (nr == 0) throw std::runtime_error("a leaf cluster got no rows");
+            Cluster* s = new_cluster(k, kn - k, l, nr, cpermed[k], 0);
             bottoms[0].push_back(s);
             (s->lvl() == 0 ? interiors[0] : interfaces[0]).push_back(s);
-            l += kn - k;
+            l += nr;
             k = kn;
         }
         for (int lvl = 1; lvl < nlevels; ++lvl) {
@@ -595,14 +657,16 @@ struct Tree {
 
     // ---------------------------------------------------------- assemble
     void assemble() {  // src/ptree.cpp:696-864
-        App = permute(A, cperm, cperm);
-        std::vector<Cluster*> cl(n);
-        for (Cluster* s : bottoms[0])
+        App = permute(A, rperm, cperm);
+        std::vector<Cluster*> cl(n), rl(m);  // cluster of each permuted column / row
+        for (Cluster* s : bottoms[0]) {
             for (int k = s->cstart; k < s->cstart + s->cols(); ++k) cl[k] = s;
+            for (int k = s->rstart; k < s->rstart + s->rows(); ++k) rl[k] = s;
+        }
         for (Cluster* self : bottoms[0]) {
             std::set<int> nb;
             for (int j = self->cstart; j < self->cstart + self->cols(); ++j)
-                for (int k = App.colptr[j]; k < App.colptr[j + 1]; ++k) nb.insert(cl[App.rowind[k]]->order);
+                for (int k = App.colptr[j]; k < App.colptr[j + 1]; ++k) nb.insert(rl[App.rowind[k]]->order);
             nb.insert(self->order);
             for (int o : nb) {
                 Cluster* nbr = bottoms[0][o - bottoms[0][0]->order];
@@ -651,6 +715,7 @@ struct Tree {
             current_bottom = ilvl;
             fillin();
             eliminate(ilvl);
+            if (!square()) reassign(ilvl);
             scale(ilvl);
             if (ilvl >= skip && ilvl < nlevels - (g_fault == 5 ? 1 : 2) && tol != 0) sparsify(ilvl);
             if (ilvl < nlevels - 1) merge();
@@ -686,6 +751,7 @@ struct Tree {
         par_for(ni, [&](int i) {
             Cluster* c = ints[i];
             if (c->cols() == 0) throw std::runtime_error("cluster has zero cols");
+            if (c->rows() < c->cols()) throw std::runtime_error("an interior has fewer rows than columns");
             int k = c->cols(), m = c->rows();
             c->T = Mat(k, k);
             geqrt(m, k, k, c->eself->A.data(), m, c->T.data(), k);
@@ -777,6 +843,79 @@ struct Tree {
         for (const auto& v : ut) trace.insert(trace.end(), v.begin(), v.end());
     }
 
+    // Row reassignment of a rectangular system (src/ptree.cpp:1149-1356; the row rule is
+    // Cluster::update_extra_rnorm, src/cluster.cpp:151-160).  After an interior c is
+    // eliminated, rows cols(c) .. rows(c) of Q_c^T A(c, :) touch no eliminated column: each
+    // goes to the out-neighbour m of c (one that is also an in-neighbour, is_cnbr) whose block
+    // A(c, m) holds the largest norm of that row (strictly larger replaces, so the first in
+    // cluster order keeps a tie; all-zero rows stay with c and are dropped -- the reference's
+    // scheduler-ordered loop drops them too when c comes after its neighbours).  row_perm
+    // stable-sorts the extra rows by target.  Every interface then appends the rows it
+    // received, in interior order, to its diagonal block and to the blocks of its in-edges
+    // from non-eliminated clusters (shift_tf + Edge::merge, :1316-1356, include/edge.h:49-57);
+    // the rows are zero in a block whose source has no edge into c.
+    void reassign(int ilvl) {
+        (void)ilvl;
+        for (Cluster* c : interiors[current_bottom]) {
+            const int k = c->cols(), ex = c->rows() - k;
+            c->r2c.assign(std::max(ex, 0), c->order);
+            c->row_perm.clear();
+            if (ex <= 0) continue;
+            std::vector<double> best(ex, 0.0);
+            std::vector<Edge*> ins(c->edgesIn.begin(), c->edgesIn.end());
+            std::sort(ins.begin(), ins.end(), [](Edge* a, Edge* b) { return a->n1->order < b->n1->order; });
+            for (Edge* e : ins) {
+                if (!c->find_out(e->n1)) continue;  // is_cnbr
+                for (int i = 0; i < ex; ++i) {
+                    double s2 = 0.0;
+                    for (int j = 0; j < e->A.c; ++j) s2 += e->A(k + i, j) * e->A(k + i, j);
+                    const double nrm = std::sqrt(s2);
+                    if (nrm > best[i]) { best[i] = nrm; c->r2c[i] = e->n1->order; }
+                }
+            }
+            c->row_perm.resize(ex);
+            for (int i = 0; i < ex; ++i) c->row_perm[i] = i;
+            std::stable_sort(c->row_perm.begin(), c->row_perm.end(),
+                             [&](int a, int b) { return c->r2c[a] < c->r2c[b]; });
+            for (Edge* eo : c->edgesOut) {
+                Cluster* nn = eo->n2;
+                if (nn == c) continue;
+                int i0 = 0;
+                while (i0 < ex && c->r2c[c->row_perm[i0]] != nn->order) ++i0;
+                int i1 = i0;
+                while (i1 < ex && c->r2c[c->row_perm[i1]] == nn->order) ++i1;
+                nn->extra_from.push_back({c->order, i0 < ex ? i0 : 0, i1 - i0});
+                nn->tot_extra += i1 - i0;
+            }
+        }
+        for (Cluster* nn : interfaces[current_bottom]) {
+            if (nn->tot_extra == 0) continue;
+            std::sort(nn->extra_from.begin(), nn->extra_from.end());
+            std::vector<Edge*> grow{nn->eself};
+            for (Edge* e : nn->edgesIn)
+                if (e->n1->lvl() != e->n1->merge_level) grow.push_back(e);
+            for (Edge* e : grow) {
+                Cluster* n1 = e->n1;
+                Mat G(e->A.r + nn->tot_extra, e->A.c);
+                G.set_block(0, 0, e->A);
+                int row = e->A.r;
+                for (const auto& t : nn->extra_from) {
+                    Cluster* c = by_order(t[0]);
+                    const int rinc = t[2];
+                    Edge* src = n1->find_out(c);
+                    if (src)
+                        for (int i = 0; i < rinc; ++i) {
+                            const int sr = c->cols() + c->row_perm[t[1] + i];
+                            for (int j = 0; j < G.c; ++j) G(row + i, j) = src->A(sr, j);
+                        }
+                    row += rinc;
+                }
+                e->A = std::move(G);
+            }
+            nn->reset_size(nn->rows() + nn->tot_extra, nn->cols());
+        }
+    }
+
     void scale(int ilvl) {
         auto& itf = interfaces[current_bottom];
         const int nf = (int)itf.size();
@@ -787,6 +926,7 @@ struct Tree {
             if (!want_sparsify(c)) return;
             Edge* e = c->eself;
             int k = c->cols(), m = c->rows();
+            if (m < k) throw std::runtime_error("a scaled interface has fewer rows than columns");
             c->Qs = e->A;
             c->Ts = Mat(k, k);
             geqrt(m, k, k, c->Qs.data(), m, c->Ts.data(), k);
@@ -860,13 +1000,49 @@ struct Tree {
             larft(k, rank, c->Qsp.data(), k, c->tausp.data(), c->Tsp.data(), rank);
             c->sparsified = true;
             c->rank = rank;
-            c->eself->A = Mat(rank, rank);
+            int row_rank = 0;
+            if (!square()) {
+                // sparsify_extra (src/ptree.cpp:441-493): the same RRQR on the extra rows of
+                // the cluster's row blocks, C_ex = [A(c, m)(k:rows, :) ...] over sparsIn
+                const int ex = c->rows() - k;
+                int spm = 0;
+                for (Edge* e : c->sparsIn) spm += e->A.c;
+                Mat Ce(ex, spm);
+                int js = 0;
+                for (Edge* e : c->sparsIn) {
+                    for (int j = 0; j < e->A.c; ++j)
+                        for (int i = 0; i < ex; ++i) Ce(i, js + j) = e->A(k + i, j);
+                    js += e->A.c;
+                }
+                const int mne = std::min(ex, spm);
+                std::vector<int> jp(std::max(spm, 1));
+                std::vector<double> te(std::max(mne, 1), 0.0);
+                if (ex > 0 && spm > 0) {
+                    geqp3(ex, spm, Ce.data(), std::max(ex, 1), jp.data(), te.data(), kPivotTieRel);
+                    std::vector<double> rii(mne);
+                    for (int i = 0; i < mne; ++i) rii[i] = Ce(i, i);
+                    row_rank = choose_rank(rii.data(), mne, tol);
+                } else {
+                    row_rank = choose_rank(nullptr, 0, tol);
+                }
+                if (g_fault == 7 && row_rank > 1) row_rank--;
+                tr[ii].push_back({ilvl, 8, ex, spm, row_rank});
+                c->nex = ex;
+                c->Qex = Ce.block(0, 0, ex, row_rank);
+                c->tauex.assign(te.begin(), te.begin() + row_rank);
+                c->Tex = Mat(row_rank, row_rank);
+                larft(ex, row_rank, c->Qex.data(), std::max(ex, 1), c->tauex.data(), c->Tex.data(), std::max(row_rank, 1));
+            }
+            // the pivot block becomes [I_rank; 0] ((rank + row_rank) x rank): src/ptree.cpp:435-438
+            // (square) and sparsify_resize_tf :1835-1850 (rectangular)
+            c->eself->A = Mat(rank + row_rank, rank);
             for (int i = 0; i < rank; ++i) c->eself->A(i, i) = 1.0;
-            c->reset_size(rank, rank);
+            c->reset_size(rank + row_rank, rank);
         });
         for (int ii = 0; ii < nf; ++ii)
             for (const auto& r : tr[ii]) {
                 trace.push_back(r);
+                if (r.op != 6) continue;
                 ranks_before[ilvl].push_back(r.m);
                 ranks_after[ilvl].push_back(r.k);
             }
@@ -882,13 +1058,21 @@ struct Tree {
                                         Acopy.data(), Acopy.r);
                     e->A = Acopy;
                 }
-                e->A.conservative_resize(c->rank, n->rank);
+                e->A.conservative_resize(c->rank + c->row_rank, n->rank);
                 if (want_sparsify(c) && g_fault != 4) {
                     Mat A11 = Acopy.block(0, 0, c->Qsp.r, n->rank);
                     larfb_left_trans(A11.r, A11.c, c->Qsp.c, c->Qsp.data(), c->Qsp.r, c->Tsp.data(), c->Tsp.r,
                                      A11.data(), A11.r);
                     for (int j = 0; j < n->rank; ++j)
                         for (int i = 0; i < c->rank; ++i) e->A(i, j) = A11(i, j);
+                    if (!square()) {  // the extra rows: Q_ex^T A(c,n)(k:rows, :), src/ptree.cpp:1891-1894
+                        Mat A21 = Acopy.block(c->Qsp.r, 0, c->nex, n->rank);
+                        if (c->row_rank > 0)
+                            larfb_left_trans(A21.r, A21.c, c->row_rank, c->Qex.data(), c->Qex.r, c->Tex.data(),
+                                             c->Tex.r, A21.data(), A21.r);
+                        for (int j = 0; j < n->rank; ++j)
+                            for (int i = 0; i < c->row_rank; ++i) e->A(c->rank + i, j) = A21(i, j);
+                    }
                 }
                 if (want_sparsify(n) || want_sparsify(c)) ta[ii].push_back({ilvl, 7, Acopy.r, Acopy.c, 0});
             }
@@ -941,11 +1125,32 @@ struct Tree {
     }
 
     // ---------------------------------------------------------- solve (preconditioner)
-    void solve(double* xv) {  // src/ptree.cpp:2788-3210
-        for (Cluster* c : bottoms[0]) {
-            c->x.assign(xv + c->cstart, xv + c->cstart + c->csize_org);
-        }
-        // forward, level by level
+    // splitD_fwd / splitD_bwd (src/ptree.cpp:2223-2242): the kept rows of a sparsified
+    // rectangular cluster, x = [x(0:rank); x_extra; x_fine] and back.
+    static void split_fwd(Cluster* c) {
+        const int x1 = c->csize_org - c->rank, x2 = (int)c->x.size() - c->csize_org;
+        std::vector<double> f(c->x.begin() + c->rank, c->x.begin() + c->csize_org);
+        std::vector<double> e(c->x.begin() + c->csize_org, c->x.end());
+        std::copy(e.begin(), e.end(), c->x.begin() + c->rank);
+        std::copy(f.begin(), f.end(), c->x.begin() + c->rank + x2);
+        (void)x1;
+    }
+    static void split_bwd(Cluster* c) {
+        const int x1 = c->csize_org - c->rank, x2 = (int)c->x.size() - c->csize_org;
+        std::vector<double> e(c->x.begin() + c->rank, c->x.begin() + c->rank + x2);
+        std::vector<double> f(c->x.begin() + c->rank + x2, c->x.begin() + c->rank + x2 + x1);
+        std::copy(f.begin(), f.end(), c->x.begin() + c->rank);
+        std::copy(e.begin(), e.end(), c->x.begin() + c->csize_org);
+    }
+    // every cluster's vector at its full length: rows at creation plus the extra rows it receives
+    void size_vectors() {
+        for (auto& up : owned_c) up->x.assign(up->rsize_org + up->tot_extra, 0.0);
+    }
+
+    // Forward half of solve(b, x) (src/ptree.cpp:2267-2598): b (m entries, permuted rows).
+    void solve_fwd(const double* b) {
+        size_vectors();
+        for (Cluster* c : bottoms[0]) std::copy(b + c->rstart, b + c->rstart + c->rsize_org, c->x.begin());
         for (int L = 0; L < nlevels; ++L) {
             if (L > 0)
                 for (Cluster* p : bottoms[L]) {  // merge_fwd :2244-2252
@@ -954,8 +1159,8 @@ struct Tree {
                         for (int i = 0; i < ch->rows(); ++i) p->x[k++] = ch->x[i];
                 }
             for (Cluster* c : interiors[L]) {  // fwd_geqrt :2846-2851
-                int m = c->eself->A.r, k = c->eself->A.c;
-                gemqrt_lt(m, 1, k, k, c->eself->A.data(), m, c->T.data(), k, c->x.data(), m);
+                int m2 = c->eself->A.r, k = c->eself->A.c;
+                gemqrt_lt(m2, 1, k, k, c->eself->A.data(), m2, c->T.data(), k, c->x.data(), m2);
             }
             for (Cluster* c : update_order[L]) {  // fwd_tsqrt :2160-2186, order_of_trans = colour order
                 int k = c->cols(), nb = std::min(32, k);
@@ -967,17 +1172,35 @@ struct Tree {
                               e->A.r);
                 }
             }
-            for (Cluster* c : interfaces[L]) {  // fwd_spars :2957-2963
+            if (!square())
+                for (Cluster* nn : interfaces[L]) {  // fwd_reassign :2395-2431 and merge_x (src/cluster.cpp:321-328)
+                    int row = nn->rsize_org;
+                    for (const auto& t : nn->extra_from) {
+                        Cluster* c = by_order(t[0]);
+                        for (int i = 0; i < t[2]; ++i) nn->x[row + i] = c->x[c->cols() + c->row_perm[t[1] + i]];
+                        row += t[2];
+                    }
+                }
+            for (Cluster* c : interfaces[L]) {  // fwd_spars :2957-2963 (+ :2529-2533)
                 if (!want_sparsify(c)) continue;
-                int m = c->Qs.r, k = c->Qs.c;
-                gemqrt_lt(m, 1, k, k, c->Qs.data(), m, c->Ts.data(), k, c->x.data(), m);
+                int m2 = c->Qs.r, k = c->Qs.c;
+                gemqrt_lt(m2, 1, k, k, c->Qs.data(), m2, c->Ts.data(), k, c->x.data(), m2);
+                if (!square() && g_fault != 8)  // orthogonalD_extra_fwd :2206-2211
+                    ormqr_lt_vec(c->nex, c->row_rank, c->Qex.data(), std::max(c->nex, 1), c->tauex.data(),
+                                 c->x.data() + c->csize_org);
                 ormqr_lt_vec(c->csize_org, c->Qsp.c, c->Qsp.data(), c->Qsp.r, c->tausp.data(), c->x.data());
+                if (!square()) split_fwd(c);
             }
         }
-        // backward
+    }
+
+    // Backward half (src/ptree.cpp:2600-3210) from the vectors as they stand; x (n entries,
+    // permuted columns) gets each leaf's first cols_org entries.
+    void solve_bwd(double* xv) {
         for (int L = nlevels - 1; L >= 0; --L) {
             for (Cluster* c : interfaces[L]) {  // bwd_spars :3146-3174
                 if (want_sparsify(c)) {
+                    if (!square()) split_bwd(c);
                     ormqr_ln_vec(c->csize_org, c->Qsp.c, c->Qsp.data(), c->Qsp.r, c->tausp.data(), c->x.data());
                     trsv_upper(c->csize_org, c->Qs.data(), c->Qs.r, c->x.data());
                 }
@@ -986,8 +1209,8 @@ struct Tree {
                 for (Edge* e : c->edgesOut) {
                     Cluster* n2 = e->n2;
                     if (n2->lvl() != c->merge_level) continue;
-                    int s = e->A.c, k2 = n2->cols();
-                    for (int j = 0; j < s; ++j) {
+                    int s2 = e->A.c, k2 = n2->cols();
+                    for (int j = 0; j < s2; ++j) {
                         double xj = c->x[j];
                         for (int i = 0; i < k2; ++i) n2->x[i] -= e->A(i, j) * xj;
                     }
@@ -1005,10 +1228,73 @@ struct Tree {
             for (int i = 0; i < c->csize_org; ++i) xv[c->cstart + i] = c->x[i];
     }
 
-    void spmv(const double* x, double* y) const {  // src/ptree.cpp:3212-3262
-        for (int i = 0; i < n; ++i) y[i] = 0.0;
+    // ParTree::solve(b, x) (src/ptree.cpp:2267): b has m entries (permuted rows), x n entries
+    // (permuted columns); square systems may pass the same array.
+    void solve(const double* b, double* x) {
+        solve_fwd(b);
+        solve_bwd(x);
+    }
+    void solve(double* xv) { solve(xv, xv); }
+
+    // The transpose of solve_bwd: y (n entries, permuted columns) into the vectors, then level by
+    // level upwards the transposes of the backward operations in reverse order -- the parents
+    // gather their children's first cols entries (tMerge::fwd, src/toperations.cpp:88-96), the
+    // interiors solve with R^T and subtract R_cn^T x_c from each in-neighbour (tQR::fwd :12-27),
+    // the sparsified interfaces solve with R_s^T (tScaleD::fwd :71-74), apply Q_sp^T
+    // (tOrthogonalD::fwd :79-81) and, for a rectangular system, splitD_fwd with zero extra
+    // entries (nothing reads them on the way back).
+    void solve_bwd_transpose(const double* y) {
+        size_vectors();
+        for (Cluster* c : bottoms[0]) std::copy(y + c->cstart, y + c->cstart + c->csize_org, c->x.begin());
+        for (int L = 0; L < nlevels; ++L) {
+            if (L > 0)
+                for (Cluster* p : bottoms[L]) {
+                    int k = 0;
+                    for (Cluster* ch : p->children)
+                        for (int i = 0; i < ch->cols(); ++i) p->x[k++] = ch->x[i];
+                }
+            for (Cluster* c : interiors[L]) trsv_upper_trans(c->cols(), c->eself->A.data(), c->eself->A.r, c->x.data());
+            for (Cluster* c : interfaces[L])
+                for (Edge* e : c->edgesOut) {
+                    Cluster* n2 = e->n2;
+                    if (n2->lvl() != c->merge_level) continue;
+                    const int s2 = e->A.c, k2 = n2->cols();
+                    for (int j = 0; j < s2; ++j) {
+                        double acc = 0.0;
+                        for (int i = 0; i < k2; ++i) acc += e->A(i, j) * n2->x[i];
+                        c->x[j] -= acc;
+                    }
+                }
+            for (Cluster* c : interfaces[L]) {
+                if (!want_sparsify(c)) continue;
+                trsv_upper_trans(c->csize_org, c->Qs.data(), c->Qs.r, c->x.data());
+                ormqr_lt_vec(c->csize_org, c->Qsp.c, c->Qsp.data(), c->Qsp.r, c->tausp.data(), c->x.data());
+                if (!square()) {
+                    std::fill(c->x.begin() + c->csize_org, c->x.end(), 0.0);
+                    split_fwd(c);
+                }
+            }
+        }
+    }
+
+    // solve_nrml (src/tree.cpp:1524-1570) on this tree: x = M^{-1} M^{-T} y, M^{-1} the backward
+    // half of the solve -- an approximate (A^T A)^{-1} y for CGLS.
+    void solve_nrml(const double* y, double* x) {
+        solve_bwd_transpose(y);
+        solve_bwd(x);
+    }
+
+    void spmv(const double* x, double* y) const {  // src/ptree.cpp:3212-3262: y (m) = App x (n)
+        for (int i = 0; i < m; ++i) y[i] = 0.0;
         for (int j = 0; j < n; ++j)
             for (int k = App.colptr[j]; k < App.colptr[j + 1]; ++k) y[App.rowind[k]] += App.val[k] * x[j];
+    }
+    void spmv_t(const double* r, double* y) const {  // y (n) = App^T r (m), the adjoint in cgls
+        for (int j = 0; j < n; ++j) {
+            double s = 0.0;
+            for (int k = App.colptr[j]; k < App.colptr[j + 1]; ++k) s += App.val[k] * r[App.rowind[k]];
+            y[j] = s;
+        }
     }
 };
 
@@ -1085,6 +1371,41 @@ static GmresResult gmres_cycle(Tree& t, const std::vector<double>& b, std::vecto
     return {k, err * r_norm};
 }
 
+// cgls (src/is.cpp:274-345) with x0 = 0 and the tree's solve_nrml as the preconditioner:
+// CG on A^T A x = A^T b, stopping when ||A^T (b - A x)||^2 < tol^2 ||A^T b||^2.  Returns the
+// iteration count as the reference counts it; *rel = ||A^T r|| / ||A^T b||.
+static int cgls(Tree& t, const double* b, double* x, int max_iters, double tol, double* rel) {
+    const int m = t.m, n = t.n;
+    std::vector<double> r(b, b + m), s(n), p(n), z(n), q(m);
+    std::fill(x, x + n, 0.0);
+    t.spmv_t(b, s.data());
+    const double rhs2 = dot(s, s.data());
+    if (rhs2 == 0.0) { *rel = 0.0; return 0; }
+    const double threshold = tol * tol * rhs2;
+    double res2 = rhs2;
+    if (res2 < threshold) { *rel = std::sqrt(res2 / rhs2); return 0; }
+    t.solve_nrml(s.data(), p.data());
+    double abs_new = dot(s, p.data());
+    int i = 0;
+    while (i < max_iters) {
+        t.spmv(p.data(), q.data());
+        const double alpha = abs_new / dot(q, q.data());
+        for (int j = 0; j < n; ++j) x[j] += alpha * p[j];
+        for (int j = 0; j < m; ++j) r[j] -= alpha * q[j];
+        t.spmv_t(r.data(), s.data());
+        res2 = dot(s, s.data());
+        if (res2 < threshold) break;
+        t.solve_nrml(s.data(), z.data());
+        const double abs_old = abs_new;
+        abs_new = dot(s, z.data());
+        const double beta = abs_new / abs_old;
+        for (int j = 0; j < n; ++j) p[j] = z[j] + beta * p[j];
+        ++i;
+    }
+    *rel = std::sqrt(res2 / rhs2);
+    return i;
+}
+
 }  // namespace orc
 
 // ====================================================================== C API (ctypes)
@@ -1097,6 +1418,7 @@ void* orc_tree_create(int n, const int* colptr, const int* rowind, const double*
     Tree* t = new Tree(nlevels, skip);
     t->tol = tol;
     t->n = n;
+    t->m = n;
     t->A.nr = t->A.nc = n;
     t->A.colptr.assign(colptr, colptr + n + 1);
     t->A.rowind.assign(rowind, rowind + colptr[n]);
@@ -1116,7 +1438,62 @@ void* orc_tree_create_parts(int n, const int* colptr, const int* rowind, const d
     return t;
 }
 
+// A rectangular system (m x n, m >= n: least squares) ordered by given leaf parts.
+void* orc_tree_create_rect(int m, int n, const int* colptr, const int* rowind, const double* val, const int* parts,
+                           int nlevels, double tol, int skip) {
+    Tree* t = new Tree(nlevels, skip);
+    t->tol = tol;
+    t->n = n;
+    t->m = m;
+    t->A.nr = m;
+    t->A.nc = n;
+    t->A.colptr.assign(colptr, colptr + n + 1);
+    t->A.rowind.assign(rowind, rowind + colptr[n]);
+    t->A.val.assign(val, val + colptr[n]);
+    t->given_parts.assign(parts, parts + n);
+    return t;
+}
+
 void orc_tree_destroy(void* h) { delete (Tree*)h; }
+
+int orc_nrows(void* h) { return ((Tree*)h)->m; }
+void orc_get_rperm(void* h, int* out) {
+    Tree* t = (Tree*)h;
+    std::memcpy(out, t->rperm.data(), sizeof(int) * t->m);
+}
+// Per cluster (order): rstart, rows at creation, extra rows received, row_rank, and for an
+// interior the number of its extra rows that went to another cluster.
+void orc_get_rows(void* h, int* out /* 5 * ncl */) {
+    Tree* t = (Tree*)h;
+    for (auto& up : t->owned_c) {
+        Cluster* c = up.get();
+        int* o = out + 5 * c->order;
+        o[0] = c->rstart; o[1] = c->rsize_org; o[2] = c->tot_extra; o[3] = c->row_rank;
+        int moved = 0;
+        for (int v : c->r2c) moved += (v != c->order);
+        o[4] = moved;
+    }
+}
+// Extra-row targets of an interior (cluster order per extra row, in row order); returns the count.
+int orc_get_r2c(void* h, int order, int* out, int cap) {
+    Tree* t = (Tree*)h;
+    Cluster* c = t->owned_c[order].get();
+    const int ex = (int)c->r2c.size();
+    if (ex > cap) return -1;
+    std::copy(c->r2c.begin(), c->r2c.end(), out);
+    return ex;
+}
+void orc_solve_ls(void* h, const double* b, double* x) { ((Tree*)h)->solve(b, x); }
+void orc_solve_nrml(void* h, const double* y, double* x) { ((Tree*)h)->solve_nrml(y, x); }
+void orc_spmv_t(void* h, const double* r, double* y) { ((Tree*)h)->spmv_t(r, y); }
+int orc_cgls(void* h, const double* b, double* x, int iters, double tol, double* rel) {
+    try {
+        return cgls(*(Tree*)h, b, x, iters, tol, rel);
+    } catch (std::exception& e) {
+        fprintf(stderr, "oracle: %s\n", e.what());
+        return -1;
+    }
+}
 
 int orc_partition_assemble(void* h) {
     try {

@@ -277,6 +277,38 @@ def make_batch(count: int = 48, sides=BATCH_SIDES, seed0: int = 3, tol: float = 
                  sizes, levels, tol)
 
 
+def gradient_rows(n: int, d: int) -> sp.csc_matrix:
+    """Forward differences x(i + e_ax) - x(i) on every interior face of the n^d grid, axis by
+    axis, faces in natural order: (d n^(d-1) (n-1)) x n^d."""
+    N = n ** d
+    idx = np.arange(N).reshape((n,) * d)
+    rows, cols, vals = [], [], []
+    r0 = 0
+    for ax in range(d):
+        lo = idx.take(np.arange(n - 1), axis=ax).ravel()
+        hi = idx.take(np.arange(1, n), axis=ax).ravel()
+        k = np.arange(r0, r0 + lo.size)
+        rows += [k, k]
+        cols += [hi, lo]
+        vals += [np.ones(lo.size), -np.ones(lo.size)]
+        r0 += lo.size
+    return sp.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(r0, N))
+
+
+def least_squares(n: int, d: int, seed: int = 11, lam: float = 0.5, tol: float = 1e-2, nlevels: int | None = None) -> Problem:
+    """An overdetermined system A = [K; lam G] (m > n): K the variable-coefficient diffusion
+    operator of config 2's kind on the n^d grid (coefficients exp(N(0,1)) from seed), G the
+    gradient rows (gradient_rows), then the driver's column scaling A <- A D 10.  b ~ U[-1, 1]
+    (m entries) from std::mt19937(seed + 1).  Solved in the least-squares sense."""
+    K = diffusion_varcoef(n, d, seed=seed, sigma=1.0)
+    A = sp.vstack([K, lam * gradient_rows(n, d)], format="csc")
+    A.sort_indices()
+    m, N = A.shape
+    lv = nlevels if nlevels is not None else max(2, default_levels(N))
+    return Problem("ls_diffusion%dd_%d" % (d, n), column_scale(A), mt_uniform(seed + 1, m), n, d, lv, tol,
+                   meta={"m": m})
+
+
 def default_levels(ncols: int) -> int:
     """nlevels = ceil(log2(ncols/64)) with integer division (pspaQR.cpp:108-113)."""
     return int(math.ceil(math.log2(ncols // 64)))
