@@ -73,7 +73,8 @@ int spaqr_tree_get_parts(const spaqr_tree* t, int* out, int cap);
 /* Factor precision: 64 (reference FP64 path) or 32 (FP32 fronts; GMRES stays FP64).
    No counterpart in the reference.  Set before spaqr_tree_assemble. */
 int spaqr_tree_set_precision(spaqr_tree* t, int bits);
-/* ParTree::partition(SpMat& A): A is n x n CSC (host), sorted row indices. src/ptree.cpp:495 */
+/* ParTree::partition(SpMat& A): A is nrows x ncols CSC (host), sorted row indices, nrows >= ncols
+   (nrows > ncols: a least-squares system, below).                          src/ptree.cpp:495 */
 int spaqr_tree_partition(spaqr_tree* t, int nrows, int ncols, const int* colptr, const int* rowind,
                          const double* val);
 /* A batch of independent square systems factored together (no reference counterpart: the
@@ -187,6 +188,37 @@ int spaqr_fgmres(spaqr_tree* t, const double* b_dev, double* x_dev, int iters, d
                  double true_tol, spaqr_gmres_result* out, double* hist, int hist_cap, int* hist_len);
 /* Tree::rows() / cols() and ParTree::nrows_local / ncols_local (one rank)  src/tree.cpp:23 */
 int spaqr_tree_n(const spaqr_tree* t);
+int spaqr_tree_m(const spaqr_tree* t);
+
+/* ---- Rectangular systems (least squares, m > n).  spaqr_tree_partition accepts nrows > ncols
+   without coordinates (leaf parts or the graph bisection; the reference's geometric path assigns
+   rows by HSL's bipartite matching, src/partition.cpp:103-134, absent here): each row is filed
+   under a cluster by form_rmap (src/tree.cpp:102-165) and the factorization reassigns the extra
+   rows of every eliminated interior (src/ptree.cpp:1149-1356) and compresses them
+   (sparsify_extra :441-493).  GMRES and spaqr_tree_solve are square-only. */
+/* row permutation: natural index of permuted row i (include/tree.h:53)                        */
+int spaqr_tree_get_rperm(const spaqr_tree* t, int* out);
+/* host vectors of m entries between the natural and the permuted row ordering (as permute) */
+int spaqr_tree_permute_rows(const spaqr_tree* t, int to_natural, const double* in, double* out);
+/* per cluster, 5 ints: first permuted row, rows at creation, extra rows received, rows kept
+   beyond the columns (rows - cols; after sparsify_extra its rank), an interior's extra rows that
+   went to a neighbour (include/cluster.h:153, 224-233)                                        */
+int spaqr_tree_get_rows(const spaqr_tree* t, int* out);
+/* ParTree::solve(VectorXd b, VectorXd& x): b (m, permuted rows) -> x (n, permuted columns),
+   device vectors.  For a square system it is spaqr_tree_solve out of place.  src/ptree.cpp:2267 */
+int spaqr_tree_solve_ls(spaqr_tree* t, const double* b_dev, double* x_dev);
+/* solve_nrml (src/tree.cpp:1524-1570, the sequential Tree's; ParTree has none): x = M^{-1}
+   M^{-T} y with M^{-1} the backward half of the solve, the transposed operations of
+   include/toperations.h:14-195 on the way up -- an approximate (A^T A)^{-1} y.  n-vectors,
+   permuted columns, device.                                                                  */
+int spaqr_tree_solve_nrml(spaqr_tree* t, const double* y_dev, double* x_dev);
+/* y = A^T r (r m entries, y n; permuted orderings, device): mat.adjoint() in src/is.cpp:294-321 */
+int spaqr_tree_spmv_t(spaqr_tree* t, const double* r_dev, double* y_dev);
+/* cgls(A, b, x, M, iters, tol) with M = the tree's solve_nrml and x0 = 0: CG on the normal
+   equations until ||A^T (b - A x)|| < tol ||A^T b||.  *iters_out is the reference's count,
+   *rel = ||A^T r|| / ||A^T b||.  b (m) and x (n) device, permuted orderings.  src/is.cpp:274-345 */
+int spaqr_cgls(spaqr_tree* t, const double* b_dev, double* x_dev, int iters, double tol, int* iters_out,
+               double* rel);
 /* column permutation (cperm, natural index of permuted position i)        include/tree.h:53 */
 int spaqr_tree_get_cperm(const spaqr_tree* t, int* out);
 /* cluster table, 13 ints per cluster in creation order: merge level, self (lvl, sep), l (lvl,

@@ -97,6 +97,14 @@ EXPORTS = {
     "spaqr_gmres": (_i, [_vp, _vp, _vp, _i, _d, _i, _d, C.POINTER(GmresResult), _vp, _i, C.POINTER(_i)]),
     "spaqr_fgmres": (_i, [_vp, _vp, _vp, _i, _d, _i, _d, C.POINTER(GmresResult), _vp, _i, C.POINTER(_i)]),
     "spaqr_tree_n": (_i, [_vp]),
+    "spaqr_tree_m": (_i, [_vp]),
+    "spaqr_tree_get_rperm": (_i, [_vp, _vp]),
+    "spaqr_tree_permute_rows": (_i, [_vp, _i, _vp, _vp]),
+    "spaqr_tree_get_rows": (_i, [_vp, _vp]),
+    "spaqr_tree_solve_ls": (_i, [_vp, _vp, _vp]),
+    "spaqr_tree_solve_nrml": (_i, [_vp, _vp, _vp]),
+    "spaqr_tree_spmv_t": (_i, [_vp, _vp, _vp]),
+    "spaqr_cgls": (_i, [_vp, _vp, _vp, _i, _d, C.POINTER(_i), C.POINTER(_d)]),
     "spaqr_tree_get_cperm": (_i, [_vp, _vp]),
     "spaqr_tree_num_clusters": (_i, [_vp]),
     "spaqr_tree_get_clusters": (_i, [_vp, _vp]),

@@ -128,7 +128,7 @@ int spaqr_tree_set_scale(spaqr_tree* h, int s) {
 }
 int spaqr_tree_set_square(spaqr_tree* h, int s) {
     TREE_OR_FAIL(h);
-    if (s != 1) { h->t->last_error = "only square systems are supported"; return -2; }
+    if (s != 0 && s != 1) { h->t->last_error = "set_square takes 0 (least squares, m > n) or 1"; return -2; }
     h->t->square = s;
     return 0;
 }
@@ -336,6 +336,56 @@ int spaqr_fgmres(spaqr_tree* h, const double* b, double* x, int iters, double to
     return rc;
 }
 int spaqr_tree_n(const spaqr_tree* h) { return (h && h->t) ? h->t->n_unknowns() : -1; }
+int spaqr_tree_m(const spaqr_tree* h) { return (h && h->t) ? h->t->n_rows() : -1; }
+int spaqr_tree_get_rperm(const spaqr_tree* h, int* out) {
+    TREE_OR_FAIL(h);
+    if (!out) return -2;
+    const auto& p = h->t->rperm();
+    std::memcpy(out, p.data(), sizeof(int) * p.size());
+    return 0;
+}
+int spaqr_tree_permute_rows(const spaqr_tree* h, int to_natural, const double* in, double* out) {
+    TREE_OR_FAIL(h);
+    if (!in || !out || in == out) return -3;
+    const auto& p = h->t->rperm();
+    if (to_natural)
+        for (size_t k = 0; k < p.size(); ++k) out[p[k]] = in[k];
+    else
+        for (size_t k = 0; k < p.size(); ++k) out[k] = in[p[k]];
+    return 0;
+}
+int spaqr_tree_get_rows(const spaqr_tree* h, int* out) {
+    TREE_OR_FAIL(h);
+    if (!out) return -2;
+    for (int o = 0; o < h->t->num_clusters(); ++o) {
+        const Cluster& c = h->t->cluster(o);
+        int* r = out + 5 * o;
+        int moved = 0;
+        for (int v : c.r2c) moved += (v != o);
+        r[0] = c.rstart; r[1] = c.rows_org; r[2] = c.tot_extra; r[3] = c.row_rank; r[4] = moved;
+    }
+    return 0;
+}
+int spaqr_tree_solve_ls(spaqr_tree* h, const double* b, double* x) {
+    TREE_OR_FAIL(h);
+    if (!b || !x) return -2;
+    return h->t->solve_ls_dev(b, x);
+}
+int spaqr_tree_solve_nrml(spaqr_tree* h, const double* y, double* x) {
+    TREE_OR_FAIL(h);
+    if (!y || !x) return -2;
+    return h->t->solve_nrml_dev(y, x);
+}
+int spaqr_tree_spmv_t(spaqr_tree* h, const double* r, double* y) {
+    TREE_OR_FAIL(h);
+    if (!r || !y) return -2;
+    return h->t->spmv_t_dev(r, y);
+}
+int spaqr_cgls(spaqr_tree* h, const double* b, double* x, int iters, double tol, int* iters_out, double* rel) {
+    TREE_OR_FAIL(h);
+    if (!b || !x || iters < 0 || !iters_out || !rel) return -2;
+    return h->t->cgls_dev(b, x, iters, tol, iters_out, rel);
+}
 int spaqr_tree_get_cperm(const spaqr_tree* h, int* out) {
     TREE_OR_FAIL(h);
     const auto& p = h->t->cperm();

@@ -3362,3 +3362,167 @@ template hipError_t launch_geqp3_coop<float>(const Geqp3CoopTask<float>*, const 
                                              size_t, bool, hipStream_t, bool);
 
 }  // namespace spaqr
+
+// =======================================================================================
+// Least squares (rectangular systems): the row bookkeeping of the row reassignment and the
+// transposed backward pass of the preconditioner (solve_nrml).  None of these is on the
+// square headline's path; each is one workgroup (or one thread) per item, bound by latency.
+// =======================================================================================
+namespace spaqr {
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// out[i] = || A(i, 0:cols) || in FP64: threads over rows (consecutive rows are consecutive in
+// a column, so each column's loads coalesce).
+template <typename S>
+__global__ __launch_bounds__(WG) void k_row_norms(const RowNormTask<S>* __restrict__ tasks) {
+    const RowNormTask<S> t = tasks[blockIdx.x];
+    for (int i = threadIdx.x; i < t.rows; i += WG) {
+        double s = 0.0;
+        for (int j = 0; j < t.cols; ++j) {
+            const double a = (double)t.A[i + (size_t)j * t.lda];
+            s += a * a;
+        }
+        t.out[i] = sqrt(s);
+    }
+}
+
+// dst = [old; the gathered rows]: the grown block of an edge after the row reassignment.
+template <typename S>
+__global__ __launch_bounds__(WG) void k_row_gather(const RowGatherTask<S>* __restrict__ tasks,
+                                                   const RowSrc<S>* __restrict__ src) {
+    const RowGatherTask<S> t = tasks[blockIdx.x];
+    const int rn = t.rows_old + t.nextra;
+    const long total = (long)rn * t.cols;
+    for (long e = threadIdx.x; e < total; e += WG) {
+        const int i = (int)(e % rn), j = (int)(e / rn);
+        S v;
+        if (i < t.rows_old) {
+            v = t.old[i + (size_t)j * t.lds];
+        } else {
+            const RowSrc<S> r = src[t.src0 + i - t.rows_old];
+            v = r.p ? r.p[(size_t)j * r.ld] : S(0);
+        }
+        t.dst[i + (size_t)j * t.ldd] = v;
+    }
+}
+
+// dst[di[q]] = si[q] < 0 ? 0 : src[si[q]]
+template <typename S>
+__global__ __launch_bounds__(WG) void k_vec_gather(const S* __restrict__ src, S* __restrict__ dst,
+                                                   const long long* __restrict__ si, const long long* __restrict__ di,
+                                                   int n) {
+    const int q = blockIdx.x * WG + threadIdx.x;
+    if (q >= n) return;
+    const long long s = si[q];
+    dst[di[q]] = s < 0 ? S(0) : src[s];
+}
+
+// x <- R^{-T} x for R upper triangular (k x k, ldr), x in global memory, 32-row blocks from the
+// top: the waves first subtract the solved entries' contributions from the block (one wave per
+// column of R, lanes along the column), then wave 0 solves the block's lower triangle R^T by
+// lane reads.
+template <typename S, int NT>
+__device__ void wg_trsv_upper_t(int k, const S* __restrict__ R, int ldr, S* x) {
+    constexpr int NW = NT / 64;
+    __shared__ double part[32];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int p0 = 0; p0 < k; p0 += NB) {
+        const int kb = min(NB, k - p0);
+        for (int jj = wave; jj < kb; jj += NW) {
+            const S* __restrict__ Rc = R + (size_t)(p0 + jj) * ldr;
+            double s = 0.0;
+            for (int i = lane; i < p0; i += 64) s += (double)Rc[i] * (double)x[i];
+            s = wave_sum(s);
+            if (lane == 0) part[jj] = s;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            const int l = min(lane, kb - 1);
+            S b = (lane < kb) ? x[p0 + l] - (S)part[l] : S(0);
+            for (int j = 0; j < kb; ++j) {
+                const S xj = lane_read(b, j) / R[(size_t)(p0 + j) * (ldr + 1)];
+                if (lane == j) b = xj;
+                else if (lane > j && lane < kb) b -= R[(p0 + j) + (size_t)(p0 + lane) * ldr] * xj;
+            }
+            if (lane < kb) x[p0 + lane] = b;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void k_vec_trsv_t(const VecTrsvTask<S>* __restrict__ tasks) {
+    const VecTrsvTask<S> t = tasks[blockIdx.x];
+    wg_trsv_upper_t<S, NT>(t.k, t.R, t.ldr, t.x);
+}
+
+// x[0:cols] -= sum over links of A(0:rows, 0:cols)^T y: one wave per column of x.
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void k_vec_gemvt(const VecGemvtTask<S>* __restrict__ tasks,
+                                                  const VecGemvtLink<S>* __restrict__ links) {
+    constexpr int NW = NT / 64;
+    const VecGemvtTask<S> t = tasks[blockIdx.x];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = wave; j < t.cols; j += NW) {
+        double s = 0.0;
+        for (int li = 0; li < t.nlinks; ++li) {
+            const VecGemvtLink<S> L = links[t.link0 + li];
+            const S* __restrict__ Aj = L.A + (size_t)j * L.lda;
+            for (int i = lane; i < L.rows; i += 64) s += (double)Aj[i] * (double)L.y[i];
+        }
+        s = wave_sum(s);
+        if (lane == 0) t.x[j] -= (S)s;
+    }
+}
+
+// The transpose of the sparsified interface's backward step: x[0:k] <- R_s^{-T} x[0:k] (R_s the
+// upper k x k part of Qs, ld ldq), then x[0:k] <- Qsp^T x[0:k].
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void k_vec_spars_t(const VecSparsTask<S>* __restrict__ tasks) {
+    __shared__ VecSmem<S, NT> sm;
+    const VecSparsTask<S> t = tasks[blockIdx.x];
+    wg_trsv_upper_t<S, NT>(t.k, t.Qs, t.ldq, t.x);
+    __syncthreads();
+    if (t.r > 0) wg_vec_q<S, NT>(t.k, t.r, t.Qsp, t.k, t.Tsp, t.x, true, sm);
+}
+
+template <typename S>
+void launch_row_norms(const RowNormTask<S>* d, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_row_norms<S>, dim3(n), dim3(WG), 0, s, d);
+}
+template <typename S>
+void launch_row_gather(const RowGatherTask<S>* d, const RowSrc<S>* src, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_row_gather<S>, dim3(n), dim3(WG), 0, s, d, src);
+}
+template <typename S>
+void launch_vec_gather(const S* src, S* dst, const long long* si, const long long* di, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_vec_gather<S>, dim3((n + WG - 1) / WG), dim3(WG), 0, s, src, dst, si, di, n);
+}
+template <typename S>
+void launch_vec_trsv_t(const VecTrsvTask<S>* d, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL((k_vec_trsv_t<S, WG>), dim3(n), dim3(WG), 0, s, d);
+}
+template <typename S>
+void launch_vec_gemvt(const VecGemvtTask<S>* d, const VecGemvtLink<S>* l, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL((k_vec_gemvt<S, WG>), dim3(n), dim3(WG), 0, s, d, l);
+}
+template <typename S>
+void launch_vec_spars_t(const VecSparsTask<S>* d, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL((k_vec_spars_t<S, WG>), dim3(n), dim3(WG), 0, s, d);
+}
+
+#define SPAQR_INSTANTIATE_LSQ(S)                                                                     \
+    template void launch_row_norms<S>(const RowNormTask<S>*, int, hipStream_t);                      \
+    template void launch_row_gather<S>(const RowGatherTask<S>*, const RowSrc<S>*, int, hipStream_t); \
+    template void launch_vec_gather<S>(const S*, S*, const long long*, const long long*, int, hipStream_t); \
+    template void launch_vec_trsv_t<S>(const VecTrsvTask<S>*, int, hipStream_t);                    \
+    template void launch_vec_gemvt<S>(const VecGemvtTask<S>*, const VecGemvtLink<S>*, int, hipStream_t); \
+    template void launch_vec_spars_t<S>(const VecSparsTask<S>*, int, hipStream_t);
+SPAQR_INSTANTIATE_LSQ(double)
+SPAQR_INSTANTIATE_LSQ(float)
+
+}  // namespace spaqr

@@ -93,4 +93,15 @@ template <typename S> struct VecGemvLink { const S* A; const S* y; int lda, s; }
 template <typename S> struct VecTrsvTask { const S* R; S* x; int k, ldr, link0, nlinks; };
 template <typename S> struct VecCopyTask { const S* src; S* dst; int n, pad; };
 
+// ---- least squares (rectangular systems)
+// out[i] = ||A(i, 0:cols)|| (FP64) for i < rows; A points at the first row, ld lda.
+template <typename S> struct RowNormTask { const S* A; double* out; int lda, rows, cols, pad; };
+// One gathered row: p at its first entry (stride ld), or nullptr for a zero row.
+template <typename S> struct RowSrc { const S* p; int ld, pad; };
+// dst ((rows_old + nextra) x cols, ldd) = [old (rows_old x cols, lds); rows src[src0 .. src0+nextra)].
+template <typename S> struct RowGatherTask { S* dst; const S* old; int ldd, lds, rows_old, cols, src0, nextra; };
+// x[0:cols] -= sum_l A_l(0:rows_l, 0:cols)^T y_l (ld lda_l): the transposed off-diagonal products.
+template <typename S> struct VecGemvtLink { const S* A; const S* y; int lda, rows; };
+template <typename S> struct VecGemvtTask { S* x; int cols, link0, nlinks, pad; };
+
 }  // namespace spaqr

@@ -78,6 +78,18 @@ hipError_t launch_geqp3_coop(const Geqp3CoopTask<S>* d_tasks, const int* d_blk2t
                              double tol, double tie_rel, size_t lds_bytes, bool hbm, hipStream_t s,
                              bool cooperative = true);
 
+// least squares (rectangular systems)
+template <typename S> void launch_row_norms(const RowNormTask<S>* d, int n, hipStream_t s);
+template <typename S> void launch_row_gather(const RowGatherTask<S>* d, const RowSrc<S>* src, int n, hipStream_t s);
+// dst[di[q]] = si[q] < 0 ? 0 : src[si[q]], q < n
+template <typename S>
+void launch_vec_gather(const S* src, S* dst, const long long* si, const long long* di, int n, hipStream_t s);
+// x <- R^{-T} x (the links of the task are not used)
+template <typename S> void launch_vec_trsv_t(const VecTrsvTask<S>* d, int n, hipStream_t s);
+template <typename S> void launch_vec_gemvt(const VecGemvtTask<S>* d, const VecGemvtLink<S>* l, int n, hipStream_t s);
+// x[0:k] <- Qsp^T R_s^{-T} x[0:k]
+template <typename S> void launch_vec_spars_t(const VecSparsTask<S>* d, int n, hipStream_t s);
+
 void launch_spmv(int n, const int* rowptr, const int* col, const double* val, const double* x, double* y,
                  hipStream_t s);
 void launch_dot(int n, const double* a, const double* b, double* partial, int nblocks, double* out, int sqrt_mode,

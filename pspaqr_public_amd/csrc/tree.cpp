@@ -146,16 +146,16 @@ HostCSC ata_pattern(const HostCSC& A) {
     return G;
 }
 
-HostCSC permute_rc(const HostCSC& A, const std::vector<int>& perm) {  // A(perm, perm)
+HostCSC permute_rc(const HostCSC& A, const std::vector<int>& rperm, const std::vector<int>& cperm) {  // A(rperm, cperm)
     std::vector<int> inv(A.nr);
-    for (int i = 0; i < A.nr; ++i) inv[perm[i]] = i;
+    for (int i = 0; i < A.nr; ++i) inv[rperm[i]] = i;
     HostCSC P;
     P.nr = A.nr;
     P.nc = A.nc;
     P.colptr.assign(A.nc + 1, 0);
     std::vector<std::pair<int, double>> col;
     for (int j = 0; j < A.nc; ++j) {
-        const int oj = perm[j];
+        const int oj = cperm[j];
         col.clear();
         for (int k = A.colptr[oj]; k < A.colptr[oj + 1]; ++k) col.emplace_back(inv[A.rowind[k]], A.val[k]);
         std::sort(col.begin(), col.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
@@ -395,8 +395,28 @@ struct Tree::SolveSchedule {
         void* trsv = nullptr; int n_trsv = 0;
         void* trsv_links = nullptr;
         void* merge_bwd = nullptr; int n_merge_bwd = 0;
+        // rectangular systems: the reassigned rows' gather into the interfaces (fwd_reassign +
+        // merge_x), Q_ex^T on the extra rows, and splitD as moves through the scratch region
+        // (forward: positions sp_a -> sp_b; backward: sp_b -> sp_a)
+        void* rg_si = nullptr; void* rg_di = nullptr; int n_rg = 0;
+        void* qex = nullptr; int n_qex = 0;
+        void* sp_a = nullptr; void* sp_b = nullptr; int n_sp = 0;
+        // the transposed backward pass (solve_nrml): children's first cols entries into the
+        // parents, transposed couplings per interface, and (rectangular) splitD_fwd with zero
+        // extra entries (ts_a: source position or -1)
+        void* merge_tr = nullptr; int n_merge_tr = 0;
+        void* gemvt = nullptr; int n_gemvt = 0;
+        void* gemvt_links = nullptr;
+        void* ts_a = nullptr; void* ts_b = nullptr; int n_ts = 0;
     };
     std::vector<Level> lv;
+    // rectangular systems: the level-0 loads of b (rows) and y (columns) and the store of x, as
+    // moves between the scratch region (m entries, the FP64 vector converted) and the segments
+    void* scratch = nullptr;
+    void* iota = nullptr;  // 0, 1, 2, ... (the scratch side of every move)
+    void* ld_si = nullptr; void* ld_di = nullptr; int n_ld = 0;
+    void* lc_si = nullptr; void* lc_di = nullptr; int n_lc = 0;
+    void* st_si = nullptr; void* st_di = nullptr; int n_st = 0;
 };
 
 Tree::Tree(int nl, int sk) : nlevels(nl), skip(sk) {
@@ -449,6 +469,9 @@ Tree::~Tree() {
     if (d_rowptr) (void)hipFree(d_rowptr);
     if (d_col) (void)hipFree(d_col);
     if (d_val) (void)hipFree(d_val);
+    if (d_tptr) (void)hipFree(d_tptr);
+    if (d_tcol) (void)hipFree(d_tcol);
+    if (d_tval) (void)hipFree(d_tval);
     if (d_scratch) (void)hipFree(d_scratch);
     if (d_blocks0) (void)hipFree(d_blocks0);
     if (xw) (void)hipFree(xw);
@@ -783,6 +806,7 @@ void Tree::make_hierarchy() {
     for (auto& v : bottoms) v.clear();
     for (auto& v : interiors) v.clear();
     for (auto& v : interfaces) v.clear();
+    int rnext = 0;
     for (size_t s = 0; s < inst_n.size(); ++s) {
         const int k1 = inst_off[s] + inst_n[s];
         for (int k = inst_off[s]; k < k1;) {
@@ -794,7 +818,14 @@ void Tree::make_hierarchy() {
             c.inst = (int)s;
             c.id = cpermed[k];
             c.cstart = k;
-            c.rows = c.cols = c.rows_org = c.cols_org = c.rank = kn - k;
+            c.rstart = rnext;
+            c.cols = c.cols_org = c.rank = kn - k;
+            int nr = 0;
+            for (int q = k; q < kn; ++q) nr += c2r_count[q];
+            if (nr == 0) throw std::runtime_error("a leaf cluster got no rows");
+            c.rows = c.rows_org = nr;
+            c.row_rank = nr - c.cols;
+            rnext += nr;
             cl.push_back(c);
             bottoms[0].push_back(c.order);
             (c.lvl() == 0 ? interiors[0] : interfaces[0]).push_back(c.order);
@@ -813,11 +844,12 @@ void Tree::make_hierarchy() {
                 for (size_t k = b; k < e0;) {
                     const ClusterID pid = merge_if(cl[prev[k]].id, lvl);
                     std::vector<int> ch;
-                    int sz = 0;
-                    const int cst = cl[prev[k]].cstart;
+                    int sz = 0, rz = 0;
+                    const int cst = cl[prev[k]].cstart, rst = cl[prev[k]].rstart;
                     while (k < e0 && merge_if(cl[prev[k]].id, lvl) == pid) {
                         ch.push_back(prev[k]);
                         sz += cl[prev[k]].cols;
+                        rz += cl[prev[k]].rows;
                         k++;
                     }
                     Cluster p;
@@ -826,7 +858,10 @@ void Tree::make_hierarchy() {
                     p.inst = s;
                     p.id = pid;
                     p.cstart = cst;
-                    p.rows = p.cols = p.rows_org = p.cols_org = p.rank = sz;
+                    p.rstart = rst;
+                    p.cols = p.cols_org = p.rank = sz;
+                    p.rows = p.rows_org = rz;
+                    p.row_rank = rz - sz;
                     std::sort(ch.begin(), ch.end(), [this](int a, int b2) { return cl[a].id < cl[b2].id; });
                     p.children = ch;
                     for (int c : ch) cl[c].parent = p.order;
@@ -841,10 +876,60 @@ void Tree::make_hierarchy() {
     cl_part = cl;
 }
 
+// form_rmap without a matching (src/tree.cpp:102-165; ParTree::partition :531-544): each row of
+// A(:, cperm) is filed under the cluster whose columns carry the largest sum of squares of the
+// row (the first in column order on a tie; a zero row goes to the last column), at that
+// cluster's first column the row touches; rperm lists the rows by that column, then by row.
+void Tree::form_rmap() {
+    c2r_count.assign(n, 1);
+    if (m == n) {
+        rperm_ = cperm_;
+        return;
+    }
+    std::vector<int> cinv(n);
+    for (int k = 0; k < n; ++k) cinv[cperm_[k]] = k;
+    const HostCSC At = csc_transpose(A0);  // rows of A, natural column indices
+    std::vector<int> filed(m);
+    std::vector<std::pair<int, double>> ent;
+    c2r_count.assign(n, 0);
+    for (int i = 0; i < m; ++i) {
+        ent.clear();
+        for (int k = At.colptr[i]; k < At.colptr[i + 1]; ++k) ent.emplace_back(cinv[At.rowind[k]], At.val[k]);
+        std::sort(ent.begin(), ent.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+        double best = -1.0;
+        int cidx = -1;
+        for (size_t q = 0; q < ent.size();) {
+            const ClusterID& cid = cpermed[ent[q].first];
+            double v = ent[q].second * ent[q].second;
+            size_t qn = q + 1;
+            while (qn < ent.size() && cpermed[ent[qn].first] == cid) {
+                v += ent[qn].second * ent[qn].second;
+                ++qn;
+            }
+            if (v > best) { best = v; cidx = ent[q].first; }
+            q = qn;
+        }
+        if (cidx < 0) cidx = n - 1;
+        filed[i] = cidx;
+        c2r_count[cidx]++;
+    }
+    std::vector<int> start(n + 1, 0);
+    for (int j = 0; j < n; ++j) start[j + 1] = start[j] + c2r_count[j];
+    rperm_.assign(m, 0);
+    for (int i = 0; i < m; ++i) rperm_[start[filed[i]]++] = i;
+}
+
 int Tree::partition(const HostCSC& A) {
     try {
-        if (A.nr != A.nc) throw std::runtime_error("only square systems are supported");
+        if (A.nr < A.nc) throw std::runtime_error("A has fewer rows than columns");
+        // the reference's driver sets square from the shape (pspaQR.cpp:192); its non-square
+        // path on a square A is not built here
+        if ((A.nr == A.nc) != (square == 1))
+            throw std::runtime_error(square ? "A is rectangular: set_square(0)" : "A is square: set_square(1)");
         const bool geo = !X.empty() && xdim > 0;
+        if (A.nr != A.nc && geo)
+            throw std::runtime_error("a rectangular system is ordered without coordinates (leaf parts or the graph "
+                                     "bisection): the geometric path assigns rows by a bipartite matching");
         if (geo && (long long)X.size() != (long long)xdim * A.nc) throw std::runtime_error("coordinate array size mismatch");
         for (double v : X)
             if (!std::isfinite(v)) throw std::runtime_error("coordinates must be finite");
@@ -852,11 +937,13 @@ int Tree::partition(const HostCSC& A) {
         check_matrix(A);
         A0 = A;
         n = A.nc;
+        m = A.nr;
         inst_n.assign(1, n);
         inst_lv.assign(1, nlevels);
         inst_off.assign(1, 0);
         order_system(A, geo ? X.data() : nullptr, xdim, (!geo && !user_parts.empty()) ? user_parts.data() : nullptr,
                      nlevels, cperm_, cpermed, &leaf_parts);
+        form_rmap();
         make_hierarchy();
         return 0;
     } catch (std::exception& e) {
@@ -924,6 +1011,8 @@ int Tree::partition_batch(const HostCSC& A, const std::vector<int>& sizes, const
             }
         });
         inst_off.pop_back();
+        m = n;
+        form_rmap();
         make_hierarchy();
         return 0;
     } catch (std::exception& e) {
@@ -938,13 +1027,15 @@ int Tree::assemble() {
         if (cl_part.empty()) throw std::runtime_error("partition() first");
         ensure_stream();
         cl = cl_part;  // a factorization since partition() has resized and flagged clusters
-        App = permute_rc(A0, cperm_);
+        App = permute_rc(A0, rperm_, cperm_);
         ed.clear();
         for (auto& c : cl) { c.out.clear(); c.in.clear(); c.sout.clear(); c.sin.clear(); c.self_edge = -1; c.dist2.clear(); }
         current_bottom = 0;
-        std::vector<int> owner(n);
-        for (int o : bottoms[0])
+        std::vector<int> owner(n), rowner(m);  // leaf cluster of each permuted column / row
+        for (int o : bottoms[0]) {
             for (int k = cl[o].cstart; k < cl[o].cstart + cl[o].cols; ++k) owner[k] = o;
+            for (int k = cl[o].rstart; k < cl[o].rstart + cl[o].rows; ++k) rowner[k] = o;
+        }
         // edges with dense blocks (src/ptree.cpp:728-775)
         h_blocks0.clear();
         h_block_off.clear();
@@ -952,7 +1043,7 @@ int Tree::assemble() {
             std::set<int> nb;
             const Cluster& cs = cl[s];
             for (int j = cs.cstart; j < cs.cstart + cs.cols; ++j)
-                for (int k = App.colptr[j]; k < App.colptr[j + 1]; ++k) nb.insert(owner[App.rowind[k]]);
+                for (int k = App.colptr[j]; k < App.colptr[j + 1]; ++k) nb.insert(rowner[App.rowind[k]]);
             nb.insert(s);
             for (int o : nb) {
                 int id = new_edge(s, o);
@@ -960,7 +1051,7 @@ int Tree::assemble() {
                 size_t off = h_blocks0.size();
                 h_block_off.push_back(off);
                 h_blocks0.resize(off + (size_t)e.rows * e.cols, 0.0);
-                const int r0 = cl[o].cstart;
+                const int r0 = cl[o].rstart;
                 for (int j = 0; j < e.cols; ++j) {
                     const int col = cs.cstart + j;
                     for (int k = App.colptr[col]; k < App.colptr[col + 1]; ++k) {
@@ -1007,13 +1098,28 @@ int Tree::assemble() {
         if (d_rowptr) (void)hipFree(d_rowptr);
         if (d_col) (void)hipFree(d_col);
         if (d_val) (void)hipFree(d_val);
-        HIPCHK(hipMalloc(&d_rowptr, sizeof(int) * (n + 1)));
+        HIPCHK(hipMalloc(&d_rowptr, sizeof(int) * (m + 1)));
         HIPCHK(hipMalloc(&d_col, sizeof(int) * std::max<size_t>(R.rowind.size(), 1)));
         HIPCHK(hipMalloc(&d_val, sizeof(double) * std::max<size_t>(R.val.size(), 1)));
-        HIPCHK(hipMemcpy(d_rowptr, R.colptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_rowptr, R.colptr.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d_col, R.rowind.data(), sizeof(int) * R.rowind.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d_val, R.val.data(), sizeof(double) * R.val.size(), hipMemcpyHostToDevice));
         d_nnz = (long long)R.val.size();
+        // the adjoint's CSR is App's CSC
+        if (d_tptr) (void)hipFree(d_tptr);
+        if (d_tcol) (void)hipFree(d_tcol);
+        if (d_tval) (void)hipFree(d_tval);
+        d_tptr = nullptr;
+        d_tcol = nullptr;
+        d_tval = nullptr;
+        if (rect()) {
+            HIPCHK(hipMalloc(&d_tptr, sizeof(int) * (n + 1)));
+            HIPCHK(hipMalloc(&d_tcol, sizeof(int) * std::max<size_t>(App.rowind.size(), 1)));
+            HIPCHK(hipMalloc(&d_tval, sizeof(double) * std::max<size_t>(App.val.size(), 1)));
+            HIPCHK(hipMemcpy(d_tptr, App.colptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(d_tcol, App.rowind.data(), sizeof(int) * App.rowind.size(), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(d_tval, App.val.data(), sizeof(double) * App.val.size(), hipMemcpyHostToDevice));
+        }
         if (!d_scratch) HIPCHK(hipMalloc(&d_scratch, sizeof(double) * (dot_blocks + 4096)));
         factorized = false;
         return 0;
@@ -1378,6 +1484,7 @@ int Tree::factorize_t() {
     // (0-2 one-wave by size, 3-8 C-resident, 9 workgroup) and from there, each bucket once,
     // into pinned staging and one upload.
     auto spars_bucket = [&](const SparsApplyTask<S>& t) -> int {
+        if (rect()) return 9;  // the generic kernel for the rectangular shapes (kc < mc, row offsets)
         if (small_tasks) {
             int d = -1;
             if (t.ident) d = t.rc <= 64 ? 16 : -1;
@@ -1443,6 +1550,7 @@ int Tree::factorize_t() {
             const int c = ints[i];
             Cluster& C = cl[c];
             if (C.cols == 0) throw std::runtime_error("cluster has zero cols");
+            if (C.rows < C.cols) throw std::runtime_error("an interior has fewer rows than columns");
             if (C.out.empty() || ed[C.out[0]].n2 != c) throw std::runtime_error("first out-edge of an interior must be itself");
             st.max_front = std::max(st.max_front, C.cols);
             C.T = arena.alloc((size_t)NB * C.cols * sizeof(S));
@@ -1680,6 +1788,8 @@ int Tree::factorize_t() {
             HIPCHK(hipGetLastError());
         }
         hacc(2, ht);
+        // ---- row reassignment of a rectangular system (src/ptree.cpp:1149-1356)
+        if (rect()) reassign_rows<S>(L);
         // ---- scaling descriptors (src/ptree.cpp:269-286, 1621-1633)
         std::vector<CopyTask<S>> scopy;
         std::vector<GeqrtTask<S>> sgeq;
@@ -1687,6 +1797,7 @@ int Tree::factorize_t() {
             Cluster& C = cl[c];
             if (!want_sparsify(C)) continue;
             Edge& es = E(C.self_edge);
+            if (es.rows < es.cols) throw std::runtime_error("a scaled interface has fewer rows than columns");
             C.Qs = arena.alloc((size_t)es.rows * es.cols * sizeof(S));
             C.Ts = arena.alloc((size_t)NB * es.cols * sizeof(S));
             scopy.push_back({Ap(C.self_edge), (S*)C.Qs, es.rows, es.cols, es.rows, es.rows, 0, 0});
@@ -1868,6 +1979,9 @@ int Tree::factorize_t() {
         std::vector<char> used_sketch;
         int* d_ranks = nullptr;
         int* d_rrqr_err = nullptr;
+        std::vector<CopyTask<S>> gatx;   // rectangular systems: the extra rows' RRQR
+        std::vector<Geqp3Task<S>> gqx;
+        int* d_xranks = nullptr;         // per sparsified interface (order of gq), 0 when no task
         // the level guard of src/ptree.cpp:1566,1635; want_sparsify applies each system's own
         // top-two-levels rule
         const bool spars_level = (L >= skip && tol != 0);
@@ -1904,6 +2018,35 @@ int Tree::factorize_t() {
                 if (rank_mode == 0) prof_add(P_GEQP3, 0.0, SZ * 3.0 * k * w);
                 gq_cluster.push_back(who[q]);
             }
+            // sparsify_extra (src/ptree.cpp:441-493): the same RRQR on each interface's extra
+            // rows, C_ex = [A(c, m)(k:rows, :) ...] over its row blocks, always by geqp3
+            if (rect() && !who.empty()) {
+                d_xranks = (int*)arena.alloc(sizeof(int) * who.size());
+                launch_fill<int>(d_xranks, who.size(), 0, stream_);
+                for (size_t q = 0; q < who.size(); ++q) {
+                    Cluster& C = cl[who[q]];
+                    const int k = C.cols, ex = C.rows - C.cols;
+                    int w = 0;
+                    for (int e : C.sin) w += ed[e].cols;
+                    C.nex = ex;
+                    if (ex <= 0 || w <= 0) continue;
+                    const int kmax = std::min(ex, w);
+                    S* Cb = (S*)arena.alloc((size_t)ex * w * sizeof(S));
+                    int js = 0;
+                    for (int e : C.sin) {
+                        gatx.push_back({Ap(e) + k, Cb + (size_t)js * ex, ex, ed[e].cols, ed[e].rows, ex, 0, 0});
+                        js += ed[e].cols;
+                    }
+                    C.Qex = arena.alloc((size_t)ex * kmax * sizeof(S));
+                    C.tauex = arena.alloc((size_t)kmax * sizeof(S));
+                    C.Tex = arena.alloc((size_t)NB * kmax * sizeof(S));
+                    S* vn = (S*)arena.alloc((size_t)2 * w * sizeof(S));
+                    int* jp = (int*)arena.alloc((size_t)w * sizeof(int));
+                    gqx.push_back({Cb, (S*)C.Qex, (S*)C.tauex, (S*)C.Tex, vn, vn + w, jp, d_xranks + q, ex, w, 0, 0});
+                    prof_add(P_GATHER, 0.0, SZ * 2.0 * ex * w);
+                    prof_add(P_GEQP3, 0.0, SZ * 3.0 * ex * w);
+                }
+            }
         }
         hacc(16, ht);
         {
@@ -1926,6 +2069,14 @@ int Tree::factorize_t() {
                 } else {
                     st.flops += rrqr_level<S>(gq, gat, gat_begin, gq_cluster, L, d_ranks, d_rrqr_err, used_sketch);
                 }
+                if (!gqx.empty()) {
+                    CopyTask<S>* d_gatx = upload_vec(gatx);
+                    timed(P_GATHER, 1, (long)gatx.size(), [&] { launch_copy<S>(d_gatx, (int)gatx.size(), stream_, copy_parts(gatx)); });
+                    int nd = 0;
+                    timed(P_GEQP3, 0, (long)gqx.size(),
+                          [&] { geqp3_grouped<S>(gqx, tol, kPivotTieRel, stream_, mem, d_rrqr_err, &nd); });
+                    if (profiling) prof[P_GEQP3].launches += nd;
+                }
             }
             HIPCHK(hipGetLastError());
         }
@@ -1936,9 +2087,11 @@ int Tree::factorize_t() {
                 hacc(2, ht);
                 merge_structure();  // needs no rank: runs while the device works too
             }
-            std::vector<int> ranks(gq.size());
+            std::vector<int> ranks(gq.size()), xranks(d_xranks ? gq.size() : 0, 0);
             int rrqr_err = 0;
             HIPCHK(hipMemcpyAsync(ranks.data(), d_ranks, sizeof(int) * ranks.size(), hipMemcpyDeviceToHost, stream_));
+            if (d_xranks)
+                HIPCHK(hipMemcpyAsync(xranks.data(), d_xranks, sizeof(int) * xranks.size(), hipMemcpyDeviceToHost, stream_));
             HIPCHK(hipMemcpyAsync(&rrqr_err, d_rrqr_err, sizeof(int), hipMemcpyDeviceToHost, stream_));
             sync_stream();
             if (rrqr_err) throw std::runtime_error("cooperative geqp3: a barrier wait exceeded its spin limit");
@@ -1957,7 +2110,11 @@ int Tree::factorize_t() {
                     }
                 C.sparsified = true;
                 C.rank = r;
-                C.rows = C.cols = r;  // reset_size (src/ptree.cpp:435-438)
+                // reset_size (src/ptree.cpp:435-438); a rectangular system keeps the extra rows' rank
+                // below the pivot block's [I_r; 0] (sparsify_resize_tf :1835-1850)
+                C.row_rank = d_xranks ? xranks[q] : 0;
+                C.cols = r;
+                C.rows = r + C.row_rank;
             }
             st.n_sparsified = (int)gq.size();
             if (graph_rec) {
@@ -2032,6 +2189,7 @@ int Tree::factorize_t() {
             cl[p].rows = cl[p].rows_org = rs;
             cl[p].cols = cl[p].cols_org = cs;
             cl[p].rank = cs;
+            cl[p].row_rank = rs - cs;
         }
         hacc(13, ht);
         parallel_ranges((int)(ed.size() - ne_merge), 4096, [&](int b, int e, int) {
@@ -2087,17 +2245,41 @@ int Tree::factorize_t() {
                     t.nc = eo.cols;
                     t.lda = eo.rows;
                     t.kn = eo.cols;
-                    t.kc = eo.rows;
+                    t.kc = NO.sparsified ? NO.cols_org : eo.rows;
                     t.rn = SO.cols;
-                    t.rc = NO.rows;
+                    t.rc = NO.sparsified ? NO.cols : NO.rows;
                     ++P.blocks;
                     if (no == so) {
                         t.ident = SO.sparsified ? 1 : 0;
+                        if (t.ident) t.rc = SO.cols;  // [I_r; 0]: the rows below are zero already
                         P.tasks[spars_bucket(t)].push_back(t);
                         continue;
                     }
                     if (SO.sparsified) { t.Vn = (const S*)SO.Qsp; t.Tn = (const S*)SO.Tsp; t.ldvn = eo.cols; }
-                    if (NO.sparsified) { t.Vc = (const S*)NO.Qsp; t.Tc = (const S*)NO.Tsp; t.ldvc = eo.rows; }
+                    if (NO.sparsified) { t.Vc = (const S*)NO.Qsp; t.Tc = (const S*)NO.Tsp; t.ldvc = NO.cols_org; }
+                    // the extra rows of a sparsified rectangular interface: Q_ex^T A(c, n)(k:rows, :),
+                    // kept below its rank rows (src/ptree.cpp:1891-1894)
+                    if (NO.sparsified && NO.row_rank > 0) {
+                        SparsApplyTask<S> x = t;
+                        x.A = t.A + NO.cols_org;
+                        x.mc = NO.nex;
+                        x.kc = NO.nex;
+                        x.Vc = (const S*)NO.Qex;
+                        x.Tc = (const S*)NO.Tex;
+                        x.ldvc = NO.nex;
+                        x.rc = NO.row_rank;
+                        x.dst = t.dst + NO.cols;
+                        x.Vn = nullptr;
+                        x.Tn = nullptr;
+                        for (int c0 = 0; c0 < x.rn; c0 += kSparsCols) {
+                            SparsApplyTask<S> u = x;
+                            u.A = x.A + (size_t)c0 * x.lda;
+                            u.dst = x.dst + (size_t)c0 * x.ldd;
+                            u.rn = std::min(kSparsCols, x.rn - c0);
+                            u.pad = 2;
+                            P.tasks[spars_bucket(u)].push_back(u);
+                        }
+                    }
                     const double fl = (SO.sparsified ? 4.0 * eo.rows * eo.cols * SO.cols : 0.0) +
                                       (NO.sparsified ? 4.0 * eo.rows * SO.cols * NO.cols : 0.0);
                     P.flops += fl;
@@ -2173,6 +2355,119 @@ int Tree::factorize_t() {
     }
     t_factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
+}
+
+// ---------------------------------------------------------------------------- row reassignment
+// A rectangular system's level step after the elimination (src/ptree.cpp:1149-1356 with
+// Cluster::update_extra_rnorm, src/cluster.cpp:151-160).  Rows cols(c) .. rows(c) of every
+// eliminated interior c's row blocks Q_c^T A(c, :) touch no eliminated column; each goes to the
+// out-neighbour m of c that is also an in-neighbour (is_cnbr) whose block holds the largest norm
+// of that row -- a strictly larger norm replaces, so on a tie the first neighbour in cluster order
+// keeps the row, and all-zero rows stay with c and are dropped.  row_perm stable-sorts the extra
+// rows by target.  Each interface then appends the rows it received, in interior order, to its
+// diagonal block and to the blocks of its in-edges from non-eliminated clusters (shift_tf +
+// Edge::merge, include/edge.h:49-57); a block whose source has no edge into c gets zero rows.
+// The row norms are the one device read-back; the grown blocks are one gather launch.
+template <typename S>
+void Tree::reassign_rows(int L) {
+    const auto& ints = interiors[L];
+    struct Cand { int c, m, e; size_t off; };
+    std::vector<Cand> cands;
+    std::vector<RowNormTask<S>> tasks;
+    size_t tot = 0;
+    for (int c : ints) {
+        Cluster& C = cl[c];
+        const int ex = C.rows - C.cols;
+        C.r2c.assign(std::max(ex, 0), c);
+        C.row_perm.clear();
+        if (ex <= 0) continue;
+        const size_t first = cands.size();
+        for (int e : C.in) {
+            const int mm = ed[e].n1;
+            if (find_out(C, mm) >= 0) cands.push_back({c, mm, e, 0});
+        }
+        std::sort(cands.begin() + first, cands.end(), [](const Cand& a, const Cand& b) { return a.m < b.m; });
+        for (size_t q = first; q < cands.size(); ++q) {
+            cands[q].off = tot;
+            const Edge& E = ed[cands[q].e];
+            tasks.push_back({(const S*)E.A + C.cols, nullptr, E.rows, ex, E.cols, 0});
+            tot += (size_t)ex;
+        }
+    }
+    std::vector<double> nrm(tot);
+    if (tot) {
+        double* d_norm = (double*)arena.alloc(tot * sizeof(double));
+        for (size_t q = 0; q < tasks.size(); ++q) tasks[q].out = d_norm + cands[q].off;
+        launch_row_norms<S>(upload_vec(tasks), (int)tasks.size(), stream_);
+        HIPCHK(hipMemcpyAsync(nrm.data(), d_norm, tot * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        sync_stream();
+    }
+    // targets and their order, then the runs per out-neighbour
+    std::vector<double> best;
+    for (size_t q0 = 0; q0 < cands.size();) {
+        const int c = cands[q0].c;
+        Cluster& C = cl[c];
+        const int ex = C.rows - C.cols;
+        best.assign(ex, 0.0);
+        size_t q1 = q0;
+        for (; q1 < cands.size() && cands[q1].c == c; ++q1)
+            for (int i = 0; i < ex; ++i) {
+                const double v = nrm[cands[q1].off + i];
+                if (v > best[i]) { best[i] = v; C.r2c[i] = cands[q1].m; }
+            }
+        q0 = q1;
+    }
+    for (int c : ints) {
+        Cluster& C = cl[c];
+        const int ex = (int)C.r2c.size();
+        if (ex == 0) continue;
+        C.row_perm.resize(ex);
+        for (int i = 0; i < ex; ++i) C.row_perm[i] = i;
+        std::stable_sort(C.row_perm.begin(), C.row_perm.end(), [&](int a, int b) { return C.r2c[a] < C.r2c[b]; });
+        for (size_t j = 1; j < C.out.size(); ++j) {
+            const int nn = ed[C.out[j]].n2;
+            int i0 = 0;
+            while (i0 < ex && C.r2c[C.row_perm[i0]] != nn) ++i0;
+            int i1 = i0;
+            while (i1 < ex && C.r2c[C.row_perm[i1]] == nn) ++i1;
+            if (i1 > i0) {
+                cl[nn].extra_from.push_back({c, i0, i1 - i0});
+                cl[nn].tot_extra += i1 - i0;
+            }
+        }
+    }
+    // the grown blocks
+    std::vector<RowGatherTask<S>> gt;
+    std::vector<RowSrc<S>> src;
+    for (int nn : interfaces[L]) {
+        Cluster& N = cl[nn];
+        if (N.tot_extra == 0) continue;
+        std::sort(N.extra_from.begin(), N.extra_from.end());
+        std::vector<int> grow{N.self_edge};
+        for (int e : N.in)
+            if (cl[ed[e].n1].lvl() != L) grow.push_back(e);
+        for (int e : grow) {
+            Edge& E = ed[e];
+            const Cluster& N1 = cl[E.n1];
+            const int rn = E.rows + N.tot_extra;
+            S* dst = (S*)arena.alloc((size_t)rn * std::max(E.cols, 1) * sizeof(S));
+            gt.push_back({dst, (const S*)E.A, rn, E.rows, E.rows, E.cols, (int)src.size(), N.tot_extra});
+            for (const auto& t : N.extra_from) {
+                const Cluster& C = cl[t[0]];
+                const int fe = find_out(N1, t[0]);  // the block A(c, n1)
+                for (int i = 0; i < t[2]; ++i)
+                    src.push_back(fe >= 0 ? RowSrc<S>{(const S*)ed[fe].A + C.cols + C.row_perm[t[1] + i], ed[fe].rows, 0}
+                                          : RowSrc<S>{nullptr, 0, 0});
+            }
+            E.A = dst;
+            E.rows = rn;
+        }
+        N.rows += N.tot_extra;
+        N.rank = N.cols;
+        N.row_rank = N.rows - N.cols;
+    }
+    launch_row_gather<S>(upload_vec(gt), upload_vec(src), (int)gt.size(), stream_);
+    HIPCHK(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------- large fronts
@@ -2329,6 +2624,8 @@ int Tree::factorize() {
         if (graphs) {
             if (rank_mode != 0 || rank_log)
                 throw std::runtime_error("graph mode needs rank mode 0 (geqp3 ranks) and no rank log");
+            if (rect())
+                throw std::runtime_error("graph mode is kept for square systems (the row reassignment reads back mid-level)");
             if (gplan.valid && gplan.sig == plan_signature()) {
                 const int r = prec_fact == 32 ? replay_t<float>() : replay_t<double>();
                 if (r == 0) return 0;
@@ -2381,7 +2678,11 @@ void Tree::solve_sched_begin() {
     sched->lv.resize(nlevels);
     for (int o : bottoms[0]) cl[o].xoff = (size_t)cl[o].cstart;
     sched->off = (size_t)n;
-    const size_t cap = std::max<size_t>((size_t)n * (size_t)nlevels, 1) * sizeof(S);
+    // a rectangular system's level-0 segments are placed once its row reassignment is known; a
+    // level's clusters then hold at most 2m rows (the extra rows are counted by the interior and
+    // by the interface that receives them)
+    if (rect()) sched->off = 0;
+    const size_t cap = std::max<size_t>(rect() ? 2 * (size_t)m * (size_t)nlevels : (size_t)n * (size_t)nlevels, 1) * sizeof(S);
     // kept across factorizations (of either precision) and only grown: a free would wait for
     // the device
     if (!xw || xw_bytes < cap) {
@@ -2391,6 +2692,13 @@ void Tree::solve_sched_begin() {
         HIPCHK(hipMalloc(&xw, cap));
         xw_bytes = cap;
     }
+    if (rect()) {
+        const size_t ns = 2 * (size_t)m + n;
+        sched->scratch = arena.alloc(ns * sizeof(S));
+        std::vector<long long> io(ns);
+        for (size_t i = 0; i < ns; ++i) io[i] = (long long)i;
+        sched->iota = upload_vec(io);
+    }
 }
 
 template <typename S>
@@ -2398,20 +2706,21 @@ void Tree::solve_sched_levels(int upto) {
     S* X0 = (S*)xw;
     for (; sched->built < upto; ++sched->built) {
         const int L = sched->built;
-        if (L > 0)
+        if (L > 0 || rect())
             for (int o : bottoms[L]) {
                 cl[o].xoff = sched->off;
-                sched->off += (size_t)cl[o].rows_org;
+                sched->off += (size_t)cl[o].rows_org + (size_t)cl[o].tot_extra;
             }
         if (sched->off * sizeof(S) > xw_bytes) throw std::runtime_error("solve workspace bound exceeded");
         auto& lv = sched->lv[L];
         if (L > 0) {
-            std::vector<VecCopyTask<S>> mf, mb;
+            std::vector<VecCopyTask<S>> mf, mb, mt;
             for (int p : bottoms[L]) {
                 int k = 0, kc = 0;
                 for (int ch : cl[p].children) {
                     mf.push_back({X0 + cl[ch].xoff, X0 + cl[p].xoff + k, cl[ch].rows, 0});
                     mb.push_back({X0 + cl[p].xoff + kc, X0 + cl[ch].xoff, cl[ch].cols, 0});
+                    mt.push_back({X0 + cl[ch].xoff, X0 + cl[p].xoff + kc, cl[ch].cols, 0});
                     k += cl[ch].rows;
                     kc += cl[ch].cols;
                 }
@@ -2420,6 +2729,73 @@ void Tree::solve_sched_levels(int upto) {
             lv.n_merge_fwd = (int)mf.size();
             lv.merge_bwd = upload_vec(mb);
             lv.n_merge_bwd = (int)mb.size();
+            lv.merge_tr = upload_vec(mt);
+            lv.n_merge_tr = (int)mt.size();
+        }
+        if (rect() && L == 0) {  // loads and stores through the scratch region
+            std::vector<long long> a, b, c, d, e, f;
+            for (int o : bottoms[0]) {
+                const Cluster& C = cl[o];
+                for (int i = 0; i < C.rows_org; ++i) { a.push_back(C.rstart + i); b.push_back((long long)C.xoff + i); }
+                for (int i = 0; i < C.cols_org; ++i) {
+                    c.push_back(C.cstart + i);
+                    d.push_back((long long)C.xoff + i);
+                    e.push_back((long long)C.xoff + i);
+                    f.push_back(C.cstart + i);
+                }
+            }
+            sched->ld_si = upload_vec(a); sched->ld_di = upload_vec(b); sched->n_ld = (int)a.size();
+            sched->lc_si = upload_vec(c); sched->lc_di = upload_vec(d); sched->n_lc = (int)c.size();
+            sched->st_si = upload_vec(e); sched->st_di = upload_vec(f); sched->n_st = (int)e.size();
+        }
+        if (rect()) {
+            std::vector<long long> rs, rd, sa, sb, ta, tb;
+            std::vector<VecQTask<S>> qx;
+            for (int nn : interfaces[L]) {
+                const Cluster& N = cl[nn];
+                long long pos = (long long)N.xoff + N.rows_org;
+                for (const auto& t : N.extra_from) {
+                    const Cluster& C = cl[t[0]];
+                    for (int i = 0; i < t[2]; ++i) {
+                        rs.push_back((long long)C.xoff + C.cols + C.row_perm[t[1] + i]);
+                        rd.push_back(pos++);
+                    }
+                }
+                if (!N.sparsified) continue;
+                if (N.row_rank > 0)
+                    qx.push_back({(const S*)N.Qex, (const S*)N.Tex, X0 + N.xoff + N.cols_org, N.nex, N.row_rank, N.nex, 1});
+                const long long b0 = (long long)N.xoff;
+                const int size = N.rows_org + N.tot_extra, x2 = size - N.cols_org, x1 = N.cols_org - N.rank;
+                for (int i = 0; i < x2; ++i) { sa.push_back(b0 + N.cols_org + i); sb.push_back(b0 + N.rank + i); }
+                for (int i = 0; i < x1; ++i) { sa.push_back(b0 + N.rank + i); sb.push_back(b0 + N.rank + x2 + i); }
+                for (int i = 0; i < x1; ++i) { ta.push_back(b0 + N.rank + i); tb.push_back(b0 + N.rank + x2 + i); }
+                for (int i = 0; i < x2; ++i) { ta.push_back(-1); tb.push_back(b0 + N.rank + i); }
+            }
+            lv.rg_si = upload_vec(rs); lv.rg_di = upload_vec(rd); lv.n_rg = (int)rs.size();
+            lv.qex = upload_vec(qx); lv.n_qex = (int)qx.size();
+            lv.sp_a = upload_vec(sa); lv.sp_b = upload_vec(sb); lv.n_sp = (int)sa.size();
+            lv.ts_a = upload_vec(ta); lv.ts_b = upload_vec(tb); lv.n_ts = (int)ta.size();
+        }
+        {  // transposed couplings: per source cluster of an interior's in-edge, links in interior order
+            std::map<int, std::pair<int, std::vector<VecGemvtLink<S>>>> by_src;  // source -> (its columns, links)
+            for (int c : interiors[L]) {
+                const Cluster& C = cl[c];
+                for (int e : C.in) {
+                    const Edge& ei = ed[e];
+                    auto& v = by_src[ei.n1];
+                    v.first = ei.cols;
+                    v.second.push_back({(const S*)ei.A, X0 + C.xoff, ei.rows, C.cols});
+                }
+            }
+            std::vector<VecGemvtTask<S>> gt;
+            std::vector<VecGemvtLink<S>> gl;
+            for (auto& kv : by_src) {
+                gt.push_back({X0 + cl[kv.first].xoff, kv.second.first, (int)gl.size(), (int)kv.second.second.size(), 0});
+                gl.insert(gl.end(), kv.second.second.begin(), kv.second.second.end());
+            }
+            lv.gemvt = upload_vec(gt);
+            lv.n_gemvt = (int)gt.size();
+            lv.gemvt_links = upload_vec(gl);
         }
         std::vector<VecQTask<S>> q;
         int ncol = 0;
@@ -2572,8 +2948,192 @@ void Tree::solve_phase_t(int L, int phase, int dir) {
         } else {
             launch_vec_trsv<S>((const VecTrsvTask<S>*)lv.trsv, (const VecGemvLink<S>*)lv.trsv_links, lv.n_trsv, stream_);
         }
+    } else if (dir) {
+        move_t<S>(lv.rg_si, lv.rg_di, lv.n_rg, false);  // fwd_reassign + merge_x (src/ptree.cpp:2395-2431)
+        launch_vec_spars<S>((const VecSparsTask<S>*)lv.spars, lv.n_spars, 1, stream_);
+        launch_vec_q<S>((const VecQTask<S>*)lv.qex, lv.n_qex, stream_);  // orthogonalD_extra_fwd :2206-2211
+        move_t<S>(lv.sp_a, lv.sp_b, lv.n_sp, true);  // splitD_fwd :2223-2232
     } else {
-        launch_vec_spars<S>((const VecSparsTask<S>*)lv.spars, lv.n_spars, dir ? 1 : 0, stream_);
+        move_t<S>(lv.sp_b, lv.sp_a, lv.n_sp, true);  // splitD_bwd :2234-2242
+        launch_vec_spars<S>((const VecSparsTask<S>*)lv.spars, lv.n_spars, 0, stream_);
+    }
+}
+
+// Workspace moves X[di[q]] = X[si[q]] (si < 0: zero), q < n; through the scratch region when
+// the positions of one call overlap.
+template <typename S>
+void Tree::move_t(const void* si, const void* di, int cnt, bool staged) {
+    if (cnt <= 0) return;
+    S* X0 = (S*)xw;
+    const auto a = (const long long*)si;
+    const auto b = (const long long*)di;
+    if (!staged) {
+        launch_vec_gather<S>(X0, X0, a, b, cnt, stream_);
+        return;
+    }
+    S* T = (S*)sched->scratch;
+    const auto io = (const long long*)sched->iota;
+    launch_vec_gather<S>(X0, T, a, io, cnt, stream_);
+    launch_vec_gather<S>(T, X0, io, b, cnt, stream_);
+}
+
+// One level of the transposed backward pass, the first half of solve_nrml (the transposes of
+// merge_bwd, bwd_trsv with its couplings, and bwd_spars, in reverse order; src/toperations.cpp
+// tMerge::fwd :88-96, tQR::fwd :12-27, tScaleD::fwd :71-74, tOrthogonalD::fwd :79-81, tSplitD).
+template <typename S>
+void Tree::solve_phase_tr(int L) {
+    auto& lv = sched->lv[L];
+    if (L > 0) launch_vec_copy<S>((const VecCopyTask<S>*)lv.merge_tr, lv.n_merge_tr, stream_);
+    launch_vec_trsv_t<S>((const VecTrsvTask<S>*)lv.trsv, lv.n_trsv, stream_);
+    launch_vec_gemvt<S>((const VecGemvtTask<S>*)lv.gemvt, (const VecGemvtLink<S>*)lv.gemvt_links, lv.n_gemvt, stream_);
+    launch_vec_spars_t<S>((const VecSparsTask<S>*)lv.spars, lv.n_spars, stream_);
+    move_t<S>(lv.ts_a, lv.ts_b, lv.n_ts, true);
+}
+
+// b (m entries, permuted rows) into the level-0 segments, and the segments' first cols_org
+// entries out to x (n, permuted columns); a square system's level-0 segments are x itself.
+template <typename S>
+void Tree::load_rows_t(const double* b) {
+    S* X0 = (S*)xw;
+    if (!rect()) {
+        launch_convert_d2s<S>(n, b, X0, stream_);
+        return;
+    }
+    S* T = (S*)sched->scratch;
+    launch_convert_d2s<S>(m, b, T, stream_);
+    launch_vec_gather<S>(T, X0, (const long long*)sched->ld_si, (const long long*)sched->ld_di, sched->n_ld, stream_);
+}
+template <typename S>
+void Tree::load_cols_t(const double* y) {
+    S* X0 = (S*)xw;
+    if (!rect()) {
+        launch_convert_d2s<S>(n, y, X0, stream_);
+        return;
+    }
+    S* T = (S*)sched->scratch;
+    launch_convert_d2s<S>(n, y, T, stream_);
+    launch_vec_gather<S>(T, X0, (const long long*)sched->lc_si, (const long long*)sched->lc_di, sched->n_lc, stream_);
+}
+template <typename S>
+void Tree::store_cols_t(double* x) {
+    S* X0 = (S*)xw;
+    if (!rect()) {
+        launch_convert_s2d<S>(n, X0, x, stream_);
+        return;
+    }
+    S* T = (S*)sched->scratch;
+    launch_vec_gather<S>(X0, T, (const long long*)sched->st_si, (const long long*)sched->st_di, sched->n_st, stream_);
+    launch_convert_s2d<S>(n, T, x, stream_);
+}
+
+template <typename S>
+int Tree::solve_ls_t(const double* b, double* x) {
+    load_rows_t<S>(b);
+    for (int L = 0; L < nlevels; ++L)
+        for (int ph = 0; ph < 3; ++ph) solve_phase_t<S>(L, ph, 1);
+    for (int L = nlevels - 1; L >= 0; --L)
+        for (int ph = 2; ph >= 0; --ph) solve_phase_t<S>(L, ph, 0);
+    store_cols_t<S>(x);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <typename S>
+int Tree::solve_nrml_t(const double* y, double* x) {
+    load_cols_t<S>(y);
+    for (int L = 0; L < nlevels; ++L) solve_phase_tr<S>(L);
+    for (int L = nlevels - 1; L >= 0; --L)
+        for (int ph = 2; ph >= 0; --ph) solve_phase_t<S>(L, ph, 0);
+    store_cols_t<S>(x);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int Tree::solve_ls_dev(const double* b, double* x) {
+    try {
+        if (!factorized || !sched) throw std::runtime_error("factorize() first");
+        int rc = 0;
+        timed(P_SOLVE, 1, 1, [&] { rc = (prec_fact == 32) ? solve_ls_t<float>(b, x) : solve_ls_t<double>(b, x); });
+        prof_add(P_SOLVE, sched->flops, sched->bytes + 8.0 * (n + m));
+        return rc;
+    } catch (std::exception& e) {
+        last_error = e.what();
+        return -1;
+    }
+}
+
+int Tree::solve_nrml_dev(const double* y, double* x) {
+    try {
+        if (!factorized || !sched) throw std::runtime_error("factorize() first");
+        int rc = 0;
+        timed(P_SOLVE, 1, 1, [&] { rc = (prec_fact == 32) ? solve_nrml_t<float>(y, x) : solve_nrml_t<double>(y, x); });
+        prof_add(P_SOLVE, sched->flops, sched->bytes + 16.0 * n);
+        return rc;
+    } catch (std::exception& e) {
+        last_error = e.what();
+        return -1;
+    }
+}
+
+// cgls (src/is.cpp:274-345) with x0 = 0 on device vectors: CG on the normal equations with
+// solve_nrml as the preconditioner, stopping when ||A^T (b - A x)||^2 < tol^2 ||A^T b||^2.  The
+// scalars (five dot products per iteration) come back to the host, as the reference's do.
+int Tree::cgls_dev(const double* b, double* x, int iters, double tol_, int* iters_out, double* rel) {
+    try {
+        if (!factorized) throw std::runtime_error("factorize() first");
+        if (!d_tptr) throw std::runtime_error("cgls needs the adjoint product (a rectangular system)");
+        double* W = krylov_ws(sizeof(double) * (2 * (size_t)m + 3 * (size_t)n + 4));
+        double* r = W;
+        double* q = r + m;
+        double* sv = q + m;
+        double* p = sv + n;
+        double* z = p + n;
+        double* part = d_scratch;
+        double* hdev = d_scratch + dot_blocks;
+        auto dot = [&](int len, const double* a, const double* c) {
+            launch_dot(len, a, c, part, dot_blocks, hdev, 0, stream_);
+            double h = 0.0;
+            HIPCHK(hipMemcpyAsync(&h, hdev, sizeof(double), hipMemcpyDeviceToHost, stream_));
+            sync_stream();
+            return h;
+        };
+        launch_fill<double>(x, (size_t)n, 0.0, stream_);
+        HIPCHK(hipMemcpyAsync(r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, stream_));
+        if (spmv_t_dev(b, sv) != 0) throw std::runtime_error(last_error);
+        const double rhs2 = dot(n, sv, sv);
+        *iters_out = 0;
+        if (rhs2 == 0.0) { *rel = 0.0; return 0; }
+        const double threshold = tol_ * tol_ * rhs2;
+        double res2 = rhs2;
+        if (res2 < threshold) { *rel = std::sqrt(res2 / rhs2); return 0; }
+        if (solve_nrml_dev(sv, p) != 0) throw std::runtime_error(last_error);
+        double abs_new = dot(n, sv, p);
+        int i = 0;
+        while (i < iters) {
+            if (spmv_dev(p, q) != 0) throw std::runtime_error(last_error);
+            const double qq = dot(m, q, q);
+            if (!std::isfinite(qq) || qq == 0.0) throw std::runtime_error("cgls: A p vanished or is not finite");
+            const double alpha = abs_new / qq;
+            launch_axpby(n, alpha, p, 1.0, x, stream_);
+            launch_axpby(m, -alpha, q, 1.0, r, stream_);
+            if (spmv_t_dev(r, sv) != 0) throw std::runtime_error(last_error);
+            res2 = dot(n, sv, sv);
+            if (res2 < threshold) break;
+            if (solve_nrml_dev(sv, z) != 0) throw std::runtime_error(last_error);
+            const double abs_old = abs_new;
+            abs_new = dot(n, sv, z);
+            const double beta = abs_new / abs_old;
+            launch_axpby(n, 1.0, z, beta, p, stream_);
+            ++i;
+        }
+        HIPCHK(hipGetLastError());
+        sync_stream();
+        *iters_out = i;
+        *rel = std::sqrt(res2 / rhs2);
+        return 0;
+    } catch (std::exception& e) {
+        last_error = e.what();
+        return -1;
     }
 }
 
@@ -2593,6 +3153,7 @@ int Tree::solve_t(double* x) {
 int Tree::solve_load(const double* x) {
     try {
         if (!factorized || !sched) throw std::runtime_error("factorize() first");
+        if (rect()) throw std::runtime_error("the phase-by-phase solve is kept for square systems");
         if (prec_fact == 32) launch_convert_d2s<float>(n, x, (float*)xw, stream_);
         else launch_convert_d2s<double>(n, x, (double*)xw, stream_);
         HIPCHK(hipGetLastError());
@@ -2633,6 +3194,7 @@ int Tree::solve_store(double* x) {
 int Tree::solve_dev(double* x) {
     try {
         if (!factorized || !sched) throw std::runtime_error("factorize() first");
+        if (rect()) throw std::runtime_error("a rectangular system's solve maps m rows to n columns: use solve_ls");
         int rc = 0;
         timed(P_SOLVE, 1, 1, [&] { rc = (prec_fact == 32) ? solve_t<float>(x) : solve_t<double>(x); });
         // plus the FP64 vector in and out
@@ -2646,8 +3208,21 @@ int Tree::solve_dev(double* x) {
 
 int Tree::spmv_dev(const double* x, double* y) {
     try {
-        timed(P_SPMV, 1, 1, [&] { launch_spmv(n, d_rowptr, d_col, d_val, x, y, stream_); });
-        prof_add(P_SPMV, 2.0 * d_nnz, 12.0 * d_nnz + 4.0 * (n + 1) + 16.0 * n);
+        timed(P_SPMV, 1, 1, [&] { launch_spmv(m, d_rowptr, d_col, d_val, x, y, stream_); });
+        prof_add(P_SPMV, 2.0 * d_nnz, 12.0 * d_nnz + 4.0 * (m + 1) + 8.0 * (n + m));
+        HIPCHK(hipGetLastError());
+        return 0;
+    } catch (std::exception& e) {
+        last_error = e.what();
+        return -1;
+    }
+}
+
+int Tree::spmv_t_dev(const double* r, double* y) {
+    try {
+        if (!d_tptr) throw std::runtime_error("the adjoint product is kept for rectangular systems");
+        timed(P_SPMV, 1, 1, [&] { launch_spmv(n, d_tptr, d_tcol, d_tval, r, y, stream_); });
+        prof_add(P_SPMV, 2.0 * d_nnz, 12.0 * d_nnz + 4.0 * (n + 1) + 8.0 * (n + m));
         HIPCHK(hipGetLastError());
         return 0;
     } catch (std::exception& e) {
@@ -2713,6 +3288,7 @@ int Tree::gmres_dev(const double* b, double* x, int iters, double tol_, int rest
     // Restarts (restart > 0) continue from the current iterate with the same absolute target.
     try {
         if (!factorized) throw std::runtime_error("factorize() first");
+        if (rect()) throw std::runtime_error("GMRES needs a square system; a least-squares system is solved by cgls");
         const int m = (restart > 0) ? std::min(restart, iters) : iters;
         if (m + 2 > 4096) throw std::runtime_error("restart length too large (at most 4094 steps per cycle)");
         const size_t nv = (size_t)(m + 1) + (flexible ? (size_t)m : 0) + 1;  // basis, Z, work vector

@@ -44,8 +44,18 @@ struct Cluster {
     int order = -1, merge_level = 0;
     int inst = 0;                         // system of a batch this cluster belongs to
     ClusterID id;
-    int cstart = 0;
+    int cstart = 0, rstart = 0;
     int rows = 0, cols = 0, rows_org = 0, cols_org = 0, rank = 0;
+    // rectangular systems: rows kept beyond the columns (rows - cols; after the extra-row
+    // sparsification its rank), an interior's extra-row targets and their order (row_perm), an
+    // interface's received rows {interior, start in its row_perm, count} in interior order,
+    // and the extra-row sparsification's reflectors (nex rows)
+    int row_rank = 0, tot_extra = 0, nex = 0;
+    std::vector<int> r2c, row_perm;
+    std::vector<std::array<int, 3>> extra_from;
+    void* Qex = nullptr;
+    void* Tex = nullptr;
+    void* tauex = nullptr;
     int parent = -1;
     std::vector<int> children;
     int rpos = -1, cpos = -1;
@@ -129,7 +139,7 @@ public:
     // configuration (include/tree.h:137-143; include/ptree.h:123-125)
     double tol = 0.1;
     int nlevels, skip;
-    int scale = 1, square = 1, verbose = 0;
+    int scale = 1, square = 1, verbose = 0;  // square: 1 for m == n, 0 for a least-squares A (m > n)
     float order = 1.0f;
     int prec = 64;  // 64: FP64 factors; 32: FP32 factors (GMRES stays FP64)
     std::vector<double> X;  // coordinates, d x n
@@ -137,7 +147,13 @@ public:
     std::vector<int> user_parts;  // leaf parts given in place of coordinates (the METIS path)
     std::vector<int> leaf_parts;  // the leaf parts of the last partition, natural order
 
+    // A may have more rows than columns (m > n, a least-squares system): it is then ordered
+    // without coordinates (the caller's leaf parts or the graph bisection) and its rows are
+    // filed under the clusters by form_rmap.
     int partition(const HostCSC& A);
+    int n_rows() const { return m; }
+    bool rect() const { return m != n; }
+    const std::vector<int>& rperm() const { return rperm_; }
     // A batch of independent square systems factored together: A is their block-diagonal
     // matrix (system s owns unknowns [off_s, off_s + n_s), in order), X the coordinates of all
     // unknowns; system s gets its own nested dissection with levels[s] levels.  Every tree
@@ -162,7 +178,15 @@ public:
     int solve_load(const double* x);
     int solve_phase(int level, int phase, int dir);
     int solve_store(double* x);
-    int spmv_dev(const double* x, double* y);       // y = A_perm x, device vectors
+    int spmv_dev(const double* x, double* y);       // y = A_perm x, device vectors (y: m entries)
+    int spmv_t_dev(const double* r, double* y);     // y = A_perm^T r (r: m entries, y: n)
+    // ParTree::solve(b, x) (src/ptree.cpp:2267): b (m, permuted rows) -> x (n, permuted columns)
+    int solve_ls_dev(const double* b, double* x);
+    // x = M^{-1} M^{-T} y, M^{-1} the backward half of the solve: the preconditioner of cgls
+    int solve_nrml_dev(const double* y, double* x);
+    // cgls (src/is.cpp:274-345) on App x = b in the least-squares sense, x0 = 0; returns the
+    // iteration count as the reference counts it; *rel = ||A^T r|| / ||A^T b||.
+    int cgls_dev(const double* b, double* x, int iters, double tol, int* iters_out, double* rel);
     int gmres_dev(const double* b, double* x, int iters, double tol, int restart, double true_tol,
                   GmresResult* res, double* hist, int hist_cap, int* hist_len, bool flexible = false);
     int n_unknowns() const { return n; }
@@ -216,9 +240,11 @@ public:
     double t_factor_ms = 0.0;
 
 private:
-    int n = 0;
+    int n = 0, m = 0;
     HostCSC A0, App;
-    std::vector<int> cperm_;
+    std::vector<int> cperm_, rperm_;
+    std::vector<int> c2r_count;  // rows filed under each permuted column (all 1 for square systems)
+    void form_rmap();
     std::vector<int> inst_n, inst_lv, inst_off;  // per system: unknowns, levels, first position
     int* d_seg = nullptr;                        // device copy of inst_off plus n (batch GMRES)
     void make_hierarchy();
@@ -280,6 +306,9 @@ private:
     int* d_rowptr = nullptr;
     int* d_col = nullptr;
     double* d_val = nullptr;
+    int* d_tptr = nullptr;  // CSR of App^T (CSC of App): the adjoint SpMV of cgls
+    int* d_tcol = nullptr;
+    double* d_tval = nullptr;
     long long d_nnz = 0;
     double* d_scratch = nullptr;  // dot partials + scalars
     int dot_blocks = 512;
@@ -331,6 +360,14 @@ private:
     template <typename S> void solve_sched_begin();
     template <typename S> void solve_sched_levels(int upto);
     template <typename S> int solve_t(double* x);
+    template <typename S> void reassign_rows(int L);
+    template <typename S> int solve_ls_t(const double* b, double* x);
+    template <typename S> int solve_nrml_t(const double* y, double* x);
+    template <typename S> void solve_phase_tr(int level);  // one level of the transposed backward pass
+    template <typename S> void move_t(const void* si, const void* di, int cnt, bool staged);
+    template <typename S> void load_rows_t(const double* b);
+    template <typename S> void load_cols_t(const double* y);
+    template <typename S> void store_cols_t(double* x);
     template <typename S> void solve_phase_t(int level, int phase, int dir);
     void* upload(const void* h, size_t bytes);
     // pinned staging for descriptor uploads: one host copy, then an asynchronous DMA; the

@@ -29,6 +29,7 @@ class ParTree:
             raise RuntimeError("spaqr_tree_create failed: %s" % lib().spaqr_last_error(None).decode())
         self.nlevels = nlevels
         self._n = 0
+        self._m = 0
         self._keep = []
         register(self)
 
@@ -97,7 +98,7 @@ class ParTree:
     def partition(self, A):
         A = A.tocsc()
         A.sort_indices()
-        self._n = A.shape[1]
+        self._m, self._n = A.shape
         cp = np.ascontiguousarray(A.indptr, dtype=np.int32)
         ri = np.ascontiguousarray(A.indices, dtype=np.int32)
         va = np.ascontiguousarray(A.data, dtype=np.float64)
@@ -147,6 +148,7 @@ class ParTree:
         A = A.tocsc()
         A.sort_indices()
         self._n = A.shape[1]
+        self._m = A.shape[0]
         cp = np.ascontiguousarray(A.indptr, dtype=np.int32)
         ri = np.ascontiguousarray(A.indices, dtype=np.int32)
         va = np.ascontiguousarray(A.data, dtype=np.float64)
@@ -188,8 +190,8 @@ class ParTree:
 
     def assemble(self, A=None):
         check(lib().spaqr_tree_assemble(self._h), self._h, "assemble")
-        self._bx = DeviceBuffer(8 * self._n)
-        self._by = DeviceBuffer(8 * self._n)
+        self._bx = DeviceBuffer(8 * max(self._m, self._n))
+        self._by = DeviceBuffer(8 * max(self._m, self._n))
 
     def factorize(self) -> int:
         check(lib().spaqr_tree_factorize(self._h), self._h, "factorize")
@@ -206,7 +208,59 @@ class ParTree:
         self._bx.upload(np.asarray(x, dtype=np.float64))
         check(lib().spaqr_tree_spmv(self._h, self._bx.ptr, self._by.ptr), self._h, "spmv")
         check(lib().spaqr_tree_sync(self._h), self._h, "sync")
+        return self._by.download(np.float64, self._m)
+
+    # ---- least squares (rectangular A, m > n)
+    def spmv_t(self, r: np.ndarray) -> np.ndarray:
+        """A^T r (permuted orderings): r has m entries, the result n."""
+        self._bx.upload(np.asarray(r, dtype=np.float64))
+        check(lib().spaqr_tree_spmv_t(self._h, self._bx.ptr, self._by.ptr), self._h, "spmv_t")
+        check(lib().spaqr_tree_sync(self._h), self._h, "sync")
         return self._by.download(np.float64, self._n)
+
+    def solve_ls(self, b: np.ndarray) -> np.ndarray:
+        """ParTree::solve(b, x): b (m, permuted rows) -> x (n, permuted columns)."""
+        self._bx.upload(np.asarray(b, dtype=np.float64))
+        check(lib().spaqr_tree_solve_ls(self._h, self._bx.ptr, self._by.ptr), self._h, "solve_ls")
+        check(lib().spaqr_tree_sync(self._h), self._h, "sync")
+        return self._by.download(np.float64, self._n)
+
+    def solve_nrml(self, y: np.ndarray) -> np.ndarray:
+        """x = M^{-1} M^{-T} y (n entries, permuted columns): cgls's preconditioner."""
+        self._bx.upload(np.asarray(y, dtype=np.float64))
+        check(lib().spaqr_tree_solve_nrml(self._h, self._bx.ptr, self._by.ptr), self._h, "solve_nrml")
+        check(lib().spaqr_tree_sync(self._h), self._h, "sync")
+        return self._by.download(np.float64, self._n)
+
+    def cgls(self, b: np.ndarray, iters: int = 300, tol: float = 1e-12):
+        """cgls(A, b, x, M, iters, tol) with x0 = 0 (src/is.cpp:274-345); returns (x, iterations,
+        ||A^T r|| / ||A^T b||).  b has m entries in the permuted row ordering, x n entries."""
+        self._bx.upload(np.asarray(b, dtype=np.float64))
+        it, rel = C.c_int(0), C.c_double(0.0)
+        check(lib().spaqr_cgls(self._h, self._bx.ptr, self._by.ptr, iters, tol, C.byref(it), C.byref(rel)), self._h,
+              "cgls")
+        return self._by.download(np.float64, self._n), it.value, rel.value
+
+    def cgls_device(self, b_ptr, x_ptr, iters=300, tol=1e-12):
+        it, rel = C.c_int(0), C.c_double(0.0)
+        check(lib().spaqr_cgls(self._h, b_ptr, x_ptr, iters, tol, C.byref(it), C.byref(rel)), self._h, "cgls")
+        return it.value, rel.value
+
+    @property
+    def m(self) -> int:
+        return self._m
+
+    def rperm(self) -> np.ndarray:
+        p = np.zeros(self._m, dtype=np.int32)
+        check(lib().spaqr_tree_get_rperm(self._h, p.ctypes.data), self._h, "get_rperm")
+        return p
+
+    def rows(self) -> np.ndarray:
+        """Per cluster: first row, rows at creation, extra rows received, row_rank, extra rows moved."""
+        nc = lib().spaqr_tree_num_clusters(self._h)
+        out = np.zeros(5 * nc, dtype=np.int32)
+        check(lib().spaqr_tree_get_rows(self._h, out.ctypes.data), self._h, "get_rows")
+        return out.reshape(nc, 5)
 
     def gmres(self, b: np.ndarray, iters: int = 300, tol: float = 1e-12, restart: int = 0, true_tol: float = 1e-12,
               flexible: bool = False):
@@ -316,9 +370,13 @@ def build_tree(problem, precision: int = 64, nlevels=None, tol=None, skip=None) 
     t.set_scale(1)
     t.set_tol(problem.tol if tol is None else tol)
     t.set_order(1)
-    t.set_square(1)
+    t.set_square(1 if problem.A.shape[0] == problem.A.shape[1] else 0)
     t.set_precision(precision)
-    if getattr(problem, "X", None) is not None:
+    rect = problem.A.shape[0] != problem.A.shape[1]
+    if rect:  # ordered without coordinates: the problem's leaf parts, else the graph bisection
+        if getattr(problem, "parts", None) is not None:
+            t.set_parts(problem.parts)
+    elif getattr(problem, "X", None) is not None:
         t.set_Xcoo(problem.X)
     elif problem.n_side > 0 and problem.dim > 0:
         t.set_Xcoo(linspace_nd(problem.n_side, problem.dim))
