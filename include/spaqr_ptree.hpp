@@ -189,14 +189,15 @@ class ParTree {
     /* src/ptree.cpp:495, 696, 873 */
     template <class SpMat> void partition(SpMat& A) {
         n_ = (int)A.cols();
+        m_ = (int)A.rows();
         detail::check(spaqr_tree_partition(t_, (int)A.rows(), (int)A.cols(), A.outerIndexPtr(), A.innerIndexPtr(),
                                            A.valuePtr()),
                       t_, "partition");
     }
     template <class SpMat> void assemble(SpMat&) {
         detail::check(spaqr_tree_assemble(t_), t_, "assemble");
-        x_.resize(n_);
-        y_.resize(n_);
+        x_.resize(m_ > n_ ? m_ : n_);
+        y_.resize(m_ > n_ ? m_ : n_);
         ops_.clear();
     }
     int factorize() {
@@ -204,8 +205,19 @@ class ParTree {
         return 0;
     }
 
-    /* solve(VectorXd b, VectorXd& x), src/ptree.cpp:2268: b and x in the natural ordering */
+    /* solve(VectorXd b, VectorXd& x), src/ptree.cpp:2268: b and x in the natural ordering; for a
+       rectangular A, b has m entries and x gets n (the preconditioner's least-squares answer) */
     template <class Vec> void solve(Vec b, Vec& x) const {
+        if (m_ != n_) {
+            std::vector<double> bt(m_), xt(n_);
+            detail::check(spaqr_tree_permute_rows(t_, 0, b.data(), bt.data()), t_, "permute_rows");
+            x_.put(bt.data(), m_);
+            detail::check(spaqr_tree_solve_ls(t_, x_.get(), y_.get()), t_, "solve");
+            y_.take(xt.data(), n_);
+            x.resize(n_);
+            detail::check(spaqr_tree_permute(t_, 1, xt.data(), x.data()), t_, "permute");
+            return;
+        }
         std::vector<double> bt(n_), xt(n_);
         detail::check(spaqr_tree_permute(t_, 0, b.data(), bt.data()), t_, "permute");
         x_.put(bt.data(), n_);
@@ -220,13 +232,25 @@ class ParTree {
         detail::check(spaqr_tree_solve(t_, x_.get()), t_, "solve");
         x_.take(x.data(), n_);
     }
-    /* spmv(VectorXd x), src/ptree.cpp:3212: A x in the tree ordering */
+    /* spmv(VectorXd x), src/ptree.cpp:3212: A x in the tree orderings (m entries) */
     template <class Vec> Vec spmv(Vec x) const {
         x_.put(x.data(), n_);
         detail::check(spaqr_tree_spmv(t_, x_.get(), y_.get()), t_, "spmv");
         Vec y(x);
-        y_.take(y.data(), n_);
+        y.resize(m_);
+        y_.take(y.data(), m_);
         return y;
+    }
+    /* solve_nrml(VectorXd b, VectorXd& x) (the sequential Tree's, src/tree.cpp:1524-1570):
+       x ~ (A^T A)^{-1} b, both n entries in the natural ordering -- cgls's preconditioner */
+    template <class Vec> void solve_nrml(Vec b, Vec& x) const {
+        std::vector<double> bt(n_), xt(n_);
+        detail::check(spaqr_tree_permute(t_, 0, b.data(), bt.data()), t_, "permute");
+        x_.put(bt.data(), n_);
+        detail::check(spaqr_tree_solve_nrml(t_, x_.get(), y_.get()), t_, "solve_nrml");
+        y_.take(xt.data(), n_);
+        x.resize(n_);
+        detail::check(spaqr_tree_permute(t_, 1, xt.data(), x.data()), t_, "permute");
     }
     /* distribute_x(x, x_dist), src/ptree.cpp:3295-3304 with one rank: x <- P x, x_dist = x */
     template <class Vec> void distribute_x(Vec& x, Vec& x_dist) {
@@ -244,7 +268,7 @@ class ParTree {
         x.resize(n_);
         detail::check(spaqr_tree_permute(t_, 1, t.data(), x.data()), t_, "permute");
     }
-    int nrows_local() const { return n_; }
+    int nrows_local() const { return m_; }
     int ncols_local() const { return n_; }
     int nranks() const { return 1; }
 
@@ -273,7 +297,7 @@ class ParTree {
 
   private:
     spaqr_tree* t_ = nullptr;
-    int nlevels_ = 0, n_ = 0;
+    int nlevels_ = 0, n_ = 0, m_ = 0;
     mutable detail::DeviceVec x_, y_;
     std::vector<std::unique_ptr<Operation>> ops_;
 };
@@ -293,6 +317,31 @@ unsigned long dist_gmres(const SpMat&, Vec& b, const ParTree& M, unsigned long i
                   M.handle(), "gmres");
     xd.take(b.data(), n);
     return (unsigned long)r.iters;
+}
+
+/* cgls(mat, rhs, x, precond, iters, tol_error, verb), src/is.cpp:274-345, with x0 = 0: CG on
+   the normal equations of the least-squares system mat x ~ rhs (natural orderings; rhs m
+   entries, x n) preconditioned by precond.solve_nrml, on the device.  iters: in the maximum, out
+   the count; tol_error: in the tolerance on ||A^T r|| / ||A^T b||, out the value reached. */
+template <class MatrixType, class Rhs, class Dest>
+int cgls(const MatrixType&, const Rhs& rhs, Dest& x, const ParTree& precond, long& iters, double& tol_error,
+         bool verb = false) {
+    (void)verb;
+    const int m = precond.nrows_local(), n = precond.ncols_local();
+    std::vector<double> bt(m), xt(n);
+    detail::check(spaqr_tree_permute_rows(precond.handle(), 0, rhs.data(), bt.data()), precond.handle(), "permute_rows");
+    detail::DeviceVec bd(m), xd(n);
+    bd.put(bt.data(), m);
+    int it = 0;
+    double rel = 0.0;
+    detail::check(spaqr_cgls(precond.handle(), bd.get(), xd.get(), (int)iters, tol_error, &it, &rel), precond.handle(),
+                  "cgls");
+    xd.take(xt.data(), n);
+    x.resize(n);
+    detail::check(spaqr_tree_permute(precond.handle(), 1, xt.data(), x.data()), precond.handle(), "permute");
+    iters = it;
+    tol_error = rel;
+    return it;
 }
 
 }  // namespace spaqr

@@ -193,14 +193,17 @@ def load_problem(matrix: str, nlevels: int, tol: float, coordinates: str | None 
     """The reference driver's input path (pspaQR.cpp:85-167): A from a Matrix Market file, scaled
     A <- A D 10; coordinates from an array file (d x N), or the n^d tensor grid when n and d are
     given (they take precedence, as -n/-d overwrite --coordinates), or none -- the tree then orders
-    by graph bisection.  b from an array file, else uniform(-1, 1) from std::mt19937(seed)."""
+    by graph bisection (always for a rectangular A, which is solved in the least-squares sense).
+    b (one entry per row) from an array file, else uniform(-1, 1) from std::mt19937(seed)."""
     from . import mmio
 
     A = mmio.read_sparse(matrix)
-    if A.shape[0] != A.shape[1]:
-        raise ValueError("only square systems are supported (A is %d x %d)" % A.shape)
-    N = A.shape[1]
+    M, N = A.shape
+    if M < N:
+        raise ValueError("A has fewer rows than columns (%d x %d)" % A.shape)
     X = None
+    if M > N and (coordinates or (n > 0 and d > 0)):
+        raise ValueError("a rectangular system (least squares) is ordered without coordinates")
     if n > 0 and d > 0:
         if n ** d != N:
             raise ValueError("n^d = %d but A is %d x %d" % (n ** d, N, N))
@@ -210,7 +213,7 @@ def load_problem(matrix: str, nlevels: int, tol: float, coordinates: str | None 
             X = mmio.read_dense(coordinates)
             if X.shape[1] != N:
                 raise ValueError("the coordinate file must hold a d x N matrix (it is %d x %d)" % X.shape)
-    b = mmio.read_vector(rhs) if rhs else mt_uniform(seed, N)
+    b = mmio.read_vector(rhs) if rhs else mt_uniform(seed, M)
     name = os.path.splitext(os.path.basename(str(matrix)))[0]
     return Problem(name, column_scale(A), b, n, d, nlevels, tol, skip, X=X)
 

@@ -6,8 +6,10 @@
 // (sp_mmwrite / sp_mmread, dense_mmwrite / dense_mmread: the driver's input path), then
 // partition -> assemble -> factorize -> dist_gmres, and checks: the files give back the same
 // matrix and coordinates; true relative residual <= 1e-12; the Operation replay equals solve(x)
-// bit for bit; solve(b, x) in the natural ordering equals the permuted in-place solve.  Prints
-// one line of results and exits non-zero on any failed check.
+// bit for bit; solve(b, x) in the natural ordering equals the permuted in-place solve.  Then an
+// overdetermined system through cgls (the least-squares path, ordered without coordinates),
+// checked on the normal equations.  Prints one line of results and exits non-zero on any failed
+// check.
 #include <unistd.h>
 
 #include <cmath>
@@ -58,6 +60,47 @@ static Csc poisson2d(int m) {
         for (int k = A.colptr[j]; k < A.colptr[j + 1]; ++k) s += A.val[k] * A.val[k];
         const double d = 1.0 / std::sqrt(s);
         for (int k = A.colptr[j]; k < A.colptr[j + 1]; ++k) A.val[k] = A.val[k] * d * 10.0;
+    }
+    return A;
+}
+
+// An overdetermined system for the least-squares path: [P; G] with P the 5-point Poisson
+// operator on an m x m grid and G its forward differences (one row per interior grid edge),
+// column-scaled as the driver scales A.
+static spaqr::SparseMatrix least_squares2d(int m) {
+    const int n = m * m;
+    std::vector<std::vector<std::pair<int, double>>> cols(n);
+    for (int j = 0; j < n; ++j) {
+        const int i0 = j / m, i1 = j % m;
+        const int nb[5] = {j - m, j - 1, j, j + 1, j + m};
+        const bool ok[5] = {i0 > 0, i1 > 0, true, i1 < m - 1, i0 < m - 1};
+        for (int q = 0; q < 5; ++q)
+            if (ok[q]) cols[j].push_back({nb[q], q == 2 ? 4.0 : -1.0});
+    }
+    int r = n;
+    for (int ax = 0; ax < 2; ++ax)
+        for (int i0 = 0; i0 < m; ++i0)
+            for (int i1 = 0; i1 < m; ++i1) {
+                const int j = i0 * m + i1;
+                if ((ax == 0 && i0 == m - 1) || (ax == 1 && i1 == m - 1)) continue;
+                const int h = ax == 0 ? j + m : j + 1;
+                cols[h].push_back({r, 1.0});
+                cols[j].push_back({r, -1.0});
+                ++r;
+            }
+    spaqr::SparseMatrix A;
+    A.nr = r;
+    A.nc = n;
+    A.colptr.push_back(0);
+    for (int j = 0; j < n; ++j) {
+        double s = 0;
+        for (auto& e : cols[j]) s += e.second * e.second;
+        const double d = 10.0 / std::sqrt(s);
+        for (auto& e : cols[j]) {  // row indices increase within a column
+            A.rowind.push_back(e.first);
+            A.val.push_back(e.second * d);
+        }
+        A.colptr.push_back((int)A.rowind.size());
     }
     return A;
 }
@@ -154,9 +197,39 @@ int main() {
         for (int i = 0; i < n; ++i) de = std::fmax(de, std::fabs(xe[i] - xn[i]));
         if (de != 0.0) failed |= 8;
 
+        // least squares through cgls (solve_nrml as its preconditioner), ordered by the graph
+        // bisection (no coordinates); checked on the host: ||A^T (b - A x)|| / ||A^T b||
+        const spaqr::SparseMatrix L = least_squares2d(24);
+        std::vector<double> bl(L.nr);
+        for (auto& v : bl) v = dis(gen);
+        spaqr::ParTree tl(5, 0);
+        tl.set_tol(1e-2);
+        tl.set_scale(1);
+        tl.set_order(1.0f);
+        tl.set_square(0);
+        tl.partition(L);
+        tl.assemble(L);
+        tl.factorize();
+        std::vector<double> xl;
+        long ls_it = 200;
+        double ls_tol = 1e-12;
+        spaqr::cgls(L, bl, xl, tl, ls_it, ls_tol);
+        std::vector<double> rl = bl, gr(L.nc, 0.0), gb(L.nc, 0.0);
+        for (int j = 0; j < L.nc; ++j)
+            for (int k = L.colptr[j]; k < L.colptr[j + 1]; ++k) rl[L.rowind[k]] -= L.val[k] * xl[j];
+        for (int j = 0; j < L.nc; ++j)
+            for (int k = L.colptr[j]; k < L.colptr[j + 1]; ++k) {
+                gr[j] += L.val[k] * rl[L.rowind[k]];
+                gb[j] += L.val[k] * bl[L.rowind[k]];
+            }
+        double ng = 0, nb2 = 0;
+        for (int j = 0; j < L.nc; ++j) { ng += gr[j] * gr[j]; nb2 += gb[j] * gb[j]; }
+        const double ls_rel = std::sqrt(ng / nb2);
+        if (!(ls_rel <= 1e-11)) failed |= 64;
+
         std::printf("consumer: n=%d levels=6 gmres_iters=%lu relres=%.3e ops=%zu replay_diff=%g natural_diff=%g "
-                    "extract_diff=%g failed=%d\n",
-                    n, it, relres, t.operations().size(), dmax, dn, de, failed);
+                    "extract_diff=%g ls_m=%d ls_n=%d cgls_iters=%ld cgls_rel=%.3e ls_normal_rel=%.3e failed=%d\n",
+                    n, it, relres, t.operations().size(), dmax, dn, de, L.nr, L.nc, ls_it, ls_tol, ls_rel, failed);
     } catch (std::exception& e) {
         std::printf("consumer: error: %s\n", e.what());
         failed |= 16;

@@ -28,3 +28,4 @@ def test_cxx_consumer_runs_the_path():
     fields = dict(kv.split("=") for kv in line.split()[1:])
     assert float(fields["relres"]) <= 1e-12
     assert float(fields["replay_diff"]) == 0.0 and fields["failed"] == "0"
+    assert float(fields["ls_normal_rel"]) <= 1e-11 and int(fields["ls_m"]) > int(fields["ls_n"])

@@ -89,3 +89,50 @@ def test_square_only_paths_refuse_a_rectangular_tree():
     T.factorize()
     with pytest.raises(RuntimeError):
         T.gmres(np.zeros(P.A.shape[0]), 10)
+
+
+def test_square_tree_solve_nrml_matches_the_oracle():
+    """solve_nrml on a square tree (the transposed pass on the square structure)."""
+    from pspaqr_public_amd.problems import make_problem
+
+    P = make_problem(2, 12)
+    T = build_tree(P, 64)
+    T.factorize()
+    O = OracleTree(P.A, P.n_side, P.dim, P.nlevels, P.tol, P.skip)
+    O.factorize()
+    y = np.random.default_rng(5).standard_normal(P.A.shape[0])
+    assert _rel(T.solve_nrml(y), O.solve_nrml(y)) < 1e-10
+
+
+def test_matrix_market_least_squares(tmp_path):
+    """The driver's input path for an overdetermined A: a Matrix Market file, no coordinates
+    (graph bisection), b from std::mt19937(seed); cgls to 1e-12 in FP64."""
+    from pspaqr_public_amd import mmio
+    from pspaqr_public_amd.problems import load_problem
+
+    P0 = least_squares(20, 2, tol=1e-2, nlevels=5)
+    f = str(tmp_path / "ls.mtx")
+    mmio.write_sparse(f, P0.A)
+    P = load_problem(f, nlevels=5, tol=1e-2, seed=4)
+    assert P.A.shape == P0.A.shape and P.b.size == P0.A.shape[0]
+    T = build_tree(P, 64)
+    T.factorize()
+    rp, cp = T.rperm(), T.cperm()
+    x, it, rel = T.cgls(P.b[rp], 200, 1e-12)
+    assert rel < 1e-12
+    xd = np.linalg.lstsq(P.A.toarray(), P.b, rcond=None)[0][cp]
+    assert _rel(x, xd) < 1e-9
+
+
+def test_fp32_sketch_ranks_on_a_rectangular_tree():
+    """FP32 fronts with the sketched rank path (rank mode 2) for the column RRQR; the extra-row
+    RRQR stays geqp3.  cgls reaches 1e-12."""
+    P = _problem(32, 2, 6, 1e-2)
+    T = build_tree(P, 32)
+    T.set_rank_mode(2)
+    T.factorize()
+    rp, cp = T.rperm(), T.cperm()
+    x, it, rel = T.cgls(P.b[rp], 300, 1e-12)
+    assert rel < 1e-12
+    xd = np.linalg.lstsq(P.A.toarray(), P.b, rcond=None)[0][cp]
+    assert _rel(x, xd) < 1e-9
