@@ -189,6 +189,14 @@ int spaqr_fgmres(spaqr_tree* t, const double* b_dev, double* x_dev, int iters, d
 /* Tree::rows() / cols() and ParTree::nrows_local / ncols_local (one rank)  src/tree.cpp:23 */
 int spaqr_tree_n(const spaqr_tree* t);
 int spaqr_tree_m(const spaqr_tree* t);
+/* The per-matrix symbolic phase -- ParTree::partition and ParTree::assemble (src/ptree.cpp:
+   495-864): the geometric nested dissection, initPermutation, the block extraction, the SpMV
+   storage and the distance-two connections -- on the device (on != 0) for square systems with
+   coordinates; other systems keep the host phase.  The result is the host phase's: the same
+   permutation, clusters, blocks and SpMV storage.  Set before partition.  symbolic_ms: the
+   last partition's and assemble's wall time in ms (2 doubles). */
+int spaqr_tree_set_device_symbolic(spaqr_tree* t, int on);
+int spaqr_tree_symbolic_ms(const spaqr_tree* t, double* out);
 
 /* ---- Rectangular systems (least squares, m > n).  spaqr_tree_partition accepts nrows > ncols
    without coordinates (leaf parts or the graph bisection; the reference's geometric path assigns

@@ -98,6 +98,8 @@ EXPORTS = {
     "spaqr_fgmres": (_i, [_vp, _vp, _vp, _i, _d, _i, _d, C.POINTER(GmresResult), _vp, _i, C.POINTER(_i)]),
     "spaqr_tree_n": (_i, [_vp]),
     "spaqr_tree_m": (_i, [_vp]),
+    "spaqr_tree_set_device_symbolic": (_i, [_vp, _i]),
+    "spaqr_tree_symbolic_ms": (_i, [_vp, _vp]),
     "spaqr_tree_get_rperm": (_i, [_vp, _vp]),
     "spaqr_tree_permute_rows": (_i, [_vp, _i, _vp, _vp]),
     "spaqr_tree_get_rows": (_i, [_vp, _vp]),

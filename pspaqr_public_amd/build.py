@@ -18,7 +18,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SPAQR_ARCH", "gfx950")
 
 SOURCES = ["kernels.hip", "resident.hip", "learned.hip", "krylov.hip", "rrqr.cpp", "tree.cpp", "tree_rank.cpp",
-           "tree_batch.cpp", "status.hip", "capi.cpp", "noalloc.cpp", "mmio.cpp"]
+           "tree_batch.cpp", "status.hip", "capi.cpp", "noalloc.cpp", "mmio.cpp", "symbolic.hip"]
 
 # per-file flags: the resident-accumulator applies keep MFMA accumulators in VGPRs
 EXTRA = {"resident.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}

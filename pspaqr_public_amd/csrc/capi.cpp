@@ -337,6 +337,18 @@ int spaqr_fgmres(spaqr_tree* h, const double* b, double* x, int iters, double to
 }
 int spaqr_tree_n(const spaqr_tree* h) { return (h && h->t) ? h->t->n_unknowns() : -1; }
 int spaqr_tree_m(const spaqr_tree* h) { return (h && h->t) ? h->t->n_rows() : -1; }
+int spaqr_tree_set_device_symbolic(spaqr_tree* h, int on) {
+    TREE_OR_FAIL(h);
+    h->t->device_symbolic = on != 0;
+    return 0;
+}
+int spaqr_tree_symbolic_ms(const spaqr_tree* h, double* out) {
+    TREE_OR_FAIL(h);
+    if (!out) return -2;
+    out[0] = h->t->t_partition_ms;
+    out[1] = h->t->t_assemble_ms;
+    return 0;
+}
 int spaqr_tree_get_rperm(const spaqr_tree* h, int* out) {
     TREE_OR_FAIL(h);
     if (!out) return -2;

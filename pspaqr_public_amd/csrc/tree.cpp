@@ -18,6 +18,7 @@
 // comes after all lower-ordered interiors it conflicts with, which reproduces sequential
 // cluster order exactly.
 #include "tree.hpp"
+#include "symbolic.hpp"
 
 #include <algorithm>
 #include <array>
@@ -935,16 +936,26 @@ int Tree::partition(const HostCSC& A) {
             if (!std::isfinite(v)) throw std::runtime_error("coordinates must be finite");
         if (!geo && !user_parts.empty()) check_parts(user_parts, 0, A.nc, nlevels);
         check_matrix(A);
+        auto tp0 = std::chrono::steady_clock::now();
         A0 = A;
         n = A.nc;
         m = A.nr;
         inst_n.assign(1, n);
         inst_lv.assign(1, nlevels);
         inst_off.assign(1, 0);
-        order_system(A, geo ? X.data() : nullptr, xdim, (!geo && !user_parts.empty()) ? user_parts.data() : nullptr,
-                     nlevels, cperm_, cpermed, &leaf_parts);
+        symdev.reset();
+        if (device_symbolic && geo && m == n && nlevels > 1) {
+            order_device(A);
+            if (getenv("SPAQR_DEBUG_HOST"))
+                fprintf(stderr, "[partition/device] total %.1f ms\n",
+                        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp0).count());
+        } else {
+            order_system(A, geo ? X.data() : nullptr, xdim, (!geo && !user_parts.empty()) ? user_parts.data() : nullptr,
+                         nlevels, cperm_, cpermed, &leaf_parts);
+        }
         form_rmap();
         make_hierarchy();
+        t_partition_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp0).count();
         return 0;
     } catch (std::exception& e) {
         last_error = e.what();
@@ -1012,6 +1023,7 @@ int Tree::partition_batch(const HostCSC& A, const std::vector<int>& sizes, const
         });
         inst_off.pop_back();
         m = n;
+        symdev.reset();
         form_rmap();
         make_hierarchy();
         return 0;
@@ -1026,6 +1038,12 @@ int Tree::assemble() {
     try {
         if (cl_part.empty()) throw std::runtime_error("partition() first");
         ensure_stream();
+        auto ta0 = std::chrono::steady_clock::now();
+        if (symdev && !rect() && n_systems() == 1) {
+            assemble_device();
+            t_assemble_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta0).count();
+            return 0;
+        }
         cl = cl_part;  // a factorization since partition() has resized and flagged clusters
         App = permute_rc(A0, rperm_, cperm_);
         ed.clear();
@@ -1121,12 +1139,172 @@ int Tree::assemble() {
             HIPCHK(hipMemcpy(d_tval, App.val.data(), sizeof(double) * App.val.size(), hipMemcpyHostToDevice));
         }
         if (!d_scratch) HIPCHK(hipMalloc(&d_scratch, sizeof(double) * (dot_blocks + 4096)));
+        n_blocks0 = h_blocks0.size();
         factorized = false;
+        t_assemble_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta0).count();
         return 0;
     } catch (std::exception& e) {
         last_error = e.what();
         return -1;
     }
+}
+
+// The geometric ordering on the device (SymbolicDevice::order); the ranks of the distinct
+// level-0 IDs follow initPermutation (src/tree.cpp:48-57): its successive stable sorts by the
+// IDs merged up to each level order the vertices by the top level's merged ID, then the next
+// level's, ..., then the level-0 ID, then the natural index.
+void Tree::order_device(const HostCSC& A) {
+    ensure_stream();
+    symdev.reset(new SymbolicDevice());
+    auto rank_ids = [this](const std::vector<DevId>& ids) {
+        const int nd = (int)ids.size();
+        std::vector<std::vector<ClusterID>> chain(nd, std::vector<ClusterID>(nlevels));
+        for (int q = 0; q < nd; ++q) {
+            ClusterID c;
+            c.self = SepID(ids[q].slvl, ids[q].ssep);
+            c.l = SepID(ids[q].llvl, ids[q].lsep);
+            c.r = SepID(ids[q].rlvl, ids[q].rsep);
+            c.section = ids[q].section;
+            chain[q][0] = c;
+            for (int lvl = 1; lvl < nlevels; ++lvl) chain[q][lvl] = merge_if(chain[q][lvl - 1], lvl);
+        }
+        std::vector<int> ord(nd);
+        for (int q = 0; q < nd; ++q) ord[q] = q;
+        std::sort(ord.begin(), ord.end(), [&](int a, int b) {
+            for (int lvl = nlevels - 1; lvl >= 0; --lvl)
+                if (chain[a][lvl] != chain[b][lvl]) return chain[a][lvl] < chain[b][lvl];
+            return a < b;
+        });
+        std::vector<int> rank(nd);
+        for (int q = 0; q < nd; ++q) rank[ord[q]] = q;
+        return rank;
+    };
+    std::vector<int> did;
+    std::vector<DevId> ids;
+    symdev->order(A.nc, A.colptr.data(), A.rowind.data(), A.val.data(), X.data(), xdim, nlevels, rank_ids, cperm_, did,
+                  ids, leaf_parts, stream_);
+    cpermed.resize(n);
+    for (int k = 0; k < n; ++k) {
+        const DevId& d = ids[did[cperm_[k]]];
+        ClusterID c;
+        c.self = SepID(d.slvl, d.ssep);
+        c.l = SepID(d.llvl, d.lsep);
+        c.r = SepID(d.rlvl, d.rsep);
+        c.section = d.section;
+        cpermed[k] = c;
+    }
+}
+
+// ParTree::assemble (src/ptree.cpp:696-864) from the device ordering's matrix: the edges
+// (every level-0 cluster pair joined by a nonzero, in the order the host phase creates them),
+// the blocks scattered straight into the factor precision, App's CSR for SpMV, and the
+// distance-two connections from A^T A's cluster pairs by the same rule as the host phase.
+void Tree::assemble_device() {
+    static const bool dbg = getenv("SPAQR_DEBUG_HOST") != nullptr;
+    auto tp = std::chrono::steady_clock::now();
+    auto lap = [&](const char* what) {
+        if (!dbg) return;
+        sync_stream();
+        const auto nw = std::chrono::steady_clock::now();
+        fprintf(stderr, "[assemble/device] %s %.1f ms\n", what, std::chrono::duration<double, std::milli>(nw - tp).count());
+        tp = nw;
+    };
+    cl = cl_part;
+    ed.clear();
+    for (auto& c : cl) { c.out.clear(); c.in.clear(); c.sout.clear(); c.sin.clear(); c.self_edge = -1; c.dist2.clear(); }
+    current_bottom = 0;
+    const int n0 = (int)bottoms[0].size();
+    std::vector<int> owner(n), cstart(n0), cols(n0);
+    for (int q = 0; q < n0; ++q) {
+        const int o = bottoms[0][q];
+        if (o != q) throw std::runtime_error("level-0 clusters out of order");
+        cstart[q] = cl[o].cstart;
+        cols[q] = cl[o].cols;
+        for (int k = cl[o].cstart; k < cl[o].cstart + cl[o].cols; ++k) owner[k] = o;
+    }
+    std::vector<long long> keys;
+    lap("tables");
+    symdev->edges(cperm_, owner, n0, keys, stream_);
+    lap("edge keys");
+    h_blocks0.clear();
+    h_block_off.clear();
+    std::vector<long long> off;
+    off.reserve(keys.size());
+    size_t nel = 0;
+    for (size_t q = 0; q < keys.size(); ++q) {
+        const int s = (int)(keys[q] / n0), o = (int)(keys[q] % n0);
+        const int id = new_edge(s, o);
+        h_block_off.push_back(nel);
+        off.push_back((long long)nel);
+        nel += (size_t)ed[id].rows * ed[id].cols;
+        if (q + 1 == keys.size() || keys[q + 1] / n0 != s) sort_out(cl[s]);
+    }
+    n_blocks0 = nel;
+    lap("edges (host)");
+    if (d_blocks0) { (void)hipFree(d_blocks0); d_blocks0 = nullptr; }
+    const size_t es = (prec == 32) ? 4 : 8;
+    HIPCHK(hipMalloc(&d_blocks0, std::max<size_t>(nel * es, 256)));
+    prec_asm = prec;
+    ++asm_gen;
+    drop_graphs();
+    if (prec == 32) symdev->scatter_blocks<float>(off, cstart, cols, (float*)d_blocks0, nel, stream_);
+    else symdev->scatter_blocks<double>(off, cstart, cols, (double*)d_blocks0, nel, stream_);
+    lap("blocks");
+    // distance-two connections (src/ptree.cpp:832-854)
+    std::vector<long long> pairs;
+    symdev->ata_pairs(pairs, stream_);
+    lap("A^T A pairs");
+    // The host phase visits the clusters in order and a lower-level neighbour p contributes its
+    // own list, which is final when p comes before s and still empty when it comes after: so
+    // s depends only on earlier clusters of lower levels, and the clusters of one level (in
+    // increasing level) are independent of each other.
+    {
+        std::vector<size_t> first(n0 + 1, pairs.size());
+        for (size_t q = pairs.size(); q-- > 0;) first[pairs[q] / n0] = q;
+        for (int s = n0 - 1; s >= 0; --s) first[s] = std::min(first[s], first[s + 1]);
+        int lmax = 0;
+        for (int s = 0; s < n0; ++s) lmax = std::max(lmax, cl[s].lvl());
+        std::vector<std::vector<int>> by_lvl(lmax + 1);
+        for (int s = 0; s < n0; ++s) by_lvl[cl[s].lvl()].push_back(s);
+        for (const auto& group : by_lvl)
+            parallel_ranges((int)group.size(), 64, [&](int b, int e, int) {
+                std::vector<int> d2;
+                for (int g = b; g < e; ++g) {
+                    const int s = group[g];
+                    d2.clear();
+                    for (size_t q = first[s]; q < first[s + 1]; ++q) {
+                        const int p = (int)(pairs[q] % n0);
+                        if (cl[p].lvl() >= cl[s].lvl() && p != s) d2.push_back(p);
+                        else if (cl[p].lvl() < cl[s].lvl() && p < s)
+                            d2.insert(d2.end(), cl[p].dist2.begin(), cl[p].dist2.end());
+                    }
+                    std::sort(d2.begin(), d2.end());
+                    d2.erase(std::unique(d2.begin(), d2.end()), d2.end());
+                    cl[s].dist2 = d2;
+                }
+            });
+    }
+    lap("distance two (host)");
+    cl0 = cl;
+    ed0 = ed;
+    lap("pristine copies");
+    if (d_rowptr) (void)hipFree(d_rowptr);
+    if (d_col) (void)hipFree(d_col);
+    if (d_val) (void)hipFree(d_val);
+    d_rowptr = nullptr;
+    d_col = nullptr;
+    d_val = nullptr;
+    symdev->csr(&d_rowptr, &d_col, &d_val, stream_);
+    lap("csr");
+    d_nnz = (long long)A0.colptr[n];
+    if (d_tptr) (void)hipFree(d_tptr);
+    if (d_tcol) (void)hipFree(d_tcol);
+    if (d_tval) (void)hipFree(d_tval);
+    d_tptr = nullptr;
+    d_tcol = nullptr;
+    d_tval = nullptr;
+    if (!d_scratch) HIPCHK(hipMalloc(&d_scratch, sizeof(double) * (dot_blocks + 4096)));
+    factorized = false;
 }
 
 // ---------------------------------------------------------------------------- fill-in
@@ -1326,8 +1504,8 @@ int Tree::factorize_t() {
     seg_begin();
     // level-0 blocks: copy of the pristine assembled blocks
     {
-        S* base = (S*)arena.alloc(std::max<size_t>(h_blocks0.size(), 1) * sizeof(S));
-        HIPCHK(hipMemcpyAsync(base, d_blocks0, h_blocks0.size() * sizeof(S), hipMemcpyDeviceToDevice, stream_));
+        S* base = (S*)arena.alloc(std::max<size_t>(n_blocks0, 1) * sizeof(S));
+        HIPCHK(hipMemcpyAsync(base, d_blocks0, n_blocks0 * sizeof(S), hipMemcpyDeviceToDevice, stream_));
         for (size_t i = 0; i < ed.size(); ++i) ed[i].A = base + h_block_off[i];
     }
     d_chain_err = (int*)arena.alloc(sizeof(int));

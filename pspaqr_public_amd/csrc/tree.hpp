@@ -13,7 +13,11 @@
 
 #include "kernels.hpp"
 
+#include <memory>
+
 namespace spaqr {
+
+class SymbolicDevice;
 
 struct SepID {
     int lvl = -1, sep = 0;
@@ -145,6 +149,11 @@ public:
     std::vector<double> X;  // coordinates, d x n
     int xdim = 0;
     std::vector<int> user_parts;  // leaf parts given in place of coordinates (the METIS path)
+    // The per-matrix symbolic phase (ordering and assembly) on the device: square systems on the
+    // geometric path (coordinates).  The same permutation, clusters, blocks and SpMV storage as
+    // the host phase; other systems use the host phase.
+    bool device_symbolic = false;
+    double t_partition_ms = 0.0, t_assemble_ms = 0.0;  // the last partition() / assemble()
     std::vector<int> leaf_parts;  // the leaf parts of the last partition, natural order
 
     // A may have more rows than columns (m > n, a least-squares system): it is then ordered
@@ -245,6 +254,10 @@ private:
     std::vector<int> cperm_, rperm_;
     std::vector<int> c2r_count;  // rows filed under each permuted column (all 1 for square systems)
     void form_rmap();
+    std::unique_ptr<SymbolicDevice> symdev;  // the device symbolic phase's matrix, from partition to assemble
+    size_t n_blocks0 = 0;                    // entries of the assembled level-0 blocks
+    void order_device(const HostCSC& A);
+    void assemble_device();
     std::vector<int> inst_n, inst_lv, inst_off;  // per system: unknowns, levels, first position
     int* d_seg = nullptr;                        // device copy of inst_off plus n (batch GMRES)
     void make_hierarchy();

@@ -72,6 +72,16 @@ class ParTree:
     def set_precision(self, bits: int):
         check(lib().spaqr_tree_set_precision(self._h, int(bits)), self._h, "set_precision")
 
+    def set_device_symbolic(self, on: bool = True):
+        """Partition and assemble on the device (square systems with coordinates)."""
+        check(lib().spaqr_tree_set_device_symbolic(self._h, 1 if on else 0), self._h, "set_device_symbolic")
+
+    def symbolic_ms(self):
+        """(partition ms, assemble ms) of the last partition and assemble."""
+        out = np.zeros(2)
+        check(lib().spaqr_tree_symbolic_ms(self._h, out.ctypes.data), self._h, "symbolic_ms")
+        return float(out[0]), float(out[1])
+
     def set_Xcoo(self, X: np.ndarray):
         X = np.asfortranarray(X, dtype=np.float64)
         self._X = X
@@ -363,10 +373,12 @@ class ParTree:
         return self._h
 
 
-def build_tree(problem, precision: int = 64, nlevels=None, tol=None, skip=None) -> ParTree:
+def build_tree(problem, precision: int = 64, nlevels=None, tol=None, skip=None, device_symbolic: bool = False) -> ParTree:
     """The reference driver's set-up sequence (pspaQR.cpp:183-198) for a Problem."""
     nl = problem.nlevels if nlevels is None else nlevels
     t = ParTree(nl, problem.skip if skip is None else skip)
+    if device_symbolic:
+        t.set_device_symbolic(True)
     t.set_scale(1)
     t.set_tol(problem.tol if tol is None else tol)
     t.set_order(1)
