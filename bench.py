@@ -162,8 +162,15 @@ def main():
             wkey = None
         log("[bench] %s n=%d nnz=%d levels=%d tol=%g" % (W.name, n_total, W.A.nnz, W.nlevels, W.tol))
 
+    # the per-matrix symbolic phase (partition + assemble, outside the timed step) runs on the
+    # device for the FP32 trees and on the host for the FP64 one, both timed
+    symbolic = {}
+
     def prepare(prec, rank_mode=0, tuned=True):
-        t = build_batch_tree(W, prec) if batch else build_tree(W, prec)
+        dev = prec == 32 and not batch
+        t = build_batch_tree(W, prec) if batch else build_tree(W, prec, device_symbolic=dev)
+        if not batch:
+            symbolic.setdefault("device_ms" if dev else "host_ms", [round(v, 2) for v in t.symbolic_ms()])
         set_rank_mode(t, rank_mode)
         used = autotune.apply(t, wkey, prec) if (tuned and wkey) else {}
         if prec == 32 and args.tune:
@@ -324,6 +331,7 @@ def main():
         "factor_gflop": round(float(stats[:, 7].sum()) / 1e9, 3),
         "rank_mode": args.rank_mode,
         "rank_stats_per_step": {k: (v / max(args.steps, 1)) for k, v in rstats.items()},
+        "symbolic_partition_assemble_ms": symbolic or None,
         "roofline": roof,
         "cpu_baseline": cpu,
         "per_class": per_class,
