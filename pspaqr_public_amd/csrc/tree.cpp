@@ -418,6 +418,7 @@ struct Tree::SolveSchedule {
     void* ld_si = nullptr; void* ld_di = nullptr; int n_ld = 0;
     void* lc_si = nullptr; void* lc_di = nullptr; int n_lc = 0;
     void* st_si = nullptr; void* st_di = nullptr; int n_st = 0;
+    bool tr_built = false;  // the transposed pass's tasks (built on solve_nrml's first use)
 };
 
 Tree::Tree(int nl, int sk) : nlevels(nl), skip(sk) {
@@ -2892,13 +2893,12 @@ void Tree::solve_sched_levels(int upto) {
         if (sched->off * sizeof(S) > xw_bytes) throw std::runtime_error("solve workspace bound exceeded");
         auto& lv = sched->lv[L];
         if (L > 0) {
-            std::vector<VecCopyTask<S>> mf, mb, mt;
+            std::vector<VecCopyTask<S>> mf, mb;
             for (int p : bottoms[L]) {
                 int k = 0, kc = 0;
                 for (int ch : cl[p].children) {
                     mf.push_back({X0 + cl[ch].xoff, X0 + cl[p].xoff + k, cl[ch].rows, 0});
                     mb.push_back({X0 + cl[p].xoff + kc, X0 + cl[ch].xoff, cl[ch].cols, 0});
-                    mt.push_back({X0 + cl[ch].xoff, X0 + cl[p].xoff + kc, cl[ch].cols, 0});
                     k += cl[ch].rows;
                     kc += cl[ch].cols;
                 }
@@ -2907,8 +2907,6 @@ void Tree::solve_sched_levels(int upto) {
             lv.n_merge_fwd = (int)mf.size();
             lv.merge_bwd = upload_vec(mb);
             lv.n_merge_bwd = (int)mb.size();
-            lv.merge_tr = upload_vec(mt);
-            lv.n_merge_tr = (int)mt.size();
         }
         if (rect() && L == 0) {  // loads and stores through the scratch region
             std::vector<long long> a, b, c, d, e, f;
@@ -2953,27 +2951,6 @@ void Tree::solve_sched_levels(int upto) {
             lv.qex = upload_vec(qx); lv.n_qex = (int)qx.size();
             lv.sp_a = upload_vec(sa); lv.sp_b = upload_vec(sb); lv.n_sp = (int)sa.size();
             lv.ts_a = upload_vec(ta); lv.ts_b = upload_vec(tb); lv.n_ts = (int)ta.size();
-        }
-        {  // transposed couplings: per source cluster of an interior's in-edge, links in interior order
-            std::map<int, std::pair<int, std::vector<VecGemvtLink<S>>>> by_src;  // source -> (its columns, links)
-            for (int c : interiors[L]) {
-                const Cluster& C = cl[c];
-                for (int e : C.in) {
-                    const Edge& ei = ed[e];
-                    auto& v = by_src[ei.n1];
-                    v.first = ei.cols;
-                    v.second.push_back({(const S*)ei.A, X0 + C.xoff, ei.rows, C.cols});
-                }
-            }
-            std::vector<VecGemvtTask<S>> gt;
-            std::vector<VecGemvtLink<S>> gl;
-            for (auto& kv : by_src) {
-                gt.push_back({X0 + cl[kv.first].xoff, kv.second.first, (int)gl.size(), (int)kv.second.second.size(), 0});
-                gl.insert(gl.end(), kv.second.second.begin(), kv.second.second.end());
-            }
-            lv.gemvt = upload_vec(gt);
-            lv.n_gemvt = (int)gt.size();
-            lv.gemvt_links = upload_vec(gl);
         }
         std::vector<VecQTask<S>> q;
         int ncol = 0;
@@ -3103,6 +3080,51 @@ void Tree::solve_sched_levels(int upto) {
     }
 }
 
+// The transposed backward pass's tasks (solve_nrml only), built on its first use after a
+// factorization from the finished structure: merge^T and the transposed couplings per level.
+template <typename S>
+void Tree::solve_sched_tr() {
+    if (sched->tr_built) return;
+    S* X0 = (S*)xw;
+    for (int L = 0; L < nlevels; ++L) {
+        auto& lv = sched->lv[L];
+        if (L > 0) {
+            std::vector<VecCopyTask<S>> mt;
+            for (int p : bottoms[L]) {
+                int kc = 0;
+                for (int ch : cl[p].children) {
+                    mt.push_back({X0 + cl[ch].xoff, X0 + cl[p].xoff + kc, cl[ch].cols, 0});
+                    kc += cl[ch].cols;
+                }
+            }
+            lv.merge_tr = upload_vec(mt);
+            lv.n_merge_tr = (int)mt.size();
+        }
+        {  // transposed couplings: per source cluster of an interior's in-edge, links in interior order
+            std::map<int, std::pair<int, std::vector<VecGemvtLink<S>>>> by_src;  // source -> (its columns, links)
+            for (int c : interiors[L]) {
+                const Cluster& C = cl[c];
+                for (int e : C.in) {
+                    const Edge& ei = ed[e];
+                    auto& v = by_src[ei.n1];
+                    v.first = ei.cols;
+                    v.second.push_back({(const S*)ei.A, X0 + C.xoff, ei.rows, C.cols});
+                }
+            }
+            std::vector<VecGemvtTask<S>> gt;
+            std::vector<VecGemvtLink<S>> gl;
+            for (auto& kv : by_src) {
+                gt.push_back({X0 + cl[kv.first].xoff, kv.second.first, (int)gl.size(), (int)kv.second.second.size(), 0});
+                gl.insert(gl.end(), kv.second.second.begin(), kv.second.second.end());
+            }
+            lv.gemvt = upload_vec(gt);
+            lv.n_gemvt = (int)gt.size();
+            lv.gemvt_links = upload_vec(gl);
+        }
+    }
+    sched->tr_built = true;
+}
+
 // One phase of one level of the preconditioner apply on the solve workspace (ParTree::solve,
 // src/ptree.cpp:2798-3203): 0 merge (merge_fwd :2244-2252 / merge_bwd :3105-3123), 1 interior
 // QR (fwd_geqrt :2846-2851 and fwd_tsqrt :2160-2186 / bwd_trsv :2188-2192 with the reduction of
@@ -3218,6 +3240,7 @@ int Tree::solve_ls_t(const double* b, double* x) {
 
 template <typename S>
 int Tree::solve_nrml_t(const double* y, double* x) {
+    solve_sched_tr<S>();
     load_cols_t<S>(y);
     for (int L = 0; L < nlevels; ++L) solve_phase_tr<S>(L);
     for (int L = nlevels - 1; L >= 0; --L)

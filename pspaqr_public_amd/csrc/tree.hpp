@@ -377,6 +377,7 @@ private:
     template <typename S> int solve_ls_t(const double* b, double* x);
     template <typename S> int solve_nrml_t(const double* y, double* x);
     template <typename S> void solve_phase_tr(int level);  // one level of the transposed backward pass
+    template <typename S> void solve_sched_tr();
     template <typename S> void move_t(const void* si, const void* di, int cnt, bool staged);
     template <typename S> void load_rows_t(const double* b);
     template <typename S> void load_cols_t(const double* y);

@@ -60,10 +60,20 @@ def _kernels():
 
 
 def test_no_kernel_uses_scratch_or_spills():
+    """The rule holds for every hand-written kernel.  rocPRIM's onesweep radix sort (the
+    symbolic phase's sorts, instantiated from the library's headers) keeps a small per-lane
+    array in the private segment by its own design (tens of bytes, no spilled registers): it is
+    held to no spills, no dynamic stack and at most 256 bytes instead."""
     ks = _kernels()
     assert len(ks) > 100  # every translation unit's kernels were found
-    bad = {k: v for k, v in ks.items()
+    lib = {k: v for k, v in ks.items() if k.startswith("_ZN7rocprim")}
+    own = {k: v for k, v in ks.items() if k not in lib}
+    bad = {k: v for k, v in own.items()
            if v["private_segment_fixed_size"] != "0" or v["uses_dynamic_stack"] != "false"
+           or v["vgpr_spill_count"] != "0"}
+    assert not bad, bad
+    bad = {k: v for k, v in lib.items()
+           if int(v["private_segment_fixed_size"]) > 256 or v["uses_dynamic_stack"] != "false"
            or v["vgpr_spill_count"] != "0"}
     assert not bad, bad
     assert max(int(v["group_segment_fixed_size"]) for v in ks.values()) <= 160 * 1024
