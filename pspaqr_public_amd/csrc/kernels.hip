@@ -62,7 +62,8 @@ struct alignas(16) ColPanelSmem {
     S rowj[2][32];          // row j of the top block at step j
     S tau[32], scl[32];     // tau_j and the scale 1 / (alpha_j - beta_j) of reflector j
     S T[32 * 32];           // the panel's T, T(q, j) at j * 32 + q
-    S xb[8][kSegMax];       // column j's register rows, per segment
+    S xb[8][kSegMax + 4];   // column j's register rows, per segment (padded: the two
+                            // segments of a wave read from different banks)
 };
 template <typename S>
 union PanelSmemCol {
@@ -101,6 +102,23 @@ __device__ __forceinline__ S householder_norm_signed(S alpha, S ss) {
 }
 
 
+// tau, scal = 1 / (alpha - beta) and beta of the reflector of one panel step, from alpha and
+// ss = ||x||^2 > 0.  This scalar chain sits on the critical path of every column step, so FP32
+// uses the hardware square root and reciprocal (each within 1 ulp) in place of the correctly
+// rounded sequences; FP64 keeps the exact operations.  Same sign rule as householder_norm_signed.
+__device__ __forceinline__ void householder_fast(float alpha, float ss, float& tau, float& scal, float& beta) {
+    const float nrm = __builtin_amdgcn_sqrtf(alpha * alpha + ss);
+    const bool noise = alpha * alpha <= 1e-10f * ss;  // |alpha| <= 1e-5 ||x||
+    beta = noise ? -nrm : -copysignf(nrm, alpha);
+    tau = (beta - alpha) * __builtin_amdgcn_rcpf(beta);
+    scal = __builtin_amdgcn_rcpf(alpha - beta);
+}
+__device__ __forceinline__ void householder_fast(double alpha, double ss, double& tau, double& scal, double& beta) {
+    beta = -householder_norm_signed(alpha, ss);
+    tau = (beta - alpha) / beta;
+    scal = 1.0 / (alpha - beta);
+}
+
 // ---------------------------------------------------------------------------------------
 // Panel QR.  PENT = false: the panel is A (m x nbp, ld ldr == ldb, Rt == B): reflector j
 // is [1; A(j+1:m, j)] acting on rows j..m-1.  PENT = true: the panel is [R (nbp x nbp
@@ -132,9 +150,7 @@ __device__ __forceinline__ void wg_panel_qr_mem(int m, int nbp, S* Rt, int ldr, 
             if (ss == S(0)) {
                 tau = S(0); scal = S(1); beta = alpha;
             } else {
-                beta = -householder_norm_signed(alpha, ss);
-                tau = (beta - alpha) / beta;
-                scal = S(1) / (alpha - beta);
+                householder_fast(alpha, ss, tau, scal, beta);
             }
             Rt[j + (size_t)j * ldr] = beta;
             sm.tau = tau;
@@ -248,9 +264,7 @@ __device__ __forceinline__ void wg_panel_qr_reg(int m, int nbp, S* Rt, int ldr, 
         if (ss == S(0)) {
             tau = S(0); scal = S(1); beta = alpha;
         } else {
-            beta = -householder_norm_signed(alpha, ss);
-            tau = (beta - alpha) / beta;
-            scal = S(1) / (alpha - beta);
+            householder_fast(alpha, ss, tau, scal, beta);
         }
         S wl[32];
 #pragma unroll
@@ -345,6 +359,8 @@ template <typename S, bool PENT, int SR>
 __device__ __forceinline__ void wg_panel_qr_col(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* Tout, int ldt,
                                                 ColPanelSmem<S>& sm) {
     static_assert(SR % 4 == 0 && SR <= ColPanelSmem<S>::kSegMax, "segment length");
+    // short segments keep column j's rows in registers between the dot product and the update
+    constexpr bool kKeep = SR <= 32;
     const int tid = threadIdx.x, lane = tid & 63, c = lane & 31;
     const int g = 2 * (tid >> 6) + (lane >> 5);
     int r0 = (PENT ? 0 : nbp) + g * SR;  // this thread's first register row
@@ -365,6 +381,10 @@ __device__ __forceinline__ void wg_panel_qr_col(int m, int nbp, S* Rt, int ldr, 
 #pragma unroll
     for (int i = 0; i < SR; ++i) a[i] = (r0 + i < m && c < nbp) ? a[i] : S(0);
     __syncthreads();
+    // square panel: row j of the top block as it stands at step j goes to rowj[j & 1], written by
+    // the segment that updates that row (at step j - 1; row 0 here) and read after step j's
+    // barrier, so that the row's own update at step j never races with its readers
+    if (!PENT && g == 0) sm.rowj[0][c] = sm.top[c];
     S* xs = sm.xb[g];
     for (int j = 0; j < nbp; ++j) {
         const int par = j & 1;
@@ -373,20 +393,42 @@ __device__ __forceinline__ void wg_panel_qr_col(int m, int nbp, S* Rt, int ldr, 
 #pragma unroll
             for (int i = 0; i < SR; ++i) xs[i] = a[i];
         }
-        if ((j & 7) == g) sm.rowj[par][c] = sm.top[j * 33 + c];
         wave_sync();
-        S d0 = S(0), d1 = S(0);
-        col_chunks<S, SR>(xs, [&](int i, S x) {
-            if (i & 1) d1 += x * a[i]; else d0 += x * a[i];
-        });
-        S d = d0 + d1;
+        // pentagonal panel: row j of R changes only at step j (reflector j is [e_j; B(:, j)]), so
+        // it is read before the barrier, ahead of its own update
+        S alphaP = S(0), rowcP = S(0);
+        if (PENT) {
+            alphaP = sm.top[j * 33 + j];
+            rowcP = sm.top[j * 33 + c];
+        }
+        S xr[kKeep ? SR : 1];
+        S d = S(0);
+        if constexpr (kKeep) {
+#pragma unroll
+            for (int i = 0; i < SR; ++i) xr[i] = xs[i];
+            S d4[4] = {S(0), S(0), S(0), S(0)};
+#pragma unroll
+            for (int i = 0; i < SR; ++i) d4[i & 3] += xr[i] * a[i];
+            d = (d4[0] + d4[1]) + (d4[2] + d4[3]);
+        } else {
+            S d0 = S(0), d1 = S(0);
+            col_chunks<S, SR>(xs, [&](int i, S x) {
+                if (i & 1) d1 += x * a[i]; else d0 += x * a[i];
+            });
+            d = d0 + d1;
+        }
+        // square panel: the segment's four top rows (this segment's own writes, in program order),
+        // read once for the dot product and the update
+        S tr[4], tj[4];
         if (!PENT) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int r = g + 8 * q;
-                const S x = sm.top[r * 33 + j], y = sm.top[r * 33 + c];
-                d += (r > j) ? x * y : S(0);
+                tr[q] = sm.top[r * 33 + c];
+                tj[q] = sm.top[r * 33 + j];
             }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) d += (g + 8 * q > j) ? tj[q] * tr[q] : S(0);
         }
         sm.red[par][g][c] = d;
         __syncthreads();
@@ -398,14 +440,12 @@ __device__ __forceinline__ void wg_panel_qr_col(int m, int nbp, S* Rt, int ldr, 
             ss = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
             dc = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
         }
-        const S alpha = sm.rowj[par][j], rowc = sm.rowj[par][c];
+        const S alpha = PENT ? alphaP : sm.rowj[par][j], rowc = PENT ? rowcP : sm.rowj[par][c];
         S tau, scal, beta;
         if (ss == S(0)) {
             tau = S(0); scal = S(1); beta = alpha;
         } else {
-            beta = -householder_norm_signed(alpha, ss);
-            tau = (beta - alpha) / beta;
-            scal = S(1) / (alpha - beta);
+            householder_fast(alpha, ss, tau, scal, beta);
         }
         // w_c = tau v_j^T a_c for the trailing columns; v_j = [e_j; scal x_j]
         const S vta = (PENT ? S(0) : rowc) + scal * dc;
@@ -413,26 +453,26 @@ __device__ __forceinline__ void wg_panel_qr_col(int m, int nbp, S* Rt, int ldr, 
         const S sw = scal * w;
         if (g == 0 && c < j) sm.G[j * 32 + c] = sm.scl[c] * vta;  // column c is still unscaled
         if (tid == 0) { sm.tau[j] = tau; sm.scl[j] = scal; }
-        col_chunks<S, SR>(xs, [&](int i, S x) { a[i] -= x * sw; });
-        if (!PENT) {
-            // the segment's four top rows: every read issued, then every write (unchanged
-            // entries written back as read), no branch between them.  Lanes of one segment
-            // are one half-wave, so the column-j reads precede lane j's write of its diagonal.
-            S tr[4], tj[4];
+        if constexpr (kKeep) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int r = g + 8 * q;
-                tr[q] = sm.top[r * 33 + c];
-                tj[q] = sm.top[r * 33 + j];
-            }
+            for (int i = 0; i < SR; ++i) a[i] -= xr[i] * sw;
+        } else {
+            col_chunks<S, SR>(xs, [&](int i, S x) { a[i] -= x * sw; });
+        }
+        if (!PENT) {
+            // every write after every read of the step; unchanged entries written back as read.
+            // Lanes of one segment are one half-wave, so the column-j reads precede lane j's
+            // write of its diagonal.
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int r = g + 8 * q;
                 const S upd = (r > j) ? tr[q] - tj[q] * sw : tr[q] - w;
-                sm.top[r * 33 + c] = (r < j) ? tr[q] : ((r == j && c == j) ? beta : upd);
+                const S nv = (r < j) ? tr[q] : ((r == j && c == j) ? beta : upd);
+                sm.top[r * 33 + c] = nv;
+                if (r == j + 1) sm.rowj[par ^ 1][c] = nv;
             }
         } else if ((j & 7) == g) {
-            sm.top[j * 33 + c] = (c == j) ? beta : sm.top[j * 33 + c] - w;
+            sm.top[j * 33 + c] = (c == j) ? beta : rowcP - w;
         }
     }
     __syncthreads();
@@ -1234,9 +1274,7 @@ __global__ __launch_bounds__(64) void k_geqrt_wave(const GeqrtTask<S>* __restric
                 if (ss == S(0)) {
                     tau = S(0); scal = S(1); beta = alpha;
                 } else {
-                    beta = -householder_norm_signed(alpha, ss);
-                    tau = (beta - alpha) / beta;
-                    scal = S(1) / (alpha - beta);
+                    householder_fast(alpha, ss, tau, scal, beta);
                 }
 #pragma unroll
                 for (int r = 0; r < MR; ++r) {
@@ -1330,9 +1368,7 @@ __global__ __launch_bounds__(64) void k_tpqrt_wave(const TpqrtTask<S>* __restric
                     if (ss == S(0)) {
                         tau = S(0); scal = S(1); beta = alpha;
                     } else {
-                        beta = -householder_norm_signed(alpha, ss);
-                        tau = (beta - alpha) / beta;
-                        scal = S(1) / (alpha - beta);
+                        householder_fast(alpha, ss, tau, scal, beta);
                     }
 #pragma unroll
                     for (int i = 0; i < 64; ++i) {
