@@ -355,7 +355,7 @@ __device__ __forceinline__ void col_chunks(const S* xs, F&& f) {
 // so column j needs no case of its own.  The same dot products give the Gram column
 // G(p, j) = v_p^T v_j of the finished columns, from which wave 0 builds T at the end.
 // Same reflectors (sign rule included), R, V and T as wg_panel_qr_mem.
-template <typename S, bool PENT, int SR>
+template <typename S, bool PENT, int SR, bool VEC = false>
 __device__ __forceinline__ void wg_panel_qr_col(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* Tout, int ldt,
                                                 ColPanelSmem<S>& sm) {
     static_assert(SR % 4 == 0 && SR <= ColPanelSmem<S>::kSegMax, "segment length");
@@ -375,9 +375,19 @@ __device__ __forceinline__ void wg_panel_qr_col(int m, int nbp, S* Rt, int ldr, 
         sm.top[r * 33 + q] = (r < nbp && q < nbp && (!PENT || r <= q)) ? v : S(0);
         sm.G[e] = S(0);
     }
+    // A thread's SR rows of column c are consecutive in memory.  VEC: a segment that lies inside
+    // the panel is read four rows per 16-byte load (dword-aligned vector loads) and the one that
+    // ends past row m row by row, clamped; the second path costs registers, so only kernels with
+    // room for it (the look-ahead panel step) take it.
     S a[SR];
+    const S* colc = B + (size_t)cc * ldb;
+    if (VEC && r0 + SR <= m) {
 #pragma unroll
-    for (int i = 0; i < SR; ++i) a[i] = B[min(r0 + i, m - 1) + (size_t)cc * ldb];
+        for (int i = 0; i < SR; i += 4) __builtin_memcpy(&a[i], colc + r0 + i, 4 * sizeof(S));
+    } else {
+#pragma unroll
+        for (int i = 0; i < SR; ++i) a[i] = colc[min(r0 + i, m - 1)];
+    }
 #pragma unroll
     for (int i = 0; i < SR; ++i) a[i] = (r0 + i < m && c < nbp) ? a[i] : S(0);
     __syncthreads();
@@ -502,9 +512,17 @@ __device__ __forceinline__ void wg_panel_qr_col(int m, int nbp, S* Rt, int ldr, 
     // V below the diagonal (scaled now), R on and above it
     if (c < nbp) {
         const S sc = sm.scl[c];
+        S* colw = B + (size_t)c * ldb;
 #pragma unroll
-        for (int i = 0; i < SR; ++i)
-            if (r0 + i < m) B[r0 + i + (size_t)c * ldb] = a[i] * sc;
+        for (int i = 0; i < SR; ++i) a[i] *= sc;
+        if (VEC && r0 + SR <= m) {
+#pragma unroll
+            for (int i = 0; i < SR; i += 4) __builtin_memcpy(colw + r0 + i, &a[i], 4 * sizeof(S));
+        } else {
+#pragma unroll
+            for (int i = 0; i < SR; ++i)
+                if (r0 + i < m) colw[r0 + i] = a[i];
+        }
     }
     for (int e = tid; e < 32 * 32; e += WG) {
         const int r = e & 31, q = e >> 5;
@@ -517,17 +535,17 @@ __device__ __forceinline__ void wg_panel_qr_col(int m, int nbp, S* Rt, int ldr, 
     __syncthreads();
 }
 
-template <typename S, bool PENT>
+template <typename S, bool PENT, bool VEC = false>
 __device__ __forceinline__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* Tout, int ldt,
                                             PanelSmem<S>& sm) {
     if constexpr (sizeof(S) == 4) {
         const int rows = PENT ? m : m - nbp;  // rows below the top block
         if (m <= 0) wg_panel_qr_mem<S, PENT>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.mem);  // reads no row of B
-        else if (rows <= 64) wg_panel_qr_col<S, PENT, 8>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
-        else if (rows <= 128) wg_panel_qr_col<S, PENT, 16>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
-        else if (rows <= 256) wg_panel_qr_col<S, PENT, 32>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
-        else if (rows <= 512) wg_panel_qr_col<S, PENT, 64>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
-        else if (rows <= 1024) wg_panel_qr_col<S, PENT, 128>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
+        else if (rows <= 64) wg_panel_qr_col<S, PENT, 8, VEC>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
+        else if (rows <= 128) wg_panel_qr_col<S, PENT, 16, VEC>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
+        else if (rows <= 256) wg_panel_qr_col<S, PENT, 32, VEC>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
+        else if (rows <= 512) wg_panel_qr_col<S, PENT, 64, VEC>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
+        else if (rows <= 1024) wg_panel_qr_col<S, PENT, 128, VEC>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
         else wg_panel_qr_mem<S, PENT>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.mem);
     } else {
         // FP64 keeps the row-layout register panel (one row per thread, m <= 256)
@@ -1173,7 +1191,7 @@ __global__ __launch_bounds__(WG) void k_panel_step(const PanelStepTask<S>* __res
         wg_apply_lt<S, false>(t.a.m, t.a.kb, t.a.V, t.a.ldv, t.a.T, NB, nullptr, 0, t.a.C, t.a.ldc, t.a.ncols, sm_.ap);
     if (t.qr) {
         __syncthreads();
-        wg_panel_qr<S, false>(t.q.m, t.q.nbp, t.q.A, t.q.lda, t.q.A, t.q.lda, t.q.T, NB, sm_.pn);
+        wg_panel_qr<S, false, true>(t.q.m, t.q.nbp, t.q.A, t.q.lda, t.q.A, t.q.lda, t.q.T, NB, sm_.pn);
     }
 }
 

@@ -29,6 +29,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <queue>
 #include <set>
 #include <stdexcept>
 #include <string>
@@ -528,14 +529,37 @@ int Tree::colour_interiors(const std::vector<int>& ints) {
         std::sort(a.begin(), a.end());
         a.erase(std::unique(a.begin(), a.end()), a.end());
     }
-    std::vector<int> col(V, -1), sat(V, 0);
+    // DSatur: the next interior is the uncoloured one of most distinct neighbour colours, then
+    // most conflicts, then lowest index.  The last two keys are fixed, so they become one rank
+    // (ord), and the uncoloured interiors sit in one min-heap of ranks per saturation; an entry is
+    // stale once its interior is coloured or has moved up, and is dropped when it surfaces.
+    std::vector<int> col(V, -1), sat(V, 0), ord(V), rk(V);
+    for (int i = 0; i < V; ++i) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](int x, int y) {
+        return adj[x].size() != adj[y].size() ? adj[x].size() > adj[y].size() : x < y;
+    });
+    for (int r = 0; r < V; ++r) rk[ord[r]] = r;
+    using MinHeap = std::priority_queue<int, std::vector<int>, std::greater<int>>;
+    std::vector<MinHeap> heap(1);
+    {
+        std::vector<int> all(ord.size());
+        for (int r = 0; r < V; ++r) all[r] = r;
+        heap[0] = MinHeap(std::greater<int>(), std::move(all));
+    }
     std::vector<std::vector<char>> seen(V);
-    std::set<std::tuple<int, int, int>> next;  // (-saturation, -conflicts, index)
-    for (int i = 0; i < V; ++i) next.insert(std::make_tuple(0, -(int)adj[i].size(), i));
-    int ncol = 0;
-    while (!next.empty()) {
-        const int i = std::get<2>(*next.begin());
-        next.erase(next.begin());
+    int ncol = 0, top = 0;
+    for (int done = 0; done < V; ++done) {
+        int i = -1;
+        for (; top >= 0; --top) {
+            MinHeap& h = heap[top];
+            while (!h.empty() && (col[ord[h.top()]] >= 0 || sat[ord[h.top()]] != top)) h.pop();
+            if (!h.empty()) {
+                i = ord[h.top()];
+                h.pop();
+                break;
+            }
+        }
+        if (i < 0) throw std::runtime_error("interior colouring lost an interior");
         int c = 0;
         while (c < (int)seen[i].size() && seen[i][c]) ++c;
         col[i] = c;
@@ -544,10 +568,11 @@ int Tree::colour_interiors(const std::vector<int>& ints) {
             if (col[j] >= 0) continue;
             if ((int)seen[j].size() <= c) seen[j].resize(c + 1, 0);
             if (seen[j][c]) continue;
-            next.erase(std::make_tuple(-sat[j], -(int)adj[j].size(), j));
             seen[j][c] = 1;
-            ++sat[j];
-            next.insert(std::make_tuple(-sat[j], -(int)adj[j].size(), j));
+            const int sj = ++sat[j];
+            if ((int)heap.size() <= sj) heap.resize(sj + 1);
+            heap[sj].push(rk[j]);
+            top = std::max(top, sj);
         }
     }
     for (int i = 0; i < V; ++i) cl[ints[i]].color = col[i];
@@ -2169,33 +2194,67 @@ int Tree::factorize_t() {
             for (int c : itf)
                 if (want_sparsify(cl[c])) who.push_back(c);
             if (!who.empty()) d_ranks = (int*)arena.alloc(sizeof(int) * who.size());
-            for (size_t q = 0; q < who.size(); ++q) {
-                Cluster& C = cl[who[q]];
-                const int k = C.cols;
-                int w = 0;
-                for (int e : C.sout) w += ed[e].rows;
-                for (int e : C.sin) w += ed[e].cols;
-                const int kmax = std::max(1, std::min(k, w));
-                S* Cb = (S*)arena.alloc((size_t)k * std::max(w, 1) * sizeof(S));
-                gat_begin.push_back((int)gat.size());
-                int is = 0;
-                for (int e : C.sout) {
-                    gat.push_back({Ap(e), Cb + (size_t)is * k, k, ed[e].rows, ed[e].rows, k, 1, 0});
-                    is += ed[e].rows;
+            // Per interface: C gathered from its blocks (k x w), Q, tau, T, two norm vectors and
+            // the pivots, in one arena block at per-interface offsets (sizes first, then the
+            // tasks filled in parallel; each piece 256-byte aligned as an arena allocation is).
+            const int nw = (int)who.size();
+            auto al = [](size_t b) { return (std::max<size_t>(b, 1) + 255) & ~(size_t)255; };
+            std::vector<size_t> boff(nw + 1, 0);
+            std::vector<int> goff(nw + 1, 0), wq(nw);
+            parallel_ranges(nw, 256, [&](int b, int e, int) {
+                for (int q = b; q < e; ++q) {
+                    const Cluster& C = cl[who[q]];
+                    const int k = C.cols;
+                    int w = 0;
+                    for (int e2 : C.sout) w += ed[e2].rows;
+                    for (int e2 : C.sin) w += ed[e2].cols;
+                    const size_t kmax = (size_t)std::max(1, std::min(k, w)), w1 = (size_t)std::max(w, 1);
+                    wq[q] = w;
+                    boff[q + 1] = al((size_t)k * w1 * sizeof(S)) + al((size_t)k * kmax * sizeof(S)) + al(kmax * sizeof(S)) +
+                                  al((size_t)NB * kmax * sizeof(S)) + al(2 * w1 * sizeof(S)) + al(w1 * sizeof(int));
+                    goff[q + 1] = (int)(C.sout.size() + C.sin.size());
                 }
-                for (int e : C.sin) {
-                    gat.push_back({Ap(e), Cb + (size_t)is * k, k, ed[e].cols, ed[e].rows, k, 0, 0});
-                    is += ed[e].cols;
+            });
+            for (int q = 0; q < nw; ++q) {
+                boff[q + 1] += boff[q];
+                goff[q + 1] += goff[q];
+            }
+            char* blk = nw ? (char*)arena.alloc(boff[nw]) : nullptr;
+            gat.resize((size_t)goff[nw]);
+            gat_begin.assign(goff.begin(), goff.end() - 1);
+            gq.resize(nw);
+            gq_cluster = who;
+            std::vector<std::array<double, 2>> pacc(host_threads(), std::array<double, 2>{0.0, 0.0});
+            parallel_ranges(nw, 256, [&](int b, int e, int slot) {
+                for (int q = b; q < e; ++q) {
+                    Cluster& C = cl[who[q]];
+                    const int k = C.cols, w = wq[q];
+                    const size_t kmax = (size_t)std::max(1, std::min(k, w)), w1 = (size_t)std::max(w, 1);
+                    char* p = blk + boff[q];
+                    auto take = [&](size_t bytes) { char* r = p; p += al(bytes); return r; };
+                    S* Cb = (S*)take((size_t)k * w1 * sizeof(S));
+                    C.Qsp = take((size_t)k * kmax * sizeof(S));
+                    C.tausp = take(kmax * sizeof(S));
+                    C.Tsp = take((size_t)NB * kmax * sizeof(S));
+                    S* vn = (S*)take(2 * w1 * sizeof(S));
+                    int* jp = (int*)take(w1 * sizeof(int));
+                    int is = 0, gi = goff[q];
+                    for (int e2 : C.sout) {
+                        gat[gi++] = {Ap(e2), Cb + (size_t)is * k, k, ed[e2].rows, ed[e2].rows, k, 1, 0};
+                        is += ed[e2].rows;
+                    }
+                    for (int e2 : C.sin) {
+                        gat[gi++] = {Ap(e2), Cb + (size_t)is * k, k, ed[e2].cols, ed[e2].rows, k, 0, 0};
+                        is += ed[e2].cols;
+                    }
+                    gq[q] = {Cb, (S*)C.Qsp, (S*)C.tausp, (S*)C.Tsp, vn, vn + w1, jp, d_ranks + q, k, w, 0, 0};
+                    pacc[slot][0] += SZ * 2.0 * k * w;
+                    pacc[slot][1] += SZ * 3.0 * k * w;
                 }
-                C.Qsp = arena.alloc((size_t)k * kmax * sizeof(S));
-                C.tausp = arena.alloc((size_t)kmax * sizeof(S));
-                C.Tsp = arena.alloc((size_t)NB * kmax * sizeof(S));
-                S* vn = (S*)arena.alloc((size_t)2 * std::max(w, 1) * sizeof(S));
-                int* jp = (int*)arena.alloc((size_t)std::max(w, 1) * sizeof(int));
-                gq.push_back({Cb, (S*)C.Qsp, (S*)C.tausp, (S*)C.Tsp, vn, vn + std::max(w, 1), jp, d_ranks + q, k, w, 0, 0});
-                prof_add(P_GATHER, 0.0, SZ * 2.0 * k * w);
-                if (rank_mode == 0) prof_add(P_GEQP3, 0.0, SZ * 3.0 * k * w);
-                gq_cluster.push_back(who[q]);
+            });
+            for (const auto& a : pacc) {
+                prof_add(P_GATHER, 0.0, a[0]);
+                if (rank_mode == 0) prof_add(P_GEQP3, 0.0, a[1]);
             }
             // sparsify_extra (src/ptree.cpp:441-493): the same RRQR on each interface's extra
             // rows, C_ex = [A(c, m)(k:rows, :) ...] over its row blocks, always by geqp3
