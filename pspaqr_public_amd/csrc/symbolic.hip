@@ -151,40 +151,36 @@ __global__ void k_key_rowcount(long long nk, int n, const long long* __restrict_
 }
 
 // ---- geometric bisection (bissect_geo, src/partition.cpp:241-279)
-// Per part: the vertex count and each dimension's range.  A wave whose live lanes share one
-// part (most waves: consecutive vertices lie in the same part) reduces in registers and one
-// lane updates the part; otherwise every lane updates its own.
+// Per part: the vertex count and each dimension's range.  One pass per distinct part among a
+// wave's vertices (consecutive vertices meet few parts): that part's lanes reduce count, min
+// and max over the wave and one lane issues the atomics, so a part's counters see one update
+// per wave, not one per vertex.
 __global__ void k_minmax(int n, int d, const double* __restrict__ X, const int* __restrict__ parts,
                          unsigned long long* __restrict__ mm, int* __restrict__ cnt) {
     const int i = blockIdx.x * TB + threadIdx.x;
     const bool live = i < n;
     const int k = live ? parts[i] : -1;
-    const int k0 = __shfl(k, 0, 64);
-    const bool uniform = __all((!live) || k == k0) && k0 >= 0;
     const int lane = threadIdx.x & 63;
-    if (uniform) {
-        const int c = __popcll(__ballot(live));
-        if (lane == 0) atomicAdd(&cnt[k0], c);
+    unsigned long long left = __ballot(live && k >= 0);
+    while (left) {
+        const int leader = __ffsll((long long)left) - 1;
+        const int kk = __shfl(k, leader, 64);
+        const bool mine = live && k == kk;
+        const unsigned long long grp = __ballot(mine);
+        left &= ~grp;
+        if (lane == leader) atomicAdd(&cnt[kk], __popcll(grp));
         for (int dim = 0; dim < d; ++dim) {
-            unsigned long long lo = live ? enc_double(X[dim + (size_t)i * d]) : ~0ull;
-            unsigned long long hi = live ? lo : 0ull;
+            const unsigned long long e = mine ? enc_double(X[dim + (size_t)i * d]) : 0ull;
+            unsigned long long lo = mine ? e : ~0ull, hi = mine ? e : 0ull;
             for (int o = 32; o > 0; o >>= 1) {
                 lo = min(lo, __shfl_xor(lo, o, 64));
                 hi = max(hi, __shfl_xor(hi, o, 64));
             }
-            if (lane == 0) {
-                atomicMin(&mm[2 * ((size_t)k0 * d + dim)], lo);
-                atomicMax(&mm[2 * ((size_t)k0 * d + dim) + 1], hi);
+            if (lane == leader) {
+                atomicMin(&mm[2 * ((size_t)kk * d + dim)], lo);
+                atomicMax(&mm[2 * ((size_t)kk * d + dim) + 1], hi);
             }
         }
-        return;
-    }
-    if (!live) return;
-    atomicAdd(&cnt[k], 1);
-    for (int dim = 0; dim < d; ++dim) {
-        const unsigned long long e = enc_double(X[dim + (size_t)i * d]);
-        atomicMin(&mm[2 * ((size_t)k * d + dim)], e);
-        atomicMax(&mm[2 * ((size_t)k * d + dim) + 1], e);
     }
 }
 // the widest dimension of every part: the first whose range is strictly the largest
