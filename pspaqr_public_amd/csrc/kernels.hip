@@ -535,6 +535,139 @@ __device__ __forceinline__ void wg_panel_qr_col(int m, int nbp, S* Rt, int ldr, 
     __syncthreads();
 }
 
+// Row-layout panel for panels taller than the register panels hold: the steps of
+// wg_panel_qr_reg (thread t owns rows t, t + WG, ...; one 32-column reduction and one barrier per
+// step) with the rows read from memory in every step instead of held in registers.  Step j
+// reads the thread's rows (RB at a time, every column, so that a wave's load of one column is
+// 64 consecutive rows: whole lines) for the dot products with column j, and again to apply the
+// reflector, writing back columns j.. only.  Same reflectors (sign rule included), R, V and T as
+// wg_panel_qr_mem.
+template <typename S, bool PENT>
+__device__ __forceinline__ void wg_panel_qr_rows(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* Tout, int ldt,
+                                                 MemPanelSmem<S>& sm) {
+    constexpr int RB = sizeof(S) == 4 ? 4 : 2;  // rows per block, all of their columns in flight
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nrow = (m + WG - 1) / WG;  // rows per thread
+    const S* Bg = B;  // read through the same (non-restrict) pointer the stores use
+    for (int e = tid; e < 32 * 33; e += WG) sm.Ts[e] = S(0);
+    __syncthreads();
+    for (int j = 0; j < nbp; ++j) {
+        const int par = j & 1;
+        // PENT: row j of the separate triangle, read before the barrier (written after it)
+        S rj[32];
+        if (PENT) {
+#pragma unroll
+            for (int l = 0; l < 32; ++l) rj[l] = Rt[j + (size_t)min(l, nbp - 1) * ldr];
+        }
+        S d[32];
+#pragma unroll
+        for (int l = 0; l < 32; ++l) d[l] = S(0);
+        for (int i0 = 0; i0 < nrow; i0 += RB) {
+            S a[RB][32], x[RB];
+#pragma unroll
+            for (int u = 0; u < RB; ++u) {
+                const int R = tid + (i0 + u) * WG, Rc = min(R, m - 1);
+#pragma unroll
+                for (int l = 0; l < 32; ++l) a[u][l] = Bg[Rc + (size_t)min(l, nbp - 1) * ldb];
+                x[u] = Bg[Rc + (size_t)j * ldb];
+            }
+#pragma unroll
+            for (int u = 0; u < RB; ++u) {
+                const int R = tid + (i0 + u) * WG;
+                const bool in = R < m && (PENT || R > j);
+                const S xu = in ? x[u] : S(0);
+#pragma unroll
+                for (int l = 0; l < 32; ++l) d[l] += xu * ((l < nbp) ? a[u][l] : S(0));
+                // square panel: row j is this thread's first row when tid == j
+                if (!PENT && R == j) {
+#pragma unroll
+                    for (int l = 0; l < 32; ++l) sm.rowj[par][l] = (l < nbp) ? a[u][l] : S(0);
+                }
+            }
+        }
+        const S tot = wave_sum32<S>(d);
+        if (vec_owner(lane)) sm.red[par][wave][vec_col(lane)] = tot;
+        __syncthreads();
+        S dl[32];
+#pragma unroll
+        for (int l = 0; l < 32; ++l)
+            dl[l] = (sm.red[par][0][l] + sm.red[par][1][l]) + (sm.red[par][2][l] + sm.red[par][3][l]);
+        if (!PENT) {
+#pragma unroll
+            for (int l = 0; l < 32; ++l) rj[l] = sm.rowj[par][l];
+        }
+        S alpha = S(0), ss = S(0);
+#pragma unroll
+        for (int l = 0; l < 32; ++l) {
+            alpha = (l == j) ? rj[l] : alpha;
+            ss = (l == j) ? dl[l] : ss;
+        }
+        S tau, scal, beta;
+        if (ss == S(0)) {
+            tau = S(0); scal = S(1); beta = alpha;
+        } else {
+            householder_fast(alpha, ss, tau, scal, beta);
+        }
+        S wl[32];
+#pragma unroll
+        for (int l = 0; l < 32; ++l) wl[l] = (l > j && l < nbp) ? (rj[l] + scal * dl[l]) * tau : S(0);
+        // apply: rows below j (all rows of B when PENT) get v = scal * x in column j and
+        // a - v w in the trailing columns; row j of the square panel gets beta and a - w
+        for (int i0 = 0; i0 < nrow; i0 += RB) {
+            S a[RB][32];
+#pragma unroll
+            for (int u = 0; u < RB; ++u) {
+                const int R = tid + (i0 + u) * WG, Rc = min(R, m - 1);
+#pragma unroll
+                for (int l = 0; l < 32; ++l) a[u][l] = Bg[Rc + (size_t)min(l, nbp - 1) * ldb];
+            }
+#pragma unroll
+            for (int u = 0; u < RB; ++u) {
+                const int R = tid + (i0 + u) * WG;
+                if (R >= m || (!PENT && R < j)) continue;
+                S xj = S(0);
+#pragma unroll
+                for (int l = 0; l < 32; ++l) xj = (l == j) ? a[u][l] : xj;
+                const bool vrow = PENT || R > j;
+                const S vs = xj * scal;
+#pragma unroll
+                for (int l = 0; l < 32; ++l) {
+                    if (l < j || l >= nbp) continue;
+                    const S nv = vrow ? ((l == j) ? vs : a[u][l] - vs * wl[l]) : ((l == j) ? beta : a[u][l] - wl[l]);
+                    B[R + (size_t)l * ldb] = nv;
+                }
+            }
+        }
+        if (PENT) {
+            S mine = S(0);
+#pragma unroll
+            for (int l = 0; l < 32; ++l) mine = (l == tid) ? rj[l] - wl[l] : mine;
+            if (tid == j) Rt[j + (size_t)j * ldr] = beta;
+            else if (tid > j && tid < nbp) Rt[j + (size_t)tid * ldr] = mine;
+        }
+        // column j of T: T(q, j) = -tau sum_{p=q}^{j-1} T(q, p) t_p, t_p = v_p^T v_j
+        if (tid < j) {
+            S sacc = S(0);
+#pragma unroll
+            for (int q = 0; q < 32; ++q) {
+                if (q >= tid && q < j) {
+                    const S tq = scal * dl[q] + (PENT ? S(0) : rj[q]);
+                    sacc += sm.Ts[q * 33 + tid] * tq;
+                }
+            }
+            sm.Ts[j * 33 + tid] = -tau * sacc;
+        } else if (tid == j) {
+            sm.Ts[j * 33 + j] = tau;
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < nbp * nbp; e += WG) {
+        int i = e % nbp, jj = e / nbp;
+        Tout[i + (size_t)jj * ldt] = (i <= jj) ? sm.Ts[jj * 33 + i] : S(0);
+    }
+    __syncthreads();
+}
+
 template <typename S, bool PENT, bool VEC = false>
 __device__ __forceinline__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B, int ldb, S* Tout, int ldt,
                                             PanelSmem<S>& sm) {
@@ -546,7 +679,7 @@ __device__ __forceinline__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B
         else if (rows <= 256) wg_panel_qr_col<S, PENT, 32, VEC>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
         else if (rows <= 512) wg_panel_qr_col<S, PENT, 64, VEC>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
         else if (rows <= 1024) wg_panel_qr_col<S, PENT, 128, VEC>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
-        else wg_panel_qr_mem<S, PENT>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.mem);
+        else wg_panel_qr_rows<S, PENT>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.mem);
     } else {
         // FP64 keeps the row-layout register panel (one row per thread, m <= 256)
         if (m <= WG) wg_panel_qr_reg<S, PENT, 1>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.mem);

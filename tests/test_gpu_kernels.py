@@ -36,7 +36,8 @@ def oracle_geqrt(A):
 @pytest.mark.parametrize("prec", [64, 32])
 def test_geqrt_grouped(prec):
     rng = np.random.default_rng(1)
-    shapes = [(5, 3), (40, 40), (100, 70), (33, 33), (64, 64), (129, 97), (1, 1), (300, 31)]
+    # (1500, 64) and (2600, 40): FP32 panels taller than the register panel (streamed panel)
+    shapes = [(5, 3), (40, 40), (100, 70), (33, 33), (64, 64), (129, 97), (1, 1), (300, 31), (1500, 64), (2600, 40)]
     dt = np.float64 if prec == 64 else np.float32
     bufs, descs, ref = [], [], []
     for (m, n) in shapes:
@@ -104,7 +105,9 @@ def test_tpqrt_tpmqrt_chain(prec):
     dt = np.float64 if prec == 64 else np.float32
     tol = 1e-12 if prec == 64 else 5e-4
     # FP32 chains with links of <= 128 / 256 / 512 rows run on the C-resident kernel
-    fronts = [(45, [30, 7, 64]), (8, [5, 9]), (70, [100]), (40, [300, 20]), (66, [200, 150]), (50, [700, 30, 1000])]
+    # links of 1100 and 2500 rows: FP32 pentagonal panels taller than the register panel
+    fronts = [(45, [30, 7, 64]), (8, [5, 9]), (70, [100]), (40, [300, 20]), (66, [200, 150]), (50, [700, 30, 1000]),
+              (40, [1100, 2500])]
     keep = []
     tq_desc, tq_links, tm_desc, tm_links = [], [], [], []
     expect_R, expect_B, expect_top, expect_C = [], [], [], []
