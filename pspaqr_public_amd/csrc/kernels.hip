@@ -2,8 +2,9 @@
 //
 // Workgroup-level building blocks:
 //   wg_panel_qr   Householder QR of a <= 32-column panel (geqrt2 / tpqrt2 with L = 0),
-//                 producing the panel's T; 8 lanes of one wavefront own one column, so the
-//                 column dot products reduce with three intra-wave shuffles.
+//                 producing the panel's T: the panel in registers (FP32: a column per lane,
+//                 up to 1024 rows; FP64: a row per thread, up to 256 rows), taller panels
+//                 streamed from memory a row per thread, one barrier per column step.
 //   wg_apply_lt   H^T [Atop; C] with a block reflector of <= 32 columns (larfb 'L','T' /
 //                 tprfb 'L','T'): W = Atop + V^T C staged through LDS in 64-row chunks,
 //                 W <- T^T W, C -= V W with V rows held in registers.
@@ -681,9 +682,10 @@ __device__ __forceinline__ void wg_panel_qr(int m, int nbp, S* Rt, int ldr, S* B
         else if (rows <= 1024) wg_panel_qr_col<S, PENT, 128, VEC>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.col);
         else wg_panel_qr_rows<S, PENT>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.mem);
     } else {
-        // FP64 keeps the row-layout register panel (one row per thread, m <= 256)
+        // FP64 keeps the row-layout register panel (one row per thread, m <= 256); taller panels
+        // stream the same row layout from memory
         if (m <= WG) wg_panel_qr_reg<S, PENT, 1>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.mem);
-        else wg_panel_qr_mem<S, PENT>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.mem);
+        else wg_panel_qr_rows<S, PENT>(m, nbp, Rt, ldr, B, ldb, Tout, ldt, sm.mem);
     }
 }
 
