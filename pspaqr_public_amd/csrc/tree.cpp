@@ -1758,8 +1758,13 @@ int Tree::factorize_t() {
             if (C.out.empty() || ed[C.out[0]].n2 != c) throw std::runtime_error("first out-edge of an interior must be itself");
             st.max_front = std::max(st.max_front, C.cols);
             C.T = arena.alloc((size_t)NB * C.cols * sizeof(S));
-            for (size_t j = 1; j < C.out.size(); ++j) ed[C.out[j]].T = arena.alloc((size_t)NB * C.cols * sizeof(S));
-            lnk0[i + 1] = lnk0[i] + (int)C.out.size() - 1;
+            int nl = 0;
+            for (size_t j = 1; j < C.out.size(); ++j) {
+                const int ns = link_slices<S>(ed[C.out[j]].rows);
+                ed[C.out[j]].T = arena.alloc((size_t)NB * C.cols * sizeof(S) * ns);
+                nl += ns;
+            }
+            lnk0[i + 1] = lnk0[i] + nl;
         }
         struct ElimPart {
             std::vector<GeqrtTask<S>> geq;
@@ -1791,10 +1796,14 @@ int Tree::factorize_t() {
                     P.pf[1] += f;
                     P.pb[1] += SZ * ((double)ei.rows * C.cols + NB * C.cols + 2.0 * ei.rows * ei.cols);
                 }
-                TpqrtTask<S> tk{Ap(C.self_edge), C.cols, es.rows, lnk0[i], (int)C.out.size() - 1};
+                TpqrtTask<S> tk{Ap(C.self_edge), C.cols, es.rows, lnk0[i], lnk0[i + 1] - lnk0[i]};
+                int li = lnk0[i];
                 for (size_t j = 1; j < C.out.size(); ++j) {
                     const Edge& eo = ed[C.out[j]];
-                    tpql[lnk0[i] + j - 1] = {Ap(C.out[j]), (S*)eo.T, eo.rows, eo.rows};
+                    const int ns = link_slices<S>(eo.rows), sz = (eo.rows + ns - 1) / ns;
+                    for (int sl = 0; sl < ns; ++sl)  // a tall block: one link per row slice, T blocks side by side
+                        tpql[li++] = {Ap(C.out[j]) + (size_t)sl * sz, (S*)eo.T + (size_t)sl * NB * C.cols,
+                                      std::min(sz, eo.rows - sl * sz), eo.rows};
                     const double f = 2.0 * eo.rows * C.cols * C.cols;
                     P.flops += f;
                     P.pf[2] += f;
@@ -1904,7 +1913,9 @@ int Tree::factorize_t() {
             long n = 0;
             for (int i = b; i < e; ++i) {
                 const Cluster& C = cl[ints[i]];
-                if (!C.out.empty()) n += (long)C.in.size() * (long)(C.out.size() - 1);
+                long nl = 0;
+                for (size_t j = 1; j < C.out.size(); ++j) nl += link_slices<S>(ed[C.out[j]].rows);
+                n += (long)C.in.size() * nl;
             }
             lcount[slot + 1] = n;
         });
@@ -1928,9 +1939,14 @@ int Tree::factorize_t() {
                         const Edge& cn = ed[C.out[j]];
                         const int mn = mslot[cn.n2];
                         if (mn < 0) throw std::runtime_error("missing fill-in block");
-                        tpml[li++] = {Ap(C.out[j]), (const S*)cn.T, Ap(mn), cn.rows, cn.rows, ed[mn].rows, 0};
-                        rmax = std::max(rmax, cn.rows);
-                        rmin = std::min(rmin, cn.rows);
+                        const int ns = link_slices<S>(cn.rows), sz = (cn.rows + ns - 1) / ns;
+                        for (int sl = 0; sl < ns; ++sl) {
+                            const int r0 = sl * sz, rs = std::min(sz, cn.rows - r0);
+                            tpml[li++] = {Ap(C.out[j]) + r0, (const S*)cn.T + (size_t)sl * NB * C.cols, Ap(mn) + r0, rs,
+                                          cn.rows, ed[mn].rows, 0};
+                            rmax = std::max(rmax, rs);
+                            rmin = std::min(rmin, rs);
+                        }
                         const double f = 4.0 * cn.rows * C.cols * cm.cols;
                         acc[0] += f;
                         acc[1] += f;
@@ -3032,9 +3048,14 @@ void Tree::solve_sched_levels(int upto) {
             int rmin = INT_MAX, rmax = 0;
             for (size_t j = 1; j < C.out.size(); ++j) {
                 const Edge& eo = ed[C.out[j]];
-                links.push_back({(const S*)eo.A, (const S*)eo.T, X0 + cl[eo.n2].xoff, eo.rows, eo.rows});
-                rmin = std::min(rmin, eo.rows);
-                rmax = std::max(rmax, eo.rows);
+                const int ns = link_slices<S>(eo.rows), sz = (eo.rows + ns - 1) / ns;
+                for (int sl = 0; sl < ns; ++sl) {
+                    const int r0 = sl * sz, rs = std::min(sz, eo.rows - r0);
+                    links.push_back({(const S*)eo.A + r0, (const S*)eo.T + (size_t)sl * NB * C.cols, X0 + cl[eo.n2].xoff + r0,
+                                     rs, eo.rows});
+                    rmin = std::min(rmin, rs);
+                    rmax = std::max(rmax, rs);
+                }
             }
             t.nlinks = (int)links.size() - t.link0;
             t.rmax = rmax;

@@ -114,6 +114,16 @@ inline int trsm_res_bucket(const LarfbTask<S>& t) {
     return t.k2 <= 128 ? 0 : (t.k2 <= 256 ? 1 : 2);
 }
 
+// FP32 coupling blocks taller than kLinkRows are factored as several links, one per row slice
+// of at most kLinkRows rows (balanced), each with its own T: the QR of [R; B_0; B_1; ...] by the
+// same chain of tpqrt steps, with every panel inside the register-resident panel QR.  The same
+// slices serve the trailing update and the solve.  FP64 keeps whole blocks, as the reference.
+constexpr int kLinkRows = 1024;
+template <typename S>
+inline int link_slices(int rows) {
+    return (sizeof(S) == 4 && rows > kLinkRows) ? (rows + kLinkRows - 1) / kLinkRows : 1;
+}
+
 // Flops of an m x n Householder QR.
 template <typename S>
 inline double flops_geqrt(double m, double n) { return 2 * m * n * n - 2 * n * n * n / 3; }
