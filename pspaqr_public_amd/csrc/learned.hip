@@ -381,6 +381,109 @@ __global__ __launch_bounds__(WG) void k_tail_select(const TailTask<S>* __restric
     if (tid == 0) atomicMax(t.rank, max(max(red[0], red[1]), max(red[2], red[3])));
 }
 
+// The same selection from the leading rows of B (ld r = rY) and the squared column norms of C.
+template <typename S>
+__global__ __launch_bounds__(WG) void k_tail_lead(const TailLeadTask<S>* __restrict__ tasks, double tol) {
+    __shared__ int red[4];
+    const TailLeadTask<S> t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, grp = tid >> 4, gl = tid & 15;
+    const int rY = min(*t.ry, t.r);
+    const double thr2 = tol * tol * (double)*t.cmax2;
+    int need = 0;
+    for (int c0 = 0; c0 < t.w; c0 += WG / 16) {
+        const int c = c0 + grp;
+        const S* col = t.B + (size_t)min(c, t.w - 1) * t.r;
+        const double tot = (double)t.cn[min(c, t.w - 1)];
+        int rc = 0;
+        if (c < t.w && tot > thr2) {
+            rc = INT_MAX;
+            double run = 0.0;
+            S vnext = col[min(gl, t.r - 1)];  // the next chunk's entry is loaded one chunk ahead
+            for (int b = 0; b <= rY; b += 16) {
+                const int i = b + gl;
+                const S vcur = vnext;
+                vnext = col[min(i + 16, t.r - 1)];
+                const double v = (i < rY) ? (double)vcur : 0.0;
+                const double sq = v * v;
+                const double incl = row16_scan(sq);
+                const double tail = tot - (run + incl - sq);  // rows i .. k-1
+                const unsigned long long bal = __ballot(i <= rY && tail <= thr2);
+                const unsigned grpbits = (unsigned)((bal >> (lane & 48)) & 0xFFFFull);
+                if (grpbits) {
+                    rc = b + __ffs(grpbits) - 1;
+                    break;
+                }
+                run += __shfl(incl, (lane & 48) | 15, 64);
+            }
+        }
+        need = max(need, rc);
+    }
+    const int wn = max(max(lane_read(need, 0), lane_read(need, 16)), max(lane_read(need, 32), lane_read(need, 48)));
+    if (lane == 0) red[wv] = wn;
+    __syncthreads();
+    if (tid == 0) atomicMax(t.rank, max(max(red[0], red[1]), max(red[2], red[3])));
+}
+
+// B = M(0:r, 0:k) C: the FP32 sketch kernel's tiling (64 x 64 output tile per workgroup, the
+// inner dimension in chunks of 32 through LDS, v_mfma_f32_32x32x2f32, one 32 x 32 block per
+// wave) with M in place of C and C in place of Omega.  Loads are clamped in range and
+// zero-weighted.
+__global__ __launch_bounds__(WG) void k_tail_gemm(const TailGemmTask* __restrict__ tasks,
+                                                  const SketchTile* __restrict__ tiles) {
+    __shared__ float Ms[32][65];
+    __shared__ float Cs[64][33];
+    const SketchTile tl = tiles[blockIdx.x];
+    const TailGemmTask t = tasks[tl.task];
+    const int tid = threadIdx.x, wv = tid >> 6;
+    float acc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
+    float mv[8], cv[8];
+    auto load_step = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + q * WG, rr = e & 63, cc = e >> 6;
+            mv[q] = t.M[min(tl.r0 + rr, t.r - 1) + (size_t)min(k0 + cc, t.k - 1) * t.ldm];
+            const int i = e & 31, j = e >> 5;
+            cv[q] = t.C[min(k0 + i, t.k - 1) + (size_t)min(tl.c0 + j, t.w - 1) * t.k];
+        }
+    };
+    load_step(0);
+    for (int k0 = 0; k0 < t.k; k0 += 32) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + q * WG, rr = e & 63, cc = e >> 6;
+            Ms[cc][rr] = (k0 + cc < t.k) ? mv[q] : 0.0f;
+            const int i = e & 31, j = e >> 5;
+            Cs[j][i] = (k0 + i < t.k) ? cv[q] : 0.0f;
+        }
+        __syncthreads();
+        if (k0 + 32 < t.k) load_step(k0 + 32);
+        const int li = tid & 31, lk = (tid >> 5) & 1, wm = wv & 1, wn = wv >> 1;
+        floatx16& m = *reinterpret_cast<floatx16*>(acc);
+#pragma unroll
+        for (int kk = 0; kk < 32; kk += 2)
+            m = __builtin_amdgcn_mfma_f32_32x32x2f32(Ms[kk + lk][32 * wm + li], Cs[32 * wn + li][kk + lk], m, 0, 0, 0);
+        __syncthreads();
+    }
+    const int li = tid & 31, wm = wv & 1, wn = wv >> 1, l = tid & 63;
+    const int c = tl.c0 + 32 * wn + li;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int r = tl.r0 + 32 * wm + (q >> 2) * 8 + (l >> 5) * 4 + (q & 3);
+        if (r < t.r && c < t.w) t.B[r + (size_t)c * t.r] = acc[q];
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(WG) void k_eye(const EyeTask<S>* __restrict__ tasks) {
+    const EyeTask<S> t = tasks[blockIdx.x];
+    for (long e = threadIdx.x; e < (long)t.k * t.k; e += WG) {
+        const int i = (int)(e % t.k), j = (int)(e / t.k);
+        t.M[i + (size_t)j * t.ld] = (i == j) ? S(1) : S(0);
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // MLP: prediction (one thread per sample, weights in LDS).
 // ---------------------------------------------------------------------------------------
@@ -557,6 +660,17 @@ template <typename S>
 void launch_tail_select(const TailTask<S>* d, int n, double tol, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_tail_select<S>, dim3(n), dim3(WG), 0, s, d, tol);
 }
+template <typename S>
+void launch_tail_lead(const TailLeadTask<S>* d, int n, double tol, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_tail_lead<S>, dim3(n), dim3(WG), 0, s, d, tol);
+}
+void launch_tail_gemm(const TailGemmTask* d, const SketchTile* tiles, int ntiles, hipStream_t s) {
+    if (ntiles > 0) hipLaunchKernelGGL(k_tail_gemm, dim3(ntiles), dim3(WG), 0, s, d, tiles);
+}
+template <typename S>
+void launch_eye(const EyeTask<S>* d, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_eye<S>, dim3(n), dim3(WG), 0, s, d);
+}
 void launch_mlp_predict(const float* X, int n, const float* W, float* out, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_mlp_predict, dim3((n + WG - 1) / WG), dim3(WG), 0, s, X, n, W, out);
 }
@@ -575,7 +689,9 @@ void launch_mlp_reduce(const float* partial, int nblocks, float* grad, float* lo
     template void launch_colnorm<S>(const ColNormTask<S>*, int, hipStream_t);                                \
     template void launch_sketch<S>(const SketchTask<S>*, const SketchTile*, int, const S*, int, hipStream_t); \
     template void launch_rank_features<S>(const FeatTask<S>*, int, double, hipStream_t);                     \
-    template void launch_tail_select<S>(const TailTask<S>*, int, double, hipStream_t);
+    template void launch_tail_select<S>(const TailTask<S>*, int, double, hipStream_t);                     \
+    template void launch_tail_lead<S>(const TailLeadTask<S>*, int, double, hipStream_t);                   \
+    template void launch_eye<S>(const EyeTask<S>*, int, hipStream_t);
 SPAQR_INST(float)
 SPAQR_INST(double)
 #undef SPAQR_INST

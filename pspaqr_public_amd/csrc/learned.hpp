@@ -49,6 +49,16 @@ template <typename S> struct FeatTask {
 // column chunks (atomic max) and *rank must start at zero.
 template <typename S> struct TailTask { const S* B; const S* cmax2; const int* ry; int* rank; int k, w; };
 
+// The tail test from the leading rows only (FP32 fronts): B (r x w, ld r) holds the first
+// r = *ry rows of Q^T C and cn the squared column norms of the original C, so that
+// ||B(i:, j)||^2 = cn[j] - ||B(:i, j)||^2 (Q orthogonal); otherwise as TailTask.
+template <typename S> struct TailLeadTask { const S* B; const S* cn; const S* cmax2; const int* ry; int* rank; int r, w; };
+// B (r x w, ld r) = M(0:r, 0:k) C (k x w): M with ld ldm, C with ld k; 64 x 64 output tiles
+// (SketchTile: task, r0, c0) on FP32 MFMA.
+struct TailGemmTask { const float* M; const float* C; float* B; int r, k, w, ldm; };
+// M (k x k, ld ld) <- the identity, one workgroup per task.
+template <typename S> struct EyeTask { S* M; int k, ld; };
+
 template <typename S> void launch_gauss(S* om, int rows, int cols, int ld, uint64_t seed, hipStream_t s);
 template <typename S> void launch_colnorm(const ColNormTask<S>* d, int n, hipStream_t s);
 template <typename S>
@@ -62,6 +72,9 @@ void launch_sketch_bf16(const SketchTask<float>* d, const SketchTile* tiles, int
                         int tile_cols, hipStream_t s);
 template <typename S> void launch_rank_features(const FeatTask<S>* d, int n, double tol, hipStream_t s);
 template <typename S> void launch_tail_select(const TailTask<S>* d, int n, double tol, hipStream_t s);
+template <typename S> void launch_tail_lead(const TailLeadTask<S>* d, int n, double tol, hipStream_t s);
+void launch_tail_gemm(const TailGemmTask* d, const SketchTile* tiles, int ntiles, hipStream_t s);
+template <typename S> void launch_eye(const EyeTask<S>* d, int n, hipStream_t s);
 
 // MLP (kRankFeatures -> 64 -> 64 -> 1, tanh, sigmoid output): predicted rank fractions.
 void launch_mlp_predict(const float* X, int n, const float* W, float* out, hipStream_t s);

@@ -110,11 +110,21 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
     S* d_cmax2 = (S*)arena.alloc(sizeof(S) * std::max(nt, 1));
     float* d_X = nullptr;
     const int kColChunk = tuned(T_RANK_COLS, 128);  // columns per workgroup of the column-norm and tail kernels
+    // squared column norms of every eligible C, kept for the features and the tail test (the
+    // pivoted sketch QRs reuse the geqp3 norm workspace)
+    std::vector<S*> cnorm(nt, nullptr);
     if (ne && (feats || sketch_mode)) {
+        size_t tot_w = 0;
+        for (int q : elig) tot_w += (size_t)gq[q].w;
+        S* cbuf = (S*)arena.alloc(sizeof(S) * std::max<size_t>(tot_w, 1));
+        for (int q : elig) {
+            cnorm[q] = cbuf;
+            cbuf += gq[q].w;
+        }
         std::vector<ColNormTask<S>> cn;
         for (int q : elig)
             for (int c0 = 0; c0 < gq[q].w; c0 += kColChunk)
-                cn.push_back({gq[q].C + (size_t)c0 * gq[q].k, gq[q].vn1 + c0, d_cmax2 + q, gq[q].k,
+                cn.push_back({gq[q].C + (size_t)c0 * gq[q].k, cnorm[q] + c0, d_cmax2 + q, gq[q].k,
                               std::min(kColChunk, gq[q].w - c0)});
         auto* d_cn = upload_vec(cn);
         timed(P_RSKETCH, 2, ne, [&] {
@@ -140,7 +150,7 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
             sk.push_back({gq[q].C, Y16, k, w, kSketchFeatureCols, 0});
             qr.push_back({Y16, T16, k, std::min(kSketchFeatureCols, k), k, 0});
             const Cluster& C = cl[gq_cluster[q]];
-            ft.push_back({Y16, gq[q].vn1, d_cmax2 + q, d_X + (size_t)i * kRankFeatures, k, w, (int)C.sout.size(),
+            ft.push_back({Y16, cnorm[q], d_cmax2 + q, d_X + (size_t)i * kRankFeatures, k, w, (int)C.sout.size(),
                           (int)C.sin.size(), L, 0});
             prof_add(P_RSKETCH, 2.0 * k * w * kSketchFeatureCols, SZ * ((double)k * w + 2.0 * k * kSketchFeatureCols));
         }
@@ -245,19 +255,46 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
             sync_stream();
             std::vector<LarfbTask<S>> lb;
             std::vector<TailTask<S>> tt;
+            // FP32 interfaces of more than 1024 rows (past the C-resident applies, whose
+            // workgroup kernel re-reads C for every panel): the tail test needs only B's first r
+            // rows, Q(:, 0:r)^T C, and the column norms of C, so Q^T is formed on the identity (k
+            // columns instead of w) and its leading rows multiply C as one MFMA product.
+            // SPAQR_TAIL_GEMM=0 applies Q^T to all of C for every interface.
+            static const bool tail_gemm_env = getenv("SPAQR_TAIL_GEMM") == nullptr || atoi(getenv("SPAQR_TAIL_GEMM")) != 0;
+            const bool tail_gemm = sizeof(S) == 4 && tail_gemm_env;
+            std::vector<EyeTask<S>> eye;
+            std::vector<TailGemmTask> tg;
+            std::vector<SketchTile> tg_tiles;
+            std::vector<TailLeadTask<S>> tlead;
             int* d_tail = (int*)arena.alloc(sizeof(int) * std::max(nt, 1));
             for (int q : todo) {
                 const int k = gq[q].k, w = gq[q].w, r = ry[q];
                 for (int j = 0; j < r; ++j) flops += 4.0 * (k - j) * (s_of[q] - j);
                 applied[q] = 0;
                 if (r <= 0) continue;
-                lb.push_back({gq[q].Vout, gq[q].T, gq[q].C, nullptr, k, w, r, k, k, 0, 0, 0});
-                for (int c0 = 0; c0 < w; c0 += kColChunk)
-                    tt.push_back({gq[q].C + (size_t)c0 * k, d_cmax2 + q, d_ranks + q, d_tail + q, k,
-                                  std::min(kColChunk, w - c0)});
+                if (tail_gemm && cnorm[q] && k > 1024) {
+                    S* M = (S*)arena.alloc(sizeof(S) * (size_t)k * k);
+                    S* Bl = (S*)arena.alloc(sizeof(S) * (size_t)r * w);
+                    eye.push_back({M, k, k});
+                    lb.push_back({gq[q].Vout, gq[q].T, M, nullptr, k, k, r, k, k, 0, 0, 0});
+                    for (int r0 = 0; r0 < r; r0 += 64)
+                        for (int c0 = 0; c0 < w; c0 += 64) tg_tiles.push_back({(int)tg.size(), r0, c0, 0});
+                    tg.push_back({(const float*)M, (const float*)gq[q].C, (float*)Bl, r, k, w, k});
+                    for (int c0 = 0; c0 < w; c0 += kColChunk)
+                        tlead.push_back({Bl + (size_t)c0 * r, cnorm[q] + c0, d_cmax2 + q, d_ranks + q, d_tail + q, r,
+                                         std::min(kColChunk, w - c0)});
+                    flops += 4.0 * k * k * r - 2.0 * k * (double)r * r + 2.0 * r * (double)k * w;
+                    prof_add(P_RSKETCH, 4.0 * k * k * r - 2.0 * k * (double)r * r + 2.0 * r * (double)k * w,
+                             SZ * ((double)k * r + 2.0 * k * k + (double)k * w + (double)r * w));
+                } else {
+                    lb.push_back({gq[q].Vout, gq[q].T, gq[q].C, nullptr, k, w, r, k, k, 0, 0, 0});
+                    for (int c0 = 0; c0 < w; c0 += kColChunk)
+                        tt.push_back({gq[q].C + (size_t)c0 * k, d_cmax2 + q, d_ranks + q, d_tail + q, k,
+                                      std::min(kColChunk, w - c0)});
+                    flops += 4.0 * k * w * r - 2.0 * w * (double)r * r;
+                    prof_add(P_RSKETCH, 4.0 * k * w * r - 2.0 * w * (double)r * r, SZ * ((double)k * r + 2.0 * k * w));
+                }
                 applied[q] = 1;
-                flops += 4.0 * k * w * r - 2.0 * w * (double)r * r;
-                prof_add(P_RSKETCH, 4.0 * k * w * r - 2.0 * w * (double)r * r, SZ * ((double)k * r + 2.0 * k * w));
             }
             std::vector<int> again;
             if (!lb.empty()) {
@@ -271,14 +308,21 @@ double Tree::rrqr_level(std::vector<Geqp3Task<S>>& gq, const std::vector<CopyTas
                 LarfbTask<S>* d_lbr[4];
                 for (int q = 0; q < 4; ++q) d_lbr[q] = upload_vec(lb_res[q]);
                 auto* d_tt = upload_vec(tt);
+                auto* d_eye = upload_vec(eye);
+                auto* d_tg = upload_vec(tg);
+                auto* d_tgt = upload_vec(tg_tiles);
+                auto* d_tlead = upload_vec(tlead);
                 timed(P_RSKETCH, 3, (long)lb.size(), [&] {
                     launch_fill<int>(d_tail, (size_t)nt, 0, stream_);
+                    launch_eye<S>(d_eye, (int)eye.size(), stream_);
                     if constexpr (sizeof(S) == 4) {
                         for (int q = 0; q < 3; ++q) launch_larfb_res(d_lbr[q], (int)lb_res[q].size(), 4 << q, stream_);
                         launch_larfb_res2(d_lbr[3], (int)lb_res[3].size(), stream_);
                     }
                     launch_larfb<S>(d_lb, (int)lb_rest.size(), stream_);
+                    launch_tail_gemm(d_tg, d_tgt, (int)tg_tiles.size(), stream_);
                     launch_tail_select<S>(d_tt, (int)tt.size(), tol, stream_);
+                    launch_tail_lead<S>(d_tlead, (int)tlead.size(), tol, stream_);
                 });
                 std::vector<int> got(nt, 0);
                 HIPCHK(hipMemcpyAsync(got.data(), d_tail, sizeof(int) * nt, hipMemcpyDeviceToHost, stream_));
