@@ -413,6 +413,20 @@ int spaqr_rank_features_batched(int prec, int count, const spaqr_rank_features_d
 typedef struct { const void* B; const void* cmax2; const int* ry; int* rank; int k, w; } spaqr_tail_desc;
 int spaqr_tail_select_batched(int prec, int count, const spaqr_tail_desc* d, double tol, void* stream);
 
+/* The same selection from B's leading rows only (the product form the FP32 planner uses for
+   interfaces of more than 1024 rows; the tail test of the rank path, which stands in for the
+   residual check of random_rrqr src/util.cpp:643-715): B holds rows 0 .. r-1 of Q^T C (r x w,
+   ld r, r >= *ry), cn the squared column norms of C (w entries), and the tail of column j at row
+   i is cn[j] - ||B(0:i, j)||^2.  Otherwise as spaqr_tail_select_batched. */
+typedef struct { const void* B; const void* cn; const void* cmax2; const int* ry; int* rank; int r, w; }
+    spaqr_tail_lead_desc;
+int spaqr_tail_lead_batched(int prec, int count, const spaqr_tail_lead_desc* d, double tol, void* stream);
+
+/* B (r x w, ld r) = M(0:r, 0:k) C (k x w, ld k), M with ld ldm >= r; FP32 on the matrix cores
+   (the product form's Q(:, 0:r)^T C, with M = Q^T formed on the identity). */
+typedef struct { const float* M; const float* C; float* B; int r, k, w, ldm; } spaqr_tail_gemm_desc;
+int spaqr_tail_gemm_batched(int count, const spaqr_tail_gemm_desc* d, void* stream);
+
 /* Matrix Market files (include/mmio.hpp:174-377 of the reference: sp_mmread, dense_mmread /
    vector_mmread, sp_mmwrite, dense_mmwrite; the driver reads A and the coordinates with them,
    pspaQR.cpp:85,161-167).  Host only, no GPU.

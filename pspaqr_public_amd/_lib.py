@@ -78,6 +78,14 @@ class TailDesc(C.Structure):
     _fields_ = [("B", _vp), ("cmax2", _vp), ("ry", _vp), ("rank", _vp), ("k", _i), ("w", _i)]
 
 
+class TailLeadDesc(C.Structure):
+    _fields_ = [("B", _vp), ("cn", _vp), ("cmax2", _vp), ("ry", _vp), ("rank", _vp), ("r", _i), ("w", _i)]
+
+
+class TailGemmDesc(C.Structure):
+    _fields_ = [("M", _vp), ("C", _vp), ("B", _vp), ("r", _i), ("k", _i), ("w", _i), ("ldm", _i)]
+
+
 EXPORTS = {
     "spaqr_tree_create": (_vp, [_i, _i]),
     "spaqr_tree_destroy": (None, [_vp]),
@@ -157,6 +165,8 @@ EXPORTS = {
     "spaqr_mm_write_dense": (_i, [C.c_char_p, _i, _i, _vp]),
     "spaqr_rank_features_batched": (_i, [_i, _i, _vp, _d, _vp]),
     "spaqr_tail_select_batched": (_i, [_i, _i, _vp, _d, _vp]),
+    "spaqr_tail_lead_batched": (_i, [_i, _i, _vp, _d, _vp]),
+    "spaqr_tail_gemm_batched": (_i, [_i, _vp, _vp]),
     "spaqr_shutdown": (_i, []),
     "spaqr_geqp3_batched_noalloc": (_i, [_i, _i, _vp, _d, _d, _vp]),
     "spaqr_geqp3_noalloc_scratch_bytes": (_i, []),

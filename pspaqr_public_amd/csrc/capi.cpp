@@ -941,6 +941,37 @@ int spaqr_rank_features_batched(int prec, int count, const spaqr_rank_features_d
     });
 }
 
+int spaqr_tail_lead_batched(int prec, int count, const spaqr_tail_lead_desc* d, double tol, void* stream) {
+    if (prec != 32 && prec != 64) return -1;
+    if (count < 0 || (count > 0 && !d)) return -2;
+    for (int i = 0; i < count; ++i)
+        if (d[i].r < 1 || d[i].w < 1 || !d[i].B || !d[i].cn || !d[i].cmax2 || !d[i].ry || !d[i].rank) return -3;
+    if (!(tol >= 0)) return -4;
+    if (count == 0) return 0;
+    static_assert(sizeof(TailLeadTask<double>) == sizeof(spaqr_tail_lead_desc), "tail lead desc layout");
+    return run_batched(d, sizeof(*d) * count, nullptr, 0, stream, [&](void* dd, void*, hipStream_t s) {
+        if (prec == 64) launch_tail_lead<double>((const TailLeadTask<double>*)dd, count, tol, s);
+        else launch_tail_lead<float>((const TailLeadTask<float>*)dd, count, tol, s);
+    });
+}
+
+int spaqr_tail_gemm_batched(int count, const spaqr_tail_gemm_desc* d, void* stream) {
+    if (count < 0 || (count > 0 && !d)) return -2;
+    for (int i = 0; i < count; ++i)
+        if (d[i].r < 1 || d[i].k < 1 || d[i].w < 1 || d[i].ldm < d[i].r || !d[i].M || !d[i].C || !d[i].B) return -3;
+    if (count == 0) return 0;
+    static_assert(sizeof(TailGemmTask) == sizeof(spaqr_tail_gemm_desc), "tail gemm desc layout");
+    std::vector<SketchTile> tiles;
+    for (int i = 0; i < count; ++i)
+        for (int r0 = 0; r0 < d[i].r; r0 += 64)
+            for (int c0 = 0; c0 < d[i].w; c0 += 64) tiles.push_back({i, r0, c0, 0});
+    const int nt = (int)tiles.size();
+    return run_batched(d, sizeof(*d) * count, tiles.data(), sizeof(SketchTile) * tiles.size(), stream,
+                       [&](void* dd, void* dt, hipStream_t s) {
+                           launch_tail_gemm((const TailGemmTask*)dd, (const SketchTile*)dt, nt, s);
+                       });
+}
+
 int spaqr_tail_select_batched(int prec, int count, const spaqr_tail_desc* d, double tol, void* stream) {
     if (prec != 32 && prec != 64) return -1;
     if (count < 0 || (count > 0 && !d)) return -2;

@@ -166,6 +166,50 @@ def test_tail_select_matches_oracle(prec):
 
 
 @pytest.mark.parametrize("prec", [64, 32])
+def test_tail_lead_matches_oracle(prec):
+    """The product form of the tail test: B's leading ry rows and the column norms of the full B
+    (= those of C, Q orthogonal) select the oracle's rank on the full B."""
+    dt = np.float64 if prec == 64 else np.float32
+    rng = np.random.default_rng(18)
+    for (k, w, need, ry) in [(40, 300, 13, 40), (64, 64, 0, 64), (200, 128, 77, 120), (50, 20, 30, 29),
+                             (1200, 260, 500, 640)]:
+        B = _tail_case(rng, k, w, need)
+        cmax2 = max(1.0, float(np.max((B * B).sum(0))))
+        tol = 1e-3
+        expect = olib().orc_tail_select(k, w, F(B), ry, cmax2, tol)
+        lead = np.ascontiguousarray(B[:ry, :])
+        bb = dbuf(F(lead), dt)
+        cnb = dbuf((B.astype(np.float64) ** 2).sum(0), dt)
+        cb = dbuf(np.array([cmax2]), dt)
+        ryb, rb = dbuf(np.array([ry], np.int32), np.int32), dbuf(np.zeros(1, np.int32), np.int32)
+        isz = np.dtype(dt).itemsize
+        ds = [G.TailLeadDesc(bb.ptr + c0 * ry * isz, cnb.ptr + c0 * isz, cb.ptr, ryb.ptr, rb.ptr, ry, min(128, w - c0))
+              for c0 in range(0, w, 128)]
+        assert G.lib().spaqr_tail_lead_batched(prec, len(ds), (G.TailLeadDesc * len(ds))(*ds), tol, None) == 0
+        got = int(rb.download(np.int32, 1)[0])
+        assert got == (expect if expect >= 0 else 2 ** 31 - 1), (k, w, need, ry, got)
+
+
+def test_tail_gemm_matches_numpy():
+    """B = M(0:r, 0:k) C on the matrix cores (FP32), uneven shapes in one launch."""
+    rng = np.random.default_rng(19)
+    shapes = [(70, 130, 200), (1, 33, 5), (300, 1100, 257), (64, 64, 64)]
+    keep, ds, ref = [], [], []
+    for (r, k, w) in shapes:
+        M = rng.standard_normal((k, k)).astype(np.float32)
+        Cm = rng.standard_normal((k, w)).astype(np.float32)
+        mb, cb = dbuf(F(M), np.float32), dbuf(F(Cm), np.float32)
+        bb = G.DeviceBuffer(4 * r * w)
+        keep += [mb, cb, bb]
+        ds.append(G.TailGemmDesc(mb.ptr, cb.ptr, bb.ptr, r, k, w, k))
+        ref.append(M[:r].astype(np.float64) @ Cm.astype(np.float64))
+    assert G.lib().spaqr_tail_gemm_batched(len(ds), (G.TailGemmDesc * len(ds))(*ds), None) == 0
+    for (r, k, w), d, e, i in zip(shapes, ds, ref, range(len(ds))):
+        got = unF(keep[3 * i + 2].download(np.float32, r * w), r, w)
+        assert rel(got, e) < 1e-5, (r, k, w)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
 def test_rank_features_match_oracle(prec):
     """Features from the GPU's own QR'd 16-column sketch: equal to the oracle's features of the
     same R diagonal, and (through the uniqueness of |R_jj|) to those of the oracle's QR."""
