@@ -28,11 +28,13 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <queue>
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <tuple>
 
 #include "kernels.hpp"
@@ -507,14 +509,26 @@ void Tree::sort_out(Cluster& c) {
 // next (ties: more conflicts, then earlier in cluster order) and takes the smallest colour no
 // conflict holds.  Interiors then update shared blocks colour by colour.  Returns the number
 // of colours.
-int Tree::colour_interiors(const std::vector<int>& ints) {
-    const int V = (int)ints.size();
-    std::vector<int> slot(cl.size(), -1);  // interior -> index
-    for (int i = 0; i < V; ++i) slot[ints[i]] = i;
-    // interiors by out-neighbour, then conflicts as sorted unique lists
+// The (out-neighbour, interior index) pairs the colouring works from: the only part that reads
+// the edge list.
+std::vector<std::pair<int, int>> Tree::colour_pairs(const std::vector<int>& ints) const {
     std::vector<std::pair<int, int>> by_nb;
-    for (int i = 0; i < V; ++i)
+    for (int i = 0; i < (int)ints.size(); ++i)
         for (size_t j = 1; j < cl[ints[i]].out.size(); ++j) by_nb.push_back({ed[cl[ints[i]].out[j]].n2, i});
+    return by_nb;
+}
+
+int Tree::colour_interiors(const std::vector<int>& ints) {
+    std::vector<int> col;
+    const int ncol = dsatur_colours(colour_pairs(ints), (int)ints.size(), col);
+    for (int i = 0; i < (int)ints.size(); ++i) cl[ints[i]].color = col[i];
+    return ncol;
+}
+
+// Interiors sharing an out-neighbour conflict; DSatur over that graph (V interiors, pairs from
+// colour_pairs).  Touches no tree state, so it may run beside the fill-in.
+int Tree::dsatur_colours(std::vector<std::pair<int, int>> by_nb, int V, std::vector<int>& col) {
+    // interiors by out-neighbour, then conflicts as sorted unique lists
     std::sort(by_nb.begin(), by_nb.end());
     std::vector<std::vector<int>> adj(V);
     for (size_t a = 0; a < by_nb.size();) {
@@ -533,7 +547,8 @@ int Tree::colour_interiors(const std::vector<int>& ints) {
     // most conflicts, then lowest index.  The last two keys are fixed, so they become one rank
     // (ord), and the uncoloured interiors sit in one min-heap of ranks per saturation; an entry is
     // stale once its interior is coloured or has moved up, and is dropped when it surfaces.
-    std::vector<int> col(V, -1), sat(V, 0), ord(V), rk(V);
+    col.assign(V, -1);
+    std::vector<int> sat(V, 0), ord(V), rk(V);
     for (int i = 0; i < V; ++i) ord[i] = i;
     std::sort(ord.begin(), ord.end(), [&](int x, int y) {
         return adj[x].size() != adj[y].size() ? adj[x].size() > adj[y].size() : x < y;
@@ -575,7 +590,6 @@ int Tree::colour_interiors(const std::vector<int>& ints) {
             top = std::max(top, sj);
         }
     }
-    for (int i = 0; i < V; ++i) cl[ints[i]].color = col[i];
     return ncol;
 }
 
@@ -1894,13 +1908,34 @@ int Tree::factorize_t() {
         // different colours, and the colours run in order (the update order of the level).
         // The interiors' out lists are final here, so the colouring and then the fill-in of
         // independent interiors run while the panel work does.
-        int ncolors = colour_interiors(ints);
-        st.colors = ncolors;
-        hacc(10, ht);
+        int ncolors = 0;
         if (independent) {
-            fill_in(true);
+            // the colouring (sequential) runs on its own thread beside the fill-in (parallel); it
+            // reads only the pairs taken here, before the fill-in grows the edge list
+            std::vector<int> col;
+            std::exception_ptr cerr;
+            std::thread th([&, pairs = colour_pairs(ints)]() mutable {
+                try {
+                    ncolors = dsatur_colours(std::move(pairs), (int)ints.size(), col);
+                } catch (...) {
+                    cerr = std::current_exception();
+                }
+            });
+            try {
+                fill_in(true);
+            } catch (...) {
+                th.join();
+                throw;
+            }
+            th.join();
+            if (cerr) std::rethrow_exception(cerr);
+            for (size_t i = 0; i < ints.size(); ++i) cl[ints[i]].color = col[i];
             hacc(0, ht);
+        } else {
+            ncolors = colour_interiors(ints);
+            hacc(10, ht);
         }
+        st.colors = ncolors;
         // Trailing-update chains, built over the interiors in parallel: links into one array at
         // per-slot offsets (counted first), tasks straight into their colour's launch buckets
         // (0-2 one-wave by link rows, 3-6 C-resident, 7 workgroup), all in interior order.

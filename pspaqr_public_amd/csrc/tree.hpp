@@ -365,6 +365,8 @@ private:
     bool want_sparsify(const Cluster& c) const;
     int new_edge(int c1, int c2);
     int colour_interiors(const std::vector<int>& ints);
+    std::vector<std::pair<int, int>> colour_pairs(const std::vector<int>& ints) const;
+    static int dsatur_colours(std::vector<std::pair<int, int>> pairs, int V, std::vector<int>& col);
     void sort_out(Cluster& c);
     int find_out(const Cluster& c, int n2) const;
     bool fillin_dist2();
